@@ -1,0 +1,307 @@
+"""Benchmark: fwd+bwd particle-substeps/s of the differentiable MPM substep.
+
+Workload (one GPU): the multi-material sphere of 1M particles on a 128^3 grid
+(rigid elastic core E=1e7 inside a Drucker-Prager shell), float32.  One step
+is a 50-substep checkpointed rollout forward plus its backward (recompute of
+the segment + adjoint sweep), i.e. 50 fwd+bwd substeps; inputs are resident in
+HBM before the timed region.
+
+Prints ONE JSON line (rank 0).  Also reports, for the dominant kernel, the
+achieved algorithmic bandwidth against MI355X's 8 TB/s (HIP events on the
+stream the kernel runs on), a CPU baseline (the oracle on a bounded sample,
+host cores) and the relative error of the GPU gradient against FP64 central
+differences of the oracle on a small scene.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--n-grid", type=int, default=128)
+    ap.add_argument("--segment", type=int, default=50)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--cpu-sample", type=int, default=200000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-grad-check", action="store_true")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_stage(torch, fn, reps):
+    """Average duration (ms) of `fn` (one kernel launch) over reps, HIP events on
+    the current stream, which is the stream the library launches on."""
+    fn()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def kernel_breakdown(torch, sim, state, mat_id, reps):
+    """Per-stage event timings of one fwd+bwd substep on resident inputs."""
+    from difftactile_amd import _lib
+    prm = sim.c_prm
+    n = sim.n
+    nn = sim.prm.n_nodes
+    dev = state[0].device
+    x, v, C, F = state
+    g4 = torch.zeros(nn, 4, device=dev)
+    gv4 = torch.zeros(nn, 4, device=dev)
+    x1, v1 = torch.empty_like(x), torch.empty_like(v)
+    C1, F1 = torch.empty_like(C), torch.empty_like(F)
+    mats = _lib.ptr(sim.mats)
+    nm = len(sim.materials)
+    mid = None if mat_id is None else _lib.ptr(mat_id, torch.int32)
+    P = _lib.ptr
+    st = _lib.stream()
+    # build the grids once
+    _lib.call("dt_mpm_substep_fwd", prm, mats, nm, n, mid, P(x), P(v), P(C), P(F), P(g4), P(gv4),
+              P(x1), P(v1), P(C1), P(F1), st)
+    gen = torch.Generator(device=dev).manual_seed(0)
+    gnext = [torch.randn(t.shape, device=dev, generator=gen) for t in state]
+    gg4 = torch.zeros(nn, 4, device=dev)
+    ggv4 = torch.zeros(nn, 4, device=dev)
+    gout = [torch.empty_like(t) for t in state]
+    g4b = torch.zeros_like(g4)
+
+    def p2g():
+        _lib.call("dt_mpm_p2g_fused", prm, mats, nm, n, mid, P(x), P(v), P(C), P(F), P(g4b),
+                  P(F1), st)
+
+    def grid_op():
+        _lib.call("dt_mpm_grid_op4", prm, P(g4), None, P(gv4), st)
+
+    def g2p():
+        _lib.call("dt_mpm_g2p4", prm, n, P(x), P(gv4), P(x1), P(v1), P(C1), st)
+
+    def g2p_adj():
+        _lib.call("dt_mpm_g2p4_adj", prm, n, P(x), P(gv4), P(gnext[0]), P(gnext[1]),
+                  P(gnext[2]), P(gout[0]), P(ggv4), st)
+
+    def grid_op_adj():
+        _lib.call("dt_mpm_grid_op4_adj", prm, P(g4), None, P(ggv4), P(gg4), None, st)
+
+    def p2g_adj():
+        _lib.call("dt_mpm_p2g_fused_adj", prm, mats, nm, n, mid, P(x), P(v), P(C), P(F), P(gg4),
+                  P(gnext[3]), P(gout[0]), P(gout[1]), P(gout[2]), P(gout[3]), st)
+
+    # order matters for valid inputs: g2p_adj fills ggv4 before grid_op_adj
+    stages = [("p2g_fused", p2g), ("grid_op4", grid_op), ("g2p4", g2p), ("g2p4_adj", g2p_adj),
+              ("grid_op4_adj", grid_op_adj), ("p2g_fused_adj", p2g_adj)]
+    out = {}
+    for name, fn in stages:
+        out[name] = time_stage(torch, fn, reps)
+    active = int((g4[:, 3] > sim.prm.eps).sum().item())
+    return out, active
+
+
+def algorithmic_bytes(n, active, has_mat):
+    """Algorithmic HBM bytes per launch of each fused stage (float32).
+
+    Per particle: x 12, v 12, C 36, F 36 (+4 material id).  Per active node:
+    packed (mv, m) 16 B or (v, pad) 16 B.
+    """
+    mid = 4 if has_mat else 0
+    return {
+        # read x v C F (+id), write F_next; flush (mv, m) per active node
+        "p2g_fused": n * (96 + mid + 36) + active * 16,
+        # read (mv, m), write (v) per active node (dense pass reads every node's m)
+        "grid_op4": active * 32,
+        # read x, write x v C; gather v per active node
+        "g2p4": n * (12 + 60) + active * 16,
+        # read x, gx' gv' gC' (60), write gx; scatter g_v per active node
+        "g2p4_adj": n * (12 + 60 + 12) + active * 16,
+        "grid_op4_adj": active * 48,
+        # read x v C F (+id), gx partial, gF'; write gx gv gC gF; gather (g_mv, g_m)
+        "p2g_fused_adj": n * (96 + mid + 12 + 36 + 96) + active * 16,
+    }
+
+
+def cpu_baseline(sample, segment_steps=4):
+    """Oracle (torch, float32, host cores) fwd+bwd substeps on a sample of the workload."""
+    import torch
+    from difftactile_amd import scenes
+    from oracle import mpm as om
+    prm, mats, x, v, mid = scenes.config3_multi_material()
+    idx = np.random.default_rng(0).choice(len(x), sample, replace=False)
+    x, v, mid = x[idx], v[idx], mid[idx]
+    op = om.MPMParams(n_grid=prm.n_grid, rho=prm.rho, dt=prm.dt, gravity=prm.gravity)
+    oms = [om.Material(m.E, m.nu, m.kind, m.yield_stress) for m in mats]
+    omats = om.material_arrays(oms, mid, torch.float32)
+    st = [torch.tensor(x), torch.tensor(v), torch.zeros(sample, 3, 3),
+          torch.eye(3).repeat(sample, 1, 1)]
+    for t in st:
+        t.requires_grad_(True)
+    t0 = time.perf_counter()
+    out = om.rollout(op, tuple(st), omats, segment_steps)
+    loss = sum(o.sum() for o in out)
+    loss.backward()
+    dt_s = time.perf_counter() - t0
+    return {"value": sample * segment_steps / dt_s, "unit": "particle-substeps/s",
+            "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"oracle (torch CPU float32, autograd) fwd+bwd of {segment_steps} substeps "
+                      f"on {sample} particles drawn from the 1M-particle sphere, 128^3 grid; "
+                      f"{dt_s:.1f} s"}
+
+
+def grad_check(torch):
+    """Directional derivative of a trajectory loss w.r.t. v0: GPU adjoint (float32)
+    vs FP64 central differences of the oracle (eps 1e-4)."""
+    from difftactile_amd import scenes
+    from difftactile_amd.params import DRUCKER_PRAGER, ELASTIC, Material, MPMParams
+    from difftactile_amd.sim import MPMSim
+    from oracle import mpm as om
+    prm = MPMParams(n_grid=32, rho=1000.0, dt=1e-5)
+    x, _ = scenes.sphere_lattice(2000, prm.dx / 2, (0.5, 0.3, 0.5), jitter=0.25 * prm.dx, seed=3)
+    n = len(x)
+    d = np.linalg.norm(x - np.array([0.5, 0.3, 0.5], np.float32), axis=1)
+    mid = (d > 0.5 * d.max()).astype(np.int32)
+    mats = [Material(1e6, 0.3, ELASTIC), Material(1e5, 0.3, DRUCKER_PRAGER)]
+    rng = np.random.default_rng(5)
+    v0 = rng.normal(0, 0.3, size=(n, 3)).astype(np.float32)
+    steps = 20
+    w = rng.normal(size=(n, 3))
+    dirv = rng.normal(size=(n, 3))
+    sim = MPMSim(prm, mats, n, mat_id=mid, checkpoint_every=7)
+    st = [torch.tensor(x).cuda(), torch.tensor(v0).cuda().requires_grad_(True),
+          torch.zeros(n, 3, 3).cuda(), torch.eye(3).repeat(n, 1, 1).cuda()]
+    out = sim.rollout(*st, steps)
+    loss = (out[0] * torch.tensor(w, dtype=torch.float32).cuda()).sum()
+    g = torch.autograd.grad(loss, st[1])[0].double().cpu().numpy()
+    gpu_dd = float((g * dirv).sum())
+    op = om.MPMParams(n_grid=32, rho=1000.0, dt=1e-5)
+    oms = [om.Material(m.E, m.nu, m.kind, m.yield_stress) for m in mats]
+    omats = om.material_arrays(oms, mid, torch.float64)
+    eps = 1e-4
+
+    def L(sign):
+        vv = torch.tensor(v0, dtype=torch.float64) + sign * eps * torch.tensor(dirv)
+        s = (torch.tensor(x, dtype=torch.float64), vv, torch.zeros(n, 3, 3, dtype=torch.float64),
+             torch.eye(3, dtype=torch.float64).repeat(n, 1, 1))
+        with torch.no_grad():
+            o = om.rollout(op, s, omats, steps)
+        return float((o[0] * torch.tensor(w)).sum())
+
+    fd = (L(1) - L(-1)) / (2 * eps)
+    return {"rel_err": abs(gpu_dd - fd) / max(abs(fd), 1e-30), "gpu": gpu_dd, "fd_fp64": fd,
+            "case": f"{n} particles, 32^3, {steps} substeps, core+DP shell, d/dv0 along a "
+                    "random direction, eps 1e-4"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not "
+              "part of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from difftactile_amd import scenes
+    from difftactile_amd.sim import MPMSim
+
+    dev = torch.device("cuda:0")
+    prm, mats, x, v, mid = scenes.config3_multi_material(n_particles=args.particles,
+                                                         n_grid=args.n_grid)
+    n = len(x)
+    sim = MPMSim(prm, mats, n, mat_id=mid, device=dev, checkpoint_every=args.segment, sort=True)
+    st = [torch.tensor(x, device=dev), torch.tensor(v, device=dev),
+          torch.zeros(n, 3, 3, device=dev), torch.tensor(scenes.identity_F(n), device=dev)]
+    w = torch.randn(n, 3, device=dev, generator=torch.Generator(device=dev).manual_seed(1))
+    T = args.segment
+
+    def step():
+        leaves = [t.detach().requires_grad_(True) for t in st]
+        out = sim.rollout(*leaves, T)
+        loss = (out[0] * w).sum() + out[1].sum()
+        return torch.autograd.grad(loss, leaves[1])[0]
+
+    log(f"[bench] {n} particles, {prm.n_grid}^3, segment {T}; warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g = step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    finite = bool(torch.isfinite(g).all().item())
+    ms = wall * 1e3 / args.steps
+    value = n * T * args.steps / wall
+    log(f"[bench] {ms:.2f} ms/step, {value:.3e} particle-substeps/s, grad finite={finite}")
+
+    # per-kernel event timings on sorted, resident inputs
+    _, perm = sim.sort_order(st[0].contiguous())
+    sorted_state = [sim._gather(perm, st[0], 3), sim._gather(perm, st[1], 3),
+                    sim._gather(perm, st[2].contiguous(), 9), sim._gather(perm, st[3], 9)]
+    mat_sorted = sim._gather_i(perm, sim.mat_id)
+    times, active = kernel_breakdown(torch, sim, sorted_state, mat_sorted, args.kernel_reps)
+    ab = algorithmic_bytes(n, active, True)
+    kern = {k: {"ms": round(times[k], 5), "GBps": round(ab[k] / (times[k] * 1e-3) / 1e9, 1)}
+            for k in times}
+    dom = max(times, key=times.get)
+    achieved = ab[dom] / (times[dom] * 1e-3) / 1e9
+    log(f"[bench] kernels: {json.dumps(kern)}")
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(dom)
+    # whole-substep algorithmic bytes: 480 B/particle + 128 B/active node (+8 B/particle ids)
+    b_sub = 480 * n + 128 * active + 8 * n
+    result = {
+        "metric": "fwd+bwd particle-substeps/s at 1M particles",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"multi-material sphere (elastic core E=1e7 + Drucker-Prager "
+                               f"shell), {n} particles, {prm.n_grid}^3 grid, "
+                               f"{T}-substep checkpointed rollout fwd+bwd per step",
+                   "particles": n, "n_grid": prm.n_grid, "substeps_per_step": T,
+                   "active_nodes": active},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "substep_roofline_frac": b_sub * (value / n) / (HBM_PEAK_GBS * 1e9),
+        "kernels": kern,
+        "grad_finite": finite,
+    }
+    if not args.no_grad_check:
+        log("[bench] gradient check vs FP64 FD")
+        result["grad_check"] = grad_check(torch)
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

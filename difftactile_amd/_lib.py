@@ -1,0 +1,138 @@
+"""ctypes binding of libdifftactile_hip.so (the C-ABI in include/difftactile.h).
+
+The library is the only compute path: if it is missing or fails to load,
+every entry point raises.  There is no CPU or PyTorch fallback.
+"""
+import ctypes
+import os
+
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdifftactile_hip.so")
+
+_c_int = ctypes.c_int32
+_c_float = ctypes.c_float
+_vp = ctypes.c_void_p
+
+
+class DtMpmParams(ctypes.Structure):
+    _fields_ = [
+        ("n_grid", ctypes.c_int32), ("bound", ctypes.c_int32),
+        ("sticky_floor", ctypes.c_int32), ("clamp_grad", ctypes.c_int32),
+        ("dx", ctypes.c_float), ("inv_dx", ctypes.c_float), ("dt", ctypes.c_float),
+        ("p_vol", ctypes.c_float), ("p_mass", ctypes.c_float), ("eps", ctypes.c_float),
+        ("gravity", ctypes.c_float * 3), ("grad_clamp", ctypes.c_float),
+    ]
+
+
+class DtMaterial(ctypes.Structure):
+    _fields_ = [
+        ("mu", ctypes.c_float), ("lam", ctypes.c_float), ("yield_stress", ctypes.c_float),
+        ("dp_alpha", ctypes.c_float), ("kind", ctypes.c_int32), ("pad", ctypes.c_int32 * 3),
+    ]
+
+
+class DtFemParams(ctypes.Structure):
+    _fields_ = [
+        ("mu", ctypes.c_float), ("lam", ctypes.c_float), ("dt", ctypes.c_float),
+        ("mass", ctypes.c_float),
+    ]
+
+
+class DtContactParams(ctypes.Structure):
+    _fields_ = [
+        ("kn", ctypes.c_float), ("kd", ctypes.c_float), ("kt", ctypes.c_float),
+        ("mu_f", ctypes.c_float), ("norm_eps", ctypes.c_float), ("sdf_offset", ctypes.c_float),
+    ]
+
+
+# name -> (restype, argtypes)
+_P = ctypes.POINTER
+_SIGS = {
+    "dt_last_error": (ctypes.c_char_p, []),
+    "dt_version": (_c_int, []),
+    "dt_mpm_compute_new_F": (_c_int, [_c_int, _vp, _vp, _vp, _c_float, _vp]),
+    "dt_mpm_compute_new_F_adj": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_float, _vp]),
+    "dt_mpm_svd": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_mpm_svd_adj": (_c_int, [_c_int] + [_vp] * 7 + [_vp]),
+    "dt_mpm_p2g": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 10 + [_vp]),
+    "dt_mpm_p2g_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 7
+                       + [_vp] * 3 + [_vp] * 7 + [_vp]),
+    "dt_mpm_grid_occupy": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp]),
+    "dt_mpm_grid_op": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dt_mpm_grid_op_adj": (_c_int, [_P(DtMpmParams)] + [_vp] * 8 + [_vp]),
+    "dt_mpm_g2p": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dt_mpm_g2p_adj": (_c_int, [_P(DtMpmParams), _c_int] + [_vp] * 7 + [_vp]),
+    "dt_mpm_substep_fwd": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 10 + [_vp]),
+    "dt_mpm_substep_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 16 + [_vp]),
+    "dt_mpm_run_fwd": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
+                                _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_mpm_run_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
+                                _c_int] + [_vp] * 12 + [_vp]),
+    "dt_mpm_advance": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 8
+                       + [_c_int, _vp, _vp, _vp]),
+    "dt_mpm_zero4": (_c_int, [ctypes.c_int64, _vp, _vp]),
+    "dt_mpm_p2g_fused": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 6 + [_vp]),
+    "dt_mpm_grid_op4": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp, _vp]),
+    "dt_mpm_g2p4": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dt_mpm_g2p4_adj": (_c_int, [_P(DtMpmParams), _c_int] + [_vp] * 7 + [_vp]),
+    "dt_mpm_grid_op4_adj": (_c_int, [_P(DtMpmParams)] + [_vp] * 5 + [_vp]),
+    "dt_mpm_p2g_fused_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 10
+                             + [_vp]),
+    "dt_mpm_cell_keys": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp]),
+    "dt_sort_tmp_bytes": (ctypes.c_int64, [_c_int]),
+    "dt_sort_by_key": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),
+    "dt_gather_rows": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "dt_scatter_rows": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library once; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m difftactile_amd.csrc.build` "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Entry points this binding expects the library to export."""
+    return sorted(_SIGS)
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().dt_last_error().decode(errors="replace")
+        raise RuntimeError(f"difftactile HIP call failed ({rc}): {msg}")
+
+
+def ptr(t, dtype=torch.float32):
+    """Device pointer of a contiguous CUDA/HIP tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("tensor must live on the GPU")
+    if t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return t.data_ptr()
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args))

@@ -1,0 +1,468 @@
+// Device-side math shared by the MPM, FEM and contact kernels (gfx950).
+//
+// Conventions
+//   * 3x3 matrices are row-major float[9] (m[r*3+c]), matching the reference's
+//     ti.Matrix.field to_numpy() layout [.., 3, 3].
+//   * Functions whose results must match the oracle bit for bit (cell base,
+//     closest-triangle distance, cell-centre queries) disable FP contraction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DT_HD __device__ __forceinline__
+
+namespace dt {
+
+// ----------------------------------------------------------------------------
+// small dense algebra
+// ----------------------------------------------------------------------------
+struct M3 { float a[9]; };
+struct M3d { double a[9]; };
+
+DT_HD M3 m3_zero() { M3 r; for (int i = 0; i < 9; ++i) r.a[i] = 0.f; return r; }
+DT_HD M3 m3_eye() { M3 r = m3_zero(); r.a[0] = r.a[4] = r.a[8] = 1.f; return r; }
+DT_HD M3 m3_mul(const M3& x, const M3& y) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.a[i * 3 + j] = x.a[i * 3 + 0] * y.a[0 * 3 + j] + x.a[i * 3 + 1] * y.a[1 * 3 + j] +
+                       x.a[i * 3 + 2] * y.a[2 * 3 + j];
+  return r;
+}
+// x * y^T
+DT_HD M3 m3_mul_bt(const M3& x, const M3& y) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.a[i * 3 + j] = x.a[i * 3 + 0] * y.a[j * 3 + 0] + x.a[i * 3 + 1] * y.a[j * 3 + 1] +
+                       x.a[i * 3 + 2] * y.a[j * 3 + 2];
+  return r;
+}
+// x^T * y
+DT_HD M3 m3_mul_at(const M3& x, const M3& y) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.a[i * 3 + j] = x.a[0 * 3 + i] * y.a[0 * 3 + j] + x.a[1 * 3 + i] * y.a[1 * 3 + j] +
+                       x.a[2 * 3 + i] * y.a[2 * 3 + j];
+  return r;
+}
+DT_HD M3 m3_t(const M3& x) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.a[i * 3 + j] = x.a[j * 3 + i];
+  return r;
+}
+DT_HD float m3_det(const M3& m) {
+  return m.a[0] * (m.a[4] * m.a[8] - m.a[5] * m.a[7]) - m.a[1] * (m.a[3] * m.a[8] - m.a[5] * m.a[6]) +
+         m.a[2] * (m.a[3] * m.a[7] - m.a[4] * m.a[6]);
+}
+// cofactor matrix: d det(m) / d m
+DT_HD M3 m3_cof(const M3& m) {
+  M3 c;
+  c.a[0] = m.a[4] * m.a[8] - m.a[5] * m.a[7];
+  c.a[1] = m.a[5] * m.a[6] - m.a[3] * m.a[8];
+  c.a[2] = m.a[3] * m.a[7] - m.a[4] * m.a[6];
+  c.a[3] = m.a[2] * m.a[7] - m.a[1] * m.a[8];
+  c.a[4] = m.a[0] * m.a[8] - m.a[2] * m.a[6];
+  c.a[5] = m.a[1] * m.a[6] - m.a[0] * m.a[7];
+  c.a[6] = m.a[1] * m.a[5] - m.a[2] * m.a[4];
+  c.a[7] = m.a[2] * m.a[3] - m.a[0] * m.a[5];
+  c.a[8] = m.a[0] * m.a[4] - m.a[1] * m.a[3];
+  return c;
+}
+DT_HD M3 m3_inv(const M3& m) {
+  M3 c = m3_cof(m);
+  float id = 1.f / m3_det(m);
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.a[i * 3 + j] = c.a[j * 3 + i] * id;
+  return r;
+}
+
+DT_HD M3d m3d_from(const M3& x) { M3d r; for (int i = 0; i < 9; ++i) r.a[i] = (double)x.a[i]; return r; }
+DT_HD M3d m3d_mul(const M3d& x, const M3d& y) {
+  M3d r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.a[i * 3 + j] = x.a[i * 3 + 0] * y.a[0 * 3 + j] + x.a[i * 3 + 1] * y.a[1 * 3 + j] +
+                       x.a[i * 3 + 2] * y.a[2 * 3 + j];
+  return r;
+}
+DT_HD M3d m3d_mul_at(const M3d& x, const M3d& y) {
+  M3d r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.a[i * 3 + j] = x.a[0 * 3 + i] * y.a[0 * 3 + j] + x.a[1 * 3 + i] * y.a[1 * 3 + j] +
+                       x.a[2 * 3 + i] * y.a[2 * 3 + j];
+  return r;
+}
+DT_HD M3d m3d_mul_bt(const M3d& x, const M3d& y) {
+  M3d r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.a[i * 3 + j] = x.a[i * 3 + 0] * y.a[j * 3 + 0] + x.a[i * 3 + 1] * y.a[j * 3 + 1] +
+                       x.a[i * 3 + 2] * y.a[j * 3 + 2];
+  return r;
+}
+
+// ----------------------------------------------------------------------------
+// global memory helpers
+// ----------------------------------------------------------------------------
+DT_HD M3 ld_m3(const float* __restrict__ p) { M3 r; for (int i = 0; i < 9; ++i) r.a[i] = p[i]; return r; }
+DT_HD void st_m3(float* __restrict__ p, const M3& m) { for (int i = 0; i < 9; ++i) p[i] = m.a[i]; }
+DT_HD void add_m3(float* __restrict__ p, const M3& m) { for (int i = 0; i < 9; ++i) p[i] += m.a[i]; }
+
+DT_HD void atomic_add_f(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// ----------------------------------------------------------------------------
+// 3x3 SVD: U, V in SO(3), sigma_1 >= sigma_2 >= |sigma_3|, sign on sigma_3.
+//
+// Method: cyclic Jacobi eigen-decomposition of A^T A with exact Givens
+// rotations (fixed sweep count), column sort of B = A V by norm, then Givens
+// QR of B; sigma = diag(R).  This is the structure of McAdams et al. 2011
+// ("Computing the SVD of 3x3 matrices with minimal branching") written with
+// exact rotations instead of the quaternion approximations.
+// ----------------------------------------------------------------------------
+DT_HD void jacobi_rot(float (&S)[3][3], float (&V)[3][3], int p, int q) {
+  float apq = S[p][q];
+  float c = 1.f, s = 0.f;
+  if (fabsf(apq) > 1e-30f) {
+    float tau = (S[q][q] - S[p][p]) / (2.f * apq);
+    float t = copysignf(1.f, tau) / (fabsf(tau) + sqrtf(1.f + tau * tau));
+    c = 1.f / sqrtf(1.f + t * t);
+    s = t * c;
+  }
+  // S <- J^T S J with J = [[c, s], [-s, c]] in the (p, q) plane
+  for (int k = 0; k < 3; ++k) {
+    float skp = S[k][p], skq = S[k][q];
+    S[k][p] = c * skp - s * skq;
+    S[k][q] = s * skp + c * skq;
+  }
+  for (int k = 0; k < 3; ++k) {
+    float spk = S[p][k], sqk = S[q][k];
+    S[p][k] = c * spk - s * sqk;
+    S[q][k] = s * spk + c * sqk;
+  }
+  for (int k = 0; k < 3; ++k) {
+    float vkp = V[k][p], vkq = V[k][q];
+    V[k][p] = c * vkp - s * vkq;
+    V[k][q] = s * vkp + c * vkq;
+  }
+}
+
+DT_HD void swap_cols_neg(float (&B)[3][3], float (&V)[3][3], int i, int j) {
+  // swap columns i, j of B and V and negate the new column j to keep det V = +1
+  for (int k = 0; k < 3; ++k) {
+    float t = B[k][i]; B[k][i] = B[k][j]; B[k][j] = -t;
+    t = V[k][i]; V[k][i] = V[k][j]; V[k][j] = -t;
+  }
+}
+
+// Givens QR step: zero B[r][c] against B[p][c] (rows p < r); accumulates U.
+DT_HD void givens_qr(float (&B)[3][3], float (&U)[3][3], int p, int r, int c) {
+  float a = B[p][c], b = B[r][c];
+  float d = a * a + b * b;
+  float cs = 1.f, sn = 0.f;
+  if (d > 1e-38f) {
+    float ir = 1.f / sqrtf(d);
+    cs = a * ir;
+    sn = b * ir;
+  }
+  // rows p, r of B <- G B with G = [[cs, sn], [-sn, cs]]
+  for (int k = 0; k < 3; ++k) {
+    float bp = B[p][k], br = B[r][k];
+    B[p][k] = cs * bp + sn * br;
+    B[r][k] = -sn * bp + cs * br;
+  }
+  // U <- U G^T (columns p, r)
+  for (int k = 0; k < 3; ++k) {
+    float up = U[k][p], ur = U[k][r];
+    U[k][p] = cs * up + sn * ur;
+    U[k][r] = -sn * up + cs * ur;
+  }
+}
+
+DT_HD void svd3(const M3& A, M3& Uo, float (&sig)[3], M3& Vo) {
+  float S[3][3], V[3][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}};
+  // S = A^T A
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      S[i][j] = A.a[0 * 3 + i] * A.a[0 * 3 + j] + A.a[1 * 3 + i] * A.a[1 * 3 + j] +
+                A.a[2 * 3 + i] * A.a[2 * 3 + j];
+#pragma unroll
+  for (int sweep = 0; sweep < 6; ++sweep) {
+    jacobi_rot(S, V, 0, 1);
+    jacobi_rot(S, V, 0, 2);
+    jacobi_rot(S, V, 1, 2);
+  }
+  // B = A V
+  float B[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      B[i][j] = A.a[i * 3 + 0] * V[0][j] + A.a[i * 3 + 1] * V[1][j] + A.a[i * 3 + 2] * V[2][j];
+  // sort columns by squared norm, descending
+  float n0 = B[0][0] * B[0][0] + B[1][0] * B[1][0] + B[2][0] * B[2][0];
+  float n1 = B[0][1] * B[0][1] + B[1][1] * B[1][1] + B[2][1] * B[2][1];
+  float n2 = B[0][2] * B[0][2] + B[1][2] * B[1][2] + B[2][2] * B[2][2];
+  if (n0 < n1) { swap_cols_neg(B, V, 0, 1); float t = n0; n0 = n1; n1 = t; }
+  if (n0 < n2) { swap_cols_neg(B, V, 0, 2); float t = n0; n0 = n2; n2 = t; }
+  if (n1 < n2) { swap_cols_neg(B, V, 1, 2); float t = n1; n1 = n2; n2 = t; }
+  // QR of B
+  float U[3][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}};
+  givens_qr(B, U, 0, 1, 0);
+  givens_qr(B, U, 0, 2, 0);
+  givens_qr(B, U, 1, 2, 1);
+  sig[0] = B[0][0];
+  sig[1] = B[1][1];
+  sig[2] = B[2][2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      Uo.a[i * 3 + j] = U[i][j];
+      Vo.a[i * 3 + j] = V[i][j];
+    }
+}
+
+// ----------------------------------------------------------------------------
+// MPM stencil.  base = trunc(fl(fl(x*inv_dx) - 0.5f)); fx = fl(x*inv_dx) - base
+// ----------------------------------------------------------------------------
+DT_HD int cell_base1(float x, float inv_dx) {
+#pragma clang fp contract(off)
+  float t = x * inv_dx;
+  float u = t - 0.5f;
+  return (int)u;
+}
+
+struct Stencil {
+  int base[3];
+  float fx[3];
+  float w[3][3];   // w[offset][axis]
+  float dw[3][3];  // d w / d fx
+};
+
+DT_HD void make_stencil(const float* x, float inv_dx, int n_grid, Stencil& s) {
+#pragma clang fp contract(off)
+  for (int a = 0; a < 3; ++a) {
+    float t = x[a] * inv_dx;
+    int b = (int)(t - 0.5f);
+    // Clamp keeps the 3x3x3 stencil inside the grid.  The reference has no
+    // bounds check; particles are kept away from the walls by the boundary
+    // band, so the clamp only changes cases the reference leaves undefined.
+    b = b < 0 ? 0 : (b > n_grid - 3 ? n_grid - 3 : b);
+    s.base[a] = b;
+    float f = t - (float)b;
+    s.fx[a] = f;
+    float d0 = 1.5f - f, d1 = f - 1.f, d2 = f - 0.5f;
+    s.w[0][a] = 0.5f * (d0 * d0);
+    s.w[1][a] = 0.75f - d1 * d1;
+    s.w[2][a] = 0.5f * (d2 * d2);
+    s.dw[0][a] = -d0;
+    s.dw[1][a] = -2.f * d1;
+    s.dw[2][a] = d2;
+  }
+}
+
+// cell key used for particle sorting: 4x4x4-cell bricks in row-major order,
+// cells row-major inside a brick.  key = brick * 64 + local.
+DT_HD int cell_key(const float* x, float inv_dx, int n_grid) {
+  int b[3];
+  for (int a = 0; a < 3; ++a) {
+    int v = cell_base1(x[a], inv_dx);
+    b[a] = v < 0 ? 0 : (v > n_grid - 1 ? n_grid - 1 : v);
+  }
+  int nb = (n_grid + 3) >> 2;
+  int brick = ((b[0] >> 2) * nb + (b[1] >> 2)) * nb + (b[2] >> 2);
+  int local = (((b[0] & 3) << 2) + (b[1] & 3)) * 4 + (b[2] & 3);
+  return brick * 64 + local;
+}
+
+// ----------------------------------------------------------------------------
+// materials
+// ----------------------------------------------------------------------------
+enum { KIND_ELASTIC = 0, KIND_VON_MISES = 1, KIND_DRUCKER_PRAGER = 2 };
+
+struct Mat {
+  float mu, lam, yield_stress, dp_alpha;
+  int kind;
+};
+
+// Return map + fixed-corotated Kirchhoff stress.
+//   in : F_new, U, sig, V (pre-return-map SVD), material
+//   out: tau, F_p (post-return-map F)
+struct StressFwd {
+  M3 Fp;
+  M3 R;
+  float J;
+  float s[3];        // clamped sigma used by the return map
+  float epsn[3];     // post-return-map Hencky strain (yielded cases)
+  float eh[3];       // deviatoric strain
+  float ehn;         // |eh|_eps
+  float trv;         // tr(eps)
+  int mode;          // 0 none, 1 von Mises yield, 2 DP shear, 3 DP tension
+};
+
+DT_HD void udv(const M3& U, const float (&d)[3], const M3& V, M3& out) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      out.a[i * 3 + j] = U.a[i * 3 + 0] * d[0] * V.a[j * 3 + 0] + U.a[i * 3 + 1] * d[1] * V.a[j * 3 + 1] +
+                         U.a[i * 3 + 2] * d[2] * V.a[j * 3 + 2];
+}
+
+DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3& V, const Mat& m,
+                      StressFwd& st, M3& tau) {
+  st.mode = 0;
+  st.Fp = Fn;
+  if (m.kind != KIND_ELASTIC) {
+    float eps[3];
+    for (int i = 0; i < 3; ++i) {
+      st.s[i] = fmaxf(sig[i], 0.05f);
+      eps[i] = logf(st.s[i]);
+    }
+    float tr = eps[0] + eps[1] + eps[2];
+    st.trv = tr;
+    for (int i = 0; i < 3; ++i) st.eh[i] = eps[i] - tr / 3.f;
+    st.ehn = sqrtf(st.eh[0] * st.eh[0] + st.eh[1] * st.eh[1] + st.eh[2] * st.eh[2] + 1e-8f);
+    if (m.kind == KIND_VON_MISES) {
+      float dg = st.ehn - m.yield_stress / (2.f * m.mu);
+      if (dg > 0.f) {
+        st.mode = 1;
+        float r = dg / st.ehn;
+        for (int i = 0; i < 3; ++i) st.epsn[i] = eps[i] - r * st.eh[i];
+      }
+    } else {
+      if (tr >= 0.f) {
+        st.mode = 3;
+        for (int i = 0; i < 3; ++i) st.epsn[i] = 0.f;
+      } else {
+        float dg = st.ehn + (3.f * m.lam + 2.f * m.mu) / (2.f * m.mu) * tr * m.dp_alpha;
+        if (dg > 0.f) {
+          st.mode = 2;
+          float r = dg / st.ehn;
+          for (int i = 0; i < 3; ++i) st.epsn[i] = eps[i] - r * st.eh[i];
+        }
+      }
+    }
+    if (st.mode) {
+      float d[3] = {expf(st.epsn[0]), expf(st.epsn[1]), expf(st.epsn[2])};
+      udv(U, d, V, st.Fp);
+    }
+    st.J = m3_det(st.Fp);
+  } else {
+    st.J = sig[0] * sig[1] * sig[2];
+  }
+  st.R = m3_mul_bt(U, V);
+  // tau = 2 mu (Fp - R) Fp^T + lam J (J - 1) I
+  M3 D;
+  for (int i = 0; i < 9; ++i) D.a[i] = st.Fp.a[i] - st.R.a[i];
+  tau = m3_mul_bt(D, st.Fp);
+  float l = m.lam * st.J * (st.J - 1.f);
+  for (int i = 0; i < 9; ++i) tau.a[i] *= 2.f * m.mu;
+  tau.a[0] += l; tau.a[4] += l; tau.a[8] += l;
+}
+
+// Adjoint of stress_fwd + svd (reference svd_grad) + nothing else.
+//   in : gtau, gFp_extra (adjoint of F_next = F_p), forward quantities
+//   out: gFn (adjoint of F_new)
+// The U/V adjoints and the svd_grad ff-terms are evaluated in float64: the
+// ff_ij = 1/(s_j^2 - s_i^2) factors reach 1e8 for near-equal singular values
+// and the float32 products they multiply lose every digit there.
+DT_HD double clampd(double a) { return a >= 0.0 ? fmax(a, 1e-8) : fmin(a, -1e-8); }
+
+DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3& V, const Mat& m,
+                      const StressFwd& st, const M3& gtau, const M3& gFp_in, M3& gFn) {
+  // tau adjoint
+  M3 D;
+  for (int i = 0; i < 9; ++i) D.a[i] = st.Fp.a[i] - st.R.a[i];
+  M3 gFp = m3_mul(gtau, st.Fp);           // gtau Fp
+  M3 t2 = m3_mul_at(gtau, D);             // gtau^T (Fp - R)
+  for (int i = 0; i < 9; ++i) gFp.a[i] = 2.f * m.mu * (gFp.a[i] + t2.a[i]) + gFp_in.a[i];
+  M3 gR = m3_mul(gtau, st.Fp);
+  for (int i = 0; i < 9; ++i) gR.a[i] *= -2.f * m.mu;
+  float trg = gtau.a[0] + gtau.a[4] + gtau.a[8];
+  float gJ = m.lam * (2.f * st.J - 1.f) * trg;
+
+  double gsig[3] = {0.0, 0.0, 0.0};
+  M3d Ud = m3d_from(U), Vd = m3d_from(V);
+  // gU = gR V, gV = gR^T U (float64)
+  M3d gRd = m3d_from(gR);
+  M3d gU = m3d_mul(gRd, Vd);
+  M3d gV = m3d_mul_at(gRd, Ud);
+  gFn = m3_zero();
+
+  if (m.kind == KIND_ELASTIC) {
+    gsig[0] += (double)gJ * sig[1] * sig[2];
+    gsig[1] += (double)gJ * sig[0] * sig[2];
+    gsig[2] += (double)gJ * sig[0] * sig[1];
+    gFn = gFp;
+  } else {
+    M3 cof = m3_cof(st.Fp);
+    for (int i = 0; i < 9; ++i) gFp.a[i] += gJ * cof.a[i];
+    if (st.mode == 0) {
+      gFn = gFp;
+    } else {
+      // Fp = U diag(d) V^T
+      double d[3] = {exp((double)st.epsn[0]), exp((double)st.epsn[1]), exp((double)st.epsn[2])};
+      M3d gFpd = m3d_from(gFp);
+      M3d UtgV = m3d_mul(m3d_mul_at(Ud, gFpd), Vd);   // U^T gFp V
+      M3d gFpV = m3d_mul(gFpd, Vd);                   // gFp V
+      M3d gFptU = m3d_mul_at(gFpd, Ud);               // gFp^T U
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          gU.a[i * 3 + j] += gFpV.a[i * 3 + j] * d[j];
+          gV.a[i * 3 + j] += gFptU.a[i * 3 + j] * d[j];
+        }
+      if (st.mode != 3) {
+        double ge[3];   // adjoint of post-map eps
+        for (int i = 0; i < 3; ++i) ge[i] = UtgV.a[i * 4] * d[i];
+        double eh[3] = {st.eh[0], st.eh[1], st.eh[2]};
+        double n = st.ehn;
+        double gdot = ge[0] * eh[0] + ge[1] * eh[1] + ge[2] * eh[2];
+        double geps[3], geh[3], gtr = 0.0;
+        if (st.mode == 1) {
+          // eps' = eps - eh + (c/n) eh, c = tau_y / (2 mu)
+          double c = (double)m.yield_stress / (2.0 * m.mu);
+          for (int i = 0; i < 3; ++i) {
+            geps[i] = ge[i];
+            geh[i] = (c / n - 1.0) * ge[i] - c * gdot / (n * n * n) * eh[i];
+          }
+        } else {
+          // eps' = eps - eh - r tr eh / n, r = (3 lam + 2 mu) / (2 mu) alpha
+          double r = (3.0 * m.lam + 2.0 * m.mu) / (2.0 * m.mu) * m.dp_alpha;
+          double tr = st.trv;
+          for (int i = 0; i < 3; ++i) {
+            geps[i] = ge[i];
+            geh[i] = -ge[i] - r * tr * (ge[i] / n - gdot * eh[i] / (n * n * n));
+          }
+          gtr = -r * gdot / n;
+        }
+        double mean = (geh[0] + geh[1] + geh[2]) / 3.0;
+        for (int i = 0; i < 3; ++i) {
+          geps[i] += geh[i] - mean + gtr;
+          if (sig[i] > 0.05f) gsig[i] += geps[i] / (double)st.s[i];
+        }
+      }
+    }
+  }
+  // gU, gV from R (already) -> svd_grad in float64
+  double s2[3] = {(double)sig[0] * sig[0], (double)sig[1] * sig[1], (double)sig[2] * sig[2]};
+  M3d Xu = m3d_mul_at(Ud, gU);   // U^T gU
+  M3d Xv = m3d_mul_at(Vd, gV);   // V^T gV
+  M3d Mi;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double v = 0.0;
+      if (i != j) {
+        double ff = 1.0 / clampd(s2[j] - s2[i]);
+        double xu = Xu.a[i * 3 + j] - Xu.a[j * 3 + i];
+        double xv = Xv.a[i * 3 + j] - Xv.a[j * 3 + i];
+        v = ff * xu * (double)sig[j] + (double)sig[i] * ff * xv;
+      } else {
+        v = gsig[i];
+      }
+      Mi.a[i * 3 + j] = v;
+    }
+  M3d G = m3d_mul_bt(m3d_mul(Ud, Mi), Vd);
+  for (int i = 0; i < 9; ++i) gFn.a[i] += (float)G.a[i];
+}
+
+}  // namespace dt

@@ -1,0 +1,207 @@
+"""Checkpointed, differentiable MPM trajectories on the fused HIP path.
+
+The reference keeps a window of ``sub_steps`` frames on the device, caches
+frame 0 of every window on the host and recomputes each window in the
+backward pass (``mpm_elastic.py:300-320``, ``grasp_plastic.py:594-648``).
+Here the same structure lives entirely in HBM:
+
+* the forward pass keeps only one state every ``checkpoint_every`` substeps;
+* the backward pass recomputes one segment at a time, keeping that segment's
+  frames and grids, and sweeps the fused adjoint kernels over it in reverse.
+
+Particles are ordered by cell key (a stable integer radix sort) at the start
+of each call; the order is undone on output so callers see their own order.
+"""
+import torch
+
+from . import _lib
+from .params import material_table
+
+
+class MPMSim:
+    """One MPM object on the GPU: parameters, material table, per-particle ids."""
+
+    def __init__(self, prm, materials, n_particles, mat_id=None, device="cuda",
+                 checkpoint_every=50, sort=True):
+        self.prm = prm
+        self.c_prm = prm.c_struct()
+        self.n = int(n_particles)
+        self.device = torch.device(device)
+        self.materials = list(materials)
+        self.mats = material_table(self.materials, self.device)
+        if mat_id is None:
+            self.mat_id = None
+        else:
+            self.mat_id = torch.as_tensor(mat_id, dtype=torch.int32, device=self.device).contiguous()
+            if self.mat_id.numel() != self.n:
+                raise ValueError("mat_id must have one entry per particle")
+            mx = int(self.mat_id.max().item()) if self.n else 0
+            mn = int(self.mat_id.min().item()) if self.n else 0
+            if mn < 0 or mx >= len(self.materials):
+                raise ValueError("mat_id out of range of the material table")
+        self.checkpoint_every = int(checkpoint_every)
+        self.sort = bool(sort)
+        self._ws = {}
+
+    # -- workspace -------------------------------------------------------
+    def _buf(self, name, shape, dtype=torch.float32):
+        t = self._ws.get(name)
+        if t is None or tuple(t.shape) != tuple(shape) or t.dtype != dtype:
+            t = torch.empty(shape, dtype=dtype, device=self.device)
+            self._ws[name] = t
+        return t
+
+    def _grid_scratch(self):
+        nn = self.prm.n_nodes
+        return self._buf("g4", (nn, 4)), self._buf("gv4", (nn, 4))
+
+    # -- ordering --------------------------------------------------------
+    def sort_order(self, x):
+        """(sorted keys, permutation) of the particles at positions x by cell key."""
+        n = self.n
+        keys = self._buf("keys", (n,), torch.int32)
+        keys_out = torch.empty(n, dtype=torch.int32, device=self.device)
+        perm = torch.empty(n, dtype=torch.int32, device=self.device)
+        _lib.call("dt_mpm_cell_keys", self.c_prm, n, _lib.ptr(x), _lib.ptr(keys, torch.int32),
+                  _lib.stream())
+        tb = int(_lib.lib().dt_sort_tmp_bytes(n))
+        tmp = self._buf("sort_tmp", (max(tb, 1),), torch.uint8)
+        _lib.call("dt_sort_by_key", n, self.prm.key_bits(), _lib.ptr(keys, torch.int32),
+                  _lib.ptr(keys_out, torch.int32), _lib.ptr(perm, torch.int32),
+                  tmp.data_ptr(), tb, _lib.stream())
+        return keys_out, perm
+
+    def _gather(self, perm, t, width):
+        out = torch.empty_like(t)
+        _lib.call("dt_gather_rows", self.n, width, _lib.ptr(perm, torch.int32), _lib.ptr(t),
+                  _lib.ptr(out), _lib.stream())
+        return out
+
+    def _scatter(self, perm, t, width):
+        out = torch.empty_like(t)
+        _lib.call("dt_scatter_rows", self.n, width, _lib.ptr(perm, torch.int32), _lib.ptr(t),
+                  _lib.ptr(out), _lib.stream())
+        return out
+
+    def _gather_i(self, perm, t):
+        # int32 ids gathered through the float path (bit pattern preserved)
+        return self._gather(perm, t.view(torch.float32), 1).view(torch.int32)
+
+    # -- raw segment kernels ----------------------------------------------
+    def _mid(self, mat_id):
+        return None if mat_id is None else _lib.ptr(mat_id, torch.int32)
+
+    def run_frames(self, X, V, C, F, k, mat_id, keep_grids):
+        """Advance frames 0..k of frame-major buffers X[k+1,n,3] ... in place."""
+        nn = self.prm.n_nodes
+        if keep_grids:
+            G4 = self._buf("G4", (k, nn, 4))
+            GV4 = self._buf("GV4", (k, nn, 4))
+            s4, sv4 = None, None
+            g4p, gv4p = _lib.ptr(G4), _lib.ptr(GV4)
+        else:
+            G4 = GV4 = None
+            s4, sv4 = self._grid_scratch()
+            g4p, gv4p = None, None
+        _lib.call("dt_mpm_run_fwd", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
+                  self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
+                  g4p, gv4p, _lib.ptr(s4), _lib.ptr(sv4), _lib.stream())
+        return G4, GV4
+
+    def advance(self, state, k, mat_id):
+        """Advance state = [x, v, C, F] by k substeps in place (no history kept)."""
+        s4, sv4 = self._grid_scratch()
+        tmp = [self._buf("adv%d" % i, t.shape) for i, t in enumerate(state)]
+        _lib.call("dt_mpm_advance", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
+                  self._mid(mat_id), *[_lib.ptr(t) for t in state], *[_lib.ptr(t) for t in tmp],
+                  int(k), _lib.ptr(s4), _lib.ptr(sv4), _lib.stream())
+
+    def adj_frames(self, X, V, C, F, k, mat_id, G4, GV4, g):
+        """Reverse sweep over k substeps; g = [gx, gv, gC, gF] seeded at frame k, in place."""
+        nn = self.prm.n_nodes
+        gg4 = self._buf("gg4", (nn, 4))
+        ggv4 = self._buf("ggv4", (nn, 4))
+        scratch = [self._buf("ga%d" % i, t.shape) for i, t in enumerate(g)]
+        _lib.call("dt_mpm_run_adj", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
+                  self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
+                  _lib.ptr(G4), _lib.ptr(GV4), _lib.ptr(gg4), _lib.ptr(ggv4),
+                  *[_lib.ptr(t) for t in g], *[_lib.ptr(t) for t in scratch], _lib.stream())
+
+    def _frames(self, k, prefix="F"):
+        n = self.n
+        return (self._buf(prefix + "x", (k + 1, n, 3)), self._buf(prefix + "v", (k + 1, n, 3)),
+                self._buf(prefix + "C", (k + 1, n, 3, 3)), self._buf(prefix + "F", (k + 1, n, 3, 3)))
+
+    # -- public API ------------------------------------------------------
+    def rollout(self, x, v, C, F, n_substeps):
+        """Differentiable rollout: returns (x, v, C, F) after n_substeps."""
+        return _Rollout.apply(self, int(n_substeps), x, v, C, F)
+
+    def step_no_grad(self, x, v, C, F, n_substeps):
+        with torch.no_grad():
+            return _Rollout.forward(None, self, int(n_substeps), x, v, C, F)
+
+
+def _segments(n_substeps, k):
+    s = 0
+    out = []
+    while s < n_substeps:
+        out.append((s, min(k, n_substeps - s)))
+        s += k
+    return out
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sim, n_substeps, x, v, C, F):
+        for t in (x, v, C, F):
+            if t.device.type != "cuda" or t.dtype != torch.float32:
+                raise ValueError("state tensors must be float32 on the GPU")
+        x, v, C, F = (t.contiguous() for t in (x, v, C, F))
+        if sim.sort and sim.n > 0:
+            _, perm = sim.sort_order(x)
+            state = [sim._gather(perm, x, 3), sim._gather(perm, v, 3),
+                     sim._gather(perm, C, 9), sim._gather(perm, F, 9)]
+            mat_id = sim._gather_i(perm, sim.mat_id) if sim.mat_id is not None else None
+        else:
+            perm = None
+            state = [x.clone(), v.clone(), C.clone(), F.clone()]
+            mat_id = sim.mat_id
+        ckpts = []
+        for (s, k) in _segments(n_substeps, sim.checkpoint_every):
+            if ctx is not None:
+                ckpts.append([t.clone() for t in state])
+            sim.advance(state, k, mat_id)
+        out = state
+        if perm is not None:
+            out = [sim._scatter(perm, out[0], 3), sim._scatter(perm, out[1], 3),
+                   sim._scatter(perm, out[2], 9), sim._scatter(perm, out[3], 9)]
+        if ctx is not None:
+            ctx.sim = sim
+            ctx.n_substeps = n_substeps
+            ctx.perm = perm
+            ctx.mat_id = mat_id
+            ctx.ckpts = ckpts
+        return tuple(out)
+
+    @staticmethod
+    def backward(ctx, gx, gv, gC, gF):
+        sim = ctx.sim
+        perm = ctx.perm
+        g = [t if t is not None else torch.zeros_like(r)
+             for t, r in zip((gx, gv, gC, gF), ctx.ckpts[0])]
+        g = [t.contiguous().float() for t in g]
+        widths = (3, 3, 9, 9)
+        if perm is not None:
+            g = [sim._gather(perm, t, w) for t, w in zip(g, widths)]
+        else:
+            g = [t.clone() for t in g]
+        segs = _segments(ctx.n_substeps, sim.checkpoint_every)
+        for (s, k), ck in zip(reversed(segs), reversed(ctx.ckpts)):
+            X, V, Cb, Fb = sim._frames(k, "R%d" % k)
+            X[0].copy_(ck[0]); V[0].copy_(ck[1]); Cb[0].copy_(ck[2]); Fb[0].copy_(ck[3])
+            G4, GV4 = sim.run_frames(X, V, Cb, Fb, k, ctx.mat_id, keep_grids=True)
+            sim.adj_frames(X, V, Cb, Fb, k, ctx.mat_id, G4, GV4, g)
+        if perm is not None:
+            g = [sim._scatter(perm, t, w) for t, w in zip(g, widths)]
+        return (None, None, *g)

@@ -1,0 +1,225 @@
+/*
+ * difftactile.h -- C-ABI of libdifftactile_hip.so, the MI355X (gfx950) HIP
+ * library behind the differentiable DiffTactile substep.
+ *
+ * Every entry point is `extern "C"`, takes plain device pointers, sizes and
+ * plain-C parameter structs, enqueues its work on `stream` (a hipStream_t;
+ * NULL = the default stream) without synchronising or allocating, and returns
+ * 0 (DT_OK) or a negative status.  dt_last_error() returns a thread-local
+ * message for the last failure on the calling thread.
+ *
+ * Array layouts are those of the reference's Taichi fields (to_numpy()):
+ *   particle vectors  float[n][3]       (x_0, v_0 at one frame)
+ *   particle matrices float[n][3][3]    row-major (C_0, F_0, F_new, U/S/V_svd)
+ *   grid vectors      float[n^3][3]     node (i,j,k) at (i*n + j)*n + k
+ *   grid scalars      float[n^3], int32[n^3]
+ * Adjoint ("_adj") entry points ACCUMULATE (+=) into their gradient outputs,
+ * as Taichi's kernel.grad does, unless stated otherwise.
+ *
+ * Reference interfaces replaced (paths relative to the DiffTactile repo):
+ *   each dt_mpm_* entry point names the @ti.kernel it replaces.
+ */
+#ifndef DIFFTACTILE_H
+#define DIFFTACTILE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DT_OK 0
+#define DT_EINVAL (-1)
+#define DT_ELAUNCH (-2)
+#define DT_ENOMEM (-3)
+
+/* Material kinds of the per-particle material table. */
+#define DT_MAT_ELASTIC 0        /* fixed corotated, J = det(Sigma)   mpm_elastic.py:183-185 */
+#define DT_MAT_VON_MISES 1      /* von Mises return map, J = det(F)  mpm_plastic.py:150-166 */
+#define DT_MAT_DRUCKER_PRAGER 2 /* sand return map (no reference implementation) */
+
+/* Scalar parameters of one MPM object (mpm_elastic.py:19-66). */
+typedef struct dt_mpm_params {
+  int32_t n_grid;       /* grid nodes per axis (reference: 64, mpm_elastic.py:35)   */
+  int32_t bound;        /* wall band width (reference: 3, mpm_elastic.py:34)        */
+  int32_t sticky_floor; /* 1: zero every velocity component at j < bound (no ref.)  */
+  int32_t clamp_grad;   /* 1: grid_op_adj clips d/d grid_m to +-grad_clamp
+                           (clamp_grid, grasp_plastic.py:191-194)                   */
+  float dx, inv_dx, dt, p_vol, p_mass, eps; /* eps: mass threshold, 1e-5           */
+  float gravity[3];
+  float grad_clamp;     /* 1000 in the reference                                    */
+} dt_mpm_params;
+
+/* One row of the material table (multi_obj.py:65-74 generalised). */
+typedef struct dt_material {
+  float mu, lam;        /* Lame parameters                                          */
+  float yield_stress;   /* von Mises yield (mpm_plastic.py:69,73)                   */
+  float dp_alpha;       /* Drucker-Prager sqrt(2/3) 2 sin(phi) / (3 - sin(phi))     */
+  int32_t kind;         /* DT_MAT_*                                                 */
+  int32_t pad[3];
+} dt_material;
+
+const char* dt_last_error(void);
+int dt_version(void);
+
+/* ---- MPM object, one entry point per reference kernel (frame f views) ---- */
+
+/* compute_new_F, mpm_elastic.py:134-137: F_new = (I + dt C) F */
+int dt_mpm_compute_new_F(int32_t n, const float* C, const float* F, float* F_new, float dt,
+                         void* stream);
+/* compute_new_F.grad: gC += dt gF_new F^T; gF += (I + dt C)^T gF_new */
+int dt_mpm_compute_new_F_adj(int32_t n, const float* C, const float* F, const float* gF_new,
+                             float* gC, float* gF, float dt, void* stream);
+
+/* svd, mpm_elastic.py:139-142 (ti.svd): U, V in SO(3); S diagonal (3x3, sign on S[2][2]) */
+int dt_mpm_svd(int32_t n, const float* F_new, float* U, float* S, float* V, void* stream);
+/* svd_grad, mpm_elastic.py:144-173: gF_new += U[S_bar + ff(U^T U_bar - U_bar^T U)S
+   + S ff(V^T V_bar - V_bar^T V)]V^T, ff terms evaluated in float64 */
+int dt_mpm_svd_adj(int32_t n, const float* U, const float* S, const float* V, const float* gU,
+                   const float* gS, const float* gV, float* gF_new, void* stream);
+
+/* p2g, mpm_elastic.py:175-208 / mpm_plastic.py:221-256 / multi_obj.py:227-252.
+   Scatters into grid_mv/grid_m (accumulating, like the reference's atomics)
+   and writes F_next (= F_0[f+1]).  mat_id may be NULL (all material 0). */
+int dt_mpm_p2g(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
+               const int32_t* mat_id, const float* x, const float* v, const float* C,
+               const float* F_new, const float* U, const float* S, const float* V, float* grid_mv,
+               float* grid_m, float* F_next, void* stream);
+/* p2g.grad: reads d/d grid_mv, d/d grid_m, d/d F_next; accumulates into
+   gx, gv, gC, gF_new, gU, gS, gV (the svd factors' adjoints, consumed by
+   dt_mpm_svd_adj as the reference's svd_grad consumes U/S/V_svd.grad). */
+int dt_mpm_p2g_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
+                   const int32_t* mat_id, const float* x, const float* v, const float* C,
+                   const float* F_new, const float* U, const float* S, const float* V,
+                   const float* g_grid_mv, const float* g_grid_m, const float* gF_next, float* gx,
+                   float* gv, float* gC, float* gF_new, float* gU, float* gS, float* gV,
+                   void* stream);
+
+/* check_grid_occupy, mpm_elastic.py:210-214: occupy = 1 where grid_m > eps
+   (only ever sets 1; the caller clears, as the reference's reset does) */
+int dt_mpm_grid_occupy(const dt_mpm_params* prm, const float* grid_m, int32_t* occupy,
+                       void* stream);
+/* grid_op, mpm_elastic.py:216-240.  grid_f may be NULL (no contact impulse).
+   occupy may be NULL (occupancy computed as grid_m > eps).  Writes every
+   occupied node of grid_v_out; unoccupied nodes are left untouched. */
+int dt_mpm_grid_op(const dt_mpm_params* prm, const float* grid_mv, const float* grid_m,
+                   const float* grid_f, const int32_t* occupy, float* grid_v_out, void* stream);
+/* grid_op.grad: accumulates into g_grid_mv, g_grid_m (clipped when
+   prm->clamp_grad), g_grid_f (may be NULL). */
+int dt_mpm_grid_op_adj(const dt_mpm_params* prm, const float* grid_mv, const float* grid_m,
+                       const float* grid_f, const int32_t* occupy, const float* g_grid_v_out,
+                       float* g_grid_mv, float* g_grid_m, float* g_grid_f, void* stream);
+
+/* g2p, mpm_elastic.py:243-260: writes x_next, v_next, C_next (frame f+1) */
+int dt_mpm_g2p(const dt_mpm_params* prm, int32_t n, const float* x, const float* grid_v_out,
+               float* x_next, float* v_next, float* C_next, void* stream);
+/* g2p.grad: reads gx_next, gv_next, gC_next; accumulates gx and g_grid_v_out */
+int dt_mpm_g2p_adj(const dt_mpm_params* prm, int32_t n, const float* x, const float* grid_v_out,
+                   const float* gx_next, const float* gv_next, const float* gC_next, float* gx,
+                   float* g_grid_v_out, void* stream);
+
+/* ---- fused substep (Contact.update / update_grad for one object without
+   pads, grasp_plastic.py:151-178): compute_new_F + svd + p2g fused into one
+   particle pass, grid_op, g2p.  Grid buffers are packed float4 per node
+   (mv.x, mv.y, mv.z, m) and float4 (v.x, v.y, v.z, unused), n^3 nodes each. */
+
+/* One forward substep: zeroes grid4, scatters, grid update, gather.
+   Writes grid4 (momentum, mass), grid_v4 and the next state. */
+int dt_mpm_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats,
+                       int32_t n, const int32_t* mat_id, const float* x, const float* v,
+                       const float* C, const float* F, float* grid4, float* grid_v4,
+                       float* x_next, float* v_next, float* C_next, float* F_next, void* stream);
+/* Adjoint of dt_mpm_substep_fwd given its grid4 / grid_v4 outputs: reads the
+   next state's adjoints and OVERWRITES gx, gv, gC, gF with the adjoints of
+   the state (x, v, C, F).  g_grid4, g_grid_v4: n^3 float4 scratch. */
+int dt_mpm_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats,
+                       int32_t n, const int32_t* mat_id, const float* x, const float* v,
+                       const float* C, const float* F, const float* grid4, const float* grid_v4,
+                       const float* gx_next, const float* gv_next, const float* gC_next,
+                       const float* gF_next, float* g_grid4, float* g_grid_v4, float* gx,
+                       float* gv, float* gC, float* gF, void* stream);
+
+/* Stages of the fused substep, for callers that couple other work between
+   them (pads and contact run between p2g and grid_op):
+   zero4 -> p2g_fused -> grid_op4 -> g2p4, and in reverse
+   g2p4_adj -> grid_op4_adj -> p2g_fused_adj.  grid_f / g_grid_f ([n^3][3],
+   the contact impulse f*dt, mpm_elastic.py:130-132) may be NULL. */
+int dt_mpm_zero4(int64_t n_nodes, float* buf4, void* stream);
+/* compute_new_F + svd + return map + p2g over (x, v, C, F); accumulates into grid4 */
+int dt_mpm_p2g_fused(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
+                     const int32_t* mat_id, const float* x, const float* v, const float* C,
+                     const float* F, float* grid4, float* F_next, void* stream);
+/* grid_op on packed grids; writes every node of grid_v4 (0 where unoccupied) */
+int dt_mpm_grid_op4(const dt_mpm_params* prm, const float* grid4, const float* grid_f,
+                    float* grid_v4, void* stream);
+/* g2p from a packed velocity grid */
+int dt_mpm_g2p4(const dt_mpm_params* prm, int32_t n, const float* x, const float* grid_v4,
+                float* x_next, float* v_next, float* C_next, void* stream);
+/* g2p adjoint: OVERWRITES gx with its partial; accumulates into g_grid_v4
+   (which the caller zeroes first) */
+int dt_mpm_g2p4_adj(const dt_mpm_params* prm, int32_t n, const float* x, const float* grid_v4,
+                    const float* gx_next, const float* gv_next, const float* gC_next, float* gx,
+                    float* g_grid_v4, void* stream);
+/* grid_op adjoint: OVERWRITES g_grid4 (and g_grid_f when non-NULL) */
+int dt_mpm_grid_op4_adj(const dt_mpm_params* prm, const float* grid4, const float* grid_f,
+                        const float* g_grid_v4, float* g_grid4, float* g_grid_f, void* stream);
+/* adjoint of dt_mpm_p2g_fused: gx holds the g2p partial on entry and the full
+   adjoint on return; OVERWRITES gv, gC, gF */
+int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats,
+                         int32_t n, const int32_t* mat_id, const float* x, const float* v,
+                         const float* C, const float* F, const float* g_grid4,
+                         const float* gF_next, float* gx, float* gv, float* gC, float* gF,
+                         void* stream);
+
+/* Native trajectory loops over a frame-major state buffer: frame k of x is
+   at x + k*n*3 (likewise v, C (n*9), F (n*9)).  dt_mpm_run_fwd advances
+   frames [0, n_substeps] and, when grids4 / grids_v4 are non-NULL, keeps
+   each substep's grids (n^3 float4 per substep) for dt_mpm_run_adj;
+   otherwise grids4 / grids_v4 must be passed as single-substep scratch in
+   scratch4 / scratch_v4. */
+int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
+                   const int32_t* mat_id, float* x, float* v, float* C, float* F,
+                   int32_t n_substeps, float* grids4, float* grids_v4, float* scratch4,
+                   float* scratch_v4, void* stream);
+/* Reverse sweep over frames [0, n_substeps] stored by dt_mpm_run_fwd with
+   grids kept.  Seeds: gx..gF hold the adjoint of frame n_substeps on entry and
+   the adjoint of frame 0 on return; gx2..gF2 are same-size scratch. */
+int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
+                   const int32_t* mat_id, const float* x, const float* v, const float* C,
+                   const float* F, int32_t n_substeps, const float* grids4, const float* grids_v4,
+                   float* g_grid4, float* g_grid_v4, float* gx, float* gv, float* gC, float* gF,
+                   float* gx2, float* gv2, float* gC2, float* gF2, void* stream);
+
+/* Advances the state (x, v, C, F) by n_substeps in place, ping-ponging with
+   the same-size scratch state (x2, v2, C2, F2); grids are single-substep
+   scratch (n^3 float4 each).  Keeps no history (forward-only / checkpoint
+   segments). */
+int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
+                   const int32_t* mat_id, float* x, float* v, float* C, float* F, float* x2,
+                   float* v2, float* C2, float* F2, int32_t n_substeps, float* scratch4,
+                   float* scratch_v4, void* stream);
+
+/* ---- particle ordering ---- */
+
+/* Cell key of each particle's stencil base (4x4x4-cell bricks, row-major
+   bricks, row-major cells inside a brick): key = brick*64 + local. */
+int dt_mpm_cell_keys(const dt_mpm_params* prm, int32_t n, const float* x, int32_t* keys,
+                     void* stream);
+/* Stable LSD radix sort of (key, index) pairs: writes keys_out (sorted) and
+   perm (perm[i] = original index of the i-th particle in sorted order).
+   tmp: at least dt_sort_tmp_bytes(n) bytes of device scratch. */
+int64_t dt_sort_tmp_bytes(int32_t n);
+int dt_sort_by_key(int32_t n, int32_t key_bits, const int32_t* keys, int32_t* keys_out,
+                   int32_t* perm, void* tmp, int64_t tmp_bytes, void* stream);
+/* dst[i, :] = src[perm[i], :] for rows of `width` floats. */
+int dt_gather_rows(int32_t n, int32_t width, const int32_t* perm, const float* src, float* dst,
+                   void* stream);
+/* dst[perm[i], :] = src[i, :]. */
+int dt_scatter_rows(int32_t n, int32_t width, const int32_t* perm, const float* src, float* dst,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFFTACTILE_H */

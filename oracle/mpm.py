@@ -106,10 +106,20 @@ def cell_base_np(x, inv_dx):
 
 
 def cell_keys_np(x, inv_dx, n_grid):
-    """Row-major cell key of each particle's stencil base: ``(bx*n + by)*n + bz``."""
-    b = cell_base_np(x, inv_dx).astype(np.int64)
+    """Cell key of each particle's stencil base (the sort key of the particle order).
+
+    Bases are clamped to [0, n-1]; cells are grouped in 4x4x4 bricks:
+    ``brick = ((bx>>2)*nb + (by>>2))*nb + (bz>>2)`` with ``nb = ceil(n/4)``,
+    ``local = ((bx&3)*4 + (by&3))*4 + (bz&3)``, ``key = brick*64 + local``.
+    The reference processes particles in index order and has no key; this
+    rule is the build's own and is stated here as the parity target.
+    """
     n = int(n_grid)
-    return ((b[:, 0] * n + b[:, 1]) * n + b[:, 2]).astype(np.int64)
+    b = np.clip(cell_base_np(x, inv_dx).astype(np.int64), 0, n - 1)
+    nb = (n + 3) // 4
+    brick = ((b[:, 0] >> 2) * nb + (b[:, 1] >> 2)) * nb + (b[:, 2] >> 2)
+    local = ((b[:, 0] & 3) * 4 + (b[:, 1] & 3)) * 4 + (b[:, 2] & 3)
+    return (brick * 64 + local).astype(np.int64)
 
 
 def sort_by_cell_np(x, inv_dx, n_grid):

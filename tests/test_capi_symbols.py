@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares (CPU only)."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from difftactile_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[\w\s\*]+?\b(dt_\w+)\s*\(", src, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "dt_mpm_p2g" in names and "dt_last_error" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_matches_header():
+    assert set(_lib.declared_symbols()) == set(declared())
+
+
+def test_error_path_without_gpu_work():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    assert L.dt_version() >= 1
+    # a NULL parameter struct is rejected on the host before any launch
+    rc = L.dt_mpm_grid_occupy(None, None, None, None)
+    assert rc == -1
+    assert b"NULL" in L.dt_last_error()
