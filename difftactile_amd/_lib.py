@@ -80,6 +80,22 @@ _SIGS = {
     "dt_mpm_p2g_fused_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 10
                              + [_vp]),
     "dt_mpm_cell_keys": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp]),
+    "dt_fem_init_B": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
+    "dt_fem_update": (_c_int, [_P(DtFemParams), _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_fem_update_adj": (_c_int, [_P(DtFemParams), _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dt_fem_update2": (_c_int, [_P(DtFemParams), _c_int] + [_vp] * 10 + [_vp]),
+    "dt_fem_update2_adj": (_c_int, [_P(DtFemParams), _c_int] + [_vp] * 10 + [_vp]),
+    "dt_clamp": (_c_int, [ctypes.c_int64, _vp, _c_float, _vp]),
+    "dt_contact_centroids": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
+    "dt_contact_closest": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _c_int,
+                                    _c_int, _vp]),
+    "dt_contact_collision": (_c_int, [_P(DtMpmParams), _P(DtContactParams), _vp, _c_int, _vp,
+                                      _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                      _P(ctypes.c_float), _vp, _vp, _vp]),
+    "dt_contact_collision_adj": (_c_int, [_P(DtMpmParams), _P(DtContactParams), _vp, _c_int, _vp,
+                                          _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                          _P(ctypes.c_float), _vp, _vp, _vp, _c_int, _vp, _c_int,
+                                          _vp, _vp, _vp]),
     "dt_sort_tmp_bytes": (ctypes.c_int64, [_c_int]),
     "dt_sort_by_key": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),
     "dt_gather_rows": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),

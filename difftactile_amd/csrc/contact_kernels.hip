@@ -1,0 +1,394 @@
+// Grid penalty contact between the MPM object and an FEM pad (gfx950).
+//
+// Reference (difftactile/):
+//   find_closest            sensor_model/fem_sensor.py:343-359
+//   find_sdf                sensor_model/fem_sensor.py:361-394
+//   update_contact_force    sensor_model/fem_sensor.py:401-406, object_model/mpm_elastic.py:130-132
+//   calculate_contact_force tasks/grasp_plastic.py:204-216
+//   check_collision         tasks/grasp_plastic.py:269-276
+//   collision               tasks/grasp_plastic.py:279-299
+//
+// The closest-triangle search is the reference's brute force (first minimum
+// of the centroid distance wins), evaluated with one fixed float32 rounding
+// rule so it is reproducible bit for bit; triangle centroids are staged in LDS
+// tiles and whole blocks of unoccupied nodes exit early.
+#include "dt_common.h"
+#include "dt_device.h"
+
+namespace dt {
+
+constexpr int kTriTile = 256;
+
+struct ContactP {
+  float kn, kd, kt, mu_f, norm_eps, sdf_offset;
+};
+
+// centroid c = fl32(1/3) * ((p1 + p2) + p3), no contraction
+__global__ __launch_bounds__(kBlock) void k_centroids(int nt, const int32_t* __restrict__ tris,
+                                                      const float* __restrict__ pos,
+                                                      float4* __restrict__ cen) {
+#pragma clang fp contract(off)
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  int a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
+  const float third = 1.0f / 3.0f;
+  float x = third * ((pos[3ll * a] + pos[3ll * b]) + pos[3ll * c]);
+  float y = third * ((pos[3ll * a + 1] + pos[3ll * b + 1]) + pos[3ll * c + 1]);
+  float z = third * ((pos[3ll * a + 2] + pos[3ll * b + 2]) + pos[3ll * c + 2]);
+  cen[t] = make_float4(x, y, z, 0.f);
+}
+
+__device__ __forceinline__ float node_centre(int i, float dx) {
+#pragma clang fp contract(off)
+  return ((float)i + 0.5f) * dx;
+}
+
+__device__ __forceinline__ float centroid_dist(const float4 c, float qx, float qy, float qz) {
+#pragma clang fp contract(off)
+  float d0 = c.x - qx, d1 = c.y - qy, d2 = c.z - qz;
+  float dd = (d0 * d0 + d1 * d1) + d2 * d2;
+  return sqrtf(dd + 1e-10f);
+}
+
+// check_collision for one pad: contact_idx[node*stride + col] = closest triangle
+// for occupied nodes (grid_m[node*m_stride] > eps); other nodes are untouched.
+__global__ __launch_bounds__(kBlock) void k_closest(MpmP P, long long nn,
+                                                    const float* __restrict__ gm, int m_stride,
+                                                    int nt, const float4* __restrict__ cen,
+                                                    int32_t* __restrict__ idx, int stride,
+                                                    int col) {
+  __shared__ float4 tile[kTriTile];
+  long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool occ = nd < nn && gm[nd * m_stride] > P.eps;
+  if (!__syncthreads_or(occ)) return;
+  const int n = P.n_grid;
+  float qx = 0.f, qy = 0.f, qz = 0.f;
+  if (nd < nn) {
+    int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), k = (int)(nd % n);
+    qx = node_centre(i, P.dx);
+    qy = node_centre(j, P.dx);
+    qz = node_centre(k, P.dx);
+  }
+  float best = 100.0f;
+  int best_k = -1;
+  for (int t0 = 0; t0 < nt; t0 += kTriTile) {
+    int cnt = min(kTriTile, nt - t0);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) tile[threadIdx.x] = cen[t0 + threadIdx.x];
+    __syncthreads();
+    if (occ) {
+      for (int q = 0; q < cnt; ++q) {
+        float d = centroid_dist(tile[q], qx, qy, qz);
+        if (d < best) {
+          best = d;
+          best_k = t0 + q;
+        }
+      }
+    }
+  }
+  if (occ) idx[nd * stride + col] = best_k;
+}
+
+// ---------------------------------------------------------------------------
+// per-node contact force (forward) and its adjoint
+// ---------------------------------------------------------------------------
+struct ContactFwd {
+  float p1[3], e1[3], e2[3], cr[3], s, sgn, n[3], sdf;
+  float r[3];     // -rel_v, the relative velocity argument of the force
+  float vnr, vn, fn[3], fnn, vt[3], vtn, mag, f[3];
+  bool flag, shear, kt_branch;
+};
+
+__device__ __forceinline__ void contact_fwd(const ContactP& C, const float (&q)[3],
+                                            const float (&qv)[3], const float* __restrict__ pos,
+                                            const float* __restrict__ vel, const int (&tv)[3],
+                                            const float (&out_dir)[3], ContactFwd& w) {
+  float p1[3], p2[3], p3[3], v1[3], v2[3], v3[3];
+  for (int a = 0; a < 3; ++a) {
+    p1[a] = pos[3ll * tv[0] + a];
+    p2[a] = pos[3ll * tv[1] + a];
+    p3[a] = pos[3ll * tv[2] + a];
+    v1[a] = vel[3ll * tv[0] + a];
+    v2[a] = vel[3ll * tv[1] + a];
+    v3[a] = vel[3ll * tv[2] + a];
+    w.p1[a] = p1[a];
+    w.e1[a] = p2[a] - p1[a];
+    w.e2[a] = p3[a] - p1[a];
+  }
+  w.cr[0] = w.e1[1] * w.e2[2] - w.e1[2] * w.e2[1];
+  w.cr[1] = w.e1[2] * w.e2[0] - w.e1[0] * w.e2[2];
+  w.cr[2] = w.e1[0] * w.e2[1] - w.e1[1] * w.e2[0];
+  w.s = sqrtf(w.cr[0] * w.cr[0] + w.cr[1] * w.cr[1] + w.cr[2] * w.cr[2] + 1e-10f);
+  float nrm[3] = {w.cr[0] / w.s, w.cr[1] / w.s, w.cr[2] / w.s};
+  float dn = nrm[0] * out_dir[0] + nrm[1] * out_dir[1] + nrm[2] * out_dir[2];
+  w.sgn = dn > 0.f ? 1.f : (dn < 0.f ? -1.f : 0.f);
+  for (int a = 0; a < 3; ++a) w.n[a] = w.sgn * nrm[a];
+  float ds[3] = {q[0] - p1[0], q[1] - p1[1], q[2] - p1[2]};
+  float sdf = ds[0] * w.n[0] + ds[1] * w.n[1] + ds[2] * w.n[2];
+  float proj[3] = {q[0] - sdf * w.n[0], q[1] - sdf * w.n[1], q[2] - sdf * w.n[2]};
+  // barycentric on v0 = p3 - p1, v1 = p2 - p1
+  float vv2[3] = {proj[0] - p1[0], proj[1] - p1[1], proj[2] - p1[2]};
+  float dot00 = w.e2[0] * w.e2[0] + w.e2[1] * w.e2[1] + w.e2[2] * w.e2[2];
+  float dot01 = w.e2[0] * w.e1[0] + w.e2[1] * w.e1[1] + w.e2[2] * w.e1[2];
+  float dot02 = w.e2[0] * vv2[0] + w.e2[1] * vv2[1] + w.e2[2] * vv2[2];
+  float dot11 = w.e1[0] * w.e1[0] + w.e1[1] * w.e1[1] + w.e1[2] * w.e1[2];
+  float dot12 = w.e1[0] * vv2[0] + w.e1[1] * vv2[1] + w.e1[2] * vv2[2];
+  float inv_d = 1.f / (dot00 * dot11 - dot01 * dot01);
+  float u = (dot11 * dot02 - dot01 * dot12) * inv_d;
+  float v = (dot00 * dot12 - dot01 * dot02) * inv_d;
+  w.sdf = sdf - C.sdf_offset;
+  for (int a = 0; a < 3; ++a) {
+    float rel = qv[a] - (1.f / 3.f) * (v1[a] + v2[a] + v3[a]);
+    w.r[a] = -rel;
+  }
+  w.flag = (w.sdf < 0.f) && (u >= 0.f) && (v >= 0.f) && (u + v <= 1.f);
+  // calculate_contact_force(sdf, n, r)
+  w.vnr = w.n[0] * w.r[0] + w.n[1] * w.r[1] + w.n[2] * w.r[2];
+  w.vn = fmaxf(w.vnr, 0.f);
+  float kk = -(C.kn + C.kd * w.vn) * w.sdf;
+  for (int a = 0; a < 3; ++a) w.fn[a] = kk * w.n[a];
+  for (int a = 0; a < 3; ++a) w.vt[a] = w.r[a] - w.vnr * w.n[a];
+  w.vtn = sqrtf(w.vt[0] * w.vt[0] + w.vt[1] * w.vt[1] + w.vt[2] * w.vt[2] + C.norm_eps);
+  w.fnn = sqrtf(w.fn[0] * w.fn[0] + w.fn[1] * w.fn[1] + w.fn[2] * w.fn[2]);
+  w.shear = w.vtn > 1e-4f;
+  float a1 = C.kt * w.vtn, a2 = C.mu_f * w.fnn;
+  w.kt_branch = a1 < a2;
+  w.mag = fminf(a1, a2);
+  for (int a = 0; a < 3; ++a) w.f[a] = w.fn[a] + (w.shear ? w.vt[a] / w.vtn * w.mag : 0.f);
+}
+
+// collision for one pad.  grid: momentum mv (stride smv), mass m (stride sm),
+// impulse grid_f [n^3,3] (+= f*dt); pad external force ext [nv,3] (+= -f/3).
+__global__ __launch_bounds__(kBlock) void k_collision(MpmP P, ContactP C, long long nn,
+                                                      const float* __restrict__ mv, int smv,
+                                                      const float* __restrict__ m, int sm,
+                                                      const int32_t* __restrict__ idx, int stride,
+                                                      int col, const int32_t* __restrict__ tris,
+                                                      const float* __restrict__ pos,
+                                                      const float* __restrict__ vel, float o0,
+                                                      float o1, float o2,
+                                                      float* __restrict__ gridf,
+                                                      float* __restrict__ ext) {
+  long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nn) return;
+  float mm = m[nd * sm];
+  if (!(mm > P.eps)) return;
+  int k = idx[nd * stride + col];
+  if (k < 0) return;
+  const int n = P.n_grid;
+  int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
+  float q[3] = {node_centre(i, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
+  float inv = 1.f / (mm + P.eps);
+  float qv[3] = {mv[nd * smv] * inv, mv[nd * smv + 1] * inv, mv[nd * smv + 2] * inv};
+  int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
+  float od[3] = {o0, o1, o2};
+  ContactFwd w;
+  contact_fwd(C, q, qv, pos, vel, tv, od, w);
+  if (!w.flag) return;
+  for (int a = 0; a < 3; ++a) {
+    gridf[3 * nd + a] += w.f[a] * P.dt;
+    float e = -w.f[a] / 3.f;
+    atomic_add_f(ext + 3ll * tv[0] + a, e);
+    atomic_add_f(ext + 3ll * tv[1] + a, e);
+    atomic_add_f(ext + 3ll * tv[2] + a, e);
+  }
+}
+
+// Adjoint of k_collision.  Inputs: g_gridf (adjoint of grid_f), g_ext (adjoint
+// of the pad's external force).  Accumulates into g_mv, g_m (grid, strided),
+// g_pos, g_vel (pad, atomics).
+__global__ __launch_bounds__(kBlock) void k_collision_adj(
+    MpmP P, ContactP C, long long nn, const float* __restrict__ mv, int smv,
+    const float* __restrict__ m, int sm, const int32_t* __restrict__ idx, int stride, int col,
+    const int32_t* __restrict__ tris, const float* __restrict__ pos, const float* __restrict__ vel,
+    float o0, float o1, float o2, const float* __restrict__ g_gridf,
+    const float* __restrict__ g_ext, float* __restrict__ g_mv, int sgmv, float* __restrict__ g_m,
+    int sgm, float* __restrict__ g_pos, float* __restrict__ g_vel) {
+  long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nn) return;
+  float mm = m[nd * sm];
+  if (!(mm > P.eps)) return;
+  int k = idx[nd * stride + col];
+  if (k < 0) return;
+  const int n = P.n_grid;
+  int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
+  float q[3] = {node_centre(i, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
+  float inv = 1.f / (mm + P.eps);
+  float mvv[3] = {mv[nd * smv], mv[nd * smv + 1], mv[nd * smv + 2]};
+  float qv[3] = {mvv[0] * inv, mvv[1] * inv, mvv[2] * inv};
+  int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
+  float od[3] = {o0, o1, o2};
+  ContactFwd w;
+  contact_fwd(C, q, qv, pos, vel, tv, od, w);
+  if (!w.flag) return;
+  // adjoint of f: grid_f += f dt; ext[t] += -f/3
+  float gf[3];
+  for (int a = 0; a < 3; ++a)
+    gf[a] = g_gridf[3 * nd + a] * P.dt -
+            (g_ext[3ll * tv[0] + a] + g_ext[3ll * tv[1] + a] + g_ext[3ll * tv[2] + a]) / 3.f;
+  float gfn[3] = {gf[0], gf[1], gf[2]};
+  float gvt[3] = {0.f, 0.f, 0.f};
+  if (w.shear) {
+    // ft = vt * (mag / vtn)
+    float s = gf[0] * w.vt[0] + gf[1] * w.vt[1] + gf[2] * w.vt[2];
+    float ratio = w.mag / w.vtn;
+    for (int a = 0; a < 3; ++a) gvt[a] += ratio * gf[a];
+    float gmag = s / w.vtn;
+    float gvtn = -s * w.mag / (w.vtn * w.vtn);
+    if (w.kt_branch) {
+      gvtn += C.kt * gmag;
+    } else if (w.fnn > 0.f) {
+      float gfnn = C.mu_f * gmag;
+      for (int a = 0; a < 3; ++a) gfn[a] += gfnn * w.fn[a] / w.fnn;
+    }
+    for (int a = 0; a < 3; ++a) gvt[a] += gvtn * w.vt[a] / w.vtn;
+  }
+  // vt = r - vnr n
+  float gr[3], gn[3], gvnr = 0.f;
+  for (int a = 0; a < 3; ++a) {
+    gr[a] = gvt[a];
+    gvnr -= gvt[a] * w.n[a];
+    gn[a] = -w.vnr * gvt[a];
+  }
+  // fn = -(kn + kd vn) sdf n
+  float kk2 = -(C.kn + C.kd * w.vn) * w.sdf;
+  float gfn_n = gfn[0] * w.n[0] + gfn[1] * w.n[1] + gfn[2] * w.n[2];
+  for (int a = 0; a < 3; ++a) gn[a] += kk2 * gfn[a];
+  float gsdf = -(C.kn + C.kd * w.vn) * gfn_n;
+  float gvn = -C.kd * w.sdf * gfn_n;
+  if (w.vnr > 0.f) gvnr += gvn;
+  // vnr = n . r
+  for (int a = 0; a < 3; ++a) {
+    gn[a] += gvnr * w.r[a];
+    gr[a] += gvnr * w.n[a];
+  }
+  // r = -qv + (1/3) sum vel;  qv = mv / (m + eps)
+  float gqv[3] = {-gr[0], -gr[1], -gr[2]};
+  float gmm = 0.f;
+  for (int a = 0; a < 3; ++a) {
+    atomic_add_f(g_mv + nd * sgmv + a, gqv[a] * inv);
+    gmm -= gqv[a] * mvv[a] * inv * inv;
+    float gvv = gr[a] / 3.f;
+    atomic_add_f(g_vel + 3ll * tv[0] + a, gvv);
+    atomic_add_f(g_vel + 3ll * tv[1] + a, gvv);
+    atomic_add_f(g_vel + 3ll * tv[2] + a, gvv);
+  }
+  atomic_add_f(g_m + nd * sgm, gmm);
+  // sdf = (q - p1) . n - offset
+  float gp1[3], ge1[3], ge2[3];
+  for (int a = 0; a < 3; ++a) {
+    gn[a] += gsdf * (q[a] - w.p1[a]);
+    gp1[a] = -gsdf * w.n[a];
+  }
+  // n = sgn * cr / s
+  float gnr[3] = {w.sgn * gn[0], w.sgn * gn[1], w.sgn * gn[2]};
+  float dotc = gnr[0] * w.cr[0] + gnr[1] * w.cr[1] + gnr[2] * w.cr[2];
+  float s3 = w.s * w.s * w.s;
+  float gcr[3];
+  for (int a = 0; a < 3; ++a) gcr[a] = gnr[a] / w.s - w.cr[a] * dotc / s3;
+  // cr = e1 x e2: g_e1 = e2 x g_cr, g_e2 = g_cr x e1
+  ge1[0] = w.e2[1] * gcr[2] - w.e2[2] * gcr[1];
+  ge1[1] = w.e2[2] * gcr[0] - w.e2[0] * gcr[2];
+  ge1[2] = w.e2[0] * gcr[1] - w.e2[1] * gcr[0];
+  ge2[0] = gcr[1] * w.e1[2] - gcr[2] * w.e1[1];
+  ge2[1] = gcr[2] * w.e1[0] - gcr[0] * w.e1[2];
+  ge2[2] = gcr[0] * w.e1[1] - gcr[1] * w.e1[0];
+  for (int a = 0; a < 3; ++a) {
+    atomic_add_f(g_pos + 3ll * tv[0] + a, gp1[a] - ge1[a] - ge2[a]);
+    atomic_add_f(g_pos + 3ll * tv[1] + a, ge1[a]);
+    atomic_add_f(g_pos + 3ll * tv[2] + a, ge2[a]);
+  }
+}
+
+static ContactP contact_dev(const dt_contact_params* c) {
+  ContactP d;
+  d.kn = c->kn;
+  d.kd = c->kd;
+  d.kt = c->kt;
+  d.mu_f = c->mu_f;
+  d.norm_eps = c->norm_eps;
+  d.sdf_offset = c->sdf_offset;
+  return d;
+}
+
+}  // namespace dt
+
+using namespace dt;
+
+extern "C" {
+
+int dt_contact_centroids(int32_t n_tris, const int32_t* tris, const float* pos, float* centroids4,
+                         void* stream) {
+  DT_REQUIRE(n_tris >= 0, "n_tris must be >= 0");
+  if (n_tris == 0) return DT_OK;
+  DT_REQUIRE(tris && pos && centroids4, "NULL pointer");
+  hipLaunchKernelGGL(k_centroids, dim3(nblocks(n_tris)), dim3(kBlock), 0, (hipStream_t)stream,
+                     n_tris, tris, pos, (float4*)centroids4);
+  return check_launch("dt_contact_centroids");
+}
+
+int dt_contact_closest(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
+                       int32_t n_tris, const float* centroids4, int32_t* contact_idx,
+                       int32_t idx_stride, int32_t idx_col, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n_tris >= 0 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
+                 idx_col < idx_stride,
+             "bad size/stride");
+  DT_REQUIRE(grid_m && contact_idx && (n_tris == 0 || centroids4), "NULL pointer");
+  long long nn = (long long)prm->n_grid * prm->n_grid * prm->n_grid;
+  hipLaunchKernelGGL(k_closest, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
+                     to_dev(*prm), nn, grid_m, m_stride, n_tris, (const float4*)centroids4,
+                     contact_idx, idx_stride, idx_col);
+  return check_launch("dt_contact_closest");
+}
+
+int dt_contact_collision(const dt_mpm_params* prm, const dt_contact_params* cp,
+                         const float* grid_mv, int32_t mv_stride, const float* grid_m,
+                         int32_t m_stride, const int32_t* contact_idx, int32_t idx_stride,
+                         int32_t idx_col, int32_t n_tris, const int32_t* tris, const float* pos,
+                         const float* vel, const float* out_dir, float* grid_f, float* ext_force,
+                         void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
+  DT_REQUIRE(mv_stride >= 3 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
+                 idx_col < idx_stride && n_tris >= 0,
+             "bad size/stride");
+  DT_REQUIRE(grid_mv && grid_m && contact_idx && tris && pos && vel && grid_f && ext_force,
+             "NULL pointer");
+  long long nn = (long long)prm->n_grid * prm->n_grid * prm->n_grid;
+  hipLaunchKernelGGL(k_collision, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
+                     to_dev(*prm), contact_dev(cp), nn, grid_mv, mv_stride, grid_m, m_stride,
+                     contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0], out_dir[1],
+                     out_dir[2], grid_f, ext_force);
+  return check_launch("dt_contact_collision");
+}
+
+int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* cp,
+                             const float* grid_mv, int32_t mv_stride, const float* grid_m,
+                             int32_t m_stride, const int32_t* contact_idx, int32_t idx_stride,
+                             int32_t idx_col, int32_t n_tris, const int32_t* tris,
+                             const float* pos, const float* vel, const float* out_dir,
+                             const float* g_grid_f, const float* g_ext_force, float* g_grid_mv,
+                             int32_t gmv_stride, float* g_grid_m, int32_t gm_stride, float* g_pos,
+                             float* g_vel, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
+  DT_REQUIRE(mv_stride >= 3 && m_stride > 0 && gmv_stride >= 3 && gm_stride > 0 &&
+                 idx_stride > 0 && idx_col >= 0 && idx_col < idx_stride && n_tris >= 0,
+             "bad size/stride");
+  DT_REQUIRE(grid_mv && grid_m && contact_idx && tris && pos && vel && g_grid_f && g_ext_force &&
+                 g_grid_mv && g_grid_m && g_pos && g_vel,
+             "NULL pointer");
+  long long nn = (long long)prm->n_grid * prm->n_grid * prm->n_grid;
+  hipLaunchKernelGGL(k_collision_adj, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
+                     to_dev(*prm), contact_dev(cp), nn, grid_mv, mv_stride, grid_m, m_stride,
+                     contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0], out_dir[1],
+                     out_dir[2], g_grid_f, g_ext_force, g_grid_mv, gmv_stride, g_grid_m,
+                     gm_stride, g_pos, g_vel);
+  return check_launch("dt_contact_collision_adj");
+}
+
+}  // extern "C"

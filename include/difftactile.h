@@ -59,6 +59,19 @@ typedef struct dt_material {
   int32_t pad[3];
 } dt_material;
 
+/* FEM pad material and integration constants (fem_sensor.py:35-49). */
+typedef struct dt_fem_params {
+  float mu, lam;        /* Lame parameters of the stable Neo-Hookean model          */
+  float dt, mass;       /* substep and per-vertex mass (vol * rho)                  */
+} dt_fem_params;
+
+/* Penalty contact constants (grasp_plastic.py:57-64, 96). */
+typedef struct dt_contact_params {
+  float kn, kd, kt, mu_f; /* normal stiffness, normal damping, shear stiffness, friction */
+  float norm_eps;         /* eps of the shear-velocity norm (1e-11)                   */
+  float sdf_offset;       /* find_sdf offset (0 for the grid contact)                 */
+} dt_contact_params;
+
 const char* dt_last_error(void);
 int dt_version(void);
 
@@ -217,6 +230,72 @@ int dt_gather_rows(int32_t n, int32_t width, const int32_t* perm, const float* s
 /* dst[perm[i], :] = src[i, :]. */
 int dt_scatter_rows(int32_t n, int32_t width, const int32_t* perm, const float* src, float* dst,
                     void* stream);
+
+
+/* ---- FEM pad (sensor_model/fem_sensor.py, gripper_fem.py) ----
+   pos/vel/vpos/ext_f/cv: float[n_verts][3] at one frame; f2v int32[n_tets][4];
+   B float[n_tets][3][3]; layer_id int32[n_verts] (0 = fixed base layer). */
+
+/* init_pos, fem_sensor.py:334-338: B = [a-d, b-d, c-d]^-1 from world positions */
+int dt_fem_init_B(int32_t n_tets, const int32_t* f2v, const float* pos0, float* B, void* stream);
+/* update, fem_sensor.py:408-440: vel[v] += dt * force / mass (stable Neo-Hookean,
+   current volume |det D|/6), in place on the frame's velocities */
+int dt_fem_update(const dt_fem_params* prm, int32_t n_tets, const int32_t* f2v, const float* B,
+                  const float* pos, float* vel, void* stream);
+/* update.grad: reads g_vel (adjoint of vel after update); accumulates g_pos */
+int dt_fem_update_adj(const dt_fem_params* prm, int32_t n_tets, const int32_t* f2v,
+                      const float* B, const float* pos, const float* g_vel, float* g_pos,
+                      void* stream);
+/* update2, fem_sensor.py:443-457 (cv_next NULL) / gripper_fem.py:403-418
+   (cv_next = control_vel[f+1]): writes frame f+1 */
+int dt_fem_update2(const dt_fem_params* prm, int32_t n_verts, const int32_t* layer_id,
+                   const float* pos, const float* vel, const float* vpos, const float* ext_f,
+                   const float* cv, const float* cv_next, float* pos_next, float* vel_next,
+                   float* vpos_next, void* stream);
+/* update2.grad: accumulates into g_pos, g_vel, g_vpos, g_ext_f and (when
+   non-NULL) g_cv, g_cv_next */
+int dt_fem_update2_adj(const dt_fem_params* prm, int32_t n_verts, const int32_t* layer_id,
+                       const float* g_pos_next, const float* g_vel_next, const float* g_vpos_next,
+                       float* g_pos, float* g_vel, float* g_vpos, float* g_ext_f, float* g_cv,
+                       float* g_cv_next, void* stream);
+/* clamp_grid, grasp_plastic.py:191-202: a[i] = clip(a[i], -limit, limit) */
+int dt_clamp(int64_t n, float* a, float limit, void* stream);
+
+/* ---- grid penalty contact (tasks/grasp_plastic.py:204-299) ----
+   The grid is given as momentum (stride mv_stride floats per node) and mass
+   (stride m_stride): (grid_v_in, grid_m) -> strides 3 and 1; packed grid4 ->
+   grid4 and grid4 + 3 with strides 4 and 4.  contact_idx is int32 with
+   idx_stride entries per node (one column per pad), -1 = none.
+   out_dir is a HOST pointer to 3 floats (the pad's outward direction). */
+
+/* triangle centroids (1/3)((p1 + p2) + p3), float4 per triangle */
+int dt_contact_centroids(int32_t n_tris, const int32_t* tris, const float* pos, float* centroids4,
+                         void* stream);
+/* check_collision + find_closest (fem_sensor.py:343-359) for one pad: for each
+   node with grid_m > eps, the first triangle minimising
+   sqrt(|c - q|^2 + 1e-10) over q = cell centre ((i+.5)dx, (j+.5)dx, (k+.5)dx);
+   -1 when none is below 100.  Bit-exact float32 rule. */
+int dt_contact_closest(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
+                       int32_t n_tris, const float* centroids4, int32_t* contact_idx,
+                       int32_t idx_stride, int32_t idx_col, void* stream);
+/* collision (one pad): find_sdf + calculate_contact_force; grid_f += f*dt,
+   ext_force[tri verts] += -f/3 */
+int dt_contact_collision(const dt_mpm_params* prm, const dt_contact_params* cp,
+                         const float* grid_mv, int32_t mv_stride, const float* grid_m,
+                         int32_t m_stride, const int32_t* contact_idx, int32_t idx_stride,
+                         int32_t idx_col, int32_t n_tris, const int32_t* tris, const float* pos,
+                         const float* vel, const float* out_dir, float* grid_f, float* ext_force,
+                         void* stream);
+/* collision.grad: reads g_grid_f, g_ext_force; accumulates into g_grid_mv,
+   g_grid_m (strided like the grid) and g_pos, g_vel of the pad */
+int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* cp,
+                             const float* grid_mv, int32_t mv_stride, const float* grid_m,
+                             int32_t m_stride, const int32_t* contact_idx, int32_t idx_stride,
+                             int32_t idx_col, int32_t n_tris, const int32_t* tris,
+                             const float* pos, const float* vel, const float* out_dir,
+                             const float* g_grid_f, const float* g_ext_force, float* g_grid_mv,
+                             int32_t gmv_stride, float* g_grid_m, int32_t gm_stride, float* g_pos,
+                             float* g_vel, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,189 @@
+"""Pins the FEM and contact oracles (CPU only)."""
+import numpy as np
+import pytest
+import torch
+
+from difftactile_amd.sensor_model.meshes import dome_mesh, slab_mesh
+from oracle import contact as oc
+from oracle import fem as of
+from oracle import mpm as om
+
+F64 = torch.float64
+
+
+def _pad(seed=0, pert=0.02):
+    nodes, f2v, surf, layer = dome_mesh()
+    rng = np.random.default_rng(seed)
+    pos0 = torch.tensor(nodes, dtype=F64)
+    f2v_t = torch.tensor(f2v, dtype=torch.int64)
+    B = of.init_B(pos0, f2v_t)
+    pos = pos0 + pert * torch.tensor(rng.normal(size=nodes.shape), dtype=F64)
+    return pos0, pos, f2v_t, B, surf, layer
+
+
+# ---------------------------------------------------------------------------
+# FEM
+# ---------------------------------------------------------------------------
+def test_rest_shape_inverse():
+    pos0, _, f2v, B, _, _ = _pad()
+    p = pos0[f2v]
+    Dm = torch.stack([p[:, 0] - p[:, 3], p[:, 1] - p[:, 3], p[:, 2] - p[:, 3]], -1)
+    assert torch.allclose(Dm @ B, torch.eye(3, dtype=F64).expand_as(Dm), atol=1e-9)
+
+
+def test_stable_neo_hookean_is_energy_gradient_and_rest_free():
+    pos0, pos, f2v, B, _, _ = _pad()
+    mu, lam = 1294.01, 9201.11
+    f_rest = of.tet_forces(pos0, f2v, B, mu, lam)
+    assert f_rest.abs().max() < 1e-9
+    # P = dPsi/dF by central differences
+    rng = np.random.default_rng(1)
+    F = torch.eye(3, dtype=F64) + 0.2 * torch.tensor(rng.normal(size=(50, 3, 3)), dtype=F64)
+    p = pos[f2v[:50]]
+    # recover P from forces: H = -V P B^T with the same F
+    Fv = F.clone().requires_grad_(True)
+    psi = of.neo_hookean_energy(Fv, mu, lam).sum()
+    P_ad = torch.autograd.grad(psi, Fv)[0]
+    h = 1e-6
+    P_fd = torch.zeros_like(F)
+    for i in range(3):
+        for j in range(3):
+            E = torch.zeros_like(F)
+            E[:, i, j] = h
+            P_fd[:, i, j] = (of.neo_hookean_energy(F + E, mu, lam) -
+                             of.neo_hookean_energy(F - E, mu, lam)) / (2 * h)
+    assert torch.allclose(P_ad, P_fd, rtol=1e-6, atol=1e-4)
+    # forces of tet_forces equal -V dPsi/dF B^T column-wise
+    D = torch.stack([p[:, 0] - p[:, 3], p[:, 1] - p[:, 3], p[:, 2] - p[:, 3]], -1)
+    Fcur = D @ B[:50]
+    Fc = Fcur.clone().requires_grad_(True)
+    Pc = torch.autograd.grad(of.neo_hookean_energy(Fc, mu, lam).sum(), Fc)[0]
+    Vol = torch.abs(torch.linalg.det(D)) / 6
+    H = -Vol[:, None, None] * Pc @ B[:50].transpose(1, 2)
+    forces = of.tet_forces(pos, f2v[:50], B[:50], mu, lam)
+    assert torch.allclose(forces[:, :3], H.transpose(1, 2), rtol=1e-9, atol=1e-9)
+
+
+def test_tet_forces_balance_and_torque_free():
+    _, pos, f2v, B, _, _ = _pad(seed=2, pert=0.05)
+    f = of.tet_forces(pos, f2v, B, 1294.01, 9201.11)
+    assert f.sum(1).abs().max() < 1e-9
+    x = pos[f2v]
+    torque = torch.cross(x, f, dim=-1).sum(1)
+    assert torque.abs().max() < 1e-8 * max(1.0, f.abs().max().item())
+
+
+def test_update2_closed_form_and_base_layer():
+    n = 10
+    rng = np.random.default_rng(3)
+    pos, vel, vpos, ext, cv, cv1 = (torch.tensor(rng.normal(size=(n, 3)), dtype=F64) for _ in range(6))
+    layer = torch.tensor([0, 1] * 5)
+    dt, mass = 1e-3, 2.0
+    p1, v1, vp1 = of.update2_dome(pos, vel, vpos, ext, layer, cv, dt, mass)
+    vv = torch.where((layer == 0)[:, None], cv, vel + dt * ext / mass)
+    assert torch.allclose(v1, vv) and torch.allclose(p1, pos + dt * vv)
+    assert torch.allclose(vp1, vpos + dt * cv)
+    p2, v2, vp2 = of.update2_gripper(pos, vel, vpos, ext, layer, cv, cv1, dt, mass)
+    assert torch.allclose(v2, vv + cv1 - cv) and torch.allclose(p2, p1)
+
+
+# ---------------------------------------------------------------------------
+# contact
+# ---------------------------------------------------------------------------
+def test_find_closest_matches_scalar_loop_and_float64_argmin():
+    nodes, f2v, surf, layer = dome_mesh()
+    pos = nodes.astype(np.float32)
+    rng = np.random.default_rng(4)
+    q = rng.uniform(-2.0, 2.0, size=(300, 3)).astype(np.float32)
+    q[:5] = 200.0                                  # farther than 100: no triangle
+    got = om_find = oc.find_closest_np(q, pos, surf)
+    for i in range(0, 300, 7):
+        assert got[i] == oc.find_closest_scalar(q[i], pos, surf)
+    assert (got[:5] == -1).all()
+    # away from near-ties the float64 argmin agrees
+    c64 = pos[surf].astype(np.float64).mean(1)
+    d64 = np.linalg.norm(c64[None] - q[5:, None].astype(np.float64), axis=-1)
+    srt = np.sort(d64, axis=1)
+    clear = (srt[:, 1] - srt[:, 0]) > 1e-5
+    assert (np.argmin(d64, 1)[clear] == got[5:][clear]).all()
+
+
+def test_find_sdf_closed_forms():
+    pos = torch.tensor([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]], dtype=F64)
+    vel = torch.tensor([[0.3, 0.0, 0.0]] * 3, dtype=F64)
+    tris = torch.tensor([[0, 1, 2]])
+    q = torch.tensor([[0.2, -0.1, 0.3], [0.9, -0.1, 0.9], [0.2, 0.4, 0.3]], dtype=F64)
+    qv = torch.zeros(3, 3, dtype=F64)
+    out_dir = torch.tensor([0.0, 1.0, 0.0], dtype=F64)
+    sdf, nv, rv, flag = oc.find_sdf(q, qv, torch.zeros(3, dtype=torch.int64), pos, vel, tris, out_dir)
+    # plane y = 0 with normal +y (aligned with out_dir): sdf = y, -n = -y
+    assert torch.allclose(sdf, q[:, 1], atol=1e-9)
+    assert torch.allclose(nv, torch.tensor([0.0, -1.0, 0.0], dtype=F64).expand(3, 3), atol=1e-9)
+    assert torch.allclose(rv, -vel.mean(0).expand(3, 3))
+    assert flag.tolist() == [True, False, False]   # inside+below, outside, above
+
+
+def test_contact_force_closed_forms():
+    kn, kd, kt, mu = 10.0, 2.0, 3.0, 0.5
+    n = torch.tensor([[0.0, 1.0, 0.0]] * 3, dtype=F64)
+    sdf = torch.tensor([-0.1] * 3, dtype=F64)
+    r = torch.tensor([[0.0, 0.2, 0.0], [1e-6, 0.0, 0.0], [5.0, -0.3, 0.0]], dtype=F64)
+    f = oc.contact_force(sdf, n, r, kn, kd, kt, mu)
+    # pure normal approach: f = -(kn + kd vn) sdf n
+    assert torch.allclose(f[0], torch.tensor([0.0, (kn + kd * 0.2) * 0.1, 0.0], dtype=F64))
+    # tiny shear (< 1e-4): no tangential force
+    assert torch.allclose(f[1], torch.tensor([0.0, kn * 0.1, 0.0], dtype=F64))
+    # large shear: Coulomb cap mu |f_n|, separating normal velocity clipped to 0
+    fn = kn * 0.1
+    assert torch.allclose(f[2], torch.tensor([mu * fn, fn, 0.0], dtype=F64), atol=1e-12)
+
+
+def test_collision_newtons_third_law_and_fd():
+    prm = om.MPMParams(n_grid=32, rho=1000.0, dt=1e-4)
+    nodes, f2v, surf, layer, od = slab_mesh(16, 16, 3, h=1 / 64, origin=(0.3, 0.5, 0.3))
+    rng = np.random.default_rng(5)
+    # jitter the lattice pad so no projected node lies exactly on a triangle edge
+    pos = torch.tensor(nodes + 1e-3 * rng.normal(size=nodes.shape), dtype=F64)
+    vel = torch.tensor(0.1 * rng.normal(size=nodes.shape), dtype=F64)
+    x = torch.tensor(rng.uniform([0.3, 0.42, 0.3], [0.53, 0.58, 0.53], size=(3000, 3)), dtype=F64)
+    v = torch.tensor(rng.normal(0, 0.5, size=(3000, 3)), dtype=F64)
+    mats = om.material_arrays([om.Material(5e4, 0.3)], np.zeros(3000, int), F64)
+    Fi = torch.eye(3, dtype=F64).repeat(3000, 1, 1)
+    U, s, V = om.svd3(Fi)
+    mv, m, _ = om.p2g(prm, x, v, torch.zeros_like(Fi), Fi, U, s, V, mats)
+    nodes_o, idx = oc.check_collision(prm, m, [pos], [surf])
+    odt = torch.tensor(od, dtype=F64)
+    gf, ext = oc.collision(prm, mv, m, nodes_o, idx, [(pos, vel, surf, odt)], 1e3, 10.0, 50.0, 0.5)
+    assert (gf.abs().sum(1) > 0).sum() > 20
+    assert torch.allclose(gf.sum(0) / prm.dt + ext[0].sum(0), torch.zeros(3, dtype=F64), atol=1e-9)
+    # reverse mode vs central differences along random directions; the contact
+    # flags depend on pos only, so pos is perturbed by an amount checked to
+    # leave every flag unchanged
+    w = torch.tensor(rng.normal(size=(prm.n_grid ** 3, 3)), dtype=F64)
+    we = torch.tensor(rng.normal(size=nodes.shape), dtype=F64)
+    nodes_t = torch.as_tensor(nodes_o)
+    q = torch.as_tensor(oc.node_centres(prm, nodes_o)).to(F64)
+
+    def flags(pos_):
+        qv = mv[nodes_t] / (m[nodes_t] + prm.eps)[:, None]
+        return oc.find_sdf(q, qv, torch.as_tensor(idx[:, 0]).long(), pos_, vel,
+                           torch.as_tensor(surf).long(), odt)[3]
+
+    def L(mv_, pos_, vel_):
+        g, e = oc.collision(prm, mv_, m, nodes_o, idx, [(pos_, vel_, surf, odt)], 1e3, 10.0, 50.0, 0.5)
+        return (g * w).sum() + (e[0] * we).sum()
+
+    args = [mv.clone().requires_grad_(True), pos.clone().requires_grad_(True),
+            vel.clone().requires_grad_(True)]
+    grads = torch.autograd.grad(L(*args), args)
+    d = [torch.tensor(rng.normal(size=a.shape), dtype=F64) for a in args]
+    hs = [1e-7 * mv.abs().max().item(), 1e-9, 1e-7]
+    f0 = flags(pos)
+    assert torch.equal(flags(pos + hs[1] * d[1]), f0) and torch.equal(flags(pos - hs[1] * d[1]), f0)
+    with torch.no_grad():
+        for k in range(3):
+            plus = [a.detach() + (hs[k] * d[k] if j == k else 0) for j, a in enumerate(args)]
+            minus = [a.detach() - (hs[k] * d[k] if j == k else 0) for j, a in enumerate(args)]
+            fd = ((L(*plus) - L(*minus)) / (2 * hs[k])).item()
+            ad = (grads[k] * d[k]).sum().item()
+            assert abs(ad - fd) <= 1e-5 * max(1.0, abs(fd)), (k, ad, fd)
