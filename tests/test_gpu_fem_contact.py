@@ -1,0 +1,216 @@
+"""GPU parity of the FEM pad and grid contact kernels against the oracle.
+
+Forward within 1e-5 relative L2 of the FP64 oracle on the same float32
+inputs; adjoints within 1e-4; closest-triangle indices bit-exact against the
+float32 brute force of the oracle.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from difftactile_amd import _lib
+from difftactile_amd.contact import GridContact
+from difftactile_amd.object_model.mpm_elastic import MPMObj
+from difftactile_amd.params import MPMParams
+from difftactile_amd.sensor_model.fem_sensor import FEMDomeSensor
+from difftactile_amd.sensor_model.meshes import dome_mesh, slab_mesh
+from oracle import contact as oc
+from oracle import coupled as ocp
+from oracle import fem as of
+from oracle import mpm as om
+from tests.mpm_cases import oracle_params, rel_l2
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+P = _lib.ptr
+
+
+def dev(a, dtype=torch.float32):
+    return torch.as_tensor(np.asarray(a)).to(dtype).cuda().contiguous()
+
+
+def h64(t):
+    return t.detach().cpu().to(F64)
+
+
+def fem_params(mu, lam, dt, mass):
+    p = _lib.DtFemParams()
+    p.mu, p.lam, p.dt, p.mass = mu, lam, dt, mass
+    return p
+
+
+def _dome(seed=0, pert=0.01):
+    nodes, f2v, surf, layer = dome_mesh()
+    rng = np.random.default_rng(seed)
+    pos = (nodes + pert * rng.normal(size=nodes.shape)).astype(np.float32)
+    return nodes.astype(np.float32), pos, f2v, surf, layer
+
+
+def test_fem_init_B_update_and_adjoint(gpu):
+    nodes, pos, f2v, surf, layer = _dome()
+    nt, nv = len(f2v), len(nodes)
+    B = torch.empty(nt, 3, 3, device="cuda")
+    _lib.call("dt_fem_init_B", nt, P(dev(f2v, torch.int32), torch.int32), P(dev(nodes)), P(B),
+              _lib.stream())
+    B_ref = of.init_B(torch.tensor(nodes, dtype=F64), torch.tensor(f2v).long())
+    assert rel_l2(h64(B), B_ref) < 1e-5
+    mu, lam, dt, mass = 1294.01, 9201.11, 5e-5, 0.0147
+    prm = fem_params(mu, lam, dt, mass)
+    rng = np.random.default_rng(1)
+    vel0 = (0.01 * rng.normal(size=(nv, 3))).astype(np.float32)
+    vel = dev(vel0)
+    Bf = dev(h64(B).numpy())
+    _lib.call("dt_fem_update", prm, nt, P(dev(f2v, torch.int32), torch.int32), P(Bf), P(dev(pos)),
+              P(vel), _lib.stream())
+    posd = torch.tensor(pos, dtype=F64, requires_grad=True)
+    vref = of.update(posd, torch.tensor(vel0, dtype=F64), torch.tensor(f2v).long(), h64(Bf), mu,
+                     lam, dt, mass)
+    assert rel_l2(h64(vel) - torch.tensor(vel0, dtype=F64), vref.detach() - torch.tensor(vel0, dtype=F64)) < 1e-4
+    assert rel_l2(h64(vel), vref.detach()) < 1e-5
+    # adjoint
+    gvel = rng.normal(size=(nv, 3)).astype(np.float32)
+    gpos = torch.zeros(nv, 3, device="cuda")
+    _lib.call("dt_fem_update_adj", prm, nt, P(dev(f2v, torch.int32), torch.int32), P(Bf),
+              P(dev(pos)), P(dev(gvel)), P(gpos), _lib.stream())
+    g_ref = torch.autograd.grad((vref * torch.tensor(gvel, dtype=F64)).sum(), posd)[0]
+    assert rel_l2(h64(gpos), g_ref) < 1e-4
+
+
+@pytest.mark.parametrize("gripper", [False, True])
+def test_fem_update2_and_adjoint(gpu, gripper):
+    rng = np.random.default_rng(2)
+    nv = 1910
+    arrs = [rng.normal(size=(nv, 3)).astype(np.float32) for _ in range(6)]
+    pos, vel, vpos, ext, cv, cv1 = arrs
+    layer = rng.integers(0, 4, nv).astype(np.int32)
+    prm = fem_params(1.0, 2.0, 5e-5, 0.0147)
+    outs = [torch.empty(nv, 3, device="cuda") for _ in range(3)]
+    _lib.call("dt_fem_update2", prm, nv, P(dev(layer, torch.int32), torch.int32), P(dev(pos)),
+              P(dev(vel)), P(dev(vpos)), P(dev(ext)), P(dev(cv)),
+              P(dev(cv1)) if gripper else None, *[P(o) for o in outs], _lib.stream())
+    t = [torch.tensor(a, dtype=F64, requires_grad=True) for a in arrs]
+    L = torch.tensor(layer)
+    if gripper:
+        ref = of.update2_gripper(t[0], t[1], t[2], t[3], L, t[4], t[5], 5e-5, 0.0147)
+    else:
+        ref = of.update2_dome(t[0], t[1], t[2], t[3], L, t[4], 5e-5, 0.0147)
+    for o, r in zip(outs, ref):
+        assert rel_l2(h64(o), r.detach()) < 1e-6
+    seeds = [rng.normal(size=(nv, 3)).astype(np.float32) for _ in range(3)]
+    g = [torch.zeros(nv, 3, device="cuda") for _ in range(6)]
+    _lib.call("dt_fem_update2_adj", prm, nv, P(dev(layer, torch.int32), torch.int32),
+              *[P(dev(s)) for s in seeds], P(g[0]), P(g[1]), P(g[2]), P(g[3]), P(g[4]),
+              P(g[5]) if gripper else None, _lib.stream())
+    loss = sum((r * torch.tensor(s, dtype=F64)).sum() for r, s in zip(ref, seeds))
+    gr = torch.autograd.grad(loss, t, allow_unused=True)
+    for k, (got, want) in enumerate(zip(g, gr)):
+        if want is None or (k == 5 and not gripper):
+            continue
+        assert rel_l2(h64(got), want) < 1e-6, k
+
+
+def _contact_scene(n_grid=64, n_particles=20000, seed=0):
+    prm = MPMParams(n_grid=n_grid, rho=1000.0, dt=2e-5)
+    h = 1.0 / 128
+    nodes, f2v, surf, layer, od = slab_mesh(24, 24, 4, h=h, origin=(0.4, 0.52, 0.4))
+    rng = np.random.default_rng(seed)
+    pos = (nodes + 2e-4 * rng.normal(size=nodes.shape)).astype(np.float32)
+    x = rng.uniform([0.4, 0.44, 0.4], [0.58, 0.56, 0.58], size=(n_particles, 3)).astype(np.float32)
+    v = rng.normal(0, 0.2, size=(n_particles, 3)).astype(np.float32)
+    return prm, (nodes, f2v, surf, layer, od), pos, x, v
+
+
+def test_closest_triangle_bit_exact(gpu):
+    prm, (nodes, f2v, surf, layer, od), pos, x, v = _contact_scene()
+    op = oracle_params(prm)
+    mats = om.material_arrays([om.Material(5e4, 0.3)], np.zeros(len(x), int), F64)
+    Fi = torch.eye(3, dtype=F64).repeat(len(x), 1, 1)
+    U, s, V = om.svd3(Fi)
+    mv, m, _ = om.p2g(op, torch.tensor(x, dtype=F64), torch.tensor(v, dtype=F64),
+                      torch.zeros_like(Fi), Fi, U, s, V, mats)
+    m32 = m.float()
+    nn = prm.n_nodes
+    cen = torch.empty(len(surf), 4, device="cuda")
+    _lib.call("dt_contact_centroids", len(surf), P(dev(surf, torch.int32), torch.int32),
+              P(dev(pos)), P(cen), _lib.stream())
+    idx = torch.full((nn, 2), -7, dtype=torch.int32, device="cuda")
+    _lib.call("dt_contact_closest", prm.c_struct(), P(dev(m32)), 1, len(surf), P(cen),
+              P(idx, torch.int32), 2, 1, _lib.stream())
+    nodes_o, ref = oc.check_collision(op, m32, [torch.tensor(pos)], [surf])
+    got = idx.cpu().numpy()
+    assert len(nodes_o) > 1000
+    np.testing.assert_array_equal(got[nodes_o, 1], ref[:, 0])
+    untouched = np.ones(nn, bool)
+    untouched[nodes_o] = False
+    assert (got[untouched, 1] == -7).all() and (got[:, 0] == -7).all()
+
+
+def _build_facade(prm, mesh, pos, x, v, S=3):
+    nodes, f2v, surf, layer, od = mesh
+    obj = MPMObj(dt=prm.dt, sub_steps=S, particles=x, n_grid=prm.n_grid, rho=prm.rho,
+                 gravity=prm.gravity)
+    obj.set_material(5e4, 0.3)
+    obj.set_state(x, v)
+    pad = FEMDomeSensor(dt=prm.dt, sub_steps=S, mesh=(nodes, f2v, surf, layer, od), E=1e5, nu=0.45,
+                        mass=2e-4)
+    pad.init()
+    pad.pos.data[0] = torch.tensor(pos).cuda()
+    pad.virtual_pos.data[0] = torch.tensor(pos).cuda()
+    cv = np.zeros((len(nodes), 3), np.float32)
+    cv[:, 1] = -0.5
+    pad.control_vel.from_numpy(cv)
+    pad.set_vel(0)
+    return obj, pad
+
+
+def test_coupled_substep_forward_and_adjoint_match_oracle(gpu):
+    prm, mesh, pos, x, v = _contact_scene(n_particles=12000, seed=3)
+    nodes, f2v, surf, layer, od = mesh
+    obj, pad = _build_facade(prm, mesh, pos, x, v)
+    sc = GridContact(obj, [pad], kn=1e3, kd=10.0, kt=50.0, friction_coeff=0.5, clamp_grad=False)
+    sc.reset()
+    sc.update(0)
+    torch.cuda.synchronize()
+    # oracle, FP64, same inputs (B and the contact indices from float32 data)
+    op = oracle_params(prm)
+    n = len(x)
+    mats = om.material_arrays([om.Material(5e4, 0.3)], np.zeros(n, int), F64)
+    st = [torch.tensor(x, dtype=F64, requires_grad=True), torch.tensor(v, dtype=F64, requires_grad=True),
+          torch.zeros(n, 3, 3, dtype=F64, requires_grad=True),
+          torch.eye(3, dtype=F64).repeat(n, 1, 1).requires_grad_(True)]
+    padd = dict(pos=torch.tensor(pos, dtype=F64, requires_grad=True),
+                vel=h64(pad.vel.data[0]).requires_grad_(True),
+                vpos=torch.tensor(pos, dtype=F64), cv=h64(pad.control_vel.data), cv_next=None,
+                f2v=torch.tensor(f2v).long(), B=h64(pad.B.data), layer=torch.tensor(layer),
+                tris=surf, out_dir=torch.tensor(od, dtype=F64), mu=float(pad.mu[None]),
+                lam=float(pad.lam[None]), mass=pad.mass, dt=prm.dt)
+    out, pads_out, idx_ref, nodes_ref = ocp.coupled_substep(op, tuple(st), mats, [padd],
+                                                           (1e3, 10.0, 50.0, 0.5))
+    got_idx = sc.contact_idx.data[0].reshape(-1, 1).cpu().numpy()
+    np.testing.assert_array_equal(got_idx[nodes_ref, 0], idx_ref[:, 0])
+    assert (obj.grid_f.data[0].abs().sum(-1) > 0).sum() > 50, "scene must make contact"
+    for fld, want in zip((obj.x_0, obj.v_0, obj.C_0, obj.F_0), out):
+        assert rel_l2(h64(fld.data[1]), want.detach()) < 1e-5
+    for fld, want in zip((pad.pos, pad.vel, pad.virtual_pos), pads_out[0]):
+        assert rel_l2(h64(fld.data[1]), want.detach()) < 1e-5
+    # adjoint: seed frame-1 gradients, run update_grad, compare frame-0 gradients
+    rng = np.random.default_rng(9)
+    seeds = {}
+    for name, fld in (("x", obj.x_0), ("v", obj.v_0), ("C", obj.C_0), ("F", obj.F_0),
+                      ("pos", pad.pos), ("vel", pad.vel), ("vpos", pad.virtual_pos)):
+        s = rng.normal(size=fld.data[1].shape).astype(np.float32)
+        fld.grad.data[1] = torch.tensor(s).cuda()
+        seeds[name] = torch.tensor(s, dtype=F64)
+    sc.update_grad(0)
+    torch.cuda.synchronize()
+    loss = sum((o * seeds[k]).sum() for o, k in zip(out, ("x", "v", "C", "F")))
+    loss = loss + sum((o * seeds[k]).sum() for o, k in zip(pads_out[0], ("pos", "vel", "vpos")))
+    leaves = st + [padd["pos"], padd["vel"]]
+    gr = torch.autograd.grad(loss, leaves)
+    gots = [obj.x_0.grad.data[0], obj.v_0.grad.data[0], obj.C_0.grad.data[0],
+            obj.F_0.grad.data[0], pad.pos.grad.data[0], pad.vel.grad.data[0]]
+    for k, (got, want) in enumerate(zip(gots, gr)):
+        err = rel_l2(h64(got), want)
+        assert err < 1e-4, (k, err)
