@@ -22,6 +22,7 @@ class DtMpmParams(ctypes.Structure):
         ("dx", ctypes.c_float), ("inv_dx", ctypes.c_float), ("dt", ctypes.c_float),
         ("p_vol", ctypes.c_float), ("p_mass", ctypes.c_float), ("eps", ctypes.c_float),
         ("gravity", ctypes.c_float * 3), ("grad_clamp", ctypes.c_float),
+        ("soa", ctypes.c_int32),
     ]
 
 
@@ -100,6 +101,8 @@ _SIGS = {
     "dt_sort_by_key": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),
     "dt_gather_rows": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     "dt_scatter_rows": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "dt_gather_rows_soa": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "dt_scatter_rows_soa": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

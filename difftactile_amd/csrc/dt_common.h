@@ -34,7 +34,7 @@ __device__ __forceinline__ int xcd_block() {
 }
 
 struct MpmP {
-  int n_grid, bound, sticky_floor, clamp_grad;
+  int n_grid, bound, sticky_floor, clamp_grad, soa;
   float dx, inv_dx, dt, p_vol, p_mass, eps;
   float g[3];
   float grad_clamp;
@@ -46,6 +46,7 @@ inline MpmP to_dev(const dt_mpm_params& p) {
   d.bound = p.bound;
   d.sticky_floor = p.sticky_floor;
   d.clamp_grad = p.clamp_grad;
+  d.soa = p.soa;
   d.dx = p.dx;
   d.inv_dx = p.inv_dx;
   d.dt = p.dt;

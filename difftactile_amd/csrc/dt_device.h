@@ -115,6 +115,29 @@ DT_HD void add_m3(float* __restrict__ p, const M3& m) { for (int i = 0; i < 9; +
 
 DT_HD void atomic_add_f(float* p, float v) { unsafeAtomicAdd(p, v); }
 
+// Particle array layout: element (p, c) at p*sp + c*sc.  AoS (the reference's
+// [n][k] layout): sp = k, sc = 1.  SoA ([k][n], coalesced per component):
+// sp = 1, sc = n.
+struct PL {
+  long long sp, sc;
+};
+DT_HD PL pl_vec(int soa, long long n) { return soa ? PL{1, n} : PL{3, 1}; }
+DT_HD PL pl_mat(int soa, long long n) { return soa ? PL{1, n} : PL{9, 1}; }
+DT_HD void ld_v3(const float* __restrict__ b, long long p, PL L, float (&o)[3]) {
+  for (int c = 0; c < 3; ++c) o[c] = b[p * L.sp + c * L.sc];
+}
+DT_HD void st_v3(float* __restrict__ b, long long p, PL L, const float (&o)[3]) {
+  for (int c = 0; c < 3; ++c) b[p * L.sp + c * L.sc] = o[c];
+}
+DT_HD M3 ld_m3L(const float* __restrict__ b, long long p, PL L) {
+  M3 r;
+  for (int c = 0; c < 9; ++c) r.a[c] = b[p * L.sp + c * L.sc];
+  return r;
+}
+DT_HD void st_m3L(float* __restrict__ b, long long p, PL L, const M3& m) {
+  for (int c = 0; c < 9; ++c) b[p * L.sp + c * L.sc] = m.a[c];
+}
+
 // ----------------------------------------------------------------------------
 // 3x3 SVD: U, V in SO(3), sigma_1 >= sigma_2 >= |sigma_3|, sign on sigma_3.
 //
@@ -240,7 +263,7 @@ struct Stencil {
   float dw[3][3];  // d w / d fx
 };
 
-DT_HD void make_stencil(const float* x, float inv_dx, int n_grid, Stencil& s) {
+DT_HD void make_stencil(const float (&x)[3], float inv_dx, int n_grid, Stencil& s) {
 #pragma clang fp contract(off)
   for (int a = 0; a < 3; ++a) {
     float t = x[a] * inv_dx;
@@ -264,7 +287,7 @@ DT_HD void make_stencil(const float* x, float inv_dx, int n_grid, Stencil& s) {
 
 // cell key used for particle sorting: 4x4x4-cell bricks in row-major order,
 // cells row-major inside a brick.  key = brick * 64 + local.
-DT_HD int cell_key(const float* x, float inv_dx, int n_grid) {
+DT_HD int cell_key(const float (&x)[3], float inv_dx, int n_grid) {
   int b[3];
   for (int a = 0; a < 3; ++a) {
     int v = cell_base1(x[a], inv_dx);

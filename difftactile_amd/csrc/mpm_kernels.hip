@@ -57,6 +57,22 @@ __device__ __forceinline__ void tile_setup(bool active, const int (&base)[3], in
 
 __device__ __forceinline__ int node_index(int i, int j, int k, int n) { return (i * n + j) * n + k; }
 
+// weight / weight derivative of stencil offset o (runtime) along axis 0,
+// recomputed from the scalar fx with the same expressions as make_stencil
+// (selecting from the weight array by a runtime index would demote the stencil
+// to scratch memory)
+__device__ __forceinline__ float w_axis0(const Stencil& s, int o) {
+  float f = s.fx[0];
+  float d0 = 1.5f - f, d1 = f - 1.f, d2 = f - 0.5f;
+  float a = 0.5f * (d0 * d0), b = 0.75f - d1 * d1, c = 0.5f * (d2 * d2);
+  return o == 0 ? a : (o == 1 ? b : c);
+}
+__device__ __forceinline__ float dw_axis0(const Stencil& s, int o) {
+  float f = s.fx[0];
+  float a = -(1.5f - f), b = -2.f * (f - 1.f), c = f - 0.5f;
+  return o == 0 ? a : (o == 1 ? b : c);
+}
+
 // ----------------------------------------------------------------------------
 // per-particle helpers
 // ----------------------------------------------------------------------------
@@ -294,7 +310,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmP P, const dt_material* __res
     stress_fwd(fn, u, sig, vv, m, st, tau);
     A = affine_of(P, tau, ld_m3(C + 9ll * p));
     st_m3(F_next + 9ll * p, st.Fp);
-    make_stencil(x + 3ll * p, P.inv_dx, P.n_grid, s);
+    float xp[3] = {x[3ll * p], x[3ll * p + 1], x[3ll * p + 2]};
+    make_stencil(xp, P.inv_dx, P.n_grid, s);
     for (int a = 0; a < 3; ++a) pv[a] = P.p_mass * v[3ll * p + a];
   }
   p2g_block(P, active, s, pv, A, g, tile, sh);
@@ -319,7 +336,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_fused(MpmP P, const dt_material*
   float pv[3] = {0.f, 0.f, 0.f};
   s.base[0] = s.base[1] = s.base[2] = 0;
   if (active) {
-    M3 c = ld_m3(C + 9ll * p), f = ld_m3(F + 9ll * p);
+    PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
+    M3 c = ld_m3L(C, p, Lm), f = ld_m3L(F, p, Lm);
     M3 a = m3_eye();
     for (int i = 0; i < 9; ++i) a.a[i] += P.dt * c.a[i];
     M3 fn = m3_mul(a, f);
@@ -331,9 +349,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g_fused(MpmP P, const dt_material*
     M3 tau;
     stress_fwd(fn, u, sig, vv, m, st, tau);
     A = affine_of(P, tau, c);
-    st_m3(F_next + 9ll * p, st.Fp);
-    make_stencil(x + 3ll * p, P.inv_dx, P.n_grid, s);
-    for (int q = 0; q < 3; ++q) pv[q] = P.p_mass * v[3ll * p + q];
+    st_m3L(F_next, p, Lm, st.Fp);
+    float xp[3], vp[3];
+    ld_v3(x, p, Lv, xp);
+    ld_v3(v, p, Lv, vp);
+    make_stencil(xp, P.inv_dx, P.n_grid, s);
+    for (int q = 0; q < 3; ++q) pv[q] = P.p_mass * vp[q];
   }
   p2g_block(P, active, s, pv, A, g, tile, sh);
 }
@@ -351,8 +372,9 @@ __device__ __forceinline__ void p2g_gather_adj(const MpmP& P, const Stencil& s,
   float gfx[3] = {0.f, 0.f, 0.f};
   float gd[3] = {0.f, 0.f, 0.f};
   gA = m3_zero();
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const float wi = w_axis0(s, i), dwi = dw_axis0(s, i);
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -360,7 +382,7 @@ __device__ __forceinline__ void p2g_gather_adj(const MpmP& P, const Stencil& s,
         long long nd = node_index(s.base[0] + i, s.base[1] + j, s.base[2] + k, n);
         float g0 = gmv[nd * sgmv + 0], g1 = gmv[nd * sgmv + 1], g2 = gmv[nd * sgmv + 2];
         float g3 = gm[nd * sgm];
-        float wi = s.w[i][0], wj = s.w[j][1], wk = s.w[k][2];
+        float wj = s.w[j][1], wk = s.w[k][2];
         float w = wi * wj * wk;
         float d0 = ((float)i - s.fx[0]) * P.dx;
         float d1 = ((float)j - s.fx[1]) * P.dx;
@@ -370,7 +392,7 @@ __device__ __forceinline__ void p2g_gather_adj(const MpmP& P, const Stencil& s,
         float c2 = pv[2] + (A.a[6] * d0 + A.a[7] * d1 + A.a[8] * d2);
         // d/dw
         float gw = g0 * c0 + g1 * c1 + g2 * c2 + g3 * P.p_mass;
-        gfx[0] += gw * s.dw[i][0] * wj * wk;
+        gfx[0] += gw * dwi * wj * wk;
         gfx[1] += gw * wi * s.dw[j][1] * wk;
         gfx[2] += gw * wi * wj * s.dw[k][2];
         // d/d(p_mass v)
@@ -386,6 +408,7 @@ __device__ __forceinline__ void p2g_gather_adj(const MpmP& P, const Stencil& s,
         gd[1] += A.a[1] * wg0 + A.a[4] * wg1 + A.a[7] * wg2;
         gd[2] += A.a[2] * wg0 + A.a[5] * wg1 + A.a[8] * wg2;
       }
+  }
   // fx = x inv_dx - base;  dpos = (o - fx) dx
   for (int a = 0; a < 3; ++a) {
     gv[a] *= P.p_mass;
@@ -424,7 +447,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
   M3 c = ld_m3(C + 9ll * p);
   M3 A = affine_of(P, tau, c);
   Stencil s;
-  make_stencil(x + 3ll * p, P.inv_dx, P.n_grid, s);
+  float xp[3] = {x[3ll * p], x[3ll * p + 1], x[3ll * p + 2]};
+  make_stencil(xp, P.inv_dx, P.n_grid, s);
   float pv[3];
   for (int a = 0; a < 3; ++a) pv[a] = P.p_mass * v[3ll * p + a];
   float lgx[3] = {0.f, 0.f, 0.f}, lgv[3] = {0.f, 0.f, 0.f};
@@ -532,7 +556,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_fused_adj(
     float* __restrict__ gC_out, float* __restrict__ gF_out) {
   int p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  M3 c = ld_m3(C + 9ll * p), f = ld_m3(F + 9ll * p);
+  PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
+  M3 c = ld_m3L(C, p, Lm), f = ld_m3L(F, p, Lm);
   M3 a = m3_eye();
   for (int i = 0; i < 9; ++i) a.a[i] += P.dt * c.a[i];
   M3 fn = m3_mul(a, f);
@@ -545,27 +570,29 @@ __global__ __launch_bounds__(kBlock) void k_p2g_fused_adj(
   stress_fwd(fn, u, sig, vv, m, st, tau);
   M3 A = affine_of(P, tau, c);
   Stencil s;
-  make_stencil(x + 3ll * p, P.inv_dx, P.n_grid, s);
+  float xp[3], vp[3];
+  ld_v3(x, p, Lv, xp);
+  ld_v3(v, p, Lv, vp);
+  make_stencil(xp, P.inv_dx, P.n_grid, s);
   float pv[3];
-  for (int q = 0; q < 3; ++q) pv[q] = P.p_mass * v[3ll * p + q];
-  float lgx[3] = {gx_io[3ll * p + 0], gx_io[3ll * p + 1], gx_io[3ll * p + 2]};
+  for (int q = 0; q < 3; ++q) pv[q] = P.p_mass * vp[q];
+  float lgx[3];
+  ld_v3(gx_io, p, Lv, lgx);
   float lgv[3] = {0.f, 0.f, 0.f};
   M3 gA;
   p2g_gather_adj(P, s, pv, A, ggrid4, 4, ggrid4 + 3, 4, lgx, lgv, gA);
   M3 gtau, lgC;
   affine_adj(P, gA, gtau, lgC);
   M3 gFn;
-  stress_adj(fn, u, sig, vv, m, st, gtau, ld_m3(gFnext + 9ll * p), gFn);
+  stress_adj(fn, u, sig, vv, m, st, gtau, ld_m3L(gFnext, p, Lm), gFn);
   // F_new = (I + dt C) F
   M3 gc2 = m3_mul_bt(gFn, f);
   for (int i = 0; i < 9; ++i) lgC.a[i] += P.dt * gc2.a[i];
   M3 gf = m3_mul_at(a, gFn);
-  for (int q = 0; q < 3; ++q) {
-    gx_io[3ll * p + q] = lgx[q];
-    gv_out[3ll * p + q] = lgv[q];
-  }
-  st_m3(gC_out + 9ll * p, lgC);
-  st_m3(gF_out + 9ll * p, gf);
+  st_v3(gx_io, p, Lv, lgx);
+  st_v3(gv_out, p, Lv, lgv);
+  st_m3L(gC_out, p, Lm, lgC);
+  st_m3L(gF_out, p, Lm, gf);
 }
 
 // ----------------------------------------------------------------------------
@@ -729,15 +756,16 @@ __device__ __forceinline__ void g2p_core(const MpmP& P, const Stencil& s,
   const int n = P.n_grid;
   nv[0] = nv[1] = nv[2] = 0.f;
   nC = m3_zero();
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const float wi = w_axis0(s, i);
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         long long nd = node_index(s.base[0] + i, s.base[1] + j, s.base[2] + k, n);
         float g0 = gv[nd * sgv], g1 = gv[nd * sgv + 1], g2 = gv[nd * sgv + 2];
-        float w = s.w[i][0] * s.w[j][1] * s.w[k][2];
+        float w = wi * s.w[j][1] * s.w[k][2];
         float d[3] = {(float)i - s.fx[0], (float)j - s.fx[1], (float)k - s.fx[2]};
         nv[0] += w * g0;
         nv[1] += w * g1;
@@ -747,6 +775,7 @@ __device__ __forceinline__ void g2p_core(const MpmP& P, const Stencil& s,
         for (int r = 0; r < 3; ++r)
           for (int q = 0; q < 3; ++q) nC.a[r * 3 + q] += kw * g[r] * d[q];
       }
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __restrict__ x,
@@ -755,16 +784,19 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
                                                 float* __restrict__ C1) {
   int p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
   Stencil s;
-  make_stencil(x + 3ll * p, P.inv_dx, P.n_grid, s);
+  float xp[3];
+  ld_v3(x, p, Lv, xp);
+  make_stencil(xp, P.inv_dx, P.n_grid, s);
   float nv[3];
   M3 nC;
   g2p_core(P, s, gv, sgv, nv, nC);
-  for (int a = 0; a < 3; ++a) {
-    v1[3ll * p + a] = nv[a];
-    x1[3ll * p + a] = x[3ll * p + a] + P.dt * nv[a];
-  }
-  st_m3(C1 + 9ll * p, nC);
+  float nx[3];
+  for (int a = 0; a < 3; ++a) nx[a] = xp[a] + P.dt * nv[a];
+  st_v3(v1, p, Lv, nv);
+  st_v3(x1, p, Lv, nx);
+  st_m3L(C1, p, Lm, nC);
 }
 
 // g2p adjoint: scatter to the grid-velocity adjoint, gather gx.
@@ -786,13 +818,18 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
   s.base[0] = s.base[1] = s.base[2] = 0;
   float gvt[3] = {0.f, 0.f, 0.f}, lgx[3] = {0.f, 0.f, 0.f};
   M3 gc = m3_zero();
+  PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
   if (active) {
-    make_stencil(x + 3ll * p, P.inv_dx, nG, s);
+    float xp[3], g1[3], v1[3];
+    ld_v3(x, p, Lv, xp);
+    ld_v3(gx1, p, Lv, g1);
+    ld_v3(gv1, p, Lv, v1);
+    make_stencil(xp, P.inv_dx, nG, s);
     for (int a = 0; a < 3; ++a) {
-      gvt[a] = gv1[3ll * p + a] + P.dt * gx1[3ll * p + a];
-      lgx[a] = gx1[3ll * p + a];
+      gvt[a] = v1[a] + P.dt * g1[a];
+      lgx[a] = g1[a];
     }
-    gc = ld_m3(gC1 + 9ll * p);
+    gc = ld_m3L(gC1, p, Lm);
   }
   Tile t;
   int base[3] = {s.base[0], s.base[1], s.base[2]};
@@ -804,8 +841,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
   if (active) {
     float gfx[3] = {0.f, 0.f, 0.f}, gd[3] = {0.f, 0.f, 0.f};
     float k4 = 4.f * P.inv_dx;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      const float wi = w_axis0(s, i), dwi = dw_axis0(s, i);
 #pragma unroll
       for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -813,7 +851,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
           int bi = s.base[0] + i, bj = s.base[1] + j, bk = s.base[2] + k;
           long long nd = node_index(bi, bj, bk, nG);
           float g0 = gv[nd * sgv], g1 = gv[nd * sgv + 1], g2 = gv[nd * sgv + 2];
-          float wi = s.w[i][0], wj = s.w[j][1], wk = s.w[k][2];
+          float wj = s.w[j][1], wk = s.w[k][2];
           float w = wi * wj * wk;
           float d0 = (float)i - s.fx[0], d1 = (float)j - s.fx[1], d2 = (float)k - s.fx[2];
           // gC d
@@ -834,7 +872,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
             atomic_add_f(ggv + nd * sggv + 2, o2);
           }
           float gw = g0 * gvt[0] + g1 * gvt[1] + g2 * gvt[2] + k4 * (g0 * cd0 + g1 * cd1 + g2 * cd2);
-          gfx[0] += gw * s.dw[i][0] * wj * wk;
+          gfx[0] += gw * dwi * wj * wk;
           gfx[1] += gw * wi * s.dw[j][1] * wk;
           gfx[2] += gw * wi * wj * s.dw[k][2];
           // d/d d of k4 w g^T gC d = k4 w gC^T g
@@ -842,11 +880,13 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
           gd[1] += k4 * w * (gc.a[1] * g0 + gc.a[4] * g1 + gc.a[7] * g2);
           gd[2] += k4 * w * (gc.a[2] * g0 + gc.a[5] * g1 + gc.a[8] * g2);
         }
+    }
     // fx = x inv_dx - base; d = o - fx
     for (int a = 0; a < 3; ++a) lgx[a] += P.inv_dx * gfx[a] - P.inv_dx * gd[a];
     for (int a = 0; a < 3; ++a) {
-      if (kGxAssign) gx[3ll * p + a] = lgx[a];
-      else gx[3ll * p + a] += lgx[a];
+      long long o = p * Lv.sp + a * Lv.sc;
+      if (kGxAssign) gx[o] = lgx[a];
+      else gx[o] += lgx[a];
     }
   }
   if (t.use) {
@@ -862,7 +902,9 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(MpmP P, int n, const float
                                                       int32_t* __restrict__ keys) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  keys[p] = cell_key(x + 3ll * p, P.inv_dx, P.n_grid);
+  float xp[3];
+  ld_v3(x, p, pl_vec(P.soa, n), xp);
+  keys[p] = cell_key(xp, P.inv_dx, P.n_grid);
 }
 
 __global__ __launch_bounds__(kBlock) void k_gather_rows(long long total, int width,
@@ -874,6 +916,32 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(long long total, int wid
   long long i = e / width;
   int c = (int)(e % width);
   dst[e] = src[(long long)perm[i] * width + c];
+}
+
+// [n][w] (rows picked by perm, or identity when perm is NULL) -> [w][n]
+__global__ __launch_bounds__(kBlock) void k_gather_soa(long long n, int width,
+                                                       const int32_t* __restrict__ perm,
+                                                       const float* __restrict__ src,
+                                                       float* __restrict__ dst) {
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * width) return;
+  int c = (int)(e / n);
+  long long i = e % n;
+  long long r = perm ? (long long)perm[i] : i;
+  dst[e] = src[r * width + c];
+}
+
+// [w][n] -> [n][w] rows placed at perm[i] (or i when perm is NULL)
+__global__ __launch_bounds__(kBlock) void k_scatter_soa(long long n, int width,
+                                                        const int32_t* __restrict__ perm,
+                                                        const float* __restrict__ src,
+                                                        float* __restrict__ dst) {
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * width) return;
+  int c = (int)(e / n);
+  long long i = e % n;
+  long long r = perm ? (long long)perm[i] : i;
+  dst[r * width + c] = src[e];
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter_rows(long long total, int width,
@@ -1005,6 +1073,7 @@ int dt_mpm_p2g(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats
   if ((rc = check_mats(mats, n_mats))) return rc;
   DT_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return DT_OK;
+  DT_REQUIRE(prm->soa == 0, "dt_mpm_p2g reads the reference [n][k] layout (soa must be 0)");
   DT_REQUIRE(x && v && C && F_new && U && S && V && grid_mv && grid_m && F_next, "NULL pointer");
   Grid g{grid_mv, grid_m, 3, 1};
   hipLaunchKernelGGL(k_p2g, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream), to_dev(*prm),
@@ -1023,6 +1092,7 @@ int dt_mpm_p2g_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   if ((rc = check_mats(mats, n_mats))) return rc;
   DT_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return DT_OK;
+  DT_REQUIRE(prm->soa == 0, "dt_mpm_p2g_adj reads the reference [n][k] layout (soa must be 0)");
   DT_REQUIRE(x && v && C && F_new && U && S && V && g_grid_mv && g_grid_m && gF_next && gx && gv &&
                  gC && gF_new && gU && gS && gV,
              "NULL pointer");
@@ -1330,6 +1400,28 @@ int dt_scatter_rows(int32_t n, int32_t width, const int32_t* perm, const float* 
   hipLaunchKernelGGL(k_scatter_rows, dim3(nblocks(total)), dim3(kBlock), 0, DT_STREAM(stream),
                      total, width, perm, src, dst);
   return check_launch("dt_scatter_rows");
+}
+
+int dt_gather_rows_soa(int32_t n, int32_t width, const int32_t* perm, const float* src_rows,
+                       float* dst_soa, void* stream) {
+  DT_REQUIRE(n >= 0 && width > 0, "bad size");
+  if (n == 0) return DT_OK;
+  DT_REQUIRE(src_rows && dst_soa && src_rows != dst_soa, "NULL or aliased pointer");
+  long long total = (long long)n * width;
+  hipLaunchKernelGGL(k_gather_soa, dim3(nblocks(total)), dim3(kBlock), 0, DT_STREAM(stream),
+                     (long long)n, width, perm, src_rows, dst_soa);
+  return check_launch("dt_gather_rows_soa");
+}
+
+int dt_scatter_rows_soa(int32_t n, int32_t width, const int32_t* perm, const float* src_soa,
+                        float* dst_rows, void* stream) {
+  DT_REQUIRE(n >= 0 && width > 0, "bad size");
+  if (n == 0) return DT_OK;
+  DT_REQUIRE(src_soa && dst_rows && src_soa != dst_rows, "NULL or aliased pointer");
+  long long total = (long long)n * width;
+  hipLaunchKernelGGL(k_scatter_soa, dim3(nblocks(total)), dim3(kBlock), 0, DT_STREAM(stream),
+                     (long long)n, width, perm, src_soa, dst_rows);
+  return check_launch("dt_scatter_rows_soa");
 }
 
 }  // extern "C"

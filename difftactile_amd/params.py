@@ -47,7 +47,7 @@ class MPMParams:
     def n_nodes(self):
         return self.n_grid ** 3
 
-    def c_struct(self):
+    def c_struct(self, soa=False):
         p = _lib.DtMpmParams()
         p.n_grid = self.n_grid
         p.bound = self.bound
@@ -62,6 +62,7 @@ class MPMParams:
         for i in range(3):
             p.gravity[i] = self.gravity[i]
         p.grad_clamp = self.grad_clamp
+        p.soa = int(soa)
         return p
 
     def key_bits(self):

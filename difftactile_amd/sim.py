@@ -24,7 +24,10 @@ class MPMSim:
     def __init__(self, prm, materials, n_particles, mat_id=None, device="cuda",
                  checkpoint_every=50, sort=True):
         self.prm = prm
-        self.c_prm = prm.c_struct()
+        # fused kernels run on the [k][n] (SoA) particle layout; the caller's
+        # [n][k] arrays are transposed (and cell-sorted) at the boundary
+        self.c_prm = prm.c_struct(soa=True)
+        self.c_prm_aos = prm.c_struct(soa=False)
         self.n = int(n_particles)
         self.device = torch.device(device)
         self.materials = list(materials)
@@ -62,7 +65,7 @@ class MPMSim:
         keys = self._buf("keys", (n,), torch.int32)
         keys_out = torch.empty(n, dtype=torch.int32, device=self.device)
         perm = torch.empty(n, dtype=torch.int32, device=self.device)
-        _lib.call("dt_mpm_cell_keys", self.c_prm, n, _lib.ptr(x), _lib.ptr(keys, torch.int32),
+        _lib.call("dt_mpm_cell_keys", self.c_prm_aos, n, _lib.ptr(x), _lib.ptr(keys, torch.int32),
                   _lib.stream())
         tb = int(_lib.lib().dt_sort_tmp_bytes(n))
         tmp = self._buf("sort_tmp", (max(tb, 1),), torch.uint8)
@@ -72,20 +75,28 @@ class MPMSim:
         return keys_out, perm
 
     def _gather(self, perm, t, width):
-        out = torch.empty_like(t)
-        _lib.call("dt_gather_rows", self.n, width, _lib.ptr(perm, torch.int32), _lib.ptr(t),
-                  _lib.ptr(out), _lib.stream())
+        """[n][width] in caller order -> [width][n] in sorted order."""
+        out = torch.empty((width, self.n), dtype=torch.float32, device=self.device)
+        _lib.call("dt_gather_rows_soa", self.n, width,
+                  None if perm is None else _lib.ptr(perm, torch.int32),
+                  _lib.ptr(t.contiguous()), _lib.ptr(out), _lib.stream())
         return out
 
-    def _scatter(self, perm, t, width):
-        out = torch.empty_like(t)
-        _lib.call("dt_scatter_rows", self.n, width, _lib.ptr(perm, torch.int32), _lib.ptr(t),
+    def _scatter(self, perm, t, width, shape):
+        """[width][n] in sorted order -> caller-order tensor of `shape`."""
+        out = torch.empty(shape, dtype=torch.float32, device=self.device)
+        _lib.call("dt_scatter_rows_soa", self.n, width,
+                  None if perm is None else _lib.ptr(perm, torch.int32), _lib.ptr(t),
                   _lib.ptr(out), _lib.stream())
         return out
 
     def _gather_i(self, perm, t):
         # int32 ids gathered through the float path (bit pattern preserved)
-        return self._gather(perm, t.view(torch.float32), 1).view(torch.int32)
+        out = torch.empty_like(t)
+        _lib.call("dt_gather_rows", self.n, 1, _lib.ptr(perm, torch.int32),
+                  _lib.ptr(t.view(torch.float32)), _lib.ptr(out.view(torch.float32)),
+                  _lib.stream())
+        return out
 
     # -- raw segment kernels ----------------------------------------------
     def _mid(self, mat_id):
@@ -129,8 +140,8 @@ class MPMSim:
 
     def _frames(self, k, prefix="F"):
         n = self.n
-        return (self._buf(prefix + "x", (k + 1, n, 3)), self._buf(prefix + "v", (k + 1, n, 3)),
-                self._buf(prefix + "C", (k + 1, n, 3, 3)), self._buf(prefix + "F", (k + 1, n, 3, 3)))
+        return (self._buf(prefix + "x", (k + 1, 3, n)), self._buf(prefix + "v", (k + 1, 3, n)),
+                self._buf(prefix + "C", (k + 1, 9, n)), self._buf(prefix + "F", (k + 1, 9, n)))
 
     # -- public API ------------------------------------------------------
     def rollout(self, x, v, C, F, n_substeps):
@@ -152,56 +163,48 @@ def _segments(n_substeps, k):
 
 
 class _Rollout(torch.autograd.Function):
+    WIDTHS = (3, 3, 9, 9)
+
     @staticmethod
     def forward(ctx, sim, n_substeps, x, v, C, F):
         for t in (x, v, C, F):
             if t.device.type != "cuda" or t.dtype != torch.float32:
                 raise ValueError("state tensors must be float32 on the GPU")
         x, v, C, F = (t.contiguous() for t in (x, v, C, F))
-        if sim.sort and sim.n > 0:
-            _, perm = sim.sort_order(x)
-            state = [sim._gather(perm, x, 3), sim._gather(perm, v, 3),
-                     sim._gather(perm, C, 9), sim._gather(perm, F, 9)]
-            mat_id = sim._gather_i(perm, sim.mat_id) if sim.mat_id is not None else None
+        shapes = [t.shape for t in (x, v, C, F)]
+        perm = sim.sort_order(x)[1] if (sim.sort and sim.n > 0) else None
+        state = [sim._gather(perm, t, w) for t, w in zip((x, v, C, F), _Rollout.WIDTHS)]
+        if sim.mat_id is not None and perm is not None:
+            mat_id = sim._gather_i(perm, sim.mat_id)
         else:
-            perm = None
-            state = [x.clone(), v.clone(), C.clone(), F.clone()]
             mat_id = sim.mat_id
         ckpts = []
         for (s, k) in _segments(n_substeps, sim.checkpoint_every):
             if ctx is not None:
                 ckpts.append([t.clone() for t in state])
             sim.advance(state, k, mat_id)
-        out = state
-        if perm is not None:
-            out = [sim._scatter(perm, out[0], 3), sim._scatter(perm, out[1], 3),
-                   sim._scatter(perm, out[2], 9), sim._scatter(perm, out[3], 9)]
+        out = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(state, _Rollout.WIDTHS, shapes)]
         if ctx is not None:
             ctx.sim = sim
             ctx.n_substeps = n_substeps
             ctx.perm = perm
             ctx.mat_id = mat_id
             ctx.ckpts = ckpts
+            ctx.shapes = shapes
         return tuple(out)
 
     @staticmethod
     def backward(ctx, gx, gv, gC, gF):
         sim = ctx.sim
         perm = ctx.perm
-        g = [t if t is not None else torch.zeros_like(r)
-             for t, r in zip((gx, gv, gC, gF), ctx.ckpts[0])]
-        g = [t.contiguous().float() for t in g]
-        widths = (3, 3, 9, 9)
-        if perm is not None:
-            g = [sim._gather(perm, t, w) for t, w in zip(g, widths)]
-        else:
-            g = [t.clone() for t in g]
+        g = [t if t is not None else torch.zeros(sh, device=sim.device)
+             for t, sh in zip((gx, gv, gC, gF), ctx.shapes)]
+        g = [sim._gather(perm, t.float(), w) for t, w in zip(g, _Rollout.WIDTHS)]
         segs = _segments(ctx.n_substeps, sim.checkpoint_every)
         for (s, k), ck in zip(reversed(segs), reversed(ctx.ckpts)):
             X, V, Cb, Fb = sim._frames(k, "R%d" % k)
             X[0].copy_(ck[0]); V[0].copy_(ck[1]); Cb[0].copy_(ck[2]); Fb[0].copy_(ck[3])
             G4, GV4 = sim.run_frames(X, V, Cb, Fb, k, ctx.mat_id, keep_grids=True)
             sim.adj_frames(X, V, Cb, Fb, k, ctx.mat_id, G4, GV4, g)
-        if perm is not None:
-            g = [sim._scatter(perm, t, w) for t, w in zip(g, widths)]
+        g = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(g, _Rollout.WIDTHS, ctx.shapes)]
         return (None, None, *g)

@@ -48,6 +48,10 @@ typedef struct dt_mpm_params {
   float dx, inv_dx, dt, p_vol, p_mass, eps; /* eps: mass threshold, 1e-5           */
   float gravity[3];
   float grad_clamp;     /* 1000 in the reference                                    */
+  int32_t soa;          /* particle arrays: 0 = [n][k] (reference layout), 1 = [k][n].
+                           Honoured by dt_mpm_g2p*, dt_mpm_p2g_fused*, dt_mpm_substep_*,
+                           dt_mpm_advance, dt_mpm_run_* and dt_mpm_cell_keys; the other
+                           per-kernel entry points require 0.                       */
 } dt_mpm_params;
 
 /* One row of the material table (multi_obj.py:65-74 generalised). */
@@ -230,6 +234,12 @@ int dt_gather_rows(int32_t n, int32_t width, const int32_t* perm, const float* s
 /* dst[perm[i], :] = src[i, :]. */
 int dt_scatter_rows(int32_t n, int32_t width, const int32_t* perm, const float* src, float* dst,
                     void* stream);
+/* [n][width] -> [width][n]: dst[c*n + i] = src[perm[i]*width + c] (perm NULL = identity). */
+int dt_gather_rows_soa(int32_t n, int32_t width, const int32_t* perm, const float* src_rows,
+                       float* dst_soa, void* stream);
+/* [width][n] -> [n][width]: dst[perm[i]*width + c] = src[c*n + i] (perm NULL = identity). */
+int dt_scatter_rows_soa(int32_t n, int32_t width, const int32_t* perm, const float* src_soa,
+                        float* dst_rows, void* stream);
 
 
 /* ---- FEM pad (sensor_model/fem_sensor.py, gripper_fem.py) ----
