@@ -91,8 +91,12 @@ def kernel_breakdown(torch, sim, state, mat_id, reps):
         _lib.call("dt_mpm_p2g_fused", prm, mats, nm, n, mid, P(x), P(v), P(C), P(F), P(g4b),
                   P(F1), st)
 
+    nb = int(_lib.lib().dt_mpm_brick_count(prm))
+    flags = torch.zeros(nb, dtype=torch.int32, device=dev)
+    _lib.call("dt_mpm_mark_bricks", prm, n, x, flags, st)
+
     def grid_op():
-        _lib.call("dt_mpm_grid_op4", prm, P(g4), None, P(gv4), st)
+        _lib.call("dt_mpm_grid_op4_bricks", prm, flags, g4, gv4, st)
 
     def g2p():
         _lib.call("dt_mpm_g2p4", prm, n, P(x), P(gv4), P(x1), P(v1), P(C1), st)
@@ -102,15 +106,22 @@ def kernel_breakdown(torch, sim, state, mat_id, reps):
                   P(gnext[2]), P(gout[0]), P(ggv4), st)
 
     def grid_op_adj():
-        _lib.call("dt_mpm_grid_op4_adj", prm, P(g4), None, P(ggv4), P(gg4), None, st)
+        _lib.call("dt_mpm_grid_op4_adj_bricks", prm, flags, g4, ggv4, gg4, st)
 
     def p2g_adj():
         _lib.call("dt_mpm_p2g_fused_adj", prm, mats, nm, n, mid, P(x), P(v), P(C), P(F), P(gg4),
                   P(gnext[3]), P(gout[0]), P(gout[1]), P(gout[2]), P(gout[3]), st)
 
+    def mark():
+        _lib.call("dt_mpm_mark_bricks", prm, n, x, flags, st)
+
+    def zero():
+        _lib.call("dt_mpm_zero4_bricks", prm, flags, ggv4, st)
+
     # order matters for valid inputs: g2p_adj fills ggv4 before grid_op_adj
-    stages = [("p2g_fused", p2g), ("grid_op4", grid_op), ("g2p4", g2p), ("g2p4_adj", g2p_adj),
-              ("grid_op4_adj", grid_op_adj), ("p2g_fused_adj", p2g_adj)]
+    stages = [("mark_bricks", mark), ("zero4_bricks", zero), ("p2g_fused", p2g),
+              ("grid_op4_bricks", grid_op), ("g2p4", g2p), ("g2p4_adj", g2p_adj),
+              ("grid_op4_adj_bricks", grid_op_adj), ("p2g_fused_adj", p2g_adj)]
     out = {}
     for name, fn in stages:
         out[name] = time_stage(torch, fn, reps)
@@ -129,12 +140,14 @@ def algorithmic_bytes(n, active, has_mat):
         # read x v C F (+id), write F_next; flush (mv, m) per active node
         "p2g_fused": n * (96 + mid + 36) + active * 16,
         # read (mv, m), write (v) per active node (dense pass reads every node's m)
-        "grid_op4": active * 32,
+        "mark_bricks": n * 12,
+        "zero4_bricks": active * 16,
+        "grid_op4_bricks": active * 32,
         # read x, write x v C; gather v per active node
         "g2p4": n * (12 + 60) + active * 16,
         # read x, gx' gv' gC' (60), write gx; scatter g_v per active node
         "g2p4_adj": n * (12 + 60 + 12) + active * 16,
-        "grid_op4_adj": active * 48,
+        "grid_op4_adj_bricks": active * 48,
         # read x v C F (+id), gx partial, gF'; write gx gv gC gF; gather (g_mv, g_m)
         "p2g_fused_adj": n * (96 + mid + 12 + 36 + 96) + active * 16,
     }

@@ -67,11 +67,16 @@ _SIGS = {
     "dt_mpm_substep_fwd": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 10 + [_vp]),
     "dt_mpm_substep_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 16 + [_vp]),
     "dt_mpm_run_fwd": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
-                                _c_int, _vp, _vp, _vp, _vp, _vp]),
+                                _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dt_mpm_run_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
-                                _c_int] + [_vp] * 12 + [_vp]),
+                                _c_int] + [_vp] * 12 + [_vp, _vp]),
     "dt_mpm_advance": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 8
-                       + [_c_int, _vp, _vp, _vp]),
+                       + [_c_int, _vp, _vp, _vp, _vp]),
+    "dt_mpm_brick_count": (ctypes.c_int64, [_P(DtMpmParams)]),
+    "dt_mpm_mark_bricks": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp]),
+    "dt_mpm_zero4_bricks": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp]),
+    "dt_mpm_grid_op4_bricks": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp, _vp]),
+    "dt_mpm_grid_op4_adj_bricks": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp, _vp, _vp]),
     "dt_mpm_zero4": (_c_int, [ctypes.c_int64, _vp, _vp]),
     "dt_mpm_p2g_fused": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 6 + [_vp]),
     "dt_mpm_grid_op4": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp, _vp]),
@@ -153,5 +158,27 @@ def stream():
     return torch.cuda.current_stream().cuda_stream
 
 
+_ARG_DTYPES = (torch.float32, torch.int32, torch.uint8)
+
+
+def _arg(a):
+    if isinstance(a, torch.Tensor):
+        if not a.is_cuda:
+            raise ValueError("tensor arguments must live on the GPU")
+        if not a.is_contiguous():
+            raise ValueError("tensor arguments must be contiguous")
+        if a.dtype not in _ARG_DTYPES:
+            raise TypeError(f"unsupported tensor dtype {a.dtype}")
+        return a.data_ptr()
+    return a
+
+
 def call(name, *args):
-    check(getattr(lib(), name)(*args))
+    """Call entry point `name`.  Tensor arguments are passed as device pointers;
+    the `args` tuple keeps every tensor (temporaries included) alive until the
+    launch has been enqueued, after which stream order protects the memory: a
+    block freed here is only reused by work enqueued later on the same stream.
+    Never pass `ptr()` of a temporary: the temporary is freed before the call
+    and its block can be reused by the next argument's copy."""
+    conv = [_arg(a) for a in args]
+    check(getattr(lib(), name)(*conv))

@@ -1,0 +1,255 @@
+"""Differentiable coupled rollouts: MPM object + FEM pads + grid penalty contact.
+
+Composes the fused object stages with the pad and contact kernels in the
+order of ``Contact.update`` (``difftactile/tasks/grasp_plastic.py:151-164``):
+
+  zero grid -> p2g (fused compute_new_F/svd/return map) -> pad update ->
+  closest triangle -> collision -> grid_op -> g2p -> pad update2
+
+and its adjoint in the order of ``Contact.update_grad`` (167-178), including
+``clamp_grid`` (191-202).  Pads use the gripper form of update2 (a control
+velocity per frame, ``gripper_fem.py:403-418``), so the gradient with respect
+to the pad velocity sequence is available.  Checkpointing as in ``sim.py``:
+one object + pad state every ``checkpoint_every`` substeps in HBM; the backward
+pass recomputes each segment keeping its frames, grids, contact impulses and
+closest-triangle indices.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .sim import MPMSim, _segments
+
+P = _lib.ptr
+I32 = torch.int32
+
+
+class Pad:
+    """Static data of one FEM pad on the GPU (mesh, rest shapes, material)."""
+
+    def __init__(self, nodes_rest, f2v, tris, layer, out_dir, mu, lam, mass, dt, device="cuda"):
+        self.device = torch.device(device)
+        self.n_verts = len(nodes_rest)
+        self.n_tets = len(f2v)
+        self.n_tris = len(tris)
+        self.f2v = torch.as_tensor(np.asarray(f2v), dtype=I32, device=self.device).contiguous()
+        self.tris = torch.as_tensor(np.asarray(tris), dtype=I32, device=self.device).contiguous()
+        self.layer = torch.as_tensor(np.asarray(layer), dtype=I32, device=self.device).contiguous()
+        self.out_dir = np.ascontiguousarray(np.asarray(out_dir, dtype=np.float32))
+        rest = torch.as_tensor(np.asarray(nodes_rest), dtype=torch.float32,
+                               device=self.device).contiguous()
+        self.B = torch.empty(self.n_tets, 3, 3, device=self.device)
+        _lib.call("dt_fem_init_B", self.n_tets, self.f2v, rest, self.B, _lib.stream())
+        self.fem = _lib.DtFemParams()
+        self.fem.mu, self.fem.lam, self.fem.dt, self.fem.mass = mu, lam, dt, mass
+        self.cen = torch.empty(self.n_tris, 4, device=self.device)
+
+    def od(self):
+        return self.out_dir.ctypes.data_as(_lib.ctypes.POINTER(_lib.ctypes.c_float))
+
+
+class CoupledSim:
+    def __init__(self, sim: MPMSim, pads, kn, kd, kt, mu_f, norm_eps=1e-11, clamp_grad=True):
+        if sim.sort:
+            raise ValueError("CoupledSim keeps the caller's particle order (create MPMSim with sort=False)")
+        self.sim = sim
+        self.pads = list(pads)
+        self.cp = _lib.DtContactParams()
+        self.cp.kn, self.cp.kd, self.cp.kt, self.cp.mu_f = kn, kd, kt, mu_f
+        self.cp.norm_eps, self.cp.sdf_offset = norm_eps, 0.0
+        self.clamp_grad = bool(clamp_grad)
+        self.c_prm_clamp = sim.prm.c_struct(soa=True)
+        self.c_prm_clamp.clamp_grad = int(clamp_grad)
+        self.c_prm_clamp.grad_clamp = 1000.0
+        nn = sim.prm.n_nodes
+        d = sim.device
+        self.npad = max(1, len(self.pads))
+        self.grid_f = torch.zeros(nn, 3, device=d)
+        self.idx = torch.full((nn, self.npad), -1, dtype=I32, device=d)
+
+    # -- one substep --------------------------------------------------------
+    def _fwd(self, st, nxt, pads_in, pads_out, cv, cv1, mat_id, store=None):
+        """st/nxt: [x, v, C, F] SoA; pads_in/out: per pad [pos, vel, vpos];
+        store: dict receiving what the adjoint needs (or None)."""
+        sim, s = self.sim, _lib.stream()
+        nn = sim.prm.n_nodes
+        prm = sim.c_prm
+        g4 = store["g4"] if store else sim._grid_scratch()[0]
+        gv4 = store["gv4"] if store else sim._grid_scratch()[1]
+        gf = store["gf"] if store else self.grid_f
+        idx = store["idx"] if store else self.idx
+        mats, nm = P(sim.mats), len(sim.materials)
+        mid = None if mat_id is None else P(mat_id, I32)
+        _lib.call("dt_mpm_zero4", nn, P(g4), s)
+        _lib.call("dt_mpm_p2g_fused", prm, mats, nm, sim.n, mid, *[P(t) for t in st], P(g4),
+                  P(nxt[3]), s)
+        vel_u = []
+        for k, pad in enumerate(self.pads):
+            pos, vel, _ = pads_in[k]
+            vu = store["vel_u"][k] if store else torch.empty_like(vel)
+            vu.copy_(vel)
+            _lib.call("dt_fem_update", pad.fem, pad.n_tets, P(pad.f2v, I32), P(pad.B), P(pos),
+                      P(vu), s)
+            vel_u.append(vu)
+        idx.fill_(-1)
+        for k, pad in enumerate(self.pads):
+            _lib.call("dt_contact_centroids", pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
+                      P(pad.cen), s)
+            # grid mass = 4th float of each packed node
+            _lib.call("dt_contact_closest", prm, g4.data_ptr() + 12, 4, pad.n_tris, P(pad.cen),
+                      P(idx, I32), self.npad, k, s)
+        gf.zero_()
+        exts = []
+        for k, pad in enumerate(self.pads):
+            ext = torch.zeros(pad.n_verts, 3, device=sim.device)
+            _lib.call("dt_contact_collision", prm, self.cp, P(g4), 4, g4.data_ptr() + 12, 4,
+                      P(idx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
+                      P(vel_u[k]), pad.od(), P(gf), P(ext), s)
+            exts.append(ext)
+        _lib.call("dt_mpm_grid_op4", prm, P(g4), P(gf), P(gv4), s)
+        _lib.call("dt_mpm_g2p4", prm, sim.n, P(st[0]), P(gv4), P(nxt[0]), P(nxt[1]), P(nxt[2]), s)
+        for k, pad in enumerate(self.pads):
+            pos, _, vpos = pads_in[k]
+            o = pads_out[k]
+            _lib.call("dt_fem_update2", pad.fem, pad.n_verts, P(pad.layer, I32), P(pos),
+                      P(vel_u[k]), P(vpos), P(exts[k]), P(cv[k]), P(cv1[k]), P(o[0]), P(o[1]),
+                      P(o[2]), s)
+
+    def _adj(self, st, pads_in, store, g_next, g_pads_next, g, g_pads, g_cv, g_cv1, mat_id):
+        """Adjoint of one substep.  g_next/g_pads_next: adjoints of the outputs;
+        writes g (object, overwritten), g_pads (overwritten), accumulates g_cv, g_cv1."""
+        sim, s = self.sim, _lib.stream()
+        nn = sim.prm.n_nodes
+        prm = sim.c_prm
+        g4, gv4, gf, idx = store["g4"], store["gv4"], store["gf"], store["idx"]
+        gg4 = sim._buf("cgg4", (nn, 4))
+        ggv4 = sim._buf("cggv4", (nn, 4))
+        ggf = sim._buf("cggf", (nn, 3))
+        mats, nm = P(sim.mats), len(sim.materials)
+        mid = None if mat_id is None else P(mat_id, I32)
+        g_ext = []
+        for k, pad in enumerate(self.pads):
+            gp, gvu, gvp = g_pads[k]
+            for t in (gp, gvu, gvp):
+                t.zero_()
+            ge = torch.zeros(pad.n_verts, 3, device=sim.device)
+            gpn, gvn, gvpn = g_pads_next[k]
+            _lib.call("dt_fem_update2_adj", pad.fem, pad.n_verts, P(pad.layer, I32), P(gpn),
+                      P(gvn), P(gvpn), P(gp), P(gvu), P(gvp), P(ge), P(g_cv[k]), P(g_cv1[k]), s)
+            g_ext.append(ge)
+        _lib.call("dt_mpm_zero4", nn, P(ggv4), s)
+        _lib.call("dt_mpm_g2p4_adj", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]), P(g_next[1]),
+                  P(g_next[2]), P(g[0]), P(ggv4), s)
+        _lib.call("dt_mpm_grid_op4_adj", self.c_prm_clamp, P(g4), P(gf), P(ggv4), P(gg4), P(ggf), s)
+        if self.clamp_grad:
+            for k in range(len(self.pads)):
+                for t in (g_pads[k][0], g_pads[k][1]):
+                    _lib.call("dt_clamp", t.numel(), P(t), 1000.0, s)
+        for k, pad in enumerate(self.pads):
+            _lib.call("dt_contact_collision_adj", prm, self.cp, P(g4), 4, g4.data_ptr() + 12, 4,
+                      P(idx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
+                      P(store["vel_u"][k]), pad.od(), P(ggf), P(g_ext[k]), P(gg4), 4,
+                      gg4.data_ptr() + 12, 4, P(g_pads[k][0]), P(g_pads[k][1]), s)
+        for k, pad in enumerate(self.pads):
+            _lib.call("dt_fem_update_adj", pad.fem, pad.n_tets, P(pad.f2v, I32), P(pad.B),
+                      P(pads_in[k][0]), P(g_pads[k][1]), P(g_pads[k][0]), s)
+        _lib.call("dt_mpm_p2g_fused_adj", prm, mats, nm, sim.n, mid, *[P(t) for t in st], P(gg4),
+                  P(g_next[3]), P(g[0]), P(g[1]), P(g[2]), P(g[3]), s)
+
+    # -- public --------------------------------------------------------------
+    def rollout(self, x, v, C, F, pad_states, control_vel, n_substeps):
+        """x, v, C, F: object state ([n][k] layout); pad_states: list of (pos, vel, vpos)
+        per pad; control_vel: list of per-pad (n_substeps+1, n_verts, 3) velocity
+        sequences.  Returns (x, v, C, F, [(pos, vel, vpos) per pad]) after n_substeps,
+        differentiable with respect to every tensor input."""
+        flat = [x, v, C, F]
+        for ps in pad_states:
+            flat += list(ps)
+        flat += list(control_vel)
+        out = _CoupledRollout.apply(self, int(n_substeps), *flat)
+        obj = out[:4]
+        pads = [tuple(out[4 + 3 * k: 7 + 3 * k]) for k in range(len(self.pads))]
+        return (*obj, pads)
+
+
+class _CoupledRollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, cs, n_substeps, *flat):
+        sim = cs.sim
+        npd = len(cs.pads)
+        x, v, C, F = (t.contiguous() for t in flat[:4])
+        pads0 = [[t.contiguous().clone() for t in flat[4 + 3 * k: 7 + 3 * k]] for k in range(npd)]
+        cvs = [t.contiguous() for t in flat[4 + 3 * npd:]]
+        shapes = [t.shape for t in (x, v, C, F)]
+        W = (3, 3, 9, 9)
+        st = [sim._gather(None, t, w) for t, w in zip((x, v, C, F), W)]
+        nxt = [torch.empty_like(t) for t in st]
+        pst = pads0
+        pnx = [[torch.empty_like(t) for t in p] for p in pst]
+        ckpts = []
+        step = 0
+        for (s0, k) in _segments(n_substeps, sim.checkpoint_every):
+            ckpts.append(([t.clone() for t in st], [[t.clone() for t in p] for p in pst]))
+            for _ in range(k):
+                cs._fwd(st, nxt, pst, pnx, [c[step] for c in cvs], [c[step + 1] for c in cvs],
+                        sim.mat_id)
+                st, nxt = nxt, st
+                pst, pnx = pnx, pst
+                step += 1
+        out = [sim._scatter(None, t, w, sh) for t, w, sh in zip(st, W, shapes)]
+        for p in pst:
+            out += [t.clone() for t in p]
+        ctx.cs, ctx.n_substeps, ctx.ckpts, ctx.shapes = cs, n_substeps, ckpts, shapes
+        ctx.cv_shapes = [c.shape for c in cvs]
+        ctx.save_for_backward(*cvs)
+        return tuple(out)
+
+    @staticmethod
+    def backward(ctx, *gout):
+        cs = ctx.cs
+        sim = cs.sim
+        npd = len(cs.pads)
+        cvs = ctx.saved_tensors
+        W = (3, 3, 9, 9)
+        dev = sim.device
+        g = [sim._gather(None, (t if t is not None else torch.zeros(sh, device=dev)).float(), w)
+             for t, w, sh in zip(gout[:4], W, ctx.shapes)]
+        gp = []
+        for k in range(npd):
+            trio = gout[4 + 3 * k: 7 + 3 * k]
+            gp.append([(t if t is not None else torch.zeros(cs.pads[k].n_verts, 3, device=dev)
+                        ).contiguous().clone() for t in trio])
+        g_cv = [torch.zeros(sh, device=dev) for sh in ctx.cv_shapes]
+        gbuf = [torch.empty_like(t) for t in g]
+        gpbuf = [[torch.empty_like(t) for t in p] for p in gp]
+        nn = sim.prm.n_nodes
+        segs = _segments(ctx.n_substeps, sim.checkpoint_every)
+        for (s0, k), (ck, ckp) in zip(reversed(segs), reversed(ctx.ckpts)):
+            # recompute the segment, keeping what the adjoint needs
+            frames = [[t.clone() for t in ck]]
+            pframes = [[[t.clone() for t in p] for p in ckp]]
+            stores = []
+            for j in range(k):
+                store = {"g4": torch.empty(nn, 4, device=dev), "gv4": torch.empty(nn, 4, device=dev),
+                         "gf": torch.empty(nn, 3, device=dev),
+                         "idx": torch.empty(nn, cs.npad, dtype=I32, device=dev),
+                         "vel_u": [torch.empty(pd.n_verts, 3, device=dev) for pd in cs.pads]}
+                nxt = [torch.empty_like(t) for t in frames[-1]]
+                pn = [[torch.empty_like(t) for t in p] for p in pframes[-1]]
+                f = s0 + j
+                cs._fwd(frames[-1], nxt, pframes[-1], pn, [c[f] for c in cvs],
+                        [c[f + 1] for c in cvs], sim.mat_id, store)
+                frames.append(nxt)
+                pframes.append(pn)
+                stores.append(store)
+            for j in reversed(range(k)):
+                f = s0 + j
+                cs._adj(frames[j], pframes[j], stores[j], g, gp, gbuf, gpbuf,
+                        [gc[f] for gc in g_cv], [gc[f + 1] for gc in g_cv], sim.mat_id)
+                g, gbuf = gbuf, g
+                gp, gpbuf = gpbuf, gp
+        res = [sim._scatter(None, t, w, sh) for t, w, sh in zip(g, W, ctx.shapes)]
+        for p in gp:
+            res += list(p)
+        res += g_cv
+        return (None, None, *res)

@@ -748,6 +748,96 @@ __global__ __launch_bounds__(kBlock) void k_zero4(long long nn, float4* __restri
 }
 
 // ----------------------------------------------------------------------------
+// active 4x4x4-node bricks: the grid passes of the fused path visit only the
+// bricks some particle stencil touches (flags[brick] != 0)
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_mark_bricks(MpmP P, int n, const float* __restrict__ x,
+                                                        int32_t* __restrict__ flags) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float xp[3];
+  ld_v3(x, p, pl_vec(P.soa, n), xp);
+  Stencil s;
+  make_stencil(xp, P.inv_dx, P.n_grid, s);
+  const int nb = (P.n_grid + 3) >> 2;
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = s.base[a] >> 2;
+    hi[a] = (s.base[a] + 2) >> 2;
+  }
+  for (int i = lo[0]; i <= hi[0]; ++i)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int k = lo[2]; k <= hi[2]; ++k) flags[(i * nb + j) * nb + k] = 1;
+}
+
+// node of thread t in the brick-pass launch (4 bricks per 256-thread block);
+// returns -1 when the brick is inactive or the node lies outside the grid
+__device__ __forceinline__ long long brick_node(const MpmP& P, const int32_t* __restrict__ flags) {
+  const int n = P.n_grid, nb = (n + 3) >> 2;
+  long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= (long long)nb * nb * nb || flags[b] == 0) return -1;
+  int l = threadIdx.x & 63;
+  int bz = (int)(b % nb), by = (int)((b / nb) % nb), bx = (int)(b / ((long long)nb * nb));
+  int i = bx * 4 + (l >> 4), j = by * 4 + ((l >> 2) & 3), k = bz * 4 + (l & 3);
+  if (i >= n || j >= n || k >= n) return -1;
+  return ((long long)i * n + j) * n + k;
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero4_bricks(MpmP P, const int32_t* __restrict__ flags,
+                                                         float4* __restrict__ a) {
+  long long nd = brick_node(P, flags);
+  if (nd >= 0) a[nd] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_op4_bricks(MpmP P,
+                                                            const int32_t* __restrict__ flags,
+                                                            const float4* __restrict__ g4,
+                                                            const float* __restrict__ gf,
+                                                            float4* __restrict__ v4) {
+  long long nd = brick_node(P, flags);
+  if (nd < 0) return;
+  float4 q = g4[nd];
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (q.w > P.eps) {
+    float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+    if (gf) { f0 = gf[3 * nd]; f1 = gf[3 * nd + 1]; f2 = gf[3 * nd + 2]; }
+    float vo[3];
+    bool mask[3];
+    grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, f0, f1, f2, vo, mask);
+    out = make_float4(vo[0], vo[1], vo[2], 0.f);
+  }
+  v4[nd] = out;
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_op4_adj_bricks(MpmP P,
+                                                                const int32_t* __restrict__ flags,
+                                                                const float4* __restrict__ g4,
+                                                                const float4* __restrict__ gv4,
+                                                                float4* __restrict__ gg4) {
+  long long nd = brick_node(P, flags);
+  if (nd < 0) return;
+  float4 q = g4[nd];
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (q.w > P.eps) {
+    float vo[3];
+    bool mask[3];
+    grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, vo, mask);
+    float4 gi = gv4[nd];
+    float g0 = mask[0] ? 0.f : gi.x, g1 = mask[1] ? 0.f : gi.y, g2 = mask[2] ? 0.f : gi.z;
+    float inv_m = 1.f / (q.w + P.eps);
+    float gmm = -inv_m * inv_m * (g0 * q.x + g1 * q.y + g2 * q.z);
+    if (P.clamp_grad) gmm = fminf(fmaxf(gmm, -P.grad_clamp), P.grad_clamp);
+    out = make_float4(inv_m * g0, inv_m * g1, inv_m * g2, gmm);
+  }
+  gg4[nd] = out;
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_i32(long long n, int32_t* __restrict__ a) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// ----------------------------------------------------------------------------
 // g2p and its adjoint
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ void g2p_core(const MpmP& P, const Stencil& s,
@@ -976,19 +1066,44 @@ static int check_mats(const dt_material* mats, int n_mats) {
 
 #define DT_STREAM(s) ((hipStream_t)(s))
 
+static long long n_bricks(const dt_mpm_params* prm) {
+  long long nb = (prm->n_grid + 3) >> 2;
+  return nb * nb * nb;
+}
+
+// Marks the bricks touched by the particles at x (flags cleared first).
+static void launch_mark(const MpmP& P, const dt_mpm_params* prm, int n, const float* x,
+                        int32_t* flags, hipStream_t st) {
+  long long nbk = n_bricks(prm);
+  hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(nbk)), dim3(kBlock), 0, st, nbk, flags);
+  if (n > 0)
+    hipLaunchKernelGGL(k_mark_bricks, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, flags);
+}
+
 int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
                        const int32_t* mat_id, const float* x, const float* v, const float* C,
                        const float* F, float* grid4, float* grid_v4, float* x1, float* v1,
-                       float* C1, float* F1, hipStream_t st) {
+                       float* C1, float* F1, hipStream_t st, int32_t* flags = nullptr) {
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
-  hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)grid4);
+  int bgrid = (int)((n_bricks(prm) + 3) / 4);
+  if (flags) {
+    launch_mark(P, prm, n, x, flags, st);
+    hipLaunchKernelGGL(k_zero4_bricks, dim3(bgrid), dim3(kBlock), 0, st, P, flags,
+                       (float4*)grid4);
+  } else {
+    hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)grid4);
+  }
   Grid g{grid4, grid4 + 3, 4, 4};
   if (n > 0)
     hipLaunchKernelGGL(k_p2g_fused, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n, mat_id, x,
                        v, C, F, g, F1);
-  hipLaunchKernelGGL(k_grid_op4, dim3(nblocks(nn)), dim3(kBlock), 0, st, P, nn,
-                     (const float4*)grid4, (const float*)nullptr, (float4*)grid_v4);
+  if (flags)
+    hipLaunchKernelGGL(k_grid_op4_bricks, dim3(bgrid), dim3(kBlock), 0, st, P, flags,
+                       (const float4*)grid4, (const float*)nullptr, (float4*)grid_v4);
+  else
+    hipLaunchKernelGGL(k_grid_op4, dim3(nblocks(nn)), dim3(kBlock), 0, st, P, nn,
+                       (const float4*)grid4, (const float*)nullptr, (float4*)grid_v4);
   if (n > 0)
     hipLaunchKernelGGL(k_g2p, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x,
                        (const float*)grid_v4, 4, x1, v1, C1);
@@ -1000,16 +1115,26 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
                        const float* F, const float* grid4, const float* grid_v4,
                        const float* gx1, const float* gv1, const float* gC1, const float* gF1,
                        float* ggrid4, float* ggv4, float* gx, float* gv, float* gC, float* gF,
-                       hipStream_t st) {
+                       hipStream_t st, int32_t* flags = nullptr) {
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
-  hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)ggv4);
+  int bgrid = (int)((n_bricks(prm) + 3) / 4);
+  if (flags) {
+    launch_mark(P, prm, n, x, flags, st);
+    hipLaunchKernelGGL(k_zero4_bricks, dim3(bgrid), dim3(kBlock), 0, st, P, flags, (float4*)ggv4);
+  } else {
+    hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)ggv4);
+  }
   if (n > 0)
     hipLaunchKernelGGL(k_g2p_adj<1>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, grid_v4, 4,
                        gx1, gv1, gC1, gx, ggv4, 4);
-  hipLaunchKernelGGL(k_grid_op4_adj, dim3(nblocks(nn)), dim3(kBlock), 0, st, P, nn,
-                     (const float4*)grid4, (const float*)nullptr, (const float4*)ggv4,
-                     (float4*)ggrid4, (float*)nullptr);
+  if (flags)
+    hipLaunchKernelGGL(k_grid_op4_adj_bricks, dim3(bgrid), dim3(kBlock), 0, st, P, flags,
+                       (const float4*)grid4, (const float4*)ggv4, (float4*)ggrid4);
+  else
+    hipLaunchKernelGGL(k_grid_op4_adj, dim3(nblocks(nn)), dim3(kBlock), 0, st, P, nn,
+                       (const float4*)grid4, (const float*)nullptr, (const float4*)ggv4,
+                       (float4*)ggrid4, (float*)nullptr);
   if (n > 0)
     hipLaunchKernelGGL(k_p2g_fused_adj, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n, mat_id,
                        x, v, C, F, (const float*)ggrid4, gF1, gx, gv, gC, gF);
@@ -1200,7 +1325,7 @@ int dt_mpm_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int32_
 int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, float* x, float* v, float* C, float* F,
                    int32_t n_substeps, float* grids4, float* grids_v4, float* scratch4,
-                   float* scratch_v4, void* stream) {
+                   float* scratch_v4, int32_t* brick_flags, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
@@ -1214,7 +1339,7 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     float* gv4 = keep ? grids_v4 + (long long)s * nn * 4 : scratch_v4;
     rc = launch_substep_fwd(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o, g4,
                             gv4, x + 3 * o1, v + 3 * o1, C + 9 * o1, F + 9 * o1,
-                            DT_STREAM(stream));
+                            DT_STREAM(stream), brick_flags);
     if (rc) return rc;
   }
   return DT_OK;
@@ -1224,7 +1349,8 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                    const int32_t* mat_id, const float* x, const float* v, const float* C,
                    const float* F, int32_t n_substeps, const float* grids4, const float* grids_v4,
                    float* g_grid4, float* g_grid_v4, float* gx, float* gv, float* gC, float* gF,
-                   float* gx2, float* gv2, float* gC2, float* gF2, void* stream) {
+                   float* gx2, float* gv2, float* gC2, float* gF2, int32_t* brick_flags,
+                   void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
@@ -1237,7 +1363,8 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     long long o = (long long)s * n;
     rc = launch_substep_adj(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
                             grids4 + (long long)s * nn * 4, grids_v4 + (long long)s * nn * 4, a[0],
-                            a[1], a[2], a[3], g_grid4, g_grid_v4, b[0], b[1], b[2], b[3], st);
+                            a[1], a[2], a[3], g_grid4, g_grid_v4, b[0], b[1], b[2], b[3], st,
+                            brick_flags);
     if (rc) return rc;
     for (int q = 0; q < 4; ++q) { float* t = a[q]; a[q] = b[q]; b[q] = t; }
   }
@@ -1254,7 +1381,7 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
 int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, float* x, float* v, float* C, float* F, float* x2,
                    float* v2, float* C2, float* F2, int32_t n_substeps, float* scratch4,
-                   float* scratch_v4, void* stream) {
+                   float* scratch_v4, int32_t* brick_flags, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
@@ -1265,7 +1392,7 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   float *a[4] = {x, v, C, F}, *b[4] = {x2, v2, C2, F2};
   for (int s = 0; s < n_substeps; ++s) {
     rc = launch_substep_fwd(prm, mats, n, mat_id, a[0], a[1], a[2], a[3], scratch4, scratch_v4,
-                            b[0], b[1], b[2], b[3], st);
+                            b[0], b[1], b[2], b[3], st, brick_flags);
     if (rc) return rc;
     for (int q = 0; q < 4; ++q) { float* t = a[q]; a[q] = b[q]; b[q] = t; }
   }
@@ -1366,6 +1493,56 @@ int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int3
   hipLaunchKernelGGL(k_p2g_fused_adj, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream),
                      to_dev(*prm), mats, n, mat_id, x, v, C, F, g_grid4, gF_next, gx, gv, gC, gF);
   return check_launch("dt_mpm_p2g_fused_adj");
+}
+
+int dt_mpm_zero4_bricks(const dt_mpm_params* prm, const int32_t* flags, float* buf4,
+                        void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(flags && buf4, "NULL pointer");
+  int bgrid = (int)((n_bricks(prm) + 3) / 4);
+  hipLaunchKernelGGL(k_zero4_bricks, dim3(bgrid), dim3(kBlock), 0, DT_STREAM(stream), to_dev(*prm),
+                     flags, (float4*)buf4);
+  return check_launch("dt_mpm_zero4_bricks");
+}
+
+int dt_mpm_grid_op4_bricks(const dt_mpm_params* prm, const int32_t* flags, const float* grid4,
+                           float* grid_v4, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(flags && grid4 && grid_v4, "NULL pointer");
+  int bgrid = (int)((n_bricks(prm) + 3) / 4);
+  hipLaunchKernelGGL(k_grid_op4_bricks, dim3(bgrid), dim3(kBlock), 0, DT_STREAM(stream),
+                     to_dev(*prm), flags, (const float4*)grid4, (const float*)nullptr,
+                     (float4*)grid_v4);
+  return check_launch("dt_mpm_grid_op4_bricks");
+}
+
+int dt_mpm_grid_op4_adj_bricks(const dt_mpm_params* prm, const int32_t* flags, const float* grid4,
+                               const float* g_grid_v4, float* g_grid4, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(flags && grid4 && g_grid_v4 && g_grid4, "NULL pointer");
+  int bgrid = (int)((n_bricks(prm) + 3) / 4);
+  hipLaunchKernelGGL(k_grid_op4_adj_bricks, dim3(bgrid), dim3(kBlock), 0, DT_STREAM(stream),
+                     to_dev(*prm), flags, (const float4*)grid4, (const float4*)g_grid_v4,
+                     (float4*)g_grid4);
+  return check_launch("dt_mpm_grid_op4_adj_bricks");
+}
+
+int64_t dt_mpm_brick_count(const dt_mpm_params* prm) {
+  if (!prm || prm->n_grid < 3) return 0;
+  return n_bricks(prm);
+}
+
+int dt_mpm_mark_bricks(const dt_mpm_params* prm, int32_t n, const float* x, int32_t* flags,
+                       void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n >= 0, "n must be >= 0");
+  DT_REQUIRE(flags && (n == 0 || x), "NULL pointer");
+  launch_mark(to_dev(*prm), prm, n, x, flags, DT_STREAM(stream));
+  return check_launch("dt_mpm_mark_bricks");
 }
 
 int dt_mpm_cell_keys(const dt_mpm_params* prm, int32_t n, const float* x, int32_t* keys,

@@ -22,8 +22,9 @@ class MPMSim:
     """One MPM object on the GPU: parameters, material table, per-particle ids."""
 
     def __init__(self, prm, materials, n_particles, mat_id=None, device="cuda",
-                 checkpoint_every=50, sort=True):
+                 checkpoint_every=50, sort=True, sparse_grid=True):
         self.prm = prm
+        self.sparse_grid = bool(sparse_grid)
         # fused kernels run on the [k][n] (SoA) particle layout; the caller's
         # [n][k] arrays are transposed (and cell-sorted) at the boundary
         self.c_prm = prm.c_struct(soa=True)
@@ -58,6 +59,13 @@ class MPMSim:
         nn = self.prm.n_nodes
         return self._buf("g4", (nn, 4)), self._buf("gv4", (nn, 4))
 
+    def _flags(self):
+        """Active-brick flags for the sparse grid passes (None = dense passes)."""
+        if not self.sparse_grid:
+            return None
+        nb = int(_lib.lib().dt_mpm_brick_count(self.c_prm))
+        return _lib.ptr(self._buf("bflags", (nb,), torch.int32), torch.int32)
+
     # -- ordering --------------------------------------------------------
     def sort_order(self, x):
         """(sorted keys, permutation) of the particles at positions x by cell key."""
@@ -77,9 +85,8 @@ class MPMSim:
     def _gather(self, perm, t, width):
         """[n][width] in caller order -> [width][n] in sorted order."""
         out = torch.empty((width, self.n), dtype=torch.float32, device=self.device)
-        _lib.call("dt_gather_rows_soa", self.n, width,
-                  None if perm is None else _lib.ptr(perm, torch.int32),
-                  _lib.ptr(t.contiguous()), _lib.ptr(out), _lib.stream())
+        src = t.contiguous()
+        _lib.call("dt_gather_rows_soa", self.n, width, perm, src, out, _lib.stream())
         return out
 
     def _scatter(self, perm, t, width, shape):
@@ -116,7 +123,7 @@ class MPMSim:
             g4p, gv4p = None, None
         _lib.call("dt_mpm_run_fwd", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
                   self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
-                  g4p, gv4p, _lib.ptr(s4), _lib.ptr(sv4), _lib.stream())
+                  g4p, gv4p, _lib.ptr(s4), _lib.ptr(sv4), self._flags(), _lib.stream())
         return G4, GV4
 
     def advance(self, state, k, mat_id):
@@ -125,7 +132,7 @@ class MPMSim:
         tmp = [self._buf("adv%d" % i, t.shape) for i, t in enumerate(state)]
         _lib.call("dt_mpm_advance", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
                   self._mid(mat_id), *[_lib.ptr(t) for t in state], *[_lib.ptr(t) for t in tmp],
-                  int(k), _lib.ptr(s4), _lib.ptr(sv4), _lib.stream())
+                  int(k), _lib.ptr(s4), _lib.ptr(sv4), self._flags(), _lib.stream())
 
     def adj_frames(self, X, V, C, F, k, mat_id, G4, GV4, g):
         """Reverse sweep over k substeps; g = [gx, gv, gC, gF] seeded at frame k, in place."""
@@ -136,7 +143,8 @@ class MPMSim:
         _lib.call("dt_mpm_run_adj", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
                   self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
                   _lib.ptr(G4), _lib.ptr(GV4), _lib.ptr(gg4), _lib.ptr(ggv4),
-                  *[_lib.ptr(t) for t in g], *[_lib.ptr(t) for t in scratch], _lib.stream())
+                  *[_lib.ptr(t) for t in g], *[_lib.ptr(t) for t in scratch], self._flags(),
+                  _lib.stream())
 
     def _frames(self, k, prefix="F"):
         n = self.n

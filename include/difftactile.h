@@ -197,7 +197,7 @@ int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int3
 int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, float* x, float* v, float* C, float* F,
                    int32_t n_substeps, float* grids4, float* grids_v4, float* scratch4,
-                   float* scratch_v4, void* stream);
+                   float* scratch_v4, int32_t* brick_flags, void* stream);
 /* Reverse sweep over frames [0, n_substeps] stored by dt_mpm_run_fwd with
    grids kept.  Seeds: gx..gF hold the adjoint of frame n_substeps on entry and
    the adjoint of frame 0 on return; gx2..gF2 are same-size scratch. */
@@ -205,7 +205,8 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                    const int32_t* mat_id, const float* x, const float* v, const float* C,
                    const float* F, int32_t n_substeps, const float* grids4, const float* grids_v4,
                    float* g_grid4, float* g_grid_v4, float* gx, float* gv, float* gC, float* gF,
-                   float* gx2, float* gv2, float* gC2, float* gF2, void* stream);
+                   float* gx2, float* gv2, float* gC2, float* gF2, int32_t* brick_flags,
+                   void* stream);
 
 /* Advances the state (x, v, C, F) by n_substeps in place, ping-ponging with
    the same-size scratch state (x2, v2, C2, F2); grids are single-substep
@@ -214,7 +215,26 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
 int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, float* x, float* v, float* C, float* F, float* x2,
                    float* v2, float* C2, float* F2, int32_t n_substeps, float* scratch4,
-                   float* scratch_v4, void* stream);
+                   float* scratch_v4, int32_t* brick_flags, void* stream);
+
+/* Sparse grid passes.  brick_flags (int32[dt_mpm_brick_count]) non-NULL makes
+   dt_mpm_run_fwd / dt_mpm_run_adj / dt_mpm_advance zero and update only the
+   4x4x4-node bricks some particle stencil touches in that substep (marked by
+   dt_mpm_mark_bricks); grid nodes outside them are neither written nor read.
+   NULL keeps the dense passes over all n^3 nodes. */
+int64_t dt_mpm_brick_count(const dt_mpm_params* prm);
+/* flags[b] = 1 for every brick touched by a stencil of the particles at x
+   (flags cleared first). */
+int dt_mpm_mark_bricks(const dt_mpm_params* prm, int32_t n, const float* x, int32_t* flags,
+                       void* stream);
+/* brick-restricted forms of dt_mpm_zero4 / dt_mpm_grid_op4 / dt_mpm_grid_op4_adj
+   (no contact impulse) */
+int dt_mpm_zero4_bricks(const dt_mpm_params* prm, const int32_t* flags, float* buf4,
+                        void* stream);
+int dt_mpm_grid_op4_bricks(const dt_mpm_params* prm, const int32_t* flags, const float* grid4,
+                           float* grid_v4, void* stream);
+int dt_mpm_grid_op4_adj_bricks(const dt_mpm_params* prm, const int32_t* flags, const float* grid4,
+                               const float* g_grid_v4, float* g_grid4, void* stream);
 
 /* ---- particle ordering ---- */
 

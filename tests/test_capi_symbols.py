@@ -48,3 +48,10 @@ def test_error_path_without_gpu_work():
     rc = L.dt_mpm_grid_occupy(None, None, None, None)
     assert rc == -1
     assert b"NULL" in L.dt_last_error()
+
+
+def test_call_rejects_host_tensors_before_any_launch():
+    import torch
+    with pytest.raises(ValueError):
+        _lib._arg(torch.zeros(3))
+    assert _lib._arg(None) is None and _lib._arg(7) == 7

@@ -52,7 +52,7 @@ def test_fem_init_B_update_and_adjoint(gpu):
     nodes, pos, f2v, surf, layer = _dome()
     nt, nv = len(f2v), len(nodes)
     B = torch.empty(nt, 3, 3, device="cuda")
-    _lib.call("dt_fem_init_B", nt, P(dev(f2v, torch.int32), torch.int32), P(dev(nodes)), P(B),
+    _lib.call("dt_fem_init_B", nt, dev(f2v, torch.int32), dev(nodes), P(B),
               _lib.stream())
     B_ref = of.init_B(torch.tensor(nodes, dtype=F64), torch.tensor(f2v).long())
     assert rel_l2(h64(B), B_ref) < 1e-5
@@ -62,7 +62,7 @@ def test_fem_init_B_update_and_adjoint(gpu):
     vel0 = (0.01 * rng.normal(size=(nv, 3))).astype(np.float32)
     vel = dev(vel0)
     Bf = dev(h64(B).numpy())
-    _lib.call("dt_fem_update", prm, nt, P(dev(f2v, torch.int32), torch.int32), P(Bf), P(dev(pos)),
+    _lib.call("dt_fem_update", prm, nt, dev(f2v, torch.int32), P(Bf), dev(pos),
               P(vel), _lib.stream())
     posd = torch.tensor(pos, dtype=F64, requires_grad=True)
     vref = of.update(posd, torch.tensor(vel0, dtype=F64), torch.tensor(f2v).long(), h64(Bf), mu,
@@ -72,8 +72,8 @@ def test_fem_init_B_update_and_adjoint(gpu):
     # adjoint
     gvel = rng.normal(size=(nv, 3)).astype(np.float32)
     gpos = torch.zeros(nv, 3, device="cuda")
-    _lib.call("dt_fem_update_adj", prm, nt, P(dev(f2v, torch.int32), torch.int32), P(Bf),
-              P(dev(pos)), P(dev(gvel)), P(gpos), _lib.stream())
+    _lib.call("dt_fem_update_adj", prm, nt, dev(f2v, torch.int32), P(Bf),
+              dev(pos), dev(gvel), P(gpos), _lib.stream())
     g_ref = torch.autograd.grad((vref * torch.tensor(gvel, dtype=F64)).sum(), posd)[0]
     assert rel_l2(h64(gpos), g_ref) < 1e-4
 
@@ -87,9 +87,9 @@ def test_fem_update2_and_adjoint(gpu, gripper):
     layer = rng.integers(0, 4, nv).astype(np.int32)
     prm = fem_params(1.0, 2.0, 5e-5, 0.0147)
     outs = [torch.empty(nv, 3, device="cuda") for _ in range(3)]
-    _lib.call("dt_fem_update2", prm, nv, P(dev(layer, torch.int32), torch.int32), P(dev(pos)),
-              P(dev(vel)), P(dev(vpos)), P(dev(ext)), P(dev(cv)),
-              P(dev(cv1)) if gripper else None, *[P(o) for o in outs], _lib.stream())
+    _lib.call("dt_fem_update2", prm, nv, dev(layer, torch.int32), dev(pos),
+              dev(vel), dev(vpos), dev(ext), dev(cv),
+              dev(cv1) if gripper else None, *[P(o) for o in outs], _lib.stream())
     t = [torch.tensor(a, dtype=F64, requires_grad=True) for a in arrs]
     L = torch.tensor(layer)
     if gripper:
@@ -100,8 +100,8 @@ def test_fem_update2_and_adjoint(gpu, gripper):
         assert rel_l2(h64(o), r.detach()) < 1e-6
     seeds = [rng.normal(size=(nv, 3)).astype(np.float32) for _ in range(3)]
     g = [torch.zeros(nv, 3, device="cuda") for _ in range(6)]
-    _lib.call("dt_fem_update2_adj", prm, nv, P(dev(layer, torch.int32), torch.int32),
-              *[P(dev(s)) for s in seeds], P(g[0]), P(g[1]), P(g[2]), P(g[3]), P(g[4]),
+    _lib.call("dt_fem_update2_adj", prm, nv, dev(layer, torch.int32),
+              *[dev(s) for s in seeds], P(g[0]), P(g[1]), P(g[2]), P(g[3]), P(g[4]),
               P(g[5]) if gripper else None, _lib.stream())
     loss = sum((r * torch.tensor(s, dtype=F64)).sum() for r, s in zip(ref, seeds))
     gr = torch.autograd.grad(loss, t, allow_unused=True)
@@ -133,10 +133,10 @@ def test_closest_triangle_bit_exact(gpu):
     m32 = m.float()
     nn = prm.n_nodes
     cen = torch.empty(len(surf), 4, device="cuda")
-    _lib.call("dt_contact_centroids", len(surf), P(dev(surf, torch.int32), torch.int32),
-              P(dev(pos)), P(cen), _lib.stream())
+    _lib.call("dt_contact_centroids", len(surf), dev(surf, torch.int32),
+              dev(pos), P(cen), _lib.stream())
     idx = torch.full((nn, 2), -7, dtype=torch.int32, device="cuda")
-    _lib.call("dt_contact_closest", prm.c_struct(), P(dev(m32)), 1, len(surf), P(cen),
+    _lib.call("dt_contact_closest", prm.c_struct(), dev(m32), 1, len(surf), P(cen),
               P(idx, torch.int32), 2, 1, _lib.stream())
     nodes_o, ref = oc.check_collision(op, m32, [torch.tensor(pos)], [surf])
     got = idx.cpu().numpy()
@@ -214,3 +214,63 @@ def test_coupled_substep_forward_and_adjoint_match_oracle(gpu):
     for k, (got, want) in enumerate(zip(gots, gr)):
         err = rel_l2(h64(got), want)
         assert err < 1e-4, (k, err)
+
+
+def test_coupled_rollout_gradient_matches_oracle(gpu):
+    """Fused coupled rollout (object + pad + contact, checkpointed) vs the oracle's
+    reverse mode over the same composition, including d/d(pad velocity sequence)."""
+    from difftactile_amd.coupled import CoupledSim, Pad
+    from difftactile_amd.params import Material
+    from difftactile_amd.sim import MPMSim
+    prm, mesh, pos, x, v = _contact_scene(n_particles=6000, seed=5)
+    nodes, f2v, surf, layer, od = mesh
+    n = len(x)
+    steps = 6
+    mu, lam = 1e5 / 2 / 1.45, 1e5 * 0.45 / 1.45 / 0.1
+    mass = 2e-4
+    sim = MPMSim(prm, [Material(5e4, 0.3)], n, checkpoint_every=4, sort=False)
+    pad = Pad(nodes.astype(np.float32), f2v, surf, layer, od, mu, lam, mass, prm.dt)
+    cs = CoupledSim(sim, [pad], kn=1e3, kd=10.0, kt=50.0, mu_f=0.5, clamp_grad=False)
+    rng = np.random.default_rng(11)
+    cv = np.zeros((steps + 1, len(nodes), 3), np.float32)
+    cv[:, :, 1] = -0.5 + 0.05 * rng.normal(size=(steps + 1, 1))
+    st = [dev(x).requires_grad_(True), dev(v).requires_grad_(True),
+          torch.zeros(n, 3, 3, device="cuda", requires_grad=True),
+          torch.eye(3, device="cuda").repeat(n, 1, 1).requires_grad_(True)]
+    pvel0 = cv[0].copy()
+    pst = (dev(pos).requires_grad_(True), dev(pvel0).requires_grad_(True), dev(pos))
+    cvt = dev(cv).requires_grad_(True)
+    out = cs.rollout(*st, [pst], [cvt], steps)
+    w = [rng.normal(size=t.shape).astype(np.float32) for t in out[:4]]
+    wp = [rng.normal(size=(len(nodes), 3)).astype(np.float32) for _ in range(3)]
+    loss = sum((o * dev(wi)).sum() for o, wi in zip(out[:4], w))
+    loss = loss + sum((o * dev(wi)).sum() for o, wi in zip(out[4][0], wp))
+    leaves = st + [pst[0], pst[1], cvt]
+    grads = torch.autograd.grad(loss, leaves)
+    # oracle
+    op = oracle_params(prm)
+    mats = om.material_arrays([om.Material(5e4, 0.3)], np.zeros(n, int), F64)
+    so = [torch.tensor(x, dtype=F64, requires_grad=True), torch.tensor(v, dtype=F64, requires_grad=True),
+          torch.zeros(n, 3, 3, dtype=F64, requires_grad=True),
+          torch.eye(3, dtype=F64).repeat(n, 1, 1).requires_grad_(True)]
+    ppos = torch.tensor(pos, dtype=F64, requires_grad=True)
+    pvel = torch.tensor(pvel0, dtype=F64, requires_grad=True)
+    cvo = torch.tensor(cv, dtype=F64, requires_grad=True)
+    Bo = h64(pad.B)
+    state, padstate = tuple(so), (ppos, pvel, torch.tensor(pos, dtype=F64))
+    for f in range(steps):
+        pd = dict(pos=padstate[0], vel=padstate[1], vpos=padstate[2], cv=cvo[f], cv_next=cvo[f + 1],
+                  f2v=torch.tensor(f2v).long(), B=Bo, layer=torch.tensor(layer), tris=surf,
+                  out_dir=torch.tensor(od, dtype=F64), mu=mu, lam=lam, mass=mass, dt=prm.dt)
+        state, pads_o, _, _ = ocp.coupled_substep(op, state, mats, [pd], (1e3, 10.0, 50.0, 0.5))
+        padstate = pads_o[0]
+    for got, want in zip(out[:4], state):
+        assert rel_l2(h64(got), want.detach()) < 1e-5
+    for got, want in zip(out[4][0], padstate):
+        assert rel_l2(h64(got), want.detach()) < 1e-5
+    loss_o = sum((o * torch.tensor(wi, dtype=F64)).sum() for o, wi in zip(state, w))
+    loss_o = loss_o + sum((o * torch.tensor(wi, dtype=F64)).sum() for o, wi in zip(padstate, wp))
+    go = torch.autograd.grad(loss_o, so + [ppos, pvel, cvo])
+    for k, (a, b) in enumerate(zip(grads, go)):
+        err = rel_l2(h64(a), b)
+        assert err < 1e-3, (k, err)

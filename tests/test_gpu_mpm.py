@@ -57,7 +57,7 @@ def test_cell_keys_and_sort_order_bit_exact(gpu, n_grid, n):
     np.testing.assert_array_equal(keys_out.cpu().numpy(), keys_ref[perm_ref])
     np.testing.assert_array_equal(perm.cpu().numpy(), perm_ref)
     k = torch.empty(n, dtype=torch.int32, device="cuda")
-    call("dt_mpm_cell_keys", prm.c_struct(), n, _lib.ptr(dev(x)), _lib.ptr(k, torch.int32),
+    call("dt_mpm_cell_keys", prm.c_struct(), n, dev(x), _lib.ptr(k, torch.int32),
          _lib.stream())
     np.testing.assert_array_equal(k.cpu().numpy(), keys_ref)
 
@@ -155,13 +155,13 @@ def test_grid_op_and_g2p_forward_match_oracle(gpu):
     mv = (rng.normal(size=(nn, 3)) * m[:, None] * 3).astype(np.float32)
     gf = (rng.normal(size=(nn, 3)) * 1e-6).astype(np.float32)
     occ = torch.zeros(nn, dtype=torch.int32, device="cuda")
-    call("dt_mpm_grid_occupy", prm.c_struct(), _lib.ptr(dev(m)), _lib.ptr(occ, torch.int32),
+    call("dt_mpm_grid_occupy", prm.c_struct(), dev(m), _lib.ptr(occ, torch.int32),
          _lib.stream())
     op = oracle_params(prm)
     np.testing.assert_array_equal(occ.cpu().numpy() == 1,
                                   om.check_grid_occupy(op, torch.tensor(m)).numpy())
     vout = torch.zeros(nn, 3, device="cuda")
-    call("dt_mpm_grid_op", prm.c_struct(), _lib.ptr(dev(mv)), _lib.ptr(dev(m)), _lib.ptr(dev(gf)),
+    call("dt_mpm_grid_op", prm.c_struct(), dev(mv), dev(m), dev(gf),
          _lib.ptr(occ, torch.int32), _lib.ptr(vout), _lib.stream())
     vref = om.grid_op(op, torch.tensor(mv, dtype=F64), torch.tensor(m, dtype=F64),
                       torch.tensor(gf, dtype=F64))
@@ -171,7 +171,7 @@ def test_grid_op_and_g2p_forward_match_oracle(gpu):
     x1 = torch.empty(len(x), 3, device="cuda")
     v1 = torch.empty_like(x1)
     C1 = torch.empty(len(x), 3, 3, device="cuda")
-    call("dt_mpm_g2p", prm.c_struct(), len(x), _lib.ptr(dev(x)), _lib.ptr(vout), _lib.ptr(x1),
+    call("dt_mpm_g2p", prm.c_struct(), len(x), dev(x), _lib.ptr(vout), _lib.ptr(x1),
          _lib.ptr(v1), _lib.ptr(C1), _lib.stream())
     xr, vr, Cr = om.g2p(op, torch.tensor(x, dtype=F64), host64(vout))
     assert rel_l2(host64(v1), vr) < 1e-5
@@ -188,7 +188,7 @@ def test_grid_op_wall_truth_table(gpu, sticky):
     sign = np.where(np.random.default_rng(5).random((nn, 3)) < 0.5, -1.0, 1.0)
     mv = (sign * 1e-3).astype(np.float32)
     vout = torch.zeros(nn, 3, device="cuda")
-    call("dt_mpm_grid_op", prm.c_struct(), _lib.ptr(dev(mv)), _lib.ptr(dev(m)), None, None,
+    call("dt_mpm_grid_op", prm.c_struct(), dev(mv), dev(m), None, None,
          _lib.ptr(vout), _lib.stream())
     vref = om.grid_op(oracle_params(prm), torch.tensor(mv, dtype=F64), torch.tensor(m, dtype=F64))
     assert torch.equal(host64(vout) == 0, vref == 0)
@@ -208,8 +208,8 @@ def test_fused_substep_matches_oracle(gpu, kind):
     outs = [torch.empty(n, 3, device="cuda"), torch.empty(n, 3, device="cuda"),
             torch.empty(n, 3, 3, device="cuda"), torch.empty(n, 3, 3, device="cuda")]
     call("dt_mpm_substep_fwd", prm.c_struct(), _lib.ptr(mt), len(mats), n,
-         _lib.ptr(mid, torch.int32), _lib.ptr(dev(x)), _lib.ptr(dev(v)), _lib.ptr(dev(C)),
-         _lib.ptr(dev(F)), _lib.ptr(g4), _lib.ptr(gv4), *[_lib.ptr(t) for t in outs], _lib.stream())
+         _lib.ptr(mid, torch.int32), dev(x), dev(v), dev(C),
+         dev(F), _lib.ptr(g4), _lib.ptr(gv4), *[_lib.ptr(t) for t in outs], _lib.stream())
     op = oracle_params(prm)
     st = tuple(torch.tensor(a, dtype=F64) for a in (x, v, C, F))
     ref, (mv_r, m_r, gv_r) = om.substep(op, st, oracle_mats(mats, mat_id, F64), return_grid=True)
@@ -312,8 +312,8 @@ def test_grid_op_and_g2p_adjoints_match_oracle(gpu):
     ggmv = torch.zeros(nn, 3, device="cuda")
     ggm = torch.zeros(nn, device="cuda")
     ggf = torch.zeros(nn, 3, device="cuda")
-    call("dt_mpm_grid_op_adj", prm.c_struct(), _lib.ptr(dev(mv)), _lib.ptr(dev(m)),
-         _lib.ptr(dev(gf)), None, _lib.ptr(dev(gvo)), _lib.ptr(ggmv), _lib.ptr(ggm), _lib.ptr(ggf),
+    call("dt_mpm_grid_op_adj", prm.c_struct(), dev(mv), dev(m),
+         dev(gf), None, dev(gvo), _lib.ptr(ggmv), _lib.ptr(ggm), _lib.ptr(ggf),
          _lib.stream())
     t = [torch.tensor(a, dtype=F64, requires_grad=True) for a in (mv, m, gf)]
     vref = om.grid_op(oracle_params(prm), *t)
@@ -327,8 +327,8 @@ def test_grid_op_and_g2p_adjoints_match_oracle(gpu):
     s = [_seed((3000, 3), 30), _seed((3000, 3), 31), _seed((3000, 3, 3), 32)]
     gx = torch.zeros(3000, 3, device="cuda")
     ggv = torch.zeros(nn, 3, device="cuda")
-    call("dt_mpm_g2p_adj", prm.c_struct(), 3000, _lib.ptr(dev(x)), _lib.ptr(dev(gv_field)),
-         *[_lib.ptr(a.cuda()) for a in s], _lib.ptr(gx), _lib.ptr(ggv), _lib.stream())
+    call("dt_mpm_g2p_adj", prm.c_struct(), 3000, dev(x), dev(gv_field),
+         *[a.cuda() for a in s], _lib.ptr(gx), _lib.ptr(ggv), _lib.stream())
     xt = torch.tensor(x, dtype=F64, requires_grad=True)
     gvt = torch.tensor(gv_field, dtype=F64, requires_grad=True)
     out = om.g2p(oracle_params(prm), xt, gvt)
@@ -378,7 +378,7 @@ def test_full_size_mass_and_momentum_conservation(gpu):
     gv4 = torch.empty(nn, 4, device="cuda")
     outs = [torch.empty_like(t) for t in state]
     call("dt_mpm_substep_fwd", prm.c_struct(), _lib.ptr(mt), len(mats), n,
-         _lib.ptr(dev(mid, torch.int32), torch.int32), *[_lib.ptr(t) for t in state],
+         dev(mid, torch.int32), *[_lib.ptr(t) for t in state],
          _lib.ptr(g4), _lib.ptr(gv4), *[_lib.ptr(t) for t in outs], _lib.stream())
     gsum = g4.double().sum(0).cpu()
     assert abs(gsum[3].item() - n * prm.p_mass) / (n * prm.p_mass) < 1e-5
