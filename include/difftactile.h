@@ -327,6 +327,32 @@ int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* 
                              int32_t gmv_stride, float* g_grid_m, int32_t gm_stride, float* g_pos,
                              float* g_vel, void* stream);
 
+
+/* ---- losses on the measured path ---- */
+
+/* compute_obj_pos, tasks/grasp_elastic.py:240-243: out3[c] += sum_p x[p][c]
+   (float64, device; caller zeroes it and divides by n).  soa: x layout. */
+int dt_loss_sum_positions(int32_t n, int32_t soa, const float* x, double* out3, void* stream);
+/* adjoint of the mean position: gx[p] += g_com3 / n  (g_com3: HOST float[3]) */
+int dt_loss_com_adj(int32_t n, int32_t soa, const float* g_com3, float* gx, void* stream);
+/* extract_markers, sensor_model/fem_sensor.py:161-187 with project_3d_2d,
+   fisheye_model.py:13-30: surface vertices (surface_id) of pos / vpos mapped
+   by inv_trans16 (HOST row-major 4x4), reordered (x, z, y), projected with
+   fisheye5 = HOST {f, m, cx, cy, z_offset} into surf_loc / surf_vloc
+   [n_surface][2]; markers interpolated with k_closest inverse-distance weights
+   into pred / virt [n_markers][2]. */
+int dt_markers_fwd(int32_t n_surface, const int32_t* surface_id, const float* pos,
+                   const float* vpos, const float* inv_trans16, const float* fisheye5,
+                   int32_t n_markers, int32_t k_closest, const int32_t* interp_idx,
+                   const float* interp_w, float* surf_loc, float* surf_vloc, float* pred,
+                   float* virt, void* stream);
+/* adjoint of pred w.r.t. pos: g_surf_loc (scratch [n_surface][2], zeroed by the
+   caller) and g_pos accumulate. */
+int dt_markers_adj(int32_t n_surface, const int32_t* surface_id, const float* pos,
+                   const float* inv_trans16, const float* fisheye5, int32_t n_markers,
+                   int32_t k_closest, const int32_t* interp_idx, const float* interp_w,
+                   const float* g_pred, float* g_surf_loc, float* g_pos, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

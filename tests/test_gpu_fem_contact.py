@@ -55,7 +55,11 @@ def test_fem_init_B_update_and_adjoint(gpu):
     _lib.call("dt_fem_init_B", nt, dev(f2v, torch.int32), dev(nodes), P(B),
               _lib.stream())
     B_ref = of.init_B(torch.tensor(nodes, dtype=F64), torch.tensor(f2v).long())
-    assert rel_l2(h64(B), B_ref) < 1e-5
+    # float32 inverse: per-tet error bounded by cond(Dm) * eps32
+    cond = torch.linalg.cond(torch.linalg.inv(B_ref))
+    err = (h64(B) - B_ref).flatten(1).norm(dim=1) / B_ref.flatten(1).norm(dim=1)
+    assert (err <= 20 * cond * 1.2e-7).all()
+    assert rel_l2(h64(B), B_ref) < 1e-4
     mu, lam, dt, mass = 1294.01, 9201.11, 5e-5, 0.0147
     prm = fem_params(mu, lam, dt, mass)
     rng = np.random.default_rng(1)
