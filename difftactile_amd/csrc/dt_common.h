@@ -34,7 +34,7 @@ __device__ __forceinline__ int xcd_block() {
 }
 
 struct MpmP {
-  int n_grid, bound, sticky_floor, clamp_grad, soa;
+  int n_grid, bound, sticky_floor, clamp_grad, soa, svd_exact;
   float dx, inv_dx, dt, p_vol, p_mass, eps;
   float g[3];
   float grad_clamp;
@@ -47,6 +47,7 @@ inline MpmP to_dev(const dt_mpm_params& p) {
   d.sticky_floor = p.sticky_floor;
   d.clamp_grad = p.clamp_grad;
   d.soa = p.soa;
+  d.svd_exact = p.svd_exact;
   d.dx = p.dx;
   d.inv_dx = p.inv_dx;
   d.dt = p.dt;

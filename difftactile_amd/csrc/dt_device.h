@@ -392,7 +392,8 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
 DT_HD double clampd(double a) { return a >= 0.0 ? fmax(a, 1e-8) : fmin(a, -1e-8); }
 
 DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3& V, const Mat& m,
-                      const StressFwd& st, const M3& gtau, const M3& gFp_in, M3& gFn) {
+                      const StressFwd& st, const M3& gtau, const M3& gFp_in, M3& gFn,
+                      bool exact) {
   // tau adjoint
   M3 D;
   for (int i = 0; i < 9; ++i) D.a[i] = st.Fp.a[i] - st.R.a[i];
@@ -406,10 +407,11 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
 
   double gsig[3] = {0.0, 0.0, 0.0};
   M3d Ud = m3d_from(U), Vd = m3d_from(V);
-  // gU = gR V, gV = gR^T U (float64)
   M3d gRd = m3d_from(gR);
-  M3d gU = m3d_mul(gRd, Vd);
-  M3d gV = m3d_mul_at(gRd, Ud);
+  // U/V adjoints of the return-map path (F_p = U diag(d) V^T); the rotation
+  // path R = U V^T is kept apart so the exact mode can use its closed form
+  M3d gUp, gVp;
+  for (int i = 0; i < 9; ++i) gUp.a[i] = gVp.a[i] = 0.0;
   gFn = m3_zero();
 
   if (m.kind == KIND_ELASTIC) {
@@ -423,7 +425,6 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
     if (st.mode == 0) {
       gFn = gFp;
     } else {
-      // Fp = U diag(d) V^T
       double d[3] = {exp((double)st.epsn[0]), exp((double)st.epsn[1]), exp((double)st.epsn[2])};
       M3d gFpd = m3d_from(gFp);
       M3d UtgV = m3d_mul(m3d_mul_at(Ud, gFpd), Vd);   // U^T gFp V
@@ -431,8 +432,8 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
       M3d gFptU = m3d_mul_at(gFpd, Ud);               // gFp^T U
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
-          gU.a[i * 3 + j] += gFpV.a[i * 3 + j] * d[j];
-          gV.a[i * 3 + j] += gFptU.a[i * 3 + j] * d[j];
+          gUp.a[i * 3 + j] = gFpV.a[i * 3 + j] * d[j];
+          gVp.a[i * 3 + j] = gFptU.a[i * 3 + j] * d[j];
         }
       if (st.mode != 3) {
         double ge[3];   // adjoint of post-map eps
@@ -466,19 +467,43 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
       }
     }
   }
-  // gU, gV from R (already) -> svd_grad in float64
+  // SVD adjoint in float64:
+  //   reference form: svd_grad of gU = gR V + gUp, gV = gR^T U + gVp with the
+  //     1e-8 clamp on s_j^2 - s_i^2 (mpm_elastic.py:157-173);
+  //   exact form: rotation path as (M_ij - M_ji) / (s_i + s_j), M = U^T gR V,
+  //     and the return-map terms without the clamp (their numerators vanish
+  //     with s_i - s_j).
   double s2[3] = {(double)sig[0] * sig[0], (double)sig[1] * sig[1], (double)sig[2] * sig[2]};
+  M3d gU = gUp, gV = gVp;
+  M3d Mr;
+  if (exact) {
+    Mr = m3d_mul(m3d_mul_at(Ud, gRd), Vd);
+  } else {
+    M3d a = m3d_mul(gRd, Vd), b = m3d_mul_at(gRd, Ud);
+    for (int i = 0; i < 9; ++i) {
+      gU.a[i] += a.a[i];
+      gV.a[i] += b.a[i];
+    }
+  }
   M3d Xu = m3d_mul_at(Ud, gU);   // U^T gU
   M3d Xv = m3d_mul_at(Vd, gV);   // V^T gV
+  const double lim = exact ? 1e-30 : 1e-8;
   M3d Mi;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       double v = 0.0;
       if (i != j) {
-        double ff = 1.0 / clampd(s2[j] - s2[i]);
+        double dd = s2[j] - s2[i];
+        dd = dd >= 0.0 ? fmax(dd, lim) : fmin(dd, -lim);
+        double ff = 1.0 / dd;
         double xu = Xu.a[i * 3 + j] - Xu.a[j * 3 + i];
         double xv = Xv.a[i * 3 + j] - Xv.a[j * 3 + i];
         v = ff * xu * (double)sig[j] + (double)sig[i] * ff * xv;
+        if (exact) {
+          double den = (double)sig[i] + (double)sig[j];
+          if (fabs(den) < 1e-12) den = den < 0.0 ? -1e-12 : 1e-12;
+          v += (Mr.a[i * 3 + j] - Mr.a[j * 3 + i]) / den;
+        }
       } else {
         v = gsig[i];
       }

@@ -584,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_fused_adj(
   M3 gtau, lgC;
   affine_adj(P, gA, gtau, lgC);
   M3 gFn;
-  stress_adj(fn, u, sig, vv, m, st, gtau, ld_m3L(gFnext, p, Lm), gFn);
+  stress_adj(fn, u, sig, vv, m, st, gtau, ld_m3L(gFnext, p, Lm), gFn, P.svd_exact != 0);
   // F_new = (I + dt C) F
   M3 gc2 = m3_mul_bt(gFn, f);
   for (int i = 0; i < 9; ++i) lgC.a[i] += P.dt * gc2.a[i];

@@ -26,7 +26,7 @@ class MPMParams:
 
     def __init__(self, n_grid=64, space_scale=1.0, obj_scale=1.0, rho=1.0, dt=5e-5,
                  gravity=(0.0, -9.8, 0.0), bound=3, eps=1e-5, sticky_floor=False,
-                 clamp_grad=False, grad_clamp=1000.0):
+                 clamp_grad=False, grad_clamp=1000.0, svd_exact=True):
         self.n_grid = int(n_grid)
         self.space_scale = float(space_scale)
         self.obj_scale = float(obj_scale)
@@ -42,6 +42,8 @@ class MPMParams:
         self.sticky_floor = bool(sticky_floor)
         self.clamp_grad = bool(clamp_grad)
         self.grad_clamp = float(grad_clamp)
+        # exact SVD adjoint in the fused path (see dt_mpm_params.svd_exact)
+        self.svd_exact = bool(svd_exact)
 
     @property
     def n_nodes(self):
@@ -63,6 +65,7 @@ class MPMParams:
             p.gravity[i] = self.gravity[i]
         p.grad_clamp = self.grad_clamp
         p.soa = int(soa)
+        p.svd_exact = int(self.svd_exact)
         return p
 
     def key_bits(self):

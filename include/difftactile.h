@@ -52,6 +52,11 @@ typedef struct dt_mpm_params {
                            Honoured by dt_mpm_g2p*, dt_mpm_p2g_fused*, dt_mpm_substep_*,
                            dt_mpm_advance, dt_mpm_run_* and dt_mpm_cell_keys; the other
                            per-kernel entry points require 0.                       */
+  int32_t svd_exact;    /* fused adjoints (dt_mpm_p2g_fused_adj and the loops built on
+                           it): 0 = svd_grad as the reference writes it (1e-8 clamp on
+                           s_j^2 - s_i^2, mpm_elastic.py:149-155); 1 = exact adjoint
+                           (rotation path (M_ij - M_ji)/(s_i + s_j), no clamp). The
+                           per-kernel dt_mpm_svd_adj is always the reference form.     */
 } dt_mpm_params;
 
 /* One row of the material table (multi_obj.py:65-74 generalised). */

@@ -33,7 +33,7 @@ def coupled_substep(prm, state, mats, pads, cp, clamp=False):
     Returns (next state, list of (pos', vel', vpos'), contact_idx, nodes)."""
     x, v, C, F = state
     Fn = om.compute_new_F(C, F, prm.dt)
-    U, s, V = om.svd3(Fn)
+    U, s, V = om.svd3(Fn, clamp=1e-30 if getattr(prm, "svd_exact", False) else 1e-8)
     mv, m, F1 = om.p2g(prm, x, v, C, Fn, U, s, V, mats)
     vel_u = [of.update(p["pos"], p["vel"], p["f2v"], p["B"], p["mu"], p["lam"], p["dt"], p["mass"])
              for p in pads]

@@ -55,7 +55,8 @@ class MPMParams:
     """
 
     def __init__(self, n_grid=64, space_scale=1.0, obj_scale=1.0, rho=1.0, dt=1e-4,
-                 gravity=(0.0, -9.8, 0.0), bound=3, eps=1e-5, sticky_floor=False):
+                 gravity=(0.0, -9.8, 0.0), bound=3, eps=1e-5, sticky_floor=False,
+                 svd_exact=False):
         self.n_grid = int(n_grid)
         self.dx = float(space_scale / n_grid)
         self.inv_dx = 1.0 / self.dx
@@ -66,6 +67,10 @@ class MPMParams:
         self.bound = int(bound)
         self.eps = float(eps)
         self.sticky_floor = bool(sticky_floor)
+        # False: the reference's svd_grad (clamp |s_j^2 - s_i^2| >= 1e-8).
+        # True: exact adjoint -- rotation path in the closed form
+        # (M_ij - M_ji) / (s_i + s_j), other SVD terms without the 1e-8 clamp.
+        self.svd_exact = bool(svd_exact)
 
 
 def lame(E, nu):
@@ -194,12 +199,12 @@ def svd3_np(F):
     return U, s, V
 
 
-def _clamp_denominator(a):
+def _clamp_denominator(a, lim=1e-8):
     """``clamp`` of ``mpm_elastic.py:149-155``: |a| >= 1e-8 keeping the sign (0 -> +1e-8)."""
-    return torch.where(a >= 0, torch.clamp(a, min=1e-8), torch.clamp(a, max=-1e-8))
+    return torch.where(a >= 0, torch.clamp(a, min=lim), torch.clamp(a, max=-lim))
 
 
-def svd_grad_ref(U, sig, V, gU, gsig, gV):
+def svd_grad_ref(U, sig, V, gU, gsig, gV, clamp=1e-8):
     """Hand-written SVD adjoint of ``mpm_elastic.py:157-173`` (``single_svd_grad``).
 
     F_bar = U S_bar V^T + U [(ff o (U^T U_bar - U_bar^T U)) S] V^T
@@ -214,7 +219,7 @@ def svd_grad_ref(U, sig, V, gU, gsig, gV):
     s_term = U @ torch.diag_embed(gsig) @ Vt
     s2 = sig * sig
     diff = s2[..., None, :] - s2[..., :, None]          # [i, j] = s_j - s_i
-    ff = 1.0 / _clamp_denominator(diff)
+    ff = 1.0 / _clamp_denominator(diff, clamp)
     eye = torch.eye(3, dtype=torch.bool)
     ff = torch.where(eye, torch.zeros_like(ff), ff)
     u_term = U @ ((ff * (Ut @ gU - gU.transpose(-1, -2) @ U)) @ S) @ Vt
@@ -226,12 +231,13 @@ class _SVD3(torch.autograd.Function):
     """``ti.svd`` forward with ``svd_grad`` as its adjoint (``grasp_plastic.py:177``)."""
 
     @staticmethod
-    def forward(ctx, F):
+    def forward(ctx, F, clamp):
         U, s, V = svd3_np(F.detach().cpu().numpy())
         U = torch.from_numpy(U).to(F.dtype)
         s = torch.from_numpy(s).to(F.dtype)
         V = torch.from_numpy(V).to(F.dtype)
         ctx.save_for_backward(U, s, V)
+        ctx.clamp = clamp
         return U, s, V
 
     @staticmethod
@@ -240,12 +246,35 @@ class _SVD3(torch.autograd.Function):
         gU = torch.zeros_like(U) if gU is None else gU
         gs = torch.zeros_like(s) if gs is None else gs
         gV = torch.zeros_like(V) if gV is None else gV
-        return svd_grad_ref(U, s, V, gU, gs, gV)
+        return svd_grad_ref(U, s, V, gU, gs, gV, ctx.clamp), None
 
 
-def svd3(F):
+def svd3(F, clamp=1e-8):
     """(U, sigma, V) of each 3x3 in ``F (N,3,3)``; differentiable with ``svd_grad_ref``."""
-    return _SVD3.apply(F)
+    return _SVD3.apply(F, clamp)
+
+
+class _Polar(torch.autograd.Function):
+    """R = U V^T (rotation of the polar decomposition) with its exact adjoint:
+    F_bar = U K V^T, K_ij = (M_ij - M_ji) / (s_i + s_j), M = U^T R_bar V.
+    Equal to svd_grad's rotation terms wherever |s_j^2 - s_i^2| >= 1e-8, and
+    the correct limit where the reference's clamp is active."""
+
+    @staticmethod
+    def forward(ctx, F, U, s, V):
+        ctx.save_for_backward(U.detach(), s.detach(), V.detach())
+        return U.detach() @ V.detach().transpose(-1, -2)
+
+    @staticmethod
+    def backward(ctx, gR):
+        U, s, V = ctx.saved_tensors
+        M = U.transpose(-1, -2) @ gR @ V
+        den = s[..., :, None] + s[..., None, :]
+        tiny = torch.where(den < 0, torch.full_like(den, -1e-12), torch.full_like(den, 1e-12))
+        den = torch.where(den.abs() < 1e-12, tiny, den)
+        K = (M - M.transpose(-1, -2)) / den
+        K = K * (1 - torch.eye(3, dtype=K.dtype))
+        return U @ K @ V.transpose(-1, -2), None, None, None
 
 
 # ---------------------------------------------------------------------------
@@ -299,7 +328,8 @@ def drucker_prager(F, U, sig, V, mu, lam, alpha):
     return torch.where(changed[..., None, None], Fy, F), changed
 
 
-def particle_stress(F_new, U, sig, V, mat_mu, mat_lam, mat_kind, mat_yield, mat_alpha):
+def particle_stress(F_new, U, sig, V, mat_mu, mat_lam, mat_kind, mat_yield, mat_alpha,
+                    R=None):
     """Fixed-corotated Kirchhoff stress and the post-return-map F.
 
     ``mpm_elastic.py:183-185`` (elastic: J = det Sigma, F_p = F_new);
@@ -310,7 +340,8 @@ def particle_stress(F_new, U, sig, V, mat_mu, mat_lam, mat_kind, mat_yield, mat_
     mu = mat_mu[..., None, None]
     lam = mat_lam[..., None, None]
     Vt = V.transpose(-1, -2)
-    R = U @ Vt
+    if R is None:
+        R = U @ Vt
     F_vm, _ = von_mises(F_new, U, sig, V, mat_mu, mat_yield)
     F_dp, _ = drucker_prager(F_new, U, sig, V, mat_mu, mat_lam, mat_alpha)
     k = mat_kind[..., None, None]
@@ -346,7 +377,8 @@ def p2g(prm, x, v, C, F_new, U, sig, V, mats):
     """
     n = prm.n_grid
     base, fx, w = stencil(x, prm.inv_dx)
-    tau, F_p = particle_stress(F_new, U, sig, V, *mats)
+    R = _Polar.apply(F_new, U, sig, V) if getattr(prm, "svd_exact", False) else None
+    tau, F_p = particle_stress(F_new, U, sig, V, *mats, R=R)
     stress = (-prm.dt * prm.p_vol * 4 * prm.inv_dx * prm.inv_dx) * tau
     affine = stress + prm.p_mass * C
     flat, weight = stencil_nodes(base, w, n)
@@ -418,7 +450,7 @@ def substep(prm, state, mats, grid_f=None, return_grid=False):
     """
     x, v, C, F = state
     F_new = compute_new_F(C, F, prm.dt)
-    U, sig, V = svd3(F_new)
+    U, sig, V = svd3(F_new, clamp=1e-30 if getattr(prm, "svd_exact", False) else 1e-8)
     grid_mv, grid_m, F_next = p2g(prm, x, v, C, F_new, U, sig, V, mats)
     grid_v = grid_op(prm, grid_mv, grid_m, grid_f)
     nx, nv, nC = g2p(prm, x, grid_v)

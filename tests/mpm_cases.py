@@ -35,7 +35,8 @@ def materials(kind):
 def oracle_params(prm):
     return om.MPMParams(n_grid=prm.n_grid, space_scale=prm.space_scale, obj_scale=prm.obj_scale,
                         rho=prm.rho, dt=prm.dt, gravity=prm.gravity, bound=prm.bound,
-                        eps=prm.eps, sticky_floor=prm.sticky_floor)
+                        eps=prm.eps, sticky_floor=prm.sticky_floor,
+                        svd_exact=getattr(prm, "svd_exact", False))
 
 
 def oracle_mats(mats, mat_id, dtype):

@@ -303,3 +303,32 @@ def test_substep_reverse_mode_matches_central_differences(kind):
         lm = L(tuple(t - h * di for t, di in zip((x, v, C, F), d))).item()
     fd = (lp - lm) / (2 * h)
     assert abs(ad - fd) <= 1e-6 * max(1.0, abs(fd)), (ad, fd)
+
+
+def test_polar_rotation_adjoint_exact_at_near_repeated_singular_values():
+    """_Polar's adjoint (M_ij - M_ji)/(s_i + s_j) against FP64 central differences,
+    including F = I + O(1e-9) where the reference's clamp changes svd_grad."""
+    rng = np.random.default_rng(8)
+    for scale in (1e-9, 1e-3, 0.2):
+        F = torch.eye(3, dtype=F64) + scale * torch.tensor(rng.normal(size=(1, 3, 3)), dtype=F64)
+        W = torch.tensor(rng.normal(size=(1, 3, 3)), dtype=F64)
+
+        def loss(F):
+            U, s, V = om.svd3(F.detach())
+            return (om._Polar.apply(F, U, s, V) * W).sum()
+
+        def rot(F):
+            U, s, Vh = np.linalg.svd(F[0].numpy())
+            R = U @ Vh
+            return torch.tensor(R, dtype=F64)
+
+        Fv = F.clone().requires_grad_(True)
+        g = torch.autograd.grad(loss(Fv), Fv)[0][0]
+        fd = torch.zeros(3, 3, dtype=F64)
+        h = 1e-6  # R(F) is smooth at F = I; rounding / h stays ~1e-10
+        for i in range(3):
+            for j in range(3):
+                E = torch.zeros(1, 3, 3, dtype=F64)
+                E[0, i, j] = h
+                fd[i, j] = ((rot(F + E) - rot(F - E)) * W[0]).sum() / (2 * h)
+        assert torch.allclose(g, fd, rtol=1e-5, atol=1e-6), scale
