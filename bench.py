@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad-check", action="store_true")
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "pad_press"],
+                    help="sphere: 1M-particle multi-material sphere (default, the metric's "
+                         "workload); pad_press: slab pad pressing a von Mises block")
     return ap.parse_args()
 
 
@@ -224,6 +227,65 @@ def grad_check(torch):
                     "random direction, eps 1e-4"}
 
 
+def bench_pad_press(args, torch):
+    """Pad pressing a block: coupled fwd+bwd over the full press (500 substeps,
+    checkpoint every 50); loss = summed squared marker displacement of the pad at
+    the end; gradient w.r.t. the pad velocity sequence."""
+    import numpy as np
+    from difftactile_amd import scenes
+    from difftactile_amd.coupled import CoupledSim, Pad
+    from difftactile_amd.losses import MarkerModel, marker_displacement
+    from difftactile_amd.sim import MPMSim
+    c = scenes.config2_pad_press()
+    prm, n = c["prm"], len(c["x"])
+    T = c["n_substeps"]
+    nodes, f2v, tris, layer, od = c["mesh"]
+    sim = MPMSim(prm, c["mats"], n, checkpoint_every=50, sort=False)
+    pad = Pad(nodes.astype(np.float32), f2v, tris, layer, od, c["pad_mu"], c["pad_lam"],
+              c["pad_mass"], prm.dt)
+    cs = CoupledSim(sim, [pad], **c["contact"])
+    surf_ids = np.where(layer == layer.max())[0]
+    mm = MarkerModel(nodes, surf_ids, c["inv_T"])
+    dev = torch.device("cuda")
+    x0 = torch.tensor(c["x"], device=dev)
+    v0 = torch.tensor(c["v"], device=dev)
+    C0 = torch.zeros(n, 3, 3, device=dev)
+    F0 = torch.eye(3, device=dev).repeat(n, 1, 1)
+    pos0 = torch.tensor(nodes, dtype=torch.float32, device=dev)
+    cv = torch.tensor(c["control_vel"], device=dev)
+
+    def step():
+        cvl = cv.clone().requires_grad_(True)
+        pst = (pos0.clone(), cv[0].clone(), pos0.clone())
+        out = cs.rollout(x0, v0, C0, F0, [pst], [cvl], T)
+        pos, vel, vpos = out[4][0]
+        loss = marker_displacement(mm, pos, vpos)
+        g = torch.autograd.grad(loss, cvl)[0]
+        return loss, g
+
+    log(f"[bench] pad_press: {n} particles, {len(f2v)} tets, {T} substeps")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, g = step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gv = g.sum(1)          # d loss / d (pad velocity of frame f), summed over vertices
+    res = {"metric": "fwd+bwd particle-substeps/s (pad press)",
+           "value": n * T * args.steps / wall, "unit": "particle-substeps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+           "substeps_per_s": T * args.steps / wall, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"slab pad ({len(f2v)} tets) pressing a von Mises block "
+                                  f"({n} particles, {prm.n_grid}^3), {T} substeps fwd+bwd, "
+                                  "loss = summed marker displacement"},
+           "loss": float(loss), "grad_finite": bool(torch.isfinite(g).all()),
+           "grad_dL_dvy_first_last": [float(gv[0, 1]), float(gv[-1, 1])]}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -231,6 +293,9 @@ def main():
               "part of this build", file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.workload == "pad_press":
+        bench_pad_press(args, torch)
+        return
     from difftactile_amd import scenes
     from difftactile_amd.sim import MPMSim
 

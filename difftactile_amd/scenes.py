@@ -82,3 +82,59 @@ def config3_multi_material(n_particles=1_000_000, n_grid=128, dt=1e-5, seed=0):
 
 def identity_F(n):
     return np.tile(np.eye(3, dtype=np.float32), (n, 1, 1))
+
+
+def config2_pad_press(n_particles=200_000, n_grid=128, dt=1e-5, n_substeps=500, seed=0):
+    """Slab pad pressing an elastoplastic block.
+
+    Domain 0.1 m per side (dx = 0.1/128 m).  Block: n_particles on a lattice at
+    dx/2 spacing (+-0.25 dx seeded jitter) resting on the floor band, von Mises
+    yield 50 (E = 5e4, nu = 0.3, rho = 1000).  Pad: 40 x 40 x 6 vertex slab
+    (45,630 tets, 3,042 contact triangles), spacing dx, Neo-Hookean E = 1e5,
+    nu = 0.45, rho = 1000; contact face 0.25 dx above the block top; the fixed
+    top layer is driven down 2 mm at constant velocity over n_substeps.
+    Penalty contact kn = 1e4 (kd, kt, friction from the grasp task).
+    Returns a dict with everything the coupled runner needs.
+    """
+    from .params import VON_MISES
+    from .sensor_model.meshes import slab_mesh
+    space = 0.1
+    prm = MPMParams(n_grid=n_grid, space_scale=space, rho=1000.0, dt=dt,
+                    gravity=(0.0, -9.8, 0.0))
+    dx = prm.dx
+    spacing = dx / 2
+    # block: nx * ny * nz lattice with ny = nx / 2.5 (a slab-like block)
+    nx = int(round((n_particles * 2.5) ** (1.0 / 3.0)))
+    ny = max(1, -(-n_particles // (nx * nx)))
+    rng = np.random.default_rng(seed)
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nx), indexing="ij")
+    p = np.stack([ii, jj, kk], -1).reshape(-1, 3)[:n_particles].astype(np.float64)
+    base_y = (prm.bound + 1) * dx
+    x = np.empty_like(p)
+    x[:, 0] = 0.5 * space + (p[:, 0] - (nx - 1) / 2) * spacing
+    x[:, 1] = base_y + p[:, 1] * spacing
+    x[:, 2] = 0.5 * space + (p[:, 2] - (nx - 1) / 2) * spacing
+    x += rng.uniform(-0.25 * dx, 0.25 * dx, size=x.shape)
+    top = x[:, 1].max()
+    mats = [Material(5e4, 0.3, VON_MISES, yield_stress=50.0)]
+    # pad
+    h = dx
+    ox = 0.5 * space - 19.5 * h
+    nodes, f2v, tris, layer, out_dir = slab_mesh(40, 40, 6, h=h, origin=(ox, top + 0.25 * dx, ox))
+    E, nu, rho_pad = 1e5, 0.45, 1000.0
+    mu, lam = E / 2 / (1 + nu), E * nu / (1 + nu) / (1 - 2 * nu)
+    vol = (39 * h) * (39 * h) * (5 * h)
+    mass = rho_pad * vol / len(nodes)
+    press = 2e-3 / (n_substeps * dt)
+    cv = np.zeros((n_substeps + 1, len(nodes), 3), np.float32)
+    cv[:, :, 1] = -press
+    # camera: 1 cm behind the contact face, looking at it (pad-local +y toward the face)
+    cam_origin = np.array([0.5 * space, top + 0.25 * dx + 0.01, 0.5 * space])
+    Rwl = np.diag([1.0, -1.0, -1.0])     # world -> local rotation (180 deg about x)
+    inv_T = np.eye(4)
+    inv_T[:3, :3] = Rwl
+    inv_T[:3, 3] = -Rwl @ cam_origin
+    return dict(prm=prm, mats=mats, x=x.astype(np.float32),
+                v=np.zeros((len(x), 3), np.float32), mesh=(nodes, f2v, tris, layer, out_dir),
+                pad_mu=mu, pad_lam=lam, pad_mass=mass, control_vel=cv, inv_T=inv_T,
+                contact=dict(kn=1e4, kd=269.44, kt=154.78, mu_f=53.85), n_substeps=n_substeps)

@@ -183,6 +183,9 @@ def test_coupled_substep_forward_and_adjoint_match_oracle(gpu):
     obj, pad = _build_facade(prm, mesh, pos, x, v)
     sc = GridContact(obj, [pad], kn=1e3, kd=10.0, kt=50.0, friction_coeff=0.5, clamp_grad=False)
     sc.reset()
+    # update() adds the tet forces into vel[0] in place (fem_sensor.py:439-440):
+    # keep the pre-update velocity as the oracle's input
+    vel_before = h64(pad.vel.data[0])
     sc.update(0)
     torch.cuda.synchronize()
     # oracle, FP64, same inputs (B and the contact indices from float32 data); the
@@ -195,7 +198,7 @@ def test_coupled_substep_forward_and_adjoint_match_oracle(gpu):
           torch.zeros(n, 3, 3, dtype=F64, requires_grad=True),
           torch.eye(3, dtype=F64).repeat(n, 1, 1).requires_grad_(True)]
     padd = dict(pos=torch.tensor(pos, dtype=F64, requires_grad=True),
-                vel=h64(pad.vel.data[0]).requires_grad_(True),
+                vel=vel_before.clone().requires_grad_(True),
                 vpos=torch.tensor(pos, dtype=F64), cv=h64(pad.control_vel.data), cv_next=None,
                 f2v=torch.tensor(f2v).long(), B=h64(pad.B.data), layer=torch.tensor(layer),
                 tris=surf, out_dir=torch.tensor(od, dtype=F64), mu=float(pad.mu[None]),
