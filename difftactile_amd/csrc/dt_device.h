@@ -321,6 +321,7 @@ struct StressFwd {
   float eh[3];       // deviatoric strain
   float ehn;         // |eh|_eps
   float trv;         // tr(eps)
+  float pa, pG;      // yielded: d_k = exp(epsn_k) = pG * s_k^pa (s = max(sigma, 0.05))
   int mode;          // 0 none, 1 von Mises yield, 2 DP shear, 3 DP tension
 };
 
@@ -335,6 +336,8 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
                       StressFwd& st, M3& tau) {
   st.mode = 0;
   st.Fp = Fn;
+  st.pa = 0.f;
+  st.pG = 1.f;
   if (m.kind != KIND_ELASTIC) {
     float eps[3];
     for (int i = 0; i < 3; ++i) {
@@ -345,26 +348,32 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
     st.trv = tr;
     for (int i = 0; i < 3; ++i) st.eh[i] = eps[i] - tr / 3.f;
     st.ehn = sqrtf(st.eh[0] * st.eh[0] + st.eh[1] * st.eh[1] + st.eh[2] * st.eh[2] + 1e-8f);
+    // eps' = mean(eps) + pa (eps - mean(eps)), so d_k = pG s_k^pa with
+    // pG = exp((1 - pa) mean(eps))
     if (m.kind == KIND_VON_MISES) {
       float dg = st.ehn - m.yield_stress / (2.f * m.mu);
       if (dg > 0.f) {
         st.mode = 1;
         float r = dg / st.ehn;
         for (int i = 0; i < 3; ++i) st.epsn[i] = eps[i] - r * st.eh[i];
+        st.pa = 1.f - r;
       }
     } else {
       if (tr >= 0.f) {
         st.mode = 3;
         for (int i = 0; i < 3; ++i) st.epsn[i] = 0.f;
+        st.pa = 0.f;
       } else {
         float dg = st.ehn + (3.f * m.lam + 2.f * m.mu) / (2.f * m.mu) * tr * m.dp_alpha;
         if (dg > 0.f) {
           st.mode = 2;
           float r = dg / st.ehn;
           for (int i = 0; i < 3; ++i) st.epsn[i] = eps[i] - r * st.eh[i];
+          st.pa = 1.f - r;
         }
       }
     }
+    st.pG = st.mode == 3 ? 1.f : expf((1.f - st.pa) * tr / 3.f);
     if (st.mode) {
       float d[3] = {expf(st.epsn[0]), expf(st.epsn[1]), expf(st.epsn[2])};
       udv(U, d, V, st.Fp);
@@ -467,48 +476,76 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
       }
     }
   }
-  // SVD adjoint in float64:
+  // SVD adjoint in float64.
   //   reference form: svd_grad of gU = gR V + gUp, gV = gR^T U + gVp with the
   //     1e-8 clamp on s_j^2 - s_i^2 (mpm_elastic.py:157-173);
-  //   exact form: rotation path as (M_ij - M_ji) / (s_i + s_j), M = U^T gR V,
-  //     and the return-map terms without the clamp (their numerators vanish
-  //     with s_i - s_j).
-  double s2[3] = {(double)sig[0] * sig[0], (double)sig[1] * sig[1], (double)sig[2] * sig[2]};
-  M3d gU = gUp, gV = gVp;
-  M3d Mr;
+  //   exact form, with no small denominators: the rotation path as
+  //     (M_ij - M_ji) / (s_i + s_j), M = U^T gR V; the return-map path
+  //     (F_p = U diag(d) V^T, d_k = G s_k^a) as N_ij A_ij + N_ji B_ij with
+  //     N = U^T gF_p V, A/B = +-(d_i + d_j) / (2 (s_i + s_j)) + Dd_ij / 2 and
+  //     Dd_ij = (d_j - d_i) / (s_j - s_i) evaluated through expm1/log1p.
+  M3d Mi;
   if (exact) {
-    Mr = m3d_mul(m3d_mul_at(Ud, gRd), Vd);
+    M3d Mr = m3d_mul(m3d_mul_at(Ud, gRd), Vd);
+    const bool pl = m.kind != KIND_ELASTIC && st.mode != 0;
+    M3d N;
+    double d[3] = {1.0, 1.0, 1.0}, sc[3];
+    if (pl) {
+      N = m3d_mul(m3d_mul_at(Ud, m3d_from(gFp)), Vd);
+      for (int k = 0; k < 3; ++k) {
+        d[k] = exp((double)st.epsn[k]);
+        sc[k] = fmax((double)sig[k], 0.05);
+      }
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        if (i == j) {
+          Mi.a[i * 3 + j] = gsig[i];
+          continue;
+        }
+        double den = (double)sig[i] + (double)sig[j];
+        if (fabs(den) < 1e-12) den = den < 0.0 ? -1e-12 : 1e-12;
+        double v = (Mr.a[i * 3 + j] - Mr.a[j * 3 + i]) / den;
+        if (pl) {
+          double si = sig[i], sj = sig[j], del = sj - si, Dd = 0.0;
+          double a = st.pa, G = st.pG;
+          if (si > 0.05 && sj > 0.05) {
+            if (del != 0.0)
+              Dd = G * pow(si, a) * expm1(a * log1p(del / si)) / del;
+            else
+              Dd = G * a * pow(si, a - 1.0);
+          } else if (del != 0.0) {
+            Dd = G * (pow(sc[j], a) - pow(sc[i], a)) / del;
+          }
+          double hs = 0.5 * (d[i] + d[j]) / den;
+          v += N.a[i * 3 + j] * (hs + 0.5 * Dd) + N.a[j * 3 + i] * (-hs + 0.5 * Dd);
+        }
+        Mi.a[i * 3 + j] = v;
+      }
   } else {
+    double s2[3] = {(double)sig[0] * sig[0], (double)sig[1] * sig[1], (double)sig[2] * sig[2]};
+    M3d gU = gUp, gV = gVp;
     M3d a = m3d_mul(gRd, Vd), b = m3d_mul_at(gRd, Ud);
     for (int i = 0; i < 9; ++i) {
       gU.a[i] += a.a[i];
       gV.a[i] += b.a[i];
     }
-  }
-  M3d Xu = m3d_mul_at(Ud, gU);   // U^T gU
-  M3d Xv = m3d_mul_at(Vd, gV);   // V^T gV
-  const double lim = exact ? 1e-30 : 1e-8;
-  M3d Mi;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      double v = 0.0;
-      if (i != j) {
-        double dd = s2[j] - s2[i];
-        dd = dd >= 0.0 ? fmax(dd, lim) : fmin(dd, -lim);
-        double ff = 1.0 / dd;
-        double xu = Xu.a[i * 3 + j] - Xu.a[j * 3 + i];
-        double xv = Xv.a[i * 3 + j] - Xv.a[j * 3 + i];
-        v = ff * xu * (double)sig[j] + (double)sig[i] * ff * xv;
-        if (exact) {
-          double den = (double)sig[i] + (double)sig[j];
-          if (fabs(den) < 1e-12) den = den < 0.0 ? -1e-12 : 1e-12;
-          v += (Mr.a[i * 3 + j] - Mr.a[j * 3 + i]) / den;
+    M3d Xu = m3d_mul_at(Ud, gU);   // U^T gU
+    M3d Xv = m3d_mul_at(Vd, gV);   // V^T gV
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double v = 0.0;
+        if (i != j) {
+          double ff = 1.0 / clampd(s2[j] - s2[i]);
+          double xu = Xu.a[i * 3 + j] - Xu.a[j * 3 + i];
+          double xv = Xv.a[i * 3 + j] - Xv.a[j * 3 + i];
+          v = ff * xu * (double)sig[j] + (double)sig[i] * ff * xv;
+        } else {
+          v = gsig[i];
         }
-      } else {
-        v = gsig[i];
+        Mi.a[i * 3 + j] = v;
       }
-      Mi.a[i * 3 + j] = v;
-    }
+  }
   M3d G = m3d_mul_bt(m3d_mul(Ud, Mi), Vd);
   for (int i = 0; i < 9; ++i) gFn.a[i] += (float)G.a[i];
 }

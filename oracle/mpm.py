@@ -249,6 +249,51 @@ class _SVD3(torch.autograd.Function):
         return svd_grad_ref(U, s, V, gU, gs, gV, ctx.clamp), None
 
 
+class _UDV(torch.autograd.Function):
+    """F_p = U diag(d) V^T of the return map with an exact adjoint.
+
+    d_k = G s_k^a (s = max(sigma, 0.05)) for every yielded branch.  The
+    off-diagonal (U, V) part of the adjoint is N_ij A_ij + N_ji B_ij with
+    N = U^T F_p_bar V, A/B = +-(d_i + d_j) / (2 (s_i + s_j)) + Dd_ij / 2,
+    Dd_ij = (d_j - d_i) / (s_j - s_i) (its limit at s_i = s_j); equal to the
+    SVD adjoint formula for distinct singular values without its small
+    denominators.  The adjoint of d is diag(N) and flows back through d(s).
+    """
+
+    @staticmethod
+    def forward(ctx, F, U, V, d, s, a, G):
+        ctx.save_for_backward(U, V, d, s, a, G)
+        return U @ torch.diag_embed(d) @ V.transpose(-1, -2)
+
+    @staticmethod
+    def backward(ctx, g):
+        U, V, d, s, a, G = ctx.saved_tensors
+        N = U.transpose(-1, -2) @ g @ V
+        si, sj = s[..., :, None], s[..., None, :]
+        di, dj = d[..., :, None], d[..., None, :]
+        a_, G_ = a[..., None, None], G[..., None, None]
+        den = si + sj
+        tiny = torch.where(den < 0, torch.full_like(den, -1e-12), torch.full_like(den, 1e-12))
+        den = torch.where(den.abs() < 1e-12, tiny, den)
+        delta = sj - si
+        both = (si > 0.05) & (sj > 0.05)
+        safe_si = torch.where(both, si, torch.ones_like(si))
+        safe_del = torch.where(delta != 0, delta, torch.ones_like(delta))
+        dd_smooth = torch.where(delta != 0,
+                                G_ * safe_si ** a_ * torch.expm1(a_ * torch.log1p(delta / safe_si)) / safe_del,
+                                G_ * a_ * safe_si ** (a_ - 1))
+        sci, scj = torch.clamp(si, min=0.05), torch.clamp(sj, min=0.05)
+        dd_clamped = torch.where(delta != 0, G_ * (scj ** a_ - sci ** a_) / safe_del,
+                                 torch.zeros_like(delta))
+        Dd = torch.where(both, dd_smooth, dd_clamped)
+        hs = 0.5 * (di + dj) / den
+        Mo = N * (hs + 0.5 * Dd) + N.transpose(-1, -2) * (-hs + 0.5 * Dd)
+        Mo = Mo * (1 - torch.eye(3, dtype=Mo.dtype))
+        gF = U @ Mo @ V.transpose(-1, -2)
+        gd = torch.diagonal(N, dim1=-2, dim2=-1)
+        return gF, None, None, gd, None, None, None
+
+
 def svd3(F, clamp=1e-8):
     """(U, sigma, V) of each 3x3 in ``F (N,3,3)``; differentiable with ``svd_grad_ref``."""
     return _SVD3.apply(F, clamp)
@@ -286,7 +331,7 @@ def _norm_eps(x, eps=1e-8):
     return torch.sqrt((x * x).sum(-1) + eps)
 
 
-def von_mises(F, U, sig, V, mu, yield_stress):
+def von_mises(F, U, sig, V, mu, yield_stress, exact=False):
     """Return map of ``compute_von_mises`` (``mpm_plastic.py:150-166``).
 
     sigma <- max(sigma, 0.05); eps = log sigma; eps_hat = eps - tr(eps)/3;
@@ -300,11 +345,16 @@ def von_mises(F, U, sig, V, mu, yield_stress):
     dg = ehn - yield_stress / (2 * mu)
     yielded = dg > 0
     eps_y = eps - (dg / ehn)[..., None] * eh
-    Fy = U @ torch.diag_embed(torch.exp(eps_y)) @ V.transpose(-1, -2)
+    if exact:
+        a = (1 - dg / ehn).detach()
+        G = torch.exp((1 - a) * eps.sum(-1) / 3).detach()
+        Fy = _UDV.apply(F, U.detach(), V.detach(), torch.exp(eps_y), sig.detach(), a, G)
+    else:
+        Fy = U @ torch.diag_embed(torch.exp(eps_y)) @ V.transpose(-1, -2)
     return torch.where(yielded[..., None, None], Fy, F), yielded
 
 
-def drucker_prager(F, U, sig, V, mu, lam, alpha):
+def drucker_prager(F, U, sig, V, mu, lam, alpha, exact=False):
     """Drucker-Prager return map in Hencky strain (Klar et al. 2016, sec. 7.3).
 
     No reference implementation exists (the reference has no sand model).
@@ -323,13 +373,19 @@ def drucker_prager(F, U, sig, V, mu, lam, alpha):
     shear = (~tension) & (dg > 0)
     eps_y = eps - (dg / ehn)[..., None] * eh
     eps_new = torch.where(tension[..., None], torch.zeros_like(eps), eps_y)
-    Fy = U @ torch.diag_embed(torch.exp(eps_new)) @ V.transpose(-1, -2)
+    if exact:
+        a = torch.where(tension, torch.zeros_like(dg), 1 - dg / ehn).detach()
+        G = torch.where(tension, torch.ones_like(dg),
+                        torch.exp((1 - a) * tr / 3)).detach()
+        Fy = _UDV.apply(F, U.detach(), V.detach(), torch.exp(eps_new), sig.detach(), a, G)
+    else:
+        Fy = U @ torch.diag_embed(torch.exp(eps_new)) @ V.transpose(-1, -2)
     changed = tension | shear
     return torch.where(changed[..., None, None], Fy, F), changed
 
 
 def particle_stress(F_new, U, sig, V, mat_mu, mat_lam, mat_kind, mat_yield, mat_alpha,
-                    R=None):
+                    R=None, exact=False):
     """Fixed-corotated Kirchhoff stress and the post-return-map F.
 
     ``mpm_elastic.py:183-185`` (elastic: J = det Sigma, F_p = F_new);
@@ -342,8 +398,8 @@ def particle_stress(F_new, U, sig, V, mat_mu, mat_lam, mat_kind, mat_yield, mat_
     Vt = V.transpose(-1, -2)
     if R is None:
         R = U @ Vt
-    F_vm, _ = von_mises(F_new, U, sig, V, mat_mu, mat_yield)
-    F_dp, _ = drucker_prager(F_new, U, sig, V, mat_mu, mat_lam, mat_alpha)
+    F_vm, _ = von_mises(F_new, U, sig, V, mat_mu, mat_yield, exact)
+    F_dp, _ = drucker_prager(F_new, U, sig, V, mat_mu, mat_lam, mat_alpha, exact)
     k = mat_kind[..., None, None]
     F_p = torch.where(k == KIND_VON_MISES, F_vm, torch.where(k == KIND_DRUCKER_PRAGER, F_dp, F_new))
     J_el = sig[..., 0] * sig[..., 1] * sig[..., 2]
@@ -378,7 +434,8 @@ def p2g(prm, x, v, C, F_new, U, sig, V, mats):
     n = prm.n_grid
     base, fx, w = stencil(x, prm.inv_dx)
     R = _Polar.apply(F_new, U, sig, V) if getattr(prm, "svd_exact", False) else None
-    tau, F_p = particle_stress(F_new, U, sig, V, *mats, R=R)
+    tau, F_p = particle_stress(F_new, U, sig, V, *mats, R=R,
+                               exact=getattr(prm, "svd_exact", False))
     stress = (-prm.dt * prm.p_vol * 4 * prm.inv_dx * prm.inv_dx) * tau
     affine = stress + prm.p_mass * C
     flat, weight = stencil_nodes(base, w, n)

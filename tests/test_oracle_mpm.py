@@ -276,9 +276,10 @@ def test_grid_op_closed_form_and_walls():
 # ---------------------------------------------------------------------------
 # the composed substep: reverse mode vs FP64 central differences
 # ---------------------------------------------------------------------------
+@pytest.mark.parametrize("exact", [False, True])
 @pytest.mark.parametrize("kind", [om.KIND_ELASTIC, om.KIND_VON_MISES, om.KIND_DRUCKER_PRAGER])
-def test_substep_reverse_mode_matches_central_differences(kind):
-    prm = om.MPMParams(n_grid=16, rho=1000.0, dt=2e-5)
+def test_substep_reverse_mode_matches_central_differences(kind, exact):
+    prm = om.MPMParams(n_grid=16, rho=1000.0, dt=2e-5, svd_exact=exact)
     n = 60
     rng = np.random.default_rng(6)
     x = torch.tensor(0.5 + 0.08 * rng.normal(size=(n, 3)), dtype=F64)
@@ -332,3 +333,33 @@ def test_polar_rotation_adjoint_exact_at_near_repeated_singular_values():
                 E[0, i, j] = h
                 fd[i, j] = ((rot(F + E) - rot(F - E)) * W[0]).sum() / (2 * h)
         assert torch.allclose(g, fd, rtol=1e-5, atol=1e-6), scale
+
+
+@pytest.mark.parametrize("kind", [om.KIND_ELASTIC, om.KIND_VON_MISES, om.KIND_DRUCKER_PRAGER])
+def test_exact_adjoint_from_the_undeformed_state(kind):
+    """From F = I (all singular values equal; Drucker-Prager particles start in the
+    tension branch) the exact adjoint stays finite and matches FP64 central
+    differences over 3 substeps."""
+    prm = om.MPMParams(n_grid=16, rho=1000.0, dt=1e-5, svd_exact=True)
+    n = 80
+    rng = np.random.default_rng(9)
+    x = torch.tensor(0.5 + 0.06 * rng.normal(size=(n, 3)), dtype=F64)
+    v = torch.tensor(0.3 * rng.normal(size=(n, 3)), dtype=F64)
+    C = torch.zeros(n, 3, 3, dtype=F64)
+    F = torch.eye(3, dtype=F64).repeat(n, 1, 1)
+    mats = om.material_arrays([om.Material(5e4, 0.3, kind, yield_stress=30.0)], np.zeros(n, int), F64)
+    w = torch.tensor(rng.normal(size=(n, 3)), dtype=F64)
+
+    def L(vv):
+        out = om.rollout(prm, (x, vv, C, F), mats, 3)
+        return (out[0] * w).sum() + (out[3] * out[3]).sum()
+
+    vv = v.clone().requires_grad_(True)
+    g = torch.autograd.grad(L(vv), vv)[0]
+    assert torch.isfinite(g).all()
+    d = torch.tensor(rng.normal(size=(n, 3)), dtype=F64)
+    h = 1e-6
+    with torch.no_grad():
+        fd = (L(v + h * d) - L(v - h * d)) / (2 * h)
+    ad = (g * d).sum()
+    assert abs(ad - fd) <= 1e-5 * max(1.0, abs(fd.item())), (ad.item(), fd.item())
