@@ -64,8 +64,10 @@ def time_stage(torch, fn, reps):
 
 
 def kernel_breakdown(torch, sim, state, mat_id, reps):
-    """Per-stage event timings of one fwd+bwd substep on resident inputs."""
+    """Per-stage event timings of one fwd+bwd substep of the sparse path on
+    resident inputs (the stages dt_mpm_run_fwd / dt_mpm_run_adj launch)."""
     from difftactile_amd import _lib
+    from difftactile_amd.sim import workspace_bytes
     prm = sim.c_prm
     n = sim.n
     nn = sim.prm.n_nodes
@@ -75,56 +77,46 @@ def kernel_breakdown(torch, sim, state, mat_id, reps):
     gv4 = torch.zeros(nn, 4, device=dev)
     x1, v1 = torch.empty_like(x), torch.empty_like(v)
     C1, F1 = torch.empty_like(C), torch.empty_like(F)
-    mats = _lib.ptr(sim.mats)
+    nbytes = workspace_bytes(prm, n)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    mats = sim.mats
     nm = len(sim.materials)
-    mid = None if mat_id is None else _lib.ptr(mat_id, torch.int32)
-    P = _lib.ptr
     st = _lib.stream()
-    # build the grids once
-    _lib.call("dt_mpm_substep_fwd", prm, mats, nm, n, mid, P(x), P(v), P(C), P(F), P(g4), P(gv4),
-              P(x1), P(v1), P(C1), P(F1), st)
     gen = torch.Generator(device=dev).manual_seed(0)
     gnext = [torch.randn(t.shape, device=dev, generator=gen) for t in state]
     gg4 = torch.zeros(nn, 4, device=dev)
     ggv4 = torch.zeros(nn, 4, device=dev)
     gout = [torch.empty_like(t) for t in state]
-    g4b = torch.zeros_like(g4)
+
+    def binning():
+        _lib.call("dt_mpm_bin", prm, n, x, ws, nbytes, st)
 
     def p2g():
-        _lib.call("dt_mpm_p2g_fused", prm, mats, nm, n, mid, P(x), P(v), P(C), P(F), P(g4b),
-                  P(F1), st)
-
-    nb = int(_lib.lib().dt_mpm_brick_count(prm))
-    flags = torch.zeros(nb, dtype=torch.int32, device=dev)
-    _lib.call("dt_mpm_mark_bricks", prm, n, x, flags, st)
+        _lib.call("dt_mpm_p2g_binned", prm, mats, nm, n, mat_id, x, v, C, F, F1, ws, nbytes, st)
 
     def grid_op():
-        _lib.call("dt_mpm_grid_op4_bricks", prm, flags, g4, gv4, st)
+        _lib.call("dt_mpm_grid_op4_tiles", prm, n, ws, nbytes, g4, gv4, st)
 
     def g2p():
-        _lib.call("dt_mpm_g2p4", prm, n, P(x), P(gv4), P(x1), P(v1), P(C1), st)
+        _lib.call("dt_mpm_g2p4", prm, n, x, gv4, x1, v1, C1, st)
 
     def g2p_adj():
-        _lib.call("dt_mpm_g2p4_adj", prm, n, P(x), P(gv4), P(gnext[0]), P(gnext[1]),
-                  P(gnext[2]), P(gout[0]), P(ggv4), st)
+        _lib.call("dt_mpm_g2p4_adj_binned", prm, n, x, gv4, gnext[0], gnext[1], gnext[2],
+                  gout[0], ws, nbytes, st)
 
     def grid_op_adj():
-        _lib.call("dt_mpm_grid_op4_adj_bricks", prm, flags, g4, ggv4, gg4, st)
+        _lib.call("dt_mpm_grid_op4_adj_tiles", prm, n, ws, nbytes, g4, ggv4, gg4, st)
 
     def p2g_adj():
-        _lib.call("dt_mpm_p2g_fused_adj", prm, mats, nm, n, mid, P(x), P(v), P(C), P(F), P(gg4),
-                  P(gnext[3]), P(gout[0]), P(gout[1]), P(gout[2]), P(gout[3]), st)
+        _lib.call("dt_mpm_p2g_fused_adj", prm, mats, nm, n, mat_id, x, v, C, F, gg4,
+                  gnext[3], gout[0], gout[1], gout[2], gout[3], st)
 
-    def mark():
-        _lib.call("dt_mpm_mark_bricks", prm, n, x, flags, st)
-
-    def zero():
-        _lib.call("dt_mpm_zero4_bricks", prm, flags, ggv4, st)
-
-    # order matters for valid inputs: g2p_adj fills ggv4 before grid_op_adj
-    stages = [("mark_bricks", mark), ("zero4_bricks", zero), ("p2g_fused", p2g),
-              ("grid_op4_bricks", grid_op), ("g2p4", g2p), ("g2p4_adj", g2p_adj),
-              ("grid_op4_adj_bricks", grid_op_adj), ("p2g_fused_adj", p2g_adj)]
+    # order matters for valid inputs: each stage consumes the previous one's output
+    stages = [("bin", binning), ("p2g_binned", p2g), ("grid_op4_tiles", grid_op), ("g2p4", g2p),
+              ("g2p4_adj_binned", g2p_adj), ("grid_op4_adj_tiles", grid_op_adj),
+              ("p2g_fused_adj", p2g_adj)]
+    for _, fn in stages:
+        fn()
     out = {}
     for name, fn in stages:
         out[name] = time_stage(torch, fn, reps)
@@ -140,17 +132,18 @@ def algorithmic_bytes(n, active, has_mat):
     """
     mid = 4 if has_mat else 0
     return {
-        # read x v C F (+id), write F_next; flush (mv, m) per active node
-        "p2g_fused": n * (96 + mid + 36) + active * 16,
-        # read (mv, m), write (v) per active node (dense pass reads every node's m)
-        "mark_bricks": n * 12,
-        "zero4_bricks": active * 16,
-        "grid_op4_bricks": active * 32,
+        # read x (mark + key), write the permutation
+        "bin": n * 16,
+        # read x v C F (+id), write F_next; produce (mv, m) per active node
+        "p2g_binned": n * (96 + mid + 36) + active * 16,
+        # read (mv, m), write (v) per active node
+        "grid_op4_tiles": active * 32,
         # read x, write x v C; gather v per active node
         "g2p4": n * (12 + 60) + active * 16,
-        # read x, gx' gv' gC' (60), write gx; scatter g_v per active node
-        "g2p4_adj": n * (12 + 60 + 12) + active * 16,
-        "grid_op4_adj_bricks": active * 48,
+        # read x, gx' gv' gC' (60), write gx; produce g_v per active node
+        "g2p4_adj_binned": n * (12 + 60 + 12) + active * 16,
+        # read (mv, m), g_v; write (g_mv, g_m) per active node
+        "grid_op4_adj_tiles": active * 48,
         # read x v C F (+id), gx partial, gF'; write gx gv gC gF; gather (g_mv, g_m)
         "p2g_fused_adj": n * (96 + mid + 12 + 36 + 96) + active * 16,
     }

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdifftact
 _c_int = ctypes.c_int32
 _c_float = ctypes.c_float
 _vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
 
 
 class DtMpmParams(ctypes.Structure):
@@ -67,11 +68,20 @@ _SIGS = {
     "dt_mpm_substep_fwd": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 10 + [_vp]),
     "dt_mpm_substep_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 16 + [_vp]),
     "dt_mpm_run_fwd": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
-                                _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+                                _c_int, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "dt_mpm_run_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
-                                _c_int] + [_vp] * 12 + [_vp, _vp]),
+                                _c_int] + [_vp] * 12 + [_vp, _sz, _vp]),
     "dt_mpm_advance": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 8
-                       + [_c_int, _vp, _vp, _vp, _vp]),
+                       + [_c_int, _vp, _vp, _vp, _sz, _vp]),
+    "dt_mpm_workspace_bytes": (_c_int, [_P(DtMpmParams), _c_int, _P(_sz)]),
+    "dt_mpm_bin": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _sz, _vp]),
+    "dt_mpm_bin_perm": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _P(_vp), _P(_vp)]),
+    "dt_mpm_p2g_binned": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 5
+                          + [_vp, _sz, _vp]),
+    "dt_mpm_grid_op4_tiles": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _vp, _vp]),
+    "dt_mpm_g2p4_adj_binned": (_c_int, [_P(DtMpmParams), _c_int] + [_vp] * 6 + [_vp, _sz, _vp]),
+    "dt_mpm_grid_op4_adj_tiles": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _vp, _vp,
+                                           _vp]),
     "dt_mpm_brick_count": (ctypes.c_int64, [_P(DtMpmParams)]),
     "dt_mpm_mark_bricks": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp]),
     "dt_mpm_zero4_bricks": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp]),

@@ -1,0 +1,122 @@
+// Per-substep binning of particles by stencil-base cell (counting sort).
+//
+// k_bin_count gives every particle its rank inside its cell with one global
+// atomic per run of equal keys in a wavefront (cell-sorted particles make a
+// run of every cell's particles in the wave), an exclusive scan turns the
+// counts into cell starts, and k_bin_fill writes the permutation.  The order
+// of particles inside a cell follows the atomics and is not deterministic
+// across waves; nothing downstream depends on it beyond float summation order.
+#include <hipcub/hipcub.hpp>
+
+#include "dt_bin.h"
+#include "dt_device.h"
+
+namespace dt {
+
+long long brick_total(const dt_mpm_params* prm) {
+  long long nb = (prm->n_grid + 3) >> 2;
+  return nb * nb * nb;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
+  const long long nbk = brick_total(prm), ncell = nbk * 64;
+  size_t scan_bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
+                                   (int32_t*)nullptr, (int)(ncell + 1));
+  const size_t sz[8] = {sizeof(int32_t) * (size_t)nbk,
+                        sizeof(int32_t) * (size_t)(ncell + 1),
+                        sizeof(int32_t) * (size_t)(ncell + 1),
+                        sizeof(int32_t) * (size_t)n,
+                        sizeof(int32_t) * (size_t)n,
+                        sizeof(int32_t) * (size_t)n,
+                        sizeof(float4) * (size_t)nbk * kTileNodes,
+                        scan_bytes};
+  size_t off[8], total = 0;
+  for (int i = 0; i < 8; ++i) {
+    off[i] = total;
+    total += align256(sz[i] ? sz[i] : 1);
+  }
+  if (base && ws) {
+    char* b = static_cast<char*>(base);
+    ws->flags = reinterpret_cast<int32_t*>(b + off[0]);
+    ws->count = reinterpret_cast<int32_t*>(b + off[1]);
+    ws->start = reinterpret_cast<int32_t*>(b + off[2]);
+    ws->key = reinterpret_cast<int32_t*>(b + off[3]);
+    ws->rank = reinterpret_cast<int32_t*>(b + off[4]);
+    ws->perm = reinterpret_cast<int32_t*>(b + off[5]);
+    ws->tiles = reinterpret_cast<float4*>(b + off[6]);
+    ws->scan_tmp = b + off[7];
+    ws->scan_bytes = scan_bytes;
+    ws->n_bricks = nbk;
+    ws->n_cells = ncell;
+  }
+  return total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_zero(long long n, int32_t* __restrict__ a) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// Active lanes form a prefix of every wavefront (p = block * 256 + thread).
+__global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float* __restrict__ x,
+                                                      int32_t* __restrict__ key,
+                                                      int32_t* __restrict__ rank,
+                                                      int32_t* __restrict__ count) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = p < n;
+  int k = -1;
+  if (active) {
+    float xp[3];
+    ld_v3(x, p, pl_vec(P.soa, n), xp);
+    k = bin_key(xp, P.inv_dx, P.n_grid);
+  }
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(k, 1);
+  const bool head = active && (lane == 0 || prev != k);
+  const unsigned long long hm = __ballot(head);
+  const unsigned long long act = __ballot(active);
+  const unsigned long long le = (2ull << lane) - 1ull;   // lanes <= this lane
+  const int my_head = active ? 63 - __clzll(hm & le) : 0;
+  int old = 0;
+  if (head) {
+    const unsigned long long later = hm & ~le;
+    const int end = later ? __ffsll((long long)later) - 1 : __popcll(act);
+    old = atomicAdd(&count[k], end - lane);
+  }
+  const int base = __shfl(old, my_head);
+  if (active) {
+    key[p] = k;
+    rank[p] = base + (lane - my_head);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __restrict__ key,
+                                                     const int32_t* __restrict__ rank,
+                                                     const int32_t* __restrict__ start,
+                                                     int32_t* __restrict__ perm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) perm[start[key[p]] + rank[p]] = p;
+}
+
+int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st) {
+  const long long nc = ws.n_cells + 1;
+  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nc)), dim3(kBlock), 0, st, nc, ws.count);
+  if (n > 0)
+    hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
+                       ws.rank, ws.count);
+  size_t bytes = ws.scan_bytes;
+  if (hipcub::DeviceScan::ExclusiveSum(ws.scan_tmp, bytes, ws.count, ws.start, (int)nc, st) !=
+      hipSuccess) {
+    set_error("binning scan failed");
+    return DT_ELAUNCH;
+  }
+  if (n > 0)
+    hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
+                       ws.start, ws.perm);
+  return check_launch("particle binning");
+}
+
+}  // namespace dt

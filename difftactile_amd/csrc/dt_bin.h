@@ -1,0 +1,35 @@
+// Per-substep particle binning for the brick-owned scatter passes.
+//
+// Particles are binned by the cell of their stencil base, cells ordered
+// brick-major (key = brick * 64 + local, 4x4x4-cell bricks).  After
+// launch_bin, perm[start[c] .. start[c+1]) lists the particles of cell c, so
+// the particles of brick b are perm[start[64 b] .. start[64 b + 64]).
+#pragma once
+#include "dt_common.h"
+
+namespace dt {
+
+constexpr int kTileEdge = 6;   // a brick's 4 cells plus the stencil's 2-node reach
+constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
+
+// Views into one caller-provided workspace buffer.
+struct BinWs {
+  int32_t* flags;   // [n_bricks]    bricks some stencil touches
+  int32_t* count;   // [n_cells + 1] particles per cell (last entry stays 0)
+  int32_t* start;   // [n_cells + 1] exclusive prefix sum of count
+  int32_t* key;     // [n]           cell key of each particle
+  int32_t* rank;    // [n]           position of the particle inside its cell
+  int32_t* perm;    // [n]           particles in cell order
+  float4* tiles;    // [n_bricks * kTileNodes] per-brick partial node sums
+  void* scan_tmp;
+  size_t scan_bytes;
+  long long n_bricks, n_cells;
+};
+
+long long brick_total(const dt_mpm_params* prm);
+// Total bytes of the workspace for n particles; fills *ws when base != NULL.
+size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws);
+// count -> scan -> fill on stream st (flags untouched).
+int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st);
+
+}  // namespace dt

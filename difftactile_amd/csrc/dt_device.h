@@ -299,6 +299,22 @@ DT_HD int cell_key(const float (&x)[3], float inv_dx, int n_grid) {
   return brick * 64 + local;
 }
 
+// Binning key of a particle: the cell of its (clamped) stencil base, ordered
+// like cell_key (brick * 64 + local).  Uses the base expression of
+// make_stencil so the key names exactly the cell whose stencil the particle
+// scatters through.
+DT_HD int bin_key(const float (&x)[3], float inv_dx, int n_grid) {
+  int b[3];
+  for (int a = 0; a < 3; ++a) {
+    int v = cell_base1(x[a], inv_dx);
+    b[a] = v < 0 ? 0 : (v > n_grid - 3 ? n_grid - 3 : v);
+  }
+  int nb = (n_grid + 3) >> 2;
+  int brick = ((b[0] >> 2) * nb + (b[1] >> 2)) * nb + (b[2] >> 2);
+  int local = (((b[0] & 3) << 2) + (b[1] & 3)) * 4 + (b[2] & 3);
+  return brick * 64 + local;
+}
+
 // ----------------------------------------------------------------------------
 // materials
 // ----------------------------------------------------------------------------

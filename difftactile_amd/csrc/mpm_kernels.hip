@@ -9,20 +9,27 @@
 //   check_grid_occupy mpm_elastic.py:210-214  grid_op mpm_elastic.py:216-240
 //   g2p           mpm_elastic.py:243-260
 //
-// Scatter strategy (p2g and g2p adjoint): a block first reduces the bounding
-// box of its particles' stencil bases.  When the box (+2 nodes of stencil
-// reach) fits kTileNodes, the block accumulates into an LDS tile with LDS
-// float atomics and flushes each touched node once with global float atomics;
-// otherwise it falls back to direct global atomics.  Cell-sorted particles make
-// the LDS path the common one; correctness never depends on the order.
+// Scatter strategy (p2g and g2p adjoint).  Every particle's contribution to
+// stencil node base + o (o in {0,1,2}^3) has the form
+//     w_o(fx) * (b + M o),     w_o = wx[o0] wy[o1] wz[o2],
+// with b, M per particle (p2g: b = p_mass v - A fx dx, M = A dx; g2p adjoint:
+// b = gvt - 4 inv_dx gC fx, M = 4 inv_dx gC).  A block stages (w, b, M) of its
+// particles in LDS and splits them into runs of consecutive particles that
+// share a stencil base (cell).  One work item per (run, o1, o2) sums the run's
+// contributions to the three nodes o0 = 0..2 and adds each sum to the grid
+// with one global float atomic per component.  With cell-sorted particles a
+// run holds all of a cell's particles in the block (about 8 at two particles
+// per cell edge), so a node receives one atomic per neighbouring cell instead
+// of one per particle, and no LDS float atomics are used (those execute per
+// lane on gfx950 and dominated the per-particle scatter).  Unsorted input
+// gives runs of one particle and stays correct.
 #include <climits>
 
+#include "dt_bin.h"
 #include "dt_common.h"
 #include "dt_device.h"
 
 namespace dt {
-
-constexpr int kTileNodes = 1024;
 
 struct Grid {
   float* mv;   // momentum (or any 3-vector)
@@ -30,30 +37,18 @@ struct Grid {
   int smv, sm; // strides in floats
 };
 
-struct Tile {
-  int o[3], e[3];
-  bool use;
-};
+constexpr int kStage = kBlock + 1;   // padded row stride of the LDS staging rows
 
-__device__ __forceinline__ void tile_setup(bool active, const int (&base)[3], int* sh, Tile& t) {
-  if (threadIdx.x == 0) {
-    sh[0] = sh[1] = sh[2] = INT_MAX;
-    sh[3] = sh[4] = sh[5] = INT_MIN;
-  }
-  __syncthreads();
-  if (active) {
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(&sh[a], base[a]);
-      atomicMax(&sh[3 + a], base[a]);
-    }
-  }
-  __syncthreads();
-  for (int a = 0; a < 3; ++a) {
-    t.o[a] = sh[a];
-    t.e[a] = sh[3 + a] - sh[a] + 3;
-  }
-  t.use = (sh[0] <= sh[3]) && (t.e[0] * t.e[1] * t.e[2] <= kTileNodes);
-}
+// LDS staging of one block's scatter: row r of w/b/M holds component r of every
+// particle (w rows: wx0..2, wy0..2, wz0..2; M rows: row-major 3x3).
+struct RunStage {
+  float w[9 * kStage];
+  float b[3 * kStage];
+  float M[9 * kStage];
+  int key[kBlock];           // linear node index of the stencil base, -1 when inactive
+  int start[kBlock + 1];     // first particle of each run, start[R] = active count
+  int wtot[kBlock / 64];
+};
 
 __device__ __forceinline__ int node_index(int i, int j, int k, int n) { return (i * n + j) * n + k; }
 
@@ -96,98 +91,108 @@ __device__ __forceinline__ M3 affine_of(const MpmP& P, const M3& tau, const M3& 
   return A;
 }
 
-// Scatter one particle's 27 contributions (mv: weight*(p_mass v + A dpos),
-// m: weight*p_mass) either into the LDS tile or directly into the grid.
-template <bool kLds>
-__device__ __forceinline__ void p2g_scatter(const MpmP& P, const Stencil& s, const float (&pv)[3],
-                                            const M3& A, const Tile& t, float* tile, Grid g) {
-  const int n = P.n_grid;
+// Stage one particle (thread) and split the block into runs; returns the run
+// count.  Active threads must be a prefix of the block (p < n).
+__device__ __forceinline__ int stage_runs(RunStage& sh, bool active, int n_act, const Stencil& s,
+                                          int n_grid, const float (&b)[3], const M3& M) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int key = active ? (s.base[0] * n_grid + s.base[1]) * n_grid + s.base[2] : -1;
+  sh.key[t] = key;
+  if (active) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+      for (int o = 0; o < 3; ++o) sh.w[(3 * a + o) * kStage + t] = s.w[o][a];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float w = s.w[i][0] * s.w[j][1] * s.w[k][2];
-        float d0 = ((float)i - s.fx[0]) * P.dx;
-        float d1 = ((float)j - s.fx[1]) * P.dx;
-        float d2 = ((float)k - s.fx[2]) * P.dx;
-        float c0 = w * (pv[0] + (A.a[0] * d0 + A.a[1] * d1 + A.a[2] * d2));
-        float c1 = w * (pv[1] + (A.a[3] * d0 + A.a[4] * d1 + A.a[5] * d2));
-        float c2 = w * (pv[2] + (A.a[6] * d0 + A.a[7] * d1 + A.a[8] * d2));
-        float cm = w * P.p_mass;
-        int bi = s.base[0] + i, bj = s.base[1] + j, bk = s.base[2] + k;
-        if (kLds) {
-          int l = ((bi - t.o[0]) * t.e[1] + (bj - t.o[1])) * t.e[2] + (bk - t.o[2]);
-          atomicAdd(&tile[l * 4 + 0], c0);
-          atomicAdd(&tile[l * 4 + 1], c1);
-          atomicAdd(&tile[l * 4 + 2], c2);
-          atomicAdd(&tile[l * 4 + 3], cm);
-        } else {
-          long long nd = node_index(bi, bj, bk, n);
-          atomic_add_f(g.mv + nd * g.smv + 0, c0);
-          atomic_add_f(g.mv + nd * g.smv + 1, c1);
-          atomic_add_f(g.mv + nd * g.smv + 2, c2);
-          atomic_add_f(g.m + nd * g.sm, cm);
-        }
+    for (int c = 0; c < 3; ++c) sh.b[c * kStage + t] = b[c];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) sh.M[r * kStage + t] = M.a[r];
+  }
+  __syncthreads();
+  const bool head = active && (t == 0 || sh.key[t - 1] != key);
+  const unsigned long long hm = __ballot(head);
+  const int below = __popcll(hm & ((1ull << lane) - 1ull));
+  if (lane == 0) sh.wtot[wv] = __popcll(hm);
+  __syncthreads();
+  int off = 0, R = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / 64; ++i) {
+    if (i < wv) off += sh.wtot[i];
+    R += sh.wtot[i];
+  }
+  if (head) sh.start[off + below] = t;
+  if (t == 0) sh.start[R] = n_act;
+  __syncthreads();
+  return R;
+}
+
+// Sum each run's contributions per node and add them to the grid (see the
+// file header).  kMass adds p_mass * sum(w) to the mass channel.
+template <bool kMass>
+__device__ __forceinline__ void run_scatter(const RunStage& sh, int R, int n_grid, Grid g,
+                                            float p_mass) {
+  const long long n2 = (long long)n_grid * n_grid;
+  for (int it = threadIdx.x; it < R * 9; it += blockDim.x) {
+    const int run = it / 9, jk = it - run * 9;
+    const int j = jk / 3, k = jk - 3 * j;
+    const float fj = (float)j, fk = (float)k;
+    const int q0 = sh.start[run], q1 = sh.start[run + 1];
+    float acc[3][3], accm[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      accm[i] = 0.f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[i][c] = 0.f;
+    }
+    for (int q = q0; q < q1; ++q) {
+      const float wyz = sh.w[(3 + j) * kStage + q] * sh.w[(6 + k) * kStage + q];
+      float u[3], m0[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        m0[c] = sh.M[(3 * c) * kStage + q];
+        u[c] = sh.b[c * kStage + q] + sh.M[(3 * c + 1) * kStage + q] * fj +
+               sh.M[(3 * c + 2) * kStage + q] * fk;
       }
-}
-
-__device__ __forceinline__ void tile_zero(float* tile, const Tile& t, int comps) {
-  int vol = t.e[0] * t.e[1] * t.e[2] * comps;
-  for (int l = threadIdx.x; l < vol; l += blockDim.x) tile[l] = 0.f;
-}
-
-// flush a (mv, m) tile: one global atomic per component of each touched node
-__device__ __forceinline__ void tile_flush4(const float* tile, const Tile& t, int n, Grid g) {
-  int vol = t.e[0] * t.e[1] * t.e[2];
-  for (int l = threadIdx.x; l < vol; l += blockDim.x) {
-    float cm = tile[l * 4 + 3];
-    if (cm != 0.f) {
-      int k = l % t.e[2], r = l / t.e[2];
-      int j = r % t.e[1], i = r / t.e[1];
-      long long nd = node_index(t.o[0] + i, t.o[1] + j, t.o[2] + k, n);
-      atomic_add_f(g.mv + nd * g.smv + 0, tile[l * 4 + 0]);
-      atomic_add_f(g.mv + nd * g.smv + 1, tile[l * 4 + 1]);
-      atomic_add_f(g.mv + nd * g.smv + 2, tile[l * 4 + 2]);
-      atomic_add_f(g.m + nd * g.sm, cm);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float wi = sh.w[i * kStage + q] * wyz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[i][c] += wi * (u[c] + m0[c] * (float)i);
+        accm[i] += wi;
+      }
+    }
+    const long long nd0 = (long long)sh.key[q0] + (long long)j * n_grid + k;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const long long nd = nd0 + i * n2;
+      atomic_add_f(g.mv + nd * g.smv + 0, acc[i][0]);
+      atomic_add_f(g.mv + nd * g.smv + 1, acc[i][1]);
+      atomic_add_f(g.mv + nd * g.smv + 2, acc[i][2]);
+      if (kMass) atomic_add_f(g.m + nd * g.sm, p_mass * accm[i]);
     }
   }
 }
 
-// flush a 3-vector tile (adjoint scatters); a node is skipped when all zero
-__device__ __forceinline__ void tile_flush3(const float* tile, const Tile& t, int n, float* out,
-                                            int stride) {
-  int vol = t.e[0] * t.e[1] * t.e[2];
-  for (int l = threadIdx.x; l < vol; l += blockDim.x) {
-    float a = tile[l * 4 + 0], b = tile[l * 4 + 1], c = tile[l * 4 + 2];
-    if (a != 0.f || b != 0.f || c != 0.f) {
-      int k = l % t.e[2], r = l / t.e[2];
-      int j = r % t.e[1], i = r / t.e[1];
-      long long nd = node_index(t.o[0] + i, t.o[1] + j, t.o[2] + k, n);
-      atomic_add_f(out + nd * stride + 0, a);
-      atomic_add_f(out + nd * stride + 1, b);
-      atomic_add_f(out + nd * stride + 2, c);
-    }
-  }
+__device__ __forceinline__ int block_active(int n) {
+  const int first = xcd_block() * blockDim.x;
+  return n - first >= (int)blockDim.x ? (int)blockDim.x : (n > first ? n - first : 0);
 }
 
-// Shared body of the p2g kernels once the per-particle stress is known.
-__device__ __forceinline__ void p2g_block(const MpmP& P, bool active, const Stencil& s,
-                                          const float (&pv)[3], const M3& A, Grid g, float* tile,
-                                          int* sh) {
-  Tile t;
-  int base[3] = {s.base[0], s.base[1], s.base[2]};
-  tile_setup(active, base, sh, t);
-  if (t.use) {
-    tile_zero(tile, t, 4);
-    __syncthreads();
-    if (active) p2g_scatter<true>(P, s, pv, A, t, tile, g);
-    __syncthreads();
-    tile_flush4(tile, t, P.n_grid, g);
-  } else if (active) {
-    p2g_scatter<false>(P, s, pv, A, t, tile, g);
+// Shared body of the p2g kernels once the per-particle stress is known:
+// contribution w_o (p_mass v + A (o - fx) dx), mass w_o p_mass.
+__device__ __forceinline__ void p2g_block(const MpmP& P, bool active, int n, const Stencil& s,
+                                          const float (&pv)[3], const M3& A, Grid g,
+                                          RunStage& sh) {
+  float b[3] = {0.f, 0.f, 0.f};
+  M3 M = m3_zero();
+  if (active) {
+    for (int c = 0; c < 3; ++c) {
+      b[c] = pv[c] - (A.a[3 * c] * s.fx[0] + A.a[3 * c + 1] * s.fx[1] + A.a[3 * c + 2] * s.fx[2]) * P.dx;
+      for (int a = 0; a < 3; ++a) M.a[3 * c + a] = A.a[3 * c + a] * P.dx;
+    }
   }
+  const int R = stage_runs(sh, active, block_active(n), s, P.n_grid, b, M);
+  run_scatter<true>(sh, R, P.n_grid, g, P.p_mass);
 }
 
 // ----------------------------------------------------------------------------
@@ -292,8 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmP P, const dt_material* __res
                                                 const float* __restrict__ S,
                                                 const float* __restrict__ V, Grid g,
                                                 float* __restrict__ F_next) {
-  __shared__ float tile[kTileNodes * 4];
-  __shared__ int sh[6];
+  __shared__ RunStage sh;
   int p = xcd_block() * blockDim.x + threadIdx.x;
   bool active = p < n;
   Stencil s;
@@ -314,12 +318,45 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmP P, const dt_material* __res
     make_stencil(xp, P.inv_dx, P.n_grid, s);
     for (int a = 0; a < 3; ++a) pv[a] = P.p_mass * v[3ll * p + a];
   }
-  p2g_block(P, active, s, pv, A, g, tile, sh);
+  p2g_block(P, active, n, s, pv, A, g, sh);
 }
 
 // ----------------------------------------------------------------------------
 // fused compute_new_F + svd + return map + p2g: reads (x, v, C, F)
 // ----------------------------------------------------------------------------
+// compute_new_F + svd + return map of particle p; writes F_next and returns
+// the stencil and the scatter coefficients (b, M) of the p2g contribution
+// w_o (p_mass v + A (o - fx) dx) = w_o (b + M o)
+__device__ __forceinline__ void p2g_fused_particle(const MpmP& P, const dt_material* mats, int n,
+                                                   const int32_t* mat_id, const float* x,
+                                                   const float* v, const float* C, const float* F,
+                                                   float* F_next, int p, Stencil& s,
+                                                   float (&b)[3], M3& M) {
+  PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
+  M3 c = ld_m3L(C, p, Lm), f = ld_m3L(F, p, Lm);
+  M3 a = m3_eye();
+  for (int i = 0; i < 9; ++i) a.a[i] += P.dt * c.a[i];
+  M3 fn = m3_mul(a, f);
+  M3 u, vv;
+  float sig[3];
+  svd3(fn, u, sig, vv);
+  Mat m = load_mat(mats, mat_id, p);
+  StressFwd st;
+  M3 tau;
+  stress_fwd(fn, u, sig, vv, m, st, tau);
+  const M3 A = affine_of(P, tau, c);
+  st_m3L(F_next, p, Lm, st.Fp);
+  float xp[3], vp[3];
+  ld_v3(x, p, Lv, xp);
+  ld_v3(v, p, Lv, vp);
+  make_stencil(xp, P.inv_dx, P.n_grid, s);
+  for (int q = 0; q < 3; ++q) {
+    b[q] = P.p_mass * vp[q] -
+           (A.a[3 * q] * s.fx[0] + A.a[3 * q + 1] * s.fx[1] + A.a[3 * q + 2] * s.fx[2]) * P.dx;
+    for (int r = 0; r < 3; ++r) M.a[3 * q + r] = A.a[3 * q + r] * P.dx;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_p2g_fused(MpmP P, const dt_material* __restrict__ mats,
                                                       int n, const int32_t* __restrict__ mat_id,
                                                       const float* __restrict__ x,
@@ -327,36 +364,16 @@ __global__ __launch_bounds__(kBlock) void k_p2g_fused(MpmP P, const dt_material*
                                                       const float* __restrict__ C,
                                                       const float* __restrict__ F, Grid g,
                                                       float* __restrict__ F_next) {
-  __shared__ float tile[kTileNodes * 4];
-  __shared__ int sh[6];
+  __shared__ RunStage sh;
   int p = xcd_block() * blockDim.x + threadIdx.x;
   bool active = p < n;
   Stencil s;
-  M3 A;
-  float pv[3] = {0.f, 0.f, 0.f};
   s.base[0] = s.base[1] = s.base[2] = 0;
-  if (active) {
-    PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
-    M3 c = ld_m3L(C, p, Lm), f = ld_m3L(F, p, Lm);
-    M3 a = m3_eye();
-    for (int i = 0; i < 9; ++i) a.a[i] += P.dt * c.a[i];
-    M3 fn = m3_mul(a, f);
-    M3 u, vv;
-    float sig[3];
-    svd3(fn, u, sig, vv);
-    Mat m = load_mat(mats, mat_id, p);
-    StressFwd st;
-    M3 tau;
-    stress_fwd(fn, u, sig, vv, m, st, tau);
-    A = affine_of(P, tau, c);
-    st_m3L(F_next, p, Lm, st.Fp);
-    float xp[3], vp[3];
-    ld_v3(x, p, Lv, xp);
-    ld_v3(v, p, Lv, vp);
-    make_stencil(xp, P.inv_dx, P.n_grid, s);
-    for (int q = 0; q < 3; ++q) pv[q] = P.p_mass * vp[q];
-  }
-  p2g_block(P, active, s, pv, A, g, tile, sh);
+  float b[3] = {0.f, 0.f, 0.f};
+  M3 M = m3_zero();
+  if (active) p2g_fused_particle(P, mats, n, mat_id, x, v, C, F, F_next, p, s, b, M);
+  const int R = stage_runs(sh, active, block_active(n), s, P.n_grid, b, M);
+  run_scatter<true>(sh, R, P.n_grid, g, P.p_mass);
 }
 
 // ----------------------------------------------------------------------------
@@ -832,6 +849,86 @@ __global__ __launch_bounds__(kBlock) void k_grid_op4_adj_bricks(MpmP P,
   gg4[nd] = out;
 }
 
+// Node value from the brick tiles: node g (brick B, local l) receives the tile
+// of brick B - d for every d in {0,1}^3 with l + 4 d < 6 whose brick holds
+// particles.  Fixed summation order.
+__device__ __forceinline__ float4 tile_sum(const MpmP& P, long long nd,
+                                           const float4* __restrict__ tiles,
+                                           const int32_t* __restrict__ start) {
+  const int n = P.n_grid, nb = (n + 3) >> 2;
+  const int g2 = (int)(nd % n), g1 = (int)((nd / n) % n), g0 = (int)(nd / ((long long)n * n));
+  float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int d0 = 0; d0 < 2; ++d0) {
+    const int B0 = (g0 >> 2) - d0, l0 = (g0 & 3) + 4 * d0;
+    if (B0 < 0 || l0 >= kTileEdge) continue;
+    for (int d1 = 0; d1 < 2; ++d1) {
+      const int B1 = (g1 >> 2) - d1, l1 = (g1 & 3) + 4 * d1;
+      if (B1 < 0 || l1 >= kTileEdge) continue;
+      for (int d2 = 0; d2 < 2; ++d2) {
+        const int B2 = (g2 >> 2) - d2, l2 = (g2 & 3) + 4 * d2;
+        if (B2 < 0 || l2 >= kTileEdge) continue;
+        const long long B = ((long long)B0 * nb + B1) * nb + B2;
+        if (start[B * 64] == start[B * 64 + 64]) continue;
+        const float4 q = tiles[B * kTileNodes + (l0 * kTileEdge + l1) * kTileEdge + l2];
+        r.x += q.x;
+        r.y += q.y;
+        r.z += q.z;
+        r.w += q.w;
+      }
+    }
+  }
+  return r;
+}
+
+// grid_op over the active bricks with (mv, m) summed from the p2g tiles;
+// writes the summed grid4 and grid_v4
+__global__ __launch_bounds__(kBlock) void k_grid_op4_tiles(MpmP P, const int32_t* __restrict__ flags,
+                                                           const float4* __restrict__ tiles,
+                                                           const int32_t* __restrict__ start,
+                                                           float4* __restrict__ g4,
+                                                           float4* __restrict__ v4) {
+  long long nd = brick_node(P, flags);
+  if (nd < 0) return;
+  const float4 q = tile_sum(P, nd, tiles, start);
+  g4[nd] = q;
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (q.w > P.eps) {
+    float vo[3];
+    bool mask[3];
+    grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, vo, mask);
+    out = make_float4(vo[0], vo[1], vo[2], 0.f);
+  }
+  v4[nd] = out;
+}
+
+// grid_op adjoint over the active bricks with d/d grid_v_out summed from the
+// g2p-adjoint tiles; writes the summed gv4 and gg4
+__global__ __launch_bounds__(kBlock) void k_grid_op4_adj_tiles(MpmP P,
+                                                               const int32_t* __restrict__ flags,
+                                                               const float4* __restrict__ g4,
+                                                               const float4* __restrict__ tiles,
+                                                               const int32_t* __restrict__ start,
+                                                               float4* __restrict__ gv4,
+                                                               float4* __restrict__ gg4) {
+  long long nd = brick_node(P, flags);
+  if (nd < 0) return;
+  const float4 gi = tile_sum(P, nd, tiles, start);
+  gv4[nd] = gi;
+  const float4 q = g4[nd];
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (q.w > P.eps) {
+    float vo[3];
+    bool mask[3];
+    grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, vo, mask);
+    float g0 = mask[0] ? 0.f : gi.x, g1 = mask[1] ? 0.f : gi.y, g2 = mask[2] ? 0.f : gi.z;
+    float inv_m = 1.f / (q.w + P.eps);
+    float gmm = -inv_m * inv_m * (g0 * q.x + g1 * q.y + g2 * q.z);
+    if (P.clamp_grad) gmm = fminf(fmaxf(gmm, -P.grad_clamp), P.grad_clamp);
+    out = make_float4(inv_m * g0, inv_m * g1, inv_m * g2, gmm);
+  }
+  gg4[nd] = out;
+}
+
 __global__ __launch_bounds__(kBlock) void k_zero_i32(long long n, int32_t* __restrict__ a) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
@@ -889,8 +986,72 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
   st_m3L(C1, p, Lm, nC);
 }
 
-// g2p adjoint: scatter to the grid-velocity adjoint, gather gx.
-//   gx_mode 0: gx += (reference kernel form)   1: gx = (fused form)
+// g2p adjoint of particle p: gathers the grid-velocity adjoint into gx
+// (kGxAssign 1: gx =, 0: gx +=) and returns the stencil and the scatter
+// coefficients of d/d grid_v_out[node] += w_o (gvt + 4 inv_dx gC (o - fx))
+// = w_o (b + M o)
+template <int kGxAssign>
+__device__ __forceinline__ void g2p_adj_particle(const MpmP& P, int n, const float* x,
+                                                 const float* gv, int sgv, const float* gx1,
+                                                 const float* gv1, const float* gC1, float* gx,
+                                                 int p, Stencil& s, float (&b)[3], M3& M) {
+  const int nG = P.n_grid;
+  PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
+  float gvt[3], lgx[3];
+  float xp[3], g1[3], v1[3];
+  ld_v3(x, p, Lv, xp);
+  ld_v3(gx1, p, Lv, g1);
+  ld_v3(gv1, p, Lv, v1);
+  make_stencil(xp, P.inv_dx, nG, s);
+  for (int a = 0; a < 3; ++a) {
+    gvt[a] = v1[a] + P.dt * g1[a];
+    lgx[a] = g1[a];
+  }
+  const M3 gc = ld_m3L(gC1, p, Lm);
+  const float k4 = 4.f * P.inv_dx;
+  float gfx[3] = {0.f, 0.f, 0.f}, gd[3] = {0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const float wi = w_axis0(s, i), dwi = dw_axis0(s, i);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        long long nd = node_index(s.base[0] + i, s.base[1] + j, s.base[2] + k, nG);
+        float g0 = gv[nd * sgv], gg1 = gv[nd * sgv + 1], g2 = gv[nd * sgv + 2];
+        float wj = s.w[j][1], wk = s.w[k][2];
+        float w = wi * wj * wk;
+        float d0 = (float)i - s.fx[0], d1 = (float)j - s.fx[1], d2 = (float)k - s.fx[2];
+        // gC d
+        float cd0 = gc.a[0] * d0 + gc.a[1] * d1 + gc.a[2] * d2;
+        float cd1 = gc.a[3] * d0 + gc.a[4] * d1 + gc.a[5] * d2;
+        float cd2 = gc.a[6] * d0 + gc.a[7] * d1 + gc.a[8] * d2;
+        float gw = g0 * gvt[0] + gg1 * gvt[1] + g2 * gvt[2] +
+                   k4 * (g0 * cd0 + gg1 * cd1 + g2 * cd2);
+        gfx[0] += gw * dwi * wj * wk;
+        gfx[1] += gw * wi * s.dw[j][1] * wk;
+        gfx[2] += gw * wi * wj * s.dw[k][2];
+        // d/d d of k4 w g^T gC d = k4 w gC^T g
+        gd[0] += k4 * w * (gc.a[0] * g0 + gc.a[3] * gg1 + gc.a[6] * g2);
+        gd[1] += k4 * w * (gc.a[1] * g0 + gc.a[4] * gg1 + gc.a[7] * g2);
+        gd[2] += k4 * w * (gc.a[2] * g0 + gc.a[5] * gg1 + gc.a[8] * g2);
+      }
+  }
+  // fx = x inv_dx - base; d = o - fx
+  for (int a = 0; a < 3; ++a) lgx[a] += P.inv_dx * gfx[a] - P.inv_dx * gd[a];
+  for (int a = 0; a < 3; ++a) {
+    long long o = p * Lv.sp + a * Lv.sc;
+    if (kGxAssign) gx[o] = lgx[a];
+    else gx[o] += lgx[a];
+  }
+  for (int c = 0; c < 3; ++c) {
+    b[c] = gvt[c] - k4 * (gc.a[3 * c] * s.fx[0] + gc.a[3 * c + 1] * s.fx[1] +
+                          gc.a[3 * c + 2] * s.fx[2]);
+    for (int a = 0; a < 3; ++a) M.a[3 * c + a] = k4 * gc.a[3 * c + a];
+  }
+}
+
+// g2p adjoint: gather gx, scatter to the grid-velocity adjoint by runs
 template <int kGxAssign>
 __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* __restrict__ x,
                                                     const float* __restrict__ gv, int sgv,
@@ -899,89 +1060,211 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
                                                     const float* __restrict__ gC1,
                                                     float* __restrict__ gx,
                                                     float* __restrict__ ggv, int sggv) {
-  __shared__ float tile[kTileNodes * 4];
-  __shared__ int sh[6];
-  const int nG = P.n_grid;
+  __shared__ RunStage sh;
   int p = xcd_block() * blockDim.x + threadIdx.x;
   bool active = p < n;
   Stencil s;
   s.base[0] = s.base[1] = s.base[2] = 0;
-  float gvt[3] = {0.f, 0.f, 0.f}, lgx[3] = {0.f, 0.f, 0.f};
-  M3 gc = m3_zero();
-  PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
-  if (active) {
-    float xp[3], g1[3], v1[3];
-    ld_v3(x, p, Lv, xp);
-    ld_v3(gx1, p, Lv, g1);
-    ld_v3(gv1, p, Lv, v1);
-    make_stencil(xp, P.inv_dx, nG, s);
-    for (int a = 0; a < 3; ++a) {
-      gvt[a] = v1[a] + P.dt * g1[a];
-      lgx[a] = g1[a];
+  float b[3] = {0.f, 0.f, 0.f};
+  M3 M = m3_zero();
+  if (active) g2p_adj_particle<kGxAssign>(P, n, x, gv, sgv, gx1, gv1, gC1, gx, p, s, b, M);
+  const int R = stage_runs(sh, active, block_active(n), s, P.n_grid, b, M);
+  Grid g{ggv, nullptr, sggv, 0};
+  run_scatter<false>(sh, R, P.n_grid, g, 0.f);
+}
+
+// ----------------------------------------------------------------------------
+// brick-owned scatter over binned particles: block b processes the particles
+// of brick b (perm order: the brick's 64 cells one after another) in chunks of
+// 256 and owns the 6x6x6-node tile of partial sums its stencils reach.  Per
+// chunk:
+//   1. thread t computes particle t and stages (w, b, M) in LDS;
+//   2. work item (cell, j, k) sums, over the cell's particles, the
+//      contributions w_o (b + M o) to the three nodes o = (i, j, k), i = 0..2,
+//      and writes them to its own LDS slots (lanes of one cell read the same
+//      staged particle: LDS broadcast);
+//   3. tile node thread t < 216 adds the up to 27 slots that name it.
+// No atomics anywhere; the tiles are summed per node by the grid passes below.
+// ----------------------------------------------------------------------------
+struct BrickStage {
+  // particle q: pd[6q+0] = (wx0, wx1, wx2, b0), pd[6q+1] = (wy0.., b1),
+  // pd[6q+2] = (wz0.., b2), pd[6q+3..5] = M row-major (3 floats of padding)
+  float4 pd[6 * kBlock];
+  float4 slot[64 * 27];   // (cell, j, k, i): (sum w u0, w u1, w u2, w)
+  int cs[65];             // cell starts of the brick's 64 cells (positions in perm)
+};
+
+__device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Stencil& s,
+                                               const float (&b)[3], const M3& M) {
+  float4* q = sh.pd + 6 * t;
+  q[0] = make_float4(s.w[0][0], s.w[1][0], s.w[2][0], b[0]);
+  q[1] = make_float4(s.w[0][1], s.w[1][1], s.w[2][1], b[1]);
+  q[2] = make_float4(s.w[0][2], s.w[1][2], s.w[2][2], b[2]);
+  q[3] = make_float4(M.a[0], M.a[1], M.a[2], M.a[3]);
+  q[4] = make_float4(M.a[4], M.a[5], M.a[6], M.a[7]);
+  q[5] = make_float4(M.a[8], 0.f, 0.f, 0.f);
+}
+
+__device__ __forceinline__ float pick3(const float4& v, int i) {
+  return i == 0 ? v.x : (i == 1 ? v.y : v.z);
+}
+
+// step 2: items over the cells touched by perm positions [c0, c1)
+__device__ __forceinline__ void brick_items(BrickStage& sh, int c0, int c1) {
+  // cells overlapping the chunk: [cell_lo, cell_hi]
+  int cell_lo = 0, cell_hi = 63;
+  while (sh.cs[cell_lo + 1] <= c0) ++cell_lo;
+  while (sh.cs[cell_hi] >= c1) --cell_hi;
+  const int n_items = (cell_hi - cell_lo + 1) * 9;
+  for (int it = threadIdx.x; it < n_items; it += blockDim.x) {
+    const int cell = cell_lo + it / 9, jk = it % 9;
+    const int j = jk / 3, k = jk - 3 * j;
+    const float fj = (float)j, fk = (float)k;
+    const int q0 = max(sh.cs[cell], c0) - c0, q1 = min(sh.cs[cell + 1], c1) - c0;
+    if (q0 >= q1) continue;
+    float4 acc0 = make_float4(0.f, 0.f, 0.f, 0.f), acc1 = acc0, acc2 = acc0;
+    for (int q = q0; q < q1; ++q) {
+      const float4* pq = sh.pd + 6 * q;
+      const float4 X = pq[0], Y = pq[1], Z = pq[2], M0 = pq[3], M1 = pq[4], M2 = pq[5];
+      const float wyz = pick3(Y, j) * pick3(Z, k);
+      // u = b + M (0, j, k); node i adds w_i (u + M (i, 0, 0))
+      const float u0 = X.w + M0.y * fj + M0.z * fk;
+      const float u1 = Y.w + M1.x * fj + M1.y * fk;
+      const float u2 = Z.w + M1.w * fj + M2.x * fk;
+      const float m00 = M0.x, m10 = M0.w, m20 = M1.z;
+      const float w0 = X.x * wyz, w1 = X.y * wyz, w2 = X.z * wyz;
+      acc0.x += w0 * u0;
+      acc0.y += w0 * u1;
+      acc0.z += w0 * u2;
+      acc0.w += w0;
+      acc1.x += w1 * (u0 + m00);
+      acc1.y += w1 * (u1 + m10);
+      acc1.z += w1 * (u2 + m20);
+      acc1.w += w1;
+      acc2.x += w2 * (u0 + 2.f * m00);
+      acc2.y += w2 * (u1 + 2.f * m10);
+      acc2.z += w2 * (u2 + 2.f * m20);
+      acc2.w += w2;
     }
-    gc = ld_m3L(gC1, p, Lm);
+    float4* sl = sh.slot + (cell * 9 + jk) * 3;
+    sl[0] = acc0;
+    sl[1] = acc1;
+    sl[2] = acc2;
   }
-  Tile t;
-  int base[3] = {s.base[0], s.base[1], s.base[2]};
-  tile_setup(active, base, sh, t);
-  if (t.use) {
-    tile_zero(tile, t, 4);
+}
+
+// step 3: tile node thread adds the slots of the cells that reach it
+__device__ __forceinline__ void brick_gather(const BrickStage& sh, int c0, int c1, float4& acc) {
+  const int t = threadIdx.x;
+  if (t >= kTileNodes) return;
+  const int a0 = t / (kTileEdge * kTileEdge), a1 = (t / kTileEdge) % kTileEdge,
+            a2 = t % kTileEdge;
+#pragma unroll
+  for (int o0 = 0; o0 < 3; ++o0) {
+    const int ca = a0 - o0;
+    if (ca < 0 || ca > 3) continue;
+#pragma unroll
+    for (int o1 = 0; o1 < 3; ++o1) {
+      const int cb = a1 - o1;
+      if (cb < 0 || cb > 3) continue;
+#pragma unroll
+      for (int o2 = 0; o2 < 3; ++o2) {
+        const int cc = a2 - o2;
+        if (cc < 0 || cc > 3) continue;
+        const int cell = (ca * 4 + cb) * 4 + cc;
+        if (max(sh.cs[cell], c0) >= min(sh.cs[cell + 1], c1)) continue;
+        const float4 v = sh.slot[(cell * 9 + o1 * 3 + o2) * 3 + o0];
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void brick_accumulate(BrickStage& sh, int c0, int c1, float4& acc) {
+  brick_items(sh, c0, c1);
+  __syncthreads();
+  brick_gather(sh, c0, c1, acc);
+}
+
+__device__ __forceinline__ bool brick_range(const int32_t* __restrict__ start, long long b,
+                                            BrickStage& sh, int& s0, int& s1) {
+  s0 = start[b * 64];
+  s1 = start[b * 64 + 64];
+  if (s0 == s1) return false;
+  if (threadIdx.x <= 64) sh.cs[threadIdx.x] = start[b * 64 + threadIdx.x];
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_p2g_binned(MpmP P, const dt_material* __restrict__ mats,
+                                                       int n, const int32_t* __restrict__ mat_id,
+                                                       const float* __restrict__ x,
+                                                       const float* __restrict__ v,
+                                                       const float* __restrict__ C,
+                                                       const float* __restrict__ F,
+                                                       float* __restrict__ F_next,
+                                                       const int32_t* __restrict__ perm,
+                                                       const int32_t* __restrict__ start,
+                                                       float4* __restrict__ tiles) {
+  __shared__ BrickStage sh;
+  const long long b = blockIdx.x;
+  int s0, s1;
+  if (!brick_range(start, b, sh, s0, s1)) return;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int c0 = s0; c0 < s1; c0 += kBlock) {
+    const int c1 = min(c0 + (int)kBlock, s1);
+    const int i = c0 + threadIdx.x;
+    if (i < c1) {
+      Stencil s;
+      float bb[3];
+      M3 M;
+      p2g_fused_particle(P, mats, n, mat_id, x, v, C, F, F_next, perm[i], s, bb, M);
+      stage_particle(sh, threadIdx.x, s, bb, M);
+    }
+    __syncthreads();
+    brick_accumulate(sh, c0, c1, acc);
     __syncthreads();
   }
-  if (active) {
-    float gfx[3] = {0.f, 0.f, 0.f}, gd[3] = {0.f, 0.f, 0.f};
-    float k4 = 4.f * P.inv_dx;
-#pragma unroll 1
-    for (int i = 0; i < 3; ++i) {
-      const float wi = w_axis0(s, i), dwi = dw_axis0(s, i);
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          int bi = s.base[0] + i, bj = s.base[1] + j, bk = s.base[2] + k;
-          long long nd = node_index(bi, bj, bk, nG);
-          float g0 = gv[nd * sgv], g1 = gv[nd * sgv + 1], g2 = gv[nd * sgv + 2];
-          float wj = s.w[j][1], wk = s.w[k][2];
-          float w = wi * wj * wk;
-          float d0 = (float)i - s.fx[0], d1 = (float)j - s.fx[1], d2 = (float)k - s.fx[2];
-          // gC d
-          float cd0 = gc.a[0] * d0 + gc.a[1] * d1 + gc.a[2] * d2;
-          float cd1 = gc.a[3] * d0 + gc.a[4] * d1 + gc.a[5] * d2;
-          float cd2 = gc.a[6] * d0 + gc.a[7] * d1 + gc.a[8] * d2;
-          float o0 = w * gvt[0] + k4 * w * cd0;
-          float o1 = w * gvt[1] + k4 * w * cd1;
-          float o2 = w * gvt[2] + k4 * w * cd2;
-          if (t.use) {
-            int l = ((bi - t.o[0]) * t.e[1] + (bj - t.o[1])) * t.e[2] + (bk - t.o[2]);
-            atomicAdd(&tile[l * 4 + 0], o0);
-            atomicAdd(&tile[l * 4 + 1], o1);
-            atomicAdd(&tile[l * 4 + 2], o2);
-          } else {
-            atomic_add_f(ggv + nd * sggv + 0, o0);
-            atomic_add_f(ggv + nd * sggv + 1, o1);
-            atomic_add_f(ggv + nd * sggv + 2, o2);
-          }
-          float gw = g0 * gvt[0] + g1 * gvt[1] + g2 * gvt[2] + k4 * (g0 * cd0 + g1 * cd1 + g2 * cd2);
-          gfx[0] += gw * dwi * wj * wk;
-          gfx[1] += gw * wi * s.dw[j][1] * wk;
-          gfx[2] += gw * wi * wj * s.dw[k][2];
-          // d/d d of k4 w g^T gC d = k4 w gC^T g
-          gd[0] += k4 * w * (gc.a[0] * g0 + gc.a[3] * g1 + gc.a[6] * g2);
-          gd[1] += k4 * w * (gc.a[1] * g0 + gc.a[4] * g1 + gc.a[7] * g2);
-          gd[2] += k4 * w * (gc.a[2] * g0 + gc.a[5] * g1 + gc.a[8] * g2);
-        }
-    }
-    // fx = x inv_dx - base; d = o - fx
-    for (int a = 0; a < 3; ++a) lgx[a] += P.inv_dx * gfx[a] - P.inv_dx * gd[a];
-    for (int a = 0; a < 3; ++a) {
-      long long o = p * Lv.sp + a * Lv.sc;
-      if (kGxAssign) gx[o] = lgx[a];
-      else gx[o] += lgx[a];
-    }
+  if (threadIdx.x < kTileNodes) {
+    acc.w *= P.p_mass;
+    tiles[b * kTileNodes + threadIdx.x] = acc;
   }
-  if (t.use) {
+}
+
+__global__ __launch_bounds__(kBlock) void k_g2p_adj_binned(MpmP P, int n,
+                                                           const float* __restrict__ x,
+                                                           const float* __restrict__ gv, int sgv,
+                                                           const float* __restrict__ gx1,
+                                                           const float* __restrict__ gv1,
+                                                           const float* __restrict__ gC1,
+                                                           float* __restrict__ gx,
+                                                           const int32_t* __restrict__ perm,
+                                                           const int32_t* __restrict__ start,
+                                                           float4* __restrict__ tiles) {
+  __shared__ BrickStage sh;
+  const long long b = blockIdx.x;
+  int s0, s1;
+  if (!brick_range(start, b, sh, s0, s1)) return;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int c0 = s0; c0 < s1; c0 += kBlock) {
+    const int c1 = min(c0 + (int)kBlock, s1);
+    const int i = c0 + threadIdx.x;
+    if (i < c1) {
+      Stencil s;
+      float bb[3];
+      M3 M;
+      g2p_adj_particle<1>(P, n, x, gv, sgv, gx1, gv1, gC1, gx, perm[i], s, bb, M);
+      stage_particle(sh, threadIdx.x, s, bb, M);
+    }
     __syncthreads();
-    tile_flush3(tile, t, nG, ggv, sggv);
+    brick_accumulate(sh, c0, c1, acc);
+    __syncthreads();
+  }
+  if (threadIdx.x < kTileNodes) {
+    acc.w = 0.f;
+    tiles[b * kTileNodes + threadIdx.x] = acc;
   }
 }
 
@@ -1080,30 +1363,40 @@ static void launch_mark(const MpmP& P, const dt_mpm_params* prm, int n, const fl
     hipLaunchKernelGGL(k_mark_bricks, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, flags);
 }
 
+// mark the active bricks and bin the particles of x
+static int launch_mark_bin(const MpmP& P, const dt_mpm_params* prm, int n, const float* x,
+                           const BinWs& ws, hipStream_t st) {
+  launch_mark(P, prm, n, x, ws.flags, st);
+  return launch_bin(P, n, x, ws, st);
+}
+
+// One forward substep.  ws == NULL: dense grid passes and the run scatter;
+// otherwise active-brick passes and the brick-owned binned scatter.
 int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
                        const int32_t* mat_id, const float* x, const float* v, const float* C,
                        const float* F, float* grid4, float* grid_v4, float* x1, float* v1,
-                       float* C1, float* F1, hipStream_t st, int32_t* flags = nullptr) {
+                       float* C1, float* F1, hipStream_t st, const BinWs* ws = nullptr) {
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
-  int bgrid = (int)((n_bricks(prm) + 3) / 4);
-  if (flags) {
-    launch_mark(P, prm, n, x, flags, st);
-    hipLaunchKernelGGL(k_zero4_bricks, dim3(bgrid), dim3(kBlock), 0, st, P, flags,
-                       (float4*)grid4);
+  if (ws) {
+    int bgrid = (int)((ws->n_bricks + 3) / 4);
+    int rc = launch_mark_bin(P, prm, n, x, *ws, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_p2g_binned, dim3((unsigned)ws->n_bricks), dim3(kBlock), 0, st, P, mats,
+                       n, mat_id, x, v, C, F, F1, (const int32_t*)ws->perm,
+                       (const int32_t*)ws->start, ws->tiles);
+    hipLaunchKernelGGL(k_grid_op4_tiles, dim3(bgrid), dim3(kBlock), 0, st, P,
+                       (const int32_t*)ws->flags, (const float4*)ws->tiles,
+                       (const int32_t*)ws->start, (float4*)grid4, (float4*)grid_v4);
   } else {
     hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)grid4);
-  }
-  Grid g{grid4, grid4 + 3, 4, 4};
-  if (n > 0)
-    hipLaunchKernelGGL(k_p2g_fused, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n, mat_id, x,
-                       v, C, F, g, F1);
-  if (flags)
-    hipLaunchKernelGGL(k_grid_op4_bricks, dim3(bgrid), dim3(kBlock), 0, st, P, flags,
-                       (const float4*)grid4, (const float*)nullptr, (float4*)grid_v4);
-  else
+    Grid g{grid4, grid4 + 3, 4, 4};
+    if (n > 0)
+      hipLaunchKernelGGL(k_p2g_fused, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n, mat_id,
+                         x, v, C, F, g, F1);
     hipLaunchKernelGGL(k_grid_op4, dim3(nblocks(nn)), dim3(kBlock), 0, st, P, nn,
                        (const float4*)grid4, (const float*)nullptr, (float4*)grid_v4);
+  }
   if (n > 0)
     hipLaunchKernelGGL(k_g2p, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x,
                        (const float*)grid_v4, 4, x1, v1, C1);
@@ -1115,30 +1408,47 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
                        const float* F, const float* grid4, const float* grid_v4,
                        const float* gx1, const float* gv1, const float* gC1, const float* gF1,
                        float* ggrid4, float* ggv4, float* gx, float* gv, float* gC, float* gF,
-                       hipStream_t st, int32_t* flags = nullptr) {
+                       hipStream_t st, const BinWs* ws = nullptr) {
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
-  int bgrid = (int)((n_bricks(prm) + 3) / 4);
-  if (flags) {
-    launch_mark(P, prm, n, x, flags, st);
-    hipLaunchKernelGGL(k_zero4_bricks, dim3(bgrid), dim3(kBlock), 0, st, P, flags, (float4*)ggv4);
+  if (ws) {
+    int bgrid = (int)((ws->n_bricks + 3) / 4);
+    int rc = launch_mark_bin(P, prm, n, x, *ws, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_g2p_adj_binned, dim3((unsigned)ws->n_bricks), dim3(kBlock), 0, st, P, n,
+                       x, grid_v4, 4, gx1, gv1, gC1, gx, (const int32_t*)ws->perm,
+                       (const int32_t*)ws->start, ws->tiles);
+    hipLaunchKernelGGL(k_grid_op4_adj_tiles, dim3(bgrid), dim3(kBlock), 0, st, P,
+                       (const int32_t*)ws->flags, (const float4*)grid4, (const float4*)ws->tiles,
+                       (const int32_t*)ws->start, (float4*)ggv4, (float4*)ggrid4);
   } else {
     hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)ggv4);
-  }
-  if (n > 0)
-    hipLaunchKernelGGL(k_g2p_adj<1>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, grid_v4, 4,
-                       gx1, gv1, gC1, gx, ggv4, 4);
-  if (flags)
-    hipLaunchKernelGGL(k_grid_op4_adj_bricks, dim3(bgrid), dim3(kBlock), 0, st, P, flags,
-                       (const float4*)grid4, (const float4*)ggv4, (float4*)ggrid4);
-  else
+    if (n > 0)
+      hipLaunchKernelGGL(k_g2p_adj<1>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, grid_v4, 4,
+                         gx1, gv1, gC1, gx, ggv4, 4);
     hipLaunchKernelGGL(k_grid_op4_adj, dim3(nblocks(nn)), dim3(kBlock), 0, st, P, nn,
                        (const float4*)grid4, (const float*)nullptr, (const float4*)ggv4,
                        (float4*)ggrid4, (float*)nullptr);
+  }
   if (n > 0)
     hipLaunchKernelGGL(k_p2g_fused_adj, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n, mat_id,
                        x, v, C, F, (const float*)ggrid4, gF1, gx, gv, gC, gF);
   return check_launch("dt_mpm_substep_adj");
+}
+
+// Workspace views for n particles; error when the buffer is missing or short.
+static int workspace_views(const dt_mpm_params* prm, int n, void* workspace, size_t bytes,
+                           BinWs& ws) {
+  const size_t need = bin_ws_layout(prm, n, nullptr, nullptr);
+  DT_REQUIRE(workspace != nullptr, "NULL workspace");
+  if (bytes < need) {
+    set_error("workspace too small: " + std::to_string(bytes) + " bytes given, " +
+              std::to_string(need) + " needed (dt_mpm_workspace_bytes)");
+    return DT_EINVAL;
+  }
+  DT_REQUIRE(((uintptr_t)workspace & 255) == 0, "workspace must be 256-byte aligned");
+  bin_ws_layout(prm, n, workspace, &ws);
+  return DT_OK;
 }
 
 }  // namespace dt
@@ -1325,11 +1635,14 @@ int dt_mpm_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int32_
 int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, float* x, float* v, float* C, float* F,
                    int32_t n_substeps, float* grids4, float* grids_v4, float* scratch4,
-                   float* scratch_v4, int32_t* brick_flags, void* stream) {
+                   float* scratch_v4, void* workspace, size_t workspace_bytes, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
   DT_REQUIRE(n >= 0 && n_substeps >= 0, "n and n_substeps must be >= 0");
+  BinWs ws{};
+  if (workspace && (rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
+  const BinWs* wp = workspace ? &ws : nullptr;
   bool keep = grids4 != nullptr && grids_v4 != nullptr;
   DT_REQUIRE(keep || (scratch4 && scratch_v4), "need kept grids or scratch grids");
   long long nn = grid_nodes(prm);
@@ -1339,7 +1652,7 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     float* gv4 = keep ? grids_v4 + (long long)s * nn * 4 : scratch_v4;
     rc = launch_substep_fwd(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o, g4,
                             gv4, x + 3 * o1, v + 3 * o1, C + 9 * o1, F + 9 * o1,
-                            DT_STREAM(stream), brick_flags);
+                            DT_STREAM(stream), wp);
     if (rc) return rc;
   }
   return DT_OK;
@@ -1349,12 +1662,15 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                    const int32_t* mat_id, const float* x, const float* v, const float* C,
                    const float* F, int32_t n_substeps, const float* grids4, const float* grids_v4,
                    float* g_grid4, float* g_grid_v4, float* gx, float* gv, float* gC, float* gF,
-                   float* gx2, float* gv2, float* gC2, float* gF2, int32_t* brick_flags,
-                   void* stream) {
+                   float* gx2, float* gv2, float* gC2, float* gF2, void* workspace,
+                   size_t workspace_bytes, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
   DT_REQUIRE(n >= 0 && n_substeps >= 0, "n and n_substeps must be >= 0");
+  BinWs ws{};
+  if (workspace && (rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
+  const BinWs* wp = workspace ? &ws : nullptr;
   DT_REQUIRE(grids4 && grids_v4 && g_grid4 && g_grid_v4, "NULL grid pointer");
   long long nn = grid_nodes(prm);
   hipStream_t st = DT_STREAM(stream);
@@ -1364,7 +1680,7 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     rc = launch_substep_adj(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
                             grids4 + (long long)s * nn * 4, grids_v4 + (long long)s * nn * 4, a[0],
                             a[1], a[2], a[3], g_grid4, g_grid_v4, b[0], b[1], b[2], b[3], st,
-                            brick_flags);
+                            wp);
     if (rc) return rc;
     for (int q = 0; q < 4; ++q) { float* t = a[q]; a[q] = b[q]; b[q] = t; }
   }
@@ -1381,18 +1697,21 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
 int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, float* x, float* v, float* C, float* F, float* x2,
                    float* v2, float* C2, float* F2, int32_t n_substeps, float* scratch4,
-                   float* scratch_v4, int32_t* brick_flags, void* stream) {
+                   float* scratch_v4, void* workspace, size_t workspace_bytes, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
   DT_REQUIRE(n >= 0 && n_substeps >= 0, "n and n_substeps must be >= 0");
   DT_REQUIRE(scratch4 && scratch_v4, "NULL grid scratch");
+  BinWs ws{};
+  if (workspace && (rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
+  const BinWs* wp = workspace ? &ws : nullptr;
   DT_REQUIRE(n == 0 || (x && v && C && F && x2 && v2 && C2 && F2), "NULL pointer");
   hipStream_t st = DT_STREAM(stream);
   float *a[4] = {x, v, C, F}, *b[4] = {x2, v2, C2, F2};
   for (int s = 0; s < n_substeps; ++s) {
     rc = launch_substep_fwd(prm, mats, n, mat_id, a[0], a[1], a[2], a[3], scratch4, scratch_v4,
-                            b[0], b[1], b[2], b[3], st, brick_flags);
+                            b[0], b[1], b[2], b[3], st, wp);
     if (rc) return rc;
     for (int q = 0; q < 4; ++q) { float* t = a[q]; a[q] = b[q]; b[q] = t; }
   }
@@ -1533,6 +1852,100 @@ int dt_mpm_grid_op4_adj_bricks(const dt_mpm_params* prm, const int32_t* flags, c
 int64_t dt_mpm_brick_count(const dt_mpm_params* prm) {
   if (!prm || prm->n_grid < 3) return 0;
   return n_bricks(prm);
+}
+
+int dt_mpm_workspace_bytes(const dt_mpm_params* prm, int32_t n, size_t* bytes) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n >= 0, "n must be >= 0");
+  DT_REQUIRE(bytes != nullptr, "NULL bytes");
+  *bytes = bin_ws_layout(prm, n, nullptr, nullptr);
+  return DT_OK;
+}
+
+int dt_mpm_bin(const dt_mpm_params* prm, int32_t n, const float* x, void* workspace,
+               size_t workspace_bytes, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n >= 0, "n must be >= 0");
+  DT_REQUIRE(n == 0 || x, "NULL x");
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
+  return launch_mark_bin(to_dev(*prm), prm, n, x, ws, DT_STREAM(stream));
+}
+
+int dt_mpm_bin_perm(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                    size_t workspace_bytes, const int32_t** perm, const int32_t** cell_start) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
+  if (perm) *perm = ws.perm;
+  if (cell_start) *cell_start = ws.start;
+  return DT_OK;
+}
+
+int dt_mpm_p2g_binned(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
+                      const int32_t* mat_id, const float* x, const float* v, const float* C,
+                      const float* F, float* F_next, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  if ((rc = check_mats(mats, n_mats))) return rc;
+  DT_REQUIRE(n >= 0, "n must be >= 0");
+  DT_REQUIRE(n == 0 || (x && v && C && F && F_next), "NULL pointer");
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
+  hipLaunchKernelGGL(k_p2g_binned, dim3((unsigned)ws.n_bricks), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev(*prm), mats, n, mat_id, x, v, C, F, F_next,
+                     (const int32_t*)ws.perm, (const int32_t*)ws.start, ws.tiles);
+  return check_launch("dt_mpm_p2g_binned");
+}
+
+int dt_mpm_grid_op4_tiles(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                          size_t workspace_bytes, float* grid4, float* grid_v4, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(grid4 && grid_v4, "NULL grid pointer");
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
+  hipLaunchKernelGGL(k_grid_op4_tiles, dim3((unsigned)((ws.n_bricks + 3) / 4)), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.flags,
+                     (const float4*)ws.tiles, (const int32_t*)ws.start, (float4*)grid4,
+                     (float4*)grid_v4);
+  return check_launch("dt_mpm_grid_op4_tiles");
+}
+
+int dt_mpm_g2p4_adj_binned(const dt_mpm_params* prm, int32_t n, const float* x,
+                           const float* grid_v4, const float* gx_next, const float* gv_next,
+                           const float* gC_next, float* gx, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n >= 0, "n must be >= 0");
+  DT_REQUIRE(grid_v4, "NULL grid pointer");
+  DT_REQUIRE(n == 0 || (x && gx_next && gv_next && gC_next && gx), "NULL pointer");
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
+  hipLaunchKernelGGL(k_g2p_adj_binned, dim3((unsigned)ws.n_bricks), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev(*prm), n, x, grid_v4, 4, gx_next, gv_next,
+                     gC_next, gx, (const int32_t*)ws.perm, (const int32_t*)ws.start, ws.tiles);
+  return check_launch("dt_mpm_g2p4_adj_binned");
+}
+
+int dt_mpm_grid_op4_adj_tiles(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                              size_t workspace_bytes, const float* grid4, float* g_grid_v4,
+                              float* g_grid4, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(grid4 && g_grid_v4 && g_grid4, "NULL grid pointer");
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
+  hipLaunchKernelGGL(k_grid_op4_adj_tiles, dim3((unsigned)((ws.n_bricks + 3) / 4)), dim3(kBlock),
+                     0, DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.flags,
+                     (const float4*)grid4, (const float4*)ws.tiles, (const int32_t*)ws.start,
+                     (float4*)g_grid_v4, (float4*)g_grid4);
+  return check_launch("dt_mpm_grid_op4_adj_tiles");
 }
 
 int dt_mpm_mark_bricks(const dt_mpm_params* prm, int32_t n, const float* x, int32_t* flags,

@@ -12,10 +12,19 @@ Here the same structure lives entirely in HBM:
 Particles are ordered by cell key (a stable integer radix sort) at the start
 of each call; the order is undone on output so callers see their own order.
 """
+import ctypes
+
 import torch
 
 from . import _lib
 from .params import material_table
+
+
+def workspace_bytes(c_prm, n):
+    """Device workspace bytes of the sparse path for n particles."""
+    out = ctypes.c_size_t(0)
+    _lib.check(_lib.lib().dt_mpm_workspace_bytes(c_prm, int(n), ctypes.byref(out)))
+    return int(out.value)
 
 
 class MPMSim:
@@ -59,12 +68,13 @@ class MPMSim:
         nn = self.prm.n_nodes
         return self._buf("g4", (nn, 4)), self._buf("gv4", (nn, 4))
 
-    def _flags(self):
-        """Active-brick flags for the sparse grid passes (None = dense passes)."""
+    def _workspace(self):
+        """(pointer, bytes) of the sparse-path workspace; (None, 0) selects the
+        dense grid passes."""
         if not self.sparse_grid:
-            return None
-        nb = int(_lib.lib().dt_mpm_brick_count(self.c_prm))
-        return _lib.ptr(self._buf("bflags", (nb,), torch.int32), torch.int32)
+            return None, 0
+        nbytes = workspace_bytes(self.c_prm, self.n)
+        return _lib.ptr(self._buf("bin_ws", (nbytes,), torch.uint8), torch.uint8), nbytes
 
     # -- ordering --------------------------------------------------------
     def sort_order(self, x):
@@ -123,7 +133,7 @@ class MPMSim:
             g4p, gv4p = None, None
         _lib.call("dt_mpm_run_fwd", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
                   self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
-                  g4p, gv4p, _lib.ptr(s4), _lib.ptr(sv4), self._flags(), _lib.stream())
+                  g4p, gv4p, _lib.ptr(s4), _lib.ptr(sv4), *self._workspace(), _lib.stream())
         return G4, GV4
 
     def advance(self, state, k, mat_id):
@@ -132,7 +142,7 @@ class MPMSim:
         tmp = [self._buf("adv%d" % i, t.shape) for i, t in enumerate(state)]
         _lib.call("dt_mpm_advance", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
                   self._mid(mat_id), *[_lib.ptr(t) for t in state], *[_lib.ptr(t) for t in tmp],
-                  int(k), _lib.ptr(s4), _lib.ptr(sv4), self._flags(), _lib.stream())
+                  int(k), _lib.ptr(s4), _lib.ptr(sv4), *self._workspace(), _lib.stream())
 
     def adj_frames(self, X, V, C, F, k, mat_id, G4, GV4, g):
         """Reverse sweep over k substeps; g = [gx, gv, gC, gF] seeded at frame k, in place."""
@@ -143,7 +153,7 @@ class MPMSim:
         _lib.call("dt_mpm_run_adj", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
                   self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
                   _lib.ptr(G4), _lib.ptr(GV4), _lib.ptr(gg4), _lib.ptr(ggv4),
-                  *[_lib.ptr(t) for t in g], *[_lib.ptr(t) for t in scratch], self._flags(),
+                  *[_lib.ptr(t) for t in g], *[_lib.ptr(t) for t in scratch], *self._workspace(),
                   _lib.stream())
 
     def _frames(self, k, prefix="F"):

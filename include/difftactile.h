@@ -22,6 +22,7 @@
 #ifndef DIFFTACTILE_H
 #define DIFFTACTILE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -198,11 +199,16 @@ int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int3
    frames [0, n_substeps] and, when grids4 / grids_v4 are non-NULL, keeps
    each substep's grids (n^3 float4 per substep) for dt_mpm_run_adj;
    otherwise grids4 / grids_v4 must be passed as single-substep scratch in
-   scratch4 / scratch_v4. */
+   scratch4 / scratch_v4.  workspace (dt_mpm_workspace_bytes(prm, n) bytes,
+   256-byte aligned) selects the sparse path: each substep bins the particles
+   by cell, visits only the 4x4x4-node bricks some stencil touches, and
+   scatters with the brick-owned tile passes; grid nodes outside those bricks
+   are neither written nor read.  NULL keeps the dense passes over all n^3
+   nodes. */
 int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, float* x, float* v, float* C, float* F,
                    int32_t n_substeps, float* grids4, float* grids_v4, float* scratch4,
-                   float* scratch_v4, int32_t* brick_flags, void* stream);
+                   float* scratch_v4, void* workspace, size_t workspace_bytes, void* stream);
 /* Reverse sweep over frames [0, n_substeps] stored by dt_mpm_run_fwd with
    grids kept.  Seeds: gx..gF hold the adjoint of frame n_substeps on entry and
    the adjoint of frame 0 on return; gx2..gF2 are same-size scratch. */
@@ -210,8 +216,8 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                    const int32_t* mat_id, const float* x, const float* v, const float* C,
                    const float* F, int32_t n_substeps, const float* grids4, const float* grids_v4,
                    float* g_grid4, float* g_grid_v4, float* gx, float* gv, float* gC, float* gF,
-                   float* gx2, float* gv2, float* gC2, float* gF2, int32_t* brick_flags,
-                   void* stream);
+                   float* gx2, float* gv2, float* gC2, float* gF2, void* workspace,
+                   size_t workspace_bytes, void* stream);
 
 /* Advances the state (x, v, C, F) by n_substeps in place, ping-ponging with
    the same-size scratch state (x2, v2, C2, F2); grids are single-substep
@@ -220,13 +226,49 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
 int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, float* x, float* v, float* C, float* F, float* x2,
                    float* v2, float* C2, float* F2, int32_t n_substeps, float* scratch4,
-                   float* scratch_v4, int32_t* brick_flags, void* stream);
+                   float* scratch_v4, void* workspace, size_t workspace_bytes, void* stream);
 
-/* Sparse grid passes.  brick_flags (int32[dt_mpm_brick_count]) non-NULL makes
-   dt_mpm_run_fwd / dt_mpm_run_adj / dt_mpm_advance zero and update only the
-   4x4x4-node bricks some particle stencil touches in that substep (marked by
-   dt_mpm_mark_bricks); grid nodes outside them are neither written nor read.
-   NULL keeps the dense passes over all n^3 nodes. */
+/* ---- sparse path: workspace, binning and brick-owned scatter passes ----
+   (no reference counterpart: the reference scatters with per-particle
+   atomics into a dense grid, mpm_elastic.py:175-208; these stages give the
+   same grids) */
+
+/* Bytes of device workspace the sparse path needs for n particles: brick
+   flags, per-cell counts and starts, per-particle key / rank / permutation,
+   one 6x6x6-node float4 tile per brick, scan scratch. */
+int dt_mpm_workspace_bytes(const dt_mpm_params* prm, int32_t n, size_t* bytes);
+/* Marks the active bricks of x and bins the particles by stencil-base cell
+   (cells brick-major as in dt_mpm_cell_keys, but clamped to the stencil's
+   [0, n_grid-3] range). */
+int dt_mpm_bin(const dt_mpm_params* prm, int32_t n, const float* x, void* workspace,
+               size_t workspace_bytes, void* stream);
+/* Device pointers into the workspace: perm (n) and cell_start (n_cells + 1)
+   of the last dt_mpm_bin. */
+int dt_mpm_bin_perm(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                    size_t workspace_bytes, const int32_t** perm, const int32_t** cell_start);
+/* Brick-owned fused p2g over the binned particles: writes F_next and the
+   per-brick tiles in the workspace (after dt_mpm_bin on the same x). */
+int dt_mpm_p2g_binned(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
+                      const int32_t* mat_id, const float* x, const float* v, const float* C,
+                      const float* F, float* F_next, void* workspace, size_t workspace_bytes,
+                      void* stream);
+/* Sums the p2g tiles into grid4 (mv, m) on the active bricks and applies
+   grid_op there (grid_v4). */
+int dt_mpm_grid_op4_tiles(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                          size_t workspace_bytes, float* grid4, float* grid_v4, void* stream);
+/* Brick-owned g2p adjoint over the binned particles: OVERWRITES gx with its
+   partial and writes the grid-velocity adjoint tiles. */
+int dt_mpm_g2p4_adj_binned(const dt_mpm_params* prm, int32_t n, const float* x,
+                           const float* grid_v4, const float* gx_next, const float* gv_next,
+                           const float* gC_next, float* gx, void* workspace,
+                           size_t workspace_bytes, void* stream);
+/* Sums the adjoint tiles into g_grid_v4 on the active bricks and applies the
+   grid_op adjoint there (g_grid4). */
+int dt_mpm_grid_op4_adj_tiles(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                              size_t workspace_bytes, const float* grid4, float* g_grid_v4,
+                              float* g_grid4, void* stream);
+
+/* Active-brick passes on explicit flags. */
 int64_t dt_mpm_brick_count(const dt_mpm_params* prm);
 /* flags[b] = 1 for every brick touched by a stencil of the particles at x
    (flags cleared first). */

@@ -119,8 +119,18 @@ def cell_keys_np(x, inv_dx, n_grid):
     The reference processes particles in index order and has no key; this
     rule is the build's own and is stated here as the parity target.
     """
-    n = int(n_grid)
-    b = np.clip(cell_base_np(x, inv_dx).astype(np.int64), 0, n - 1)
+    return _brick_keys(cell_base_np(x, inv_dx), int(n_grid), int(n_grid) - 1)
+
+
+def bin_keys_np(x, inv_dx, n_grid):
+    """Binning key of the sparse scatter: as ``cell_keys_np`` but with the base
+    clamped to [0, n-3], the range the 3x3x3 stencil is clamped to, so the key
+    names the cell whose stencil the particle scatters through."""
+    return _brick_keys(cell_base_np(x, inv_dx), int(n_grid), int(n_grid) - 3)
+
+
+def _brick_keys(base, n, hi):
+    b = np.clip(base.astype(np.int64), 0, hi)
     nb = (n + 3) // 4
     brick = ((b[:, 0] >> 2) * nb + (b[:, 1] >> 2)) * nb + (b[:, 2] >> 2)
     local = ((b[:, 0] & 3) * 4 + (b[:, 1] & 3)) * 4 + (b[:, 2] & 3)

@@ -387,3 +387,114 @@ def test_full_size_mass_and_momentum_conservation(gpu):
     mom = prm.p_mass * state[1].double().sum(0).cpu()
     assert (gsum[:3] - mom).abs().max().item() / mom.abs().max().item() < 1e-4
     assert torch.isfinite(outs[0]).all() and torch.isfinite(outs[3]).all()
+
+
+# ---------------------------------------------------------------------------
+# sparse path: per-substep binning and the brick-owned scatter
+# ---------------------------------------------------------------------------
+def _workspace(c_prm, n):
+    from difftactile_amd.sim import workspace_bytes
+    nbytes = workspace_bytes(c_prm, n)
+    return torch.zeros(nbytes, dtype=torch.uint8, device="cuda"), nbytes
+
+
+def _bin_arrays(c_prm, n, ws, nbytes, n_cells):
+    import ctypes
+    pp, cp = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(_lib.lib().dt_mpm_bin_perm(c_prm, n, ws.data_ptr(), nbytes, ctypes.byref(pp),
+                                          ctypes.byref(cp)))
+    o_p = pp.value - ws.data_ptr()
+    o_c = cp.value - ws.data_ptr()
+    perm = ws[o_p:o_p + 4 * n].view(torch.int32).cpu().numpy()
+    start = ws[o_c:o_c + 4 * (n_cells + 1)].view(torch.int32).cpu().numpy()
+    return perm, start
+
+
+@pytest.mark.parametrize("n_grid,n,sort", [(32, 20000, False), (64, 50000, True), (30, 7000, False)])
+def test_binning_is_a_counting_sort_by_stencil_cell(gpu, n_grid, n, sort):
+    prm = default_prm(n_grid)
+    rng = np.random.default_rng(n_grid + n)
+    # the whole box and a margin outside it (clamped bases)
+    x = rng.uniform(-0.05, 1.05, size=(n, 3)).astype(np.float32)
+    keys = om.bin_keys_np(x, prm.inv_dx, n_grid)
+    if sort:
+        x = x[np.argsort(keys, kind="stable")]
+        keys = om.bin_keys_np(x, prm.inv_dx, n_grid)
+    c_prm = prm.c_struct()
+    ws, nbytes = _workspace(c_prm, n)
+    xg = dev(x)
+    call("dt_mpm_bin", c_prm, n, xg, ws, nbytes, _lib.stream())
+    nb = (n_grid + 3) // 4
+    n_cells = nb ** 3 * 64
+    perm, start = _bin_arrays(c_prm, n, ws, nbytes, n_cells)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    assert np.all(np.diff(keys[perm]) >= 0)
+    want = np.searchsorted(np.sort(keys), np.arange(n_cells + 1), side="left")
+    assert np.array_equal(start, want)
+
+
+def test_binning_of_no_particles(gpu):
+    prm = default_prm(32)
+    c_prm = prm.c_struct()
+    ws, nbytes = _workspace(c_prm, 0)
+    ws.fill_(7)
+    call("dt_mpm_bin", c_prm, 0, None, ws, nbytes, _lib.stream())
+    _, start = _bin_arrays(c_prm, 0, ws, nbytes, 8 ** 3 * 64)
+    assert not start.any()
+
+
+def test_workspace_too_small_is_rejected(gpu):
+    prm = default_prm(32)
+    c_prm = prm.c_struct()
+    ws, nbytes = _workspace(c_prm, 100)
+    x = torch.rand(100, 3, device="cuda")
+    with pytest.raises(RuntimeError, match="workspace too small"):
+        call("dt_mpm_bin", c_prm, 100, x, ws, nbytes - 256, _lib.stream())
+
+
+@pytest.mark.parametrize("kind,sort", [("multi", False), ("multi", True), ("von_mises", True)])
+def test_sparse_substep_matches_dense_and_oracle(gpu, kind, sort):
+    """One substep forward and adjoint through dt_mpm_run_fwd / dt_mpm_run_adj,
+    sparse (binned, brick tiles) against dense (run scatter) and the oracle."""
+    prm, mats, mat_id, x, v, C, F = _facade_inputs(kind, n=6000, seed=31)
+    if sort:
+        o = np.argsort(om.bin_keys_np(x, prm.inv_dx, prm.n_grid), kind="stable")
+        x, v, C, F, mat_id = x[o], v[o], C[o], F[o], mat_id[o]
+    n, nn = len(x), prm.n_nodes
+    c_prm = prm.c_struct()
+    mt = material_table(mats, "cuda")
+    mid = dev(mat_id, torch.int32)
+    seeds = [_seed(t.shape, 300 + i) for i, t in enumerate((x, v, C, F))]
+    res = {}
+    for mode in ("dense", "sparse"):
+        frames = [torch.zeros((2,) + a.shape, device="cuda") for a in (x, v, C, F)]
+        for fr, a in zip(frames, (x, v, C, F)):
+            fr[0] = dev(a)
+        G4 = torch.zeros(1, nn, 4, device="cuda")
+        GV4 = torch.zeros(1, nn, 4, device="cuda")
+        ws, nbytes = _workspace(c_prm, n) if mode == "sparse" else (None, 0)
+        call("dt_mpm_run_fwd", c_prm, mt, len(mats), n, mid, *frames, 1, G4, GV4, None, None,
+             ws, nbytes, _lib.stream())
+        g = [s.cuda().contiguous() for s in seeds]
+        g2 = [torch.empty_like(t) for t in g]
+        gg4 = torch.zeros(nn, 4, device="cuda")
+        ggv4 = torch.zeros(nn, 4, device="cuda")
+        call("dt_mpm_run_adj", c_prm, mt, len(mats), n, mid, *frames, 1, G4, GV4, gg4, ggv4,
+             *g, *g2, ws, nbytes, _lib.stream())
+        res[mode] = ([f[1] for f in frames], G4[0], g)
+    # sparse vs dense: same arithmetic up to float summation order
+    for a, b in zip(res["sparse"][0], res["dense"][0]):
+        assert rel_l2(host64(a), host64(b)) < 1e-6
+    assert rel_l2(host64(res["sparse"][1]), host64(res["dense"][1])) < 1e-6
+    for a, b in zip(res["sparse"][2], res["dense"][2]):
+        assert rel_l2(host64(a), host64(b)) < 1e-5
+    # sparse vs the FP64 oracle
+    st = [torch.tensor(a, dtype=F64, requires_grad=True) for a in (x, v, C, F)]
+    out = om.substep(oracle_params(prm), tuple(st), oracle_mats(mats, mat_id, F64))
+    for got, want in zip(res["sparse"][0], out):
+        assert rel_l2(host64(got), want.detach()) < 1e-5
+    loss = sum((o_ * s.to(F64)).sum() for o_, s in zip(out, seeds))
+    grads = torch.autograd.grad(loss, st)
+    for name, got, want in zip("xvCF", res["sparse"][2], grads):
+        err = rel_l2(host64(got), want)
+        assert err < 1e-4, (name, err)
