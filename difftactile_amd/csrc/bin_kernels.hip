@@ -25,16 +25,17 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   size_t scan_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
                                    (int32_t*)nullptr, (int)(ncell + 1));
-  const size_t sz[8] = {sizeof(int32_t) * (size_t)nbk,
-                        sizeof(int32_t) * (size_t)(ncell + 1),
+  const size_t sz[9] = {sizeof(int32_t) * (size_t)nbk,
+                        sizeof(int32_t) * (size_t)(ncell + 2),
                         sizeof(int32_t) * (size_t)(ncell + 1),
                         sizeof(int32_t) * (size_t)n,
                         sizeof(int32_t) * (size_t)n,
                         sizeof(int32_t) * (size_t)n,
                         sizeof(float4) * (size_t)nbk * kTileNodes,
-                        scan_bytes};
-  size_t off[8], total = 0;
-  for (int i = 0; i < 8; ++i) {
+                        scan_bytes,
+                        sizeof(int32_t) * (size_t)nbk};
+  size_t off[9], total = 0;
+  for (int i = 0; i < 9; ++i) {
     off[i] = total;
     total += align256(sz[i] ? sz[i] : 1);
   }
@@ -48,6 +49,8 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
     ws->perm = reinterpret_cast<int32_t*>(b + off[5]);
     ws->tiles = reinterpret_cast<float4*>(b + off[6]);
     ws->scan_tmp = b + off[7];
+    ws->list = reinterpret_cast<int32_t*>(b + off[8]);
+    ws->n_list = ws->count + ncell + 1;
     ws->scan_bytes = scan_bytes;
     ws->n_bricks = nbk;
     ws->n_cells = ncell;
@@ -101,9 +104,26 @@ __global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __res
   if (p < n) perm[start[key[p]] + rank[p]] = p;
 }
 
+// list[] gets every brick that holds particles (one atomic per wavefront)
+__global__ __launch_bounds__(kBlock) void k_brick_list(long long n_bricks,
+                                                       const int32_t* __restrict__ start,
+                                                       int32_t* __restrict__ list,
+                                                       int32_t* __restrict__ n_list) {
+  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool occ = b < n_bricks && start[b * 64 + 64] > start[b * 64];
+  const unsigned long long m = __ballot(occ);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(n_list, __popcll(m));
+  base = __shfl(base, leader);
+  if (occ) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)b;
+}
+
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st) {
   const long long nc = ws.n_cells + 1;
-  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nc)), dim3(kBlock), 0, st, nc, ws.count);
+  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nc + 1)), dim3(kBlock), 0, st, nc + 1, ws.count);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
                        ws.rank, ws.count);
@@ -116,6 +136,8 @@ int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
                        ws.start, ws.perm);
+  hipLaunchKernelGGL(k_brick_list, dim3(nblocks(ws.n_bricks)), dim3(kBlock), 0, st, ws.n_bricks,
+                     (const int32_t*)ws.start, ws.list, ws.n_list);
   return check_launch("particle binning");
 }
 

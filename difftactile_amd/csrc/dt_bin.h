@@ -15,7 +15,10 @@ constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
 // Views into one caller-provided workspace buffer.
 struct BinWs {
   int32_t* flags;   // [n_bricks]    bricks some stencil touches
-  int32_t* count;   // [n_cells + 1] particles per cell (last entry stays 0)
+  int32_t* count;   // [n_cells + 2] particles per cell; [n_cells] stays 0,
+                    //   [n_cells + 1] counts the occupied bricks
+  int32_t* n_list;  // = count + n_cells + 1
+  int32_t* list;    // [n_bricks]    occupied bricks (any order)
   int32_t* start;   // [n_cells + 1] exclusive prefix sum of count
   int32_t* key;     // [n]           cell key of each particle
   int32_t* rank;    // [n]           position of the particle inside its cell
@@ -29,7 +32,9 @@ struct BinWs {
 long long brick_total(const dt_mpm_params* prm);
 // Total bytes of the workspace for n particles; fills *ws when base != NULL.
 size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws);
-// count -> scan -> fill on stream st (flags untouched).
+// Persistent grid of the brick-owned passes (blocks stride over the list).
+constexpr int kBrickBlocks = 2048;
+// count -> scan -> fill -> occupied-brick list on stream st (flags untouched).
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st);
 
 }  // namespace dt

@@ -1189,14 +1189,43 @@ __device__ __forceinline__ void brick_accumulate(BrickStage& sh, int c0, int c1,
   brick_gather(sh, c0, c1, acc);
 }
 
-__device__ __forceinline__ bool brick_range(const int32_t* __restrict__ start, long long b,
+__device__ __forceinline__ void brick_range(const int32_t* __restrict__ start, long long b,
                                             BrickStage& sh, int& s0, int& s1) {
   s0 = start[b * 64];
   s1 = start[b * 64 + 64];
-  if (s0 == s1) return false;
   if (threadIdx.x <= 64) sh.cs[threadIdx.x] = start[b * 64 + threadIdx.x];
-  return true;
 }
+
+// Blocks stride over the occupied-brick list (a persistent grid: unoccupied
+// bricks cost nothing).
+#define DT_BRICK_LOOP_BEGIN                                          \
+  __shared__ BrickStage sh;                                          \
+  const int n_occ = *n_list;                                         \
+  for (int li = blockIdx.x; li < n_occ; li += gridDim.x) {           \
+    const long long b = list[li];                                    \
+    int s0, s1;                                                      \
+    brick_range(start, b, sh, s0, s1);                               \
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);                    \
+    for (int c0 = s0; c0 < s1; c0 += kBlock) {                       \
+      const int c1 = min(c0 + (int)kBlock, s1);                      \
+      const int i = c0 + threadIdx.x;                                \
+      if (i < c1) {                                                  \
+        Stencil s;                                                   \
+        float bb[3];                                                 \
+        M3 M;
+
+#define DT_BRICK_LOOP_END(mass_scale)                                \
+        stage_particle(sh, threadIdx.x, s, bb, M);                   \
+      }                                                              \
+      __syncthreads();                                               \
+      brick_accumulate(sh, c0, c1, acc);                             \
+      __syncthreads();                                               \
+    }                                                                \
+    if (threadIdx.x < kTileNodes) {                                  \
+      acc.w *= (mass_scale);                                         \
+      tiles[b * kTileNodes + threadIdx.x] = acc;                     \
+    }                                                                \
+  }
 
 __global__ __launch_bounds__(kBlock) void k_p2g_binned(MpmP P, const dt_material* __restrict__ mats,
                                                        int n, const int32_t* __restrict__ mat_id,
@@ -1207,30 +1236,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g_binned(MpmP P, const dt_material
                                                        float* __restrict__ F_next,
                                                        const int32_t* __restrict__ perm,
                                                        const int32_t* __restrict__ start,
+                                                       const int32_t* __restrict__ list,
+                                                       const int32_t* __restrict__ n_list,
                                                        float4* __restrict__ tiles) {
-  __shared__ BrickStage sh;
-  const long long b = blockIdx.x;
-  int s0, s1;
-  if (!brick_range(start, b, sh, s0, s1)) return;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int c0 = s0; c0 < s1; c0 += kBlock) {
-    const int c1 = min(c0 + (int)kBlock, s1);
-    const int i = c0 + threadIdx.x;
-    if (i < c1) {
-      Stencil s;
-      float bb[3];
-      M3 M;
-      p2g_fused_particle(P, mats, n, mat_id, x, v, C, F, F_next, perm[i], s, bb, M);
-      stage_particle(sh, threadIdx.x, s, bb, M);
-    }
-    __syncthreads();
-    brick_accumulate(sh, c0, c1, acc);
-    __syncthreads();
-  }
-  if (threadIdx.x < kTileNodes) {
-    acc.w *= P.p_mass;
-    tiles[b * kTileNodes + threadIdx.x] = acc;
-  }
+  DT_BRICK_LOOP_BEGIN
+  p2g_fused_particle(P, mats, n, mat_id, x, v, C, F, F_next, perm[i], s, bb, M);
+  DT_BRICK_LOOP_END(P.p_mass)
 }
 
 __global__ __launch_bounds__(kBlock) void k_g2p_adj_binned(MpmP P, int n,
@@ -1242,31 +1253,16 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj_binned(MpmP P, int n,
                                                            float* __restrict__ gx,
                                                            const int32_t* __restrict__ perm,
                                                            const int32_t* __restrict__ start,
+                                                           const int32_t* __restrict__ list,
+                                                           const int32_t* __restrict__ n_list,
                                                            float4* __restrict__ tiles) {
-  __shared__ BrickStage sh;
-  const long long b = blockIdx.x;
-  int s0, s1;
-  if (!brick_range(start, b, sh, s0, s1)) return;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int c0 = s0; c0 < s1; c0 += kBlock) {
-    const int c1 = min(c0 + (int)kBlock, s1);
-    const int i = c0 + threadIdx.x;
-    if (i < c1) {
-      Stencil s;
-      float bb[3];
-      M3 M;
-      g2p_adj_particle<1>(P, n, x, gv, sgv, gx1, gv1, gC1, gx, perm[i], s, bb, M);
-      stage_particle(sh, threadIdx.x, s, bb, M);
-    }
-    __syncthreads();
-    brick_accumulate(sh, c0, c1, acc);
-    __syncthreads();
-  }
-  if (threadIdx.x < kTileNodes) {
-    acc.w = 0.f;
-    tiles[b * kTileNodes + threadIdx.x] = acc;
-  }
+  DT_BRICK_LOOP_BEGIN
+  g2p_adj_particle<1>(P, n, x, gv, sgv, gx1, gv1, gC1, gx, perm[i], s, bb, M);
+  DT_BRICK_LOOP_END(0.f)
 }
+
+#undef DT_BRICK_LOOP_BEGIN
+#undef DT_BRICK_LOOP_END
 
 // ----------------------------------------------------------------------------
 // ordering helpers
@@ -1363,6 +1359,10 @@ static void launch_mark(const MpmP& P, const dt_mpm_params* prm, int n, const fl
     hipLaunchKernelGGL(k_mark_bricks, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, flags);
 }
 
+static unsigned brick_blocks(const BinWs& ws) {
+  return (unsigned)(ws.n_bricks < kBrickBlocks ? ws.n_bricks : kBrickBlocks);
+}
+
 // mark the active bricks and bin the particles of x
 static int launch_mark_bin(const MpmP& P, const dt_mpm_params* prm, int n, const float* x,
                            const BinWs& ws, hipStream_t st) {
@@ -1382,9 +1382,10 @@ int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
     int bgrid = (int)((ws->n_bricks + 3) / 4);
     int rc = launch_mark_bin(P, prm, n, x, *ws, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_p2g_binned, dim3((unsigned)ws->n_bricks), dim3(kBlock), 0, st, P, mats,
-                       n, mat_id, x, v, C, F, F1, (const int32_t*)ws->perm,
-                       (const int32_t*)ws->start, ws->tiles);
+    hipLaunchKernelGGL(k_p2g_binned, dim3(brick_blocks(*ws)), dim3(kBlock), 0, st, P, mats, n,
+                       mat_id, x, v, C, F, F1, (const int32_t*)ws->perm,
+                       (const int32_t*)ws->start, (const int32_t*)ws->list,
+                       (const int32_t*)ws->n_list, ws->tiles);
     hipLaunchKernelGGL(k_grid_op4_tiles, dim3(bgrid), dim3(kBlock), 0, st, P,
                        (const int32_t*)ws->flags, (const float4*)ws->tiles,
                        (const int32_t*)ws->start, (float4*)grid4, (float4*)grid_v4);
@@ -1415,9 +1416,10 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
     int bgrid = (int)((ws->n_bricks + 3) / 4);
     int rc = launch_mark_bin(P, prm, n, x, *ws, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_g2p_adj_binned, dim3((unsigned)ws->n_bricks), dim3(kBlock), 0, st, P, n,
-                       x, grid_v4, 4, gx1, gv1, gC1, gx, (const int32_t*)ws->perm,
-                       (const int32_t*)ws->start, ws->tiles);
+    hipLaunchKernelGGL(k_g2p_adj_binned, dim3(brick_blocks(*ws)), dim3(kBlock), 0, st, P, n, x,
+                       grid_v4, 4, gx1, gv1, gC1, gx, (const int32_t*)ws->perm,
+                       (const int32_t*)ws->start, (const int32_t*)ws->list,
+                       (const int32_t*)ws->n_list, ws->tiles);
     hipLaunchKernelGGL(k_grid_op4_adj_tiles, dim3(bgrid), dim3(kBlock), 0, st, P,
                        (const int32_t*)ws->flags, (const float4*)grid4, (const float4*)ws->tiles,
                        (const int32_t*)ws->start, (float4*)ggv4, (float4*)ggrid4);
@@ -1896,9 +1898,10 @@ int dt_mpm_p2g_binned(const dt_mpm_params* prm, const dt_material* mats, int32_t
   DT_REQUIRE(n == 0 || (x && v && C && F && F_next), "NULL pointer");
   BinWs ws{};
   if ((rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
-  hipLaunchKernelGGL(k_p2g_binned, dim3((unsigned)ws.n_bricks), dim3(kBlock), 0,
-                     DT_STREAM(stream), to_dev(*prm), mats, n, mat_id, x, v, C, F, F_next,
-                     (const int32_t*)ws.perm, (const int32_t*)ws.start, ws.tiles);
+  hipLaunchKernelGGL(k_p2g_binned, dim3(brick_blocks(ws)), dim3(kBlock), 0, DT_STREAM(stream),
+                     to_dev(*prm), mats, n, mat_id, x, v, C, F, F_next, (const int32_t*)ws.perm,
+                     (const int32_t*)ws.start, (const int32_t*)ws.list,
+                     (const int32_t*)ws.n_list, ws.tiles);
   return check_launch("dt_mpm_p2g_binned");
 }
 
@@ -1927,9 +1930,10 @@ int dt_mpm_g2p4_adj_binned(const dt_mpm_params* prm, int32_t n, const float* x,
   DT_REQUIRE(n == 0 || (x && gx_next && gv_next && gC_next && gx), "NULL pointer");
   BinWs ws{};
   if ((rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
-  hipLaunchKernelGGL(k_g2p_adj_binned, dim3((unsigned)ws.n_bricks), dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_g2p_adj_binned, dim3(brick_blocks(ws)), dim3(kBlock), 0,
                      DT_STREAM(stream), to_dev(*prm), n, x, grid_v4, 4, gx_next, gv_next,
-                     gC_next, gx, (const int32_t*)ws.perm, (const int32_t*)ws.start, ws.tiles);
+                     gC_next, gx, (const int32_t*)ws.perm, (const int32_t*)ws.start,
+                     (const int32_t*)ws.list, (const int32_t*)ws.n_list, ws.tiles);
   return check_launch("dt_mpm_g2p4_adj_binned");
 }
 
