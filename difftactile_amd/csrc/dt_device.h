@@ -416,9 +416,9 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
 // and the float32 products they multiply lose every digit there.
 DT_HD double clampd(double a) { return a >= 0.0 ? fmax(a, 1e-8) : fmin(a, -1e-8); }
 
+template <bool kExact>
 DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3& V, const Mat& m,
-                      const StressFwd& st, const M3& gtau, const M3& gFp_in, M3& gFn,
-                      bool exact) {
+                      const StressFwd& st, const M3& gtau, const M3& gFp_in, M3& gFn) {
   // tau adjoint
   M3 D;
   for (int i = 0; i < 9; ++i) D.a[i] = st.Fp.a[i] - st.R.a[i];
@@ -431,12 +431,15 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
   float gJ = m.lam * (2.f * st.J - 1.f) * trg;
 
   double gsig[3] = {0.0, 0.0, 0.0};
-  M3d Ud = m3d_from(U), Vd = m3d_from(V);
-  M3d gRd = m3d_from(gR);
-  // U/V adjoints of the return-map path (F_p = U diag(d) V^T); the rotation
-  // path R = U V^T is kept apart so the exact mode can use its closed form
+  // U/V adjoints of the return-map path (F_p = U diag(d) V^T), reference form
+  // only; the rotation path R = U V^T is kept apart so the exact form can use
+  // its closed form
   M3d gUp, gVp;
-  for (int i = 0; i < 9; ++i) gUp.a[i] = gVp.a[i] = 0.0;
+  if (!kExact)
+    for (int i = 0; i < 9; ++i) gUp.a[i] = gVp.a[i] = 0.0;
+  M3 N = m3_zero();   // U^T gF_p V of the return-map path
+  bool pl = false;
+  double d[3] = {1.0, 1.0, 1.0};
   gFn = m3_zero();
 
   if (m.kind == KIND_ELASTIC) {
@@ -450,19 +453,22 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
     if (st.mode == 0) {
       gFn = gFp;
     } else {
-      double d[3] = {exp((double)st.epsn[0]), exp((double)st.epsn[1]), exp((double)st.epsn[2])};
-      M3d gFpd = m3d_from(gFp);
-      M3d UtgV = m3d_mul(m3d_mul_at(Ud, gFpd), Vd);   // U^T gFp V
-      M3d gFpV = m3d_mul(gFpd, Vd);                   // gFp V
-      M3d gFptU = m3d_mul_at(gFpd, Ud);               // gFp^T U
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-          gUp.a[i * 3 + j] = gFpV.a[i * 3 + j] * d[j];
-          gVp.a[i * 3 + j] = gFptU.a[i * 3 + j] * d[j];
-        }
+      pl = true;
+      for (int k = 0; k < 3; ++k) d[k] = exp((double)st.epsn[k]);
+      N = m3_mul(m3_mul_at(U, gFp), V);
+      if (!kExact) {
+        M3d Ud = m3d_from(U), Vd = m3d_from(V), gFpd = m3d_from(gFp);
+        M3d gFpV = m3d_mul(gFpd, Vd);      // gFp V
+        M3d gFptU = m3d_mul_at(gFpd, Ud);  // gFp^T U
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            gUp.a[i * 3 + j] = gFpV.a[i * 3 + j] * d[j];
+            gVp.a[i * 3 + j] = gFptU.a[i * 3 + j] * d[j];
+          }
+      }
       if (st.mode != 3) {
         double ge[3];   // adjoint of post-map eps
-        for (int i = 0; i < 3; ++i) ge[i] = UtgV.a[i * 4] * d[i];
+        for (int i = 0; i < 3; ++i) ge[i] = (double)N.a[i * 4] * d[i];
         double eh[3] = {st.eh[0], st.eh[1], st.eh[2]};
         double n = st.ehn;
         double gdot = ge[0] * eh[0] + ge[1] * eh[1] + ge[2] * eh[2];
@@ -492,36 +498,27 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
       }
     }
   }
-  // SVD adjoint in float64.
-  //   reference form: svd_grad of gU = gR V + gUp, gV = gR^T U + gVp with the
-  //     1e-8 clamp on s_j^2 - s_i^2 (mpm_elastic.py:157-173);
-  //   exact form, with no small denominators: the rotation path as
+  // SVD adjoint: G = U Mi V^T.
+  //   exact form (matrix products in float32, the per-pair coefficients in
+  //     float64, no small denominators): the rotation path as
   //     (M_ij - M_ji) / (s_i + s_j), M = U^T gR V; the return-map path
   //     (F_p = U diag(d) V^T, d_k = G s_k^a) as N_ij A_ij + N_ji B_ij with
   //     N = U^T gF_p V, A/B = +-(d_i + d_j) / (2 (s_i + s_j)) + Dd_ij / 2 and
-  //     Dd_ij = (d_j - d_i) / (s_j - s_i) evaluated through expm1/log1p.
-  M3d Mi;
-  if (exact) {
-    M3d Mr = m3d_mul(m3d_mul_at(Ud, gRd), Vd);
-    const bool pl = m.kind != KIND_ELASTIC && st.mode != 0;
-    M3d N;
-    double d[3] = {1.0, 1.0, 1.0}, sc[3];
-    if (pl) {
-      N = m3d_mul(m3d_mul_at(Ud, m3d_from(gFp)), Vd);
-      for (int k = 0; k < 3; ++k) {
-        d[k] = exp((double)st.epsn[k]);
-        sc[k] = fmax((double)sig[k], 0.05);
-      }
-    }
+  //     Dd_ij = (d_j - d_i) / (s_j - s_i) evaluated through expm1/log1p;
+  //   reference form (float64): svd_grad of gU = gR V + gUp, gV = gR^T U + gVp
+  //     with the 1e-8 clamp on s_j^2 - s_i^2 (mpm_elastic.py:157-173).
+  M3 Mi;
+  if (kExact) {
+    const M3 Mr = m3_mul(m3_mul_at(U, gR), V);
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
         if (i == j) {
-          Mi.a[i * 3 + j] = gsig[i];
+          Mi.a[i * 3 + j] = (float)gsig[i];
           continue;
         }
         double den = (double)sig[i] + (double)sig[j];
         if (fabs(den) < 1e-12) den = den < 0.0 ? -1e-12 : 1e-12;
-        double v = (Mr.a[i * 3 + j] - Mr.a[j * 3 + i]) / den;
+        double v = ((double)Mr.a[i * 3 + j] - (double)Mr.a[j * 3 + i]) / den;
         if (pl) {
           double si = sig[i], sj = sig[j], del = sj - si, Dd = 0.0;
           double a = st.pa, G = st.pG;
@@ -531,14 +528,15 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
             else
               Dd = G * a * pow(si, a - 1.0);
           } else if (del != 0.0) {
-            Dd = G * (pow(sc[j], a) - pow(sc[i], a)) / del;
+            Dd = G * (pow(fmax(sj, 0.05), a) - pow(fmax(si, 0.05), a)) / del;
           }
           double hs = 0.5 * (d[i] + d[j]) / den;
-          v += N.a[i * 3 + j] * (hs + 0.5 * Dd) + N.a[j * 3 + i] * (-hs + 0.5 * Dd);
+          v += (double)N.a[i * 3 + j] * (hs + 0.5 * Dd) + (double)N.a[j * 3 + i] * (-hs + 0.5 * Dd);
         }
-        Mi.a[i * 3 + j] = v;
+        Mi.a[i * 3 + j] = (float)v;
       }
   } else {
+    M3d Ud = m3d_from(U), Vd = m3d_from(V), gRd = m3d_from(gR);
     double s2[3] = {(double)sig[0] * sig[0], (double)sig[1] * sig[1], (double)sig[2] * sig[2]};
     M3d gU = gUp, gV = gVp;
     M3d a = m3d_mul(gRd, Vd), b = m3d_mul_at(gRd, Ud);
@@ -548,6 +546,7 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
     }
     M3d Xu = m3d_mul_at(Ud, gU);   // U^T gU
     M3d Xv = m3d_mul_at(Vd, gV);   // V^T gV
+    M3d Mid;
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
         double v = 0.0;
@@ -559,11 +558,13 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
         } else {
           v = gsig[i];
         }
-        Mi.a[i * 3 + j] = v;
+        Mid.a[i * 3 + j] = v;
       }
+    M3d Gd = m3d_mul_bt(m3d_mul(Ud, Mid), Vd);
+    for (int i = 0; i < 9; ++i) gFn.a[i] += (float)Gd.a[i];
+    return;
   }
-  M3d G = m3d_mul_bt(m3d_mul(Ud, Mi), Vd);
-  for (int i = 0; i < 9; ++i) gFn.a[i] += (float)G.a[i];
+  const M3 G = m3_mul_bt(m3_mul(U, Mi), V);
+  for (int i = 0; i < 9; ++i) gFn.a[i] += G.a[i];
 }
-
 }  // namespace dt

@@ -565,6 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
 // fused adjoint of (compute_new_F + svd + return map + p2g): writes the
 // complete adjoint of (x, v, C, F); gx_io holds the g2p-adjoint partial on
 // entry.
+template <bool kExact>
 __global__ __launch_bounds__(kBlock) void k_p2g_fused_adj(
     MpmP P, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
     const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ C,
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_fused_adj(
   M3 gtau, lgC;
   affine_adj(P, gA, gtau, lgC);
   M3 gFn;
-  stress_adj(fn, u, sig, vv, m, st, gtau, ld_m3L(gFnext, p, Lm), gFn, P.svd_exact != 0);
+  stress_adj<kExact>(fn, u, sig, vv, m, st, gtau, ld_m3L(gFnext, p, Lm), gFn);
   // F_new = (I + dt C) F
   M3 gc2 = m3_mul_bt(gFn, f);
   for (int i = 0; i < 9; ++i) lgC.a[i] += P.dt * gc2.a[i];
@@ -1359,6 +1360,18 @@ static void launch_mark(const MpmP& P, const dt_mpm_params* prm, int n, const fl
     hipLaunchKernelGGL(k_mark_bricks, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, flags);
 }
 
+static void launch_p2g_fused_adj(const MpmP& P, hipStream_t st, const dt_material* mats, int n,
+                                 const int32_t* mat_id, const float* x, const float* v,
+                                 const float* C, const float* F, const float* ggrid4,
+                                 const float* gF1, float* gx, float* gv, float* gC, float* gF) {
+  if (P.svd_exact)
+    hipLaunchKernelGGL(k_p2g_fused_adj<true>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n,
+                       mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF);
+  else
+    hipLaunchKernelGGL(k_p2g_fused_adj<false>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n,
+                       mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF);
+}
+
 static unsigned brick_blocks(const BinWs& ws) {
   return (unsigned)(ws.n_bricks < kBrickBlocks ? ws.n_bricks : kBrickBlocks);
 }
@@ -1433,8 +1446,7 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
                        (float4*)ggrid4, (float*)nullptr);
   }
   if (n > 0)
-    hipLaunchKernelGGL(k_p2g_fused_adj, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n, mat_id,
-                       x, v, C, F, (const float*)ggrid4, gF1, gx, gv, gC, gF);
+    launch_p2g_fused_adj(P, st, mats, n, mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF);
   return check_launch("dt_mpm_substep_adj");
 }
 
@@ -1811,8 +1823,8 @@ int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int3
   DT_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return DT_OK;
   DT_REQUIRE(x && v && C && F && g_grid4 && gF_next && gx && gv && gC && gF, "NULL pointer");
-  hipLaunchKernelGGL(k_p2g_fused_adj, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream),
-                     to_dev(*prm), mats, n, mat_id, x, v, C, F, g_grid4, gF_next, gx, gv, gC, gF);
+  launch_p2g_fused_adj(to_dev(*prm), DT_STREAM(stream), mats, n, mat_id, x, v, C, F, g_grid4,
+                       gF_next, gx, gv, gC, gF);
   return check_launch("dt_mpm_p2g_fused_adj");
 }
 
