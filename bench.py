@@ -301,6 +301,7 @@ def main():
           torch.zeros(n, 3, 3, device=dev), torch.tensor(scenes.identity_F(n), device=dev)]
     w = torch.randn(n, 3, device=dev, generator=torch.Generator(device=dev).manual_seed(1))
     T = args.segment
+    resident = sim.segment_bytes(T) <= sim.resident_bytes
 
     def step():
         leaves = [t.detach().requires_grad_(True) for t in st]
@@ -356,8 +357,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"multi-material sphere (elastic core E=1e7 + Drucker-Prager "
                                f"shell), {n} particles, {prm.n_grid}^3 grid, "
-                               f"{T}-substep checkpointed rollout fwd+bwd per step",
+                               f"{T}-substep rollout fwd+bwd per step"
+                               + (" (segment frames and grids kept in HBM for the adjoint "
+                                  "sweep)" if resident else " (segment recomputed from its "
+                                  "checkpoint in the backward pass)"),
                    "particles": n, "n_grid": prm.n_grid, "substeps_per_step": T,
+                   "segment_resident": resident,
                    "active_nodes": active},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},

@@ -2,8 +2,11 @@
 //
 // k_bin_count gives every particle its rank inside its cell with one global
 // atomic per run of equal keys in a wavefront (cell-sorted particles make a
-// run of every cell's particles in the wave), an exclusive scan turns the
-// counts into cell starts, and k_bin_fill writes the permutation.  The order
+// run of every cell's particles in the wave).  Cell starts come from a
+// two-level scan: one wavefront per brick scans its 64 cell counts, a device
+// scan runs over the brick totals, and k_bin_starts adds the two (and derives
+// the brick flags and the occupied-brick list from the brick totals).
+// k_bin_fill writes the permutation.  The order
 // of particles inside a cell follows the atomics and is not deterministic
 // across waves; nothing downstream depends on it beyond float summation order.
 #include <hipcub/hipcub.hpp>
@@ -24,8 +27,8 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   const long long nbk = brick_total(prm), ncell = nbk * 64;
   size_t scan_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
-                                   (int32_t*)nullptr, (int)(ncell + 1));
-  const size_t sz[9] = {sizeof(int32_t) * (size_t)nbk,
+                                   (int32_t*)nullptr, (int)(nbk + 1));
+  const size_t sz[11] = {sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t) * (size_t)(ncell + 2),
                         sizeof(int32_t) * (size_t)(ncell + 1),
                         sizeof(int32_t) * (size_t)n,
@@ -33,9 +36,11 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
                         sizeof(int32_t) * (size_t)n,
                         sizeof(float4) * (size_t)nbk * kTileNodes,
                         scan_bytes,
-                        sizeof(int32_t) * (size_t)nbk};
-  size_t off[9], total = 0;
-  for (int i = 0; i < 9; ++i) {
+                        sizeof(int32_t) * (size_t)nbk,
+                        sizeof(int32_t) * (size_t)(nbk + 1),
+                        sizeof(int32_t) * (size_t)(nbk + 1)};
+  size_t off[11], total = 0;
+  for (int i = 0; i < 11; ++i) {
     off[i] = total;
     total += align256(sz[i] ? sz[i] : 1);
   }
@@ -50,6 +55,8 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
     ws->tiles = reinterpret_cast<float4*>(b + off[6]);
     ws->scan_tmp = b + off[7];
     ws->list = reinterpret_cast<int32_t*>(b + off[8]);
+    ws->brick_total = reinterpret_cast<int32_t*>(b + off[9]);
+    ws->brick_start = reinterpret_cast<int32_t*>(b + off[10]);
     ws->n_list = ws->count + ncell + 1;
     ws->scan_bytes = scan_bytes;
     ws->n_bricks = nbk;
@@ -104,40 +111,76 @@ __global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __res
   if (p < n) perm[start[key[p]] + rank[p]] = p;
 }
 
-// list[] gets every brick that holds particles (one atomic per wavefront)
-__global__ __launch_bounds__(kBlock) void k_brick_list(long long n_bricks,
-                                                       const int32_t* __restrict__ start,
+// One wavefront per brick: count[c] -> exclusive prefix inside the brick,
+// brick_total[b] = particle count of the brick.
+__global__ __launch_bounds__(kBlock) void k_cell_scan(long long n_bricks, int32_t* __restrict__ count,
+                                                      int32_t* __restrict__ brick_total) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long b = c >> 6;
+  if (b >= n_bricks) return;   // whole wavefronts
+  const int lane = threadIdx.x & 63;
+  const int v = count[c];
+  int incl = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(incl, d);
+    if (lane >= d) incl += u;
+  }
+  count[c] = incl - v;
+  if (lane == 63) brick_total[b] = incl;
+  if (b == 0 && lane == 0) brick_total[n_bricks] = 0;
+}
+
+__device__ __forceinline__ bool brick_occupied(const int32_t* __restrict__ total, int nb, int i,
+                                               int j, int k) {
+  return i >= 0 && j >= 0 && k >= 0 && total[((long long)i * nb + j) * nb + k] > 0;
+}
+
+// start[c] = brick_start[brick] + prefix; per brick (lane 0 of its wavefront):
+// flags = brick or one of its -1 neighbours holds particles (a stencil reaches
+// at most one brick past its base's), list += occupied bricks.
+__global__ __launch_bounds__(kBlock) void k_bin_starts(long long n_bricks, int nb,
+                                                       const int32_t* __restrict__ prefix,
+                                                       const int32_t* __restrict__ brick_total,
+                                                       const int32_t* __restrict__ brick_start,
+                                                       int32_t* __restrict__ start,
+                                                       int32_t* __restrict__ flags,
                                                        int32_t* __restrict__ list,
                                                        int32_t* __restrict__ n_list) {
-  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool occ = b < n_bricks && start[b * 64 + 64] > start[b * 64];
-  const unsigned long long m = __ballot(occ);
-  if (m == 0ull) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((long long)m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(n_list, __popcll(m));
-  base = __shfl(base, leader);
-  if (occ) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)b;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long b = c >> 6;
+  if (b >= n_bricks) return;   // whole wavefronts
+  start[c] = brick_start[b] + prefix[c];
+  if (c == 0) start[n_bricks * 64] = brick_start[n_bricks];
+  if ((threadIdx.x & 63) != 0) return;
+  const int bk = (int)(b % nb), bj = (int)((b / nb) % nb), bi = (int)(b / ((long long)nb * nb));
+  bool any = false;
+  for (int d = 0; d < 8; ++d)
+    any = any || brick_occupied(brick_total, nb, bi - (d >> 2), bj - ((d >> 1) & 1), bk - (d & 1));
+  flags[b] = any ? 1 : 0;
+  if (brick_total[b] > 0) list[atomicAdd(n_list, 1)] = (int32_t)b;
 }
 
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st) {
-  const long long nc = ws.n_cells + 1;
-  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nc + 1)), dim3(kBlock), 0, st, nc + 1, ws.count);
+  const long long nc = ws.n_cells;
+  const int nb = (P.n_grid + 3) >> 2;
+  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nc + 2)), dim3(kBlock), 0, st, nc + 2, ws.count);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
                        ws.rank, ws.count);
+  hipLaunchKernelGGL(k_cell_scan, dim3(nblocks(nc)), dim3(kBlock), 0, st, ws.n_bricks, ws.count,
+                     ws.brick_total);
   size_t bytes = ws.scan_bytes;
-  if (hipcub::DeviceScan::ExclusiveSum(ws.scan_tmp, bytes, ws.count, ws.start, (int)nc, st) !=
-      hipSuccess) {
+  if (hipcub::DeviceScan::ExclusiveSum(ws.scan_tmp, bytes, ws.brick_total, ws.brick_start,
+                                       (int)(ws.n_bricks + 1), st) != hipSuccess) {
     set_error("binning scan failed");
     return DT_ELAUNCH;
   }
+  hipLaunchKernelGGL(k_bin_starts, dim3(nblocks(nc)), dim3(kBlock), 0, st, ws.n_bricks, nb,
+                     (const int32_t*)ws.count, (const int32_t*)ws.brick_total,
+                     (const int32_t*)ws.brick_start, ws.start, ws.flags, ws.list, ws.n_list);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
                        ws.start, ws.perm);
-  hipLaunchKernelGGL(k_brick_list, dim3(nblocks(ws.n_bricks)), dim3(kBlock), 0, st, ws.n_bricks,
-                     (const int32_t*)ws.start, ws.list, ws.n_list);
   return check_launch("particle binning");
 }
 

@@ -14,9 +14,13 @@ constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
 
 // Views into one caller-provided workspace buffer.
 struct BinWs {
-  int32_t* flags;   // [n_bricks]    bricks some stencil touches
-  int32_t* count;   // [n_cells + 2] particles per cell; [n_cells] stays 0,
-                    //   [n_cells + 1] counts the occupied bricks
+  int32_t* flags;   // [n_bricks]    bricks some stencil may touch: the
+                    //   occupied bricks and their +1 neighbours
+  int32_t* count;   // [n_cells + 2] particles per cell, then the exclusive
+                    //   prefix inside the brick; [n_cells + 1] counts the
+                    //   occupied bricks
+  int32_t* brick_total;   // [n_bricks + 1] particles per brick
+  int32_t* brick_start;   // [n_bricks + 1] exclusive prefix sum of brick_total
   int32_t* n_list;  // = count + n_cells + 1
   int32_t* list;    // [n_bricks]    occupied bricks (any order)
   int32_t* start;   // [n_cells + 1] exclusive prefix sum of count
@@ -34,7 +38,8 @@ long long brick_total(const dt_mpm_params* prm);
 size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws);
 // Persistent grid of the brick-owned passes (blocks stride over the list).
 constexpr int kBrickBlocks = 2048;
-// count -> scan -> fill -> occupied-brick list on stream st (flags untouched).
+// count -> per-brick cell scan -> brick scan -> cell starts, brick flags and
+// occupied-brick list -> fill, on stream st.
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st);
 
 }  // namespace dt

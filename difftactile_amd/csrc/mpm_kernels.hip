@@ -1376,10 +1376,9 @@ static unsigned brick_blocks(const BinWs& ws) {
   return (unsigned)(ws.n_bricks < kBrickBlocks ? ws.n_bricks : kBrickBlocks);
 }
 
-// mark the active bricks and bin the particles of x
-static int launch_mark_bin(const MpmP& P, const dt_mpm_params* prm, int n, const float* x,
+// bin the particles of x (also sets the active-brick flags)
+static int launch_mark_bin(const MpmP& P, const dt_mpm_params*, int n, const float* x,
                            const BinWs& ws, hipStream_t st) {
-  launch_mark(P, prm, n, x, ws.flags, st);
   return launch_bin(P, n, x, ws, st);
 }
 

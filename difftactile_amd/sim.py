@@ -7,7 +7,10 @@ Here the same structure lives entirely in HBM:
 
 * the forward pass keeps only one state every ``checkpoint_every`` substeps;
 * the backward pass recomputes one segment at a time, keeping that segment's
-  frames and grids, and sweeps the fused adjoint kernels over it in reverse.
+  frames and grids, and sweeps the fused adjoint kernels over it in reverse;
+* segments whose frames and grids fit ``resident_bytes`` (HBM set aside for the
+  trajectory) are kept from the forward pass instead, so the backward pass
+  does not recompute them.  Both give the same gradients.
 
 Particles are ordered by cell key (a stable integer radix sort) at the start
 of each call; the order is undone on output so callers see their own order.
@@ -31,8 +34,14 @@ class MPMSim:
     """One MPM object on the GPU: parameters, material table, per-particle ids."""
 
     def __init__(self, prm, materials, n_particles, mat_id=None, device="cuda",
-                 checkpoint_every=50, sort=True, sparse_grid=True):
+                 checkpoint_every=50, sort=True, sparse_grid=True, resident_bytes=None):
         self.prm = prm
+        if resident_bytes is None:
+            # a quarter of the device's memory for resident segments
+            dev = torch.device(device)
+            total = torch.cuda.get_device_properties(dev).total_memory if dev.type == "cuda" else 0
+            resident_bytes = total // 4
+        self.resident_bytes = int(resident_bytes)
         self.sparse_grid = bool(sparse_grid)
         # fused kernels run on the [k][n] (SoA) particle layout; the caller's
         # [n][k] arrays are transposed (and cell-sorted) at the boundary
@@ -119,12 +128,20 @@ class MPMSim:
     def _mid(self, mat_id):
         return None if mat_id is None else _lib.ptr(mat_id, torch.int32)
 
-    def run_frames(self, X, V, C, F, k, mat_id, keep_grids):
-        """Advance frames 0..k of frame-major buffers X[k+1,n,3] ... in place."""
+    def segment_bytes(self, k):
+        """HBM bytes of one k-substep segment's frames and grids."""
+        return 4 * (self.n * 24 * (k + 1) + 2 * 4 * k * self.prm.n_nodes)
+
+    def run_frames(self, X, V, C, F, k, mat_id, keep_grids, grids=None):
+        """Advance frames 0..k of frame-major buffers X[k+1,n,3] ... in place.
+        keep_grids: keep every substep's grids (in `grids` when given)."""
         nn = self.prm.n_nodes
         if keep_grids:
-            G4 = self._buf("G4", (k, nn, 4))
-            GV4 = self._buf("GV4", (k, nn, 4))
+            if grids is not None:
+                G4, GV4 = grids
+            else:
+                G4 = self._buf("G4", (k, nn, 4))
+                GV4 = self._buf("GV4", (k, nn, 4))
             s4, sv4 = None, None
             g4p, gv4p = _lib.ptr(G4), _lib.ptr(GV4)
         else:
@@ -197,10 +214,26 @@ class _Rollout(torch.autograd.Function):
         else:
             mat_id = sim.mat_id
         ckpts = []
+        used = 0
         for (s, k) in _segments(n_substeps, sim.checkpoint_every):
-            if ctx is not None:
-                ckpts.append([t.clone() for t in state])
-            sim.advance(state, k, mat_id)
+            nbytes = sim.segment_bytes(k)
+            if ctx is not None and used + nbytes <= sim.resident_bytes:
+                # resident segment: keep frames and grids for the backward pass
+                used += nbytes
+                n, nn = sim.n, sim.prm.n_nodes
+                fr = [torch.empty((k + 1, w, n), device=sim.device) for w in _Rollout.WIDTHS]
+                grids = (torch.empty((k, nn, 4), device=sim.device),
+                         torch.empty((k, nn, 4), device=sim.device))
+                for f, t in zip(fr, state):
+                    f[0].copy_(t)
+                sim.run_frames(*fr, k, mat_id, keep_grids=True, grids=grids)
+                for f, t in zip(fr, state):
+                    t.copy_(f[k])
+                ckpts.append((fr, grids))
+            else:
+                if ctx is not None:
+                    ckpts.append([t.clone() for t in state])
+                sim.advance(state, k, mat_id)
         out = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(state, _Rollout.WIDTHS, shapes)]
         if ctx is not None:
             ctx.sim = sim
@@ -220,9 +253,13 @@ class _Rollout(torch.autograd.Function):
         g = [sim._gather(perm, t.float(), w) for t, w in zip(g, _Rollout.WIDTHS)]
         segs = _segments(ctx.n_substeps, sim.checkpoint_every)
         for (s, k), ck in zip(reversed(segs), reversed(ctx.ckpts)):
-            X, V, Cb, Fb = sim._frames(k, "R%d" % k)
-            X[0].copy_(ck[0]); V[0].copy_(ck[1]); Cb[0].copy_(ck[2]); Fb[0].copy_(ck[3])
-            G4, GV4 = sim.run_frames(X, V, Cb, Fb, k, ctx.mat_id, keep_grids=True)
+            if isinstance(ck, tuple):           # resident segment
+                (X, V, Cb, Fb), (G4, GV4) = ck
+            else:                               # recompute from the checkpoint
+                X, V, Cb, Fb = sim._frames(k, "R%d" % k)
+                X[0].copy_(ck[0]); V[0].copy_(ck[1]); Cb[0].copy_(ck[2]); Fb[0].copy_(ck[3])
+                G4, GV4 = sim.run_frames(X, V, Cb, Fb, k, ctx.mat_id, keep_grids=True)
             sim.adj_frames(X, V, Cb, Fb, k, ctx.mat_id, G4, GV4, g)
+        ctx.ckpts = None
         g = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(g, _Rollout.WIDTHS, ctx.shapes)]
         return (None, None, *g)

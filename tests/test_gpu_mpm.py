@@ -365,6 +365,29 @@ def test_rollout_gradient_matches_oracle(gpu, kind, sort):
         assert err < 1e-3, (name, err)
 
 
+def test_resident_and_recomputed_segments_agree(gpu):
+    """Segments kept from the forward pass, recomputed from checkpoints, and a
+    mix of both give the same trajectory and gradients."""
+    prm, mats, mat_id, x, v, C, F = _facade_inputs("multi", n=2000, seed=29)
+    C = C * 0.1
+    steps, seg = 12, 5
+    res = {}
+    probe = MPMSim(prm, mats, len(x), mat_id=mat_id, checkpoint_every=seg, resident_bytes=0)
+    budgets = {"resident": 1 << 40, "recompute": 0, "mixed": probe.segment_bytes(seg) + 1}
+    w = [_seed(t.shape, 70 + i) for i, t in enumerate((x, v, C, F))]
+    for name, budget in budgets.items():
+        sim = MPMSim(prm, mats, len(x), mat_id=mat_id, checkpoint_every=seg, resident_bytes=budget)
+        st = [dev(a).requires_grad_(True) for a in (x, v, C, F)]
+        out = sim.rollout(*st, steps)
+        loss = sum((o * wi.cuda()).sum() for o, wi in zip(out, w))
+        res[name] = ([o.detach() for o in out], torch.autograd.grad(loss, st))
+    for name in ("recompute", "mixed"):
+        for a, b in zip(res[name][0], res["resident"][0]):
+            assert rel_l2(host64(a), host64(b)) < 1e-6
+        for a, b in zip(res[name][1], res["resident"][1]):
+            assert rel_l2(host64(a), host64(b)) < 1e-5
+
+
 # ---------------------------------------------------------------------------
 # full-size, size-independent properties
 # ---------------------------------------------------------------------------
