@@ -130,34 +130,47 @@ __global__ __launch_bounds__(kBlock) void k_cell_scan(long long n_bricks, int32_
   if (b == 0 && lane == 0) brick_total[n_bricks] = 0;
 }
 
-__device__ __forceinline__ bool brick_occupied(const int32_t* __restrict__ total, int nb, int i,
-                                               int j, int k) {
-  return i >= 0 && j >= 0 && k >= 0 && total[((long long)i * nb + j) * nb + k] > 0;
-}
-
-// start[c] = brick_start[brick] + prefix; per brick (lane 0 of its wavefront):
-// flags = brick or one of its -1 neighbours holds particles (a stencil reaches
-// at most one brick past its base's), list += occupied bricks.
-__global__ __launch_bounds__(kBlock) void k_bin_starts(long long n_bricks, int nb,
+// start[c] = brick_start[brick] + prefix inside the brick.
+__global__ __launch_bounds__(kBlock) void k_bin_starts(long long n_bricks,
                                                        const int32_t* __restrict__ prefix,
-                                                       const int32_t* __restrict__ brick_total,
                                                        const int32_t* __restrict__ brick_start,
-                                                       int32_t* __restrict__ start,
-                                                       int32_t* __restrict__ flags,
-                                                       int32_t* __restrict__ list,
-                                                       int32_t* __restrict__ n_list) {
+                                                       int32_t* __restrict__ start) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long b = c >> 6;
-  if (b >= n_bricks) return;   // whole wavefronts
+  if (b >= n_bricks) return;
   start[c] = brick_start[b] + prefix[c];
   if (c == 0) start[n_bricks * 64] = brick_start[n_bricks];
-  if ((threadIdx.x & 63) != 0) return;
-  const int bk = (int)(b % nb), bj = (int)((b / nb) % nb), bi = (int)(b / ((long long)nb * nb));
-  bool any = false;
-  for (int d = 0; d < 8; ++d)
-    any = any || brick_occupied(brick_total, nb, bi - (d >> 2), bj - ((d >> 1) & 1), bk - (d & 1));
-  flags[b] = any ? 1 : 0;
-  if (brick_total[b] > 0) list[atomicAdd(n_list, 1)] = (int32_t)b;
+}
+
+// Per brick: flags = the brick or one of its -1 neighbours holds particles (a
+// stencil reaches at most one brick past its base's); list += occupied bricks
+// (one atomic per wavefront).
+__global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int nb,
+                                                        const int32_t* __restrict__ total,
+                                                        int32_t* __restrict__ flags,
+                                                        int32_t* __restrict__ list,
+                                                        int32_t* __restrict__ n_list) {
+  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool occ = false;
+  if (b < n_bricks) {
+    const int bk = (int)(b % nb), bj = (int)((b / nb) % nb), bi = (int)(b / ((long long)nb * nb));
+    int any = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const int i = bi - (d >> 2), j = bj - ((d >> 1) & 1), k = bk - (d & 1);
+      if (i >= 0 && j >= 0 && k >= 0) any |= total[((long long)i * nb + j) * nb + k];
+    }
+    flags[b] = any != 0 ? 1 : 0;
+    occ = total[b] > 0;
+  }
+  const unsigned long long m = __ballot(occ);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(n_list, __popcll(m));
+  base = __shfl(base, leader);
+  if (occ) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)b;
 }
 
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st) {
@@ -175,9 +188,10 @@ int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_
     set_error("binning scan failed");
     return DT_ELAUNCH;
   }
-  hipLaunchKernelGGL(k_bin_starts, dim3(nblocks(nc)), dim3(kBlock), 0, st, ws.n_bricks, nb,
-                     (const int32_t*)ws.count, (const int32_t*)ws.brick_total,
-                     (const int32_t*)ws.brick_start, ws.start, ws.flags, ws.list, ws.n_list);
+  hipLaunchKernelGGL(k_bin_starts, dim3(nblocks(nc)), dim3(kBlock), 0, st, ws.n_bricks,
+                     (const int32_t*)ws.count, (const int32_t*)ws.brick_start, ws.start);
+  hipLaunchKernelGGL(k_brick_flags, dim3(nblocks(ws.n_bricks)), dim3(kBlock), 0, st, ws.n_bricks,
+                     nb, (const int32_t*)ws.brick_total, ws.flags, ws.list, ws.n_list);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
                        ws.start, ws.perm);

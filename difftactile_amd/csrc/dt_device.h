@@ -141,37 +141,58 @@ DT_HD void st_m3L(float* __restrict__ b, long long p, PL L, const M3& m) {
 // ----------------------------------------------------------------------------
 // 3x3 SVD: U, V in SO(3), sigma_1 >= sigma_2 >= |sigma_3|, sign on sigma_3.
 //
-// Method: cyclic Jacobi eigen-decomposition of A^T A with exact Givens
-// rotations (fixed sweep count), column sort of B = A V by norm, then Givens
-// QR of B; sigma = diag(R).  This is the structure of McAdams et al. 2011
-// ("Computing the SVD of 3x3 matrices with minimal branching") written with
-// exact rotations instead of the quaternion approximations.
+// Method: cyclic Jacobi eigen-decomposition of the symmetric S = A^T A with
+// exact Jacobi rotations (fixed sweep count), column sort of B = A V by norm,
+// then Givens QR of B; sigma = diag(R).  This is the structure of McAdams et
+// al. 2011 ("Computing the SVD of 3x3 matrices with minimal branching") with
+// exact rotations instead of the quaternion approximations.  S is kept as its
+// six distinct entries and each rotation uses the closed-form update of the
+// symmetric Schur decomposition (Golub & Van Loan, Matrix Computations,
+// sec. 8.5.2): with t = tan(theta) zeroing s_pq,
+//   s_pp -= t s_pq, s_qq += t s_pq, s_pq = 0,
+//   (s_rp, s_rq) <- (c s_rp - s s_rq, s s_rp + c s_rq).
+// Reciprocals and square roots use the hardware v_rcp / v_rsq / v_sqrt (about
+// 1 ulp): a rotation only has to be orthogonal to working precision, which
+// c = rsq(1 + t^2), s = t c guarantees.
 // ----------------------------------------------------------------------------
-DT_HD void jacobi_rot(float (&S)[3][3], float (&V)[3][3], int p, int q) {
-  float apq = S[p][q];
-  float c = 1.f, s = 0.f;
+DT_HD float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+DT_HD float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+DT_HD float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// rotation angle zeroing apq in the (p, q) plane of a symmetric matrix
+DT_HD void jacobi_cs(float app, float aqq, float apq, float& c, float& s, float& t) {
+  c = 1.f;
+  s = 0.f;
+  t = 0.f;
   if (fabsf(apq) > 1e-30f) {
-    float tau = (S[q][q] - S[p][p]) / (2.f * apq);
-    float t = copysignf(1.f, tau) / (fabsf(tau) + sqrtf(1.f + tau * tau));
-    c = 1.f / sqrtf(1.f + t * t);
+    const float tau = (aqq - app) * fast_rcp(2.f * apq);
+    t = copysignf(fast_rcp(fabsf(tau) + fast_sqrt(1.f + tau * tau)), tau);
+    c = fast_rsq(1.f + t * t);
     s = t * c;
   }
-  // S <- J^T S J with J = [[c, s], [-s, c]] in the (p, q) plane
+}
+
+DT_HD void rot_cols(float (&V)[3][3], int p, int q, float c, float s) {
   for (int k = 0; k < 3; ++k) {
-    float skp = S[k][p], skq = S[k][q];
-    S[k][p] = c * skp - s * skq;
-    S[k][q] = s * skp + c * skq;
-  }
-  for (int k = 0; k < 3; ++k) {
-    float spk = S[p][k], sqk = S[q][k];
-    S[p][k] = c * spk - s * sqk;
-    S[q][k] = s * spk + c * sqk;
-  }
-  for (int k = 0; k < 3; ++k) {
-    float vkp = V[k][p], vkq = V[k][q];
+    const float vkp = V[k][p], vkq = V[k][q];
     V[k][p] = c * vkp - s * vkq;
     V[k][q] = s * vkp + c * vkq;
   }
+}
+
+// One rotation on the symmetric (s_pp, s_qq, s_pq, s_rp, s_rq) with r the
+// third index; V accumulates the rotations.
+DT_HD void jacobi_step(float& spp, float& sqq, float& spq, float& srp, float& srq,
+                       float (&V)[3][3], int p, int q) {
+  float c, s, t;
+  jacobi_cs(spp, sqq, spq, c, s, t);
+  spp -= t * spq;
+  sqq += t * spq;
+  spq = 0.f;
+  const float rp = srp, rq = srq;
+  srp = c * rp - s * rq;
+  srq = s * rp + c * rq;
+  rot_cols(V, p, q, c, s);
 }
 
 DT_HD void swap_cols_neg(float (&B)[3][3], float (&V)[3][3], int i, int j) {
@@ -188,7 +209,7 @@ DT_HD void givens_qr(float (&B)[3][3], float (&U)[3][3], int p, int r, int c) {
   float d = a * a + b * b;
   float cs = 1.f, sn = 0.f;
   if (d > 1e-38f) {
-    float ir = 1.f / sqrtf(d);
+    float ir = fast_rsq(d);
     cs = a * ir;
     sn = b * ir;
   }
@@ -207,17 +228,19 @@ DT_HD void givens_qr(float (&B)[3][3], float (&U)[3][3], int p, int r, int c) {
 }
 
 DT_HD void svd3(const M3& A, M3& Uo, float (&sig)[3], M3& Vo) {
-  float S[3][3], V[3][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}};
-  // S = A^T A
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      S[i][j] = A.a[0 * 3 + i] * A.a[0 * 3 + j] + A.a[1 * 3 + i] * A.a[1 * 3 + j] +
-                A.a[2 * 3 + i] * A.a[2 * 3 + j];
+  float V[3][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}};
+  // S = A^T A (six distinct entries)
+  auto ata = [&](int i, int j) {
+    return A.a[0 * 3 + i] * A.a[0 * 3 + j] + A.a[1 * 3 + i] * A.a[1 * 3 + j] +
+           A.a[2 * 3 + i] * A.a[2 * 3 + j];
+  };
+  float s00 = ata(0, 0), s11 = ata(1, 1), s22 = ata(2, 2);
+  float s01 = ata(0, 1), s02 = ata(0, 2), s12 = ata(1, 2);
 #pragma unroll
   for (int sweep = 0; sweep < 6; ++sweep) {
-    jacobi_rot(S, V, 0, 1);
-    jacobi_rot(S, V, 0, 2);
-    jacobi_rot(S, V, 1, 2);
+    jacobi_step(s00, s11, s01, s02, s12, V, 0, 1);   // r = 2: (s_20, s_21)
+    jacobi_step(s00, s22, s02, s01, s12, V, 0, 2);   // r = 1: (s_10, s_12)
+    jacobi_step(s11, s22, s12, s01, s02, V, 1, 2);   // r = 0: (s_01, s_02)
   }
   // B = A V
   float B[3][3];
@@ -454,7 +477,7 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
       gFn = gFp;
     } else {
       pl = true;
-      for (int k = 0; k < 3; ++k) d[k] = exp((double)st.epsn[k]);
+      for (int k = 0; k < 3; ++k) d[k] = (double)expf(st.epsn[k]);
       N = m3_mul(m3_mul_at(U, gFp), V);
       if (!kExact) {
         M3d Ud = m3d_from(U), Vd = m3d_from(V), gFpd = m3d_from(gFp);
@@ -520,17 +543,22 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
         if (fabs(den) < 1e-12) den = den < 0.0 ? -1e-12 : 1e-12;
         double v = ((double)Mr.a[i * 3 + j] - (double)Mr.a[j * 3 + i]) / den;
         if (pl) {
-          double si = sig[i], sj = sig[j], del = sj - si, Dd = 0.0;
-          double a = st.pa, G = st.pG;
-          if (si > 0.05 && sj > 0.05) {
-            if (del != 0.0)
-              Dd = G * pow(si, a) * expm1(a * log1p(del / si)) / del;
-            else
-              Dd = G * a * pow(si, a - 1.0);
+          // Dd = (d_j - d_i) / (s_j - s_i) with d_k = G s_k^a: near s_i = s_j
+          // from the binomial series of ((1 + x)^a - 1) / x, x = del / s_i
+          // (|x| < 0.1, six terms, truncation below 1e-7 relative), else as
+          // the plain divided difference of the forward pass's d
+          const double si = sig[i], sj = sig[j], del = sj - si, a = st.pa;
+          double Dd = 0.0;
+          const double x = del / si;
+          if (si > 0.05 && sj > 0.05 && fabs(x) < 0.1) {
+            // a (1 + (a-1)/2 x (1 + (a-2)/3 x (1 + ... (a-5)/6 x))), inside out
+            double ser = 0.0;
+            for (int k = 5; k >= 1; --k) ser = (a - k) / (k + 1) * (1.0 + x * ser);
+            Dd = d[i] / si * a * (1.0 + x * ser);
           } else if (del != 0.0) {
-            Dd = G * (pow(fmax(sj, 0.05), a) - pow(fmax(si, 0.05), a)) / del;
+            Dd = (d[j] - d[i]) / del;
           }
-          double hs = 0.5 * (d[i] + d[j]) / den;
+          const double hs = 0.5 * (d[i] + d[j]) / den;
           v += (double)N.a[i * 3 + j] * (hs + 0.5 * Dd) + (double)N.a[j * 3 + i] * (-hs + 0.5 * Dd);
         }
         Mi.a[i * 3 + j] = (float)v;
