@@ -1077,22 +1077,23 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
 // ----------------------------------------------------------------------------
 // brick-owned scatter over binned particles: block b processes the particles
 // of brick b (perm order: the brick's 64 cells one after another) in chunks of
-// 256 and owns the 6x6x6-node tile of partial sums its stencils reach.  Per
-// chunk:
-//   1. thread t computes particle t and stages (w, b, M) in LDS;
-//   2. work item (cell, j, k) sums, over the cell's particles, the
-//      contributions w_o (b + M o) to the three nodes o = (i, j, k), i = 0..2,
-//      and writes them to its own LDS slots (lanes of one cell read the same
-//      staged particle: LDS broadcast);
-//   3. tile node thread t < 216 adds the up to 27 slots that name it.
+// 256 and owns the 6x6x6-node tile of partial sums its stencils reach.
+//   per chunk  1. thread t computes particle t and stages (w, b, M) in LDS;
+//              2. work item (cell, j, k) sums, over the cell's particles, the
+//                 contributions w_o (b + M o) to the three nodes o = (i, j, k),
+//                 i = 0..2, into its own LDS slots (lanes of one cell read the
+//                 same staged particle: LDS broadcast);
+//   per brick  3. tile node thread t < 216 adds the 27 slots that can name it
+//                 (slots of absent cells stay zero).
 // No atomics anywhere; the tiles are summed per node by the grid passes below.
 // ----------------------------------------------------------------------------
+constexpr int kSlots = 64 * 27;
 struct BrickStage {
   // particle q: pd[6q+0] = (wx0, wx1, wx2, b0), pd[6q+1] = (wy0.., b1),
   // pd[6q+2] = (wz0.., b2), pd[6q+3..5] = M row-major (3 floats of padding)
   float4 pd[6 * kBlock];
-  float4 slot[64 * 27];   // (cell, j, k, i): (sum w u0, w u1, w u2, w)
-  int cs[65];             // cell starts of the brick's 64 cells (positions in perm)
+  float4 slot[kSlots + 1];   // (cell, j, k, i): (sum w u0, w u1, w u2, w); last stays 0
+  int cs[65];                // cell starts of the brick's 64 cells (positions in perm)
 };
 
 __device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Stencil& s,
@@ -1106,17 +1107,23 @@ __device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Sten
   q[5] = make_float4(M.a[8], 0.f, 0.f, 0.f);
 }
 
-__device__ __forceinline__ float pick3(const float4& v, int i) {
-  return i == 0 ? v.x : (i == 1 ? v.y : v.z);
+// brick start: zero the slots, load the cell starts
+__device__ __forceinline__ void brick_begin(BrickStage& sh, const int32_t* __restrict__ start,
+                                            long long b, int& s0, int& s1) {
+  for (int l = threadIdx.x; l < kSlots + 1; l += blockDim.x)
+    sh.slot[l] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (threadIdx.x <= 64) sh.cs[threadIdx.x] = start[b * 64 + threadIdx.x];
+  s0 = start[b * 64];
+  s1 = start[b * 64 + 64];
 }
 
 // step 2: items over the cells touched by perm positions [c0, c1)
 __device__ __forceinline__ void brick_items(BrickStage& sh, int c0, int c1) {
-  // cells overlapping the chunk: [cell_lo, cell_hi]
-  int cell_lo = 0, cell_hi = 63;
+  int cell_lo = 0, cell_hi = 63;   // cells overlapping the chunk
   while (sh.cs[cell_lo + 1] <= c0) ++cell_lo;
   while (sh.cs[cell_hi] >= c1) --cell_hi;
   const int n_items = (cell_hi - cell_lo + 1) * 9;
+  const float* pf = reinterpret_cast<const float*>(sh.pd);
   for (int it = threadIdx.x; it < n_items; it += blockDim.x) {
     const int cell = cell_lo + it / 9, jk = it % 9;
     const int j = jk / 3, k = jk - 3 * j;
@@ -1126,12 +1133,14 @@ __device__ __forceinline__ void brick_items(BrickStage& sh, int c0, int c1) {
     float4 acc0 = make_float4(0.f, 0.f, 0.f, 0.f), acc1 = acc0, acc2 = acc0;
     for (int q = q0; q < q1; ++q) {
       const float4* pq = sh.pd + 6 * q;
-      const float4 X = pq[0], Y = pq[1], Z = pq[2], M0 = pq[3], M1 = pq[4], M2 = pq[5];
-      const float wyz = pick3(Y, j) * pick3(Z, k);
+      const float* fq = pf + 24 * q;
+      const float4 X = pq[0], M0 = pq[3], M1 = pq[4];
+      const float wy = fq[4 + j], b1 = fq[7], wz = fq[8 + k], b2 = fq[11], m22 = fq[20];
+      const float wyz = wy * wz;
       // u = b + M (0, j, k); node i adds w_i (u + M (i, 0, 0))
       const float u0 = X.w + M0.y * fj + M0.z * fk;
-      const float u1 = Y.w + M1.x * fj + M1.y * fk;
-      const float u2 = Z.w + M1.w * fj + M2.x * fk;
+      const float u1 = b1 + M1.x * fj + M1.y * fk;
+      const float u2 = b2 + M1.w * fj + m22 * fk;
       const float m00 = M0.x, m10 = M0.w, m20 = M1.z;
       const float w0 = X.x * wyz, w1 = X.y * wyz, w2 = X.z * wyz;
       acc0.x += w0 * u0;
@@ -1148,53 +1157,37 @@ __device__ __forceinline__ void brick_items(BrickStage& sh, int c0, int c1) {
       acc2.w += w2;
     }
     float4* sl = sh.slot + (cell * 9 + jk) * 3;
-    sl[0] = acc0;
-    sl[1] = acc1;
-    sl[2] = acc2;
+    const float4 o0 = sl[0], o1 = sl[1], o2 = sl[2];
+    sl[0] = make_float4(o0.x + acc0.x, o0.y + acc0.y, o0.z + acc0.z, o0.w + acc0.w);
+    sl[1] = make_float4(o1.x + acc1.x, o1.y + acc1.y, o1.z + acc1.z, o1.w + acc1.w);
+    sl[2] = make_float4(o2.x + acc2.x, o2.y + acc2.y, o2.z + acc2.z, o2.w + acc2.w);
   }
 }
 
-// step 3: tile node thread adds the slots of the cells that reach it
-__device__ __forceinline__ void brick_gather(const BrickStage& sh, int c0, int c1, float4& acc) {
+// step 3: tile node thread t adds the slots of the cells that reach it
+__device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const int t = threadIdx.x;
-  if (t >= kTileNodes) return;
+  if (t >= kTileNodes) return acc;
   const int a0 = t / (kTileEdge * kTileEdge), a1 = (t / kTileEdge) % kTileEdge,
             a2 = t % kTileEdge;
 #pragma unroll
-  for (int o0 = 0; o0 < 3; ++o0) {
-    const int ca = a0 - o0;
-    if (ca < 0 || ca > 3) continue;
+  for (int o0 = 0; o0 < 3; ++o0)
 #pragma unroll
-    for (int o1 = 0; o1 < 3; ++o1) {
-      const int cb = a1 - o1;
-      if (cb < 0 || cb > 3) continue;
+    for (int o1 = 0; o1 < 3; ++o1)
 #pragma unroll
       for (int o2 = 0; o2 < 3; ++o2) {
-        const int cc = a2 - o2;
-        if (cc < 0 || cc > 3) continue;
+        const int ca = a0 - o0, cb = a1 - o1, cc = a2 - o2;
+        const bool in = (unsigned)ca < 4u && (unsigned)cb < 4u && (unsigned)cc < 4u;
         const int cell = (ca * 4 + cb) * 4 + cc;
-        if (max(sh.cs[cell], c0) >= min(sh.cs[cell + 1], c1)) continue;
-        const float4 v = sh.slot[(cell * 9 + o1 * 3 + o2) * 3 + o0];
+        const int idx = in ? (cell * 9 + o1 * 3 + o2) * 3 + o0 : kSlots;
+        const float4 v = sh.slot[idx];
         acc.x += v.x;
         acc.y += v.y;
         acc.z += v.z;
         acc.w += v.w;
       }
-    }
-  }
-}
-
-__device__ __forceinline__ void brick_accumulate(BrickStage& sh, int c0, int c1, float4& acc) {
-  brick_items(sh, c0, c1);
-  __syncthreads();
-  brick_gather(sh, c0, c1, acc);
-}
-
-__device__ __forceinline__ void brick_range(const int32_t* __restrict__ start, long long b,
-                                            BrickStage& sh, int& s0, int& s1) {
-  s0 = start[b * 64];
-  s1 = start[b * 64 + 64];
-  if (threadIdx.x <= 64) sh.cs[threadIdx.x] = start[b * 64 + threadIdx.x];
+  return acc;
 }
 
 // Blocks stride over the occupied-brick list (a persistent grid: unoccupied
@@ -1205,8 +1198,7 @@ __device__ __forceinline__ void brick_range(const int32_t* __restrict__ start, l
   for (int li = blockIdx.x; li < n_occ; li += gridDim.x) {           \
     const long long b = list[li];                                    \
     int s0, s1;                                                      \
-    brick_range(start, b, sh, s0, s1);                               \
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);                    \
+    brick_begin(sh, start, b, s0, s1);                               \
     for (int c0 = s0; c0 < s1; c0 += kBlock) {                       \
       const int c1 = min(c0 + (int)kBlock, s1);                      \
       const int i = c0 + threadIdx.x;                                \
@@ -1219,13 +1211,15 @@ __device__ __forceinline__ void brick_range(const int32_t* __restrict__ start, l
         stage_particle(sh, threadIdx.x, s, bb, M);                   \
       }                                                              \
       __syncthreads();                                               \
-      brick_accumulate(sh, c0, c1, acc);                             \
+      brick_items(sh, c0, c1);                                       \
       __syncthreads();                                               \
     }                                                                \
+    float4 acc = brick_gather(sh);                                   \
     if (threadIdx.x < kTileNodes) {                                  \
       acc.w *= (mass_scale);                                         \
       tiles[b * kTileNodes + threadIdx.x] = acc;                     \
     }                                                                \
+    __syncthreads();                                                 \
   }
 
 __global__ __launch_bounds__(kBlock) void k_p2g_binned(MpmP P, const dt_material* __restrict__ mats,
