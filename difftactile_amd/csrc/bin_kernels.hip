@@ -1,14 +1,13 @@
-// Per-substep binning of particles by stencil-base cell (counting sort).
+// Per-substep binning of particles by the brick of their stencil base
+// (counting sort over bricks).
 //
-// k_bin_count gives every particle its rank inside its cell with one global
-// atomic per run of equal keys in a wavefront (cell-sorted particles make a
-// run of every cell's particles in the wave).  Cell starts come from a
-// two-level scan: one wavefront per brick scans its 64 cell counts, a device
-// scan runs over the brick totals, and k_bin_starts adds the two (and derives
-// the brick flags and the occupied-brick list from the brick totals).
-// k_bin_fill writes the permutation.  The order
-// of particles inside a cell follows the atomics and is not deterministic
-// across waves; nothing downstream depends on it beyond float summation order.
+// k_bin_count gives every particle its rank inside its brick with one global
+// atomic per run of equal keys in a wavefront (cell-sorted particles make long
+// runs), an exclusive scan over the brick counts gives the brick starts, and
+// k_bin_fill writes the permutation.  k_brick_flags derives the active-brick
+// flags (occupied bricks and their +1 neighbours, which a stencil can reach)
+// and the occupied-brick list from the counts.  The order inside a brick
+// follows the atomics; the brick passes sort each chunk by cell themselves.
 #include <hipcub/hipcub.hpp>
 
 #include "dt_bin.h"
@@ -24,23 +23,21 @@ long long brick_total(const dt_mpm_params* prm) {
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
-  const long long nbk = brick_total(prm), ncell = nbk * 64;
+  const long long nbk = brick_total(prm);
   size_t scan_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
                                    (int32_t*)nullptr, (int)(nbk + 1));
-  const size_t sz[11] = {sizeof(int32_t) * (size_t)nbk,
-                        sizeof(int32_t) * (size_t)(ncell + 2),
-                        sizeof(int32_t) * (size_t)(ncell + 1),
+  const size_t sz[9] = {sizeof(int32_t) * (size_t)nbk,
+                        sizeof(int32_t) * (size_t)(nbk + 2),
+                        sizeof(int32_t) * (size_t)(nbk + 1),
+                        sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t) * (size_t)n,
                         sizeof(int32_t) * (size_t)n,
                         sizeof(int32_t) * (size_t)n,
                         sizeof(float4) * (size_t)nbk * kTileNodes,
-                        scan_bytes,
-                        sizeof(int32_t) * (size_t)nbk,
-                        sizeof(int32_t) * (size_t)(nbk + 1),
-                        sizeof(int32_t) * (size_t)(nbk + 1)};
-  size_t off[11], total = 0;
-  for (int i = 0; i < 11; ++i) {
+                        scan_bytes};
+  size_t off[9], total = 0;
+  for (int i = 0; i < 9; ++i) {
     off[i] = total;
     total += align256(sz[i] ? sz[i] : 1);
   }
@@ -48,19 +45,16 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
     char* b = static_cast<char*>(base);
     ws->flags = reinterpret_cast<int32_t*>(b + off[0]);
     ws->count = reinterpret_cast<int32_t*>(b + off[1]);
+    ws->n_list = ws->count + nbk + 1;
     ws->start = reinterpret_cast<int32_t*>(b + off[2]);
-    ws->key = reinterpret_cast<int32_t*>(b + off[3]);
-    ws->rank = reinterpret_cast<int32_t*>(b + off[4]);
-    ws->perm = reinterpret_cast<int32_t*>(b + off[5]);
-    ws->tiles = reinterpret_cast<float4*>(b + off[6]);
-    ws->scan_tmp = b + off[7];
-    ws->list = reinterpret_cast<int32_t*>(b + off[8]);
-    ws->brick_total = reinterpret_cast<int32_t*>(b + off[9]);
-    ws->brick_start = reinterpret_cast<int32_t*>(b + off[10]);
-    ws->n_list = ws->count + ncell + 1;
+    ws->list = reinterpret_cast<int32_t*>(b + off[3]);
+    ws->key = reinterpret_cast<int32_t*>(b + off[4]);
+    ws->rank = reinterpret_cast<int32_t*>(b + off[5]);
+    ws->perm = reinterpret_cast<int32_t*>(b + off[6]);
+    ws->tiles = reinterpret_cast<float4*>(b + off[7]);
+    ws->scan_tmp = b + off[8];
     ws->scan_bytes = scan_bytes;
     ws->n_bricks = nbk;
-    ws->n_cells = ncell;
   }
   return total;
 }
@@ -81,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float
   if (active) {
     float xp[3];
     ld_v3(x, p, pl_vec(P.soa, n), xp);
-    k = bin_key(xp, P.inv_dx, P.n_grid);
+    k = bin_key(xp, P.inv_dx, P.n_grid) >> 6;
   }
   const int lane = threadIdx.x & 63;
   const int prev = __shfl_up(k, 1);
@@ -111,42 +105,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __res
   if (p < n) perm[start[key[p]] + rank[p]] = p;
 }
 
-// One wavefront per brick: count[c] -> exclusive prefix inside the brick,
-// brick_total[b] = particle count of the brick.
-__global__ __launch_bounds__(kBlock) void k_cell_scan(long long n_bricks, int32_t* __restrict__ count,
-                                                      int32_t* __restrict__ brick_total) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long b = c >> 6;
-  if (b >= n_bricks) return;   // whole wavefronts
-  const int lane = threadIdx.x & 63;
-  const int v = count[c];
-  int incl = v;
-  for (int d = 1; d < 64; d <<= 1) {
-    const int u = __shfl_up(incl, d);
-    if (lane >= d) incl += u;
-  }
-  count[c] = incl - v;
-  if (lane == 63) brick_total[b] = incl;
-  if (b == 0 && lane == 0) brick_total[n_bricks] = 0;
-}
-
-// start[c] = brick_start[brick] + prefix inside the brick.
-__global__ __launch_bounds__(kBlock) void k_bin_starts(long long n_bricks,
-                                                       const int32_t* __restrict__ prefix,
-                                                       const int32_t* __restrict__ brick_start,
-                                                       int32_t* __restrict__ start) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long b = c >> 6;
-  if (b >= n_bricks) return;
-  start[c] = brick_start[b] + prefix[c];
-  if (c == 0) start[n_bricks * 64] = brick_start[n_bricks];
-}
-
 // Per brick: flags = the brick or one of its -1 neighbours holds particles (a
 // stencil reaches at most one brick past its base's); list += occupied bricks
 // (one atomic per wavefront).
 __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int nb,
-                                                        const int32_t* __restrict__ total,
+                                                        const int32_t* __restrict__ count,
                                                         int32_t* __restrict__ flags,
                                                         int32_t* __restrict__ list,
                                                         int32_t* __restrict__ n_list) {
@@ -158,10 +121,10 @@ __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int 
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
       const int i = bi - (d >> 2), j = bj - ((d >> 1) & 1), k = bk - (d & 1);
-      if (i >= 0 && j >= 0 && k >= 0) any |= total[((long long)i * nb + j) * nb + k];
+      if (i >= 0 && j >= 0 && k >= 0) any |= count[((long long)i * nb + j) * nb + k];
     }
     flags[b] = any != 0 ? 1 : 0;
-    occ = total[b] > 0;
+    occ = count[b] > 0;
   }
   const unsigned long long m = __ballot(occ);
   if (m == 0ull) return;
@@ -174,24 +137,20 @@ __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int 
 }
 
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st) {
-  const long long nc = ws.n_cells;
+  const long long nbk = ws.n_bricks;
   const int nb = (P.n_grid + 3) >> 2;
-  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nc + 2)), dim3(kBlock), 0, st, nc + 2, ws.count);
+  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nbk + 2)), dim3(kBlock), 0, st, nbk + 2, ws.count);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
                        ws.rank, ws.count);
-  hipLaunchKernelGGL(k_cell_scan, dim3(nblocks(nc)), dim3(kBlock), 0, st, ws.n_bricks, ws.count,
-                     ws.brick_total);
   size_t bytes = ws.scan_bytes;
-  if (hipcub::DeviceScan::ExclusiveSum(ws.scan_tmp, bytes, ws.brick_total, ws.brick_start,
-                                       (int)(ws.n_bricks + 1), st) != hipSuccess) {
+  if (hipcub::DeviceScan::ExclusiveSum(ws.scan_tmp, bytes, ws.count, ws.start, (int)(nbk + 1),
+                                       st) != hipSuccess) {
     set_error("binning scan failed");
     return DT_ELAUNCH;
   }
-  hipLaunchKernelGGL(k_bin_starts, dim3(nblocks(nc)), dim3(kBlock), 0, st, ws.n_bricks,
-                     (const int32_t*)ws.count, (const int32_t*)ws.brick_start, ws.start);
-  hipLaunchKernelGGL(k_brick_flags, dim3(nblocks(ws.n_bricks)), dim3(kBlock), 0, st, ws.n_bricks,
-                     nb, (const int32_t*)ws.brick_total, ws.flags, ws.list, ws.n_list);
+  hipLaunchKernelGGL(k_brick_flags, dim3(nblocks(nbk)), dim3(kBlock), 0, st, nbk, nb,
+                     (const int32_t*)ws.count, ws.flags, ws.list, ws.n_list);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
                        ws.start, ws.perm);

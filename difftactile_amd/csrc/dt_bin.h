@@ -1,9 +1,9 @@
 // Per-substep particle binning for the brick-owned scatter passes.
 //
-// Particles are binned by the cell of their stencil base, cells ordered
-// brick-major (key = brick * 64 + local, 4x4x4-cell bricks).  After
-// launch_bin, perm[start[c] .. start[c+1]) lists the particles of cell c, so
-// the particles of brick b are perm[start[64 b] .. start[64 b + 64]).
+// Particles are binned by the 4x4x4-cell brick of their stencil base.  After
+// launch_bin, perm[start[b] .. start[b+1]) lists the particles of brick b (in
+// no particular order inside the brick); the brick passes order each chunk of
+// a brick by cell in LDS.
 #pragma once
 #include "dt_common.h"
 
@@ -14,23 +14,20 @@ constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
 
 // Views into one caller-provided workspace buffer.
 struct BinWs {
-  int32_t* flags;   // [n_bricks]    bricks some stencil may touch: the
+  int32_t* flags;   // [n_bricks]     bricks some stencil may touch: the
                     //   occupied bricks and their +1 neighbours
-  int32_t* count;   // [n_cells + 2] particles per cell, then the exclusive
-                    //   prefix inside the brick; [n_cells + 1] counts the
-                    //   occupied bricks
-  int32_t* brick_total;   // [n_bricks + 1] particles per brick
-  int32_t* brick_start;   // [n_bricks + 1] exclusive prefix sum of brick_total
-  int32_t* n_list;  // = count + n_cells + 1
-  int32_t* list;    // [n_bricks]    occupied bricks (any order)
-  int32_t* start;   // [n_cells + 1] exclusive prefix sum of count
-  int32_t* key;     // [n]           cell key of each particle
-  int32_t* rank;    // [n]           position of the particle inside its cell
-  int32_t* perm;    // [n]           particles in cell order
+  int32_t* count;   // [n_bricks + 2] particles per brick ([n_bricks] stays 0);
+                    //   [n_bricks + 1] counts the occupied bricks
+  int32_t* n_list;  // = count + n_bricks + 1
+  int32_t* start;   // [n_bricks + 1] exclusive prefix sum of count
+  int32_t* list;    // [n_bricks]     occupied bricks (any order)
+  int32_t* key;     // [n]            brick of each particle
+  int32_t* rank;    // [n]            position of the particle inside its brick
+  int32_t* perm;    // [n]            particles in brick order
   float4* tiles;    // [n_bricks * kTileNodes] per-brick partial node sums
   void* scan_tmp;
   size_t scan_bytes;
-  long long n_bricks, n_cells;
+  long long n_bricks;
 };
 
 long long brick_total(const dt_mpm_params* prm);
@@ -38,8 +35,7 @@ long long brick_total(const dt_mpm_params* prm);
 size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws);
 // Persistent grid of the brick-owned passes (blocks stride over the list).
 constexpr int kBrickBlocks = 2048;
-// count -> per-brick cell scan -> brick scan -> cell starts, brick flags and
-// occupied-brick list -> fill, on stream st.
+// count -> brick scan -> brick flags and occupied-brick list -> fill, on st.
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st);
 
 }  // namespace dt

@@ -566,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
 // complete adjoint of (x, v, C, F); gx_io holds the g2p-adjoint partial on
 // entry.
 template <bool kExact>
-__global__ __launch_bounds__(kBlock) void k_p2g_fused_adj(
+__global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
     MpmP P, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
     const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ C,
     const float* __restrict__ F, const float* __restrict__ ggrid4,
@@ -869,7 +869,7 @@ __device__ __forceinline__ float4 tile_sum(const MpmP& P, long long nd,
         const int B2 = (g2 >> 2) - d2, l2 = (g2 & 3) + 4 * d2;
         if (B2 < 0 || l2 >= kTileEdge) continue;
         const long long B = ((long long)B0 * nb + B1) * nb + B2;
-        if (start[B * 64] == start[B * 64 + 64]) continue;
+        if (start[B] == start[B + 1]) continue;
         const float4 q = tiles[B * kTileNodes + (l0 * kTileEdge + l1) * kTileEdge + l2];
         r.x += q.x;
         r.y += q.y;
@@ -1076,9 +1076,10 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
 
 // ----------------------------------------------------------------------------
 // brick-owned scatter over binned particles: block b processes the particles
-// of brick b (perm order: the brick's 64 cells one after another) in chunks of
-// 256 and owns the 6x6x6-node tile of partial sums its stencils reach.
-//   per chunk  1. thread t computes particle t and stages (w, b, M) in LDS;
+// of brick b in chunks of 256 and owns the 6x6x6-node tile of partial sums its
+// stencils reach.
+//   per chunk  1. thread t computes particle t; an LDS counting sort by cell
+//                 (64 bins) gives its staging slot, where it stages (w, b, M);
 //              2. work item (cell, j, k) sums, over the cell's particles, the
 //                 contributions w_o (b + M o) to the three nodes o = (i, j, k),
 //                 i = 0..2, into its own LDS slots (lanes of one cell read the
@@ -1093,8 +1094,31 @@ struct BrickStage {
   // pd[6q+2] = (wz0.., b2), pd[6q+3..5] = M row-major (3 floats of padding)
   float4 pd[6 * kBlock];
   float4 slot[kSlots + 1];   // (cell, j, k, i): (sum w u0, w u1, w u2, w); last stays 0
-  int cs[65];                // cell starts of the brick's 64 cells (positions in perm)
+  int cs[65];                // cell starts of the chunk (staging slots)
+  int hist[64];              // particles per cell of the chunk
+  int cells[64];             // occupied cells of the chunk
+  int n_cells;
 };
+
+__device__ __forceinline__ int local_cell(const Stencil& s) {
+  return ((s.base[0] & 3) * 4 + (s.base[1] & 3)) * 4 + (s.base[2] & 3);
+}
+
+// wavefront 0: cell starts (exclusive scan of hist) and occupied-cell list
+__device__ __forceinline__ void chunk_scan(BrickStage& sh) {
+  const int l = threadIdx.x;   // 0..63, one full wavefront
+  const int h = sh.hist[l];
+  int incl = h;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(incl, d);
+    if (l >= d) incl += u;
+  }
+  sh.cs[l] = incl - h;
+  if (l == 63) sh.cs[64] = incl;
+  const unsigned long long m = __ballot(h > 0);
+  if (h > 0) sh.cells[__popcll(m & ((1ull << l) - 1ull))] = l;
+  if (l == 0) sh.n_cells = __popcll(m);
+}
 
 __device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Stencil& s,
                                                const float (&b)[3], const M3& M) {
@@ -1107,29 +1131,24 @@ __device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Sten
   q[5] = make_float4(M.a[8], 0.f, 0.f, 0.f);
 }
 
-// brick start: zero the slots, load the cell starts
+// brick start: zero the slots, read the brick's range of perm
 __device__ __forceinline__ void brick_begin(BrickStage& sh, const int32_t* __restrict__ start,
                                             long long b, int& s0, int& s1) {
   for (int l = threadIdx.x; l < kSlots + 1; l += blockDim.x)
     sh.slot[l] = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (threadIdx.x <= 64) sh.cs[threadIdx.x] = start[b * 64 + threadIdx.x];
-  s0 = start[b * 64];
-  s1 = start[b * 64 + 64];
+  s0 = start[b];
+  s1 = start[b + 1];
 }
 
-// step 2: items over the cells touched by perm positions [c0, c1)
-__device__ __forceinline__ void brick_items(BrickStage& sh, int c0, int c1) {
-  int cell_lo = 0, cell_hi = 63;   // cells overlapping the chunk
-  while (sh.cs[cell_lo + 1] <= c0) ++cell_lo;
-  while (sh.cs[cell_hi] >= c1) --cell_hi;
-  const int n_items = (cell_hi - cell_lo + 1) * 9;
+// step 2: items over the occupied cells of the chunk
+__device__ __forceinline__ void brick_items(BrickStage& sh) {
+  const int n_items = sh.n_cells * 9;
   const float* pf = reinterpret_cast<const float*>(sh.pd);
   for (int it = threadIdx.x; it < n_items; it += blockDim.x) {
-    const int cell = cell_lo + it / 9, jk = it % 9;
+    const int cell = sh.cells[it / 9], jk = it % 9;
     const int j = jk / 3, k = jk - 3 * j;
     const float fj = (float)j, fk = (float)k;
-    const int q0 = max(sh.cs[cell], c0) - c0, q1 = min(sh.cs[cell + 1], c1) - c0;
-    if (q0 >= q1) continue;
+    const int q0 = sh.cs[cell], q1 = sh.cs[cell + 1];
     float4 acc0 = make_float4(0.f, 0.f, 0.f, 0.f), acc1 = acc0, acc2 = acc0;
     for (int q = q0; q < q1; ++q) {
       const float4* pq = sh.pd + 6 * q;
@@ -1202,16 +1221,24 @@ __device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
     for (int c0 = s0; c0 < s1; c0 += kBlock) {                       \
       const int c1 = min(c0 + (int)kBlock, s1);                      \
       const int i = c0 + threadIdx.x;                                \
-      if (i < c1) {                                                  \
-        Stencil s;                                                   \
-        float bb[3];                                                 \
-        M3 M;
+      if (threadIdx.x < 64) sh.hist[threadIdx.x] = 0;                \
+      __syncthreads();                                               \
+      Stencil s;                                                     \
+      float bb[3];                                                   \
+      M3 M;                                                          \
+      int lc = 0, r = 0;                                             \
+      if (i < c1) {
 
 #define DT_BRICK_LOOP_END(mass_scale)                                \
-        stage_particle(sh, threadIdx.x, s, bb, M);                   \
+        lc = local_cell(s);                                          \
+        r = atomicAdd(&sh.hist[lc], 1);                              \
       }                                                              \
       __syncthreads();                                               \
-      brick_items(sh, c0, c1);                                       \
+      if (threadIdx.x < 64) chunk_scan(sh);                          \
+      __syncthreads();                                               \
+      if (i < c1) stage_particle(sh, sh.cs[lc] + r, s, bb, M);       \
+      __syncthreads();                                               \
+      brick_items(sh);                                               \
       __syncthreads();                                               \
     }                                                                \
     float4 acc = brick_gather(sh);                                   \
@@ -1882,13 +1909,13 @@ int dt_mpm_bin(const dt_mpm_params* prm, int32_t n, const float* x, void* worksp
 }
 
 int dt_mpm_bin_perm(const dt_mpm_params* prm, int32_t n, const void* workspace,
-                    size_t workspace_bytes, const int32_t** perm, const int32_t** cell_start) {
+                    size_t workspace_bytes, const int32_t** perm, const int32_t** brick_start) {
   int rc = validate_params(prm);
   if (rc) return rc;
   BinWs ws{};
   if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
   if (perm) *perm = ws.perm;
-  if (cell_start) *cell_start = ws.start;
+  if (brick_start) *brick_start = ws.start;
   return DT_OK;
 }
 

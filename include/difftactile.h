@@ -234,18 +234,19 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
    same grids) */
 
 /* Bytes of device workspace the sparse path needs for n particles: brick
-   flags, per-cell counts and starts, per-particle key / rank / permutation,
-   one 6x6x6-node float4 tile per brick, scan scratch. */
+   flags, per-brick counts and starts, occupied-brick list, per-particle key /
+   rank / permutation, one 6x6x6-node float4 tile per brick, scan scratch. */
 int dt_mpm_workspace_bytes(const dt_mpm_params* prm, int32_t n, size_t* bytes);
-/* Marks the active bricks of x and bins the particles by stencil-base cell
-   (cells brick-major as in dt_mpm_cell_keys, but clamped to the stencil's
-   [0, n_grid-3] range). */
+/* Bins the particles of x by the 4x4x4-cell brick of their stencil base
+   (bricks numbered as in dt_mpm_cell_keys, bases clamped to the stencil's
+   [0, n_grid-3] range) and marks the active bricks. */
 int dt_mpm_bin(const dt_mpm_params* prm, int32_t n, const float* x, void* workspace,
                size_t workspace_bytes, void* stream);
-/* Device pointers into the workspace: perm (n) and cell_start (n_cells + 1)
-   of the last dt_mpm_bin. */
+/* Device pointers into the workspace: perm (n) and brick_start
+   (dt_mpm_brick_count + 1) of the last dt_mpm_bin; the particles of brick b are
+   perm[brick_start[b] .. brick_start[b+1]) in no particular order. */
 int dt_mpm_bin_perm(const dt_mpm_params* prm, int32_t n, const void* workspace,
-                    size_t workspace_bytes, const int32_t** perm, const int32_t** cell_start);
+                    size_t workspace_bytes, const int32_t** perm, const int32_t** brick_start);
 /* Brick-owned fused p2g over the binned particles: writes F_next and the
    per-brick tiles in the workspace (after dt_mpm_bin on the same x). */
 int dt_mpm_p2g_binned(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,

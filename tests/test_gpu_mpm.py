@@ -421,7 +421,7 @@ def _workspace(c_prm, n):
     return torch.zeros(nbytes, dtype=torch.uint8, device="cuda"), nbytes
 
 
-def _bin_arrays(c_prm, n, ws, nbytes, n_cells):
+def _bin_arrays(c_prm, n, ws, nbytes, n_bricks):
     import ctypes
     pp, cp = ctypes.c_void_p(), ctypes.c_void_p()
     _lib.check(_lib.lib().dt_mpm_bin_perm(c_prm, n, ws.data_ptr(), nbytes, ctypes.byref(pp),
@@ -429,12 +429,12 @@ def _bin_arrays(c_prm, n, ws, nbytes, n_cells):
     o_p = pp.value - ws.data_ptr()
     o_c = cp.value - ws.data_ptr()
     perm = ws[o_p:o_p + 4 * n].view(torch.int32).cpu().numpy()
-    start = ws[o_c:o_c + 4 * (n_cells + 1)].view(torch.int32).cpu().numpy()
+    start = ws[o_c:o_c + 4 * (n_bricks + 1)].view(torch.int32).cpu().numpy()
     return perm, start
 
 
 @pytest.mark.parametrize("n_grid,n,sort", [(32, 20000, False), (64, 50000, True), (30, 7000, False)])
-def test_binning_is_a_counting_sort_by_stencil_cell(gpu, n_grid, n, sort):
+def test_binning_is_a_counting_sort_by_stencil_brick(gpu, n_grid, n, sort):
     prm = default_prm(n_grid)
     rng = np.random.default_rng(n_grid + n)
     # the whole box and a margin outside it (clamped bases)
@@ -443,16 +443,16 @@ def test_binning_is_a_counting_sort_by_stencil_cell(gpu, n_grid, n, sort):
     if sort:
         x = x[np.argsort(keys, kind="stable")]
         keys = om.bin_keys_np(x, prm.inv_dx, n_grid)
+    bricks = keys >> 6
     c_prm = prm.c_struct()
     ws, nbytes = _workspace(c_prm, n)
     xg = dev(x)
     call("dt_mpm_bin", c_prm, n, xg, ws, nbytes, _lib.stream())
-    nb = (n_grid + 3) // 4
-    n_cells = nb ** 3 * 64
-    perm, start = _bin_arrays(c_prm, n, ws, nbytes, n_cells)
+    n_bricks = ((n_grid + 3) // 4) ** 3
+    perm, start = _bin_arrays(c_prm, n, ws, nbytes, n_bricks)
     assert np.array_equal(np.sort(perm), np.arange(n))
-    assert np.all(np.diff(keys[perm]) >= 0)
-    want = np.searchsorted(np.sort(keys), np.arange(n_cells + 1), side="left")
+    assert np.all(np.diff(bricks[perm]) >= 0)
+    want = np.searchsorted(np.sort(bricks), np.arange(n_bricks + 1), side="left")
     assert np.array_equal(start, want)
 
 
@@ -462,7 +462,7 @@ def test_binning_of_no_particles(gpu):
     ws, nbytes = _workspace(c_prm, 0)
     ws.fill_(7)
     call("dt_mpm_bin", c_prm, 0, None, ws, nbytes, _lib.stream())
-    _, start = _bin_arrays(c_prm, 0, ws, nbytes, 8 ** 3 * 64)
+    _, start = _bin_arrays(c_prm, 0, ws, nbytes, 8 ** 3)
     assert not start.any()
 
 
