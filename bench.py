@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad-check", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one plain 256 MiB device-to-device copy (the reference "
+                         "kernel of known traffic for rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
     ap.add_argument("--workload", default="sphere", choices=["sphere", "pad_press"],
                     help="sphere: 1M-particle multi-material sphere (default, the metric's "
                          "workload); pad_press: slab pad pressing a von Mises block")
@@ -336,6 +339,12 @@ def main():
     dom = max(times, key=times.get)
     achieved = ab[dom] / (times[dom] * 1e-3) / 1e9
     log(f"[bench] kernels: {json.dumps(kern)}")
+    if args.calib_copy:
+        src = torch.ones(64 * 2 ** 20, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):

@@ -565,6 +565,17 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
 // fused adjoint of (compute_new_F + svd + return map + p2g): writes the
 // complete adjoint of (x, v, C, F); gx_io holds the g2p-adjoint partial on
 // entry.
+// Opaque copy: the compiler may not reuse values computed from the original
+// (forces the SVD below to be recomputed instead of kept in registers).
+__device__ __forceinline__ void opaque(M3& m) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) asm volatile("" : "+v"(m.a[i]));
+}
+
+// Two passes over the particle keep the register footprint small: the first
+// recomputes the stress to gather the affine adjoint from the grid, the second
+// recomputes the SVD and the return map (not kept across the gather) for the
+// stress adjoint.
 template <bool kExact>
 __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
     MpmP P, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
@@ -575,40 +586,55 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
   int p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
+  const Mat m = load_mat(mats, mat_id, p);
+  M3 gtau, lgC;
+  {
+    // pass 1: affine A, gather d/d(mv, m) over the stencil
+    const M3 c = ld_m3L(C, p, Lm), f = ld_m3L(F, p, Lm);
+    M3 a = m3_eye();
+    for (int i = 0; i < 9; ++i) a.a[i] += P.dt * c.a[i];
+    const M3 fn = m3_mul(a, f);
+    M3 u, vv;
+    float sig[3];
+    svd3(fn, u, sig, vv);
+    StressFwd st;
+    M3 tau;
+    stress_fwd(fn, u, sig, vv, m, st, tau);
+    const M3 A = affine_of(P, tau, c);
+    Stencil s;
+    float xp[3], vp[3];
+    ld_v3(x, p, Lv, xp);
+    ld_v3(v, p, Lv, vp);
+    make_stencil(xp, P.inv_dx, P.n_grid, s);
+    float pv[3];
+    for (int q = 0; q < 3; ++q) pv[q] = P.p_mass * vp[q];
+    float lgx[3];
+    ld_v3(gx_io, p, Lv, lgx);
+    float lgv[3] = {0.f, 0.f, 0.f};
+    M3 gA;
+    p2g_gather_adj(P, s, pv, A, ggrid4, 4, ggrid4 + 3, 4, lgx, lgv, gA);
+    st_v3(gx_io, p, Lv, lgx);
+    st_v3(gv_out, p, Lv, lgv);
+    affine_adj(P, gA, gtau, lgC);
+  }
+  // pass 2: SVD and return map again, stress adjoint, F_new = (I + dt C) F
   M3 c = ld_m3L(C, p, Lm), f = ld_m3L(F, p, Lm);
+  opaque(c);
+  opaque(f);
   M3 a = m3_eye();
   for (int i = 0; i < 9; ++i) a.a[i] += P.dt * c.a[i];
-  M3 fn = m3_mul(a, f);
+  const M3 fn = m3_mul(a, f);
   M3 u, vv;
   float sig[3];
   svd3(fn, u, sig, vv);
-  Mat m = load_mat(mats, mat_id, p);
   StressFwd st;
   M3 tau;
   stress_fwd(fn, u, sig, vv, m, st, tau);
-  M3 A = affine_of(P, tau, c);
-  Stencil s;
-  float xp[3], vp[3];
-  ld_v3(x, p, Lv, xp);
-  ld_v3(v, p, Lv, vp);
-  make_stencil(xp, P.inv_dx, P.n_grid, s);
-  float pv[3];
-  for (int q = 0; q < 3; ++q) pv[q] = P.p_mass * vp[q];
-  float lgx[3];
-  ld_v3(gx_io, p, Lv, lgx);
-  float lgv[3] = {0.f, 0.f, 0.f};
-  M3 gA;
-  p2g_gather_adj(P, s, pv, A, ggrid4, 4, ggrid4 + 3, 4, lgx, lgv, gA);
-  M3 gtau, lgC;
-  affine_adj(P, gA, gtau, lgC);
   M3 gFn;
   stress_adj<kExact>(fn, u, sig, vv, m, st, gtau, ld_m3L(gFnext, p, Lm), gFn);
-  // F_new = (I + dt C) F
   M3 gc2 = m3_mul_bt(gFn, f);
   for (int i = 0; i < 9; ++i) lgC.a[i] += P.dt * gc2.a[i];
   M3 gf = m3_mul_at(a, gFn);
-  st_v3(gx_io, p, Lv, lgx);
-  st_v3(gv_out, p, Lv, lgv);
   st_m3L(gC_out, p, Lm, lgC);
   st_m3L(gF_out, p, Lm, gf);
 }
