@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--n-grid", type=int, default=128)
     ap.add_argument("--segment", type=int, default=50)
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--breakdown", action="store_true",
+                    help="also time each stage in isolation on the initial state")
     ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad-check", action="store_true")
@@ -125,6 +127,37 @@ def kernel_breakdown(torch, sim, state, mat_id, reps):
         out[name] = time_stage(torch, fn, reps)
     active = int((g4[:, 3] > sim.prm.eps).sum().item())
     return out, active
+
+
+STAGES = {"bin": 0, "p2g_binned": 1, "grid_op4_tiles": 2, "g2p4": 3, "g2p4_adj_binned": 4,
+          "grid_op4_adj_tiles": 5, "p2g_fused_adj": 6}
+
+
+def live_stage_times(_lib):
+    """Average duration (ms) and launch count of every stage over the launches
+    since dt_timing_enable(1): HIP event pairs recorded by the library around
+    each launch on its stream."""
+    import ctypes
+    out = {}
+    for name, sid in STAGES.items():
+        t, c = ctypes.c_double(0.0), ctypes.c_int64(0)
+        _lib.check(_lib.lib().dt_timing_read(sid, ctypes.byref(t), ctypes.byref(c)))
+        if c.value:
+            out[name] = (t.value / c.value, int(c.value))
+    return out
+
+
+def active_nodes(torch, sim, st):
+    """Grid nodes with mass above the occupancy threshold after the first P2G
+    of the workload (dense single-substep entry point)."""
+    from difftactile_amd import _lib
+    n, nn = sim.n, sim.prm.n_nodes
+    g4 = torch.empty(nn, 4, device=st[0].device)
+    gv4 = torch.empty(nn, 4, device=st[0].device)
+    outs = [torch.empty_like(t) for t in st]
+    _lib.call("dt_mpm_substep_fwd", sim.c_prm_aos, sim.mats, len(sim.materials), n, sim.mat_id,
+              *[t.contiguous() for t in st], g4, gv4, *outs, _lib.stream())
+    return int((g4[:, 3] > sim.prm.eps).sum().item())
 
 
 def algorithmic_bytes(n, active, has_mat):
@@ -317,28 +350,42 @@ def main():
         step()
     torch.cuda.synchronize()
     log(f"[bench] timing {args.steps} steps")
+    from difftactile_amd import _lib
+    lib = _lib.lib()
     t0 = time.perf_counter()
     for i in range(args.steps):
         g = step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # stage durations: one more step of the same workload with every stage launch
+    # bracketed by HIP events on its stream (kept out of the timed region: the
+    # event records add gaps between launches, not time to the kernels)
+    _lib.check(lib.dt_timing_enable(1))
+    step()
+    torch.cuda.synchronize()
+    times = live_stage_times(_lib)
+    _lib.check(lib.dt_timing_enable(0))
     finite = bool(torch.isfinite(g).all().item())
     ms = wall * 1e3 / args.steps
     value = n * T * args.steps / wall
     log(f"[bench] {ms:.2f} ms/step, {value:.3e} particle-substeps/s, grad finite={finite}")
 
-    # per-kernel event timings on sorted, resident inputs
-    _, perm = sim.sort_order(st[0].contiguous())
-    sorted_state = [sim._gather(perm, st[0], 3), sim._gather(perm, st[1], 3),
-                    sim._gather(perm, st[2].contiguous(), 9), sim._gather(perm, st[3], 9)]
-    mat_sorted = sim._gather_i(perm, sim.mat_id)
-    times, active = kernel_breakdown(torch, sim, sorted_state, mat_sorted, args.kernel_reps)
+    active = active_nodes(torch, sim, st)
     ab = algorithmic_bytes(n, active, True)
-    kern = {k: {"ms": round(times[k], 5), "GBps": round(ab[k] / (times[k] * 1e-3) / 1e9, 1)}
-            for k in times}
-    dom = max(times, key=times.get)
-    achieved = ab[dom] / (times[dom] * 1e-3) / 1e9
-    log(f"[bench] kernels: {json.dumps(kern)}")
+    kern = {k: {"ms": round(t, 5), "launches": c, "GBps": round(ab[k] / (t * 1e-3) / 1e9, 1)}
+            for k, (t, c) in times.items()}
+    dom = max(times, key=lambda k: times[k][0])
+    achieved = ab[dom] / (times[dom][0] * 1e-3) / 1e9
+    log(f"[bench] stages (average over the timed launches): {json.dumps(kern)}")
+    isolated = None
+    if args.breakdown:
+        _, perm = sim.sort_order(st[0].contiguous())
+        sorted_state = [sim._gather(perm, st[0], 3), sim._gather(perm, st[1], 3),
+                        sim._gather(perm, st[2].contiguous(), 9), sim._gather(perm, st[3], 9)]
+        mat_sorted = sim._gather_i(perm, sim.mat_id)
+        iso, _ = kernel_breakdown(torch, sim, sorted_state, mat_sorted, args.kernel_reps)
+        isolated = {k: round(v, 5) for k, v in iso.items()}
+        log(f"[bench] isolated stages on the initial state (ms): {json.dumps(isolated)}")
     if args.calib_copy:
         src = torch.ones(64 * 2 ** 20, device=dev)
         dst = torch.empty_like(src)
@@ -379,6 +426,8 @@ def main():
         "kernels": kern,
         "grad_finite": finite,
     }
+    if isolated is not None:
+        result["kernels_isolated_ms"] = isolated
     if not args.no_grad_check:
         log("[bench] gradient check vs FP64 FD")
         result["grad_check"] = grad_check(torch)

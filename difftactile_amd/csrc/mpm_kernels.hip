@@ -28,6 +28,7 @@
 #include "dt_bin.h"
 #include "dt_common.h"
 #include "dt_device.h"
+#include "dt_timing.h"
 
 namespace dt {
 
@@ -1439,12 +1440,19 @@ int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
   long long nn = grid_nodes(prm);
   if (ws) {
     int bgrid = (int)((ws->n_bricks + 3) / 4);
-    int rc = launch_mark_bin(P, prm, n, x, *ws, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_p2g_binned, dim3(brick_blocks(*ws)), dim3(kBlock), 0, st, P, mats, n,
+    {
+      StageTimer tm(STAGE_BIN, st);
+      int rc = launch_mark_bin(P, prm, n, x, *ws, st);
+      if (rc) return rc;
+    }
+    {
+      StageTimer tm(STAGE_P2G, st);
+      hipLaunchKernelGGL(k_p2g_binned, dim3(brick_blocks(*ws)), dim3(kBlock), 0, st, P, mats, n,
                        mat_id, x, v, C, F, F1, (const int32_t*)ws->perm,
                        (const int32_t*)ws->start, (const int32_t*)ws->list,
                        (const int32_t*)ws->n_list, ws->tiles);
+    }
+    StageTimer tm(STAGE_GRID_OP, st);
     hipLaunchKernelGGL(k_grid_op4_tiles, dim3(bgrid), dim3(kBlock), 0, st, P,
                        (const int32_t*)ws->flags, (const float4*)ws->tiles,
                        (const int32_t*)ws->start, (float4*)grid4, (float4*)grid_v4);
@@ -1457,9 +1465,11 @@ int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
     hipLaunchKernelGGL(k_grid_op4, dim3(nblocks(nn)), dim3(kBlock), 0, st, P, nn,
                        (const float4*)grid4, (const float*)nullptr, (float4*)grid_v4);
   }
-  if (n > 0)
+  if (n > 0) {
+    StageTimer tm(STAGE_G2P, st);
     hipLaunchKernelGGL(k_g2p, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x,
                        (const float*)grid_v4, 4, x1, v1, C1);
+  }
   return check_launch("dt_mpm_substep_fwd");
 }
 
@@ -1473,12 +1483,19 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
   long long nn = grid_nodes(prm);
   if (ws) {
     int bgrid = (int)((ws->n_bricks + 3) / 4);
-    int rc = launch_mark_bin(P, prm, n, x, *ws, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_g2p_adj_binned, dim3(brick_blocks(*ws)), dim3(kBlock), 0, st, P, n, x,
+    {
+      StageTimer tm(STAGE_BIN, st);
+      int rc = launch_mark_bin(P, prm, n, x, *ws, st);
+      if (rc) return rc;
+    }
+    {
+      StageTimer tm(STAGE_G2P_ADJ, st);
+      hipLaunchKernelGGL(k_g2p_adj_binned, dim3(brick_blocks(*ws)), dim3(kBlock), 0, st, P, n, x,
                        grid_v4, 4, gx1, gv1, gC1, gx, (const int32_t*)ws->perm,
                        (const int32_t*)ws->start, (const int32_t*)ws->list,
                        (const int32_t*)ws->n_list, ws->tiles);
+    }
+    StageTimer tm(STAGE_GRID_OP_ADJ, st);
     hipLaunchKernelGGL(k_grid_op4_adj_tiles, dim3(bgrid), dim3(kBlock), 0, st, P,
                        (const int32_t*)ws->flags, (const float4*)grid4, (const float4*)ws->tiles,
                        (const int32_t*)ws->start, (float4*)ggv4, (float4*)ggrid4);
@@ -1491,8 +1508,10 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
                        (const float4*)grid4, (const float*)nullptr, (const float4*)ggv4,
                        (float4*)ggrid4, (float*)nullptr);
   }
-  if (n > 0)
+  if (n > 0) {
+    StageTimer tm(STAGE_P2G_ADJ, st);
     launch_p2g_fused_adj(P, st, mats, n, mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF);
+  }
   return check_launch("dt_mpm_substep_adj");
 }
 

@@ -85,6 +85,21 @@ typedef struct dt_contact_params {
 const char* dt_last_error(void);
 int dt_version(void);
 
+/* ---- stage timing (no reference counterpart: measurement support) ----
+   dt_timing_enable(1) clears the records and makes every stage launch of
+   dt_mpm_run_fwd / dt_mpm_run_adj / dt_mpm_advance record a HIP event pair on
+   its stream; dt_timing_read synchronizes on those events and returns the
+   summed duration and the launch count of one stage since the last enable. */
+#define DT_STAGE_BIN 0          /* binning: counts, scan, flags, permutation */
+#define DT_STAGE_P2G 1          /* brick-owned fused P2G */
+#define DT_STAGE_GRID_OP 2      /* tile sums + grid_op */
+#define DT_STAGE_G2P 3
+#define DT_STAGE_G2P_ADJ 4      /* brick-owned G2P adjoint */
+#define DT_STAGE_GRID_OP_ADJ 5
+#define DT_STAGE_P2G_ADJ 6      /* fused P2G + stress adjoint */
+int dt_timing_enable(int32_t on);
+int dt_timing_read(int32_t stage, double* total_ms, int64_t* launches);
+
 /* ---- MPM object, one entry point per reference kernel (frame f views) ---- */
 
 /* compute_new_F, mpm_elastic.py:134-137: F_new = (I + dt C) F */
