@@ -363,3 +363,27 @@ def test_exact_adjoint_from_the_undeformed_state(kind):
         fd = (L(v + h * d) - L(v - h * d)) / (2 * h)
     ad = (g * d).sum()
     assert abs(ad - fd) <= 1e-5 * max(1.0, abs(fd.item())), (ad.item(), fd.item())
+
+
+def test_bin_keys_name_the_cell_of_the_stencil_base():
+    """The sparse path's binning key decodes to the stencil base the scatter
+    uses (oracle stencil, no clamp needed inside the grid) and to the clamped
+    base [0, n-3] outside it."""
+    n = 32
+    inv_dx = float(n)
+    rng = np.random.default_rng(7)
+    x = rng.uniform(2.0 / n, 1.0 - 3.0 / n, size=(5000, 3)).astype(np.float32)
+    keys = om.bin_keys_np(x, inv_dx, n)
+    nb = (n + 3) // 4
+    brick, local = keys >> 6, keys & 63
+    bx, by, bz = brick // (nb * nb), (brick // nb) % nb, brick % nb
+    base = np.stack([bx * 4 + (local >> 4), by * 4 + ((local >> 2) & 3), bz * 4 + (local & 3)], 1)
+    want, _, _ = om.stencil(torch.tensor(x), inv_dx)
+    assert np.array_equal(base, want.numpy())
+    assert np.array_equal(keys, om.cell_keys_np(x, inv_dx, n))
+    out = rng.uniform(-0.2, 1.2, size=(5000, 3)).astype(np.float32)
+    k2 = om.bin_keys_np(out, inv_dx, n)
+    b2, l2 = k2 >> 6, k2 & 63
+    dec = np.stack([(b2 // (nb * nb)) * 4 + (l2 >> 4), ((b2 // nb) % nb) * 4 + ((l2 >> 2) & 3),
+                    (b2 % nb) * 4 + (l2 & 3)], 1)
+    assert dec.min() >= 0 and dec.max() <= n - 3
