@@ -147,6 +147,13 @@ def live_stage_times(_lib):
     return out
 
 
+def graph_stats(_lib):
+    import ctypes
+    h, m = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(_lib.lib().dt_graph_stats(ctypes.byref(h), ctypes.byref(m)))
+    return int(h.value), int(m.value)
+
+
 def active_nodes(torch, sim, st):
     """Grid nodes with mass above the occupancy threshold after the first P2G
     of the workload (dense single-substep entry point)."""
@@ -352,11 +359,13 @@ def main():
     log(f"[bench] timing {args.steps} steps")
     from difftactile_amd import _lib
     lib = _lib.lib()
+    h0, m0 = graph_stats(_lib)
     t0 = time.perf_counter()
     for i in range(args.steps):
         g = step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    h1, m1 = graph_stats(_lib)
     # stage durations: one more step of the same workload with every stage launch
     # bracketed by HIP events on its stream (kept out of the timed region: the
     # event records add gaps between launches, not time to the kernels)
@@ -424,6 +433,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "substep_roofline_frac": b_sub * (value / n) / (HBM_PEAK_GBS * 1e9),
         "kernels": kern,
+        "graph_replays": {"hits": h1 - h0, "captures": m1 - m0},
         "grad_finite": finite,
     }
     if isolated is not None:

@@ -1,0 +1,111 @@
+// HIP-graph cache for the trajectory loops.  A 50-substep segment issues
+// ~900 small launches; replaying them as one graph removes the per-launch
+// host and dispatch gaps.  Graphs are keyed by every argument of the loop, so
+// a replay is exactly the launch sequence that was captured; the cache keeps
+// the most recently used entries.  The captured sequences contain kernels
+// only (no memset or memcpy nodes).
+#include <list>
+#include <mutex>
+#include <utility>
+
+#include "dt_graph.h"
+
+namespace dt {
+
+namespace {
+struct Entry {
+  std::vector<uint64_t> key;
+  hipGraphExec_t exec;
+};
+constexpr size_t kMaxGraphs = 16;
+std::mutex g_mu;
+std::list<Entry> g_cache;   // front = most recently used
+bool g_enabled = true;
+long long g_hits = 0, g_misses = 0;
+std::vector<std::pair<int, hipStream_t>> g_capture_streams;   // per device
+
+hipStream_t capture_stream(int dev) {
+  for (auto& d : g_capture_streams)
+    if (d.first == dev) return d.second;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  g_capture_streams.push_back({dev, s});
+  return s;
+}
+}  // namespace
+
+int run_graphed(std::vector<uint64_t> key, hipStream_t st,
+                const std::function<int(hipStream_t)>& body) {
+  if (!g_enabled || timing_enabled()) return body(st);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return body(st);
+  key.push_back((uint64_t)dev);
+  std::lock_guard<std::mutex> lock(g_mu);
+  for (auto it = g_cache.begin(); it != g_cache.end(); ++it) {
+    if (it->key == key) {
+      g_cache.splice(g_cache.begin(), g_cache, it);
+      ++g_hits;
+      if (hipGraphLaunch(g_cache.front().exec, st) != hipSuccess) {
+        set_error("hipGraphLaunch failed");
+        return DT_ELAUNCH;
+      }
+      return DT_OK;
+    }
+  }
+  ++g_misses;
+  hipStream_t cs = capture_stream(dev);
+  if (!cs) return body(st);
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return body(st);
+  }
+  int rc = body(cs);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(cs, &g);
+  if (rc != DT_OK || e != hipSuccess || g == nullptr) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    if (rc != DT_OK) return rc;
+    set_error("graph capture failed");
+    return DT_ELAUNCH;
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    set_error("hipGraphInstantiate failed");
+    return DT_ELAUNCH;
+  }
+  g_cache.push_front({std::move(key), exec});
+  while (g_cache.size() > kMaxGraphs) {
+    (void)hipGraphExecDestroy(g_cache.back().exec);
+    g_cache.pop_back();
+  }
+  if (hipGraphLaunch(exec, st) != hipSuccess) {
+    set_error("hipGraphLaunch failed");
+    return DT_ELAUNCH;
+  }
+  return DT_OK;
+}
+
+}  // namespace dt
+
+using namespace dt;
+
+extern "C" {
+
+int dt_graphs_enable(int32_t on) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_enabled = on != 0;
+  return DT_OK;
+}
+
+int dt_graph_stats(int64_t* hits, int64_t* misses) {
+  DT_REQUIRE(hits && misses, "NULL output");
+  std::lock_guard<std::mutex> lock(g_mu);
+  *hits = g_hits;
+  *misses = g_misses;
+  return DT_OK;
+}
+
+}  // extern "C"

@@ -28,6 +28,7 @@
 #include "dt_bin.h"
 #include "dt_common.h"
 #include "dt_device.h"
+#include "dt_graph.h"
 #include "dt_timing.h"
 
 namespace dt {
@@ -1725,16 +1726,28 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   bool keep = grids4 != nullptr && grids_v4 != nullptr;
   DT_REQUIRE(keep || (scratch4 && scratch_v4), "need kept grids or scratch grids");
   long long nn = grid_nodes(prm);
-  for (int s = 0; s < n_substeps; ++s) {
-    long long o = (long long)s * n, o1 = (long long)(s + 1) * n;
-    float* g4 = keep ? grids4 + (long long)s * nn * 4 : scratch4;
-    float* gv4 = keep ? grids_v4 + (long long)s * nn * 4 : scratch_v4;
-    rc = launch_substep_fwd(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o, g4,
-                            gv4, x + 3 * o1, v + 3 * o1, C + 9 * o1, F + 9 * o1,
-                            DT_STREAM(stream), wp);
-    if (rc) return rc;
-  }
-  return DT_OK;
+  auto body = [&](hipStream_t st) -> int {
+    for (int s = 0; s < n_substeps; ++s) {
+      long long o = (long long)s * n, o1 = (long long)(s + 1) * n;
+      float* g4 = keep ? grids4 + (long long)s * nn * 4 : scratch4;
+      float* gv4 = keep ? grids_v4 + (long long)s * nn * 4 : scratch_v4;
+      int r = launch_substep_fwd(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
+                                 g4, gv4, x + 3 * o1, v + 3 * o1, C + 9 * o1, F + 9 * o1, st, wp);
+      if (r) return r;
+    }
+    return DT_OK;
+  };
+  std::vector<uint64_t> key{1};
+  key_add(key, *prm);
+  for (const void* q : {(const void*)mats, (const void*)mat_id, (const void*)x, (const void*)v,
+                        (const void*)C, (const void*)F, (const void*)grids4,
+                        (const void*)grids_v4, (const void*)scratch4, (const void*)scratch_v4,
+                        (const void*)workspace})
+    key_add(key, q);
+  key_add(key, (long long)n_mats << 32 | (uint32_t)n);
+  key_add(key, (long long)n_substeps);
+  key_add(key, workspace_bytes);
+  return run_graphed(std::move(key), DT_STREAM(stream), body);
 }
 
 int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
@@ -1754,15 +1767,35 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   long long nn = grid_nodes(prm);
   hipStream_t st = DT_STREAM(stream);
   float *a[4] = {gx, gv, gC, gF}, *b[4] = {gx2, gv2, gC2, gF2};
-  for (int s = n_substeps - 1; s >= 0; --s) {
-    long long o = (long long)s * n;
-    rc = launch_substep_adj(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
-                            grids4 + (long long)s * nn * 4, grids_v4 + (long long)s * nn * 4, a[0],
-                            a[1], a[2], a[3], g_grid4, g_grid_v4, b[0], b[1], b[2], b[3], st,
-                            wp);
-    if (rc) return rc;
-    for (int q = 0; q < 4; ++q) { float* t = a[q]; a[q] = b[q]; b[q] = t; }
-  }
+  auto body = [&](hipStream_t cs) -> int {
+    float *p[4] = {gx, gv, gC, gF}, *q[4] = {gx2, gv2, gC2, gF2};
+    for (int s = n_substeps - 1; s >= 0; --s) {
+      long long o = (long long)s * n;
+      int r = launch_substep_adj(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
+                                 grids4 + (long long)s * nn * 4, grids_v4 + (long long)s * nn * 4,
+                                 p[0], p[1], p[2], p[3], g_grid4, g_grid_v4, q[0], q[1], q[2], q[3],
+                                 cs, wp);
+      if (r) return r;
+      for (int k = 0; k < 4; ++k) { float* t = p[k]; p[k] = q[k]; q[k] = t; }
+    }
+    return DT_OK;
+  };
+  std::vector<uint64_t> key{2};
+  key_add(key, *prm);
+  for (const void* ptr : {(const void*)mats, (const void*)mat_id, (const void*)x, (const void*)v,
+                          (const void*)C, (const void*)F, (const void*)grids4,
+                          (const void*)grids_v4, (const void*)g_grid4, (const void*)g_grid_v4,
+                          (const void*)gx, (const void*)gv, (const void*)gC, (const void*)gF,
+                          (const void*)gx2, (const void*)gv2, (const void*)gC2, (const void*)gF2,
+                          (const void*)workspace})
+    key_add(key, ptr);
+  key_add(key, (long long)n_mats << 32 | (uint32_t)n);
+  key_add(key, (long long)n_substeps);
+  key_add(key, workspace_bytes);
+  if ((rc = run_graphed(std::move(key), st, body))) return rc;
+  // the result sits in gx.. after an even number of substeps, else in gx2..
+  if (n_substeps % 2 == 1)
+    for (int k = 0; k < 4; ++k) { float* t = a[k]; a[k] = b[k]; b[k] = t; }
   // result sits in a[]; move it into the caller's seed buffers when needed
   if (a[0] != gx) {
     hipMemcpyAsync(gx, a[0], sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st);
@@ -1788,12 +1821,29 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   DT_REQUIRE(n == 0 || (x && v && C && F && x2 && v2 && C2 && F2), "NULL pointer");
   hipStream_t st = DT_STREAM(stream);
   float *a[4] = {x, v, C, F}, *b[4] = {x2, v2, C2, F2};
-  for (int s = 0; s < n_substeps; ++s) {
-    rc = launch_substep_fwd(prm, mats, n, mat_id, a[0], a[1], a[2], a[3], scratch4, scratch_v4,
-                            b[0], b[1], b[2], b[3], st, wp);
-    if (rc) return rc;
-    for (int q = 0; q < 4; ++q) { float* t = a[q]; a[q] = b[q]; b[q] = t; }
-  }
+  auto body = [&](hipStream_t cs) -> int {
+    float *p[4] = {x, v, C, F}, *q[4] = {x2, v2, C2, F2};
+    for (int s = 0; s < n_substeps; ++s) {
+      int r = launch_substep_fwd(prm, mats, n, mat_id, p[0], p[1], p[2], p[3], scratch4,
+                                 scratch_v4, q[0], q[1], q[2], q[3], cs, wp);
+      if (r) return r;
+      for (int k = 0; k < 4; ++k) { float* t = p[k]; p[k] = q[k]; q[k] = t; }
+    }
+    return DT_OK;
+  };
+  std::vector<uint64_t> key{3};
+  key_add(key, *prm);
+  for (const void* ptr : {(const void*)mats, (const void*)mat_id, (const void*)x, (const void*)v,
+                          (const void*)C, (const void*)F, (const void*)x2, (const void*)v2,
+                          (const void*)C2, (const void*)F2, (const void*)scratch4,
+                          (const void*)scratch_v4, (const void*)workspace})
+    key_add(key, ptr);
+  key_add(key, (long long)n_mats << 32 | (uint32_t)n);
+  key_add(key, (long long)n_substeps);
+  key_add(key, workspace_bytes);
+  if ((rc = run_graphed(std::move(key), st, body))) return rc;
+  if (n_substeps % 2 == 1)
+    for (int k = 0; k < 4; ++k) { float* t = a[k]; a[k] = b[k]; b[k] = t; }
   if (a[0] != x && n > 0) {
     hipMemcpyAsync(x, a[0], sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st);
     hipMemcpyAsync(v, a[1], sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st);

@@ -30,6 +30,8 @@ hipEvent_t take_event() {
 }
 }  // namespace
 
+bool timing_enabled() { return g_on; }
+
 StageTimer::StageTimer(int stage, hipStream_t st) : stage_(stage), st_(st), e1_(nullptr) {
   if (!g_on) return;
   hipEvent_t e0 = take_event(), e1 = take_event();

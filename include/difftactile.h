@@ -100,6 +100,14 @@ int dt_version(void);
 int dt_timing_enable(int32_t on);
 int dt_timing_read(int32_t stage, double* total_ms, int64_t* launches);
 
+/* ---- graph replay (no reference counterpart: launch-overhead removal) ----
+   dt_mpm_run_fwd / dt_mpm_run_adj / dt_mpm_advance capture their launch
+   sequence into a HIP graph the first time they see a set of arguments and
+   replay it with one hipGraphLaunch afterwards (16 most recent argument sets
+   kept per process).  On by default; off while stage timing is on. */
+int dt_graphs_enable(int32_t on);
+int dt_graph_stats(int64_t* hits, int64_t* misses);
+
 /* ---- MPM object, one entry point per reference kernel (frame f views) ---- */
 
 /* compute_new_F, mpm_elastic.py:134-137: F_new = (I + dt C) F */

@@ -521,3 +521,57 @@ def test_sparse_substep_matches_dense_and_oracle(gpu, kind, sort):
     for name, got, want in zip("xvCF", res["sparse"][2], grads):
         err = rel_l2(host64(got), want)
         assert err < 1e-4, (name, err)
+
+
+def test_graph_replays_match_eager_launches(gpu):
+    """dt_mpm_run_fwd / dt_mpm_run_adj captured into a HIP graph and replayed
+    several times, with eager work and synchronisation between the replays,
+    against the eager launches of the same arguments."""
+    import ctypes
+    prm, mats, mat_id, x, v, C, F = _facade_inputs("multi", n=4000, seed=37)
+    C = C * 0.1
+    n, nn, k = len(x), prm.n_nodes, 4
+    c_prm = prm.c_struct()
+    mt = material_table(mats, "cuda")
+    mid = dev(mat_id, torch.int32)
+    frames = [torch.zeros((k + 1,) + a.shape, device="cuda") for a in (x, v, C, F)]
+    G4 = torch.zeros(k, nn, 4, device="cuda")
+    GV4 = torch.zeros(k, nn, 4, device="cuda")
+    ws, nbytes = _workspace(c_prm, n)
+    seeds = [_seed((n,) + a.shape[1:], 500 + i).cuda() for i, a in enumerate((x, v, C, F))]
+    g = [torch.empty_like(s) for s in seeds]
+    g2 = [torch.empty_like(s) for s in seeds]
+    gg4 = torch.zeros(nn, 4, device="cuda")
+    ggv4 = torch.zeros(nn, 4, device="cuda")
+
+    def run():
+        for fr, a in zip(frames, (x, v, C, F)):
+            fr[0] = dev(a)
+        call("dt_mpm_run_fwd", c_prm, mt, len(mats), n, mid, *frames, k, G4, GV4, None, None,
+             ws, nbytes, _lib.stream())
+        for t, s in zip(g, seeds):
+            t.copy_(s)
+        call("dt_mpm_run_adj", c_prm, mt, len(mats), n, mid, *frames, k, G4, GV4, gg4, ggv4,
+             *g, *g2, ws, nbytes, _lib.stream())
+        torch.cuda.synchronize()
+        return [f[k].clone() for f in frames] + [t.clone() for t in g]
+
+    lib = _lib.lib()
+    _lib.check(lib.dt_graphs_enable(0))
+    try:
+        eager = run()
+    finally:
+        _lib.check(lib.dt_graphs_enable(1))
+    h0, m0 = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(lib.dt_graph_stats(ctypes.byref(h0), ctypes.byref(m0)))
+    for rep in range(3):
+        got = run()
+        junk = torch.randn(1 << 20, device="cuda").sum()   # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+        for a, b in zip(got, eager):
+            assert rel_l2(host64(a), host64(b)) < 1e-5, rep
+    h1, m1 = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(lib.dt_graph_stats(ctypes.byref(h1), ctypes.byref(m1)))
+    assert m1.value - m0.value == 2          # one capture each for fwd and adj
+    assert h1.value - h0.value == 4          # then replays
