@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define DT_HD __device__ __forceinline__
 
 namespace dt {
@@ -158,6 +160,8 @@ DT_HD void st_m3L(float* __restrict__ b, long long p, PL L, const M3& m) {
 DT_HD float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 DT_HD float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 DT_HD float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+DT_HD float rcp_r(float x) { return fast_rcp(x); }
+DT_HD double rcp_r(double x) { return 1.0 / x; }
 
 // rotation angle zeroing apq in the (p, q) plane of a symmetric matrix
 DT_HD void jacobi_cs(float app, float aqq, float apq, float& c, float& s, float& t) {
@@ -381,11 +385,11 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
     float eps[3];
     for (int i = 0; i < 3; ++i) {
       st.s[i] = fmaxf(sig[i], 0.05f);
-      eps[i] = logf(st.s[i]);
+      eps[i] = __logf(st.s[i]);
     }
     float tr = eps[0] + eps[1] + eps[2];
     st.trv = tr;
-    for (int i = 0; i < 3; ++i) st.eh[i] = eps[i] - tr / 3.f;
+    for (int i = 0; i < 3; ++i) st.eh[i] = eps[i] - tr * (1.f / 3.f);
     st.ehn = sqrtf(st.eh[0] * st.eh[0] + st.eh[1] * st.eh[1] + st.eh[2] * st.eh[2] + 1e-8f);
     // eps' = mean(eps) + pa (eps - mean(eps)), so d_k = pG s_k^pa with
     // pG = exp((1 - pa) mean(eps))
@@ -393,7 +397,7 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
       float dg = st.ehn - m.yield_stress / (2.f * m.mu);
       if (dg > 0.f) {
         st.mode = 1;
-        float r = dg / st.ehn;
+        float r = dg * fast_rcp(st.ehn);
         for (int i = 0; i < 3; ++i) st.epsn[i] = eps[i] - r * st.eh[i];
         st.pa = 1.f - r;
       }
@@ -406,15 +410,15 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
         float dg = st.ehn + (3.f * m.lam + 2.f * m.mu) / (2.f * m.mu) * tr * m.dp_alpha;
         if (dg > 0.f) {
           st.mode = 2;
-          float r = dg / st.ehn;
+          float r = dg * fast_rcp(st.ehn);
           for (int i = 0; i < 3; ++i) st.epsn[i] = eps[i] - r * st.eh[i];
           st.pa = 1.f - r;
         }
       }
     }
-    st.pG = st.mode == 3 ? 1.f : expf((1.f - st.pa) * tr / 3.f);
+    st.pG = st.mode == 3 ? 1.f : __expf((1.f - st.pa) * tr * (1.f / 3.f));
     if (st.mode) {
-      float d[3] = {expf(st.epsn[0]), expf(st.epsn[1]), expf(st.epsn[2])};
+      float d[3] = {__expf(st.epsn[0]), __expf(st.epsn[1]), __expf(st.epsn[2])};
       udv(U, d, V, st.Fp);
     }
     st.J = m3_det(st.Fp);
@@ -442,6 +446,10 @@ DT_HD double clampd(double a) { return a >= 0.0 ? fmax(a, 1e-8) : fmin(a, -1e-8)
 template <bool kExact>
 DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3& V, const Mat& m,
                       const StressFwd& st, const M3& gtau, const M3& gFp_in, M3& gFn) {
+  // Scalar type of the return-map and pair-coefficient terms: the exact form's
+  // terms are well conditioned (no small denominators) and run in float32;
+  // the reference form keeps float64 for its 1/(s_j^2 - s_i^2) factors.
+  using R = typename std::conditional<kExact, float, double>::type;
   // tau adjoint
   M3 D;
   for (int i = 0; i < 9; ++i) D.a[i] = st.Fp.a[i] - st.R.a[i];
@@ -453,7 +461,7 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
   float trg = gtau.a[0] + gtau.a[4] + gtau.a[8];
   float gJ = m.lam * (2.f * st.J - 1.f) * trg;
 
-  double gsig[3] = {0.0, 0.0, 0.0};
+  R gsig[3] = {R(0), R(0), R(0)};
   // U/V adjoints of the return-map path (F_p = U diag(d) V^T), reference form
   // only; the rotation path R = U V^T is kept apart so the exact form can use
   // its closed form
@@ -462,13 +470,13 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
     for (int i = 0; i < 9; ++i) gUp.a[i] = gVp.a[i] = 0.0;
   M3 N = m3_zero();   // U^T gF_p V of the return-map path
   bool pl = false;
-  double d[3] = {1.0, 1.0, 1.0};
+  R d[3] = {R(1), R(1), R(1)};
   gFn = m3_zero();
 
   if (m.kind == KIND_ELASTIC) {
-    gsig[0] += (double)gJ * sig[1] * sig[2];
-    gsig[1] += (double)gJ * sig[0] * sig[2];
-    gsig[2] += (double)gJ * sig[0] * sig[1];
+    gsig[0] += (R)gJ * sig[1] * sig[2];
+    gsig[1] += (R)gJ * sig[0] * sig[2];
+    gsig[2] += (R)gJ * sig[0] * sig[1];
     gFn = gFp;
   } else {
     M3 cof = m3_cof(st.Fp);
@@ -477,7 +485,7 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
       gFn = gFp;
     } else {
       pl = true;
-      for (int k = 0; k < 3; ++k) d[k] = (double)expf(st.epsn[k]);
+      for (int k = 0; k < 3; ++k) d[k] = (R)__expf(st.epsn[k]);
       N = m3_mul(m3_mul_at(U, gFp), V);
       if (!kExact) {
         M3d Ud = m3d_from(U), Vd = m3d_from(V), gFpd = m3d_from(gFp);
@@ -490,33 +498,35 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
           }
       }
       if (st.mode != 3) {
-        double ge[3];   // adjoint of post-map eps
-        for (int i = 0; i < 3; ++i) ge[i] = (double)N.a[i * 4] * d[i];
-        double eh[3] = {st.eh[0], st.eh[1], st.eh[2]};
-        double n = st.ehn;
-        double gdot = ge[0] * eh[0] + ge[1] * eh[1] + ge[2] * eh[2];
-        double geps[3], geh[3], gtr = 0.0;
+        R ge[3];   // adjoint of post-map eps
+        for (int i = 0; i < 3; ++i) ge[i] = (R)N.a[i * 4] * d[i];
+        const R eh[3] = {st.eh[0], st.eh[1], st.eh[2]};
+        const R n = st.ehn;
+        const R gdot = ge[0] * eh[0] + ge[1] * eh[1] + ge[2] * eh[2];
+        R geps[3], geh[3], gtr = R(0);
         if (st.mode == 1) {
           // eps' = eps - eh + (c/n) eh, c = tau_y / (2 mu)
-          double c = (double)m.yield_stress / (2.0 * m.mu);
+          const R c = (R)m.yield_stress * rcp_r(R(2) * (R)m.mu);
+          const R rn = rcp_r(n);
           for (int i = 0; i < 3; ++i) {
             geps[i] = ge[i];
-            geh[i] = (c / n - 1.0) * ge[i] - c * gdot / (n * n * n) * eh[i];
+            geh[i] = (c * rn - R(1)) * ge[i] - c * gdot * (rn * rn * rn) * eh[i];
           }
         } else {
           // eps' = eps - eh - r tr eh / n, r = (3 lam + 2 mu) / (2 mu) alpha
-          double r = (3.0 * m.lam + 2.0 * m.mu) / (2.0 * m.mu) * m.dp_alpha;
-          double tr = st.trv;
+          const R r = (R(3) * (R)m.lam + R(2) * (R)m.mu) * rcp_r(R(2) * (R)m.mu) * (R)m.dp_alpha;
+          const R tr = st.trv;
+          const R rn = rcp_r(n);
           for (int i = 0; i < 3; ++i) {
             geps[i] = ge[i];
-            geh[i] = -ge[i] - r * tr * (ge[i] / n - gdot * eh[i] / (n * n * n));
+            geh[i] = -ge[i] - r * tr * (ge[i] * rn - gdot * eh[i] * (rn * rn * rn));
           }
-          gtr = -r * gdot / n;
+          gtr = -r * gdot * rn;
         }
-        double mean = (geh[0] + geh[1] + geh[2]) / 3.0;
+        const R mean = (geh[0] + geh[1] + geh[2]) * (R(1) / R(3));
         for (int i = 0; i < 3; ++i) {
           geps[i] += geh[i] - mean + gtr;
-          if (sig[i] > 0.05f) gsig[i] += geps[i] / (double)st.s[i];
+          if (sig[i] > 0.05f) gsig[i] += geps[i] * rcp_r((R)st.s[i]);
         }
       }
     }
@@ -527,7 +537,7 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
   //     (M_ij - M_ji) / (s_i + s_j), M = U^T gR V; the return-map path
   //     (F_p = U diag(d) V^T, d_k = G s_k^a) as N_ij A_ij + N_ji B_ij with
   //     N = U^T gF_p V, A/B = +-(d_i + d_j) / (2 (s_i + s_j)) + Dd_ij / 2 and
-  //     Dd_ij = (d_j - d_i) / (s_j - s_i) evaluated through expm1/log1p;
+  //     Dd_ij = (d_j - d_i) / (s_j - s_i), near s_i = s_j from a series;
   //   reference form (float64): svd_grad of gU = gR V + gUp, gV = gR^T U + gVp
   //     with the 1e-8 clamp on s_j^2 - s_i^2 (mpm_elastic.py:157-173).
   M3 Mi;
@@ -539,29 +549,35 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
           Mi.a[i * 3 + j] = (float)gsig[i];
           continue;
         }
-        double den = (double)sig[i] + (double)sig[j];
-        if (fabs(den) < 1e-12) den = den < 0.0 ? -1e-12 : 1e-12;
-        double v = ((double)Mr.a[i * 3 + j] - (double)Mr.a[j * 3 + i]) / den;
+        float den = sig[i] + sig[j];
+        if (fabsf(den) < 1e-12f) den = den < 0.f ? -1e-12f : 1e-12f;
+        const float rden = fast_rcp(den);
+        float v = (Mr.a[i * 3 + j] - Mr.a[j * 3 + i]) * rden;
         if (pl) {
           // Dd = (d_j - d_i) / (s_j - s_i) with d_k = G s_k^a: near s_i = s_j
           // from the binomial series of ((1 + x)^a - 1) / x, x = del / s_i
           // (|x| < 0.1, six terms, truncation below 1e-7 relative), else as
           // the plain divided difference of the forward pass's d
-          const double si = sig[i], sj = sig[j], del = sj - si, a = st.pa;
-          double Dd = 0.0;
-          const double x = del / si;
-          if (si > 0.05 && sj > 0.05 && fabs(x) < 0.1) {
+          // (s_j - s_i is exact in float32 for s within a factor 2: Sterbenz)
+          const float si = sig[i], sj = sig[j], del = sj - si, a = st.pa;
+          float Dd = 0.f;
+          const float rsi = fast_rcp(si);
+          const float x = del * rsi;
+          if (si > 0.05f && sj > 0.05f && fabsf(x) < 0.1f) {
             // a (1 + (a-1)/2 x (1 + (a-2)/3 x (1 + ... (a-5)/6 x))), inside out
-            double ser = 0.0;
-            for (int k = 5; k >= 1; --k) ser = (a - k) / (k + 1) * (1.0 + x * ser);
-            Dd = d[i] / si * a * (1.0 + x * ser);
-          } else if (del != 0.0) {
-            Dd = (d[j] - d[i]) / del;
+            constexpr float kInv[7] = {0.f, 1.f, 1.f / 2.f, 1.f / 3.f, 1.f / 4.f, 1.f / 5.f,
+                                       1.f / 6.f};
+            float ser = 0.f;
+#pragma unroll
+            for (int k = 5; k >= 1; --k) ser = (a - k) * kInv[k + 1] * (1.f + x * ser);
+            Dd = (float)d[i] * rsi * a * (1.f + x * ser);
+          } else if (del != 0.f) {
+            Dd = (float)(d[j] - d[i]) * fast_rcp(del);
           }
-          const double hs = 0.5 * (d[i] + d[j]) / den;
-          v += (double)N.a[i * 3 + j] * (hs + 0.5 * Dd) + (double)N.a[j * 3 + i] * (-hs + 0.5 * Dd);
+          const float hs = 0.5f * (float)(d[i] + d[j]) * rden;
+          v += N.a[i * 3 + j] * (hs + 0.5f * Dd) + N.a[j * 3 + i] * (-hs + 0.5f * Dd);
         }
-        Mi.a[i * 3 + j] = (float)v;
+        Mi.a[i * 3 + j] = v;
       }
   } else {
     M3d Ud = m3d_from(U), Vd = m3d_from(V), gRd = m3d_from(gR);
