@@ -78,6 +78,7 @@ _SIGS = {
     "dt_mpm_advance": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 8
                        + [_c_int, _vp, _vp, _vp, _sz, _vp]),
     "dt_mpm_workspace_bytes": (_c_int, [_P(DtMpmParams), _c_int, _P(_sz)]),
+    "dt_mpm_workspace_history_bytes": (_c_int, [_P(DtMpmParams), _c_int, _c_int, _P(_sz)]),
     "dt_mpm_bin": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _sz, _vp]),
     "dt_mpm_bin_perm": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _P(_vp), _P(_vp)]),
     "dt_mpm_p2g_binned": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 5

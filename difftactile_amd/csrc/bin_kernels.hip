@@ -59,9 +59,33 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   return total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bin_zero(long long n, int32_t* __restrict__ a) {
+__global__ __launch_bounds__(kBlock) void k_bin_zero(long long n, int32_t* __restrict__ a,
+                                                     int32_t* __restrict__ one) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
+  if (i == 0) *one = 0;
+}
+
+size_t bin_history_slot_bytes(const dt_mpm_params* prm, int n) {
+  const long long nbk = brick_total(prm);
+  return align256(sizeof(int32_t) * (size_t)n) + align256(sizeof(int32_t) * (size_t)(nbk + 1)) +
+         align256(sizeof(int32_t) * (size_t)nbk) + align256(sizeof(int32_t) * (size_t)nbk) + 256;
+}
+
+BinWs bin_slot(const BinWs& ws, int s, int n) {
+  BinWs v = ws;
+  char* b = ws.hist + (size_t)s * ws.hist_slot_bytes;
+  const long long nbk = ws.n_bricks;
+  v.perm = reinterpret_cast<int32_t*>(b);
+  b += align256(sizeof(int32_t) * (size_t)n);
+  v.start = reinterpret_cast<int32_t*>(b);
+  b += align256(sizeof(int32_t) * (size_t)(nbk + 1));
+  v.list = reinterpret_cast<int32_t*>(b);
+  b += align256(sizeof(int32_t) * (size_t)nbk);
+  v.flags = reinterpret_cast<int32_t*>(b);
+  b += align256(sizeof(int32_t) * (size_t)nbk);
+  v.n_list = reinterpret_cast<int32_t*>(b);
+  return v;
 }
 
 // Active lanes form a prefix of every wavefront (p = block * 256 + thread).
@@ -139,7 +163,8 @@ __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int 
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st) {
   const long long nbk = ws.n_bricks;
   const int nb = (P.n_grid + 3) >> 2;
-  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nbk + 2)), dim3(kBlock), 0, st, nbk + 2, ws.count);
+  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nbk + 2)), dim3(kBlock), 0, st, nbk + 2, ws.count,
+                     ws.n_list);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
                        ws.rank, ws.count);

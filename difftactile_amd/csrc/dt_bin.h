@@ -28,7 +28,18 @@ struct BinWs {
   void* scan_tmp;
   size_t scan_bytes;
   long long n_bricks;
+  // Optional binning history (one slot per substep of a trajectory): the
+  // perm / start / list / n_list / flags of substep s live in slot s so the
+  // adjoint sweep can reuse the forward pass's binning.
+  char* hist = nullptr;
+  size_t hist_slot_bytes = 0;
+  int hist_slots = 0;
 };
+
+// Bytes of one history slot for n particles.
+size_t bin_history_slot_bytes(const dt_mpm_params* prm, int n);
+// View of ws whose per-substep arrays live in history slot s.
+BinWs bin_slot(const BinWs& ws, int s, int n);
 
 long long brick_total(const dt_mpm_params* prm);
 // Total bytes of the workspace for n particles; fills *ws when base != NULL.

@@ -24,6 +24,9 @@
 // lane on gfx950 and dominated the per-particle scatter).  Unsorted input
 // gives runs of one particle and stays correct.
 #include <climits>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "dt_bin.h"
 #include "dt_common.h"
@@ -1479,12 +1482,12 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
                        const float* F, const float* grid4, const float* grid_v4,
                        const float* gx1, const float* gv1, const float* gC1, const float* gF1,
                        float* ggrid4, float* ggv4, float* gx, float* gv, float* gC, float* gF,
-                       hipStream_t st, const BinWs* ws = nullptr) {
+                       hipStream_t st, const BinWs* ws = nullptr, bool binned = false) {
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
   if (ws) {
     int bgrid = (int)((ws->n_bricks + 3) / 4);
-    {
+    if (!binned) {
       StageTimer tm(STAGE_BIN, st);
       int rc = launch_mark_bin(P, prm, n, x, *ws, st);
       if (rc) return rc;
@@ -1517,8 +1520,10 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
 }
 
 // Workspace views for n particles; error when the buffer is missing or short.
+// With history_substeps > 0 and room for that many history slots after the
+// base layout, the views also carry the binning history.
 static int workspace_views(const dt_mpm_params* prm, int n, void* workspace, size_t bytes,
-                           BinWs& ws) {
+                           BinWs& ws, int history_substeps = 0) {
   const size_t need = bin_ws_layout(prm, n, nullptr, nullptr);
   DT_REQUIRE(workspace != nullptr, "NULL workspace");
   if (bytes < need) {
@@ -1528,8 +1533,49 @@ static int workspace_views(const dt_mpm_params* prm, int n, void* workspace, siz
   }
   DT_REQUIRE(((uintptr_t)workspace & 255) == 0, "workspace must be 256-byte aligned");
   bin_ws_layout(prm, n, workspace, &ws);
+  const size_t slot = bin_history_slot_bytes(prm, n);
+  if (history_substeps > 0 && bytes >= need + slot * (size_t)history_substeps) {
+    ws.hist = static_cast<char*>(workspace) + need;
+    ws.hist_slot_bytes = slot;
+    ws.hist_slots = history_substeps;
+  }
   return DT_OK;
 }
+
+// Which trajectory the binning history of a workspace describes: set by a
+// history-keeping dt_mpm_run_fwd, checked by dt_mpm_run_adj.
+namespace {
+struct HistRecord {
+  const void* ws;
+  const float* x;
+  int n, n_substeps;
+};
+std::vector<HistRecord> g_hist;
+std::mutex g_hist_mu;
+
+void hist_set(const void* ws, const float* x, int n, int k) {
+  std::lock_guard<std::mutex> lock(g_hist_mu);
+  for (auto& r : g_hist)
+    if (r.ws == ws) {
+      r = {ws, x, n, k};
+      return;
+    }
+  g_hist.push_back({ws, x, n, k});
+}
+
+bool hist_matches(const void* ws, const float* x, int n, int k) {
+  std::lock_guard<std::mutex> lock(g_hist_mu);
+  for (const auto& r : g_hist)
+    if (r.ws == ws) return r.x == x && r.n == n && r.n_substeps == k;
+  return false;
+}
+
+void hist_clear(const void* ws) {
+  std::lock_guard<std::mutex> lock(g_hist_mu);
+  for (auto& r : g_hist)
+    if (r.ws == ws) r.x = nullptr;
+}
+}  // namespace
 
 }  // namespace dt
 
@@ -1720,17 +1766,21 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
   DT_REQUIRE(n >= 0 && n_substeps >= 0, "n and n_substeps must be >= 0");
-  BinWs ws{};
-  if (workspace && (rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
-  const BinWs* wp = workspace ? &ws : nullptr;
   bool keep = grids4 != nullptr && grids_v4 != nullptr;
   DT_REQUIRE(keep || (scratch4 && scratch_v4), "need kept grids or scratch grids");
+  BinWs ws{};
+  if (workspace &&
+      (rc = workspace_views(prm, n, workspace, workspace_bytes, ws, keep ? n_substeps : 0)))
+    return rc;
+  const bool hist = workspace && keep && ws.hist_slots >= n_substeps && n_substeps > 0;
   long long nn = grid_nodes(prm);
   auto body = [&](hipStream_t st) -> int {
     for (int s = 0; s < n_substeps; ++s) {
       long long o = (long long)s * n, o1 = (long long)(s + 1) * n;
       float* g4 = keep ? grids4 + (long long)s * nn * 4 : scratch4;
       float* gv4 = keep ? grids_v4 + (long long)s * nn * 4 : scratch_v4;
+      BinWs slot = hist ? bin_slot(ws, s, n) : ws;
+      const BinWs* wp = workspace ? &slot : nullptr;
       int r = launch_substep_fwd(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
                                  g4, gv4, x + 3 * o1, v + 3 * o1, C + 9 * o1, F + 9 * o1, st, wp);
       if (r) return r;
@@ -1747,7 +1797,12 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   key_add(key, (long long)n_mats << 32 | (uint32_t)n);
   key_add(key, (long long)n_substeps);
   key_add(key, workspace_bytes);
-  return run_graphed(std::move(key), DT_STREAM(stream), body);
+  rc = run_graphed(std::move(key), DT_STREAM(stream), body);
+  if (workspace) {
+    if (hist && rc == DT_OK) hist_set(workspace, x, n, n_substeps);
+    else hist_clear(workspace);
+  }
+  return rc;
 }
 
 int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
@@ -1761,8 +1816,11 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   if ((rc = check_mats(mats, n_mats))) return rc;
   DT_REQUIRE(n >= 0 && n_substeps >= 0, "n and n_substeps must be >= 0");
   BinWs ws{};
-  if (workspace && (rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
-  const BinWs* wp = workspace ? &ws : nullptr;
+  if (workspace && (rc = workspace_views(prm, n, workspace, workspace_bytes, ws, n_substeps)))
+    return rc;
+  // reuse the forward pass's binning when this workspace holds it for these frames
+  const bool hist = workspace && ws.hist_slots >= n_substeps && n_substeps > 0 &&
+                    hist_matches(workspace, x, n, n_substeps);
   DT_REQUIRE(grids4 && grids_v4 && g_grid4 && g_grid_v4, "NULL grid pointer");
   long long nn = grid_nodes(prm);
   hipStream_t st = DT_STREAM(stream);
@@ -1771,16 +1829,18 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     float *p[4] = {gx, gv, gC, gF}, *q[4] = {gx2, gv2, gC2, gF2};
     for (int s = n_substeps - 1; s >= 0; --s) {
       long long o = (long long)s * n;
+      BinWs slot = hist ? bin_slot(ws, s, n) : ws;
+      const BinWs* wp = workspace ? &slot : nullptr;
       int r = launch_substep_adj(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
                                  grids4 + (long long)s * nn * 4, grids_v4 + (long long)s * nn * 4,
                                  p[0], p[1], p[2], p[3], g_grid4, g_grid_v4, q[0], q[1], q[2], q[3],
-                                 cs, wp);
+                                 cs, wp, hist);
       if (r) return r;
       for (int k = 0; k < 4; ++k) { float* t = p[k]; p[k] = q[k]; q[k] = t; }
     }
     return DT_OK;
   };
-  std::vector<uint64_t> key{2};
+  std::vector<uint64_t> key{hist ? 4u : 2u};
   key_add(key, *prm);
   for (const void* ptr : {(const void*)mats, (const void*)mat_id, (const void*)x, (const void*)v,
                           (const void*)C, (const void*)F, (const void*)grids4,
@@ -1989,6 +2049,16 @@ int dt_mpm_workspace_bytes(const dt_mpm_params* prm, int32_t n, size_t* bytes) {
   DT_REQUIRE(n >= 0, "n must be >= 0");
   DT_REQUIRE(bytes != nullptr, "NULL bytes");
   *bytes = bin_ws_layout(prm, n, nullptr, nullptr);
+  return DT_OK;
+}
+
+int dt_mpm_workspace_history_bytes(const dt_mpm_params* prm, int32_t n, int32_t n_substeps,
+                                   size_t* bytes) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n >= 0 && n_substeps >= 0, "n and n_substeps must be >= 0");
+  DT_REQUIRE(bytes != nullptr, "NULL bytes");
+  *bytes = bin_history_slot_bytes(prm, n) * (size_t)n_substeps;
   return DT_OK;
 }
 

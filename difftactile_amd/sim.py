@@ -23,11 +23,17 @@ from . import _lib
 from .params import material_table
 
 
-def workspace_bytes(c_prm, n):
-    """Device workspace bytes of the sparse path for n particles."""
+def workspace_bytes(c_prm, n, history_substeps=0):
+    """Device workspace bytes of the sparse path for n particles (plus the
+    binning history of a history_substeps trajectory)."""
     out = ctypes.c_size_t(0)
     _lib.check(_lib.lib().dt_mpm_workspace_bytes(c_prm, int(n), ctypes.byref(out)))
-    return int(out.value)
+    total = int(out.value)
+    if history_substeps:
+        _lib.check(_lib.lib().dt_mpm_workspace_history_bytes(c_prm, int(n), int(history_substeps),
+                                                             ctypes.byref(out)))
+        total += int(out.value)
+    return total
 
 
 class MPMSim:
@@ -77,13 +83,18 @@ class MPMSim:
         nn = self.prm.n_nodes
         return self._buf("g4", (nn, 4)), self._buf("gv4", (nn, 4))
 
-    def _workspace(self):
+    def _workspace(self, history_substeps=0):
         """(pointer, bytes) of the sparse-path workspace; (None, 0) selects the
-        dense grid passes."""
+        dense grid passes.  With history_substeps the workspace also holds the
+        binning history of a segment that long (reused by its adjoint sweep)."""
         if not self.sparse_grid:
             return None, 0
-        nbytes = workspace_bytes(self.c_prm, self.n)
-        return _lib.ptr(self._buf("bin_ws", (nbytes,), torch.uint8), torch.uint8), nbytes
+        need = workspace_bytes(self.c_prm, self.n, history_substeps)
+        t = self._ws.get("bin_ws")
+        if t is None or t.numel() < need:
+            t = torch.empty(need, dtype=torch.uint8, device=self.device)
+            self._ws["bin_ws"] = t
+        return _lib.ptr(t, torch.uint8), t.numel()
 
     # -- ordering --------------------------------------------------------
     def sort_order(self, x):
@@ -150,7 +161,8 @@ class MPMSim:
             g4p, gv4p = None, None
         _lib.call("dt_mpm_run_fwd", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
                   self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
-                  g4p, gv4p, _lib.ptr(s4), _lib.ptr(sv4), *self._workspace(), _lib.stream())
+                  g4p, gv4p, _lib.ptr(s4), _lib.ptr(sv4),
+                  *self._workspace(k if keep_grids else 0), _lib.stream())
         return G4, GV4
 
     def advance(self, state, k, mat_id):
@@ -170,8 +182,8 @@ class MPMSim:
         _lib.call("dt_mpm_run_adj", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
                   self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
                   _lib.ptr(G4), _lib.ptr(GV4), _lib.ptr(gg4), _lib.ptr(ggv4),
-                  *[_lib.ptr(t) for t in g], *[_lib.ptr(t) for t in scratch], *self._workspace(),
-                  _lib.stream())
+                  *[_lib.ptr(t) for t in g], *[_lib.ptr(t) for t in scratch],
+                  *self._workspace(k), _lib.stream())
 
     def _frames(self, k, prefix="F"):
         n = self.n

@@ -260,6 +260,13 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
    flags, per-brick counts and starts, occupied-brick list, per-particle key /
    rank / permutation, one 6x6x6-node float4 tile per brick, scan scratch. */
 int dt_mpm_workspace_bytes(const dt_mpm_params* prm, int32_t n, size_t* bytes);
+/* Extra bytes, after dt_mpm_workspace_bytes, for the binning history of an
+   n_substeps trajectory.  A workspace with that room makes a grid-keeping
+   dt_mpm_run_fwd store each substep's binning, and a following dt_mpm_run_adj
+   on the same frames (same x pointer, n and n_substeps, frames unmodified in
+   between) reuses it instead of binning again. */
+int dt_mpm_workspace_history_bytes(const dt_mpm_params* prm, int32_t n, int32_t n_substeps,
+                                   size_t* bytes);
 /* Bins the particles of x by the 4x4x4-cell brick of their stencil base
    (bricks numbered as in dt_mpm_cell_keys, bases clamped to the stencil's
    [0, n_grid-3] range) and marks the active bricks. */

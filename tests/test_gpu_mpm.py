@@ -575,3 +575,56 @@ def test_graph_replays_match_eager_launches(gpu):
     _lib.check(lib.dt_graph_stats(ctypes.byref(h1), ctypes.byref(m1)))
     assert m1.value - m0.value == 2          # one capture each for fwd and adj
     assert h1.value - h0.value == 4          # then replays
+
+
+def test_adjoint_reuses_forward_binning_history(gpu):
+    """A workspace with history room: dt_mpm_run_adj right after the
+    grid-keeping dt_mpm_run_fwd of the same frames reuses the stored binning;
+    after another trajectory was run on the workspace it bins again.  Both
+    match the adjoint of a workspace without history."""
+    from difftactile_amd.sim import workspace_bytes
+    prm, mats, mat_id, x, v, C, F = _facade_inputs("multi", n=3000, seed=41)
+    C = C * 0.1
+    n, nn, k = len(x), prm.n_nodes, 3
+    c_prm = prm.c_struct()
+    mt = material_table(mats, "cuda")
+    mid = dev(mat_id, torch.int32)
+    seeds = [_seed((n,) + a.shape[1:], 600 + i).cuda() for i, a in enumerate((x, v, C, F))]
+
+    def frames_of(xx):
+        fr = [torch.zeros((k + 1,) + a.shape, device="cuda") for a in (xx, v, C, F)]
+        for f, a in zip(fr, (xx, v, C, F)):
+            f[0] = dev(a)
+        return fr, torch.zeros(k, nn, 4, device="cuda"), torch.zeros(k, nn, 4, device="cuda")
+
+    def fwd(fr, G4, GV4, ws, nbytes):
+        call("dt_mpm_run_fwd", c_prm, mt, len(mats), n, mid, *fr, k, G4, GV4, None, None, ws,
+             nbytes, _lib.stream())
+
+    def adj(fr, G4, GV4, ws, nbytes):
+        g = [s_.clone() for s_ in seeds]
+        g2 = [torch.empty_like(s_) for s_ in seeds]
+        gg4 = torch.zeros(nn, 4, device="cuda")
+        ggv4 = torch.zeros(nn, 4, device="cuda")
+        call("dt_mpm_run_adj", c_prm, mt, len(mats), n, mid, *fr, k, G4, GV4, gg4, ggv4, *g, *g2,
+             ws, nbytes, _lib.stream())
+        torch.cuda.synchronize()
+        return g
+
+    big = workspace_bytes(c_prm, n, k)
+    ws_h = torch.zeros(big, dtype=torch.uint8, device="cuda")
+    ws_0, n0 = _workspace(c_prm, n)
+    fr, G4, GV4 = frames_of(x)
+    fwd(fr, G4, GV4, ws_0, n0)
+    want = adj(fr, G4, GV4, ws_0, n0)
+    fr_h, G4h, GV4h = frames_of(x)
+    fwd(fr_h, G4h, GV4h, ws_h, big)
+    got = adj(fr_h, G4h, GV4h, ws_h, big)                 # reuses the history
+    for a, b in zip(got, want):
+        assert rel_l2(host64(a), host64(b)) < 1e-5
+    x2 = x + np.float32(0.5 * prm.dx)
+    fr2, G42, GV42 = frames_of(x2)
+    fwd(fr2, G42, GV42, ws_h, big)                         # history now describes x2
+    got2 = adj(fr_h, G4h, GV4h, ws_h, big)                # must bin again
+    for a, b in zip(got2, want):
+        assert rel_l2(host64(a), host64(b)) < 1e-5
