@@ -144,7 +144,8 @@ DT_HD void st_m3L(float* __restrict__ b, long long p, PL L, const M3& m) {
 // 3x3 SVD: U, V in SO(3), sigma_1 >= sigma_2 >= |sigma_3|, sign on sigma_3.
 //
 // Method: cyclic Jacobi eigen-decomposition of the symmetric S = A^T A with
-// exact Jacobi rotations (fixed sweep count), column sort of B = A V by norm,
+// exact Jacobi rotations (up to 6 sweeps, stopping at convergence), column
+// sort of B = A V by norm,
 // then Givens QR of B; sigma = diag(R).  This is the structure of McAdams et
 // al. 2011 ("Computing the SVD of 3x3 matrices with minimal branching") with
 // exact rotations instead of the quaternion approximations.  S is kept as its
@@ -240,8 +241,14 @@ DT_HD void svd3(const M3& A, M3& Uo, float (&sig)[3], M3& Vo) {
   };
   float s00 = ata(0, 0), s11 = ata(1, 1), s22 = ata(2, 2);
   float s01 = ata(0, 1), s02 = ata(0, 2), s12 = ata(1, 2);
-#pragma unroll
+  // at most 6 sweeps; stop once the off-diagonal part is below float32
+  // resolution of the diagonal (|off|^2 <= 1e-14 |diag|^2): quadratic
+  // convergence makes that 2-4 sweeps
+#pragma unroll 1
   for (int sweep = 0; sweep < 6; ++sweep) {
+    const float off = s01 * s01 + s02 * s02 + s12 * s12;
+    const float dg = s00 * s00 + s11 * s11 + s22 * s22;
+    if (off <= 1e-14f * dg) break;
     jacobi_step(s00, s11, s01, s02, s12, V, 0, 1);   // r = 2: (s_20, s_21)
     jacobi_step(s00, s22, s02, s01, s12, V, 0, 2);   // r = 1: (s_10, s_12)
     jacobi_step(s11, s22, s12, s01, s02, V, 1, 2);   // r = 0: (s_01, s_02)
