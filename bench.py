@@ -276,7 +276,7 @@ def bench_pad_press(args, torch):
     prm, n = c["prm"], len(c["x"])
     T = c["n_substeps"]
     nodes, f2v, tris, layer, od = c["mesh"]
-    sim = MPMSim(prm, c["mats"], n, checkpoint_every=50, sort=False)
+    sim = MPMSim(prm, c["mats"], n, checkpoint_every=50, sort=True)
     pad = Pad(nodes.astype(np.float32), f2v, tris, layer, od, c["pad_mu"], c["pad_lam"],
               c["pad_mass"], prm.dt)
     cs = CoupledSim(sim, [pad], **c["contact"])

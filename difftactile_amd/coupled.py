@@ -6,6 +6,11 @@ order of ``Contact.update`` (``difftactile/tasks/grasp_plastic.py:151-164``):
   zero grid -> p2g (fused compute_new_F/svd/return map) -> pad update ->
   closest triangle -> collision -> grid_op -> g2p -> pad update2
 
+With the object's sparse grid (the default), p2g and the g2p adjoint run as
+the brick-owned binned passes and their tiles are summed into the dense grid
+the contact kernels read; particles are cell-sorted at the start of a
+rollout (undone on output) when the object sorts.
+
 and its adjoint in the order of ``Contact.update_grad`` (167-178), including
 ``clamp_grid`` (191-202).  Pads use the gripper form of update2 (a control
 velocity per frame, ``gripper_fem.py:403-418``), so the gradient with respect
@@ -50,8 +55,6 @@ class Pad:
 
 class CoupledSim:
     def __init__(self, sim: MPMSim, pads, kn, kd, kt, mu_f, norm_eps=1e-11, clamp_grad=True):
-        if sim.sort:
-            raise ValueError("CoupledSim keeps the caller's particle order (create MPMSim with sort=False)")
         self.sim = sim
         self.pads = list(pads)
         self.cp = _lib.DtContactParams()
@@ -81,8 +84,15 @@ class CoupledSim:
         mats, nm = P(sim.mats), len(sim.materials)
         mid = None if mat_id is None else P(mat_id, I32)
         _lib.call("dt_mpm_zero4", nn, P(g4), s)
-        _lib.call("dt_mpm_p2g_fused", prm, mats, nm, sim.n, mid, *[P(t) for t in st], P(g4),
-                  P(nxt[3]), s)
+        if sim.sparse_grid:
+            ws, nb = sim._workspace()
+            _lib.call("dt_mpm_bin", prm, sim.n, P(st[0]), ws, nb, s)
+            _lib.call("dt_mpm_p2g_binned", prm, mats, nm, sim.n, mid, *[P(t) for t in st],
+                      P(nxt[3]), ws, nb, s)
+            _lib.call("dt_mpm_tiles_sum", prm, sim.n, ws, nb, P(g4), s)
+        else:
+            _lib.call("dt_mpm_p2g_fused", prm, mats, nm, sim.n, mid, *[P(t) for t in st], P(g4),
+                      P(nxt[3]), s)
         vel_u = []
         for k, pad in enumerate(self.pads):
             pos, vel, _ = pads_in[k]
@@ -138,8 +148,15 @@ class CoupledSim:
                       P(gvn), P(gvpn), P(gp), P(gvu), P(gvp), P(ge), P(g_cv[k]), P(g_cv1[k]), s)
             g_ext.append(ge)
         _lib.call("dt_mpm_zero4", nn, P(ggv4), s)
-        _lib.call("dt_mpm_g2p4_adj", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]), P(g_next[1]),
-                  P(g_next[2]), P(g[0]), P(ggv4), s)
+        if sim.sparse_grid:
+            ws, nb = sim._workspace()
+            _lib.call("dt_mpm_bin", prm, sim.n, P(st[0]), ws, nb, s)
+            _lib.call("dt_mpm_g2p4_adj_binned", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]),
+                      P(g_next[1]), P(g_next[2]), P(g[0]), ws, nb, s)
+            _lib.call("dt_mpm_tiles_sum", prm, sim.n, ws, nb, P(ggv4), s)
+        else:
+            _lib.call("dt_mpm_g2p4_adj", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]), P(g_next[1]),
+                      P(g_next[2]), P(g[0]), P(ggv4), s)
         _lib.call("dt_mpm_grid_op4_adj", self.c_prm_clamp, P(g4), P(gf), P(ggv4), P(gg4), P(ggf), s)
         if self.clamp_grad:
             for k in range(len(self.pads)):
@@ -182,7 +199,12 @@ class _CoupledRollout(torch.autograd.Function):
         cvs = [t.contiguous() for t in flat[4 + 3 * npd:]]
         shapes = [t.shape for t in (x, v, C, F)]
         W = (3, 3, 9, 9)
-        st = [sim._gather(None, t, w) for t, w in zip((x, v, C, F), W)]
+        perm = sim.sort_order(x)[1] if (sim.sort and sim.n > 0) else None
+        st = [sim._gather(perm, t, w) for t, w in zip((x, v, C, F), W)]
+        if sim.mat_id is not None and perm is not None:
+            mat_id = sim._gather_i(perm, sim.mat_id)
+        else:
+            mat_id = sim.mat_id
         nxt = [torch.empty_like(t) for t in st]
         pst = pads0
         pnx = [[torch.empty_like(t) for t in p] for p in pst]
@@ -192,14 +214,15 @@ class _CoupledRollout(torch.autograd.Function):
             ckpts.append(([t.clone() for t in st], [[t.clone() for t in p] for p in pst]))
             for _ in range(k):
                 cs._fwd(st, nxt, pst, pnx, [c[step] for c in cvs], [c[step + 1] for c in cvs],
-                        sim.mat_id)
+                        mat_id)
                 st, nxt = nxt, st
                 pst, pnx = pnx, pst
                 step += 1
-        out = [sim._scatter(None, t, w, sh) for t, w, sh in zip(st, W, shapes)]
+        out = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(st, W, shapes)]
         for p in pst:
             out += [t.clone() for t in p]
         ctx.cs, ctx.n_substeps, ctx.ckpts, ctx.shapes = cs, n_substeps, ckpts, shapes
+        ctx.perm, ctx.mat_id = perm, mat_id
         ctx.cv_shapes = [c.shape for c in cvs]
         ctx.save_for_backward(*cvs)
         return tuple(out)
@@ -212,7 +235,7 @@ class _CoupledRollout(torch.autograd.Function):
         cvs = ctx.saved_tensors
         W = (3, 3, 9, 9)
         dev = sim.device
-        g = [sim._gather(None, (t if t is not None else torch.zeros(sh, device=dev)).float(), w)
+        g = [sim._gather(ctx.perm, (t if t is not None else torch.zeros(sh, device=dev)).float(), w)
              for t, w, sh in zip(gout[:4], W, ctx.shapes)]
         gp = []
         for k in range(npd):
@@ -238,17 +261,17 @@ class _CoupledRollout(torch.autograd.Function):
                 pn = [[torch.empty_like(t) for t in p] for p in pframes[-1]]
                 f = s0 + j
                 cs._fwd(frames[-1], nxt, pframes[-1], pn, [c[f] for c in cvs],
-                        [c[f + 1] for c in cvs], sim.mat_id, store)
+                        [c[f + 1] for c in cvs], ctx.mat_id, store)
                 frames.append(nxt)
                 pframes.append(pn)
                 stores.append(store)
             for j in reversed(range(k)):
                 f = s0 + j
                 cs._adj(frames[j], pframes[j], stores[j], g, gp, gbuf, gpbuf,
-                        [gc[f] for gc in g_cv], [gc[f + 1] for gc in g_cv], sim.mat_id)
+                        [gc[f] for gc in g_cv], [gc[f + 1] for gc in g_cv], ctx.mat_id)
                 g, gbuf = gbuf, g
                 gp, gpbuf = gpbuf, gp
-        res = [sim._scatter(None, t, w, sh) for t, w, sh in zip(g, W, ctx.shapes)]
+        res = [sim._scatter(ctx.perm, t, w, sh) for t, w, sh in zip(g, W, ctx.shapes)]
         for p in gp:
             res += list(p)
         res += g_cv

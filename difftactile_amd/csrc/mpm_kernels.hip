@@ -912,6 +912,16 @@ __device__ __forceinline__ float4 tile_sum(const MpmP& P, long long nd,
   return r;
 }
 
+// node values of the active bricks summed from the tiles (no grid_op)
+__global__ __launch_bounds__(kBlock) void k_tiles_sum(MpmP P, const int32_t* __restrict__ flags,
+                                                      const float4* __restrict__ tiles,
+                                                      const int32_t* __restrict__ start,
+                                                      float4* __restrict__ out4) {
+  long long nd = brick_node(P, flags);
+  if (nd < 0) return;
+  out4[nd] = tile_sum(P, nd, tiles, start);
+}
+
 // grid_op over the active bricks with (mv, m) summed from the p2g tiles;
 // writes the summed grid4 and grid_v4
 __global__ __launch_bounds__(kBlock) void k_grid_op4_tiles(MpmP P, const int32_t* __restrict__ flags,
@@ -2114,6 +2124,19 @@ int dt_mpm_grid_op4_tiles(const dt_mpm_params* prm, int32_t n, const void* works
                      (const float4*)ws.tiles, (const int32_t*)ws.start, (float4*)grid4,
                      (float4*)grid_v4);
   return check_launch("dt_mpm_grid_op4_tiles");
+}
+
+int dt_mpm_tiles_sum(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                     size_t workspace_bytes, float* out4, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(out4, "NULL grid pointer");
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
+  hipLaunchKernelGGL(k_tiles_sum, dim3((unsigned)((ws.n_bricks + 3) / 4)), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.flags,
+                     (const float4*)ws.tiles, (const int32_t*)ws.start, (float4*)out4);
+  return check_launch("dt_mpm_tiles_sum");
 }
 
 int dt_mpm_g2p4_adj_binned(const dt_mpm_params* prm, int32_t n, const float* x,

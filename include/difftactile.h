@@ -287,6 +287,10 @@ int dt_mpm_p2g_binned(const dt_mpm_params* prm, const dt_material* mats, int32_t
    grid_op there (grid_v4). */
 int dt_mpm_grid_op4_tiles(const dt_mpm_params* prm, int32_t n, const void* workspace,
                           size_t workspace_bytes, float* grid4, float* grid_v4, void* stream);
+/* Writes the tile sums of the last brick pass (p2g: (mv, m); g2p adjoint:
+   (d/d v, 0)) into out4 on the active bricks; other nodes are untouched. */
+int dt_mpm_tiles_sum(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                     size_t workspace_bytes, float* out4, void* stream);
 /* Brick-owned g2p adjoint over the binned particles: OVERWRITES gx with its
    partial and writes the grid-velocity adjoint tiles. */
 int dt_mpm_g2p4_adj_binned(const dt_mpm_params* prm, int32_t n, const float* x,

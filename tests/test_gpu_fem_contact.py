@@ -233,9 +233,12 @@ def test_coupled_substep_forward_and_adjoint_match_oracle(gpu):
         assert err < 1e-4, (k, err)
 
 
-def test_coupled_rollout_gradient_matches_oracle(gpu):
+@pytest.mark.parametrize("sort,sparse", [(False, False), (False, True), (True, True)])
+def test_coupled_rollout_gradient_matches_oracle(gpu, sort, sparse):
     """Fused coupled rollout (object + pad + contact, checkpointed) vs the oracle's
-    reverse mode over the same composition, including d/d(pad velocity sequence)."""
+    reverse mode over the same composition, including d/d(pad velocity sequence);
+    dense object stages, binned brick passes on the caller's order, and binned
+    passes on cell-sorted particles."""
     from difftactile_amd.coupled import CoupledSim, Pad
     from difftactile_amd.params import Material
     from difftactile_amd.sim import MPMSim
@@ -245,7 +248,7 @@ def test_coupled_rollout_gradient_matches_oracle(gpu):
     steps = 6
     mu, lam = 1e5 / 2 / 1.45, 1e5 * 0.45 / 1.45 / 0.1
     mass = 2e-4
-    sim = MPMSim(prm, [Material(5e4, 0.3)], n, checkpoint_every=4, sort=False)
+    sim = MPMSim(prm, [Material(5e4, 0.3)], n, checkpoint_every=4, sort=sort, sparse_grid=sparse)
     pad = Pad(nodes.astype(np.float32), f2v, surf, layer, od, mu, lam, mass, prm.dt)
     cs = CoupledSim(sim, [pad], kn=1e3, kd=10.0, kt=50.0, mu_f=0.5, clamp_grad=False)
     rng = np.random.default_rng(11)
