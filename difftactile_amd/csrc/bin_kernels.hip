@@ -27,7 +27,7 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   size_t scan_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
                                    (int32_t*)nullptr, (int)(nbk + 1));
-  const size_t sz[9] = {sizeof(int32_t) * (size_t)nbk,
+  const size_t sz[11] = {sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t) * (size_t)(nbk + 2),
                         sizeof(int32_t) * (size_t)(nbk + 1),
                         sizeof(int32_t) * (size_t)nbk,
@@ -35,9 +35,11 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
                         sizeof(int32_t) * (size_t)n,
                         sizeof(int32_t) * (size_t)n,
                         sizeof(float4) * (size_t)nbk * kTileNodes,
-                        scan_bytes};
-  size_t off[9], total = 0;
-  for (int i = 0; i < 9; ++i) {
+                        scan_bytes,
+                        sizeof(int32_t) * (size_t)nbk,
+                        sizeof(int32_t)};
+  size_t off[11], total = 0;
+  for (int i = 0; i < 11; ++i) {
     off[i] = total;
     total += align256(sz[i] ? sz[i] : 1);
   }
@@ -53,6 +55,8 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
     ws->perm = reinterpret_cast<int32_t*>(b + off[6]);
     ws->tiles = reinterpret_cast<float4*>(b + off[7]);
     ws->scan_tmp = b + off[8];
+    ws->alist = reinterpret_cast<int32_t*>(b + off[9]);
+    ws->n_alist = reinterpret_cast<int32_t*>(b + off[10]);
     ws->scan_bytes = scan_bytes;
     ws->n_bricks = nbk;
   }
@@ -60,16 +64,20 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_bin_zero(long long n, int32_t* __restrict__ a,
-                                                     int32_t* __restrict__ one) {
+                                                     int32_t* __restrict__ one,
+                                                     int32_t* __restrict__ two) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
-  if (i == 0) *one = 0;
+  if (i == 0) {
+    *one = 0;
+    *two = 0;
+  }
 }
 
 size_t bin_history_slot_bytes(const dt_mpm_params* prm, int n) {
   const long long nbk = brick_total(prm);
   return align256(sizeof(int32_t) * (size_t)n) + align256(sizeof(int32_t) * (size_t)(nbk + 1)) +
-         align256(sizeof(int32_t) * (size_t)nbk) + align256(sizeof(int32_t) * (size_t)nbk) + 256;
+         3 * align256(sizeof(int32_t) * (size_t)nbk) + 512;
 }
 
 BinWs bin_slot(const BinWs& ws, int s, int n) {
@@ -84,7 +92,10 @@ BinWs bin_slot(const BinWs& ws, int s, int n) {
   b += align256(sizeof(int32_t) * (size_t)nbk);
   v.flags = reinterpret_cast<int32_t*>(b);
   b += align256(sizeof(int32_t) * (size_t)nbk);
+  v.alist = reinterpret_cast<int32_t*>(b);
+  b += align256(sizeof(int32_t) * (size_t)nbk);
   v.n_list = reinterpret_cast<int32_t*>(b);
+  v.n_alist = reinterpret_cast<int32_t*>(b + 256);
   return v;
 }
 
@@ -129,16 +140,31 @@ __global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __res
   if (p < n) perm[start[key[p]] + rank[p]] = p;
 }
 
+// Appends the lanes with pred set to list (one atomic per wavefront).
+__device__ __forceinline__ void wave_append(bool pred, int32_t value, int32_t* __restrict__ list,
+                                            int32_t* __restrict__ n_list) {
+  const unsigned long long m = __ballot(pred);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(n_list, __popcll(m));
+  base = __shfl(base, leader);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
 // Per brick: flags = the brick or one of its -1 neighbours holds particles (a
-// stencil reaches at most one brick past its base's); list += occupied bricks
-// (one atomic per wavefront).
+// stencil reaches at most one brick past its base's); list += occupied
+// bricks, alist += flagged bricks.
 __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int nb,
                                                         const int32_t* __restrict__ count,
                                                         int32_t* __restrict__ flags,
                                                         int32_t* __restrict__ list,
-                                                        int32_t* __restrict__ n_list) {
+                                                        int32_t* __restrict__ n_list,
+                                                        int32_t* __restrict__ alist,
+                                                        int32_t* __restrict__ n_alist) {
   const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool occ = false;
+  bool occ = false, act = false;
   if (b < n_bricks) {
     const int bk = (int)(b % nb), bj = (int)((b / nb) % nb), bi = (int)(b / ((long long)nb * nb));
     int any = 0;
@@ -147,24 +173,19 @@ __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int 
       const int i = bi - (d >> 2), j = bj - ((d >> 1) & 1), k = bk - (d & 1);
       if (i >= 0 && j >= 0 && k >= 0) any |= count[((long long)i * nb + j) * nb + k];
     }
-    flags[b] = any != 0 ? 1 : 0;
+    act = any != 0;
+    flags[b] = act ? 1 : 0;
     occ = count[b] > 0;
   }
-  const unsigned long long m = __ballot(occ);
-  if (m == 0ull) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((long long)m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(n_list, __popcll(m));
-  base = __shfl(base, leader);
-  if (occ) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)b;
+  wave_append(occ, (int32_t)b, list, n_list);
+  wave_append(act, (int32_t)b, alist, n_alist);
 }
 
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st) {
   const long long nbk = ws.n_bricks;
   const int nb = (P.n_grid + 3) >> 2;
   hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nbk + 2)), dim3(kBlock), 0, st, nbk + 2, ws.count,
-                     ws.n_list);
+                     ws.n_list, ws.n_alist);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
                        ws.rank, ws.count);
@@ -175,7 +196,8 @@ int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_
     return DT_ELAUNCH;
   }
   hipLaunchKernelGGL(k_brick_flags, dim3(nblocks(nbk)), dim3(kBlock), 0, st, nbk, nb,
-                     (const int32_t*)ws.count, ws.flags, ws.list, ws.n_list);
+                     (const int32_t*)ws.count, ws.flags, ws.list, ws.n_list, ws.alist,
+                     ws.n_alist);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
                        ws.start, ws.perm);

@@ -21,6 +21,8 @@ struct BinWs {
   int32_t* n_list;  // = count + n_bricks + 1
   int32_t* start;   // [n_bricks + 1] exclusive prefix sum of count
   int32_t* list;    // [n_bricks]     occupied bricks (any order)
+  int32_t* alist;   // [n_bricks]     active (flagged) bricks (any order)
+  int32_t* n_alist; // count of alist
   int32_t* key;     // [n]            brick of each particle
   int32_t* rank;    // [n]            position of the particle inside its brick
   int32_t* perm;    // [n]            particles in brick order
@@ -29,7 +31,7 @@ struct BinWs {
   size_t scan_bytes;
   long long n_bricks;
   // Optional binning history (one slot per substep of a trajectory): the
-  // perm / start / list / n_list / flags of substep s live in slot s so the
+  // perm / start / list / n_list / flags / alist / n_alist of substep s live in slot s so the
   // adjoint sweep can reuse the forward pass's binning.
   char* hist = nullptr;
   size_t hist_slot_bytes = 0;

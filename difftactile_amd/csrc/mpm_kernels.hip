@@ -912,64 +912,84 @@ __device__ __forceinline__ float4 tile_sum(const MpmP& P, long long nd,
   return r;
 }
 
+// Node of lane threadIdx&63 in active brick b (-1 outside the grid).
+__device__ __forceinline__ long long active_node(const MpmP& P, int b) {
+  const int n = P.n_grid, nb = (n + 3) >> 2;
+  const int l = threadIdx.x & 63;
+  const int bz = b % nb, by = (b / nb) % nb, bx = b / (nb * nb);
+  const int i = bx * 4 + (l >> 4), j = by * 4 + ((l >> 2) & 3), k = bz * 4 + (l & 3);
+  if (i >= n || j >= n || k >= n) return -1;
+  return ((long long)i * n + j) * n + k;
+}
+
+// Grid passes over the active-brick list: a persistent grid, 4 bricks (one per
+// wavefront) per block and iteration.
+#define DT_ACTIVE_NODES(nd)                                                          \
+  const int n_act = *n_alist;                                                        \
+  for (int ai = blockIdx.x * 4 + (threadIdx.x >> 6); ai < n_act; ai += gridDim.x * 4) \
+    for (long long nd = active_node(P, alist[ai]); nd >= 0; nd = -1)
+
 // node values of the active bricks summed from the tiles (no grid_op)
-__global__ __launch_bounds__(kBlock) void k_tiles_sum(MpmP P, const int32_t* __restrict__ flags,
+__global__ __launch_bounds__(kBlock) void k_tiles_sum(MpmP P, const int32_t* __restrict__ alist,
+                                                      const int32_t* __restrict__ n_alist,
                                                       const float4* __restrict__ tiles,
                                                       const int32_t* __restrict__ start,
                                                       float4* __restrict__ out4) {
-  long long nd = brick_node(P, flags);
-  if (nd < 0) return;
-  out4[nd] = tile_sum(P, nd, tiles, start);
+  DT_ACTIVE_NODES(nd) { out4[nd] = tile_sum(P, nd, tiles, start); }
 }
 
 // grid_op over the active bricks with (mv, m) summed from the p2g tiles;
 // writes the summed grid4 and grid_v4
-__global__ __launch_bounds__(kBlock) void k_grid_op4_tiles(MpmP P, const int32_t* __restrict__ flags,
+__global__ __launch_bounds__(kBlock) void k_grid_op4_tiles(MpmP P, const int32_t* __restrict__ alist,
+                                                           const int32_t* __restrict__ n_alist,
                                                            const float4* __restrict__ tiles,
                                                            const int32_t* __restrict__ start,
                                                            float4* __restrict__ g4,
                                                            float4* __restrict__ v4) {
-  long long nd = brick_node(P, flags);
-  if (nd < 0) return;
-  const float4 q = tile_sum(P, nd, tiles, start);
-  g4[nd] = q;
-  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (q.w > P.eps) {
-    float vo[3];
-    bool mask[3];
-    grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, vo, mask);
-    out = make_float4(vo[0], vo[1], vo[2], 0.f);
+  DT_ACTIVE_NODES(nd) {
+    const float4 q = tile_sum(P, nd, tiles, start);
+    g4[nd] = q;
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q.w > P.eps) {
+      float vo[3];
+      bool mask[3];
+      grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, vo, mask);
+      out = make_float4(vo[0], vo[1], vo[2], 0.f);
+    }
+    v4[nd] = out;
   }
-  v4[nd] = out;
 }
 
 // grid_op adjoint over the active bricks with d/d grid_v_out summed from the
 // g2p-adjoint tiles; writes the summed gv4 and gg4
 __global__ __launch_bounds__(kBlock) void k_grid_op4_adj_tiles(MpmP P,
-                                                               const int32_t* __restrict__ flags,
+                                                               const int32_t* __restrict__ alist,
+                                                               const int32_t* __restrict__ n_alist,
                                                                const float4* __restrict__ g4,
                                                                const float4* __restrict__ tiles,
                                                                const int32_t* __restrict__ start,
                                                                float4* __restrict__ gv4,
                                                                float4* __restrict__ gg4) {
-  long long nd = brick_node(P, flags);
-  if (nd < 0) return;
-  const float4 gi = tile_sum(P, nd, tiles, start);
-  gv4[nd] = gi;
-  const float4 q = g4[nd];
-  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (q.w > P.eps) {
-    float vo[3];
-    bool mask[3];
-    grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, vo, mask);
-    float g0 = mask[0] ? 0.f : gi.x, g1 = mask[1] ? 0.f : gi.y, g2 = mask[2] ? 0.f : gi.z;
-    float inv_m = 1.f / (q.w + P.eps);
-    float gmm = -inv_m * inv_m * (g0 * q.x + g1 * q.y + g2 * q.z);
-    if (P.clamp_grad) gmm = fminf(fmaxf(gmm, -P.grad_clamp), P.grad_clamp);
-    out = make_float4(inv_m * g0, inv_m * g1, inv_m * g2, gmm);
+  DT_ACTIVE_NODES(nd) {
+    const float4 gi = tile_sum(P, nd, tiles, start);
+    gv4[nd] = gi;
+    const float4 q = g4[nd];
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q.w > P.eps) {
+      float vo[3];
+      bool mask[3];
+      grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, 0.f, 0.f, 0.f, vo, mask);
+      float g0 = mask[0] ? 0.f : gi.x, g1 = mask[1] ? 0.f : gi.y, g2 = mask[2] ? 0.f : gi.z;
+      float inv_m = 1.f / (q.w + P.eps);
+      float gmm = -inv_m * inv_m * (g0 * q.x + g1 * q.y + g2 * q.z);
+      if (P.clamp_grad) gmm = fminf(fmaxf(gmm, -P.grad_clamp), P.grad_clamp);
+      out = make_float4(inv_m * g0, inv_m * g1, inv_m * g2, gmm);
+    }
+    gg4[nd] = out;
   }
-  gg4[nd] = out;
 }
+
+#undef DT_ACTIVE_NODES
 
 __global__ __launch_bounds__(kBlock) void k_zero_i32(long long n, int32_t* __restrict__ a) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1434,6 +1454,11 @@ static void launch_p2g_fused_adj(const MpmP& P, hipStream_t st, const dt_materia
                        mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF);
 }
 
+static unsigned active_blocks(const BinWs& ws) {
+  const long long b = (ws.n_bricks + 3) / 4;
+  return (unsigned)(b < 1024 ? b : 1024);
+}
+
 static unsigned brick_blocks(const BinWs& ws) {
   return (unsigned)(ws.n_bricks < kBrickBlocks ? ws.n_bricks : kBrickBlocks);
 }
@@ -1453,7 +1478,6 @@ int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
   if (ws) {
-    int bgrid = (int)((ws->n_bricks + 3) / 4);
     {
       StageTimer tm(STAGE_BIN, st);
       int rc = launch_mark_bin(P, prm, n, x, *ws, st);
@@ -1467,9 +1491,10 @@ int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
                        (const int32_t*)ws->n_list, ws->tiles);
     }
     StageTimer tm(STAGE_GRID_OP, st);
-    hipLaunchKernelGGL(k_grid_op4_tiles, dim3(bgrid), dim3(kBlock), 0, st, P,
-                       (const int32_t*)ws->flags, (const float4*)ws->tiles,
-                       (const int32_t*)ws->start, (float4*)grid4, (float4*)grid_v4);
+    hipLaunchKernelGGL(k_grid_op4_tiles, dim3(active_blocks(*ws)), dim3(kBlock), 0, st, P,
+                       (const int32_t*)ws->alist, (const int32_t*)ws->n_alist,
+                       (const float4*)ws->tiles, (const int32_t*)ws->start, (float4*)grid4,
+                       (float4*)grid_v4);
   } else {
     hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)grid4);
     Grid g{grid4, grid4 + 3, 4, 4};
@@ -1496,7 +1521,6 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
   if (ws) {
-    int bgrid = (int)((ws->n_bricks + 3) / 4);
     if (!binned) {
       StageTimer tm(STAGE_BIN, st);
       int rc = launch_mark_bin(P, prm, n, x, *ws, st);
@@ -1510,8 +1534,9 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
                        (const int32_t*)ws->n_list, ws->tiles);
     }
     StageTimer tm(STAGE_GRID_OP_ADJ, st);
-    hipLaunchKernelGGL(k_grid_op4_adj_tiles, dim3(bgrid), dim3(kBlock), 0, st, P,
-                       (const int32_t*)ws->flags, (const float4*)grid4, (const float4*)ws->tiles,
+    hipLaunchKernelGGL(k_grid_op4_adj_tiles, dim3(active_blocks(*ws)), dim3(kBlock), 0, st, P,
+                       (const int32_t*)ws->alist, (const int32_t*)ws->n_alist,
+                       (const float4*)grid4, (const float4*)ws->tiles,
                        (const int32_t*)ws->start, (float4*)ggv4, (float4*)ggrid4);
   } else {
     hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)ggv4);
@@ -2119,10 +2144,10 @@ int dt_mpm_grid_op4_tiles(const dt_mpm_params* prm, int32_t n, const void* works
   DT_REQUIRE(grid4 && grid_v4, "NULL grid pointer");
   BinWs ws{};
   if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
-  hipLaunchKernelGGL(k_grid_op4_tiles, dim3((unsigned)((ws.n_bricks + 3) / 4)), dim3(kBlock), 0,
-                     DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.flags,
-                     (const float4*)ws.tiles, (const int32_t*)ws.start, (float4*)grid4,
-                     (float4*)grid_v4);
+  hipLaunchKernelGGL(k_grid_op4_tiles, dim3(active_blocks(ws)), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.alist,
+                     (const int32_t*)ws.n_alist, (const float4*)ws.tiles,
+                     (const int32_t*)ws.start, (float4*)grid4, (float4*)grid_v4);
   return check_launch("dt_mpm_grid_op4_tiles");
 }
 
@@ -2133,8 +2158,8 @@ int dt_mpm_tiles_sum(const dt_mpm_params* prm, int32_t n, const void* workspace,
   DT_REQUIRE(out4, "NULL grid pointer");
   BinWs ws{};
   if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
-  hipLaunchKernelGGL(k_tiles_sum, dim3((unsigned)((ws.n_bricks + 3) / 4)), dim3(kBlock), 0,
-                     DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.flags,
+  hipLaunchKernelGGL(k_tiles_sum, dim3(active_blocks(ws)), dim3(kBlock), 0, DT_STREAM(stream),
+                     to_dev(*prm), (const int32_t*)ws.alist, (const int32_t*)ws.n_alist,
                      (const float4*)ws.tiles, (const int32_t*)ws.start, (float4*)out4);
   return check_launch("dt_mpm_tiles_sum");
 }
@@ -2165,10 +2190,10 @@ int dt_mpm_grid_op4_adj_tiles(const dt_mpm_params* prm, int32_t n, const void* w
   DT_REQUIRE(grid4 && g_grid_v4 && g_grid4, "NULL grid pointer");
   BinWs ws{};
   if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
-  hipLaunchKernelGGL(k_grid_op4_adj_tiles, dim3((unsigned)((ws.n_bricks + 3) / 4)), dim3(kBlock),
-                     0, DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.flags,
-                     (const float4*)grid4, (const float4*)ws.tiles, (const int32_t*)ws.start,
-                     (float4*)g_grid_v4, (float4*)g_grid4);
+  hipLaunchKernelGGL(k_grid_op4_adj_tiles, dim3(active_blocks(ws)), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.alist,
+                     (const int32_t*)ws.n_alist, (const float4*)grid4, (const float4*)ws.tiles,
+                     (const int32_t*)ws.start, (float4*)g_grid_v4, (float4*)g_grid4);
   return check_launch("dt_mpm_grid_op4_adj_tiles");
 }
 
