@@ -104,6 +104,8 @@ _SIGS = {
     "dt_mpm_cell_keys": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp]),
     "dt_fem_init_B": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
     "dt_fem_update": (_c_int, [_P(DtFemParams), _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_fem_update_csr": (_c_int, [_P(DtFemParams), _c_int, _c_int] + [_vp] * 7 + [_vp]),
+    "dt_fem_update_adj_csr": (_c_int, [_P(DtFemParams), _c_int, _c_int] + [_vp] * 8 + [_vp]),
     "dt_fem_update_adj": (_c_int, [_P(DtFemParams), _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dt_fem_update2": (_c_int, [_P(DtFemParams), _c_int] + [_vp] * 10 + [_vp]),
     "dt_fem_update2_adj": (_c_int, [_P(DtFemParams), _c_int] + [_vp] * 10 + [_vp]),

@@ -29,6 +29,16 @@ P = _lib.ptr
 I32 = torch.int32
 
 
+def vertex_tet_csr(f2v, n_verts):
+    """(start[n_verts+1], entries[4 n_tets]) with entries 4*tet + corner grouped
+    by vertex (stable in tet order)."""
+    f2v = np.asarray(f2v, dtype=np.int64).reshape(-1, 4)
+    key = f2v.reshape(-1)
+    order = np.argsort(key, kind="stable")
+    start = np.searchsorted(key[order], np.arange(n_verts + 1), side="left").astype(np.int32)
+    return start, order.astype(np.int32)
+
+
 class Pad:
     """Static data of one FEM pad on the GPU (mesh, rest shapes, material)."""
 
@@ -48,6 +58,11 @@ class Pad:
         self.fem = _lib.DtFemParams()
         self.fem.mu, self.fem.lam, self.fem.dt, self.fem.mass = mu, lam, dt, mass
         self.cen = torch.empty(self.n_tris, 4, device=self.device)
+        # vertex -> (tet, corner) CSR for the atomic-free tet-force passes
+        vs, vt = vertex_tet_csr(np.asarray(f2v), self.n_verts)
+        self.vert_start = torch.as_tensor(vs, device=self.device)
+        self.vert_tets = torch.as_tensor(vt, device=self.device)
+        self.tet_buf = torch.empty(self.n_tets, 4, 4, device=self.device)
 
     def od(self):
         return self.out_dir.ctypes.data_as(_lib.ctypes.POINTER(_lib.ctypes.c_float))
@@ -98,8 +113,8 @@ class CoupledSim:
             pos, vel, _ = pads_in[k]
             vu = store["vel_u"][k] if store else torch.empty_like(vel)
             vu.copy_(vel)
-            _lib.call("dt_fem_update", pad.fem, pad.n_tets, P(pad.f2v, I32), P(pad.B), P(pos),
-                      P(vu), s)
+            _lib.call("dt_fem_update_csr", pad.fem, pad.n_tets, pad.n_verts, pad.f2v, pad.B,
+                      pos, vu, pad.vert_start, pad.vert_tets, pad.tet_buf, s)
             vel_u.append(vu)
         idx.fill_(-1)
         for k, pad in enumerate(self.pads):
@@ -168,8 +183,9 @@ class CoupledSim:
                       P(store["vel_u"][k]), pad.od(), P(ggf), P(g_ext[k]), P(gg4), 4,
                       gg4.data_ptr() + 12, 4, P(g_pads[k][0]), P(g_pads[k][1]), s)
         for k, pad in enumerate(self.pads):
-            _lib.call("dt_fem_update_adj", pad.fem, pad.n_tets, P(pad.f2v, I32), P(pad.B),
-                      P(pads_in[k][0]), P(g_pads[k][1]), P(g_pads[k][0]), s)
+            _lib.call("dt_fem_update_adj_csr", pad.fem, pad.n_tets, pad.n_verts, pad.f2v, pad.B,
+                      pads_in[k][0], g_pads[k][1], g_pads[k][0], pad.vert_start, pad.vert_tets,
+                      pad.tet_buf, s)
         _lib.call("dt_mpm_p2g_fused_adj", prm, mats, nm, sim.n, mid, *[P(t) for t in st], P(gg4),
                   P(g_next[3]), P(g[0]), P(g[1]), P(g[2]), P(g[3]), s)
 

@@ -75,11 +75,16 @@ __device__ __forceinline__ void tet_fwd(const FemP& P, const float (&p)[4][3], c
   for (int i = 0; i < 9; ++i) t.P.a[i] = a * t.F.a[i] + b * t.dJdF.a[i];
 }
 
+// kOut == 0: scatter into vel with float atomics (the reference's vel[v] +=);
+// kOut == 1: write the tet's four vertex contributions to tbuf[4 t + k] for the
+// per-vertex gather k_fem_vertex_sum (no atomics, fixed summation order).
+template <int kOut>
 __global__ __launch_bounds__(kBlock) void k_fem_update(FemP P, int nt,
                                                        const int32_t* __restrict__ f2v,
                                                        const float* __restrict__ B,
                                                        const float* __restrict__ pos,
-                                                       float* __restrict__ vel) {
+                                                       float* __restrict__ vel,
+                                                       float4* __restrict__ tbuf) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   int iv[4] = {f2v[4 * t], f2v[4 * t + 1], f2v[4 * t + 2], f2v[4 * t + 3]};
@@ -91,25 +96,57 @@ __global__ __launch_bounds__(kBlock) void k_fem_update(FemP P, int nt,
   // H = -V P B^T; column k is the force on vertex k
   M3 H = m3_mul_bt(tf.P, Bm);
   float s = -tf.vol * P.dt / P.mass;
+  float f3x = 0.f, f3y = 0.f, f3z = 0.f;
   for (int k = 0; k < 3; ++k) {
     float fx = s * H.a[0 * 3 + k], fy = s * H.a[1 * 3 + k], fz = s * H.a[2 * 3 + k];
-    atomic_add_f(vel + 3ll * iv[k] + 0, fx);
-    atomic_add_f(vel + 3ll * iv[k] + 1, fy);
-    atomic_add_f(vel + 3ll * iv[k] + 2, fz);
-    atomic_add_f(vel + 3ll * iv[3] + 0, -fx);
-    atomic_add_f(vel + 3ll * iv[3] + 1, -fy);
-    atomic_add_f(vel + 3ll * iv[3] + 2, -fz);
+    if (kOut == 0) {
+      atomic_add_f(vel + 3ll * iv[k] + 0, fx);
+      atomic_add_f(vel + 3ll * iv[k] + 1, fy);
+      atomic_add_f(vel + 3ll * iv[k] + 2, fz);
+      atomic_add_f(vel + 3ll * iv[3] + 0, -fx);
+      atomic_add_f(vel + 3ll * iv[3] + 1, -fy);
+      atomic_add_f(vel + 3ll * iv[3] + 2, -fz);
+    } else {
+      tbuf[4ll * t + k] = make_float4(fx, fy, fz, 0.f);
+      f3x -= fx;
+      f3y -= fy;
+      f3z -= fz;
+    }
   }
+  if (kOut == 1) tbuf[4ll * t + 3] = make_float4(f3x, f3y, f3z, 0.f);
+}
+
+// out[v] += sum of tbuf[e] over the vertex's incident (tet, corner) entries e,
+// in the order of vert_tets (CSR by vertex)
+__global__ __launch_bounds__(kBlock) void k_fem_vertex_sum(int nv,
+                                                           const int32_t* __restrict__ vstart,
+                                                           const int32_t* __restrict__ vtets,
+                                                           const float4* __restrict__ tbuf,
+                                                           float* __restrict__ out) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  for (int e = vstart[v]; e < vstart[v + 1]; ++e) {
+    const float4 c = tbuf[vtets[e]];
+    sx += c.x;
+    sy += c.y;
+    sz += c.z;
+  }
+  out[3ll * v + 0] += sx;
+  out[3ll * v + 1] += sy;
+  out[3ll * v + 2] += sz;
 }
 
 // Adjoint of update: given g_vel (adjoint of vel[f] after the kernel), add
 // the adjoint of pos[f].  vel[f] itself passes its adjoint through unchanged.
+template <int kOut>
 __global__ __launch_bounds__(kBlock) void k_fem_update_adj(FemP P, int nt,
                                                            const int32_t* __restrict__ f2v,
                                                            const float* __restrict__ B,
                                                            const float* __restrict__ pos,
                                                            const float* __restrict__ gvel,
-                                                           float* __restrict__ gpos) {
+                                                           float* __restrict__ gpos,
+                                                           float4* __restrict__ tbuf) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   int iv[4] = {f2v[4 * t], f2v[4 * t + 1], f2v[4 * t + 2], f2v[4 * t + 3]};
@@ -173,14 +210,23 @@ __global__ __launch_bounds__(kBlock) void k_fem_update_adj(FemP P, int nt,
   float sg = tf.det_D > 0.f ? 1.f : (tf.det_D < 0.f ? -1.f : 0.f);
   for (int i = 0; i < 9; ++i) gD.a[i] += gvol * sg / 6.f * cof.a[i];
   // D = [a-d, b-d, c-d]
-  for (int r = 0; r < 3; ++r) {
-    float s3 = 0.f;
-    for (int k = 0; k < 3; ++k) {
-      float v = gD.a[r * 3 + k];
-      atomic_add_f(gpos + 3ll * iv[k] + r, v);
-      s3 += v;
+  if (kOut == 0) {
+    for (int r = 0; r < 3; ++r) {
+      float s3 = 0.f;
+      for (int k = 0; k < 3; ++k) {
+        float v = gD.a[r * 3 + k];
+        atomic_add_f(gpos + 3ll * iv[k] + r, v);
+        s3 += v;
+      }
+      atomic_add_f(gpos + 3ll * iv[3] + r, -s3);
     }
-    atomic_add_f(gpos + 3ll * iv[3] + r, -s3);
+  } else {
+    float s3[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < 3; ++k) {
+      tbuf[4ll * t + k] = make_float4(gD.a[0 * 3 + k], gD.a[1 * 3 + k], gD.a[2 * 3 + k], 0.f);
+      for (int r = 0; r < 3; ++r) s3[r] += gD.a[r * 3 + k];
+    }
+    tbuf[4ll * t + 3] = make_float4(-s3[0], -s3[1], -s3[2], 0.f);
   }
 }
 
@@ -280,9 +326,26 @@ int dt_fem_update(const dt_fem_params* prm, int32_t n_tets, const int32_t* f2v, 
   DT_REQUIRE(n_tets >= 0, "n_tets must be >= 0");
   if (n_tets == 0) return DT_OK;
   DT_REQUIRE(f2v && B && pos && vel, "NULL pointer");
-  hipLaunchKernelGGL(k_fem_update, dim3(nblocks(n_tets)), dim3(kBlock), 0, (hipStream_t)stream,
-                     fem_dev(prm), n_tets, f2v, B, pos, vel);
+  hipLaunchKernelGGL(k_fem_update<0>, dim3(nblocks(n_tets)), dim3(kBlock), 0, (hipStream_t)stream,
+                     fem_dev(prm), n_tets, f2v, B, pos, vel, (float4*)nullptr);
   return check_launch("dt_fem_update");
+}
+
+int dt_fem_update_csr(const dt_fem_params* prm, int32_t n_tets, int32_t n_verts,
+                      const int32_t* f2v, const float* B, const float* pos, float* vel,
+                      const int32_t* vert_start, const int32_t* vert_tets, float* tet_buf,
+                      void* stream) {
+  int rc = check_fem(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n_tets >= 0 && n_verts >= 0, "n_tets and n_verts must be >= 0");
+  if (n_tets == 0) return DT_OK;
+  DT_REQUIRE(f2v && B && pos && vel && vert_start && vert_tets && tet_buf, "NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fem_update<1>, dim3(nblocks(n_tets)), dim3(kBlock), 0, st, fem_dev(prm),
+                     n_tets, f2v, B, pos, (float*)nullptr, (float4*)tet_buf);
+  hipLaunchKernelGGL(k_fem_vertex_sum, dim3(nblocks(n_verts)), dim3(kBlock), 0, st, n_verts,
+                     vert_start, vert_tets, (const float4*)tet_buf, vel);
+  return check_launch("dt_fem_update_csr");
 }
 
 int dt_fem_update_adj(const dt_fem_params* prm, int32_t n_tets, const int32_t* f2v,
@@ -293,9 +356,29 @@ int dt_fem_update_adj(const dt_fem_params* prm, int32_t n_tets, const int32_t* f
   DT_REQUIRE(n_tets >= 0, "n_tets must be >= 0");
   if (n_tets == 0) return DT_OK;
   DT_REQUIRE(f2v && B && pos && g_vel && g_pos, "NULL pointer");
-  hipLaunchKernelGGL(k_fem_update_adj, dim3(nblocks(n_tets)), dim3(kBlock), 0,
-                     (hipStream_t)stream, fem_dev(prm), n_tets, f2v, B, pos, g_vel, g_pos);
+  hipLaunchKernelGGL(k_fem_update_adj<0>, dim3(nblocks(n_tets)), dim3(kBlock), 0,
+                     (hipStream_t)stream, fem_dev(prm), n_tets, f2v, B, pos, g_vel, g_pos,
+                     (float4*)nullptr);
   return check_launch("dt_fem_update_adj");
+}
+
+int dt_fem_update_adj_csr(const dt_fem_params* prm, int32_t n_tets, int32_t n_verts,
+                          const int32_t* f2v, const float* B, const float* pos,
+                          const float* g_vel, float* g_pos, const int32_t* vert_start,
+                          const int32_t* vert_tets, float* tet_buf, void* stream) {
+  int rc = check_fem(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n_tets >= 0 && n_verts >= 0, "n_tets and n_verts must be >= 0");
+  if (n_tets == 0) return DT_OK;
+  DT_REQUIRE(f2v && B && pos && g_vel && g_pos && vert_start && vert_tets && tet_buf,
+             "NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fem_update_adj<1>, dim3(nblocks(n_tets)), dim3(kBlock), 0, st,
+                     fem_dev(prm), n_tets, f2v, B, pos, g_vel, (float*)nullptr,
+                     (float4*)tet_buf);
+  hipLaunchKernelGGL(k_fem_vertex_sum, dim3(nblocks(n_verts)), dim3(kBlock), 0, st, n_verts,
+                     vert_start, vert_tets, (const float4*)tet_buf, g_pos);
+  return check_launch("dt_fem_update_adj_csr");
 }
 
 int dt_fem_update2(const dt_fem_params* prm, int32_t n_verts, const int32_t* layer_id,

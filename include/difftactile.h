@@ -358,6 +358,20 @@ int dt_fem_update(const dt_fem_params* prm, int32_t n_tets, const int32_t* f2v, 
 int dt_fem_update_adj(const dt_fem_params* prm, int32_t n_tets, const int32_t* f2v,
                       const float* B, const float* pos, const float* g_vel, float* g_pos,
                       void* stream);
+/* Atomic-free forms of dt_fem_update / dt_fem_update_adj (same results up to
+   float summation order): each tet writes its four vertex contributions to
+   tet_buf (16 * 4 * n_tets bytes of device scratch), then each vertex adds the
+   entries of its incident (tet, corner) pairs.  vert_start (n_verts + 1) and
+   vert_tets (4 * n_tets entries, 4 * tet + corner, grouped by vertex) are the
+   mesh's vertex-to-tet CSR, built once per mesh. */
+int dt_fem_update_csr(const dt_fem_params* prm, int32_t n_tets, int32_t n_verts,
+                      const int32_t* f2v, const float* B, const float* pos, float* vel,
+                      const int32_t* vert_start, const int32_t* vert_tets, float* tet_buf,
+                      void* stream);
+int dt_fem_update_adj_csr(const dt_fem_params* prm, int32_t n_tets, int32_t n_verts,
+                          const int32_t* f2v, const float* B, const float* pos,
+                          const float* g_vel, float* g_pos, const int32_t* vert_start,
+                          const int32_t* vert_tets, float* tet_buf, void* stream);
 /* update2, fem_sensor.py:443-457 (cv_next NULL) / gripper_fem.py:403-418
    (cv_next = control_vel[f+1]): writes frame f+1 */
 int dt_fem_update2(const dt_fem_params* prm, int32_t n_verts, const int32_t* layer_id,

@@ -88,6 +88,21 @@ def test_fem_init_B_update_and_adjoint(gpu):
     g32 = torch.autograd.grad((v32 * torch.tensor(gvel)).sum(), p32)[0]
     tol_g = max(1e-4, 3 * rel_l2(g32, g_ref))
     assert rel_l2(h64(gpos), g_ref) < tol_g
+    # atomic-free CSR form: same values up to float summation order
+    from difftactile_amd.coupled import vertex_tet_csr
+    vs, vt = vertex_tet_csr(f2v, nv)
+    vs, vt = dev(vs, torch.int32), dev(vt, torch.int32)
+    tbuf = torch.empty(nt, 4, 4, device="cuda")
+    vel2 = dev(vel0)
+    _lib.call("dt_fem_update_csr", prm, nt, nv, dev(f2v, torch.int32), Bf, dev(pos), vel2, vs, vt,
+              tbuf, _lib.stream())
+    assert rel_l2(h64(vel2), h64(vel)) < 1e-6
+    assert rel_l2(h64(vel2), vref.detach()) < tol_v
+    gpos2 = torch.zeros(nv, 3, device="cuda")
+    _lib.call("dt_fem_update_adj_csr", prm, nt, nv, dev(f2v, torch.int32), Bf, dev(pos),
+              dev(gvel), gpos2, vs, vt, tbuf, _lib.stream())
+    assert rel_l2(h64(gpos2), h64(gpos)) < 1e-5
+    assert rel_l2(h64(gpos2), g_ref) < tol_g
 
 
 @pytest.mark.parametrize("gripper", [False, True])
