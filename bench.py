@@ -156,14 +156,23 @@ def graph_stats(_lib):
 
 def active_nodes(torch, sim, st):
     """Grid nodes with mass above the occupancy threshold after the first P2G
-    of the workload (dense single-substep entry point)."""
+    of the workload (binned P2G on the sorted state)."""
     from difftactile_amd import _lib
+    from difftactile_amd.sim import workspace_bytes
     n, nn = sim.n, sim.prm.n_nodes
-    g4 = torch.empty(nn, 4, device=st[0].device)
-    gv4 = torch.empty(nn, 4, device=st[0].device)
-    outs = [torch.empty_like(t) for t in st]
-    _lib.call("dt_mpm_substep_fwd", sim.c_prm_aos, sim.mats, len(sim.materials), n, sim.mat_id,
-              *[t.contiguous() for t in st], g4, gv4, *outs, _lib.stream())
+    _, perm = sim.sort_order(st[0].contiguous())
+    ss = [sim._gather(perm, st[0], 3), sim._gather(perm, st[1], 3),
+          sim._gather(perm, st[2].contiguous(), 9), sim._gather(perm, st[3], 9)]
+    mid = sim._gather_i(perm, sim.mat_id)
+    nbytes = workspace_bytes(sim.c_prm, n)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=st[0].device)
+    g4 = torch.zeros(nn, 4, device=st[0].device)
+    F1 = torch.empty_like(ss[3])
+    s = _lib.stream()
+    _lib.call("dt_mpm_bin", sim.c_prm, n, ss[0], ws, nbytes, s)
+    _lib.call("dt_mpm_p2g_binned", sim.c_prm, sim.mats, len(sim.materials), n, mid, *ss, F1, ws,
+              nbytes, s)
+    _lib.call("dt_mpm_tiles_sum", sim.c_prm, n, ws, nbytes, g4, s)
     return int((g4[:, 3] > sim.prm.eps).sum().item())
 
 

@@ -1052,9 +1052,34 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
 // (kGxAssign 1: gx =, 0: gx +=) and returns the stencil and the scatter
 // coefficients of d/d grid_v_out[node] += w_o (gvt + 4 inv_dx gC (o - fx))
 // = w_o (b + M o)
-template <int kGxAssign>
+// Grid-velocity sources of the g2p adjoint: the global grid, or the LDS copy
+// of a brick's 6x6x6-node tile.
+struct GvGlobal {
+  const float* gv;
+  int sgv, n;
+  __device__ __forceinline__ void operator()(int i, int j, int k, float& a, float& b,
+                                             float& c) const {
+    const long long nd = node_index(i, j, k, n);
+    a = gv[nd * sgv];
+    b = gv[nd * sgv + 1];
+    c = gv[nd * sgv + 2];
+  }
+};
+struct GvTile {
+  const float4* tile;
+  int o0, o1, o2;
+  __device__ __forceinline__ void operator()(int i, int j, int k, float& a, float& b,
+                                             float& c) const {
+    const float4 q = tile[((i - o0) * kTileEdge + (j - o1)) * kTileEdge + (k - o2)];
+    a = q.x;
+    b = q.y;
+    c = q.z;
+  }
+};
+
+template <int kGxAssign, class Gv>
 __device__ __forceinline__ void g2p_adj_particle(const MpmP& P, int n, const float* x,
-                                                 const float* gv, int sgv, const float* gx1,
+                                                 const Gv& gvsrc, const float* gx1,
                                                  const float* gv1, const float* gC1, float* gx,
                                                  int p, Stencil& s, float (&b)[3], M3& M) {
   const int nG = P.n_grid;
@@ -1079,8 +1104,8 @@ __device__ __forceinline__ void g2p_adj_particle(const MpmP& P, int n, const flo
     for (int j = 0; j < 3; ++j)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        long long nd = node_index(s.base[0] + i, s.base[1] + j, s.base[2] + k, nG);
-        float g0 = gv[nd * sgv], gg1 = gv[nd * sgv + 1], g2 = gv[nd * sgv + 2];
+        float g0, gg1, g2;
+        gvsrc(s.base[0] + i, s.base[1] + j, s.base[2] + k, g0, gg1, g2);
         float wj = s.w[j][1], wk = s.w[k][2];
         float w = wi * wj * wk;
         float d0 = (float)i - s.fx[0], d1 = (float)j - s.fx[1], d2 = (float)k - s.fx[2];
@@ -1129,7 +1154,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
   s.base[0] = s.base[1] = s.base[2] = 0;
   float b[3] = {0.f, 0.f, 0.f};
   M3 M = m3_zero();
-  if (active) g2p_adj_particle<kGxAssign>(P, n, x, gv, sgv, gx1, gv1, gC1, gx, p, s, b, M);
+  if (active)
+    g2p_adj_particle<kGxAssign>(P, n, x, GvGlobal{gv, sgv, P.n_grid}, gx1, gv1, gC1, gx, p, s, b,
+                                M);
   const int R = stage_runs(sh, active, block_active(n), s, P.n_grid, b, M);
   Grid g{ggv, nullptr, sggv, 0};
   run_scatter<false>(sh, R, P.n_grid, g, 0.f);
@@ -1152,8 +1179,13 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
 constexpr int kSlots = 64 * 27;
 struct BrickStage {
   // particle q: pd[6q+0] = (wx0, wx1, wx2, b0), pd[6q+1] = (wy0.., b1),
-  // pd[6q+2] = (wz0.., b2), pd[6q+3..5] = M row-major (3 floats of padding)
-  float4 pd[6 * kBlock];
+  // pd[6q+2] = (wz0.., b2), pd[6q+3..5] = M row-major (3 floats of padding).
+  // Before the staging of a chunk the same LDS holds the brick's 6x6x6-node
+  // grid-velocity tile read by the g2p adjoint's gather (gtile).
+  union {
+    float4 pd[6 * kBlock];
+    float4 gtile[kTileNodes];
+  };
   float4 slot[kSlots + 1];   // (cell, j, k, i): (sum w u0, w u1, w u2, w); last stays 0
   int cs[65];                // cell starts of the chunk (staging slots)
   int hist[64];              // particles per cell of the chunk
@@ -1272,7 +1304,7 @@ __device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
 
 // Blocks stride over the occupied-brick list (a persistent grid: unoccupied
 // bricks cost nothing).
-#define DT_BRICK_LOOP_BEGIN                                          \
+#define DT_BRICK_LOOP_BEGIN(PROLOGUE)                                \
   __shared__ BrickStage sh;                                          \
   const int n_occ = *n_list;                                         \
   for (int li = blockIdx.x; li < n_occ; li += gridDim.x) {           \
@@ -1282,6 +1314,7 @@ __device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
     for (int c0 = s0; c0 < s1; c0 += kBlock) {                       \
       const int c1 = min(c0 + (int)kBlock, s1);                      \
       const int i = c0 + threadIdx.x;                                \
+      PROLOGUE;                                                      \
       if (threadIdx.x < 64) sh.hist[threadIdx.x] = 0;                \
       __syncthreads();                                               \
       Stencil s;                                                     \
@@ -1322,9 +1355,33 @@ __global__ __launch_bounds__(kBlock) void k_p2g_binned(MpmP P, const dt_material
                                                        const int32_t* __restrict__ list,
                                                        const int32_t* __restrict__ n_list,
                                                        float4* __restrict__ tiles) {
-  DT_BRICK_LOOP_BEGIN
+  DT_BRICK_LOOP_BEGIN((void)0)
   p2g_fused_particle(P, mats, n, mat_id, x, v, C, F, F_next, perm[i], s, bb, M);
   DT_BRICK_LOOP_END(P.p_mass)
+}
+
+// loads the 6x6x6 node tile of brick b (zeros outside the grid) and points src
+// at it; the chunk loop's first barrier orders it before use, and the staging
+// that overwrites it comes two barriers after the last read
+__device__ __forceinline__ void load_gv_tile(const MpmP& P, long long b, const float* gv, int sgv,
+                                             float4* gtile, GvTile& src) {
+  const int n = P.n_grid, nb = (n + 3) >> 2;
+  const int bz = (int)(b % nb), by = (int)((b / nb) % nb), bx = (int)(b / ((long long)nb * nb));
+  src.tile = gtile;
+  src.o0 = bx * 4;
+  src.o1 = by * 4;
+  src.o2 = bz * 4;
+  const int t = threadIdx.x;
+  if (t < kTileNodes) {
+    const int i = src.o0 + t / (kTileEdge * kTileEdge), j = src.o1 + (t / kTileEdge) % kTileEdge,
+              k = src.o2 + t % kTileEdge;
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < n && j < n && k < n) {
+      const long long nd = node_index(i, j, k, n);
+      q = make_float4(gv[nd * sgv], gv[nd * sgv + 1], gv[nd * sgv + 2], 0.f);
+    }
+    gtile[t] = q;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_g2p_adj_binned(MpmP P, int n,
@@ -1339,8 +1396,11 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj_binned(MpmP P, int n,
                                                            const int32_t* __restrict__ list,
                                                            const int32_t* __restrict__ n_list,
                                                            float4* __restrict__ tiles) {
-  DT_BRICK_LOOP_BEGIN
-  g2p_adj_particle<1>(P, n, x, gv, sgv, gx1, gv1, gC1, gx, perm[i], s, bb, M);
+  // the brick's 6x6x6 grid-velocity tile in LDS (aliasing the staging area,
+  // reloaded per chunk): the stencils of its particles read only these nodes
+  GvTile src{nullptr, 0, 0, 0};
+  DT_BRICK_LOOP_BEGIN(load_gv_tile(P, b, gv, sgv, sh.gtile, src))
+  g2p_adj_particle<1>(P, n, x, src, gx1, gv1, gC1, gx, perm[i], s, bb, M);
   DT_BRICK_LOOP_END(0.f)
 }
 
