@@ -578,9 +578,9 @@ __device__ __forceinline__ void opaque(M3& m) {
 }
 
 // Two passes over the particle keep the register footprint small: the first
-// recomputes the stress to gather the affine adjoint from the grid, the second
-// recomputes the SVD and the return map (not kept across the gather) for the
-// stress adjoint.
+// recomputes the SVD and the stress to gather the affine adjoint from the grid
+// and parks U, sigma, V in LDS (one column per thread); the second reloads them
+// and recomputes only the return map for the stress adjoint.
 template <bool kExact>
 __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
     MpmP P, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
@@ -588,6 +588,8 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
     const float* __restrict__ F, const float* __restrict__ ggrid4,
     const float* __restrict__ gFnext, float* __restrict__ gx_io, float* __restrict__ gv_out,
     float* __restrict__ gC_out, float* __restrict__ gF_out) {
+  __shared__ float park[21][kBlock];   // U (9), sigma (3), V (9) of each thread's particle
+  const int t = threadIdx.x;
   int p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
@@ -602,6 +604,11 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
     M3 u, vv;
     float sig[3];
     svd3(fn, u, sig, vv);
+    for (int r = 0; r < 9; ++r) {
+      park[r][t] = u.a[r];
+      park[12 + r][t] = vv.a[r];
+    }
+    for (int r = 0; r < 3; ++r) park[9 + r][t] = sig[r];
     StressFwd st;
     M3 tau;
     stress_fwd(fn, u, sig, vv, m, st, tau);
@@ -622,7 +629,8 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
     st_v3(gv_out, p, Lv, lgv);
     affine_adj(P, gA, gtau, lgC);
   }
-  // pass 2: SVD and return map again, stress adjoint, F_new = (I + dt C) F
+  // pass 2: return map again from the parked SVD, stress adjoint,
+  // F_new = (I + dt C) F
   M3 c = ld_m3L(C, p, Lm), f = ld_m3L(F, p, Lm);
   opaque(c);
   opaque(f);
@@ -631,7 +639,11 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
   const M3 fn = m3_mul(a, f);
   M3 u, vv;
   float sig[3];
-  svd3(fn, u, sig, vv);
+  for (int r = 0; r < 9; ++r) {
+    u.a[r] = park[r][t];
+    vv.a[r] = park[12 + r][t];
+  }
+  for (int r = 0; r < 3; ++r) sig[r] = park[9 + r][t];
   StressFwd st;
   M3 tau;
   stress_fwd(fn, u, sig, vv, m, st, tau);
