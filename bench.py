@@ -326,7 +326,7 @@ def bench_pad_press(args, torch):
            "config": {"workload": f"slab pad ({len(f2v)} tets) pressing a von Mises block "
                                   f"({n} particles, {prm.n_grid}^3), {T} substeps fwd+bwd, "
                                   "loss = summed marker displacement"},
-           "loss": float(loss), "grad_finite": bool(torch.isfinite(g).all()),
+           "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(g).all()),
            "grad_dL_dvy_first_last": [float(gv[0, 1]), float(gv[-1, 1])]}
     print(json.dumps(res), flush=True)
 
