@@ -73,6 +73,131 @@ __device__ __forceinline__ float dw_axis0(const Stencil& s, int o) {
   return o == 0 ? a : (o == 1 ? b : c);
 }
 
+// Stencil node sources.  at(base) binds a particle's stencil base; the bound
+// accessor returns node base + (i, j, k) as (x, y, z, w).
+struct NodeRef4 {              // float4 nodes, row-major n^3
+  const float4* p;
+  int n;
+  __device__ __forceinline__ float4 operator()(int i, int j, int k) const {
+    return p[(i * n + j) * n + k];
+  }
+};
+struct Nodes4 {
+  const float4* g;
+  int n;
+  __device__ __forceinline__ NodeRef4 at(const int (&b)[3]) const {
+    return NodeRef4{g + node_index(b[0], b[1], b[2], n), n};
+  }
+};
+template <bool kW>
+struct NodeRefStrided {        // 3 floats at stride s (+ a 4th array at stride sq)
+  const float* p;
+  const float* q;
+  int s, sq, n;
+  __device__ __forceinline__ float4 operator()(int i, int j, int k) const {
+    const int o = (i * n + j) * n + k;
+    return make_float4(p[(long long)o * s], p[(long long)o * s + 1], p[(long long)o * s + 2],
+                       kW ? q[(long long)o * sq] : 0.f);
+  }
+};
+template <bool kW>
+struct NodesStrided {
+  const float* g;
+  const float* gw;
+  int s, sw, n;
+  __device__ __forceinline__ NodeRefStrided<kW> at(const int (&b)[3]) const {
+    const long long o = node_index(b[0], b[1], b[2], n);
+    return NodeRefStrided<kW>{g + o * s, kW ? gw + o * sw : nullptr, s, sw, n};
+  }
+};
+
+// Sum-factorised gather over the 27 stencil nodes o = (i, j, k) of grid values
+// g_o with weights w_o = wx_i wy_j wz_k:
+//   G      = sum_o w_o g_o.xyz
+//   GO[r]  = sum_o w_o g_o.xyz o_r                           (kMoments)
+//   gfx[a] = sum_o (d w_o / d fx_a) (g_o.xyz . (b + M o) + g_o.w m3)   (kGw)
+// The k sums are formed per (i, j) row and the j sums per i, so a node costs
+// about fifteen FMAs instead of the forty-odd of the expanded per-node form.
+template <bool kGw, bool kMoments, class Ref>
+__device__ __forceinline__ void stencil_gather(const Stencil& s, const Ref& g, const float (&b)[3],
+                                               const M3& M, float m3, float (&G)[3],
+                                               float (&GO)[3][3], float (&gfx)[3]) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    G[q] = 0.f;
+    gfx[q] = 0.f;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) GO[r][q] = 0.f;
+  }
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const float wi = w_axis0(s, i), dwi = dw_axis0(s, i), fi = (float)i;
+    float ci[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) ci[q] = b[q] + fi * M.a[3 * q];
+    float Pi = 0.f, Qi = 0.f, Ri = 0.f;
+    float Gi[3] = {0.f, 0.f, 0.f}, GJ[3] = {0.f, 0.f, 0.f}, GK[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float wj = s.w[j][1], dwj = s.dw[j][1];
+      float cij[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) cij[q] = j == 0 ? ci[q] : ci[q] + (float)j * M.a[3 * q + 1];
+      float Pk = 0.f, Rk = 0.f, H[3] = {0.f, 0.f, 0.f}, HK[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float wk = s.w[k][2], dwk = s.dw[k][2];
+        const float4 v = g(i, j, k);
+        if (kGw) {
+          float c[3];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) c[q] = k == 0 ? cij[q] : cij[q] + (float)k * M.a[3 * q + 2];
+          const float gw = v.x * c[0] + v.y * c[1] + v.z * c[2] + v.w * m3;
+          Pk += wk * gw;
+          Rk += dwk * gw;
+        }
+        H[0] += wk * v.x;
+        H[1] += wk * v.y;
+        H[2] += wk * v.z;
+        if (kMoments && k > 0) {
+          const float kw = (float)k * wk;
+          HK[0] += kw * v.x;
+          HK[1] += kw * v.y;
+          HK[2] += kw * v.z;
+        }
+      }
+      if (kGw) {
+        Pi += wj * Pk;
+        Qi += dwj * Pk;
+        Ri += wj * Rk;
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        Gi[q] += wj * H[q];
+        if (kMoments) {
+          if (j > 0) GJ[q] += ((float)j * wj) * H[q];
+          GK[q] += wj * HK[q];
+        }
+      }
+    }
+    if (kGw) {
+      gfx[0] += dwi * Pi;
+      gfx[1] += wi * Qi;
+      gfx[2] += wi * Ri;
+    }
+    const float iw = fi * wi;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      G[q] += wi * Gi[q];
+      if (kMoments) {
+        GO[0][q] += iw * Gi[q];
+        GO[1][q] += wi * GJ[q];
+        GO[2][q] += wi * GK[q];
+      }
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------
 // per-particle helpers
 // ----------------------------------------------------------------------------
@@ -382,59 +507,37 @@ __global__ __launch_bounds__(kBlock) void k_p2g_fused(MpmP P, const dt_material*
 }
 
 // ----------------------------------------------------------------------------
-// p2g adjoint core: gathers the grid adjoints over the 27 nodes.
+// p2g adjoint core: gathers the grid adjoints (d/d mv in xyz, d/d m in w) over
+// the 27 nodes.  The forward contribution to node o is w_o (pv + A d_o, m) with
+// d_o = (o - fx) dx = w_o (b + M o, m), b = pv - A fx dx, M = A dx.
 //   outputs gx (+= through weights and dpos), gv, gA (adjoint of affine)
 // ----------------------------------------------------------------------------
+template <class Ref>
 __device__ __forceinline__ void p2g_gather_adj(const MpmP& P, const Stencil& s,
-                                               const float (&pv)[3], const M3& A,
-                                               const float* __restrict__ gmv, int sgmv,
-                                               const float* __restrict__ gm, int sgm,
+                                               const float (&pv)[3], const M3& A, const Ref& g,
                                                float (&gx)[3], float (&gv)[3], M3& gA) {
-  const int n = P.n_grid;
-  float gfx[3] = {0.f, 0.f, 0.f};
-  float gd[3] = {0.f, 0.f, 0.f};
-  gA = m3_zero();
-#pragma unroll 1
-  for (int i = 0; i < 3; ++i) {
-    const float wi = w_axis0(s, i), dwi = dw_axis0(s, i);
+  float b[3];
+  M3 M;
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+  for (int q = 0; q < 3; ++q) {
+    b[q] = pv[q] - (A.a[3 * q] * s.fx[0] + A.a[3 * q + 1] * s.fx[1] + A.a[3 * q + 2] * s.fx[2]) *
+                       P.dx;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        long long nd = node_index(s.base[0] + i, s.base[1] + j, s.base[2] + k, n);
-        float g0 = gmv[nd * sgmv + 0], g1 = gmv[nd * sgmv + 1], g2 = gmv[nd * sgmv + 2];
-        float g3 = gm[nd * sgm];
-        float wj = s.w[j][1], wk = s.w[k][2];
-        float w = wi * wj * wk;
-        float d0 = ((float)i - s.fx[0]) * P.dx;
-        float d1 = ((float)j - s.fx[1]) * P.dx;
-        float d2 = ((float)k - s.fx[2]) * P.dx;
-        float c0 = pv[0] + (A.a[0] * d0 + A.a[1] * d1 + A.a[2] * d2);
-        float c1 = pv[1] + (A.a[3] * d0 + A.a[4] * d1 + A.a[5] * d2);
-        float c2 = pv[2] + (A.a[6] * d0 + A.a[7] * d1 + A.a[8] * d2);
-        // d/dw
-        float gw = g0 * c0 + g1 * c1 + g2 * c2 + g3 * P.p_mass;
-        gfx[0] += gw * dwi * wj * wk;
-        gfx[1] += gw * wi * s.dw[j][1] * wk;
-        gfx[2] += gw * wi * wj * s.dw[k][2];
-        // d/d(p_mass v)
-        gv[0] += w * g0;
-        gv[1] += w * g1;
-        gv[2] += w * g2;
-        // d/dA and d/d dpos
-        float wg0 = w * g0, wg1 = w * g1, wg2 = w * g2;
-        gA.a[0] += wg0 * d0; gA.a[1] += wg0 * d1; gA.a[2] += wg0 * d2;
-        gA.a[3] += wg1 * d0; gA.a[4] += wg1 * d1; gA.a[5] += wg1 * d2;
-        gA.a[6] += wg2 * d0; gA.a[7] += wg2 * d1; gA.a[8] += wg2 * d2;
-        gd[0] += A.a[0] * wg0 + A.a[3] * wg1 + A.a[6] * wg2;
-        gd[1] += A.a[1] * wg0 + A.a[4] * wg1 + A.a[7] * wg2;
-        gd[2] += A.a[2] * wg0 + A.a[5] * wg1 + A.a[8] * wg2;
-      }
+    for (int r = 0; r < 3; ++r) M.a[3 * q + r] = A.a[3 * q + r] * P.dx;
   }
-  // fx = x inv_dx - base;  dpos = (o - fx) dx
+  float G[3], GO[3][3], gfx[3];
+  stencil_gather<true, true>(s, g, b, M, P.p_mass, G, GO, gfx);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    gv[q] = P.p_mass * G[q];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) gA.a[3 * q + r] = P.dx * (GO[r][q] - G[q] * s.fx[r]);
+  }
+  // fx = x inv_dx - base;  dpos = (o - fx) dx:  d/dx through dpos is -A^T G
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
-    gv[a] *= P.p_mass;
-    gx[a] += P.inv_dx * gfx[a] - P.inv_dx * P.dx * gd[a];
+    const float gd = A.a[a] * G[0] + A.a[3 + a] * G[1] + A.a[6 + a] * G[2];
+    gx[a] += P.inv_dx * gfx[a] - P.inv_dx * P.dx * gd;
   }
 }
 
@@ -475,7 +578,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
   for (int a = 0; a < 3; ++a) pv[a] = P.p_mass * v[3ll * p + a];
   float lgx[3] = {0.f, 0.f, 0.f}, lgv[3] = {0.f, 0.f, 0.f};
   M3 gA;
-  p2g_gather_adj(P, s, pv, A, gmv, 3, gm, 1, lgx, lgv, gA);
+  p2g_gather_adj(P, s, pv, A, NodesStrided<true>{gmv, gm, 3, 1, P.n_grid}.at(s.base), lgx, lgv,
+                 gA);
   M3 gtau, lgC;
   affine_adj(P, gA, gtau, lgC);
   for (int a = 0; a < 3; ++a) {
@@ -624,7 +728,8 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
     ld_v3(gx_io, p, Lv, lgx);
     float lgv[3] = {0.f, 0.f, 0.f};
     M3 gA;
-    p2g_gather_adj(P, s, pv, A, ggrid4, 4, ggrid4 + 3, 4, lgx, lgv, gA);
+    p2g_gather_adj(P, s, pv, A, Nodes4{(const float4*)ggrid4, P.n_grid}.at(s.base), lgx, lgv,
+                   gA);
     st_v3(gx_io, p, Lv, lgx);
     st_v3(gv_out, p, Lv, lgv);
     affine_adj(P, gA, gtau, lgC);
@@ -1011,32 +1116,18 @@ __global__ __launch_bounds__(kBlock) void k_zero_i32(long long n, int32_t* __res
 // ----------------------------------------------------------------------------
 // g2p and its adjoint
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ void g2p_core(const MpmP& P, const Stencil& s,
-                                         const float* __restrict__ gv, int sgv, float (&nv)[3],
-                                         M3& nC) {
-  const int n = P.n_grid;
-  nv[0] = nv[1] = nv[2] = 0.f;
-  nC = m3_zero();
-#pragma unroll 1
-  for (int i = 0; i < 3; ++i) {
-    const float wi = w_axis0(s, i);
+// new v = sum_o w_o gv_o, new C = 4 inv_dx sum_o w_o gv_o (o - fx)^T
+template <class Ref>
+__device__ __forceinline__ void g2p_core(const MpmP& P, const Stencil& s, const Ref& g,
+                                         float (&nv)[3], M3& nC) {
+  const float zb[3] = {0.f, 0.f, 0.f};
+  float GO[3][3], gfx[3];
+  stencil_gather<false, true>(s, g, zb, m3_zero(), 0.f, nv, GO, gfx);
+  const float k4 = 4.f * P.inv_dx;
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+  for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        long long nd = node_index(s.base[0] + i, s.base[1] + j, s.base[2] + k, n);
-        float g0 = gv[nd * sgv], g1 = gv[nd * sgv + 1], g2 = gv[nd * sgv + 2];
-        float w = wi * s.w[j][1] * s.w[k][2];
-        float d[3] = {(float)i - s.fx[0], (float)j - s.fx[1], (float)k - s.fx[2]};
-        nv[0] += w * g0;
-        nv[1] += w * g1;
-        nv[2] += w * g2;
-        float kw = 4.f * P.inv_dx * w;
-        float g[3] = {g0, g1, g2};
-        for (int r = 0; r < 3; ++r)
-          for (int q = 0; q < 3; ++q) nC.a[r * 3 + q] += kw * g[r] * d[q];
-      }
-  }
+    for (int q = 0; q < 3; ++q) nC.a[r * 3 + q] = k4 * (GO[q][r] - nv[r] * s.fx[q]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __restrict__ x,
@@ -1052,7 +1143,10 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
   make_stencil(xp, P.inv_dx, P.n_grid, s);
   float nv[3];
   M3 nC;
-  g2p_core(P, s, gv, sgv, nv, nC);
+  if (sgv == 4)
+    g2p_core(P, s, Nodes4{(const float4*)gv, P.n_grid}.at(s.base), nv, nC);
+  else
+    g2p_core(P, s, NodesStrided<false>{gv, nullptr, sgv, 0, P.n_grid}.at(s.base), nv, nC);
   float nx[3];
   for (int a = 0; a < 3; ++a) nx[a] = xp[a] + P.dt * nv[a];
   st_v3(v1, p, Lv, nv);
@@ -1064,28 +1158,19 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
 // (kGxAssign 1: gx =, 0: gx +=) and returns the stencil and the scatter
 // coefficients of d/d grid_v_out[node] += w_o (gvt + 4 inv_dx gC (o - fx))
 // = w_o (b + M o)
-// Grid-velocity sources of the g2p adjoint: the global grid, or the LDS copy
-// of a brick's 6x6x6-node tile.
-struct GvGlobal {
-  const float* gv;
-  int sgv, n;
-  __device__ __forceinline__ void operator()(int i, int j, int k, float& a, float& b,
-                                             float& c) const {
-    const long long nd = node_index(i, j, k, n);
-    a = gv[nd * sgv];
-    b = gv[nd * sgv + 1];
-    c = gv[nd * sgv + 2];
+// Grid-velocity sources of the g2p adjoint: the global grid (Nodes4 or
+// NodesStrided), or the LDS copy of a brick's 6x6x6-node tile.
+struct NodeRefTile {
+  const float4* p;
+  __device__ __forceinline__ float4 operator()(int i, int j, int k) const {
+    return p[(i * kTileEdge + j) * kTileEdge + k];
   }
 };
 struct GvTile {
   const float4* tile;
   int o0, o1, o2;
-  __device__ __forceinline__ void operator()(int i, int j, int k, float& a, float& b,
-                                             float& c) const {
-    const float4 q = tile[((i - o0) * kTileEdge + (j - o1)) * kTileEdge + (k - o2)];
-    a = q.x;
-    b = q.y;
-    c = q.z;
+  __device__ __forceinline__ NodeRefTile at(const int (&b)[3]) const {
+    return NodeRefTile{tile + ((b[0] - o0) * kTileEdge + (b[1] - o1)) * kTileEdge + (b[2] - o2)};
   }
 };
 
@@ -1108,45 +1193,23 @@ __device__ __forceinline__ void g2p_adj_particle(const MpmP& P, int n, const flo
   }
   const M3 gc = ld_m3L(gC1, p, Lm);
   const float k4 = 4.f * P.inv_dx;
-  float gfx[3] = {0.f, 0.f, 0.f}, gd[3] = {0.f, 0.f, 0.f};
-#pragma unroll 1
-  for (int i = 0; i < 3; ++i) {
-    const float wi = w_axis0(s, i), dwi = dw_axis0(s, i);
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float g0, gg1, g2;
-        gvsrc(s.base[0] + i, s.base[1] + j, s.base[2] + k, g0, gg1, g2);
-        float wj = s.w[j][1], wk = s.w[k][2];
-        float w = wi * wj * wk;
-        float d0 = (float)i - s.fx[0], d1 = (float)j - s.fx[1], d2 = (float)k - s.fx[2];
-        // gC d
-        float cd0 = gc.a[0] * d0 + gc.a[1] * d1 + gc.a[2] * d2;
-        float cd1 = gc.a[3] * d0 + gc.a[4] * d1 + gc.a[5] * d2;
-        float cd2 = gc.a[6] * d0 + gc.a[7] * d1 + gc.a[8] * d2;
-        float gw = g0 * gvt[0] + gg1 * gvt[1] + g2 * gvt[2] +
-                   k4 * (g0 * cd0 + gg1 * cd1 + g2 * cd2);
-        gfx[0] += gw * dwi * wj * wk;
-        gfx[1] += gw * wi * s.dw[j][1] * wk;
-        gfx[2] += gw * wi * wj * s.dw[k][2];
-        // d/d d of k4 w g^T gC d = k4 w gC^T g
-        gd[0] += k4 * w * (gc.a[0] * g0 + gc.a[3] * gg1 + gc.a[6] * g2);
-        gd[1] += k4 * w * (gc.a[1] * g0 + gc.a[4] * gg1 + gc.a[7] * g2);
-        gd[2] += k4 * w * (gc.a[2] * g0 + gc.a[5] * gg1 + gc.a[8] * g2);
-      }
-  }
-  // fx = x inv_dx - base; d = o - fx
-  for (int a = 0; a < 3; ++a) lgx[a] += P.inv_dx * gfx[a] - P.inv_dx * gd[a];
-  for (int a = 0; a < 3; ++a) {
-    long long o = p * Lv.sp + a * Lv.sc;
-    if (kGxAssign) gx[o] = lgx[a];
-    else gx[o] += lgx[a];
-  }
   for (int c = 0; c < 3; ++c) {
     b[c] = gvt[c] - k4 * (gc.a[3 * c] * s.fx[0] + gc.a[3 * c + 1] * s.fx[1] +
                           gc.a[3 * c + 2] * s.fx[2]);
     for (int a = 0; a < 3; ++a) M.a[3 * c + a] = k4 * gc.a[3 * c + a];
+  }
+  // d/d w_o of w_o gv_o . (gvt + 4 inv_dx gC (o - fx)) = gv_o . (b + M o)
+  float G[3], GO[3][3], gfx[3];
+  stencil_gather<true, false>(s, gvsrc.at(s.base), b, M, 0.f, G, GO, gfx);
+  // fx = x inv_dx - base; d = o - fx: d/dx through d is -M^T G
+  for (int a = 0; a < 3; ++a) {
+    const float gd = M.a[a] * G[0] + M.a[3 + a] * G[1] + M.a[6 + a] * G[2];
+    lgx[a] += P.inv_dx * gfx[a] - P.inv_dx * gd;
+  }
+  for (int a = 0; a < 3; ++a) {
+    long long o = p * Lv.sp + a * Lv.sc;
+    if (kGxAssign) gx[o] = lgx[a];
+    else gx[o] += lgx[a];
   }
 }
 
@@ -1166,9 +1229,14 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
   s.base[0] = s.base[1] = s.base[2] = 0;
   float b[3] = {0.f, 0.f, 0.f};
   M3 M = m3_zero();
-  if (active)
-    g2p_adj_particle<kGxAssign>(P, n, x, GvGlobal{gv, sgv, P.n_grid}, gx1, gv1, gC1, gx, p, s, b,
-                                M);
+  if (active) {
+    if (sgv == 4)
+      g2p_adj_particle<kGxAssign>(P, n, x, Nodes4{(const float4*)gv, P.n_grid}, gx1, gv1, gC1, gx,
+                                  p, s, b, M);
+    else
+      g2p_adj_particle<kGxAssign>(P, n, x, NodesStrided<false>{gv, nullptr, sgv, 0, P.n_grid}, gx1,
+                                  gv1, gC1, gx, p, s, b, M);
+  }
   const int R = stage_runs(sh, active, block_active(n), s, P.n_grid, b, M);
   Grid g{ggv, nullptr, sggv, 0};
   run_scatter<false>(sh, R, P.n_grid, g, 0.f);
