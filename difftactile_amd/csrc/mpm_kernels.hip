@@ -201,8 +201,7 @@ __device__ __forceinline__ void stencil_gather(const Stencil& s, const Ref& g, c
 // ----------------------------------------------------------------------------
 // per-particle helpers
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ Mat load_mat(const dt_material* mats, const int32_t* mat_id, int p) {
-  int id = mat_id ? mat_id[p] : 0;
+__device__ __forceinline__ Mat mat_row(const dt_material* mats, int id) {
   const dt_material& m = mats[id];
   Mat r;
   r.mu = m.mu;
@@ -211,6 +210,9 @@ __device__ __forceinline__ Mat load_mat(const dt_material* mats, const int32_t* 
   r.dp_alpha = m.dp_alpha;
   r.kind = m.kind;
   return r;
+}
+__device__ __forceinline__ Mat load_mat(const dt_material* mats, const int32_t* mat_id, int p) {
+  return mat_row(mats, mat_id ? mat_id[p] : 0);
 }
 
 // affine = stress + p_mass C with stress = (-dt p_vol 4 inv_dx^2) tau
@@ -454,37 +456,58 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmP P, const dt_material* __res
 // ----------------------------------------------------------------------------
 // fused compute_new_F + svd + return map + p2g: reads (x, v, C, F)
 // ----------------------------------------------------------------------------
+// inputs of the fused p2g of one particle
+struct P2GIn {
+  float x[3], v[3];
+  M3 c, f;
+  int mid;
+};
+__device__ __forceinline__ void p2g_load(const MpmP& P, int n, const int32_t* mat_id,
+                                         const float* x, const float* v, const float* C,
+                                         const float* F, int p, P2GIn& in) {
+  PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
+  in.c = ld_m3L(C, p, Lm);
+  in.f = ld_m3L(F, p, Lm);
+  ld_v3(x, p, Lv, in.x);
+  ld_v3(v, p, Lv, in.v);
+  in.mid = mat_id ? mat_id[p] : 0;
+}
+
 // compute_new_F + svd + return map of particle p; writes F_next and returns
 // the stencil and the scatter coefficients (b, M) of the p2g contribution
 // w_o (p_mass v + A (o - fx) dx) = w_o (b + M o)
+__device__ __forceinline__ void p2g_fused_compute(const MpmP& P, const dt_material* mats, int n,
+                                                  const P2GIn& in, float* F_next, int p,
+                                                  Stencil& s, float (&b)[3], M3& M) {
+  PL Lm = pl_mat(P.soa, n);
+  M3 a = m3_eye();
+  for (int i = 0; i < 9; ++i) a.a[i] += P.dt * in.c.a[i];
+  M3 fn = m3_mul(a, in.f);
+  M3 u, vv;
+  float sig[3];
+  svd3(fn, u, sig, vv);
+  Mat m = mat_row(mats, in.mid);
+  StressFwd st;
+  M3 tau;
+  stress_fwd(fn, u, sig, vv, m, st, tau);
+  const M3 A = affine_of(P, tau, in.c);
+  st_m3L(F_next, p, Lm, st.Fp);
+  make_stencil(in.x, P.inv_dx, P.n_grid, s);
+  for (int q = 0; q < 3; ++q) {
+    b[q] = P.p_mass * in.v[q] -
+           (A.a[3 * q] * s.fx[0] + A.a[3 * q + 1] * s.fx[1] + A.a[3 * q + 2] * s.fx[2]) * P.dx;
+    for (int r = 0; r < 3; ++r) M.a[3 * q + r] = A.a[3 * q + r] * P.dx;
+  }
+}
+
 __device__ __forceinline__ void p2g_fused_particle(const MpmP& P, const dt_material* mats, int n,
                                                    const int32_t* mat_id, const float* x,
                                                    const float* v, const float* C, const float* F,
                                                    float* F_next, int p, Stencil& s,
                                                    float (&b)[3], M3& M) {
-  PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
-  M3 c = ld_m3L(C, p, Lm), f = ld_m3L(F, p, Lm);
-  M3 a = m3_eye();
-  for (int i = 0; i < 9; ++i) a.a[i] += P.dt * c.a[i];
-  M3 fn = m3_mul(a, f);
-  M3 u, vv;
-  float sig[3];
-  svd3(fn, u, sig, vv);
-  Mat m = load_mat(mats, mat_id, p);
-  StressFwd st;
-  M3 tau;
-  stress_fwd(fn, u, sig, vv, m, st, tau);
-  const M3 A = affine_of(P, tau, c);
-  st_m3L(F_next, p, Lm, st.Fp);
-  float xp[3], vp[3];
-  ld_v3(x, p, Lv, xp);
-  ld_v3(v, p, Lv, vp);
-  make_stencil(xp, P.inv_dx, P.n_grid, s);
-  for (int q = 0; q < 3; ++q) {
-    b[q] = P.p_mass * vp[q] -
-           (A.a[3 * q] * s.fx[0] + A.a[3 * q + 1] * s.fx[1] + A.a[3 * q + 2] * s.fx[2]) * P.dx;
-    for (int r = 0; r < 3; ++r) M.a[3 * q + r] = A.a[3 * q + r] * P.dx;
-  }
+  P2GIn in;
+  p2g_load(P, n, mat_id, x, v, C, F, p, in);
+  p2g_fused_compute(P, mats, n, in, F_next, p, s, b, M);
 }
 
 __global__ __launch_bounds__(kBlock) void k_p2g_fused(MpmP P, const dt_material* __restrict__ mats,
@@ -1174,24 +1197,33 @@ struct GvTile {
   }
 };
 
-template <int kGxAssign, class Gv>
-__device__ __forceinline__ void g2p_adj_particle(const MpmP& P, int n, const float* x,
-                                                 const Gv& gvsrc, const float* gx1,
-                                                 const float* gv1, const float* gC1, float* gx,
-                                                 int p, Stencil& s, float (&b)[3], M3& M) {
-  const int nG = P.n_grid;
+// inputs of the g2p adjoint of one particle
+struct G2PAdjIn {
+  float x[3], g1[3], v1[3];
+  M3 gc;
+};
+__device__ __forceinline__ void g2p_adj_load(const MpmP& P, int n, const float* x, const float* gx1,
+                                             const float* gv1, const float* gC1, int p,
+                                             G2PAdjIn& in) {
   PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
+  ld_v3(x, p, Lv, in.x);
+  ld_v3(gx1, p, Lv, in.g1);
+  ld_v3(gv1, p, Lv, in.v1);
+  in.gc = ld_m3L(gC1, p, Lm);
+}
+
+template <int kGxAssign, class Gv>
+__device__ __forceinline__ void g2p_adj_compute(const MpmP& P, int n, const G2PAdjIn& in,
+                                                const Gv& gvsrc, float* gx, int p, Stencil& s,
+                                                float (&b)[3], M3& M) {
+  PL Lv = pl_vec(P.soa, n);
   float gvt[3], lgx[3];
-  float xp[3], g1[3], v1[3];
-  ld_v3(x, p, Lv, xp);
-  ld_v3(gx1, p, Lv, g1);
-  ld_v3(gv1, p, Lv, v1);
-  make_stencil(xp, P.inv_dx, nG, s);
+  make_stencil(in.x, P.inv_dx, P.n_grid, s);
   for (int a = 0; a < 3; ++a) {
-    gvt[a] = v1[a] + P.dt * g1[a];
-    lgx[a] = g1[a];
+    gvt[a] = in.v1[a] + P.dt * in.g1[a];
+    lgx[a] = in.g1[a];
   }
-  const M3 gc = ld_m3L(gC1, p, Lm);
+  const M3& gc = in.gc;
   const float k4 = 4.f * P.inv_dx;
   for (int c = 0; c < 3; ++c) {
     b[c] = gvt[c] - k4 * (gc.a[3 * c] * s.fx[0] + gc.a[3 * c + 1] * s.fx[1] +
@@ -1211,6 +1243,16 @@ __device__ __forceinline__ void g2p_adj_particle(const MpmP& P, int n, const flo
     if (kGxAssign) gx[o] = lgx[a];
     else gx[o] += lgx[a];
   }
+}
+
+template <int kGxAssign, class Gv>
+__device__ __forceinline__ void g2p_adj_particle(const MpmP& P, int n, const float* x,
+                                                 const Gv& gvsrc, const float* gx1,
+                                                 const float* gv1, const float* gC1, float* gx,
+                                                 int p, Stencil& s, float (&b)[3], M3& M) {
+  G2PAdjIn in;
+  g2p_adj_load(P, n, x, gx1, gv1, gC1, p, in);
+  g2p_adj_compute<kGxAssign>(P, n, in, gvsrc, gx, p, s, b, M);
 }
 
 // g2p adjoint: gather gx, scatter to the grid-velocity adjoint by runs
@@ -1304,15 +1346,6 @@ __device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Sten
   q[5] = make_float4(M.a[8], 0.f, 0.f, 0.f);
 }
 
-// brick start: zero the slots, read the brick's range of perm
-__device__ __forceinline__ void brick_begin(BrickStage& sh, const int32_t* __restrict__ start,
-                                            long long b, int& s0, int& s1) {
-  for (int l = threadIdx.x; l < kSlots + 1; l += blockDim.x)
-    sh.slot[l] = make_float4(0.f, 0.f, 0.f, 0.f);
-  s0 = start[b];
-  s1 = start[b + 1];
-}
-
 // step 2: items over the occupied cells of the chunk
 __device__ __forceinline__ void brick_items(BrickStage& sh) {
   const int n_items = sh.n_cells * 9;
@@ -1383,45 +1416,99 @@ __device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
 }
 
 // Blocks stride over the occupied-brick list (a persistent grid: unoccupied
-// bricks cost nothing).
-#define DT_BRICK_LOOP_BEGIN(PROLOGUE)                                \
-  __shared__ BrickStage sh;                                          \
-  const int n_occ = *n_list;                                         \
-  for (int li = blockIdx.x; li < n_occ; li += gridDim.x) {           \
-    const long long b = list[li];                                    \
-    int s0, s1;                                                      \
-    brick_begin(sh, start, b, s0, s1);                               \
-    for (int c0 = s0; c0 < s1; c0 += kBlock) {                       \
-      const int c1 = min(c0 + (int)kBlock, s1);                      \
-      const int i = c0 + threadIdx.x;                                \
-      PROLOGUE;                                                      \
-      if (threadIdx.x < 64) sh.hist[threadIdx.x] = 0;                \
-      __syncthreads();                                               \
-      Stencil s;                                                     \
-      float bb[3];                                                   \
-      M3 M;                                                          \
-      int lc = 0, r = 0;                                             \
-      if (i < c1) {
-
-#define DT_BRICK_LOOP_END(mass_scale)                                \
-        lc = local_cell(s);                                          \
-        r = atomicAdd(&sh.hist[lc], 1);                              \
-      }                                                              \
-      __syncthreads();                                               \
-      if (threadIdx.x < 64) chunk_scan(sh);                          \
-      __syncthreads();                                               \
-      if (i < c1) stage_particle(sh, sh.cs[lc] + r, s, bb, M);       \
-      __syncthreads();                                               \
-      brick_items(sh);                                               \
-      __syncthreads();                                               \
-    }                                                                \
-    float4 acc = brick_gather(sh);                                   \
-    if (threadIdx.x < kTileNodes) {                                  \
-      acc.w *= (mass_scale);                                         \
-      tiles[b * kTileNodes + threadIdx.x] = acc;                     \
-    }                                                                \
-    __syncthreads();                                                 \
+// bricks cost nothing), chunk by chunk, software-pipelined: while chunk k is
+// sorted and scattered, the inputs of chunk k + 1 (the next of the brick, or
+// the first of the block's next brick) are in flight, and perm of chunk k + 1
+// is read at the start of chunk k so those loads need not wait for it.
+//   load(p, in)                 issues the loads of particle p's inputs
+//   compute(in, p, b, s, bb, M) particle p of brick b: stencil and (b, M)
+//   fetch(b)                    (kTile) node t of brick b's 6x6x6 grid tile,
+//                               stored to sh.gtile at every chunk start of b
+//                               (the staging of a chunk overwrites it)
+template <bool kTile, class In, class Load, class Compute, class Fetch>
+__device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __restrict__ perm,
+                                           const int32_t* __restrict__ start,
+                                           const int32_t* __restrict__ list,
+                                           const int32_t* __restrict__ n_list,
+                                           float4* __restrict__ tiles, float mass_scale,
+                                           const Load& load, const Compute& compute,
+                                           const Fetch& fetch) {
+  const int t = threadIdx.x;
+  const int n_occ = *n_list;
+  int li = blockIdx.x;
+  if (li >= n_occ) return;
+  long long b = list[li];
+  int s0 = start[b], s1 = start[b + 1];
+  In in;
+  int p = 0;
+  float4 tq = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (kTile) tq = fetch(b);
+  if (s0 + t < min(s0 + (int)kBlock, s1)) {
+    p = perm[s0 + t];
+    load(p, in);
   }
+  for (;;) {
+    const int li2 = li + gridDim.x;
+    long long b2 = -1;
+    int s02 = 0, s12 = 0;
+    if (li2 < n_occ) {
+      b2 = list[li2];
+      s02 = start[b2];
+      s12 = start[b2 + 1];
+    }
+    for (int l = t; l < kSlots + 1; l += blockDim.x) sh.slot[l] = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 tq2 = tq;
+    for (int c0 = s0; c0 < s1; c0 += kBlock) {
+      const int c1 = min(c0 + (int)kBlock, s1);
+      const int i = c0 + t;
+      // the chunk after this one
+      const bool last = c1 >= s1;
+      const int nc0 = last ? s02 : c1;
+      const int nc1 = last ? min(s02 + (int)kBlock, s12) : min(c1 + (int)kBlock, s1);
+      const bool has_next = nc0 + t < nc1;
+      int pn = 0;
+      if (has_next) pn = perm[nc0 + t];
+      if (kTile) {
+        if (t < kTileNodes) sh.gtile[t] = tq;
+        if (last && b2 >= 0) tq2 = fetch(b2);
+      }
+      if (t < 64) sh.hist[t] = 0;
+      __syncthreads();
+      Stencil s;
+      float bb[3];
+      M3 M;
+      int lc = 0, r = 0;
+      if (i < c1) {
+        compute(in, p, b, s, bb, M);
+        lc = local_cell(s);
+        r = atomicAdd(&sh.hist[lc], 1);
+      }
+      if (has_next) {
+        p = pn;
+        load(p, in);
+      }
+      __syncthreads();
+      if (t < 64) chunk_scan(sh);
+      __syncthreads();
+      if (i < c1) stage_particle(sh, sh.cs[lc] + r, s, bb, M);
+      __syncthreads();
+      brick_items(sh);
+      __syncthreads();
+    }
+    float4 acc = brick_gather(sh);
+    if (t < kTileNodes) {
+      acc.w *= mass_scale;
+      tiles[b * kTileNodes + t] = acc;
+    }
+    __syncthreads();
+    if (b2 < 0) break;
+    li = li2;
+    b = b2;
+    s0 = s02;
+    s1 = s12;
+    tq = tq2;
+  }
+}
 
 __global__ __launch_bounds__(kBlock) void k_p2g_binned(MpmP P, const dt_material* __restrict__ mats,
                                                        int n, const int32_t* __restrict__ mat_id,
@@ -1435,33 +1522,33 @@ __global__ __launch_bounds__(kBlock) void k_p2g_binned(MpmP P, const dt_material
                                                        const int32_t* __restrict__ list,
                                                        const int32_t* __restrict__ n_list,
                                                        float4* __restrict__ tiles) {
-  DT_BRICK_LOOP_BEGIN((void)0)
-  p2g_fused_particle(P, mats, n, mat_id, x, v, C, F, F_next, perm[i], s, bb, M);
-  DT_BRICK_LOOP_END(P.p_mass)
+  __shared__ BrickStage sh;
+  brick_loop<false, P2GIn>(
+      sh, perm, start, list, n_list, tiles, P.p_mass,
+      [&](int p, P2GIn& in) { p2g_load(P, n, mat_id, x, v, C, F, p, in); },
+      [&](const P2GIn& in, int p, long long, Stencil& s, float(&bb)[3], M3& M) {
+        p2g_fused_compute(P, mats, n, in, F_next, p, s, bb, M);
+      },
+      [](long long) { return make_float4(0.f, 0.f, 0.f, 0.f); });
 }
 
-// loads the 6x6x6 node tile of brick b (zeros outside the grid) and points src
-// at it; the chunk loop's first barrier orders it before use, and the staging
-// that overwrites it comes two barriers after the last read
-__device__ __forceinline__ void load_gv_tile(const MpmP& P, long long b, const float* gv, int sgv,
-                                             float4* gtile, GvTile& src) {
+// node t of the 6x6x6-node tile of brick b (zero outside the grid and for
+// t >= 216)
+__device__ __forceinline__ float4 fetch_gv_tile(const MpmP& P, long long b, const float* gv,
+                                                int sgv) {
   const int n = P.n_grid, nb = (n + 3) >> 2;
   const int bz = (int)(b % nb), by = (int)((b / nb) % nb), bx = (int)(b / ((long long)nb * nb));
-  src.tile = gtile;
-  src.o0 = bx * 4;
-  src.o1 = by * 4;
-  src.o2 = bz * 4;
   const int t = threadIdx.x;
+  float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
   if (t < kTileNodes) {
-    const int i = src.o0 + t / (kTileEdge * kTileEdge), j = src.o1 + (t / kTileEdge) % kTileEdge,
-              k = src.o2 + t % kTileEdge;
-    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int i = bx * 4 + t / (kTileEdge * kTileEdge), j = by * 4 + (t / kTileEdge) % kTileEdge,
+              k = bz * 4 + t % kTileEdge;
     if (i < n && j < n && k < n) {
       const long long nd = node_index(i, j, k, n);
       q = make_float4(gv[nd * sgv], gv[nd * sgv + 1], gv[nd * sgv + 2], 0.f);
     }
-    gtile[t] = q;
   }
+  return q;
 }
 
 __global__ __launch_bounds__(kBlock) void k_g2p_adj_binned(MpmP P, int n,
@@ -1476,16 +1563,20 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj_binned(MpmP P, int n,
                                                            const int32_t* __restrict__ list,
                                                            const int32_t* __restrict__ n_list,
                                                            float4* __restrict__ tiles) {
-  // the brick's 6x6x6 grid-velocity tile in LDS (aliasing the staging area,
-  // reloaded per chunk): the stencils of its particles read only these nodes
-  GvTile src{nullptr, 0, 0, 0};
-  DT_BRICK_LOOP_BEGIN(load_gv_tile(P, b, gv, sgv, sh.gtile, src))
-  g2p_adj_particle<1>(P, n, x, src, gx1, gv1, gC1, gx, perm[i], s, bb, M);
-  DT_BRICK_LOOP_END(0.f)
+  // the brick's 6x6x6 grid-velocity tile sits in LDS (aliasing the staging
+  // area): the stencils of its particles read only these nodes
+  __shared__ BrickStage sh;
+  const int nb = (P.n_grid + 3) >> 2;
+  brick_loop<true, G2PAdjIn>(
+      sh, perm, start, list, n_list, tiles, 0.f,
+      [&](int p, G2PAdjIn& in) { g2p_adj_load(P, n, x, gx1, gv1, gC1, p, in); },
+      [&](const G2PAdjIn& in, int p, long long b, Stencil& s, float(&bb)[3], M3& M) {
+        const GvTile src{sh.gtile, (int)(b / ((long long)nb * nb)) * 4, (int)((b / nb) % nb) * 4,
+                         (int)(b % nb) * 4};
+        g2p_adj_compute<1>(P, n, in, src, gx, p, s, bb, M);
+      },
+      [&](long long b) { return fetch_gv_tile(P, b, gv, sgv); });
 }
-
-#undef DT_BRICK_LOOP_BEGIN
-#undef DT_BRICK_LOOP_END
 
 // ----------------------------------------------------------------------------
 // ordering helpers

@@ -475,11 +475,19 @@ def test_workspace_too_small_is_rejected(gpu):
         call("dt_mpm_bin", c_prm, 100, x, ws, nbytes - 256, _lib.stream())
 
 
-@pytest.mark.parametrize("kind,sort", [("multi", False), ("multi", True), ("von_mises", True)])
-def test_sparse_substep_matches_dense_and_oracle(gpu, kind, sort):
+@pytest.mark.parametrize("kind,sort,cluster", [("multi", False, False), ("multi", True, False),
+                                               ("von_mises", True, False),
+                                               ("multi", False, True)])
+def test_sparse_substep_matches_dense_and_oracle(gpu, kind, sort, cluster):
     """One substep forward and adjoint through dt_mpm_run_fwd / dt_mpm_run_adj,
-    sparse (binned, brick tiles) against dense (run scatter) and the oracle."""
+    sparse (binned, brick tiles) against dense (run scatter) and the oracle.
+    cluster: 700 particles share one cell (one brick holds more than two
+    256-particle chunks, one cell's slots accumulate over several chunks)."""
     prm, mats, mat_id, x, v, C, F = _facade_inputs(kind, n=6000, seed=31)
+    if cluster:
+        u = np.random.default_rng(5).uniform(size=(700, 3)).astype(np.float32)
+        x[:700] = ((16.6 + 0.8 * u) * prm.dx).astype(np.float32)
+        assert len(np.unique(om.bin_keys_np(x[:700], prm.inv_dx, prm.n_grid))) == 1
     if sort:
         o = np.argsort(om.bin_keys_np(x, prm.inv_dx, prm.n_grid), kind="stable")
         x, v, C, F, mat_id = x[o], v[o], C[o], F[o], mat_id[o]
