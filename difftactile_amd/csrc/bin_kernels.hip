@@ -112,23 +112,10 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float
     ld_v3(x, p, pl_vec(P.soa, n), xp);
     k = bin_key(xp, P.inv_dx, P.n_grid) >> 6;
   }
-  const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(k, 1);
-  const bool head = active && (lane == 0 || prev != k);
-  const unsigned long long hm = __ballot(head);
-  const unsigned long long act = __ballot(active);
-  const unsigned long long le = (2ull << lane) - 1ull;   // lanes <= this lane
-  const int my_head = active ? 63 - __clzll(hm & le) : 0;
-  int old = 0;
-  if (head) {
-    const unsigned long long later = hm & ~le;
-    const int end = later ? __ffsll((long long)later) - 1 : __popcll(act);
-    old = atomicAdd(&count[k], end - lane);
-  }
-  const int base = __shfl(old, my_head);
+  const int r = wave_run_rank(active, k, count);
   if (active) {
     key[p] = k;
-    rank[p] = base + (lane - my_head);
+    rank[p] = r;
   }
 }
 

@@ -46,6 +46,28 @@ BinWs bin_slot(const BinWs& ws, int s, int n);
 long long brick_total(const dt_mpm_params* prm);
 // Total bytes of the workspace for n particles; fills *ws when base != NULL.
 size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws);
+// Rank of this lane among the lanes of its key: lanes with equal keys in
+// consecutive lane order form a run, and the first lane of each run adds the
+// run's length to counts[key] with one atomic (LDS or global).  Active lanes
+// must form a prefix of the wavefront.  Returns the lane's slot among its key:
+// the count before the run's atomic plus its offset in the run.
+__device__ __forceinline__ int wave_run_rank(bool active, int key, int* counts) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1);
+  const bool head = active && (lane == 0 || prev != key);
+  const unsigned long long hm = __ballot(head);
+  const unsigned long long act = __ballot(active);
+  const unsigned long long le = (2ull << lane) - 1ull;   // lanes <= this lane
+  const int my_head = active ? 63 - __clzll(hm & le) : 0;
+  int old = 0;
+  if (head) {
+    const unsigned long long later = hm & ~le;
+    const int end = later ? __ffsll((long long)later) - 1 : __popcll(act);
+    old = atomicAdd(&counts[key], end - lane);
+  }
+  return __shfl(old, my_head) + (lane - my_head);
+}
+
 // Persistent grid of the brick-owned passes (blocks stride over the list).
 constexpr int kBrickBlocks = 2048;
 // count -> brick scan -> brick flags and occupied-brick list -> fill, on st.

@@ -1477,12 +1477,14 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
       Stencil s;
       float bb[3];
       M3 M;
-      int lc = 0, r = 0;
+      int lc = 0;
       if (i < c1) {
         compute(in, p, b, s, bb, M);
         lc = local_cell(s);
-        r = atomicAdd(&sh.hist[lc], 1);
       }
+      // the chunk's particles arrive mostly in cell order: one LDS atomic per
+      // run of equal cells
+      const int r = wave_run_rank(i < c1, lc, sh.hist);
       if (has_next) {
         p = pn;
         load(p, in);
@@ -1510,7 +1512,7 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_p2g_binned(MpmP P, const dt_material* __restrict__ mats,
+__global__ __launch_bounds__(kBlock, 3) void k_p2g_binned(MpmP P, const dt_material* __restrict__ mats,
                                                        int n, const int32_t* __restrict__ mat_id,
                                                        const float* __restrict__ x,
                                                        const float* __restrict__ v,
@@ -1551,7 +1553,7 @@ __device__ __forceinline__ float4 fetch_gv_tile(const MpmP& P, long long b, cons
   return q;
 }
 
-__global__ __launch_bounds__(kBlock) void k_g2p_adj_binned(MpmP P, int n,
+__global__ __launch_bounds__(kBlock, 3) void k_g2p_adj_binned(MpmP P, int n,
                                                            const float* __restrict__ x,
                                                            const float* __restrict__ gv, int sgv,
                                                            const float* __restrict__ gx1,
