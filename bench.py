@@ -114,7 +114,7 @@ def kernel_breakdown(torch, sim, state, mat_id, reps):
 
     def p2g_adj():
         _lib.call("dt_mpm_p2g_fused_adj", prm, mats, nm, n, mat_id, x, v, C, F, gg4,
-                  gnext[3], gout[0], gout[1], gout[2], gout[3], st)
+                  gnext[3], gout[0], gout[1], gout[2], gout[3], None, st)
 
     # order matters for valid inputs: each stage consumes the previous one's output
     stages = [("bin", binning), ("p2g_binned", p2g), ("grid_op4_tiles", grid_op), ("g2p4", g2p),

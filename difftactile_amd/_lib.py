@@ -63,18 +63,19 @@ _SIGS = {
     "dt_mpm_svd_adj": (_c_int, [_c_int] + [_vp] * 7 + [_vp]),
     "dt_mpm_p2g": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 10 + [_vp]),
     "dt_mpm_p2g_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 7
-                       + [_vp] * 3 + [_vp] * 7 + [_vp]),
+                       + [_vp] * 3 + [_vp] * 7 + [_vp, _vp]),
     "dt_mpm_grid_occupy": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp]),
     "dt_mpm_grid_op": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp, _vp, _vp, _vp]),
     "dt_mpm_grid_op_adj": (_c_int, [_P(DtMpmParams)] + [_vp] * 8 + [_vp]),
     "dt_mpm_g2p": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dt_mpm_g2p_adj": (_c_int, [_P(DtMpmParams), _c_int] + [_vp] * 7 + [_vp]),
     "dt_mpm_substep_fwd": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 10 + [_vp]),
-    "dt_mpm_substep_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 16 + [_vp]),
+    "dt_mpm_substep_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 16
+                           + [_vp, _vp]),
     "dt_mpm_run_fwd": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
                                 _c_int, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "dt_mpm_run_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
-                                _c_int] + [_vp] * 12 + [_vp, _sz, _vp]),
+                                _c_int] + [_vp] * 12 + [_vp, _sz, _vp, _vp]),
     "dt_mpm_advance": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 8
                        + [_c_int, _vp, _vp, _vp, _sz, _vp]),
     "dt_mpm_workspace_bytes": (_c_int, [_P(DtMpmParams), _c_int, _P(_sz)]),
@@ -100,7 +101,7 @@ _SIGS = {
     "dt_mpm_g2p4_adj": (_c_int, [_P(DtMpmParams), _c_int] + [_vp] * 7 + [_vp]),
     "dt_mpm_grid_op4_adj": (_c_int, [_P(DtMpmParams)] + [_vp] * 5 + [_vp]),
     "dt_mpm_p2g_fused_adj": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 10
-                             + [_vp]),
+                             + [_vp, _vp]),
     "dt_mpm_cell_keys": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp]),
     "dt_fem_init_B": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
     "dt_fem_update": (_c_int, [_P(DtFemParams), _c_int, _vp, _vp, _vp, _vp, _vp]),

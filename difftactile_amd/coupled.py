@@ -187,7 +187,7 @@ class CoupledSim:
                       pads_in[k][0], g_pads[k][1], g_pads[k][0], pad.vert_start, pad.vert_tets,
                       pad.tet_buf, s)
         _lib.call("dt_mpm_p2g_fused_adj", prm, mats, nm, sim.n, mid, *[P(t) for t in st], P(gg4),
-                  P(g_next[3]), P(g[0]), P(g[1]), P(g[2]), P(g[3]), s)
+                  P(g_next[3]), P(g[0]), P(g[1]), P(g[2]), P(g[3]), sim._mat_grad_ptr(), s)
 
     # -- public --------------------------------------------------------------
     def rollout(self, x, v, C, F, pad_states, control_vel, n_substeps):

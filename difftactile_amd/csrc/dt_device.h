@@ -444,7 +444,10 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
 
 // Adjoint of stress_fwd + svd (reference svd_grad) + nothing else.
 //   in : gtau, gFp_extra (adjoint of F_next = F_p), forward quantities
-//   out: gFn (adjoint of F_new)
+//   out: gFn (adjoint of F_new); gmu, glam (adjoints of the material's mu and
+//        lam: the stress terms and, inside the yield surface's return maps,
+//        the von Mises threshold tau_y / (2 mu) and the Drucker-Prager cone
+//        slope (3 lam + 2 mu) / (2 mu) alpha)
 // The U/V adjoints and the svd_grad ff-terms are evaluated in float64: the
 // ff_ij = 1/(s_j^2 - s_i^2) factors reach 1e8 for near-equal singular values
 // and the float32 products they multiply lose every digit there.
@@ -452,7 +455,8 @@ DT_HD double clampd(double a) { return a >= 0.0 ? fmax(a, 1e-8) : fmin(a, -1e-8)
 
 template <bool kExact>
 DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3& V, const Mat& m,
-                      const StressFwd& st, const M3& gtau, const M3& gFp_in, M3& gFn) {
+                      const StressFwd& st, const M3& gtau, const M3& gFp_in, M3& gFn, float& gmu,
+                      float& glam) {
   // Scalar type of the return-map and pair-coefficient terms: the exact form's
   // terms are well conditioned (no small denominators) and run in float32;
   // the reference form keeps float64 for its 1/(s_j^2 - s_i^2) factors.
@@ -462,11 +466,16 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
   for (int i = 0; i < 9; ++i) D.a[i] = st.Fp.a[i] - st.R.a[i];
   M3 gFp = m3_mul(gtau, st.Fp);           // gtau Fp
   M3 t2 = m3_mul_at(gtau, D);             // gtau^T (Fp - R)
+  M3 gR = gFp;
+  // d tau / d mu = 2 (Fp - R) Fp^T: gtau : that = 2 (gtau Fp) : (Fp - R)
+  float gm = 0.f;
+  for (int i = 0; i < 9; ++i) gm += gFp.a[i] * D.a[i];
   for (int i = 0; i < 9; ++i) gFp.a[i] = 2.f * m.mu * (gFp.a[i] + t2.a[i]) + gFp_in.a[i];
-  M3 gR = m3_mul(gtau, st.Fp);
   for (int i = 0; i < 9; ++i) gR.a[i] *= -2.f * m.mu;
   float trg = gtau.a[0] + gtau.a[4] + gtau.a[8];
   float gJ = m.lam * (2.f * st.J - 1.f) * trg;
+  gmu = 2.f * gm;
+  glam = trg * st.J * (st.J - 1.f);
 
   R gsig[3] = {R(0), R(0), R(0)};
   // U/V adjoints of the return-map path (F_p = U diag(d) V^T), reference form
@@ -512,15 +521,17 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
         const R gdot = ge[0] * eh[0] + ge[1] * eh[1] + ge[2] * eh[2];
         R geps[3], geh[3], gtr = R(0);
         if (st.mode == 1) {
-          // eps' = eps - eh + (c/n) eh, c = tau_y / (2 mu)
+          // eps' = eps - eh + (c/n) eh, c = tau_y / (2 mu); dc/dmu = -c / mu
           const R c = (R)m.yield_stress * rcp_r(R(2) * (R)m.mu);
           const R rn = rcp_r(n);
           for (int i = 0; i < 3; ++i) {
             geps[i] = ge[i];
             geh[i] = (c * rn - R(1)) * ge[i] - c * gdot * (rn * rn * rn) * eh[i];
           }
+          gmu += (float)(-gdot * rn * c * rcp_r((R)m.mu));
         } else {
-          // eps' = eps - eh - r tr eh / n, r = (3 lam + 2 mu) / (2 mu) alpha
+          // eps' = eps - eh - r tr eh / n, r = (3 lam + 2 mu) / (2 mu) alpha;
+          // dr/dlam = 3 alpha / (2 mu), dr/dmu = -3 alpha lam / (2 mu^2)
           const R r = (R(3) * (R)m.lam + R(2) * (R)m.mu) * rcp_r(R(2) * (R)m.mu) * (R)m.dp_alpha;
           const R tr = st.trv;
           const R rn = rcp_r(n);
@@ -529,6 +540,10 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
             geh[i] = -ge[i] - r * tr * (ge[i] * rn - gdot * eh[i] * (rn * rn * rn));
           }
           gtr = -r * gdot * rn;
+          const R gr = -tr * gdot * rn;
+          const R h = R(3) * (R)m.dp_alpha * rcp_r(R(2) * (R)m.mu);
+          glam += (float)(gr * h);
+          gmu += (float)(-gr * h * (R)m.lam * rcp_r((R)m.mu));
         }
         const R mean = (geh[0] + geh[1] + geh[2]) * (R(1) / R(3));
         for (int i = 0; i < 3; ++i) {
@@ -617,5 +632,12 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
   }
   const M3 G = m3_mul_bt(m3_mul(U, Mi), V);
   for (int i = 0; i < 9; ++i) gFn.a[i] += G.a[i];
+}
+
+template <bool kExact>
+DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3& V, const Mat& m,
+                      const StressFwd& st, const M3& gtau, const M3& gFp_in, M3& gFn) {
+  float gmu, glam;
+  stress_adj<kExact>(Fn, U, sig, V, m, st, gtau, gFp_in, gFn, gmu, glam);
 }
 }  // namespace dt

@@ -573,8 +573,46 @@ __device__ __forceinline__ void affine_adj(const MpmP& P, const M3& gA, M3& gtau
   }
 }
 
-// p2g.grad (reference kernel form): emits gU, gS, gV for svd_grad, and the
-// direct F_new adjoint.
+// Adds the block's per-particle material adjoints (gmu, glam) into
+// g_mats[2 id], g_mats[2 id + 1]: per wavefront one shuffle sum per material
+// present, one LDS atomic per wavefront and material, then one global atomic
+// per block and material (ids beyond the LDS table add per wavefront).  Every
+// thread of the block calls it.
+constexpr int kMatSlots = 16;
+__device__ __forceinline__ void block_mat_grad(bool active, int id, float gmu, float glam,
+                                               int n_mats, float* __restrict__ g_mats) {
+  __shared__ float red[2 * kMatSlots];
+  const int t = threadIdx.x, lane = t & 63;
+  if (t < 2 * kMatSlots) red[t] = 0.f;
+  __syncthreads();
+  unsigned long long todo = __ballot(active);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int cur = __shfl(id, leader);
+    const bool mine = active && id == cur;
+    float a = mine ? gmu : 0.f, b = mine ? glam : 0.f;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off);
+      b += __shfl_xor(b, off);
+    }
+    if (lane == leader) {
+      if (cur < kMatSlots) {
+        atomicAdd(&red[2 * cur], a);
+        atomicAdd(&red[2 * cur + 1], b);
+      } else {
+        atomicAdd(&g_mats[2 * cur], a);
+        atomicAdd(&g_mats[2 * cur + 1], b);
+      }
+    }
+    todo &= ~__ballot(mine);
+  }
+  __syncthreads();
+  if (t < 2 * min(n_mats, kMatSlots) && red[t] != 0.f) atomicAdd(&g_mats[t], red[t]);
+}
+
+// p2g.grad (reference kernel form): emits gU, gS, gV for svd_grad, the direct
+// F_new adjoint and, when g_mats is non-null, the material adjoints.
 __global__ __launch_bounds__(kBlock) void k_p2g_adj(
     MpmP P, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
     const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ C,
@@ -582,9 +620,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
     const float* __restrict__ V, const float* __restrict__ gmv, const float* __restrict__ gm,
     const float* __restrict__ gFnext, float* __restrict__ gx, float* __restrict__ gv,
     float* __restrict__ gC, float* __restrict__ gFn, float* __restrict__ gU,
-    float* __restrict__ gS, float* __restrict__ gV) {
+    float* __restrict__ gS, float* __restrict__ gV, int n_mats, float* __restrict__ g_mats) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  const bool active = p < n;
+  float gmu = 0.f, glam = 0.f;
+  if (active) {
   M3 fn = ld_m3(Fn + 9ll * p), u = ld_m3(U + 9ll * p), vv = ld_m3(V + 9ll * p);
   M3 sm = ld_m3(S + 9ll * p);
   float sig[3] = {sm.a[0], sm.a[4], sm.a[8]};
@@ -623,6 +663,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
   for (int i = 0; i < 9; ++i) gR.a[i] *= -2.f * m.mu;
   float trg = gtau.a[0] + gtau.a[4] + gtau.a[8];
   float gJ = m.lam * (2.f * st.J - 1.f) * trg;
+  // d tau / d mu = 2 (Fp - R) Fp^T, d tau / d lam = J (J - 1) I
+  {
+    const M3 gtF = m3_mul(gtau, st.Fp);
+    for (int i = 0; i < 9; ++i) gmu += 2.f * gtF.a[i] * D.a[i];
+    glam = trg * st.J * (st.J - 1.f);
+  }
   M3d gUd = m3d_mul(m3d_from(gR), m3d_from(vv));
   M3d gVd = m3d_mul_at(m3d_from(gR), m3d_from(u));
   double gsig[3] = {0.0, 0.0, 0.0};
@@ -663,6 +709,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
             geps[i] = ge[i];
             geh[i] = (cc / nn - 1.0) * ge[i] - cc * gdot / (nn * nn * nn) * eh[i];
           }
+          // threshold cc = tau_y / (2 mu): d eps' / d cc = eh / n
+          gmu += (float)(-gdot / nn * cc / m.mu);
         } else {
           double r = (3.0 * m.lam + 2.0 * m.mu) / (2.0 * m.mu) * m.dp_alpha;
           double tr = st.trv;
@@ -671,6 +719,10 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
             geh[i] = -ge[i] - r * tr * (ge[i] / nn - gdot * eh[i] / (nn * nn * nn));
           }
           gtr = -r * gdot / nn;
+          // cone slope r = (3 lam + 2 mu) / (2 mu) alpha: d eps' / d r = -tr eh / n
+          const double gr = -tr * gdot / nn, h = 3.0 * m.dp_alpha / (2.0 * m.mu);
+          glam += (float)(gr * h);
+          gmu += (float)(-gr * h * m.lam / m.mu);
         }
         double mean = (geh[0] + geh[1] + geh[2]) / 3.0;
         for (int i = 0; i < 3; ++i) {
@@ -692,6 +744,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
   add_m3(gU + 9ll * p, gu);
   add_m3(gV + 9ll * p, gvv);
   add_m3(gS + 9ll * p, gs);
+  }
+  if (g_mats) block_mat_grad(active, active ? (mat_id ? mat_id[p] : 0) : 0, gmu, glam, n_mats, g_mats);
 }
 
 // fused adjoint of (compute_new_F + svd + return map + p2g): writes the
@@ -709,16 +763,14 @@ __device__ __forceinline__ void opaque(M3& m) {
 // and parks U, sigma, V in LDS (one column per thread); the second reloads them
 // and recomputes only the return map for the stress adjoint.
 template <bool kExact>
-__global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
-    MpmP P, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
+__device__ __forceinline__ void p2g_fused_adj_particle(
+    const MpmP& P, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
     const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ C,
     const float* __restrict__ F, const float* __restrict__ ggrid4,
     const float* __restrict__ gFnext, float* __restrict__ gx_io, float* __restrict__ gv_out,
-    float* __restrict__ gC_out, float* __restrict__ gF_out) {
-  __shared__ float park[21][kBlock];   // U (9), sigma (3), V (9) of each thread's particle
+    float* __restrict__ gC_out, float* __restrict__ gF_out, int p, float (*park)[kBlock],
+    float& gmu, float& glam) {
   const int t = threadIdx.x;
-  int p = xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= n) return;
   PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
   const Mat m = load_mat(mats, mat_id, p);
   M3 gtau, lgC;
@@ -776,12 +828,38 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
   M3 tau;
   stress_fwd(fn, u, sig, vv, m, st, tau);
   M3 gFn;
-  stress_adj<kExact>(fn, u, sig, vv, m, st, gtau, ld_m3L(gFnext, p, Lm), gFn);
+  stress_adj<kExact>(fn, u, sig, vv, m, st, gtau, ld_m3L(gFnext, p, Lm), gFn, gmu, glam);
   M3 gc2 = m3_mul_bt(gFn, f);
   for (int i = 0; i < 9; ++i) lgC.a[i] += P.dt * gc2.a[i];
   M3 gf = m3_mul_at(a, gFn);
   st_m3L(gC_out, p, Lm, lgC);
   st_m3L(gF_out, p, Lm, gf);
+}
+
+// kMatGrad: also adds the material adjoints (mu, lam per material) into
+// g_mats[n_mats][2]
+template <bool kExact, bool kMatGrad>
+__global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
+    MpmP P, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
+    const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ C,
+    const float* __restrict__ F, const float* __restrict__ ggrid4,
+    const float* __restrict__ gFnext, float* __restrict__ gx_io, float* __restrict__ gv_out,
+    float* __restrict__ gC_out, float* __restrict__ gF_out, int n_mats,
+    float* __restrict__ g_mats) {
+  __shared__ float park[21][kBlock];   // U (9), sigma (3), V (9) of each thread's particle
+  const int p = xcd_block() * blockDim.x + threadIdx.x;
+  float gmu = 0.f, glam = 0.f;
+  if (!kMatGrad) {
+    if (p >= n) return;
+    p2g_fused_adj_particle<kExact>(P, mats, n, mat_id, x, v, C, F, ggrid4, gFnext, gx_io, gv_out,
+                                   gC_out, gF_out, p, park, gmu, glam);
+    return;
+  }
+  const bool active = p < n;
+  if (active)
+    p2g_fused_adj_particle<kExact>(P, mats, n, mat_id, x, v, C, F, ggrid4, gFnext, gx_io, gv_out,
+                                   gC_out, gF_out, p, park, gmu, glam);
+  block_mat_grad(active, active ? (mat_id ? mat_id[p] : 0) : 0, gmu, glam, n_mats, g_mats);
 }
 
 // ----------------------------------------------------------------------------
@@ -1675,16 +1753,22 @@ static void launch_mark(const MpmP& P, const dt_mpm_params* prm, int n, const fl
     hipLaunchKernelGGL(k_mark_bricks, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, flags);
 }
 
-static void launch_p2g_fused_adj(const MpmP& P, hipStream_t st, const dt_material* mats, int n,
-                                 const int32_t* mat_id, const float* x, const float* v,
-                                 const float* C, const float* F, const float* ggrid4,
-                                 const float* gF1, float* gx, float* gv, float* gC, float* gF) {
-  if (P.svd_exact)
-    hipLaunchKernelGGL(k_p2g_fused_adj<true>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n,
-                       mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF);
-  else
-    hipLaunchKernelGGL(k_p2g_fused_adj<false>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n,
-                       mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF);
+static void launch_p2g_fused_adj(const MpmP& P, hipStream_t st, const dt_material* mats,
+                                 int n_mats, int n, const int32_t* mat_id, const float* x,
+                                 const float* v, const float* C, const float* F,
+                                 const float* ggrid4, const float* gF1, float* gx, float* gv,
+                                 float* gC, float* gF, float* g_mats) {
+#define DT_P2G_ADJ(E, G)                                                                       \
+  hipLaunchKernelGGL((k_p2g_fused_adj<E, G>), dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n, \
+                     mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF, n_mats, g_mats)
+  if (P.svd_exact) {
+    if (g_mats) DT_P2G_ADJ(true, true);
+    else DT_P2G_ADJ(true, false);
+  } else {
+    if (g_mats) DT_P2G_ADJ(false, true);
+    else DT_P2G_ADJ(false, false);
+  }
+#undef DT_P2G_ADJ
 }
 
 static unsigned active_blocks(const BinWs& ws) {
@@ -1745,12 +1829,13 @@ int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
   return check_launch("dt_mpm_substep_fwd");
 }
 
-int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
+int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n_mats, int n,
                        const int32_t* mat_id, const float* x, const float* v, const float* C,
                        const float* F, const float* grid4, const float* grid_v4,
                        const float* gx1, const float* gv1, const float* gC1, const float* gF1,
                        float* ggrid4, float* ggv4, float* gx, float* gv, float* gC, float* gF,
-                       hipStream_t st, const BinWs* ws = nullptr, bool binned = false) {
+                       float* g_mats, hipStream_t st, const BinWs* ws = nullptr,
+                       bool binned = false) {
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
   if (ws) {
@@ -1782,7 +1867,8 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n,
   }
   if (n > 0) {
     StageTimer tm(STAGE_P2G_ADJ, st);
-    launch_p2g_fused_adj(P, st, mats, n, mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF);
+    launch_p2g_fused_adj(P, st, mats, n_mats, n, mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF,
+                         g_mats);
   }
   return check_launch("dt_mpm_substep_adj");
 }
@@ -1915,7 +2001,7 @@ int dt_mpm_p2g_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                    const float* F_new, const float* U, const float* S, const float* V,
                    const float* g_grid_mv, const float* g_grid_m, const float* gF_next, float* gx,
                    float* gv, float* gC, float* gF_new, float* gU, float* gS, float* gV,
-                   void* stream) {
+                   float* g_mats, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
@@ -1927,7 +2013,7 @@ int dt_mpm_p2g_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
              "NULL pointer");
   hipLaunchKernelGGL(k_p2g_adj, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream),
                      to_dev(*prm), mats, n, mat_id, x, v, C, F_new, U, S, V, g_grid_mv, g_grid_m,
-                     gF_next, gx, gv, gC, gF_new, gU, gS, gV);
+                     gF_next, gx, gv, gC, gF_new, gU, gS, gV, n_mats, g_mats);
   return check_launch("dt_mpm_p2g_adj");
 }
 
@@ -2012,7 +2098,7 @@ int dt_mpm_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int32_
                        const float* C, const float* F, const float* grid4, const float* grid_v4,
                        const float* gx_next, const float* gv_next, const float* gC_next,
                        const float* gF_next, float* g_grid4, float* g_grid_v4, float* gx,
-                       float* gv, float* gC, float* gF, void* stream) {
+                       float* gv, float* gC, float* gF, float* g_mats, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
@@ -2021,8 +2107,8 @@ int dt_mpm_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int32_
   DT_REQUIRE(n == 0 || (x && v && C && F && gx_next && gv_next && gC_next && gF_next && gx && gv &&
                         gC && gF),
              "NULL pointer");
-  return launch_substep_adj(prm, mats, n, mat_id, x, v, C, F, grid4, grid_v4, gx_next, gv_next,
-                            gC_next, gF_next, g_grid4, g_grid_v4, gx, gv, gC, gF,
+  return launch_substep_adj(prm, mats, n_mats, n, mat_id, x, v, C, F, grid4, grid_v4, gx_next,
+                            gv_next, gC_next, gF_next, g_grid4, g_grid_v4, gx, gv, gC, gF, g_mats,
                             DT_STREAM(stream));
 }
 
@@ -2078,7 +2164,7 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                    const float* F, int32_t n_substeps, const float* grids4, const float* grids_v4,
                    float* g_grid4, float* g_grid_v4, float* gx, float* gv, float* gC, float* gF,
                    float* gx2, float* gv2, float* gC2, float* gF2, void* workspace,
-                   size_t workspace_bytes, void* stream) {
+                   size_t workspace_bytes, float* g_mats, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
@@ -2099,10 +2185,10 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
       long long o = (long long)s * n;
       BinWs slot = hist ? bin_slot(ws, s, n) : ws;
       const BinWs* wp = workspace ? &slot : nullptr;
-      int r = launch_substep_adj(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
-                                 grids4 + (long long)s * nn * 4, grids_v4 + (long long)s * nn * 4,
-                                 p[0], p[1], p[2], p[3], g_grid4, g_grid_v4, q[0], q[1], q[2], q[3],
-                                 cs, wp, hist);
+      int r = launch_substep_adj(prm, mats, n_mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o,
+                                 F + 9 * o, grids4 + (long long)s * nn * 4,
+                                 grids_v4 + (long long)s * nn * 4, p[0], p[1], p[2], p[3], g_grid4,
+                                 g_grid_v4, q[0], q[1], q[2], q[3], g_mats, cs, wp, hist);
       if (r) return r;
       for (int k = 0; k < 4; ++k) { float* t = p[k]; p[k] = q[k]; q[k] = t; }
     }
@@ -2115,7 +2201,7 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                           (const void*)grids_v4, (const void*)g_grid4, (const void*)g_grid_v4,
                           (const void*)gx, (const void*)gv, (const void*)gC, (const void*)gF,
                           (const void*)gx2, (const void*)gv2, (const void*)gC2, (const void*)gF2,
-                          (const void*)workspace})
+                          (const void*)workspace, (const void*)g_mats})
     key_add(key, ptr);
   key_add(key, (long long)n_mats << 32 | (uint32_t)n);
   key_add(key, (long long)n_substeps);
@@ -2259,15 +2345,15 @@ int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int3
                          int32_t n, const int32_t* mat_id, const float* x, const float* v,
                          const float* C, const float* F, const float* g_grid4,
                          const float* gF_next, float* gx, float* gv, float* gC, float* gF,
-                         void* stream) {
+                         float* g_mats, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
   DT_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return DT_OK;
   DT_REQUIRE(x && v && C && F && g_grid4 && gF_next && gx && gv && gC && gF, "NULL pointer");
-  launch_p2g_fused_adj(to_dev(*prm), DT_STREAM(stream), mats, n, mat_id, x, v, C, F, g_grid4,
-                       gF_next, gx, gv, gC, gF);
+  launch_p2g_fused_adj(to_dev(*prm), DT_STREAM(stream), mats, n_mats, n, mat_id, x, v, C, F,
+                       g_grid4, gF_next, gx, gv, gC, gF, g_mats);
   return check_launch("dt_mpm_p2g_fused_adj");
 }
 

@@ -197,6 +197,8 @@ class MPMBase:
 
     def _p2g_grad(self, f):
         mats = self._mat_table()
+        # material adjoints of this launch, added into mu_0.grad / lambda_0.grad
+        gm = torch.zeros((mats.shape[0], 2), dtype=torch.float32, device=self.device)
         _lib.call("dt_mpm_p2g_adj", self.prm.c_struct(), _lib.ptr(mats), mats.shape[0],
                   self._n(), self._mid(), self._p(self.x_0, f), self._p(self.v_0, f),
                   self._p(self.C_0, f), self._p(self.F_new, f), self._p(self.U_svd, f),
@@ -204,7 +206,9 @@ class MPMBase:
                   self._pg(self.grid_m, f), self._pg(self.F_0, f + 1), self._pg(self.x_0, f),
                   self._pg(self.v_0, f), self._pg(self.C_0, f), self._pg(self.F_new, f),
                   self._pg(self.U_svd, f), self._pg(self.S_svd, f), self._pg(self.V_svd, f),
-                  _lib.stream())
+                  _lib.ptr(gm), _lib.stream())
+        self.mu_0.grad.data += gm[:, 0].reshape(self.mu_0.grad.data.shape)
+        self.lambda_0.grad.data += gm[:, 1].reshape(self.lambda_0.grad.data.shape)
 
     def _check_grid_occupy(self, f):
         _lib.call("dt_mpm_grid_occupy", self.prm.c_struct(), self._p(self.grid_m, f),

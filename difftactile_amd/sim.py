@@ -40,7 +40,8 @@ class MPMSim:
     """One MPM object on the GPU: parameters, material table, per-particle ids."""
 
     def __init__(self, prm, materials, n_particles, mat_id=None, device="cuda",
-                 checkpoint_every=50, sort=True, sparse_grid=True, resident_bytes=None):
+                 checkpoint_every=50, sort=True, sparse_grid=True, resident_bytes=None,
+                 material_grad=False):
         self.prm = prm
         if resident_bytes is None:
             # a quarter of the device's memory for resident segments
@@ -70,6 +71,18 @@ class MPMSim:
         self.checkpoint_every = int(checkpoint_every)
         self.sort = bool(sort)
         self._ws = {}
+        # material_grad: backward passes accumulate d loss / d (mu, lam) of each
+        # material row here (cleared by zero_material_grad), as the reference's
+        # mu_0.grad / lambda_0.grad accumulate
+        self.material_grad = (torch.zeros((len(self.materials), 2), device=self.device)
+                              if material_grad else None)
+
+    def zero_material_grad(self):
+        if self.material_grad is not None:
+            self.material_grad.zero_()
+
+    def _mat_grad_ptr(self):
+        return None if self.material_grad is None else _lib.ptr(self.material_grad)
 
     # -- workspace -------------------------------------------------------
     def _buf(self, name, shape, dtype=torch.float32):
@@ -183,7 +196,7 @@ class MPMSim:
                   self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
                   _lib.ptr(G4), _lib.ptr(GV4), _lib.ptr(gg4), _lib.ptr(ggv4),
                   *[_lib.ptr(t) for t in g], *[_lib.ptr(t) for t in scratch],
-                  *self._workspace(k), _lib.stream())
+                  *self._workspace(k), self._mat_grad_ptr(), _lib.stream())
 
     def _frames(self, k, prefix="F"):
         n = self.n

@@ -69,6 +69,15 @@ typedef struct dt_material {
   int32_t pad[3];
 } dt_material;
 
+/* Material adjoints.  The adjoint entry points that take g_mats (a device
+   array of n_mats x 2 floats, or NULL to skip) ACCUMULATE d/d mu and d/d lam
+   of each material row into g_mats[2 id] and g_mats[2 id + 1], as Taichi's
+   p2g.grad accumulates mu_0.grad / lambda_0.grad (mpm_elastic.py:68-69,
+   multi_obj.py:67-68): the stress terms plus, inside an active return map,
+   the von Mises threshold tau_y / (2 mu) (mpm_plastic.py:157-161) and the
+   Drucker-Prager cone slope (3 lam + 2 mu) / (2 mu) alpha.  The sums are
+   float atomics (order not fixed). */
+
 /* FEM pad material and integration constants (fem_sensor.py:35-49). */
 typedef struct dt_fem_params {
   float mu, lam;        /* Lame parameters of the stable Neo-Hookean model          */
@@ -133,13 +142,14 @@ int dt_mpm_p2g(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats
                float* grid_m, float* F_next, void* stream);
 /* p2g.grad: reads d/d grid_mv, d/d grid_m, d/d F_next; accumulates into
    gx, gv, gC, gF_new, gU, gS, gV (the svd factors' adjoints, consumed by
-   dt_mpm_svd_adj as the reference's svd_grad consumes U/S/V_svd.grad). */
+   dt_mpm_svd_adj as the reference's svd_grad consumes U/S/V_svd.grad) and
+   g_mats (material adjoints, may be NULL). */
 int dt_mpm_p2g_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, const float* x, const float* v, const float* C,
                    const float* F_new, const float* U, const float* S, const float* V,
                    const float* g_grid_mv, const float* g_grid_m, const float* gF_next, float* gx,
                    float* gv, float* gC, float* gF_new, float* gU, float* gS, float* gV,
-                   void* stream);
+                   float* g_mats, void* stream);
 
 /* check_grid_occupy, mpm_elastic.py:210-214: occupy = 1 where grid_m > eps
    (only ever sets 1; the caller clears, as the reference's reset does) */
@@ -177,13 +187,14 @@ int dt_mpm_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_
                        float* x_next, float* v_next, float* C_next, float* F_next, void* stream);
 /* Adjoint of dt_mpm_substep_fwd given its grid4 / grid_v4 outputs: reads the
    next state's adjoints and OVERWRITES gx, gv, gC, gF with the adjoints of
-   the state (x, v, C, F).  g_grid4, g_grid_v4: n^3 float4 scratch. */
+   the state (x, v, C, F).  g_grid4, g_grid_v4: n^3 float4 scratch.  g_mats:
+   material adjoints, accumulated (may be NULL). */
 int dt_mpm_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats,
                        int32_t n, const int32_t* mat_id, const float* x, const float* v,
                        const float* C, const float* F, const float* grid4, const float* grid_v4,
                        const float* gx_next, const float* gv_next, const float* gC_next,
                        const float* gF_next, float* g_grid4, float* g_grid_v4, float* gx,
-                       float* gv, float* gC, float* gF, void* stream);
+                       float* gv, float* gC, float* gF, float* g_mats, void* stream);
 
 /* Stages of the fused substep, for callers that couple other work between
    them (pads and contact run between p2g and grid_op):
@@ -210,12 +221,12 @@ int dt_mpm_g2p4_adj(const dt_mpm_params* prm, int32_t n, const float* x, const f
 int dt_mpm_grid_op4_adj(const dt_mpm_params* prm, const float* grid4, const float* grid_f,
                         const float* g_grid_v4, float* g_grid4, float* g_grid_f, void* stream);
 /* adjoint of dt_mpm_p2g_fused: gx holds the g2p partial on entry and the full
-   adjoint on return; OVERWRITES gv, gC, gF */
+   adjoint on return; OVERWRITES gv, gC, gF; accumulates g_mats (may be NULL) */
 int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats,
                          int32_t n, const int32_t* mat_id, const float* x, const float* v,
                          const float* C, const float* F, const float* g_grid4,
                          const float* gF_next, float* gx, float* gv, float* gC, float* gF,
-                         void* stream);
+                         float* g_mats, void* stream);
 
 /* Native trajectory loops over a frame-major state buffer: frame k of x is
    at x + k*n*3 (likewise v, C (n*9), F (n*9)).  dt_mpm_run_fwd advances
@@ -234,13 +245,14 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                    float* scratch_v4, void* workspace, size_t workspace_bytes, void* stream);
 /* Reverse sweep over frames [0, n_substeps] stored by dt_mpm_run_fwd with
    grids kept.  Seeds: gx..gF hold the adjoint of frame n_substeps on entry and
-   the adjoint of frame 0 on return; gx2..gF2 are same-size scratch. */
+   the adjoint of frame 0 on return; gx2..gF2 are same-size scratch.  g_mats
+   accumulates the material adjoints over the sweep (may be NULL). */
 int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, const float* x, const float* v, const float* C,
                    const float* F, int32_t n_substeps, const float* grids4, const float* grids_v4,
                    float* g_grid4, float* g_grid_v4, float* gx, float* gv, float* gC, float* gF,
                    float* gx2, float* gv2, float* gC2, float* gF2, void* workspace,
-                   size_t workspace_bytes, void* stream);
+                   size_t workspace_bytes, float* g_mats, void* stream);
 
 /* Advances the state (x, v, C, F) by n_substeps in place, ping-ponging with
    the same-size scratch state (x2, v2, C2, F2); grids are single-substep

@@ -46,6 +46,16 @@ def oracle_mats(mats, mat_id, dtype):
     return om.material_arrays(oms, mat_id, dtype)
 
 
+def oracle_mats_leaf(mats, mat_id, dtype):
+    """oracle_mats with the per-material (mu, lam) as leaf tensors, so reverse
+    mode yields d/d mu and d/d lam of each material row."""
+    mu_t = torch.tensor([m.mu for m in mats], dtype=dtype, requires_grad=True)
+    lam_t = torch.tensor([m.lam for m in mats], dtype=dtype, requires_grad=True)
+    _, _, kind, yld, alpha = oracle_mats(mats, mat_id, dtype)
+    idx = torch.as_tensor(mat_id, dtype=torch.int64)
+    return (mu_t[idx], lam_t[idx], kind, yld, alpha), mu_t, lam_t
+
+
 def rel_l2(a, b):
     a = torch.as_tensor(a, dtype=torch.float64).cpu()
     b = torch.as_tensor(b, dtype=torch.float64).cpu()

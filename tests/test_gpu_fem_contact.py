@@ -209,6 +209,11 @@ def test_coupled_substep_forward_and_adjoint_match_oracle(gpu):
     op.svd_exact = False
     n = len(x)
     mats = om.material_arrays([om.Material(5e4, 0.3)], np.zeros(n, int), F64)
+    # (mu, lam) as leaves: p2g.grad accumulates mu_0.grad / lambda_0.grad
+    mu_t = torch.tensor([mats[0][0].item()], dtype=F64, requires_grad=True)
+    lam_t = torch.tensor([mats[1][0].item()], dtype=F64, requires_grad=True)
+    zi = torch.zeros(n, dtype=torch.int64)
+    mats = (mu_t[zi], lam_t[zi]) + tuple(mats[2:])
     st = [torch.tensor(x, dtype=F64, requires_grad=True), torch.tensor(v, dtype=F64, requires_grad=True),
           torch.zeros(n, 3, 3, dtype=F64, requires_grad=True),
           torch.eye(3, dtype=F64).repeat(n, 1, 1).requires_grad_(True)]
@@ -239,10 +244,11 @@ def test_coupled_substep_forward_and_adjoint_match_oracle(gpu):
     torch.cuda.synchronize()
     loss = sum((o * seeds[k]).sum() for o, k in zip(out, ("x", "v", "C", "F")))
     loss = loss + sum((o * seeds[k]).sum() for o, k in zip(pads_out[0], ("pos", "vel", "vpos")))
-    leaves = st + [padd["pos"], padd["vel"]]
+    leaves = st + [padd["pos"], padd["vel"], mu_t, lam_t]
     gr = torch.autograd.grad(loss, leaves)
     gots = [obj.x_0.grad.data[0], obj.v_0.grad.data[0], obj.C_0.grad.data[0],
-            obj.F_0.grad.data[0], pad.pos.grad.data[0], pad.vel.grad.data[0]]
+            obj.F_0.grad.data[0], pad.pos.grad.data[0], pad.vel.grad.data[0],
+            obj.mu_0.grad.data.reshape(1), obj.lambda_0.grad.data.reshape(1)]
     for k, (got, want) in enumerate(zip(gots, gr)):
         err = rel_l2(h64(got), want)
         assert err < 1e-4, (k, err)

@@ -13,7 +13,8 @@ from difftactile_amd import _lib
 from difftactile_amd.params import material_table
 from difftactile_amd.sim import MPMSim
 from oracle import mpm as om
-from tests.mpm_cases import blob, default_prm, materials, oracle_mats, oracle_params, rel_l2
+from tests.mpm_cases import (blob, default_prm, materials, oracle_mats, oracle_mats_leaf,
+                              oracle_params, rel_l2)
 
 pytestmark = pytest.mark.gpu
 
@@ -246,18 +247,22 @@ def test_fused_substep_adjoint_matches_oracle_reverse_mode(gpu, kind):
     gg4 = torch.empty(nn, 4, device="cuda")
     ggv4 = torch.empty(nn, 4, device="cuda")
     gout = [torch.full_like(t, float("nan")) for t in state]
+    gmat = torch.zeros(len(mats), 2, device="cuda")
     call("dt_mpm_substep_adj", prm.c_struct(), _lib.ptr(mt), len(mats), n,
          _lib.ptr(mid, torch.int32), *[_lib.ptr(t) for t in state], _lib.ptr(g4), _lib.ptr(gv4),
          *[_lib.ptr(t) for t in gnext], _lib.ptr(gg4), _lib.ptr(ggv4),
-         *[_lib.ptr(t) for t in gout], _lib.stream())
-    # oracle reverse mode in FP64
+         *[_lib.ptr(t) for t in gout], _lib.ptr(gmat), _lib.stream())
+    # oracle reverse mode in FP64, including the per-material (mu, lam)
     st = [torch.tensor(a, dtype=F64, requires_grad=True) for a in (x, v, C, F)]
-    out = om.substep(oracle_params(prm), tuple(st), oracle_mats(mats, mat_id, F64))
+    omats, mu_t, lam_t = oracle_mats_leaf(mats, mat_id, F64)
+    out = om.substep(oracle_params(prm), tuple(st), omats)
     loss = sum((o * s.to(F64)).sum() for o, s in zip(out, seeds))
-    grads = torch.autograd.grad(loss, st)
+    grads = torch.autograd.grad(loss, st + [mu_t, lam_t])
     for name, got, want in zip("xvCF", gout, grads):
         err = rel_l2(host64(got), want)
         assert err < 1e-4, (name, err)
+    err = rel_l2(host64(gmat), torch.stack([grads[4], grads[5]], 1))
+    assert err < 1e-4, ("mu/lam", err, host64(gmat), grads[4], grads[5])
 
 
 def test_reference_form_adjoint_chain_matches_fused(gpu):
@@ -279,10 +284,12 @@ def test_reference_form_adjoint_chain_matches_fused(gpu):
                                        zeros(Fg), zeros(Fg), zeros(Fg))
     gmv = ggrid[:, :3].contiguous()
     gm = ggrid[:, 3].contiguous()
+    gmat = torch.zeros(len(mats), 2, device="cuda")
     call("dt_mpm_p2g_adj", prm.c_struct(), _lib.ptr(mt), len(mats), n, _lib.ptr(mid, torch.int32),
          _lib.ptr(xg), _lib.ptr(vg), _lib.ptr(Cg), _lib.ptr(Fn), _lib.ptr(U), _lib.ptr(S),
          _lib.ptr(V), _lib.ptr(gmv), _lib.ptr(gm), _lib.ptr(gF1), _lib.ptr(gx), _lib.ptr(gv),
-         _lib.ptr(gC), _lib.ptr(gFn), _lib.ptr(gU), _lib.ptr(gS), _lib.ptr(gV), _lib.stream())
+         _lib.ptr(gC), _lib.ptr(gFn), _lib.ptr(gU), _lib.ptr(gS), _lib.ptr(gV), _lib.ptr(gmat),
+         _lib.stream())
     call("dt_mpm_svd_adj", n, _lib.ptr(U), _lib.ptr(S), _lib.ptr(V), _lib.ptr(gU), _lib.ptr(gS),
          _lib.ptr(gV), _lib.ptr(gFn), _lib.stream())
     call("dt_mpm_compute_new_F_adj", n, _lib.ptr(Cg), _lib.ptr(Fg), _lib.ptr(gFn), _lib.ptr(gC),
@@ -293,13 +300,15 @@ def test_reference_form_adjoint_chain_matches_fused(gpu):
     Ur, sr, Vr = om.svd3(Fn_r)
     op = oracle_params(prm)
     op.svd_exact = False          # the per-kernel chain is the reference's svd_grad
-    mv_r, m_r, F1_r = om.p2g(op, st[0], st[1], st[2], Fn_r, Ur, sr, Vr,
-                             oracle_mats(mats, mat_id, F64))
+    omats, mu_t, lam_t = oracle_mats_leaf(mats, mat_id, F64)
+    mv_r, m_r, F1_r = om.p2g(op, st[0], st[1], st[2], Fn_r, Ur, sr, Vr, omats)
     loss = (mv_r * host64(gmv)).sum() + (m_r * host64(gm)).sum() + (F1_r * host64(gF1)).sum()
-    grads = torch.autograd.grad(loss, st)
+    grads = torch.autograd.grad(loss, st + [mu_t, lam_t])
     for name, got, want in zip("xvCF", (gx, gv, gC, gF), grads):
         err = rel_l2(host64(got), want)
         assert err < 1e-4, (name, err)
+    err = rel_l2(host64(gmat), torch.stack([grads[4], grads[5]], 1))
+    assert err < 1e-4, ("mu/lam", err, host64(gmat), grads[4], grads[5])
 
 
 def test_grid_op_and_g2p_adjoints_match_oracle(gpu):
@@ -348,21 +357,26 @@ def test_rollout_gradient_matches_oracle(gpu, kind, sort):
     prm, mats, mat_id, x, v, C, F = _facade_inputs(kind, n=1500, seed=23)
     C = C * 0.1
     steps = 12
-    sim = MPMSim(prm, mats, len(x), mat_id=mat_id, checkpoint_every=5, sort=sort)
+    sim = MPMSim(prm, mats, len(x), mat_id=mat_id, checkpoint_every=5, sort=sort,
+                 material_grad=True)
     st = [dev(a).requires_grad_(True) for a in (x, v, C, F)]
     out = sim.rollout(*st, steps)
     w = [_seed(t.shape, 40 + i) for i, t in enumerate(st)]
     loss = sum((o * wi.cuda()).sum() for o, wi in zip(out, w))
     grads = torch.autograd.grad(loss, st)
     st_r = [torch.tensor(a, dtype=F64, requires_grad=True) for a in (x, v, C, F)]
-    out_r = om.rollout(oracle_params(prm), tuple(st_r), oracle_mats(mats, mat_id, F64), steps)
+    omats, mu_t, lam_t = oracle_mats_leaf(mats, mat_id, F64)
+    out_r = om.rollout(oracle_params(prm), tuple(st_r), omats, steps)
     for got, want in zip(out, out_r):
         assert rel_l2(host64(got.detach()), want.detach()) < 1e-5
     loss_r = sum((o * wi.to(F64)).sum() for o, wi in zip(out_r, w))
-    g_r = torch.autograd.grad(loss_r, st_r)
+    g_r = torch.autograd.grad(loss_r, st_r + [mu_t, lam_t])
     for name, got, want in zip("xvCF", grads, g_r):
         err = rel_l2(host64(got), want)
         assert err < 1e-3, (name, err)
+    # material adjoints accumulated over the trajectory (d loss / d mu, lam)
+    err = rel_l2(host64(sim.material_grad), torch.stack([g_r[4], g_r[5]], 1))
+    assert err < 1e-3, ("mu/lam", err)
 
 
 def test_resident_and_recomputed_segments_agree(gpu):
@@ -510,25 +524,30 @@ def test_sparse_substep_matches_dense_and_oracle(gpu, kind, sort, cluster):
         g2 = [torch.empty_like(t) for t in g]
         gg4 = torch.zeros(nn, 4, device="cuda")
         ggv4 = torch.zeros(nn, 4, device="cuda")
+        gmat = torch.zeros(len(mats), 2, device="cuda")
         call("dt_mpm_run_adj", c_prm, mt, len(mats), n, mid, *frames, 1, G4, GV4, gg4, ggv4,
-             *g, *g2, ws, nbytes, _lib.stream())
-        res[mode] = ([f[1] for f in frames], G4[0], g)
+             *g, *g2, ws, nbytes, gmat, _lib.stream())
+        res[mode] = ([f[1] for f in frames], G4[0], g, gmat)
     # sparse vs dense: same arithmetic up to float summation order
     for a, b in zip(res["sparse"][0], res["dense"][0]):
         assert rel_l2(host64(a), host64(b)) < 1e-6
     assert rel_l2(host64(res["sparse"][1]), host64(res["dense"][1])) < 1e-6
     for a, b in zip(res["sparse"][2], res["dense"][2]):
         assert rel_l2(host64(a), host64(b)) < 1e-5
+    assert rel_l2(host64(res["sparse"][3]), host64(res["dense"][3])) < 1e-5
     # sparse vs the FP64 oracle
     st = [torch.tensor(a, dtype=F64, requires_grad=True) for a in (x, v, C, F)]
-    out = om.substep(oracle_params(prm), tuple(st), oracle_mats(mats, mat_id, F64))
+    omats, mu_t, lam_t = oracle_mats_leaf(mats, mat_id, F64)
+    out = om.substep(oracle_params(prm), tuple(st), omats)
     for got, want in zip(res["sparse"][0], out):
         assert rel_l2(host64(got), want.detach()) < 1e-5
     loss = sum((o_ * s.to(F64)).sum() for o_, s in zip(out, seeds))
-    grads = torch.autograd.grad(loss, st)
+    grads = torch.autograd.grad(loss, st + [mu_t, lam_t])
     for name, got, want in zip("xvCF", res["sparse"][2], grads):
         err = rel_l2(host64(got), want)
         assert err < 1e-4, (name, err)
+    err = rel_l2(host64(res["sparse"][3]), torch.stack([grads[4], grads[5]], 1))
+    assert err < 1e-4, ("mu/lam", err)
 
 
 def test_graph_replays_match_eager_launches(gpu):
@@ -560,7 +579,7 @@ def test_graph_replays_match_eager_launches(gpu):
         for t, s in zip(g, seeds):
             t.copy_(s)
         call("dt_mpm_run_adj", c_prm, mt, len(mats), n, mid, *frames, k, G4, GV4, gg4, ggv4,
-             *g, *g2, ws, nbytes, _lib.stream())
+             *g, *g2, ws, nbytes, None, _lib.stream())
         torch.cuda.synchronize()
         return [f[k].clone() for f in frames] + [t.clone() for t in g]
 
@@ -615,7 +634,7 @@ def test_adjoint_reuses_forward_binning_history(gpu):
         gg4 = torch.zeros(nn, 4, device="cuda")
         ggv4 = torch.zeros(nn, 4, device="cuda")
         call("dt_mpm_run_adj", c_prm, mt, len(mats), n, mid, *fr, k, G4, GV4, gg4, ggv4, *g, *g2,
-             ws, nbytes, _lib.stream())
+             ws, nbytes, None, _lib.stream())
         torch.cuda.synchronize()
         return g
 

@@ -306,6 +306,42 @@ def test_substep_reverse_mode_matches_central_differences(kind, exact):
     assert abs(ad - fd) <= 1e-6 * max(1.0, abs(fd)), (ad, fd)
 
 
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("kind", [om.KIND_ELASTIC, om.KIND_VON_MISES, om.KIND_DRUCKER_PRAGER])
+def test_material_reverse_mode_matches_central_differences(kind, exact):
+    """d loss / d (mu, lam) of a 2-substep rollout, reverse mode against FP64
+    central differences (covers the stress terms and the return maps' yield
+    threshold and cone slope)."""
+    prm = om.MPMParams(n_grid=16, rho=1000.0, dt=2e-5, svd_exact=exact)
+    n = 60
+    rng = np.random.default_rng(8)
+    x = torch.tensor(0.5 + 0.08 * rng.normal(size=(n, 3)), dtype=F64)
+    v = torch.tensor(0.3 * rng.normal(size=(n, 3)), dtype=F64)
+    C = torch.tensor(rng.normal(size=(n, 3, 3)), dtype=F64)
+    F = torch.tensor(np.eye(3) + 0.2 * rng.normal(size=(n, 3, 3)), dtype=F64)
+    F = F * torch.sign(torch.linalg.det(F))[:, None, None]
+    mat = om.Material(5e4, 0.3, kind, yield_stress=30.0)
+    _, _, kinds, yld, alpha = om.material_arrays([mat], np.zeros(n, int), F64)
+    w = [torch.tensor(rng.normal(size=t.shape), dtype=F64) for t in (x, v, C, F)]
+
+    def L(mu, lam):
+        mats = (mu.expand(n), lam.expand(n), kinds, yld, alpha)
+        out = om.rollout(prm, (x, v, C, F), mats, 2)
+        return sum((o * wi).sum() for o, wi in zip(out, w))
+
+    mu = torch.tensor(mat.mu, dtype=F64, requires_grad=True)
+    lam = torch.tensor(mat.lam, dtype=F64, requires_grad=True)
+    g_mu, g_lam = torch.autograd.grad(L(mu, lam), (mu, lam))
+    with torch.no_grad():
+        for g, which in ((g_mu, 0), (g_lam, 1)):
+            h = 1e-6 * (mat.mu if which == 0 else mat.lam)
+            dp = torch.tensor(h if which == 0 else 0.0, dtype=F64)
+            dl = torch.tensor(h if which == 1 else 0.0, dtype=F64)
+            fd = (L(mu + dp, lam + dl) - L(mu - dp, lam - dl)).item() / (2 * h)
+            assert abs(g.item() - fd) <= 1e-5 * max(abs(fd), 1e-12), (which, g.item(), fd)
+            assert abs(fd) > 0.0
+
+
 def test_polar_rotation_adjoint_exact_at_near_repeated_singular_values():
     """_Polar's adjoint (M_ij - M_ji)/(s_i + s_j) against FP64 central differences,
     including F = I + O(1e-9) where the reference's clamp changes svd_grad."""
