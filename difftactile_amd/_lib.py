@@ -23,7 +23,8 @@ class DtMpmParams(ctypes.Structure):
         ("dx", ctypes.c_float), ("inv_dx", ctypes.c_float), ("dt", ctypes.c_float),
         ("p_vol", ctypes.c_float), ("p_mass", ctypes.c_float), ("eps", ctypes.c_float),
         ("gravity", ctypes.c_float * 3), ("grad_clamp", ctypes.c_float),
-        ("soa", ctypes.c_int32), ("svd_exact", ctypes.c_int32),
+        ("soa", ctypes.c_int32), ("svd_exact", ctypes.c_int32), ("n_env", ctypes.c_int32),
+        ("env_id", ctypes.c_void_p),
     ]
 
 

@@ -1,0 +1,89 @@
+"""Batched populations: independent MPM environments advanced together.
+
+Environments that share a grid size and a time step are stacked along the
+grid's x axis (``dt_mpm_params.n_env``).  One binning, one brick-owned P2G,
+one grid pass and one G2P per substep then serve all of them, and the sparse
+path visits only the bricks their particles occupy.  So an environment costs
+its own particles and nodes, not the launch sequence of a separate
+simulation.  Positions stay local to each environment: a per-particle
+environment id shifts the stencils into the environment's slab of the stacked
+grid, the walls act per environment, and environments never exchange
+momentum.
+
+The reference runs one scene per process (``tasks/*.py`` build one
+``Contact`` each); a population is a loop over such scenes.
+"""
+import copy
+
+import numpy as np
+import torch
+
+from .sim import MPMSim
+
+
+class EnvBatch:
+    """E independent MPM environments of one grid size in one ``MPMSim``.
+
+    prm:  ``MPMParams`` of one environment (grid, time step, gravity, walls).
+    envs: one ``(materials, n_particles, mat_id or None)`` per environment.
+    Further keyword arguments go to ``MPMSim`` (``checkpoint_every``,
+    ``sort``, ``sparse_grid``, ``resident_bytes``, ``material_grad``).
+    """
+
+    def __init__(self, prm, envs, device="cuda", **sim_kwargs):
+        if len(envs) < 1:
+            raise ValueError("a batch needs at least one environment")
+        self.prm = copy.copy(prm)
+        self.prm.n_env = len(envs)
+        mats, mids, ids, counts, n_mats = [], [], [], [], []
+        for e, (materials, n, mat_id) in enumerate(envs):
+            n = int(n)
+            base = len(mats)
+            mats.extend(materials)
+            mid = np.zeros(n, np.int32) if mat_id is None else np.asarray(mat_id, np.int32)
+            if mid.shape != (n,):
+                raise ValueError("mat_id of environment %d must have %d entries" % (e, n))
+            if n and (mid.min() < 0 or mid.max() >= len(materials)):
+                raise ValueError("mat_id of environment %d out of its material table" % e)
+            mids.append(mid + base)
+            ids.append(np.full(n, e, np.int32))
+            counts.append(n)
+            n_mats.append(len(materials))
+        self.counts = counts
+        self.offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.mat_offsets = np.concatenate([[0], np.cumsum(n_mats)]).astype(np.int64)
+        self.sim = MPMSim(self.prm, mats, int(self.offsets[-1]), mat_id=np.concatenate(mids),
+                          device=device, env_id=np.concatenate(ids), **sim_kwargs)
+
+    @property
+    def n_env(self):
+        return len(self.counts)
+
+    def pack(self, states):
+        """Per-environment ``(x, v, C, F)`` tuples -> the stacked state."""
+        if len(states) != self.n_env:
+            raise ValueError("expected %d environment states" % self.n_env)
+        for e, st in enumerate(states):
+            if any(t.shape[0] != self.counts[e] for t in st):
+                raise ValueError("environment %d: state rows != its particle count" % e)
+        return tuple(torch.cat([st[i] for st in states]) for i in range(4))
+
+    def unpack(self, x, v, C, F):
+        """Stacked state -> per-environment ``(x, v, C, F)`` views."""
+        return [tuple(t[a:b] for t in (x, v, C, F))
+                for a, b in zip(self.offsets[:-1], self.offsets[1:])]
+
+    def rollout(self, states, n_substeps):
+        """Differentiable rollout of every environment by n_substeps."""
+        return self.unpack(*self.sim.rollout(*self.pack(states), n_substeps))
+
+    def step_no_grad(self, states, n_substeps):
+        return self.unpack(*self.sim.step_no_grad(*self.pack(states), n_substeps))
+
+    def material_grad(self):
+        """Per-environment ``(n_materials, 2)`` views of the accumulated d/d mu,
+        d/d lam (``MPMSim(material_grad=True)``)."""
+        g = self.sim.material_grad
+        if g is None:
+            return None
+        return [g[a:b] for a, b in zip(self.mat_offsets[:-1], self.mat_offsets[1:])]

@@ -15,9 +15,11 @@
 
 namespace dt {
 
+// bricks of 4x4x4 nodes over the (n_env n_grid) x n_grid x n_grid grid
 long long brick_total(const dt_mpm_params* prm) {
-  long long nb = (prm->n_grid + 3) >> 2;
-  return nb * nb * nb;
+  const long long nb = (prm->n_grid + 3) >> 2;
+  const long long nx = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid;
+  return ((nx + 3) >> 2) * nb * nb;
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float
   if (active) {
     float xp[3];
     ld_v3(x, p, pl_vec(P.soa, n), xp);
-    k = bin_key(xp, P.inv_dx, P.n_grid) >> 6;
+    k = bin_key(xp, P.inv_dx, P.n_grid, env_x0(P, p)) >> 6;
   }
   const int r = wave_run_rank(active, k, count);
   if (active) {

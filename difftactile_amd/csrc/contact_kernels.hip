@@ -332,6 +332,7 @@ int dt_contact_closest(const dt_mpm_params* prm, const float* grid_m, int32_t m_
                        int32_t idx_stride, int32_t idx_col, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
+  DT_REQUIRE(prm->n_env <= 1, "contact kernels take one environment (n_env <= 1)");
   DT_REQUIRE(n_tris >= 0 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
                  idx_col < idx_stride,
              "bad size/stride");
@@ -351,6 +352,7 @@ int dt_contact_collision(const dt_mpm_params* prm, const dt_contact_params* cp,
                          void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
+  DT_REQUIRE(prm->n_env <= 1, "contact kernels take one environment (n_env <= 1)");
   DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
   DT_REQUIRE(mv_stride >= 3 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
                  idx_col < idx_stride && n_tris >= 0,
@@ -375,6 +377,7 @@ int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* 
                              float* g_vel, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
+  DT_REQUIRE(prm->n_env <= 1, "contact kernels take one environment (n_env <= 1)");
   DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
   DT_REQUIRE(mv_stride >= 3 && m_stride > 0 && gmv_stride >= 3 && gm_stride > 0 &&
                  idx_stride > 0 && idx_col >= 0 && idx_col < idx_stride && n_tris >= 0,

@@ -35,6 +35,8 @@ __device__ __forceinline__ int xcd_block() {
 
 struct MpmP {
   int n_grid, bound, sticky_floor, clamp_grad, soa, svd_exact;
+  int n_env, nx;   // environments stacked along x; nx = n_env n_grid x-nodes
+  const int32_t* env_id;
   float dx, inv_dx, dt, p_vol, p_mass, eps;
   float g[3];
   float grad_clamp;
@@ -48,6 +50,9 @@ inline MpmP to_dev(const dt_mpm_params& p) {
   d.clamp_grad = p.clamp_grad;
   d.soa = p.soa;
   d.svd_exact = p.svd_exact;
+  d.n_env = p.n_env > 1 ? p.n_env : 1;
+  d.nx = d.n_env * p.n_grid;
+  d.env_id = d.n_env > 1 ? p.env_id : nullptr;
   d.dx = p.dx;
   d.inv_dx = p.inv_dx;
   d.dt = p.dt;
@@ -59,6 +64,11 @@ inline MpmP to_dev(const dt_mpm_params& p) {
   d.g[2] = p.gravity[2];
   d.grad_clamp = p.grad_clamp;
   return d;
+}
+
+// First x-node of particle p's environment (environments stacked along x).
+__device__ __forceinline__ int env_x0(const MpmP& P, long long p) {
+  return P.env_id ? P.env_id[p] * P.n_grid : 0;
 }
 
 int validate_params(const dt_mpm_params* p);

@@ -297,7 +297,9 @@ struct Stencil {
   float dw[3][3];  // d w / d fx
 };
 
-DT_HD void make_stencil(const float (&x)[3], float inv_dx, int n_grid, Stencil& s) {
+// x0: first x-node of the particle's environment (e n_grid when environments
+// are stacked along x, else 0); the position x is local to the environment.
+DT_HD void make_stencil(const float (&x)[3], float inv_dx, int n_grid, Stencil& s, int x0 = 0) {
 #pragma clang fp contract(off)
   for (int a = 0; a < 3; ++a) {
     float t = x[a] * inv_dx;
@@ -306,7 +308,7 @@ DT_HD void make_stencil(const float (&x)[3], float inv_dx, int n_grid, Stencil& 
     // bounds check; particles are kept away from the walls by the boundary
     // band, so the clamp only changes cases the reference leaves undefined.
     b = b < 0 ? 0 : (b > n_grid - 3 ? n_grid - 3 : b);
-    s.base[a] = b;
+    s.base[a] = a == 0 ? b + x0 : b;
     float f = t - (float)b;
     s.fx[a] = f;
     float d0 = 1.5f - f, d1 = f - 1.f, d2 = f - 0.5f;
@@ -321,12 +323,13 @@ DT_HD void make_stencil(const float (&x)[3], float inv_dx, int n_grid, Stencil& 
 
 // cell key used for particle sorting: 4x4x4-cell bricks in row-major order,
 // cells row-major inside a brick.  key = brick * 64 + local.
-DT_HD int cell_key(const float (&x)[3], float inv_dx, int n_grid) {
+DT_HD int cell_key(const float (&x)[3], float inv_dx, int n_grid, int x0 = 0) {
   int b[3];
   for (int a = 0; a < 3; ++a) {
     int v = cell_base1(x[a], inv_dx);
     b[a] = v < 0 ? 0 : (v > n_grid - 1 ? n_grid - 1 : v);
   }
+  b[0] += x0;
   int nb = (n_grid + 3) >> 2;
   int brick = ((b[0] >> 2) * nb + (b[1] >> 2)) * nb + (b[2] >> 2);
   int local = (((b[0] & 3) << 2) + (b[1] & 3)) * 4 + (b[2] & 3);
@@ -337,12 +340,13 @@ DT_HD int cell_key(const float (&x)[3], float inv_dx, int n_grid) {
 // like cell_key (brick * 64 + local).  Uses the base expression of
 // make_stencil so the key names exactly the cell whose stencil the particle
 // scatters through.
-DT_HD int bin_key(const float (&x)[3], float inv_dx, int n_grid) {
+DT_HD int bin_key(const float (&x)[3], float inv_dx, int n_grid, int x0 = 0) {
   int b[3];
   for (int a = 0; a < 3; ++a) {
     int v = cell_base1(x[a], inv_dx);
     b[a] = v < 0 ? 0 : (v > n_grid - 3 ? n_grid - 3 : v);
   }
+  b[0] += x0;
   int nb = (n_grid + 3) >> 2;
   int brick = ((b[0] >> 2) * nb + (b[1] >> 2)) * nb + (b[2] >> 2);
   int local = (((b[0] & 3) << 2) + (b[1] & 3)) * 4 + (b[2] & 3);

@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmP P, const dt_material* __res
     A = affine_of(P, tau, ld_m3(C + 9ll * p));
     st_m3(F_next + 9ll * p, st.Fp);
     float xp[3] = {x[3ll * p], x[3ll * p + 1], x[3ll * p + 2]};
-    make_stencil(xp, P.inv_dx, P.n_grid, s);
+    make_stencil(xp, P.inv_dx, P.n_grid, s, env_x0(P, p));
     for (int a = 0; a < 3; ++a) pv[a] = P.p_mass * v[3ll * p + a];
   }
   p2g_block(P, active, n, s, pv, A, g, sh);
@@ -492,7 +492,7 @@ __device__ __forceinline__ void p2g_fused_compute(const MpmP& P, const dt_materi
   stress_fwd(fn, u, sig, vv, m, st, tau);
   const M3 A = affine_of(P, tau, in.c);
   st_m3L(F_next, p, Lm, st.Fp);
-  make_stencil(in.x, P.inv_dx, P.n_grid, s);
+  make_stencil(in.x, P.inv_dx, P.n_grid, s, env_x0(P, p));
   for (int q = 0; q < 3; ++q) {
     b[q] = P.p_mass * in.v[q] -
            (A.a[3 * q] * s.fx[0] + A.a[3 * q + 1] * s.fx[1] + A.a[3 * q + 2] * s.fx[2]) * P.dx;
@@ -636,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
   M3 A = affine_of(P, tau, c);
   Stencil s;
   float xp[3] = {x[3ll * p], x[3ll * p + 1], x[3ll * p + 2]};
-  make_stencil(xp, P.inv_dx, P.n_grid, s);
+  make_stencil(xp, P.inv_dx, P.n_grid, s, env_x0(P, p));
   float pv[3];
   for (int a = 0; a < 3; ++a) pv[a] = P.p_mass * v[3ll * p + a];
   float lgx[3] = {0.f, 0.f, 0.f}, lgv[3] = {0.f, 0.f, 0.f};
@@ -796,7 +796,7 @@ __device__ __forceinline__ void p2g_fused_adj_particle(
     float xp[3], vp[3];
     ld_v3(x, p, Lv, xp);
     ld_v3(v, p, Lv, vp);
-    make_stencil(xp, P.inv_dx, P.n_grid, s);
+    make_stencil(xp, P.inv_dx, P.n_grid, s, env_x0(P, p));
     float pv[3];
     for (int q = 0; q < 3; ++q) pv[q] = P.p_mass * vp[q];
     float lgx[3];
@@ -884,7 +884,7 @@ __device__ __forceinline__ void grid_node_fwd(const MpmP& P, long long nd, float
   vo[0] = inv_m * mv0 + inv_m * f0 + P.dt * P.g[0];
   vo[1] = inv_m * mv1 + inv_m * f1 + P.dt * P.g[1];
   vo[2] = inv_m * mv2 + inv_m * f2 + P.dt * P.g[2];
-  int idx[3] = {i, j, k};
+  int idx[3] = {i % n, j, k};   // x index inside the node's environment
   for (int a = 0; a < 3; ++a) {
     mask[a] = (idx[a] < P.bound && vo[a] < 0.f) || (idx[a] > n - P.bound && vo[a] > 0.f);
     if (P.sticky_floor && j < P.bound) mask[a] = true;
@@ -1025,7 +1025,7 @@ __global__ __launch_bounds__(kBlock) void k_mark_bricks(MpmP P, int n, const flo
   float xp[3];
   ld_v3(x, p, pl_vec(P.soa, n), xp);
   Stencil s;
-  make_stencil(xp, P.inv_dx, P.n_grid, s);
+  make_stencil(xp, P.inv_dx, P.n_grid, s, env_x0(P, p));
   const int nb = (P.n_grid + 3) >> 2;
   int lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
@@ -1040,13 +1040,13 @@ __global__ __launch_bounds__(kBlock) void k_mark_bricks(MpmP P, int n, const flo
 // node of thread t in the brick-pass launch (4 bricks per 256-thread block);
 // returns -1 when the brick is inactive or the node lies outside the grid
 __device__ __forceinline__ long long brick_node(const MpmP& P, const int32_t* __restrict__ flags) {
-  const int n = P.n_grid, nb = (n + 3) >> 2;
+  const int n = P.n_grid, nb = (n + 3) >> 2, nbx = (P.nx + 3) >> 2;
   long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (b >= (long long)nb * nb * nb || flags[b] == 0) return -1;
+  if (b >= (long long)nbx * nb * nb || flags[b] == 0) return -1;
   int l = threadIdx.x & 63;
   int bz = (int)(b % nb), by = (int)((b / nb) % nb), bx = (int)(b / ((long long)nb * nb));
   int i = bx * 4 + (l >> 4), j = by * 4 + ((l >> 2) & 3), k = bz * 4 + (l & 3);
-  if (i >= n || j >= n || k >= n) return -1;
+  if (i >= P.nx || j >= n || k >= n) return -1;
   return ((long long)i * n + j) * n + k;
 }
 
@@ -1136,7 +1136,7 @@ __device__ __forceinline__ long long active_node(const MpmP& P, int b) {
   const int l = threadIdx.x & 63;
   const int bz = b % nb, by = (b / nb) % nb, bx = b / (nb * nb);
   const int i = bx * 4 + (l >> 4), j = by * 4 + ((l >> 2) & 3), k = bz * 4 + (l & 3);
-  if (i >= n || j >= n || k >= n) return -1;
+  if (i >= P.nx || j >= n || k >= n) return -1;
   return ((long long)i * n + j) * n + k;
 }
 
@@ -1241,7 +1241,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
   Stencil s;
   float xp[3];
   ld_v3(x, p, Lv, xp);
-  make_stencil(xp, P.inv_dx, P.n_grid, s);
+  make_stencil(xp, P.inv_dx, P.n_grid, s, env_x0(P, p));
   float nv[3];
   M3 nC;
   if (sgv == 4)
@@ -1296,7 +1296,7 @@ __device__ __forceinline__ void g2p_adj_compute(const MpmP& P, int n, const G2PA
                                                 float (&b)[3], M3& M) {
   PL Lv = pl_vec(P.soa, n);
   float gvt[3], lgx[3];
-  make_stencil(in.x, P.inv_dx, P.n_grid, s);
+  make_stencil(in.x, P.inv_dx, P.n_grid, s, env_x0(P, p));
   for (int a = 0; a < 3; ++a) {
     gvt[a] = in.v1[a] + P.dt * in.g1[a];
     lgx[a] = in.g1[a];
@@ -1623,7 +1623,7 @@ __device__ __forceinline__ float4 fetch_gv_tile(const MpmP& P, long long b, cons
   if (t < kTileNodes) {
     const int i = bx * 4 + t / (kTileEdge * kTileEdge), j = by * 4 + (t / kTileEdge) % kTileEdge,
               k = bz * 4 + t % kTileEdge;
-    if (i < n && j < n && k < n) {
+    if (i < P.nx && j < n && k < n) {
       const long long nd = node_index(i, j, k, n);
       q = make_float4(gv[nd * sgv], gv[nd * sgv + 1], gv[nd * sgv + 2], 0.f);
     }
@@ -1667,7 +1667,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(MpmP P, int n, const float
   if (p >= n) return;
   float xp[3];
   ld_v3(x, p, pl_vec(P.soa, n), xp);
-  keys[p] = cell_key(xp, P.inv_dx, P.n_grid);
+  keys[p] = cell_key(xp, P.inv_dx, P.n_grid, env_x0(P, p));
 }
 
 __global__ __launch_bounds__(kBlock) void k_gather_rows(long long total, int width,
@@ -1722,12 +1722,17 @@ __global__ __launch_bounds__(kBlock) void k_scatter_rows(long long total, int wi
 // host-side launchers (used by the C-ABI layer)
 // ----------------------------------------------------------------------------
 static long long grid_nodes(const dt_mpm_params* prm) {
-  return (long long)prm->n_grid * prm->n_grid * prm->n_grid;
+  const long long e = prm->n_env > 1 ? prm->n_env : 1;
+  return e * prm->n_grid * prm->n_grid * prm->n_grid;
 }
 
 int validate_params(const dt_mpm_params* p) {
   DT_REQUIRE(p != nullptr, "dt_mpm_params is NULL");
   DT_REQUIRE(p->n_grid >= 3 && p->n_grid <= 1024, "n_grid must be in [3, 1024]");
+  DT_REQUIRE(p->n_env >= 0, "n_env must be >= 0");
+  DT_REQUIRE((long long)(p->n_env > 1 ? p->n_env : 1) * p->n_grid * p->n_grid * p->n_grid <
+                 (1ll << 31),
+             "n_env * n_grid^3 must stay below 2^31 nodes");
   DT_REQUIRE(p->dx > 0.f && p->inv_dx > 0.f, "dx and inv_dx must be positive");
   return DT_OK;
 }
@@ -1739,10 +1744,7 @@ static int check_mats(const dt_material* mats, int n_mats) {
 
 #define DT_STREAM(s) ((hipStream_t)(s))
 
-static long long n_bricks(const dt_mpm_params* prm) {
-  long long nb = (prm->n_grid + 3) >> 2;
-  return nb * nb * nb;
-}
+static long long n_bricks(const dt_mpm_params* prm) { return brick_total(prm); }
 
 // Marks the bricks touched by the particles at x (flags cleared first).
 static void launch_mark(const MpmP& P, const dt_mpm_params* prm, int n, const float* x,
@@ -2142,7 +2144,7 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     return DT_OK;
   };
   std::vector<uint64_t> key{1};
-  key_add(key, *prm);
+  key_add(key, to_dev(*prm));   // no padding bytes
   for (const void* q : {(const void*)mats, (const void*)mat_id, (const void*)x, (const void*)v,
                         (const void*)C, (const void*)F, (const void*)grids4,
                         (const void*)grids_v4, (const void*)scratch4, (const void*)scratch_v4,
@@ -2195,7 +2197,7 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     return DT_OK;
   };
   std::vector<uint64_t> key{hist ? 4u : 2u};
-  key_add(key, *prm);
+  key_add(key, to_dev(*prm));   // no padding bytes
   for (const void* ptr : {(const void*)mats, (const void*)mat_id, (const void*)x, (const void*)v,
                           (const void*)C, (const void*)F, (const void*)grids4,
                           (const void*)grids_v4, (const void*)g_grid4, (const void*)g_grid_v4,
@@ -2246,7 +2248,7 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     return DT_OK;
   };
   std::vector<uint64_t> key{3};
-  key_add(key, *prm);
+  key_add(key, to_dev(*prm));   // no padding bytes
   for (const void* ptr : {(const void*)mats, (const void*)mat_id, (const void*)x, (const void*)v,
                           (const void*)C, (const void*)F, (const void*)x2, (const void*)v2,
                           (const void*)C2, (const void*)F2, (const void*)scratch4,

@@ -26,7 +26,7 @@ class MPMParams:
 
     def __init__(self, n_grid=64, space_scale=1.0, obj_scale=1.0, rho=1.0, dt=5e-5,
                  gravity=(0.0, -9.8, 0.0), bound=3, eps=1e-5, sticky_floor=False,
-                 clamp_grad=False, grad_clamp=1000.0, svd_exact=True):
+                 clamp_grad=False, grad_clamp=1000.0, svd_exact=True, n_env=1):
         self.n_grid = int(n_grid)
         self.space_scale = float(space_scale)
         self.obj_scale = float(obj_scale)
@@ -44,10 +44,13 @@ class MPMParams:
         self.grad_clamp = float(grad_clamp)
         # exact SVD adjoint in the fused path (see dt_mpm_params.svd_exact)
         self.svd_exact = bool(svd_exact)
+        # environments stacked along x (see dt_mpm_params.n_env): environment e
+        # occupies x in [e, e + 1) * space_scale
+        self.n_env = max(1, int(n_env))
 
     @property
     def n_nodes(self):
-        return self.n_grid ** 3
+        return self.n_env * self.n_grid ** 3
 
     def c_struct(self, soa=False):
         p = _lib.DtMpmParams()
@@ -66,11 +69,13 @@ class MPMParams:
         p.grad_clamp = self.grad_clamp
         p.soa = int(soa)
         p.svd_exact = int(self.svd_exact)
+        p.n_env = self.n_env
         return p
 
     def key_bits(self):
         nb = (self.n_grid + 3) // 4
-        return max(1, int(math.ceil(math.log2(nb ** 3 * 64))))
+        nbx = (self.n_env * self.n_grid + 3) // 4
+        return max(1, int(math.ceil(math.log2(nbx * nb * nb * 64))))
 
 
 class Material:

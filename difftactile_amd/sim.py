@@ -41,7 +41,7 @@ class MPMSim:
 
     def __init__(self, prm, materials, n_particles, mat_id=None, device="cuda",
                  checkpoint_every=50, sort=True, sparse_grid=True, resident_bytes=None,
-                 material_grad=False):
+                 material_grad=False, env_id=None):
         self.prm = prm
         if resident_bytes is None:
             # a quarter of the device's memory for resident segments
@@ -68,6 +68,19 @@ class MPMSim:
             mn = int(self.mat_id.min().item()) if self.n else 0
             if mn < 0 or mx >= len(self.materials):
                 raise ValueError("mat_id out of range of the material table")
+        # environments stacked along x (prm.n_env > 1): env_id[p] names particle
+        # p's environment; positions stay local to it
+        if prm.n_env > 1:
+            if env_id is None:
+                raise ValueError("prm.n_env > 1 needs env_id (one id per particle)")
+            self.env_id = torch.as_tensor(env_id, dtype=torch.int32,
+                                          device=self.device).contiguous()
+            if self.env_id.numel() != self.n:
+                raise ValueError("env_id must have one entry per particle")
+            if self.n and (int(self.env_id.min()) < 0 or int(self.env_id.max()) >= prm.n_env):
+                raise ValueError("env_id out of range [0, n_env)")
+        else:
+            self.env_id = None
         self.checkpoint_every = int(checkpoint_every)
         self.sort = bool(sort)
         self._ws = {}
@@ -76,6 +89,13 @@ class MPMSim:
         # mu_0.grad / lambda_0.grad accumulate
         self.material_grad = (torch.zeros((len(self.materials), 2), device=self.device)
                               if material_grad else None)
+
+    def _set_env(self, env):
+        """Points the parameter structs at the environment ids of the particle
+        order the next launches see."""
+        p = None if env is None else _lib.ptr(env, torch.int32)
+        self.c_prm.env_id = p
+        self.c_prm_aos.env_id = p
 
     def zero_material_grad(self):
         if self.material_grad is not None:
@@ -116,6 +136,7 @@ class MPMSim:
         keys = self._buf("keys", (n,), torch.int32)
         keys_out = torch.empty(n, dtype=torch.int32, device=self.device)
         perm = torch.empty(n, dtype=torch.int32, device=self.device)
+        self._set_env(self.env_id)
         _lib.call("dt_mpm_cell_keys", self.c_prm_aos, n, _lib.ptr(x), _lib.ptr(keys, torch.int32),
                   _lib.stream())
         tb = int(_lib.lib().dt_sort_tmp_bytes(n))
@@ -238,6 +259,10 @@ class _Rollout(torch.autograd.Function):
             mat_id = sim._gather_i(perm, sim.mat_id)
         else:
             mat_id = sim.mat_id
+        env = sim.env_id
+        if env is not None and perm is not None:
+            env = sim._gather_i(perm, env)
+        sim._set_env(env)
         ckpts = []
         used = 0
         for (s, k) in _segments(n_substeps, sim.checkpoint_every):
@@ -265,6 +290,7 @@ class _Rollout(torch.autograd.Function):
             ctx.n_substeps = n_substeps
             ctx.perm = perm
             ctx.mat_id = mat_id
+            ctx.env = env
             ctx.ckpts = ckpts
             ctx.shapes = shapes
         return tuple(out)
@@ -277,6 +303,7 @@ class _Rollout(torch.autograd.Function):
              for t, sh in zip((gx, gv, gC, gF), ctx.shapes)]
         g = [sim._gather(perm, t.float(), w) for t, w in zip(g, _Rollout.WIDTHS)]
         segs = _segments(ctx.n_substeps, sim.checkpoint_every)
+        sim._set_env(ctx.env)
         for (s, k), ck in zip(reversed(segs), reversed(ctx.ckpts)):
             if isinstance(ck, tuple):           # resident segment
                 (X, V, Cb, Fb), (G4, GV4) = ck

@@ -58,6 +58,17 @@ typedef struct dt_mpm_params {
                            s_j^2 - s_i^2, mpm_elastic.py:149-155); 1 = exact adjoint
                            (rotation path (M_ij - M_ji)/(s_i + s_j), no clamp). The
                            per-kernel dt_mpm_svd_adj is always the reference form.     */
+  int32_t n_env;        /* independent environments stacked along x (0 or 1: one).
+                           The grid is (n_env n_grid) x n_grid x n_grid nodes;
+                           environment e owns x-nodes [e n_grid, (e+1) n_grid).
+                           Particle positions stay local to their environment;
+                           env_id[p] names particle p's environment, whose stencil
+                           (clamped as for one environment) is shifted by e n_grid
+                           nodes along x; walls apply per environment, so
+                           environments never exchange momentum.  Contact entry
+                           points require one environment.                          */
+  const int32_t* env_id; /* device array [n] of environment ids (NULL: all 0); it
+                           follows the particle order of the arrays passed with it */
 } dt_mpm_params;
 
 /* One row of the material table (multi_obj.py:65-74 generalised). */
