@@ -43,9 +43,12 @@ def parse():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device-to-device copy (the reference "
                          "kernel of known traffic for rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "pad_press"],
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "pad_press", "population"],
                     help="sphere: 1M-particle multi-material sphere (default, the metric's "
-                         "workload); pad_press: slab pad pressing a von Mises block")
+                         "workload); pad_press: slab pad pressing a von Mises block; "
+                         "population: 64 environments, batched vs a per-environment loop")
+    ap.add_argument("--pop-substeps", type=int, default=300,
+                    help="population workload: substeps per fwd+bwd step")
     return ap.parse_args()
 
 
@@ -331,6 +334,95 @@ def bench_pad_press(args, torch):
     print(json.dumps(res), flush=True)
 
 
+def bench_population(args, torch):
+    """Batched population: 64 environments (5k-200k particles, 64^3 or 128^3
+    grids) advanced fwd+bwd for --pop-substeps substeps, once as one stacked
+    batch per grid size (EnvBatch) and once as a loop over per-environment
+    simulations.  Both recompute 10-substep segments from checkpoints (no
+    resident segments: the stacked 128^3 grids of 300 substeps exceed HBM).
+    Loss per environment: its final centre of mass along a seeded direction;
+    the gradient is taken with respect to its initial velocities."""
+    from difftactile_amd import scenes
+    from difftactile_amd.batch import EnvBatch
+    from difftactile_amd.sim import MPMSim
+    envs = scenes.config4_population()
+    T, seg = args.pop_substeps, 10
+    dev = torch.device("cuda")
+    n_tot = sum(len(d["x"]) for d in envs)
+    rng = np.random.default_rng(5)
+    w = [torch.tensor(rng.normal(size=3), dtype=torch.float32, device=dev) for _ in envs]
+    xs = [torch.tensor(d["x"], device=dev) for d in envs]
+    vs = [torch.tensor(d["v"], device=dev) for d in envs]
+
+    def state(e):
+        n = len(envs[e]["x"])
+        return [xs[e], vs[e].clone().requires_grad_(True), torch.zeros(n, 3, 3, device=dev),
+                torch.eye(3, device=dev).repeat(n, 1, 1)]
+
+    groups = {}
+    for e, d in enumerate(envs):
+        groups.setdefault(d["prm"].n_grid, []).append(e)
+    batches = [(EnvBatch(envs[ids[0]]["prm"],
+                         [(envs[e]["mats"], len(envs[e]["x"]), envs[e]["mat_id"]) for e in ids],
+                         checkpoint_every=seg, sort=True, resident_bytes=0), ids)
+               for ids in groups.values()]
+    sims = [MPMSim(d["prm"], d["mats"], len(d["x"]), mat_id=d["mat_id"], checkpoint_every=seg,
+                   sort=True, resident_bytes=0) for d in envs]
+
+    def run_batched():
+        g = [None] * len(envs)
+        for batch, ids in batches:
+            sts = [state(e) for e in ids]
+            out = batch.rollout(sts, T)
+            loss = sum((o[0].mean(0) * w[e]).sum() for o, e in zip(out, ids))
+            for e, ge in zip(ids, torch.autograd.grad(loss, [st[1] for st in sts])):
+                g[e] = ge
+        return g
+
+    def run_loop():
+        g = []
+        for e, sim in enumerate(sims):
+            st = state(e)
+            out = sim.rollout(*st, T)
+            g.append(torch.autograd.grad((out[0].mean(0) * w[e]).sum(), st[1])[0])
+        return g
+
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, out
+
+    log(f"[bench] population: {len(envs)} environments, {n_tot} particles, "
+        f"{', '.join(f'{len(ids)} on {n}^3' for n, ids in groups.items())}, {T} substeps")
+    run_batched()
+    run_loop()
+    reps = max(1, min(args.steps, 2))
+    t_b, g_b = timed(run_batched, reps)
+    log(f"[bench] batched {t_b * 1e3:.1f} ms per fwd+bwd step")
+    t_l, g_l = timed(run_loop, reps)
+    log(f"[bench] per-environment loop {t_l * 1e3:.1f} ms per fwd+bwd step")
+    num = sum(float(((a - b) ** 2).sum()) for a, b in zip(g_b, g_l))
+    den = sum(float((b ** 2).sum()) for b in g_l)
+    res = {"metric": "fwd+bwd env-substeps/s (64-environment population)",
+           "value": len(envs) * T / t_b, "unit": "env-substeps/s", "n_gpus": 1, "steps": reps,
+           "warmup": 1, "ms_per_step": t_b * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "particle_substeps_per_s": n_tot * T / t_b,
+           "loop": {"ms_per_step": t_l * 1e3, "value": len(envs) * T / t_l},
+           "speedup_vs_loop": t_l / t_b,
+           "grad_rel_diff_batched_vs_loop": (num / max(den, 1e-30)) ** 0.5,
+           "config": {"workload": f"{len(envs)} environments, {n_tot} particles "
+                                  "(log-uniform 5e3-2e5 per environment), 64^3 below 3e4 "
+                                  f"particles else 128^3, {T} substeps fwd+bwd, 10-substep "
+                                  "segments recomputed; batched = one stacked launch sequence "
+                                  "per grid size, loop = one simulation per environment",
+                      "groups": {f"{n}^3": len(ids) for n, ids in groups.items()}}}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -340,6 +432,9 @@ def main():
     import torch
     if args.workload == "pad_press":
         bench_pad_press(args, torch)
+        return
+    if args.workload == "population":
+        bench_population(args, torch)
         return
     from difftactile_amd import scenes
     from difftactile_amd.sim import MPMSim

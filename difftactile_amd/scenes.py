@@ -7,7 +7,7 @@ needed.  All outputs are numpy float32 / int32 in the reference's layouts.
 """
 import numpy as np
 
-from .params import DRUCKER_PRAGER, ELASTIC, Material, MPMParams
+from .params import DRUCKER_PRAGER, ELASTIC, VON_MISES, Material, MPMParams
 
 
 def lattice_cube(n_per_axis, spacing, centre, jitter=0.0, seed=0):
@@ -78,6 +78,30 @@ def config3_multi_material(n_particles=1_000_000, n_grid=128, dt=1e-5, seed=0):
     v0 = np.tile(np.array([0.0, -0.5, 0.0], np.float32), (len(x), 1))
     v0 += rng.uniform(-0.05, 0.05, size=v0.shape).astype(np.float32)
     return prm, mats, x, v0.astype(np.float32), mat_id
+
+
+def config4_population(n_env=64, seed=0, dt=1e-5):
+    """Batched population: n_env independent objects with log-uniform particle
+    counts in [5e3, 2e5] (seeded), each a ball at 2 particles per dx per axis
+    on its own grid: 64^3 when it holds fewer than 3e4 particles, else 128^3.
+    Materials alternate between fixed corotated (E=5e4) and von Mises (E=5e4,
+    yield 50); rho = 1000; each ball starts with a seeded velocity of about
+    0.5 downwards.  Returns one dict per environment (prm, mats, x, v, mat_id)."""
+    rng = np.random.default_rng(seed)
+    counts = np.round(np.exp(rng.uniform(np.log(5e3), np.log(2e5), n_env))).astype(int)
+    envs = []
+    for e, n in enumerate(counts):
+        n_grid = 64 if n < 30000 else 128
+        prm = MPMParams(n_grid=n_grid, rho=1000.0, dt=dt, gravity=(0.0, -9.8, 0.0))
+        dx = prm.dx
+        x, _ = sphere_lattice(int(n), dx / 2, (0.5, 0.5, 0.5), jitter=0.25 * dx, seed=seed + e)
+        mats = ([Material(5e4, 0.3, ELASTIC)] if e % 2 == 0
+                else [Material(5e4, 0.3, VON_MISES, yield_stress=50.0)])
+        v = np.tile(np.array([0.0, -0.5, 0.0], np.float32), (len(x), 1))
+        v += rng.uniform(-0.1, 0.1, size=(1, 3)).astype(np.float32)
+        envs.append(dict(prm=prm, mats=mats, x=x, v=v.astype(np.float32),
+                         mat_id=np.zeros(len(x), np.int32)))
+    return envs
 
 
 def identity_F(n):
