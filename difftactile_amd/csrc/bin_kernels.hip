@@ -4,9 +4,9 @@
 // k_bin_count gives every particle its rank inside its brick with one global
 // atomic per run of equal keys in a wavefront (cell-sorted particles make long
 // runs), an exclusive scan over the brick counts gives the brick starts, and
-// k_bin_fill writes the permutation.  k_brick_flags derives the active-brick
-// flags (occupied bricks and their +1 neighbours, which a stencil can reach)
-// and the occupied-brick list from the counts.  The order inside a brick
+// k_bin_fill writes the permutation.  The active-brick list (occupied bricks
+// and their +1 neighbours, which a stencil can reach) and the occupied-brick
+// list come from the counts.  The order inside a brick
 // follows the atomics; the brick passes sort each chunk by cell themselves.
 #include <hipcub/hipcub.hpp>
 
@@ -29,8 +29,7 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   size_t scan_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
                                    (int32_t*)nullptr, (int)(nbk + 1));
-  const size_t sz[11] = {sizeof(int32_t) * (size_t)nbk,
-                        sizeof(int32_t) * (size_t)(nbk + 2),
+  const size_t sz[10] = {sizeof(int32_t) * (size_t)(nbk + 2),
                         sizeof(int32_t) * (size_t)(nbk + 1),
                         sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t) * (size_t)n,
@@ -40,27 +39,27 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
                         scan_bytes,
                         sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t)};
-  size_t off[11], total = 0;
-  for (int i = 0; i < 11; ++i) {
+  size_t off[10], total = 0;
+  for (int i = 0; i < 10; ++i) {
     off[i] = total;
     total += align256(sz[i] ? sz[i] : 1);
   }
   if (base && ws) {
     char* b = static_cast<char*>(base);
-    ws->flags = reinterpret_cast<int32_t*>(b + off[0]);
-    ws->count = reinterpret_cast<int32_t*>(b + off[1]);
+    ws->count = reinterpret_cast<int32_t*>(b + off[0]);
     ws->n_list = ws->count + nbk + 1;
-    ws->start = reinterpret_cast<int32_t*>(b + off[2]);
-    ws->list = reinterpret_cast<int32_t*>(b + off[3]);
-    ws->key = reinterpret_cast<int32_t*>(b + off[4]);
-    ws->rank = reinterpret_cast<int32_t*>(b + off[5]);
-    ws->perm = reinterpret_cast<int32_t*>(b + off[6]);
-    ws->tiles = reinterpret_cast<float4*>(b + off[7]);
-    ws->scan_tmp = b + off[8];
-    ws->alist = reinterpret_cast<int32_t*>(b + off[9]);
-    ws->n_alist = reinterpret_cast<int32_t*>(b + off[10]);
+    ws->start = reinterpret_cast<int32_t*>(b + off[1]);
+    ws->list = reinterpret_cast<int32_t*>(b + off[2]);
+    ws->key = reinterpret_cast<int32_t*>(b + off[3]);
+    ws->rank = reinterpret_cast<int32_t*>(b + off[4]);
+    ws->perm = reinterpret_cast<int32_t*>(b + off[5]);
+    ws->tiles = reinterpret_cast<float4*>(b + off[6]);
+    ws->scan_tmp = b + off[7];
+    ws->alist = reinterpret_cast<int32_t*>(b + off[8]);
+    ws->n_alist = reinterpret_cast<int32_t*>(b + off[9]);
     ws->scan_bytes = scan_bytes;
     ws->n_bricks = nbk;
+    ws->nb = (prm->n_grid + 3) >> 2;
   }
   return total;
 }
@@ -79,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_zero(long long n, int32_t* __res
 size_t bin_history_slot_bytes(const dt_mpm_params* prm, int n) {
   const long long nbk = brick_total(prm);
   return align256(sizeof(int32_t) * (size_t)n) + align256(sizeof(int32_t) * (size_t)(nbk + 1)) +
-         3 * align256(sizeof(int32_t) * (size_t)nbk) + 512;
+         2 * align256(sizeof(int32_t) * (size_t)nbk) + 512;
 }
 
 BinWs bin_slot(const BinWs& ws, int s, int n) {
@@ -91,8 +90,6 @@ BinWs bin_slot(const BinWs& ws, int s, int n) {
   v.start = reinterpret_cast<int32_t*>(b);
   b += align256(sizeof(int32_t) * (size_t)(nbk + 1));
   v.list = reinterpret_cast<int32_t*>(b);
-  b += align256(sizeof(int32_t) * (size_t)nbk);
-  v.flags = reinterpret_cast<int32_t*>(b);
   b += align256(sizeof(int32_t) * (size_t)nbk);
   v.alist = reinterpret_cast<int32_t*>(b);
   b += align256(sizeof(int32_t) * (size_t)nbk);
@@ -142,12 +139,11 @@ __device__ __forceinline__ void wave_append(bool pred, int32_t value, int32_t* _
   if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
 }
 
-// Per brick: flags = the brick or one of its -1 neighbours holds particles (a
+// Per brick: active = the brick or one of its -1 neighbours holds particles (a
 // stencil reaches at most one brick past its base's); list += occupied
-// bricks, alist += flagged bricks.
+// bricks, alist += active bricks.
 __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int nb,
                                                         const int32_t* __restrict__ count,
-                                                        int32_t* __restrict__ flags,
                                                         int32_t* __restrict__ list,
                                                         int32_t* __restrict__ n_list,
                                                         int32_t* __restrict__ alist,
@@ -163,30 +159,168 @@ __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int 
       if (i >= 0 && j >= 0 && k >= 0) any |= count[((long long)i * nb + j) * nb + k];
     }
     act = any != 0;
-    flags[b] = act ? 1 : 0;
     occ = count[b] > 0;
   }
   wave_append(occ, (int32_t)b, list, n_list);
   wave_append(act, (int32_t)b, alist, n_alist);
 }
 
-int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st) {
+// One workgroup for grids of at most 32 bricks per axis: the brick counts
+// (plus the zero sentinel) go to LDS, each thread scans a run of 33 of them
+// after a block scan of the run totals, and the occupied / active lists come
+// from one 32-bit occupancy word per brick row (i, j) (bit k), so a brick's
+// active test (it or a -1 neighbour occupied) is a shift and an OR of four
+// words.  The counts are zeroed behind the read, so the next binning on this
+// workspace needs no separate clear.
+constexpr int kScanThreads = 1024;
+constexpr int kScanRun = 33;
+constexpr int kScanEdge = 32;   // bricks per axis
+
+__global__ __launch_bounds__(kScanThreads) void k_bin_scan_lists(
+    int nb, int32_t* __restrict__ count, int32_t* __restrict__ start, int32_t* __restrict__ list,
+    int32_t* __restrict__ n_list, int32_t* __restrict__ alist, int32_t* __restrict__ n_alist) {
+  // element e at e + e / 33: the 33 elements of a run are consecutive and
+  // consecutive runs start two banks apart
+  __shared__ int32_t c[kScanThreads * (kScanRun + 1)];
+  __shared__ uint32_t word[kScanThreads];
+  __shared__ int32_t wsum[kScanThreads / 64], wocc[kScanThreads / 64], wact[kScanThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int nbk = nb * nb * nb, ne = nbk + 1;
+  {
+    int v[kScanRun];
+#pragma unroll
+    for (int r = 0; r < kScanRun; ++r) {
+      const int e = t + r * kScanThreads;
+      v[r] = e < ne ? count[e] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < kScanRun; ++r) {
+      const int e = t + r * kScanThreads;
+      if (e < ne) count[e] = 0;
+      c[e + e / kScanRun] = v[r];
+    }
+  }
+  __syncthreads();
+  // run totals, block exclusive scan over the 1024 runs
+  int* run = c + t * (kScanRun + 1);
+  int tot = 0;
+#pragma unroll
+  for (int r = 0; r < kScanRun; ++r) tot += run[r];
+  int incl = tot;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(incl, d);
+    if (lane >= d) incl += u;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int base = incl - tot;
+  for (int q = 0; q < w; ++q) base += wsum[q];
+#pragma unroll
+  for (int r = 0; r < kScanRun; ++r) {
+    const int v = run[r];
+    run[r] = base;
+    base += v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kScanRun; ++r) {
+    const int e = t + r * kScanThreads;
+    if (e < ne) start[e] = c[e + e / kScanRun];
+  }
+  // occupancy word of brick row t = (i, j): bit k set when brick (i, j, k)
+  // holds particles
+  const int nrow = nb * nb;
+  uint32_t occ = 0u;
+  if (t < nrow) {
+    const int b0 = t * nb;
+    int prev = c[b0 + b0 / kScanRun];
+    for (int k = 0; k < nb; ++k) {
+      const int e = b0 + k + 1;
+      const int nxt = c[e + e / kScanRun];
+      if (nxt > prev) occ |= 1u << k;
+      prev = nxt;
+    }
+    word[t] = occ;
+  }
+  __syncthreads();
+  uint32_t act = 0u;
+  if (t < nrow) {
+    const int i = t / nb, j = t - i * nb;
+    uint32_t x = occ;
+    if (j > 0) x |= word[t - 1];
+    if (i > 0) x |= word[t - nb];
+    if (i > 0 && j > 0) x |= word[t - nb - 1];
+    act = x | (x << 1);
+    if (nb < 32) act &= (1u << nb) - 1u;
+  }
+  // both lists ascending: per-row counts, a block scan, then each row writes
+  const int no = __popc(occ), na = __popc(act);
+  int io = no, ia = na;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int uo = __shfl_up(io, d), ua = __shfl_up(ia, d);
+    if (lane >= d) {
+      io += uo;
+      ia += ua;
+    }
+  }
+  if (lane == 63) {
+    wocc[w] = io;
+    wact[w] = ia;
+  }
+  __syncthreads();
+  int bo = io - no, ba = ia - na;
+  for (int q = 0; q < w; ++q) {
+    bo += wocc[q];
+    ba += wact[q];
+  }
+  const int rb = t * nb;
+  for (uint32_t m = occ; m; m &= m - 1u) list[bo++] = rb + __ffs((int)m) - 1;
+  for (uint32_t m = act; m; m &= m - 1u) alist[ba++] = rb + __ffs((int)m) - 1;
+  if (t == kScanThreads - 1) {
+    *n_list = bo;
+    *n_alist = ba;
+  }
+}
+
+// one workgroup when the grid is a single environment of at most 32 bricks
+// per axis; that scan leaves the counts zero
+static bool one_block_scan(const BinWs& ws, int nb) {
+  return nb <= kScanEdge && ws.n_bricks == (long long)nb * nb * nb;
+}
+
+bool bin_counts_fusable(const BinWs& ws) { return one_block_scan(ws, ws.nb); }
+
+int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st,
+               bool count_clear, bool counted) {
   const long long nbk = ws.n_bricks;
   const int nb = (P.n_grid + 3) >> 2;
-  hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nbk + 2)), dim3(kBlock), 0, st, nbk + 2, ws.count,
-                     ws.n_list, ws.n_alist);
-  if (n > 0)
-    hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
-                       ws.rank, ws.count);
-  size_t bytes = ws.scan_bytes;
-  if (hipcub::DeviceScan::ExclusiveSum(ws.scan_tmp, bytes, ws.count, ws.start, (int)(nbk + 1),
-                                       st) != hipSuccess) {
-    set_error("binning scan failed");
-    return DT_ELAUNCH;
+  const bool one_block = one_block_scan(ws, nb);
+  if (counted && !one_block) {
+    set_error("binning: fused counts need the one-workgroup scan");
+    return DT_EINVAL;
   }
-  hipLaunchKernelGGL(k_brick_flags, dim3(nblocks(nbk)), dim3(kBlock), 0, st, nbk, nb,
-                     (const int32_t*)ws.count, ws.flags, ws.list, ws.n_list, ws.alist,
-                     ws.n_alist);
+  if (!counted) {
+    if (!one_block || !count_clear)
+      hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nbk + 2)), dim3(kBlock), 0, st, nbk + 2,
+                         ws.count, ws.n_list, ws.n_alist);
+    if (n > 0)
+      hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
+                         ws.rank, ws.count);
+  }
+  if (one_block) {
+    hipLaunchKernelGGL(k_bin_scan_lists, dim3(1), dim3(kScanThreads), 0, st, nb, ws.count,
+                       ws.start, ws.list, ws.n_list, ws.alist, ws.n_alist);
+  } else {
+    size_t bytes = ws.scan_bytes;
+    if (hipcub::DeviceScan::ExclusiveSum(ws.scan_tmp, bytes, ws.count, ws.start, (int)(nbk + 1),
+                                         st) != hipSuccess) {
+      set_error("binning scan failed");
+      return DT_ELAUNCH;
+    }
+    hipLaunchKernelGGL(k_brick_flags, dim3(nblocks(nbk)), dim3(kBlock), 0, st, nbk, nb,
+                       (const int32_t*)ws.count, ws.list, ws.n_list, ws.alist,
+                       ws.n_alist);
+  }
   if (n > 0)
     hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
                        ws.start, ws.perm);

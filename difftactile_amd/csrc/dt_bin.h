@@ -14,14 +14,13 @@ constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
 
 // Views into one caller-provided workspace buffer.
 struct BinWs {
-  int32_t* flags;   // [n_bricks]     bricks some stencil may touch: the
-                    //   occupied bricks and their +1 neighbours
   int32_t* count;   // [n_bricks + 2] particles per brick ([n_bricks] stays 0);
                     //   [n_bricks + 1] counts the occupied bricks
   int32_t* n_list;  // = count + n_bricks + 1
   int32_t* start;   // [n_bricks + 1] exclusive prefix sum of count
   int32_t* list;    // [n_bricks]     occupied bricks (any order)
-  int32_t* alist;   // [n_bricks]     active (flagged) bricks (any order)
+  int32_t* alist;   // [n_bricks]     active bricks: occupied bricks and their +1
+                    //   neighbours, which a stencil can reach
   int32_t* n_alist; // count of alist
   int32_t* key;     // [n]            brick of each particle
   int32_t* rank;    // [n]            position of the particle inside its brick
@@ -30,8 +29,9 @@ struct BinWs {
   void* scan_tmp;
   size_t scan_bytes;
   long long n_bricks;
+  int nb;            // bricks along y and z
   // Optional binning history (one slot per substep of a trajectory): the
-  // perm / start / list / n_list / flags / alist / n_alist of substep s live in slot s so the
+  // perm / start / list / n_list / alist / n_alist of substep s live in slot s so the
   // adjoint sweep can reuse the forward pass's binning.
   char* hist = nullptr;
   size_t hist_slot_bytes = 0;
@@ -70,7 +70,15 @@ __device__ __forceinline__ int wave_run_rank(bool active, int key, int* counts) 
 
 // Persistent grid of the brick-owned passes (blocks stride over the list).
 constexpr int kBrickBlocks = 2048;
-// count -> brick scan -> brick flags and occupied-brick list -> fill, on st.
-int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st);
+// count -> brick scan -> occupied / active brick lists -> fill, on st.
+// count_clear: the workspace's brick counts are known to be zero (a previous
+// binning on it used the one-workgroup scan, which clears them behind it).
+// counted: key / rank / count already hold x's counts (made by the previous
+// substep's G2P); needs bin_counts_fusable.
+int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st,
+               bool count_clear = false, bool counted = false);
+// Whether this workspace's binning clears the counts behind its scan, so a
+// G2P may make the next binning's counts.
+bool bin_counts_fusable(const BinWs& ws);
 
 }  // namespace dt

@@ -1231,28 +1231,46 @@ __device__ __forceinline__ void g2p_core(const MpmP& P, const Stencil& s, const 
     for (int q = 0; q < 3; ++q) nC.a[r * 3 + q] = k4 * (GO[q][r] - nv[r] * s.fx[q]);
 }
 
+// kCount: also makes the brick counts of the next binning from x1 (the key,
+// rank and count arrays k_bin_count would write; active lanes form a prefix
+// of every wavefront, as wave_run_rank needs)
+template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __restrict__ x,
                                                 const float* __restrict__ gv, int sgv,
                                                 float* __restrict__ x1, float* __restrict__ v1,
-                                                float* __restrict__ C1) {
-  int p = xcd_block() * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
-  Stencil s;
-  float xp[3];
-  ld_v3(x, p, Lv, xp);
-  make_stencil(xp, P.inv_dx, P.n_grid, s, env_x0(P, p));
-  float nv[3];
-  M3 nC;
-  if (sgv == 4)
-    g2p_core(P, s, Nodes4{(const float4*)gv, P.n_grid}.at(s.base), nv, nC);
-  else
-    g2p_core(P, s, NodesStrided<false>{gv, nullptr, sgv, 0, P.n_grid}.at(s.base), nv, nC);
-  float nx[3];
-  for (int a = 0; a < 3; ++a) nx[a] = xp[a] + P.dt * nv[a];
-  st_v3(v1, p, Lv, nv);
-  st_v3(x1, p, Lv, nx);
-  st_m3L(C1, p, Lm, nC);
+                                                float* __restrict__ C1, int32_t* __restrict__ bkey,
+                                                int32_t* __restrict__ brank,
+                                                int32_t* __restrict__ bcount) {
+  const int p = xcd_block() * blockDim.x + threadIdx.x;
+  const bool active = p < n;
+  if (!kCount && !active) return;
+  int k = -1;
+  if (active) {
+    PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
+    Stencil s;
+    float xp[3];
+    ld_v3(x, p, Lv, xp);
+    make_stencil(xp, P.inv_dx, P.n_grid, s, env_x0(P, p));
+    float nv[3];
+    M3 nC;
+    if (sgv == 4)
+      g2p_core(P, s, Nodes4{(const float4*)gv, P.n_grid}.at(s.base), nv, nC);
+    else
+      g2p_core(P, s, NodesStrided<false>{gv, nullptr, sgv, 0, P.n_grid}.at(s.base), nv, nC);
+    float nx[3];
+    for (int a = 0; a < 3; ++a) nx[a] = xp[a] + P.dt * nv[a];
+    st_v3(v1, p, Lv, nv);
+    st_v3(x1, p, Lv, nx);
+    st_m3L(C1, p, Lm, nC);
+    if (kCount) k = bin_key(nx, P.inv_dx, P.n_grid, env_x0(P, p)) >> 6;
+  }
+  if (kCount) {
+    const int r = wave_run_rank(active, k, bcount);
+    if (active) {
+      bkey[p] = k;
+      brank[p] = r;
+    }
+  }
 }
 
 // g2p adjoint of particle p: gathers the grid-velocity adjoint into gx
@@ -1783,9 +1801,13 @@ static unsigned brick_blocks(const BinWs& ws) {
 }
 
 // bin the particles of x (also sets the active-brick flags)
+// (count_clear: an earlier binning in the same launch sequence left the
+// workspace's brick counts zero; counted: the previous substep's G2P already
+// made the counts of x)
 static int launch_mark_bin(const MpmP& P, const dt_mpm_params*, int n, const float* x,
-                           const BinWs& ws, hipStream_t st) {
-  return launch_bin(P, n, x, ws, st);
+                           const BinWs& ws, hipStream_t st, bool count_clear = false,
+                           bool counted = false) {
+  return launch_bin(P, n, x, ws, st, count_clear, counted);
 }
 
 // One forward substep.  ws == NULL: dense grid passes and the run scatter;
@@ -1793,13 +1815,15 @@ static int launch_mark_bin(const MpmP& P, const dt_mpm_params*, int n, const flo
 int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
                        const int32_t* mat_id, const float* x, const float* v, const float* C,
                        const float* F, float* grid4, float* grid_v4, float* x1, float* v1,
-                       float* C1, float* F1, hipStream_t st, const BinWs* ws = nullptr) {
+                       float* C1, float* F1, hipStream_t st, const BinWs* ws = nullptr,
+                       bool count_clear = false, bool counted = false,
+                       bool count_next = false) {
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
   if (ws) {
     {
       StageTimer tm(STAGE_BIN, st);
-      int rc = launch_mark_bin(P, prm, n, x, *ws, st);
+      int rc = launch_mark_bin(P, prm, n, x, *ws, st, count_clear, counted);
       if (rc) return rc;
     }
     {
@@ -1825,8 +1849,13 @@ int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
   }
   if (n > 0) {
     StageTimer tm(STAGE_G2P, st);
-    hipLaunchKernelGGL(k_g2p, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x,
-                       (const float*)grid_v4, 4, x1, v1, C1);
+    if (count_next)
+      hipLaunchKernelGGL(k_g2p<true>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x,
+                         (const float*)grid_v4, 4, x1, v1, C1, ws->key, ws->rank, ws->count);
+    else
+      hipLaunchKernelGGL(k_g2p<false>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x,
+                         (const float*)grid_v4, 4, x1, v1, C1, (int32_t*)nullptr,
+                         (int32_t*)nullptr, (int32_t*)nullptr);
   }
   return check_launch("dt_mpm_substep_fwd");
 }
@@ -1837,13 +1866,13 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n_
                        const float* gx1, const float* gv1, const float* gC1, const float* gF1,
                        float* ggrid4, float* ggv4, float* gx, float* gv, float* gC, float* gF,
                        float* g_mats, hipStream_t st, const BinWs* ws = nullptr,
-                       bool binned = false) {
+                       bool binned = false, bool count_clear = false) {
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
   if (ws) {
     if (!binned) {
       StageTimer tm(STAGE_BIN, st);
-      int rc = launch_mark_bin(P, prm, n, x, *ws, st);
+      int rc = launch_mark_bin(P, prm, n, x, *ws, st, count_clear);
       if (rc) return rc;
     }
     {
@@ -2061,8 +2090,9 @@ int dt_mpm_g2p(const dt_mpm_params* prm, int32_t n, const float* x, const float*
   DT_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return DT_OK;
   DT_REQUIRE(x && grid_v_out && x_next && v_next && C_next, "NULL pointer");
-  hipLaunchKernelGGL(k_g2p, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream), to_dev(*prm), n,
-                     x, grid_v_out, 3, x_next, v_next, C_next);
+  hipLaunchKernelGGL(k_g2p<false>, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream),
+                     to_dev(*prm), n, x, grid_v_out, 3, x_next, v_next, C_next, (int32_t*)nullptr,
+                     (int32_t*)nullptr, (int32_t*)nullptr);
   return check_launch("dt_mpm_g2p");
 }
 
@@ -2129,6 +2159,8 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
       (rc = workspace_views(prm, n, workspace, workspace_bytes, ws, keep ? n_substeps : 0)))
     return rc;
   const bool hist = workspace && keep && ws.hist_slots >= n_substeps && n_substeps > 0;
+  // G2P makes the next substep's brick counts when the binning clears them
+  const bool fuse = workspace && bin_counts_fusable(ws);
   long long nn = grid_nodes(prm);
   auto body = [&](hipStream_t st) -> int {
     for (int s = 0; s < n_substeps; ++s) {
@@ -2138,7 +2170,8 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
       BinWs slot = hist ? bin_slot(ws, s, n) : ws;
       const BinWs* wp = workspace ? &slot : nullptr;
       int r = launch_substep_fwd(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
-                                 g4, gv4, x + 3 * o1, v + 3 * o1, C + 9 * o1, F + 9 * o1, st, wp);
+                                 g4, gv4, x + 3 * o1, v + 3 * o1, C + 9 * o1, F + 9 * o1, st, wp,
+                                 s > 0, fuse && s > 0, fuse && s + 1 < n_substeps);
       if (r) return r;
     }
     return DT_OK;
@@ -2190,7 +2223,8 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
       int r = launch_substep_adj(prm, mats, n_mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o,
                                  F + 9 * o, grids4 + (long long)s * nn * 4,
                                  grids_v4 + (long long)s * nn * 4, p[0], p[1], p[2], p[3], g_grid4,
-                                 g_grid_v4, q[0], q[1], q[2], q[3], g_mats, cs, wp, hist);
+                                 g_grid_v4, q[0], q[1], q[2], q[3], g_mats, cs, wp, hist,
+                                 s != n_substeps - 1);
       if (r) return r;
       for (int k = 0; k < 4; ++k) { float* t = p[k]; p[k] = q[k]; q[k] = t; }
     }
@@ -2234,6 +2268,7 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   BinWs ws{};
   if (workspace && (rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
   const BinWs* wp = workspace ? &ws : nullptr;
+  const bool fuse = workspace && bin_counts_fusable(ws);
   DT_REQUIRE(n == 0 || (x && v && C && F && x2 && v2 && C2 && F2), "NULL pointer");
   hipStream_t st = DT_STREAM(stream);
   float *a[4] = {x, v, C, F}, *b[4] = {x2, v2, C2, F2};
@@ -2241,7 +2276,8 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     float *p[4] = {x, v, C, F}, *q[4] = {x2, v2, C2, F2};
     for (int s = 0; s < n_substeps; ++s) {
       int r = launch_substep_fwd(prm, mats, n, mat_id, p[0], p[1], p[2], p[3], scratch4,
-                                 scratch_v4, q[0], q[1], q[2], q[3], cs, wp);
+                                 scratch_v4, q[0], q[1], q[2], q[3], cs, wp, s > 0,
+                                 fuse && s > 0, fuse && s + 1 < n_substeps);
       if (r) return r;
       for (int k = 0; k < 4; ++k) { float* t = p[k]; p[k] = q[k]; q[k] = t; }
     }
@@ -2313,8 +2349,9 @@ int dt_mpm_g2p4(const dt_mpm_params* prm, int32_t n, const float* x, const float
   DT_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return DT_OK;
   DT_REQUIRE(x && grid_v4 && x_next && v_next && C_next, "NULL pointer");
-  hipLaunchKernelGGL(k_g2p, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream), to_dev(*prm), n,
-                     x, grid_v4, 4, x_next, v_next, C_next);
+  hipLaunchKernelGGL(k_g2p<false>, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream),
+                     to_dev(*prm), n, x, grid_v4, 4, x_next, v_next, C_next, (int32_t*)nullptr,
+                     (int32_t*)nullptr, (int32_t*)nullptr);
   return check_launch("dt_mpm_g2p4");
 }
 

@@ -279,9 +279,10 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
    atomics into a dense grid, mpm_elastic.py:175-208; these stages give the
    same grids) */
 
-/* Bytes of device workspace the sparse path needs for n particles: brick
-   flags, per-brick counts and starts, occupied-brick list, per-particle key /
-   rank / permutation, one 6x6x6-node float4 tile per brick, scan scratch. */
+/* Bytes of device workspace the sparse path needs for n particles:
+   per-brick counts and starts, occupied- and active-brick lists,
+   per-particle key / rank / permutation, one 6x6x6-node float4 tile per
+   brick, scan scratch. */
 int dt_mpm_workspace_bytes(const dt_mpm_params* prm, int32_t n, size_t* bytes);
 /* Extra bytes, after dt_mpm_workspace_bytes, for the binning history of an
    n_substeps trajectory.  A workspace with that room makes a grid-keeping

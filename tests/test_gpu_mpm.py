@@ -447,7 +447,10 @@ def _bin_arrays(c_prm, n, ws, nbytes, n_bricks):
     return perm, start
 
 
-@pytest.mark.parametrize("n_grid,n,sort", [(32, 20000, False), (64, 50000, True), (30, 7000, False)])
+# 132^3 has 35,937 bricks: past the one-workgroup scan, so the device-wide
+# scan and the flags kernel run
+@pytest.mark.parametrize("n_grid,n,sort", [(32, 20000, False), (64, 50000, True), (30, 7000, False),
+                                          (132, 30000, True)])
 def test_binning_is_a_counting_sort_by_stencil_brick(gpu, n_grid, n, sort):
     prm = default_prm(n_grid)
     rng = np.random.default_rng(n_grid + n)
@@ -461,6 +464,8 @@ def test_binning_is_a_counting_sort_by_stencil_brick(gpu, n_grid, n, sort):
     c_prm = prm.c_struct()
     ws, nbytes = _workspace(c_prm, n)
     xg = dev(x)
+    call("dt_mpm_bin", c_prm, n, xg, ws, nbytes, _lib.stream())
+    # binning again on the same workspace gives the same result
     call("dt_mpm_bin", c_prm, n, xg, ws, nbytes, _lib.stream())
     n_bricks = ((n_grid + 3) // 4) ** 3
     perm, start = _bin_arrays(c_prm, n, ws, nbytes, n_bricks)
