@@ -137,6 +137,13 @@ __device__ __forceinline__ void stencil_gather(const Stencil& s, const Ref& g, c
     for (int q = 0; q < 3; ++q) ci[q] = b[q] + fi * M.a[3 * q];
     float Pi = 0.f, Qi = 0.f, Ri = 0.f;
     float Gi[3] = {0.f, 0.f, 0.f}, GJ[3] = {0.f, 0.f, 0.f}, GK[3] = {0.f, 0.f, 0.f};
+    // the slice's nine nodes are loaded before any is used, so the loads are
+    // in flight together
+    float4 vs[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) vs[j][k] = g(i, j, k);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const float wj = s.w[j][1], dwj = s.dw[j][1];
@@ -147,7 +154,7 @@ __device__ __forceinline__ void stencil_gather(const Stencil& s, const Ref& g, c
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const float wk = s.w[k][2], dwk = s.dw[k][2];
-        const float4 v = g(i, j, k);
+        const float4 v = vs[j][k];
         if (kGw) {
           float c[3];
 #pragma unroll
