@@ -75,7 +75,10 @@ __device__ __forceinline__ float dw_axis0(const Stencil& s, int o) {
 
 // Stencil node sources.  at(base) binds a particle's stencil base; the bound
 // accessor returns node base + (i, j, k) as (x, y, z, w).
+// kBatch: global-memory sources, whose nine loads of a slice are issued
+// together; LDS sources are read where used.
 struct NodeRef4 {              // float4 nodes, row-major n^3
+  static constexpr bool kBatch = true;
   const float4* p;
   int n;
   __device__ __forceinline__ float4 operator()(int i, int j, int k) const {
@@ -91,6 +94,7 @@ struct Nodes4 {
 };
 template <bool kW>
 struct NodeRefStrided {        // 3 floats at stride s (+ a 4th array at stride sq)
+  static constexpr bool kBatch = true;
   const float* p;
   const float* q;
   int s, sq, n;
@@ -137,13 +141,15 @@ __device__ __forceinline__ void stencil_gather(const Stencil& s, const Ref& g, c
     for (int q = 0; q < 3; ++q) ci[q] = b[q] + fi * M.a[3 * q];
     float Pi = 0.f, Qi = 0.f, Ri = 0.f;
     float Gi[3] = {0.f, 0.f, 0.f}, GJ[3] = {0.f, 0.f, 0.f}, GK[3] = {0.f, 0.f, 0.f};
-    // the slice's nine nodes are loaded before any is used, so the loads are
-    // in flight together
+    // global sources: the slice's nine nodes are loaded before any is used,
+    // so the loads are in flight together
     float4 vs[3][3];
+    if constexpr (Ref::kBatch) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+      for (int j = 0; j < 3; ++j)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) vs[j][k] = g(i, j, k);
+        for (int k = 0; k < 3; ++k) vs[j][k] = g(i, j, k);
+    }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const float wj = s.w[j][1], dwj = s.dw[j][1];
@@ -154,7 +160,9 @@ __device__ __forceinline__ void stencil_gather(const Stencil& s, const Ref& g, c
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const float wk = s.w[k][2], dwk = s.dw[k][2];
-        const float4 v = vs[j][k];
+        float4 v;
+        if constexpr (Ref::kBatch) v = vs[j][k];
+        else v = g(i, j, k);
         if (kGw) {
           float c[3];
 #pragma unroll
@@ -1287,6 +1295,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
 // Grid-velocity sources of the g2p adjoint: the global grid (Nodes4 or
 // NodesStrided), or the LDS copy of a brick's 6x6x6-node tile.
 struct NodeRefTile {
+  static constexpr bool kBatch = false;
   const float4* p;
   __device__ __forceinline__ float4 operator()(int i, int j, int k) const {
     return p[(i * kTileEdge + j) * kTileEdge + k];
