@@ -115,6 +115,8 @@ _SIGS = {
     "dt_contact_centroids": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
     "dt_contact_closest": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _c_int,
                                     _c_int, _vp]),
+    "dt_contact_closest_env": (_c_int, [_P(DtMpmParams), _vp, _c_int, _vp, _c_int, _vp, _vp,
+                                        _c_int, _c_int, _vp]),
     "dt_contact_collision": (_c_int, [_P(DtMpmParams), _P(DtContactParams), _vp, _c_int, _vp,
                                       _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp,
                                       _P(ctypes.c_float), _vp, _vp, _vp]),

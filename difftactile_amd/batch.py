@@ -87,3 +87,59 @@ class EnvBatch:
         if g is None:
             return None
         return [g[a:b] for a, b in zip(self.mat_offsets[:-1], self.mat_offsets[1:])]
+
+
+def concat_pad_meshes(meshes):
+    """One mesh of the per-environment pads: ``meshes`` holds one
+    ``(nodes, f2v, tris, layer)`` per environment (positions local to it).
+    Returns ``(nodes, f2v, tris, layer, vert_start, tri_start)`` with the
+    vertex indices offset into the concatenation."""
+    nodes, f2v, tris, layer = [], [], [], []
+    vs, ts = [0], [0]
+    for nd, fv, tr, ly in meshes:
+        off = vs[-1]
+        nodes.append(np.asarray(nd, np.float32))
+        f2v.append(np.asarray(fv, np.int32) + off)
+        tris.append(np.asarray(tr, np.int32) + off)
+        layer.append(np.asarray(ly, np.int32))
+        vs.append(off + len(nd))
+        ts.append(ts[-1] + len(tr))
+    return (np.concatenate(nodes), np.concatenate(f2v), np.concatenate(tris),
+            np.concatenate(layer), np.asarray(vs, np.int64), np.asarray(ts, np.int32))
+
+
+class CoupledEnvBatch:
+    """E environments, each an MPM object and one FEM pad in penalty contact,
+    advanced together: the objects as an ``EnvBatch``, the pads concatenated
+    into one ``Pad`` whose triangles each environment's grid nodes search
+    within their own pad (``dt_contact_closest_env``).  The pads share one
+    material and one outward direction.
+
+    envs:       one ``(materials, n_particles, mat_id or None)`` per environment
+    pad_meshes: one ``(nodes_rest, f2v, tris, layer)`` per environment
+    pad:        dict(out_dir, mu, lam, mass) of the pads
+    contact:    dict(kn, kd, kt, mu_f) of the penalty contact
+    """
+
+    def __init__(self, prm, envs, pad_meshes, pad, contact, device="cuda", **sim_kwargs):
+        from .coupled import CoupledSim, Pad
+        if len(pad_meshes) != len(envs):
+            raise ValueError("one pad mesh per environment")
+        self.batch = EnvBatch(prm, envs, device=device, **sim_kwargs)
+        nodes, f2v, tris, layer, vs, ts = concat_pad_meshes(pad_meshes)
+        self.vert_offsets = vs
+        self.pad = Pad(nodes, f2v, tris, layer, pad["out_dir"], pad["mu"], pad["lam"],
+                       pad["mass"], prm.dt, device=device, tri_start=ts)
+        self.cs = CoupledSim(self.batch.sim, [self.pad], **contact)
+
+    def rollout(self, states, pad_states, control_vels, n_substeps):
+        """states: per environment (x, v, C, F); pad_states: per environment
+        (pos, vel, vpos); control_vels: per environment (n_substeps + 1, V_e, 3).
+        Returns per environment ((x, v, C, F), (pos, vel, vpos))."""
+        pst = tuple(torch.cat([p[i] for p in pad_states]) for i in range(3))
+        cv = torch.cat(list(control_vels), dim=1)
+        out = self.cs.rollout(*self.batch.pack(states), [pst], [cv], n_substeps)
+        objs = self.batch.unpack(*out[:4])
+        pads = out[4][0]
+        vo = self.vert_offsets
+        return [(o, tuple(t[vo[e]:vo[e + 1]] for t in pads)) for e, o in enumerate(objs)]

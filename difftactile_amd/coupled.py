@@ -40,10 +40,19 @@ def vertex_tet_csr(f2v, n_verts):
 
 
 class Pad:
-    """Static data of one FEM pad on the GPU (mesh, rest shapes, material)."""
+    """Static data of one FEM pad on the GPU (mesh, rest shapes, material).
 
-    def __init__(self, nodes_rest, f2v, tris, layer, out_dir, mu, lam, mass, dt, device="cuda"):
+    tri_start: for environments stacked along x (one pad per environment, the
+    pads' vertices, tets and triangles concatenated), the first triangle of
+    each environment's pad, [n_env + 1]; environment e's grid nodes contact
+    triangles [tri_start[e], tri_start[e+1]) only."""
+
+    def __init__(self, nodes_rest, f2v, tris, layer, out_dir, mu, lam, mass, dt, device="cuda",
+                 tri_start=None):
         self.device = torch.device(device)
+        self.tri_start = (None if tri_start is None else
+                          torch.as_tensor(np.asarray(tri_start), dtype=I32,
+                                          device=self.device).contiguous())
         self.n_verts = len(nodes_rest)
         self.n_tets = len(f2v)
         self.n_tris = len(tris)
@@ -121,8 +130,13 @@ class CoupledSim:
             _lib.call("dt_contact_centroids", pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
                       P(pad.cen), s)
             # grid mass = 4th float of each packed node
-            _lib.call("dt_contact_closest", prm, g4.data_ptr() + 12, 4, pad.n_tris, P(pad.cen),
-                      P(idx, I32), self.npad, k, s)
+            if pad.tri_start is None:
+                _lib.call("dt_contact_closest", prm, g4.data_ptr() + 12, 4, pad.n_tris,
+                          P(pad.cen), P(idx, I32), self.npad, k, s)
+            else:
+                _lib.call("dt_contact_closest_env", prm, g4.data_ptr() + 12, 4,
+                          P(pad.tri_start, I32), pad.n_tris, P(pad.cen), P(idx, I32), self.npad,
+                          k, s)
         gf.zero_()
         exts = []
         for k, pad in enumerate(self.pads):
@@ -221,6 +235,10 @@ class _CoupledRollout(torch.autograd.Function):
             mat_id = sim._gather_i(perm, sim.mat_id)
         else:
             mat_id = sim.mat_id
+        env = sim.env_id
+        if env is not None and perm is not None:
+            env = sim._gather_i(perm, env)
+        sim._set_env(env)
         nxt = [torch.empty_like(t) for t in st]
         pst = pads0
         pnx = [[torch.empty_like(t) for t in p] for p in pst]
@@ -238,7 +256,7 @@ class _CoupledRollout(torch.autograd.Function):
         for p in pst:
             out += [t.clone() for t in p]
         ctx.cs, ctx.n_substeps, ctx.ckpts, ctx.shapes = cs, n_substeps, ckpts, shapes
-        ctx.perm, ctx.mat_id = perm, mat_id
+        ctx.perm, ctx.mat_id, ctx.env = perm, mat_id, env
         ctx.cv_shapes = [c.shape for c in cvs]
         ctx.save_for_backward(*cvs)
         return tuple(out)
@@ -263,6 +281,7 @@ class _CoupledRollout(torch.autograd.Function):
         gpbuf = [[torch.empty_like(t) for t in p] for p in gp]
         nn = sim.prm.n_nodes
         segs = _segments(ctx.n_substeps, sim.checkpoint_every)
+        sim._set_env(ctx.env)
         for (s0, k), (ck, ckp) in zip(reversed(segs), reversed(ctx.ckpts)):
             # recompute the segment, keeping what the adjoint needs
             frames = [[t.clone() for t in ck]]

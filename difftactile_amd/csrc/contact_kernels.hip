@@ -52,32 +52,60 @@ __device__ __forceinline__ float centroid_dist(const float4 c, float qx, float q
 
 // check_collision for one pad: contact_idx[node*stride + col] = closest triangle
 // for occupied nodes (grid_m[node*m_stride] > eps); other nodes are untouched.
+// tri_start (environments stacked along x): the node of environment e = i /
+// n_grid searches only triangles [tri_start[e], tri_start[e+1]) (its own pad)
+// and positions are local to the environment; NULL: all nt triangles.
 __global__ __launch_bounds__(kBlock) void k_closest(MpmP P, long long nn,
                                                     const float* __restrict__ gm, int m_stride,
-                                                    int nt, const float4* __restrict__ cen,
+                                                    int nt, const int32_t* __restrict__ tri_start,
+                                                    const float4* __restrict__ cen,
                                                     int32_t* __restrict__ idx, int stride,
                                                     int col) {
   __shared__ float4 tile[kTriTile];
+  __shared__ int range[2];
   long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   bool occ = nd < nn && gm[nd * m_stride] > P.eps;
   if (!__syncthreads_or(occ)) return;
   const int n = P.n_grid;
   float qx = 0.f, qy = 0.f, qz = 0.f;
+  int lo = 0, hi = nt;
   if (nd < nn) {
     int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), k = (int)(nd % n);
-    qx = node_centre(i, P.dx);
+    if (tri_start) {
+      const int e = i / n;
+      lo = tri_start[e];
+      hi = tri_start[e + 1];
+    }
+    qx = node_centre(i % n, P.dx);
     qy = node_centre(j, P.dx);
     qz = node_centre(k, P.dx);
   }
+  // the block walks the union of its nodes' triangle ranges
+  int t_lo = 0, t_hi = nt;
+  if (tri_start) {
+    if (threadIdx.x == 0) {
+      range[0] = 0x7fffffff;
+      range[1] = 0;
+    }
+    __syncthreads();
+    if (occ) {
+      atomicMin(&range[0], lo);
+      atomicMax(&range[1], hi);
+    }
+    __syncthreads();
+    t_lo = range[0];
+    t_hi = range[1];
+  }
   float best = 100.0f;
   int best_k = -1;
-  for (int t0 = 0; t0 < nt; t0 += kTriTile) {
-    int cnt = min(kTriTile, nt - t0);
+  for (int t0 = t_lo; t0 < t_hi; t0 += kTriTile) {
+    int cnt = min(kTriTile, t_hi - t0);
     __syncthreads();
     if ((int)threadIdx.x < cnt) tile[threadIdx.x] = cen[t0 + threadIdx.x];
     __syncthreads();
     if (occ) {
-      for (int q = 0; q < cnt; ++q) {
+      const int q0 = max(lo - t0, 0), q1 = min(hi - t0, cnt);
+      for (int q = q0; q < q1; ++q) {
         float d = centroid_dist(tile[q], qx, qy, qz);
         if (d < best) {
           best = d;
@@ -177,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_collision(MpmP P, ContactP C, long l
   if (k < 0) return;
   const int n = P.n_grid;
   int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
-  float q[3] = {node_centre(i, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
+  float q[3] = {node_centre(i % n, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
   float inv = 1.f / (mm + P.eps);
   float qv[3] = {mv[nd * smv] * inv, mv[nd * smv + 1] * inv, mv[nd * smv + 2] * inv};
   int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
@@ -212,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_collision_adj(
   if (k < 0) return;
   const int n = P.n_grid;
   int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
-  float q[3] = {node_centre(i, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
+  float q[3] = {node_centre(i % n, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
   float inv = 1.f / (mm + P.eps);
   float mvv[3] = {mv[nd * smv], mv[nd * smv + 1], mv[nd * smv + 2]};
   float qv[3] = {mvv[0] * inv, mvv[1] * inv, mvv[2] * inv};
@@ -332,16 +360,34 @@ int dt_contact_closest(const dt_mpm_params* prm, const float* grid_m, int32_t m_
                        int32_t idx_stride, int32_t idx_col, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
-  DT_REQUIRE(prm->n_env <= 1, "contact kernels take one environment (n_env <= 1)");
   DT_REQUIRE(n_tris >= 0 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
                  idx_col < idx_stride,
              "bad size/stride");
   DT_REQUIRE(grid_m && contact_idx && (n_tris == 0 || centroids4), "NULL pointer");
+  DT_REQUIRE(prm->n_env <= 1, "stacked environments: use dt_contact_closest_env");
   long long nn = (long long)prm->n_grid * prm->n_grid * prm->n_grid;
   hipLaunchKernelGGL(k_closest, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
-                     to_dev(*prm), nn, grid_m, m_stride, n_tris, (const float4*)centroids4,
-                     contact_idx, idx_stride, idx_col);
+                     to_dev(*prm), nn, grid_m, m_stride, n_tris, (const int32_t*)nullptr,
+                     (const float4*)centroids4, contact_idx, idx_stride, idx_col);
   return check_launch("dt_contact_closest");
+}
+
+int dt_contact_closest_env(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
+                           const int32_t* tri_start, int32_t n_tris, const float* centroids4,
+                           int32_t* contact_idx, int32_t idx_stride, int32_t idx_col,
+                           void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n_tris >= 0 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
+                 idx_col < idx_stride,
+             "bad size/stride");
+  DT_REQUIRE(grid_m && contact_idx && tri_start && (n_tris == 0 || centroids4), "NULL pointer");
+  long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
+                 prm->n_grid;
+  hipLaunchKernelGGL(k_closest, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
+                     to_dev(*prm), nn, grid_m, m_stride, n_tris, tri_start,
+                     (const float4*)centroids4, contact_idx, idx_stride, idx_col);
+  return check_launch("dt_contact_closest_env");
 }
 
 int dt_contact_collision(const dt_mpm_params* prm, const dt_contact_params* cp,
@@ -352,14 +398,14 @@ int dt_contact_collision(const dt_mpm_params* prm, const dt_contact_params* cp,
                          void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
-  DT_REQUIRE(prm->n_env <= 1, "contact kernels take one environment (n_env <= 1)");
   DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
   DT_REQUIRE(mv_stride >= 3 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
                  idx_col < idx_stride && n_tris >= 0,
              "bad size/stride");
   DT_REQUIRE(grid_mv && grid_m && contact_idx && tris && pos && vel && grid_f && ext_force,
              "NULL pointer");
-  long long nn = (long long)prm->n_grid * prm->n_grid * prm->n_grid;
+  long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
+                 prm->n_grid;
   hipLaunchKernelGGL(k_collision, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
                      to_dev(*prm), contact_dev(cp), nn, grid_mv, mv_stride, grid_m, m_stride,
                      contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0], out_dir[1],
@@ -377,7 +423,6 @@ int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* 
                              float* g_vel, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
-  DT_REQUIRE(prm->n_env <= 1, "contact kernels take one environment (n_env <= 1)");
   DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
   DT_REQUIRE(mv_stride >= 3 && m_stride > 0 && gmv_stride >= 3 && gm_stride > 0 &&
                  idx_stride > 0 && idx_col >= 0 && idx_col < idx_stride && n_tris >= 0,
@@ -385,7 +430,8 @@ int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* 
   DT_REQUIRE(grid_mv && grid_m && contact_idx && tris && pos && vel && g_grid_f && g_ext_force &&
                  g_grid_mv && g_grid_m && g_pos && g_vel,
              "NULL pointer");
-  long long nn = (long long)prm->n_grid * prm->n_grid * prm->n_grid;
+  long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
+                 prm->n_grid;
   hipLaunchKernelGGL(k_collision_adj, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
                      to_dev(*prm), contact_dev(cp), nn, grid_mv, mv_stride, grid_m, m_stride,
                      contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0], out_dir[1],

@@ -65,8 +65,8 @@ typedef struct dt_mpm_params {
                            env_id[p] names particle p's environment, whose stencil
                            (clamped as for one environment) is shifted by e n_grid
                            nodes along x; walls apply per environment, so
-                           environments never exchange momentum.  Contact entry
-                           points require one environment.                          */
+                           environments never exchange momentum.  Contact takes
+                           one pad per environment (dt_contact_closest_env).       */
   const int32_t* env_id; /* device array [n] of environment ids (NULL: all 0); it
                            follows the particle order of the arrays passed with it */
 } dt_mpm_params;
@@ -428,6 +428,16 @@ int dt_contact_centroids(int32_t n_tris, const int32_t* tris, const float* pos, 
 int dt_contact_closest(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
                        int32_t n_tris, const float* centroids4, int32_t* contact_idx,
                        int32_t idx_stride, int32_t idx_col, void* stream);
+/* The same search over stacked environments (prm->n_env, one pad per
+   environment, the pads' triangles concatenated): a node of environment
+   e = i / n_grid searches triangles [tri_start[e], tri_start[e+1]) only and
+   its cell centre is local to the environment.  tri_start: device int32
+   [n_env + 1].  The collision entry points take the concatenated pads as
+   they are (their positions local to each environment). */
+int dt_contact_closest_env(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
+                           const int32_t* tri_start, int32_t n_tris, const float* centroids4,
+                           int32_t* contact_idx, int32_t idx_stride, int32_t idx_col,
+                           void* stream);
 /* collision (one pad): find_sdf + calculate_contact_force; grid_f += f*dt,
    ext_force[tri verts] += -f/3 */
 int dt_contact_collision(const dt_mpm_params* prm, const dt_contact_params* cp,
