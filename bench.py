@@ -49,6 +49,9 @@ def parse():
                          "population: 64 environments, batched vs a per-environment loop")
     ap.add_argument("--pop-substeps", type=int, default=300,
                     help="population workload: substeps per fwd+bwd step")
+    ap.add_argument("--pop-pads", action="store_true",
+                    help="population workload: one FEM pad (10,800 tets) per environment "
+                         "pressing its object")
     return ap.parse_args()
 
 
@@ -343,10 +346,13 @@ def bench_population(args, torch):
     Loss per environment: its final centre of mass along a seeded direction;
     the gradient is taken with respect to its initial velocities."""
     from difftactile_amd import scenes
-    from difftactile_amd.batch import EnvBatch
+    from difftactile_amd.batch import CoupledEnvBatch, EnvBatch
+    from difftactile_amd.coupled import CoupledSim, Pad
     from difftactile_amd.sim import MPMSim
-    envs = scenes.config4_population()
+    pads = args.pop_pads
+    envs = scenes.config4_population(pads=pads)
     T, seg = args.pop_substeps, 10
+    contact = dict(kn=1e4, kd=269.44, kt=154.78, mu_f=53.85)
     dev = torch.device("cuda")
     n_tot = sum(len(d["x"]) for d in envs)
     rng = np.random.default_rng(5)
@@ -359,33 +365,65 @@ def bench_population(args, torch):
         return [xs[e], vs[e].clone().requires_grad_(True), torch.zeros(n, 3, 3, device=dev),
                 torch.eye(3, device=dev).repeat(n, 1, 1)]
 
+    def pad_state(e):
+        nodes = torch.tensor(envs[e]["pad_mesh"][0], device=dev)
+        cv = torch.tensor(envs[e]["pad_cv"], device=dev)
+        return (nodes.clone(), cv.clone(), nodes.clone()), cv.expand(T + 1, -1, -1).contiguous()
+
     groups = {}
     for e, d in enumerate(envs):
         groups.setdefault(d["prm"].n_grid, []).append(e)
-    batches = [(EnvBatch(envs[ids[0]]["prm"],
-                         [(envs[e]["mats"], len(envs[e]["x"]), envs[e]["mat_id"]) for e in ids],
-                         checkpoint_every=seg, sort=True, resident_bytes=0), ids)
-               for ids in groups.values()]
-    sims = [MPMSim(d["prm"], d["mats"], len(d["x"]), mat_id=d["mat_id"], checkpoint_every=seg,
-                   sort=True, resident_bytes=0) for d in envs]
+    batches, sims = [], []
+    for ids in groups.values():
+        spec = [(envs[e]["mats"], len(envs[e]["x"]), envs[e]["mat_id"]) for e in ids]
+        kw = dict(checkpoint_every=seg, sort=True, resident_bytes=0)
+        if pads:
+            d0 = envs[ids[0]]
+            b = CoupledEnvBatch(d0["prm"], spec, [envs[e]["pad_mesh"] for e in ids],
+                                dict(out_dir=d0["pad_out"], **d0["pad"]), contact, **kw)
+        else:
+            b = EnvBatch(envs[ids[0]]["prm"], spec, **kw)
+        batches.append((b, ids))
+    for d in envs:
+        sim = MPMSim(d["prm"], d["mats"], len(d["x"]), mat_id=d["mat_id"], checkpoint_every=seg,
+                     sort=True, resident_bytes=0)
+        if pads:
+            nodes, f2v, tris, layer = d["pad_mesh"]
+            pad = Pad(nodes, f2v, tris, layer, d["pad_out"], d["pad"]["mu"], d["pad"]["lam"],
+                      d["pad"]["mass"], d["prm"].dt)
+            sim = CoupledSim(sim, [pad], **contact)
+        sims.append(sim)
 
     def run_batched():
         g = [None] * len(envs)
+        xs_out = [None] * len(envs)
         for batch, ids in batches:
             sts = [state(e) for e in ids]
-            out = batch.rollout(sts, T)
+            if pads:
+                ps = [pad_state(e) for e in ids]
+                res = batch.rollout(sts, [p[0] for p in ps], [p[1] for p in ps], T)
+                out = [r[0] for r in res]
+            else:
+                out = batch.rollout(sts, T)
             loss = sum((o[0].mean(0) * w[e]).sum() for o, e in zip(out, ids))
+            for e, o in zip(ids, out):
+                xs_out[e] = o[0].detach()
             for e, ge in zip(ids, torch.autograd.grad(loss, [st[1] for st in sts])):
                 g[e] = ge
-        return g
+        return g, xs_out
 
     def run_loop():
-        g = []
+        g, xs_out = [], []
         for e, sim in enumerate(sims):
             st = state(e)
-            out = sim.rollout(*st, T)
+            if pads:
+                p0, cv = pad_state(e)
+                out = sim.rollout(*st, [p0], [cv], T)
+            else:
+                out = sim.rollout(*st, T)
+            xs_out.append(out[0].detach())
             g.append(torch.autograd.grad((out[0].mean(0) * w[e]).sum(), st[1])[0])
-        return g
+        return g, xs_out
 
     def timed(fn, reps):
         torch.cuda.synchronize()
@@ -397,16 +435,26 @@ def bench_population(args, torch):
 
     log(f"[bench] population: {len(envs)} environments, {n_tot} particles, "
         f"{', '.join(f'{len(ids)} on {n}^3' for n, ids in groups.items())}, {T} substeps")
-    run_batched()
-    run_loop()
+    g_b0, _ = run_batched()
+    g_l0, _ = run_loop()
     reps = max(1, min(args.steps, 2))
-    t_b, g_b = timed(run_batched, reps)
+    t_b, (g_b, x_b) = timed(run_batched, reps)
     log(f"[bench] batched {t_b * 1e3:.1f} ms per fwd+bwd step")
-    t_l, g_l = timed(run_loop, reps)
+    t_l, (g_l, x_l) = timed(run_loop, reps)
     log(f"[bench] per-environment loop {t_l * 1e3:.1f} ms per fwd+bwd step")
     num = sum(float(((a - b) ** 2).sum()) for a, b in zip(g_b, g_l))
     den = sum(float((b ** 2).sum()) for b in g_l)
-    res = {"metric": "fwd+bwd env-substeps/s (64-environment population)",
+    rel = lambda a, b: float((a - b).norm() / max(float(b.norm()), 1e-30))
+    x_rel = [rel(a, b) for a, b in zip(x_b, x_l)]
+    g_rel = sorted(rel(a, b) for a, b in zip(g_b, g_l))
+    # the same loop twice: float atomics order sums differently from run to run
+    g_self = sorted(rel(a, b) for a, b in zip(g_l0, g_l))
+    log(f"[bench] batched vs loop: final x rel diff max {max(x_rel):.2e}; per-environment "
+        f"gradient rel diff median {g_rel[len(g_rel) // 2]:.2e}, max {g_rel[-1]:.2e}; "
+        f"gradient norms {min(float(t.norm()) for t in g_l):.3e}..{max(float(t.norm()) for t in g_l):.3e}; "
+        f"loop vs itself: median {g_self[len(g_self) // 2]:.2e}, max {g_self[-1]:.2e}")
+    res = {"metric": "fwd+bwd env-substeps/s (64-environment population"
+                     + (", one FEM pad each)" if pads else ")"),
            "value": len(envs) * T / t_b, "unit": "env-substeps/s", "n_gpus": 1, "steps": reps,
            "warmup": 1, "ms_per_step": t_b * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -414,11 +462,16 @@ def bench_population(args, torch):
            "loop": {"ms_per_step": t_l * 1e3, "value": len(envs) * T / t_l},
            "speedup_vs_loop": t_l / t_b,
            "grad_rel_diff_batched_vs_loop": (num / max(den, 1e-30)) ** 0.5,
+           "state_rel_diff_batched_vs_loop_max": max(x_rel),
+           "grad_rel_diff_per_env_median": g_rel[len(g_rel) // 2],
+           "grad_rel_diff_loop_vs_itself_median": g_self[len(g_self) // 2],
            "config": {"workload": f"{len(envs)} environments, {n_tot} particles "
                                   "(log-uniform 5e3-2e5 per environment), 64^3 below 3e4 "
                                   f"particles else 128^3, {T} substeps fwd+bwd, 10-substep "
                                   "segments recomputed; batched = one stacked launch sequence "
-                                  "per grid size, loop = one simulation per environment",
+                                  "per grid size, loop = one simulation per environment"
+                                  + ("; each environment with a 10,800-tet FEM pad pressing "
+                                     "its object (penalty contact)" if pads else ""),
                       "groups": {f"{n}^3": len(ids) for n, ids in groups.items()}}}
     print(json.dumps(res), flush=True)
 

@@ -80,13 +80,19 @@ def config3_multi_material(n_particles=1_000_000, n_grid=128, dt=1e-5, seed=0):
     return prm, mats, x, v0.astype(np.float32), mat_id
 
 
-def config4_population(n_env=64, seed=0, dt=1e-5):
+def config4_population(n_env=64, seed=0, dt=1e-5, pads=False):
     """Batched population: n_env independent objects with log-uniform particle
     counts in [5e3, 2e5] (seeded), each a ball at 2 particles per dx per axis
     on its own grid: 64^3 when it holds fewer than 3e4 particles, else 128^3.
     Materials alternate between fixed corotated (E=5e4) and von Mises (E=5e4,
     yield 50); rho = 1000; each ball starts with a seeded velocity of about
-    0.5 downwards.  Returns one dict per environment (prm, mats, x, v, mat_id)."""
+    0.5 downwards.  Returns one dict per environment (prm, mats, x, v, mat_id).
+
+    pads: each environment also gets a slab pad of 16 x 16 x 9 vertices
+    (10,800 tets, spacing 1/128, stable Neo-Hookean E = 1e5, nu = 0.45) resting
+    a quarter cell above its ball and driven down at 0.4 m/s (the C2 press
+    rate); the dict then carries pad_mesh (nodes, f2v, tris, layer), pad_out,
+    pad (mu, lam, mass) and pad_cv (the per-vertex control velocity)."""
     rng = np.random.default_rng(seed)
     counts = np.round(np.exp(rng.uniform(np.log(5e3), np.log(2e5), n_env))).astype(int)
     envs = []
@@ -99,8 +105,25 @@ def config4_population(n_env=64, seed=0, dt=1e-5):
                 else [Material(5e4, 0.3, VON_MISES, yield_stress=50.0)])
         v = np.tile(np.array([0.0, -0.5, 0.0], np.float32), (len(x), 1))
         v += rng.uniform(-0.1, 0.1, size=(1, 3)).astype(np.float32)
-        envs.append(dict(prm=prm, mats=mats, x=x, v=v.astype(np.float32),
-                         mat_id=np.zeros(len(x), np.int32)))
+        env = dict(prm=prm, mats=mats, x=x, v=v.astype(np.float32),
+                   mat_id=np.zeros(len(x), np.int32))
+        if pads:
+            from .sensor_model.meshes import slab_mesh
+            h = 1.0 / 128
+            top = float(np.max(x[:, 1]))
+            nodes, f2v, tris, layer, od = slab_mesh(16, 16, 9, h=h,
+                                                    origin=(0.5 - 7.5 * h, top + 0.25 * dx,
+                                                            0.5 - 7.5 * h))
+            E_p, nu_p = 1e5, 0.45
+            vol = (15 * h) * (15 * h) * (8 * h)
+            cv = np.zeros((len(nodes), 3), np.float32)
+            cv[:, 1] = -0.4
+            env.update(pad_mesh=(nodes.astype(np.float32), f2v, tris, layer), pad_out=od,
+                       pad=dict(mu=E_p / 2 / (1 + nu_p),
+                                lam=E_p * nu_p / (1 + nu_p) / (1 - 2 * nu_p),
+                                mass=1000.0 * vol / len(nodes)),
+                       pad_cv=cv)
+        envs.append(env)
     return envs
 
 
