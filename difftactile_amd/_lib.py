@@ -24,7 +24,8 @@ class DtMpmParams(ctypes.Structure):
         ("p_vol", ctypes.c_float), ("p_mass", ctypes.c_float), ("eps", ctypes.c_float),
         ("gravity", ctypes.c_float * 3), ("grad_clamp", ctypes.c_float),
         ("soa", ctypes.c_int32), ("svd_exact", ctypes.c_int32), ("n_env", ctypes.c_int32),
-        ("env_id", ctypes.c_void_p),
+        ("env_id", ctypes.c_void_p), ("active_bricks", ctypes.c_void_p),
+        ("n_active_bricks", ctypes.c_void_p),
     ]
 
 
@@ -83,6 +84,8 @@ _SIGS = {
     "dt_mpm_workspace_history_bytes": (_c_int, [_P(DtMpmParams), _c_int, _c_int, _P(_sz)]),
     "dt_mpm_bin": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _sz, _vp]),
     "dt_mpm_bin_perm": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _P(_vp), _P(_vp)]),
+    "dt_mpm_active_bricks": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _P(_vp), _P(_vp)]),
+    "dt_mpm_zero_nodes": (_c_int, [_P(DtMpmParams), _vp, _c_int, _vp]),
     "dt_mpm_p2g_binned": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 5
                           + [_vp, _sz, _vp]),
     "dt_mpm_grid_op4_tiles": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _vp, _vp]),

@@ -94,6 +94,16 @@ class CoupledSim:
         self.grid_f = torch.zeros(nn, 3, device=d)
         self.idx = torch.full((nn, self.npad), -1, dtype=I32, device=d)
 
+    def _node_set(self, base, ws, nb):
+        """Copy of the parameter struct base whose grid-node passes run over
+        the active bricks of the workspace's last binning."""
+        p = type(base).from_buffer_copy(base)
+        a, c = _lib.ctypes.c_void_p(), _lib.ctypes.c_void_p()
+        _lib.call("dt_mpm_active_bricks", base, self.sim.n, ws, nb, _lib.ctypes.byref(a),
+                  _lib.ctypes.byref(c))
+        p.active_bricks, p.n_active_bricks = a.value, c.value
+        return p
+
     # -- one substep --------------------------------------------------------
     def _fwd(self, st, nxt, pads_in, pads_out, cv, cv1, mat_id, store=None):
         """st/nxt: [x, v, C, F] SoA; pads_in/out: per pad [pos, vel, vpos];
@@ -107,14 +117,18 @@ class CoupledSim:
         idx = store["idx"] if store else self.idx
         mats, nm = P(sim.mats), len(sim.materials)
         mid = None if mat_id is None else P(mat_id, I32)
-        _lib.call("dt_mpm_zero4", nn, P(g4), s)
         if sim.sparse_grid:
+            # every grid pass below runs over the active bricks only: the
+            # tiles sum writes each of their nodes, nodes elsewhere are neither
+            # written nor read
             ws, nb = sim._workspace()
             _lib.call("dt_mpm_bin", prm, sim.n, P(st[0]), ws, nb, s)
+            prm = self._node_set(prm, ws, nb)
             _lib.call("dt_mpm_p2g_binned", prm, mats, nm, sim.n, mid, *[P(t) for t in st],
                       P(nxt[3]), ws, nb, s)
             _lib.call("dt_mpm_tiles_sum", prm, sim.n, ws, nb, P(g4), s)
         else:
+            _lib.call("dt_mpm_zero4", nn, P(g4), s)
             _lib.call("dt_mpm_p2g_fused", prm, mats, nm, sim.n, mid, *[P(t) for t in st], P(g4),
                       P(nxt[3]), s)
         vel_u = []
@@ -125,7 +139,8 @@ class CoupledSim:
             _lib.call("dt_fem_update_csr", pad.fem, pad.n_tets, pad.n_verts, pad.f2v, pad.B,
                       pos, vu, pad.vert_start, pad.vert_tets, pad.tet_buf, s)
             vel_u.append(vu)
-        idx.fill_(-1)
+        if not sim.sparse_grid:
+            idx.fill_(-1)
         for k, pad in enumerate(self.pads):
             _lib.call("dt_contact_centroids", pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
                       P(pad.cen), s)
@@ -137,7 +152,10 @@ class CoupledSim:
                 _lib.call("dt_contact_closest_env", prm, g4.data_ptr() + 12, 4,
                           P(pad.tri_start, I32), pad.n_tris, P(pad.cen), P(idx, I32), self.npad,
                           k, s)
-        gf.zero_()
+        if sim.sparse_grid:
+            _lib.call("dt_mpm_zero_nodes", prm, P(gf), 3, s)
+        else:
+            gf.zero_()
         exts = []
         for k, pad in enumerate(self.pads):
             ext = torch.zeros(pad.n_verts, 3, device=sim.device)
@@ -176,17 +194,22 @@ class CoupledSim:
             _lib.call("dt_fem_update2_adj", pad.fem, pad.n_verts, P(pad.layer, I32), P(gpn),
                       P(gvn), P(gvpn), P(gp), P(gvu), P(gvp), P(ge), P(g_cv[k]), P(g_cv1[k]), s)
             g_ext.append(ge)
-        _lib.call("dt_mpm_zero4", nn, P(ggv4), s)
+        prm_clamp = self.c_prm_clamp
         if sim.sparse_grid:
+            # the binning of the forward substep's positions again: the same
+            # active bricks as the stored grids
             ws, nb = sim._workspace()
             _lib.call("dt_mpm_bin", prm, sim.n, P(st[0]), ws, nb, s)
+            prm = self._node_set(prm, ws, nb)
+            prm_clamp = self._node_set(prm_clamp, ws, nb)
             _lib.call("dt_mpm_g2p4_adj_binned", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]),
                       P(g_next[1]), P(g_next[2]), P(g[0]), ws, nb, s)
             _lib.call("dt_mpm_tiles_sum", prm, sim.n, ws, nb, P(ggv4), s)
         else:
+            _lib.call("dt_mpm_zero4", nn, P(ggv4), s)
             _lib.call("dt_mpm_g2p4_adj", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]), P(g_next[1]),
                       P(g_next[2]), P(g[0]), P(ggv4), s)
-        _lib.call("dt_mpm_grid_op4_adj", self.c_prm_clamp, P(g4), P(gf), P(ggv4), P(gg4), P(ggf), s)
+        _lib.call("dt_mpm_grid_op4_adj", prm_clamp, P(g4), P(gf), P(ggv4), P(gg4), P(ggf), s)
         if self.clamp_grad:
             for k in range(len(self.pads)):
                 for t in (g_pads[k][0], g_pads[k][1]):

@@ -63,58 +63,61 @@ __global__ __launch_bounds__(kBlock) void k_closest(MpmP P, long long nn,
                                                     int col) {
   __shared__ float4 tile[kTriTile];
   __shared__ int range[2];
-  long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool occ = nd < nn && gm[nd * m_stride] > P.eps;
-  if (!__syncthreads_or(occ)) return;
-  const int n = P.n_grid;
-  float qx = 0.f, qy = 0.f, qz = 0.f;
-  int lo = 0, hi = nt;
-  if (nd < nn) {
-    int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), k = (int)(nd % n);
+  // every thread runs the same rounds (nd = -1: idle lane), so the barriers
+  // below are uniform; a round without an occupied node is skipped
+  node_loop_all(P, nn, [&](long long nd) {
+    const bool occ = nd >= 0 && gm[nd * m_stride] > P.eps;
+    if (!__syncthreads_or(occ)) return;
+    const int n = P.n_grid;
+    float qx = 0.f, qy = 0.f, qz = 0.f;
+    int lo = 0, hi = nt;
+    if (nd >= 0) {
+      int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), k = (int)(nd % n);
+      if (tri_start) {
+        const int e = i / n;
+        lo = tri_start[e];
+        hi = tri_start[e + 1];
+      }
+      qx = node_centre(i % n, P.dx);
+      qy = node_centre(j, P.dx);
+      qz = node_centre(k, P.dx);
+    }
+    // the block walks the union of its nodes' triangle ranges
+    int t_lo = 0, t_hi = nt;
     if (tri_start) {
-      const int e = i / n;
-      lo = tri_start[e];
-      hi = tri_start[e + 1];
+      if (threadIdx.x == 0) {
+        range[0] = 0x7fffffff;
+        range[1] = 0;
+      }
+      __syncthreads();
+      if (occ) {
+        atomicMin(&range[0], lo);
+        atomicMax(&range[1], hi);
+      }
+      __syncthreads();
+      t_lo = range[0];
+      t_hi = range[1];
     }
-    qx = node_centre(i % n, P.dx);
-    qy = node_centre(j, P.dx);
-    qz = node_centre(k, P.dx);
-  }
-  // the block walks the union of its nodes' triangle ranges
-  int t_lo = 0, t_hi = nt;
-  if (tri_start) {
-    if (threadIdx.x == 0) {
-      range[0] = 0x7fffffff;
-      range[1] = 0;
-    }
-    __syncthreads();
-    if (occ) {
-      atomicMin(&range[0], lo);
-      atomicMax(&range[1], hi);
-    }
-    __syncthreads();
-    t_lo = range[0];
-    t_hi = range[1];
-  }
-  float best = 100.0f;
-  int best_k = -1;
-  for (int t0 = t_lo; t0 < t_hi; t0 += kTriTile) {
-    int cnt = min(kTriTile, t_hi - t0);
-    __syncthreads();
-    if ((int)threadIdx.x < cnt) tile[threadIdx.x] = cen[t0 + threadIdx.x];
-    __syncthreads();
-    if (occ) {
-      const int q0 = max(lo - t0, 0), q1 = min(hi - t0, cnt);
-      for (int q = q0; q < q1; ++q) {
-        float d = centroid_dist(tile[q], qx, qy, qz);
-        if (d < best) {
-          best = d;
-          best_k = t0 + q;
+    float best = 100.0f;
+    int best_k = -1;
+    for (int t0 = t_lo; t0 < t_hi; t0 += kTriTile) {
+      int cnt = min(kTriTile, t_hi - t0);
+      __syncthreads();
+      if ((int)threadIdx.x < cnt) tile[threadIdx.x] = cen[t0 + threadIdx.x];
+      __syncthreads();
+      if (occ) {
+        const int q0 = max(lo - t0, 0), q1 = min(hi - t0, cnt);
+        for (int q = q0; q < q1; ++q) {
+          float d = centroid_dist(tile[q], qx, qy, qz);
+          if (d < best) {
+            best = d;
+            best_k = t0 + q;
+          }
         }
       }
     }
-  }
-  if (occ) idx[nd * stride + col] = best_k;
+    if (occ) idx[nd * stride + col] = best_k;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -197,29 +200,29 @@ __global__ __launch_bounds__(kBlock) void k_collision(MpmP P, ContactP C, long l
                                                       float o1, float o2,
                                                       float* __restrict__ gridf,
                                                       float* __restrict__ ext) {
-  long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= nn) return;
-  float mm = m[nd * sm];
-  if (!(mm > P.eps)) return;
-  int k = idx[nd * stride + col];
-  if (k < 0) return;
-  const int n = P.n_grid;
-  int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
-  float q[3] = {node_centre(i % n, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
-  float inv = 1.f / (mm + P.eps);
-  float qv[3] = {mv[nd * smv] * inv, mv[nd * smv + 1] * inv, mv[nd * smv + 2] * inv};
-  int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
-  float od[3] = {o0, o1, o2};
-  ContactFwd w;
-  contact_fwd(C, q, qv, pos, vel, tv, od, w);
-  if (!w.flag) return;
-  for (int a = 0; a < 3; ++a) {
-    gridf[3 * nd + a] += w.f[a] * P.dt;
-    float e = -w.f[a] / 3.f;
-    atomic_add_f(ext + 3ll * tv[0] + a, e);
-    atomic_add_f(ext + 3ll * tv[1] + a, e);
-    atomic_add_f(ext + 3ll * tv[2] + a, e);
-  }
+  node_loop(P, nn, [&](long long nd) {
+    float mm = m[nd * sm];
+    if (!(mm > P.eps)) return;
+    int k = idx[nd * stride + col];
+    if (k < 0) return;
+    const int n = P.n_grid;
+    int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
+    float q[3] = {node_centre(i % n, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
+    float inv = 1.f / (mm + P.eps);
+    float qv[3] = {mv[nd * smv] * inv, mv[nd * smv + 1] * inv, mv[nd * smv + 2] * inv};
+    int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
+    float od[3] = {o0, o1, o2};
+    ContactFwd w;
+    contact_fwd(C, q, qv, pos, vel, tv, od, w);
+    if (!w.flag) return;
+    for (int a = 0; a < 3; ++a) {
+      gridf[3 * nd + a] += w.f[a] * P.dt;
+      float e = -w.f[a] / 3.f;
+      atomic_add_f(ext + 3ll * tv[0] + a, e);
+      atomic_add_f(ext + 3ll * tv[1] + a, e);
+      atomic_add_f(ext + 3ll * tv[2] + a, e);
+    }
+  });
 }
 
 // Adjoint of k_collision.  Inputs: g_gridf (adjoint of grid_f), g_ext (adjoint
@@ -232,100 +235,100 @@ __global__ __launch_bounds__(kBlock) void k_collision_adj(
     float o0, float o1, float o2, const float* __restrict__ g_gridf,
     const float* __restrict__ g_ext, float* __restrict__ g_mv, int sgmv, float* __restrict__ g_m,
     int sgm, float* __restrict__ g_pos, float* __restrict__ g_vel) {
-  long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= nn) return;
-  float mm = m[nd * sm];
-  if (!(mm > P.eps)) return;
-  int k = idx[nd * stride + col];
-  if (k < 0) return;
-  const int n = P.n_grid;
-  int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
-  float q[3] = {node_centre(i % n, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
-  float inv = 1.f / (mm + P.eps);
-  float mvv[3] = {mv[nd * smv], mv[nd * smv + 1], mv[nd * smv + 2]};
-  float qv[3] = {mvv[0] * inv, mvv[1] * inv, mvv[2] * inv};
-  int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
-  float od[3] = {o0, o1, o2};
-  ContactFwd w;
-  contact_fwd(C, q, qv, pos, vel, tv, od, w);
-  if (!w.flag) return;
-  // adjoint of f: grid_f += f dt; ext[t] += -f/3
-  float gf[3];
-  for (int a = 0; a < 3; ++a)
-    gf[a] = g_gridf[3 * nd + a] * P.dt -
-            (g_ext[3ll * tv[0] + a] + g_ext[3ll * tv[1] + a] + g_ext[3ll * tv[2] + a]) / 3.f;
-  float gfn[3] = {gf[0], gf[1], gf[2]};
-  float gvt[3] = {0.f, 0.f, 0.f};
-  if (w.shear) {
-    // ft = vt * (mag / vtn)
-    float s = gf[0] * w.vt[0] + gf[1] * w.vt[1] + gf[2] * w.vt[2];
-    float ratio = w.mag / w.vtn;
-    for (int a = 0; a < 3; ++a) gvt[a] += ratio * gf[a];
-    float gmag = s / w.vtn;
-    float gvtn = -s * w.mag / (w.vtn * w.vtn);
-    if (w.kt_branch) {
-      gvtn += C.kt * gmag;
-    } else if (w.fnn > 0.f) {
-      float gfnn = C.mu_f * gmag;
-      for (int a = 0; a < 3; ++a) gfn[a] += gfnn * w.fn[a] / w.fnn;
+  node_loop(P, nn, [&](long long nd) {
+    float mm = m[nd * sm];
+    if (!(mm > P.eps)) return;
+    int k = idx[nd * stride + col];
+    if (k < 0) return;
+    const int n = P.n_grid;
+    int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
+    float q[3] = {node_centre(i % n, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
+    float inv = 1.f / (mm + P.eps);
+    float mvv[3] = {mv[nd * smv], mv[nd * smv + 1], mv[nd * smv + 2]};
+    float qv[3] = {mvv[0] * inv, mvv[1] * inv, mvv[2] * inv};
+    int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
+    float od[3] = {o0, o1, o2};
+    ContactFwd w;
+    contact_fwd(C, q, qv, pos, vel, tv, od, w);
+    if (!w.flag) return;
+    // adjoint of f: grid_f += f dt; ext[t] += -f/3
+    float gf[3];
+    for (int a = 0; a < 3; ++a)
+      gf[a] = g_gridf[3 * nd + a] * P.dt -
+              (g_ext[3ll * tv[0] + a] + g_ext[3ll * tv[1] + a] + g_ext[3ll * tv[2] + a]) / 3.f;
+    float gfn[3] = {gf[0], gf[1], gf[2]};
+    float gvt[3] = {0.f, 0.f, 0.f};
+    if (w.shear) {
+      // ft = vt * (mag / vtn)
+      float s = gf[0] * w.vt[0] + gf[1] * w.vt[1] + gf[2] * w.vt[2];
+      float ratio = w.mag / w.vtn;
+      for (int a = 0; a < 3; ++a) gvt[a] += ratio * gf[a];
+      float gmag = s / w.vtn;
+      float gvtn = -s * w.mag / (w.vtn * w.vtn);
+      if (w.kt_branch) {
+        gvtn += C.kt * gmag;
+      } else if (w.fnn > 0.f) {
+        float gfnn = C.mu_f * gmag;
+        for (int a = 0; a < 3; ++a) gfn[a] += gfnn * w.fn[a] / w.fnn;
+      }
+      for (int a = 0; a < 3; ++a) gvt[a] += gvtn * w.vt[a] / w.vtn;
     }
-    for (int a = 0; a < 3; ++a) gvt[a] += gvtn * w.vt[a] / w.vtn;
-  }
-  // vt = r - vnr n
-  float gr[3], gn[3], gvnr = 0.f;
-  for (int a = 0; a < 3; ++a) {
-    gr[a] = gvt[a];
-    gvnr -= gvt[a] * w.n[a];
-    gn[a] = -w.vnr * gvt[a];
-  }
-  // fn = -(kn + kd vn) sdf n
-  float kk2 = -(C.kn + C.kd * w.vn) * w.sdf;
-  float gfn_n = gfn[0] * w.n[0] + gfn[1] * w.n[1] + gfn[2] * w.n[2];
-  for (int a = 0; a < 3; ++a) gn[a] += kk2 * gfn[a];
-  float gsdf = -(C.kn + C.kd * w.vn) * gfn_n;
-  float gvn = -C.kd * w.sdf * gfn_n;
-  if (w.vnr > 0.f) gvnr += gvn;
-  // vnr = n . r
-  for (int a = 0; a < 3; ++a) {
-    gn[a] += gvnr * w.r[a];
-    gr[a] += gvnr * w.n[a];
-  }
-  // r = -qv + (1/3) sum vel;  qv = mv / (m + eps)
-  float gqv[3] = {-gr[0], -gr[1], -gr[2]};
-  float gmm = 0.f;
-  for (int a = 0; a < 3; ++a) {
-    atomic_add_f(g_mv + nd * sgmv + a, gqv[a] * inv);
-    gmm -= gqv[a] * mvv[a] * inv * inv;
-    float gvv = gr[a] / 3.f;
-    atomic_add_f(g_vel + 3ll * tv[0] + a, gvv);
-    atomic_add_f(g_vel + 3ll * tv[1] + a, gvv);
-    atomic_add_f(g_vel + 3ll * tv[2] + a, gvv);
-  }
-  atomic_add_f(g_m + nd * sgm, gmm);
-  // sdf = (q - p1) . n - offset
-  float gp1[3], ge1[3], ge2[3];
-  for (int a = 0; a < 3; ++a) {
-    gn[a] += gsdf * (q[a] - w.p1[a]);
-    gp1[a] = -gsdf * w.n[a];
-  }
-  // n = sgn * cr / s
-  float gnr[3] = {w.sgn * gn[0], w.sgn * gn[1], w.sgn * gn[2]};
-  float dotc = gnr[0] * w.cr[0] + gnr[1] * w.cr[1] + gnr[2] * w.cr[2];
-  float s3 = w.s * w.s * w.s;
-  float gcr[3];
-  for (int a = 0; a < 3; ++a) gcr[a] = gnr[a] / w.s - w.cr[a] * dotc / s3;
-  // cr = e1 x e2: g_e1 = e2 x g_cr, g_e2 = g_cr x e1
-  ge1[0] = w.e2[1] * gcr[2] - w.e2[2] * gcr[1];
-  ge1[1] = w.e2[2] * gcr[0] - w.e2[0] * gcr[2];
-  ge1[2] = w.e2[0] * gcr[1] - w.e2[1] * gcr[0];
-  ge2[0] = gcr[1] * w.e1[2] - gcr[2] * w.e1[1];
-  ge2[1] = gcr[2] * w.e1[0] - gcr[0] * w.e1[2];
-  ge2[2] = gcr[0] * w.e1[1] - gcr[1] * w.e1[0];
-  for (int a = 0; a < 3; ++a) {
-    atomic_add_f(g_pos + 3ll * tv[0] + a, gp1[a] - ge1[a] - ge2[a]);
-    atomic_add_f(g_pos + 3ll * tv[1] + a, ge1[a]);
-    atomic_add_f(g_pos + 3ll * tv[2] + a, ge2[a]);
-  }
+    // vt = r - vnr n
+    float gr[3], gn[3], gvnr = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      gr[a] = gvt[a];
+      gvnr -= gvt[a] * w.n[a];
+      gn[a] = -w.vnr * gvt[a];
+    }
+    // fn = -(kn + kd vn) sdf n
+    float kk2 = -(C.kn + C.kd * w.vn) * w.sdf;
+    float gfn_n = gfn[0] * w.n[0] + gfn[1] * w.n[1] + gfn[2] * w.n[2];
+    for (int a = 0; a < 3; ++a) gn[a] += kk2 * gfn[a];
+    float gsdf = -(C.kn + C.kd * w.vn) * gfn_n;
+    float gvn = -C.kd * w.sdf * gfn_n;
+    if (w.vnr > 0.f) gvnr += gvn;
+    // vnr = n . r
+    for (int a = 0; a < 3; ++a) {
+      gn[a] += gvnr * w.r[a];
+      gr[a] += gvnr * w.n[a];
+    }
+    // r = -qv + (1/3) sum vel;  qv = mv / (m + eps)
+    float gqv[3] = {-gr[0], -gr[1], -gr[2]};
+    float gmm = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      atomic_add_f(g_mv + nd * sgmv + a, gqv[a] * inv);
+      gmm -= gqv[a] * mvv[a] * inv * inv;
+      float gvv = gr[a] / 3.f;
+      atomic_add_f(g_vel + 3ll * tv[0] + a, gvv);
+      atomic_add_f(g_vel + 3ll * tv[1] + a, gvv);
+      atomic_add_f(g_vel + 3ll * tv[2] + a, gvv);
+    }
+    atomic_add_f(g_m + nd * sgm, gmm);
+    // sdf = (q - p1) . n - offset
+    float gp1[3], ge1[3], ge2[3];
+    for (int a = 0; a < 3; ++a) {
+      gn[a] += gsdf * (q[a] - w.p1[a]);
+      gp1[a] = -gsdf * w.n[a];
+    }
+    // n = sgn * cr / s
+    float gnr[3] = {w.sgn * gn[0], w.sgn * gn[1], w.sgn * gn[2]};
+    float dotc = gnr[0] * w.cr[0] + gnr[1] * w.cr[1] + gnr[2] * w.cr[2];
+    float s3 = w.s * w.s * w.s;
+    float gcr[3];
+    for (int a = 0; a < 3; ++a) gcr[a] = gnr[a] / w.s - w.cr[a] * dotc / s3;
+    // cr = e1 x e2: g_e1 = e2 x g_cr, g_e2 = g_cr x e1
+    ge1[0] = w.e2[1] * gcr[2] - w.e2[2] * gcr[1];
+    ge1[1] = w.e2[2] * gcr[0] - w.e2[0] * gcr[2];
+    ge1[2] = w.e2[0] * gcr[1] - w.e2[1] * gcr[0];
+    ge2[0] = gcr[1] * w.e1[2] - gcr[2] * w.e1[1];
+    ge2[1] = gcr[2] * w.e1[0] - gcr[0] * w.e1[2];
+    ge2[2] = gcr[0] * w.e1[1] - gcr[1] * w.e1[0];
+    for (int a = 0; a < 3; ++a) {
+      atomic_add_f(g_pos + 3ll * tv[0] + a, gp1[a] - ge1[a] - ge2[a]);
+      atomic_add_f(g_pos + 3ll * tv[1] + a, ge1[a]);
+      atomic_add_f(g_pos + 3ll * tv[2] + a, ge2[a]);
+    }
+  });
 }
 
 static ContactP contact_dev(const dt_contact_params* c) {
@@ -366,8 +369,8 @@ int dt_contact_closest(const dt_mpm_params* prm, const float* grid_m, int32_t m_
   DT_REQUIRE(grid_m && contact_idx && (n_tris == 0 || centroids4), "NULL pointer");
   DT_REQUIRE(prm->n_env <= 1, "stacked environments: use dt_contact_closest_env");
   long long nn = (long long)prm->n_grid * prm->n_grid * prm->n_grid;
-  hipLaunchKernelGGL(k_closest, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
-                     to_dev(*prm), nn, grid_m, m_stride, n_tris, (const int32_t*)nullptr,
+  hipLaunchKernelGGL(k_closest, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
+                     (hipStream_t)stream, to_dev_nodes(*prm), nn, grid_m, m_stride, n_tris, (const int32_t*)nullptr,
                      (const float4*)centroids4, contact_idx, idx_stride, idx_col);
   return check_launch("dt_contact_closest");
 }
@@ -384,8 +387,8 @@ int dt_contact_closest_env(const dt_mpm_params* prm, const float* grid_m, int32_
   DT_REQUIRE(grid_m && contact_idx && tri_start && (n_tris == 0 || centroids4), "NULL pointer");
   long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
                  prm->n_grid;
-  hipLaunchKernelGGL(k_closest, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
-                     to_dev(*prm), nn, grid_m, m_stride, n_tris, tri_start,
+  hipLaunchKernelGGL(k_closest, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
+                     (hipStream_t)stream, to_dev_nodes(*prm), nn, grid_m, m_stride, n_tris, tri_start,
                      (const float4*)centroids4, contact_idx, idx_stride, idx_col);
   return check_launch("dt_contact_closest_env");
 }
@@ -406,8 +409,8 @@ int dt_contact_collision(const dt_mpm_params* prm, const dt_contact_params* cp,
              "NULL pointer");
   long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
                  prm->n_grid;
-  hipLaunchKernelGGL(k_collision, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
-                     to_dev(*prm), contact_dev(cp), nn, grid_mv, mv_stride, grid_m, m_stride,
+  hipLaunchKernelGGL(k_collision, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
+                     (hipStream_t)stream, to_dev_nodes(*prm), contact_dev(cp), nn, grid_mv, mv_stride, grid_m, m_stride,
                      contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0], out_dir[1],
                      out_dir[2], grid_f, ext_force);
   return check_launch("dt_contact_collision");
@@ -432,8 +435,8 @@ int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* 
              "NULL pointer");
   long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
                  prm->n_grid;
-  hipLaunchKernelGGL(k_collision_adj, dim3(nblocks(nn)), dim3(kBlock), 0, (hipStream_t)stream,
-                     to_dev(*prm), contact_dev(cp), nn, grid_mv, mv_stride, grid_m, m_stride,
+  hipLaunchKernelGGL(k_collision_adj, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
+                     (hipStream_t)stream, to_dev_nodes(*prm), contact_dev(cp), nn, grid_mv, mv_stride, grid_m, m_stride,
                      contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0], out_dir[1],
                      out_dir[2], g_grid_f, g_ext_force, g_grid_mv, gmv_stride, g_grid_m,
                      gm_stride, g_pos, g_vel);

@@ -37,6 +37,8 @@ struct MpmP {
   int n_grid, bound, sticky_floor, clamp_grad, soa, svd_exact;
   int n_env, nx;   // environments stacked along x; nx = n_env n_grid x-nodes
   const int32_t* env_id;
+  const int32_t* alist;     // node set of the grid-node passes (NULL: all nodes)
+  const int32_t* n_alist;
   float dx, inv_dx, dt, p_vol, p_mass, eps;
   float g[3];
   float grad_clamp;
@@ -53,6 +55,9 @@ inline MpmP to_dev(const dt_mpm_params& p) {
   d.n_env = p.n_env > 1 ? p.n_env : 1;
   d.nx = d.n_env * p.n_grid;
   d.env_id = d.n_env > 1 ? p.env_id : nullptr;
+  // the node set is honoured only by the entry points documented to take it
+  d.alist = nullptr;
+  d.n_alist = nullptr;
   d.dx = p.dx;
   d.inv_dx = p.inv_dx;
   d.dt = p.dt;
@@ -69,6 +74,67 @@ inline MpmP to_dev(const dt_mpm_params& p) {
 // First x-node of particle p's environment (environments stacked along x).
 __device__ __forceinline__ int env_x0(const MpmP& P, long long p) {
   return P.env_id ? P.env_id[p] * P.n_grid : 0;
+}
+
+// Node of lane threadIdx&63 in 4x4x4-node brick b (-1 outside the grid).
+__device__ __forceinline__ long long active_node(const MpmP& P, int b) {
+  const int n = P.n_grid, nb = (n + 3) >> 2;
+  const int l = threadIdx.x & 63;
+  const int bz = b % nb, by = (b / nb) % nb, bx = b / (nb * nb);
+  const int i = bx * 4 + (l >> 4), j = by * 4 + ((l >> 2) & 3), k = bz * 4 + (l & 3);
+  if (i >= P.nx || j >= n || k >= n) return -1;
+  return ((long long)i * n + j) * n + k;
+}
+
+// Node loops of the grid-node passes: every node of the n-node grid (one per
+// thread), or the nodes of the active bricks (P.alist: a persistent grid, four
+// bricks per block and round).  f(nd) is called with nd = -1 for idle lanes
+// (node_loop_all: every thread calls f the same number of times, so f may
+// hold block barriers).
+template <class F>
+__device__ __forceinline__ void node_loop(const MpmP& P, long long nn, const F& f) {
+  if (P.alist) {
+    const int n_act = *P.n_alist;
+    for (int ai = blockIdx.x * 4 + (threadIdx.x >> 6); ai < n_act; ai += gridDim.x * 4) {
+      const long long nd = active_node(P, P.alist[ai]);
+      if (nd >= 0) f(nd);
+    }
+  } else {
+    const long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd < nn) f(nd);
+  }
+}
+template <class F>
+__device__ __forceinline__ void node_loop_all(const MpmP& P, long long nn, const F& f) {
+  if (P.alist) {
+    const int n_act = *P.n_alist;
+    for (int b0 = blockIdx.x * 4; b0 < n_act; b0 += gridDim.x * 4) {
+      const int ai = b0 + (threadIdx.x >> 6);
+      f(ai < n_act ? active_node(P, P.alist[ai]) : -1ll);
+    }
+  } else {
+    const long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    f(nd < nn ? nd : -1ll);
+  }
+}
+
+// P with the node set of prm, and the launch grid of the node loops
+inline MpmP to_dev_nodes(const dt_mpm_params& p) {
+  MpmP d = to_dev(p);
+  if (p.active_bricks && p.n_active_bricks) {
+    d.alist = p.active_bricks;
+    d.n_alist = p.n_active_bricks;
+  }
+  return d;
+}
+inline int node_blocks(const dt_mpm_params& p, long long nn) {
+  if (p.active_bricks && p.n_active_bricks) {
+    const long long nb = (p.n_grid + 3) >> 2;
+    const long long nbx = ((long long)(p.n_env > 1 ? p.n_env : 1) * p.n_grid + 3) >> 2;
+    const long long b = (nbx * nb * nb + 3) / 4;
+    return (int)(b < 1024 ? b : 1024);
+  }
+  return nblocks(nn);
 }
 
 int validate_params(const dt_mpm_params* p);

@@ -893,6 +893,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_occupy(MpmP P, long long nn,
 __device__ __forceinline__ void grid_node_fwd(const MpmP& P, long long nd, float mv0, float mv1,
                                               float mv2, float m, float f0, float f1, float f2,
                                               float (&vo)[3], bool (&mask)[3]) {
+  // one rounding sequence wherever this is inlined (dense, node-set and tile
+  // loops): no contraction, so hoisting dt*g out of a loop cannot change it
+#pragma clang fp contract(off)
   const int n = P.n_grid;
   int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), k = (int)(nd % n);
   float inv_m = 1.f / (m + P.eps);
@@ -974,19 +977,19 @@ __global__ __launch_bounds__(kBlock) void k_grid_op4(MpmP P, long long nn,
                                                      const float4* __restrict__ g4,
                                                      const float* __restrict__ gf,
                                                      float4* __restrict__ v4) {
-  long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= nn) return;
-  float4 q = g4[nd];
-  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (q.w > P.eps) {
-    float f0 = 0.f, f1 = 0.f, f2 = 0.f;
-    if (gf) { f0 = gf[3 * nd]; f1 = gf[3 * nd + 1]; f2 = gf[3 * nd + 2]; }
-    float vo[3];
-    bool mask[3];
-    grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, f0, f1, f2, vo, mask);
-    out = make_float4(vo[0], vo[1], vo[2], 0.f);
-  }
-  v4[nd] = out;
+  node_loop(P, nn, [&](long long nd) {
+    float4 q = g4[nd];
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q.w > P.eps) {
+      float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+      if (gf) { f0 = gf[3 * nd]; f1 = gf[3 * nd + 1]; f2 = gf[3 * nd + 2]; }
+      float vo[3];
+      bool mask[3];
+      grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, f0, f1, f2, vo, mask);
+      out = make_float4(vo[0], vo[1], vo[2], 0.f);
+    }
+    v4[nd] = out;
+  });
 }
 
 // packed grid_op adjoint: gv4 -> ggrid4 (overwritten); optional contact
@@ -997,31 +1000,40 @@ __global__ __launch_bounds__(kBlock) void k_grid_op4_adj(MpmP P, long long nn,
                                                          const float4* __restrict__ gv4,
                                                          float4* __restrict__ gg4,
                                                          float* __restrict__ ggf) {
-  long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= nn) return;
-  float4 q = g4[nd];
-  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-  float4 outf = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (q.w > P.eps) {
-    float f0 = 0.f, f1 = 0.f, f2 = 0.f;
-    if (gf) { f0 = gf[3 * nd]; f1 = gf[3 * nd + 1]; f2 = gf[3 * nd + 2]; }
-    float vo[3];
-    bool mask[3];
-    grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, f0, f1, f2, vo, mask);
-    float4 gi = gv4[nd];
-    float g0 = mask[0] ? 0.f : gi.x, g1 = mask[1] ? 0.f : gi.y, g2 = mask[2] ? 0.f : gi.z;
-    float inv_m = 1.f / (q.w + P.eps);
-    float gmm = -inv_m * inv_m * (g0 * (q.x + f0) + g1 * (q.y + f1) + g2 * (q.z + f2));
-    if (P.clamp_grad) gmm = fminf(fmaxf(gmm, -P.grad_clamp), P.grad_clamp);
-    out = make_float4(inv_m * g0, inv_m * g1, inv_m * g2, gmm);
-    outf = make_float4(inv_m * g0, inv_m * g1, inv_m * g2, 0.f);
-  }
-  gg4[nd] = out;
-  if (ggf) {
-    ggf[3 * nd] = outf.x;
-    ggf[3 * nd + 1] = outf.y;
-    ggf[3 * nd + 2] = outf.z;
-  }
+  node_loop(P, nn, [&](long long nd) {
+#pragma clang fp contract(off)
+    float4 q = g4[nd];
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 outf = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q.w > P.eps) {
+      float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+      if (gf) { f0 = gf[3 * nd]; f1 = gf[3 * nd + 1]; f2 = gf[3 * nd + 2]; }
+      float vo[3];
+      bool mask[3];
+      grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, f0, f1, f2, vo, mask);
+      float4 gi = gv4[nd];
+      float g0 = mask[0] ? 0.f : gi.x, g1 = mask[1] ? 0.f : gi.y, g2 = mask[2] ? 0.f : gi.z;
+      float inv_m = 1.f / (q.w + P.eps);
+      float gmm = -inv_m * inv_m * (g0 * (q.x + f0) + g1 * (q.y + f1) + g2 * (q.z + f2));
+      if (P.clamp_grad) gmm = fminf(fmaxf(gmm, -P.grad_clamp), P.grad_clamp);
+      out = make_float4(inv_m * g0, inv_m * g1, inv_m * g2, gmm);
+      outf = make_float4(inv_m * g0, inv_m * g1, inv_m * g2, 0.f);
+    }
+    gg4[nd] = out;
+    if (ggf) {
+      ggf[3 * nd] = outf.x;
+      ggf[3 * nd + 1] = outf.y;
+      ggf[3 * nd + 2] = outf.z;
+    }
+  });
+}
+
+// zero width floats per node of the node set
+__global__ __launch_bounds__(kBlock) void k_zero_nodes(MpmP P, long long nn, float* __restrict__ a,
+                                                       int width) {
+  node_loop(P, nn, [&](long long nd) {
+    for (int c = 0; c < width; ++c) a[nd * width + c] = 0.f;
+  });
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero4(long long nn, float4* __restrict__ a) {
@@ -1145,16 +1157,6 @@ __device__ __forceinline__ float4 tile_sum(const MpmP& P, long long nd,
   return r;
 }
 
-// Node of lane threadIdx&63 in active brick b (-1 outside the grid).
-__device__ __forceinline__ long long active_node(const MpmP& P, int b) {
-  const int n = P.n_grid, nb = (n + 3) >> 2;
-  const int l = threadIdx.x & 63;
-  const int bz = b % nb, by = (b / nb) % nb, bx = b / (nb * nb);
-  const int i = bx * 4 + (l >> 4), j = by * 4 + ((l >> 2) & 3), k = bz * 4 + (l & 3);
-  if (i >= P.nx || j >= n || k >= n) return -1;
-  return ((long long)i * n + j) * n + k;
-}
-
 // Grid passes over the active-brick list: a persistent grid, 4 bricks (one per
 // wavefront) per block and iteration.
 #define DT_ACTIVE_NODES(nd)                                                          \
@@ -1204,6 +1206,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_op4_adj_tiles(MpmP P,
                                                                float4* __restrict__ gv4,
                                                                float4* __restrict__ gg4) {
   DT_ACTIVE_NODES(nd) {
+#pragma clang fp contract(off)
     const float4 gi = tile_sum(P, nd, tiles, start);
     gv4[nd] = gi;
     const float4 q = g4[nd];
@@ -2353,8 +2356,9 @@ int dt_mpm_grid_op4(const dt_mpm_params* prm, const float* grid4, const float* g
   if (rc) return rc;
   DT_REQUIRE(grid4 && grid_v4, "NULL pointer");
   long long nn = grid_nodes(prm);
-  hipLaunchKernelGGL(k_grid_op4, dim3(nblocks(nn)), dim3(kBlock), 0, DT_STREAM(stream),
-                     to_dev(*prm), nn, (const float4*)grid4, grid_f, (float4*)grid_v4);
+  hipLaunchKernelGGL(k_grid_op4, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev_nodes(*prm), nn, (const float4*)grid4, grid_f,
+                     (float4*)grid_v4);
   return check_launch("dt_mpm_grid_op4");
 }
 
@@ -2390,10 +2394,20 @@ int dt_mpm_grid_op4_adj(const dt_mpm_params* prm, const float* grid4, const floa
   if (rc) return rc;
   DT_REQUIRE(grid4 && g_grid_v4 && g_grid4, "NULL pointer");
   long long nn = grid_nodes(prm);
-  hipLaunchKernelGGL(k_grid_op4_adj, dim3(nblocks(nn)), dim3(kBlock), 0, DT_STREAM(stream),
-                     to_dev(*prm), nn, (const float4*)grid4, grid_f, (const float4*)g_grid_v4,
-                     (float4*)g_grid4, g_grid_f);
+  hipLaunchKernelGGL(k_grid_op4_adj, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev_nodes(*prm), nn, (const float4*)grid4, grid_f,
+                     (const float4*)g_grid_v4, (float4*)g_grid4, g_grid_f);
   return check_launch("dt_mpm_grid_op4_adj");
+}
+
+int dt_mpm_zero_nodes(const dt_mpm_params* prm, float* buf, int32_t width, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(buf && width > 0, "NULL buffer or width < 1");
+  long long nn = grid_nodes(prm);
+  hipLaunchKernelGGL(k_zero_nodes, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev_nodes(*prm), nn, buf, (int)width);
+  return check_launch("dt_mpm_zero_nodes");
 }
 
 int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats,
@@ -2480,6 +2494,18 @@ int dt_mpm_bin(const dt_mpm_params* prm, int32_t n, const float* x, void* worksp
   BinWs ws{};
   if ((rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
   return launch_mark_bin(to_dev(*prm), prm, n, x, ws, DT_STREAM(stream));
+}
+
+int dt_mpm_active_bricks(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                         size_t workspace_bytes, const int32_t** active_bricks,
+                         const int32_t** n_active_bricks) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
+  if (active_bricks) *active_bricks = ws.alist;
+  if (n_active_bricks) *n_active_bricks = ws.n_alist;
+  return DT_OK;
 }
 
 int dt_mpm_bin_perm(const dt_mpm_params* prm, int32_t n, const void* workspace,

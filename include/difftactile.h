@@ -69,6 +69,14 @@ typedef struct dt_mpm_params {
                            one pad per environment (dt_contact_closest_env).       */
   const int32_t* env_id; /* device array [n] of environment ids (NULL: all 0); it
                            follows the particle order of the arrays passed with it */
+  const int32_t* active_bricks;   /* optional node set of the grid-node passes
+                           (dt_mpm_grid_op4(_adj), dt_mpm_zero_nodes and the contact
+                           passes): device list of 4x4x4-node bricks and, */
+  const int32_t* n_active_bricks; /* device count of it.  NULL: every node.  The
+                           active-brick list of a binned workspace
+                           (dt_mpm_active_bricks) covers every node a stencil of
+                           its particles touches, so passes restricted to it leave
+                           the other nodes unwritten and unread.                 */
 } dt_mpm_params;
 
 /* One row of the material table (multi_obj.py:65-74 generalised). */
@@ -296,6 +304,14 @@ int dt_mpm_workspace_history_bytes(const dt_mpm_params* prm, int32_t n, int32_t 
    [0, n_grid-3] range) and marks the active bricks. */
 int dt_mpm_bin(const dt_mpm_params* prm, int32_t n, const float* x, void* workspace,
                size_t workspace_bytes, void* stream);
+/* Device pointers of the last dt_mpm_bin's active-brick list and its count
+   (for dt_mpm_params.active_bricks / n_active_bricks). */
+int dt_mpm_active_bricks(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                         size_t workspace_bytes, const int32_t** active_bricks,
+                         const int32_t** n_active_bricks);
+/* Zeroes width floats per node over the node set of prm (all nodes, or the
+   active bricks). */
+int dt_mpm_zero_nodes(const dt_mpm_params* prm, float* buf, int32_t width, void* stream);
 /* Device pointers into the workspace: perm (n) and brick_start
    (dt_mpm_brick_count + 1) of the last dt_mpm_bin; the particles of brick b are
    perm[brick_start[b] .. brick_start[b+1]) in no particular order. */

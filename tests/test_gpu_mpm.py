@@ -485,6 +485,83 @@ def test_binning_of_no_particles(gpu):
     assert not start.any()
 
 
+@pytest.mark.parametrize("n_env", [1, 3])
+def test_node_set_passes_cover_the_active_bricks_only(gpu, n_env):
+    """dt_mpm_active_bricks: the last binning's brick list holds every brick a
+    particle's 3x3x3 stencil touches (and only those or their stencil-side
+    neighbours, each once).  dt_mpm_zero_nodes and dt_mpm_grid_op4 restricted
+    to it write exactly the listed bricks' nodes, equal bit for bit to the
+    dense pass there, and leave every other node as it was."""
+    import ctypes
+    n_grid, n = 32, 4000
+    prm = default_prm(n_grid)
+    prm.n_env = n_env
+    rng = np.random.default_rng(17 + n_env)
+    x = rng.uniform(0.2, 0.55, size=(n, 3)).astype(np.float32)
+    env = rng.integers(0, n_env, n).astype(np.int32)
+    c_prm = prm.c_struct()   # x is [n, 3]
+    env_t = dev(env, torch.int32)
+    if n_env > 1:
+        c_prm.env_id = env_t.data_ptr()
+    ws, nbytes = _workspace(c_prm, n)
+    call("dt_mpm_bin", c_prm, n, dev(x), ws, nbytes, _lib.stream())
+    a, c = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(_lib.lib().dt_mpm_active_bricks(c_prm, n, ws.data_ptr(), nbytes, ctypes.byref(a),
+                                               ctypes.byref(c)))
+    o_a, o_c = a.value - ws.data_ptr(), c.value - ws.data_ptr()
+    n_act = int(ws[o_c:o_c + 4].view(torch.int32).item())
+    act = ws[o_a:o_a + 4 * n_act].view(torch.int32).cpu().numpy().astype(np.int64)
+    assert len(np.unique(act)) == n_act
+    nb = (n_grid + 3) // 4
+    base = np.clip(om.cell_base_np(x, prm.inv_dx).astype(np.int64), 0, n_grid - 3)
+    base[:, 0] += env.astype(np.int64) * n_grid
+    off = np.stack(np.meshgrid(*[np.arange(3)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    nodes = (base[:, None, :] + off[None]).reshape(-1, 3) >> 2
+    touched = np.unique((nodes[:, 0] * nb + nodes[:, 1]) * nb + nodes[:, 2])
+    assert np.isin(touched, act).all()
+    nbrs = (base[:, None, :] >> 2) + np.stack(np.meshgrid(*[np.arange(2)] * 3, indexing="ij"),
+                                              -1).reshape(-1, 3)[None]
+    nbrs = nbrs.reshape(-1, 3)
+    allowed = np.unique((nbrs[:, 0] * nb + nbrs[:, 1]) * nb + nbrs[:, 2])
+    assert np.isin(act, allowed).all()
+    # nodes of the listed bricks (inside the stacked grid)
+    nx = n_env * n_grid
+    ijk = np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    bx, by, bz = act // (nb * nb), (act // nb) % nb, act % nb
+    nd = np.stack([bx, by, bz], -1)[:, None, :] * 4 + ijk[None]
+    nd = nd.reshape(-1, 3)
+    nd = nd[(nd[:, 0] < nx) & (nd[:, 1] < n_grid) & (nd[:, 2] < n_grid)]
+    in_set = np.zeros(nx * n_grid * n_grid, bool)
+    in_set[(nd[:, 0] * n_grid + nd[:, 1]) * n_grid + nd[:, 2]] = True
+    sel = torch.from_numpy(in_set)
+    c_set = type(c_prm).from_buffer_copy(c_prm)
+    c_set.active_bricks, c_set.n_active_bricks = a.value, c.value
+    nn = nx * n_grid * n_grid
+    buf = torch.full((nn, 3), 7.0, device="cuda")
+    call("dt_mpm_zero_nodes", c_set, buf, 3, _lib.stream())
+    h = buf.cpu()
+    assert (h[sel] == 0).all() and (h[~sel] == 7.0).all()
+    g4 = torch.rand(nn, 4, device="cuda") - 0.5
+    g4[:, 3] = torch.rand(nn, device="cuda") * (torch.rand(nn, device="cuda") < 0.5)
+    gf = 0.01 * torch.randn(nn, 3, device="cuda")
+    v_dense = torch.empty(nn, 4, device="cuda")
+    call("dt_mpm_grid_op4", c_prm, g4, gf, v_dense, _lib.stream())
+    v_set = torch.full((nn, 4), 9.0, device="cuda")
+    call("dt_mpm_grid_op4", c_set, g4, gf, v_set, _lib.stream())
+    hv, hd = v_set.cpu(), v_dense.cpu()
+    assert torch.equal(hv[sel], hd[sel]), (hv[sel] - hd[sel]).abs().max()
+    assert (hv[~sel] == 9.0).all()
+    gv = torch.randn(nn, 4, device="cuda")
+    out_d = [torch.empty(nn, 4, device="cuda"), torch.empty(nn, 3, device="cuda")]
+    call("dt_mpm_grid_op4_adj", c_prm, g4, gf, gv, *out_d, _lib.stream())
+    out_s = [torch.full((nn, 4), 9.0, device="cuda"), torch.full((nn, 3), 9.0, device="cuda")]
+    call("dt_mpm_grid_op4_adj", c_set, g4, gf, gv, *out_s, _lib.stream())
+    for a_s, a_d in zip(out_s, out_d):
+        hs, hd = a_s.cpu(), a_d.cpu()
+        assert torch.equal(hs[sel], hd[sel]), (hs[sel] - hd[sel]).abs().max()
+        assert (hs[~sel] == 9.0).all()
+
+
 def test_workspace_too_small_is_rejected(gpu):
     prm = default_prm(32)
     c_prm = prm.c_struct()
