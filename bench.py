@@ -312,7 +312,9 @@ def bench_pad_press(args, torch):
         pos, vel, vpos = out[4][0]
         loss = marker_displacement(mm, pos, vpos)
         g = torch.autograd.grad(loss, cvl)[0]
-        return loss, g
+        # contact evidence: how far the press pushed the block down
+        dy = float((out[0][:, 1] - x0[:, 1]).min().detach())
+        return loss, g, dy
 
     log(f"[bench] pad_press: {n} particles, {len(f2v)} tets, {T} substeps")
     for _ in range(args.warmup):
@@ -320,7 +322,7 @@ def bench_pad_press(args, torch):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        loss, g = step()
+        loss, g, dy = step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gv = g.sum(1)          # d loss / d (pad velocity of frame f), summed over vertices
@@ -333,7 +335,8 @@ def bench_pad_press(args, torch):
                                   f"({n} particles, {prm.n_grid}^3), {T} substeps fwd+bwd, "
                                   "loss = summed marker displacement"},
            "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(g).all()),
-           "grad_dL_dvy_first_last": [float(gv[0, 1]), float(gv[-1, 1])]}
+           "grad_dL_dvy_first_last": [float(gv[0, 1]), float(gv[-1, 1])],
+           "block_min_dy": dy}
     print(json.dumps(res), flush=True)
 
 

@@ -16,6 +16,11 @@ _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
 
 
+# centroids per bounding sphere of the accelerated closest-triangle search
+# (DT_CONTACT_TRI_RUN)
+TRI_RUN = 8
+
+
 class DtMpmParams(ctypes.Structure):
     _fields_ = [
         ("n_grid", ctypes.c_int32), ("bound", ctypes.c_int32),
@@ -118,6 +123,9 @@ _SIGS = {
     "dt_contact_centroids": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
     "dt_contact_closest": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _c_int,
                                     _c_int, _vp]),
+    "dt_contact_tri_batches": (_c_int, [_c_int, _vp, _vp, _vp]),
+    "dt_contact_closest_batched": (_c_int, [_P(DtMpmParams), _vp, _c_int, _vp, _c_int, _vp, _vp,
+                                            _vp, _vp, _c_int, _c_int, _vp]),
     "dt_contact_closest_env": (_c_int, [_P(DtMpmParams), _vp, _c_int, _vp, _c_int, _vp, _vp,
                                         _c_int, _c_int, _vp]),
     "dt_contact_collision": (_c_int, [_P(DtMpmParams), _P(DtContactParams), _vp, _c_int, _vp,

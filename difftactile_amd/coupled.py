@@ -67,6 +67,9 @@ class Pad:
         self.fem = _lib.DtFemParams()
         self.fem.mu, self.fem.lam, self.fem.dt, self.fem.mass = mu, lam, dt, mass
         self.cen = torch.empty(self.n_tris, 4, device=self.device)
+        # bounding spheres of runs of centroids (the accelerated search)
+        runs = (self.n_tris + _lib.TRI_RUN - 1) // _lib.TRI_RUN
+        self.tri_bounds = torch.empty(max(1, runs), 4, device=self.device)
         # vertex -> (tet, corner) CSR for the atomic-free tet-force passes
         vs, vt = vertex_tet_csr(np.asarray(f2v), self.n_verts)
         self.vert_start = torch.as_tensor(vs, device=self.device)
@@ -115,6 +118,8 @@ class CoupledSim:
         gv4 = store["gv4"] if store else sim._grid_scratch()[1]
         gf = store["gf"] if store else self.grid_f
         idx = store["idx"] if store else self.idx
+        # closest-triangle starts: the previous substep's result
+        hint = store.get("hint", idx) if store else idx
         mats, nm = P(sim.mats), len(sim.materials)
         mid = None if mat_id is None else P(mat_id, I32)
         if sim.sparse_grid:
@@ -144,14 +149,11 @@ class CoupledSim:
         for k, pad in enumerate(self.pads):
             _lib.call("dt_contact_centroids", pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
                       P(pad.cen), s)
+            _lib.call("dt_contact_tri_batches", pad.n_tris, P(pad.cen), P(pad.tri_bounds), s)
             # grid mass = 4th float of each packed node
-            if pad.tri_start is None:
-                _lib.call("dt_contact_closest", prm, g4.data_ptr() + 12, 4, pad.n_tris,
-                          P(pad.cen), P(idx, I32), self.npad, k, s)
-            else:
-                _lib.call("dt_contact_closest_env", prm, g4.data_ptr() + 12, 4,
-                          P(pad.tri_start, I32), pad.n_tris, P(pad.cen), P(idx, I32), self.npad,
-                          k, s)
+            _lib.call("dt_contact_closest_batched", prm, g4.data_ptr() + 12, 4,
+                      None if pad.tri_start is None else P(pad.tri_start, I32), pad.n_tris,
+                      P(pad.cen), P(pad.tri_bounds), P(hint, I32), P(idx, I32), self.npad, k, s)
         if sim.sparse_grid:
             _lib.call("dt_mpm_zero_nodes", prm, P(gf), 3, s)
         else:
@@ -314,6 +316,7 @@ class _CoupledRollout(torch.autograd.Function):
                 store = {"g4": torch.empty(nn, 4, device=dev), "gv4": torch.empty(nn, 4, device=dev),
                          "gf": torch.empty(nn, 3, device=dev),
                          "idx": torch.empty(nn, cs.npad, dtype=I32, device=dev),
+                         "hint": stores[-1]["idx"] if stores else cs.idx,
                          "vel_u": [torch.empty(pd.n_verts, 3, device=dev) for pd in cs.pads]}
                 nxt = [torch.empty_like(t) for t in frames[-1]]
                 pn = [[torch.empty_like(t) for t in p] for p in pframes[-1]]

@@ -10,18 +10,25 @@
 //
 // The closest-triangle search is the reference's brute force (first minimum
 // of the centroid distance wins), evaluated with one fixed float32 rounding
-// rule so it is reproducible bit for bit; triangle centroids are staged in LDS
-// tiles and whole blocks of unoccupied nodes exit early.
+// rule so it is reproducible bit for bit; chunks of unoccupied nodes are
+// skipped whole.
 #include "dt_common.h"
 #include "dt_device.h"
 
 namespace dt {
 
-constexpr int kTriTile = 256;
-
 struct ContactP {
   float kn, kd, kt, mu_f, norm_eps, sdf_offset;
 };
+
+// Lower bound on the squared centroid distance from the expanded form,
+// lb = (1 - 4e-6)(|c|^2 + |q|^2) - 2 c.q: three FMAs per triangle.  With
+// S = |c|^2 + |q|^2 and u = 2^-24, its rounding error is below 13u S (about
+// 13 roundings of terms bounded by 2S), while the float32 squared distance the
+// search compares is at least |c - q|^2 - 12u S (six relative roundings of a
+// value at most 2S).  So lb < |c - q|^2 - (4e-6 - 25u) S < that distance, and a
+// triangle with lb >= the node's best squared distance cannot win.
+constexpr float kBoundShrink = 1.0f - 4e-6f;
 
 // centroid c = fl32(1/3) * ((p1 + p2) + p3), no contraction
 __global__ __launch_bounds__(kBlock) void k_centroids(int nt, const int32_t* __restrict__ tris,
@@ -35,7 +42,8 @@ __global__ __launch_bounds__(kBlock) void k_centroids(int nt, const int32_t* __r
   float x = third * ((pos[3ll * a] + pos[3ll * b]) + pos[3ll * c]);
   float y = third * ((pos[3ll * a + 1] + pos[3ll * b + 1]) + pos[3ll * c + 1]);
   float z = third * ((pos[3ll * a + 2] + pos[3ll * b + 2]) + pos[3ll * c + 2]);
-  cen[t] = make_float4(x, y, z, 0.f);
+  // |c|^2 shrunk by 4e-6: part of the search's lower bound on a squared distance
+  cen[t] = make_float4(x, y, z, kBoundShrink * ((x * x + y * y) + z * z));
 }
 
 __device__ __forceinline__ float node_centre(int i, float dx) {
@@ -43,11 +51,13 @@ __device__ __forceinline__ float node_centre(int i, float dx) {
   return ((float)i + 0.5f) * dx;
 }
 
-__device__ __forceinline__ float centroid_dist(const float4 c, float qx, float qy, float qz) {
+// squared centroid distance plus the 1e-10 of the reference's norm: its
+// square root is the distance the search compares
+__device__ __forceinline__ float centroid_dist2(const float4 c, float qx, float qy, float qz) {
 #pragma clang fp contract(off)
   float d0 = c.x - qx, d1 = c.y - qy, d2 = c.z - qz;
   float dd = (d0 * d0 + d1 * d1) + d2 * d2;
-  return sqrtf(dd + 1e-10f);
+  return dd + 1e-10f;
 }
 
 // check_collision for one pad: contact_idx[node*stride + col] = closest triangle
@@ -55,24 +65,139 @@ __device__ __forceinline__ float centroid_dist(const float4 c, float qx, float q
 // tri_start (environments stacked along x): the node of environment e = i /
 // n_grid searches only triangles [tri_start[e], tri_start[e+1]) (its own pad)
 // and positions are local to the environment; NULL: all nt triangles.
-__global__ __launch_bounds__(kBlock) void k_closest(MpmP P, long long nn,
+//
+// The reference scans triangles in index order from (100, -1) and keeps a
+// distance only when it is strictly smaller: the result is the smallest index
+// among those at the minimum distance (-1 if none is below 100).  That is the
+// lexicographic minimum of (distance, index), which no visiting order
+// changes, so the search here visits triangles in whatever order skips most
+// of them:
+//
+// * a block takes one chunk of 64 nodes at a time (an active brick, or 64
+//   consecutive nodes); its eight wavefronts each take an eighth of the
+//   chunk's triangle range, reading centroids at wave-uniform indices
+//   (scalar loads), and the parts' minima are combined at the end;
+// * a node may start from a hinted triangle (its result of the previous
+//   substep, say: any index in its range is a valid start), which makes its
+//   best distance tight from the first batch on;
+// * with batch bounds (a sphere around each run of kTriBatch centroids), a
+//   batch whose sphere lies beyond the best distance of all 64 nodes is
+//   skipped without loading its centroids;
+// * inside a visited batch, groups of four centroids whose distance lower
+//   bounds (centroid .w) all exceed the best are skipped.
+//
+// Skips use bounds with relative margins far above float32 rounding (stated
+// at each test), so a skipped triangle's float32 distance is strictly larger
+// than the best one: it could neither win nor tie.
+constexpr int kClosestThreads = 512;
+constexpr int kClosestWaves = kClosestThreads / 64;
+constexpr int kTriBatch = DT_CONTACT_TRI_RUN;
+
+// best = lexicographic minimum of (distance, index) so far, best_s its squared
+// distance, thr_s = best_s (1 + 1e-6): every s whose square root (correctly
+// rounded, so monotone) is at most best lies below thr_s.
+struct Closest {
+  float best, best_s, thr_s;
+  int best_k;
+};
+
+template <bool kPerLane>
+__device__ __forceinline__ void closest_step(const float4 c, int t, int lo, int hi, float qx,
+                                             float qy, float qz, Closest& r) {
+  const float sq = centroid_dist2(c, qx, qy, qz);
+  if (sq <= r.thr_s && (!kPerLane || (t >= lo && t < hi))) {
+    const float d = sqrtf(sq);
+    if (d < r.best || (d == r.best && t < r.best_k)) {
+      r.best = d;
+      r.best_s = sq;
+      r.best_k = t;
+      r.thr_s = sq * (1.0f + 1e-6f);
+    }
+  }
+}
+
+// kTriBatch centroids from t (wave-uniform), loaded eight at a time (scalar
+// loads issued together); groups of four whose lower bounds lb (kBoundShrink,
+// qb = (1 - 4e-6)|q|^2) all exceed thr_s are skipped (s > lb > thr_s)
+template <bool kPerLane>
+__device__ __forceinline__ void closest_batch(const float4* __restrict__ cen, int t, int lo, int hi,
+                                              float qx, float qy, float qz, float qb,
+                                              Closest& r) {
+  constexpr int kG = 4, kL = 8;
+  const float mx = -2.f * qx, my = -2.f * qy, mz = -2.f * qz;
+#pragma unroll
+  for (int h = 0; h < kTriBatch; h += kL) {
+    float4 c[kL];
+#pragma unroll
+    for (int u = 0; u < kL; ++u) c[u] = cen[t + h + u];
+#pragma unroll
+    for (int g = 0; g < kL; g += kG) {
+      float lb = 3.4e38f;
+#pragma unroll
+      for (int u = g; u < g + kG; ++u)
+        lb = fminf(lb, fmaf(c[u].x, mx, fmaf(c[u].y, my, fmaf(c[u].z, mz, c[u].w + qb))));
+      if (lb <= r.thr_s) {
+#pragma unroll
+        for (int u = g; u < g + kG; ++u)
+          closest_step<kPerLane>(c[u], t + h + u, lo, hi, qx, qy, qz, r);
+      }
+    }
+  }
+}
+
+// lexicographic (distance, index) order; index -1 (nothing found) is larger
+// than any found triangle
+__device__ __forceinline__ bool closest_before(float d, int k, float bd, int bk) {
+  if (k < 0) return false;
+  if (bk < 0) return true;
+  return d < bd || (d == bd && k < bk);
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <bool kBounds>
+__global__ __launch_bounds__(kClosestThreads) void k_closest(MpmP P, long long nn,
                                                     const float* __restrict__ gm, int m_stride,
                                                     int nt, const int32_t* __restrict__ tri_start,
                                                     const float4* __restrict__ cen,
-                                                    int32_t* __restrict__ idx, int stride,
-                                                    int col) {
-  __shared__ float4 tile[kTriTile];
-  __shared__ int range[2];
-  // every thread runs the same rounds (nd = -1: idle lane), so the barriers
-  // below are uniform; a round without an occupied node is skipped
-  node_loop_all(P, nn, [&](long long nd) {
+                                                    const float4* __restrict__ bounds,
+                                                    const int32_t* hint, int32_t* idx,
+                                                    int stride, int col) {
+  __shared__ float s_best[kClosestWaves][64];
+  __shared__ int s_k[kClosestWaves][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = P.n_grid;
+  const long long n_chunks = P.alist ? (long long)*P.n_alist : (nn + 63) / 64;
+  for (long long ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
+    // every wave of the block holds the same 64 nodes, so the tests below
+    // agree across the block and the barriers are uniform
+    // node coordinates from the brick and lane (or the node index)
+    int i, j, k;
+    long long nd;
+    if (P.alist) {
+      const int nb = (n + 3) >> 2, b = P.alist[ch];
+      const int bz = b % nb, by = (b / nb) % nb, bx = b / (nb * nb);
+      i = bx * 4 + (lane >> 4), j = by * 4 + ((lane >> 2) & 3), k = bz * 4 + (lane & 3);
+      nd = (i < P.nx && j < n && k < n) ? ((long long)i * n + j) * n + k : -1;
+    } else {
+      nd = ch * 64 + lane;
+      if (nd >= nn) nd = -1;
+      const long long nn2 = (long long)n * n;
+      i = (int)(nd / nn2), j = (int)((nd / n) % n), k = (int)(nd % n);
+    }
     const bool occ = nd >= 0 && gm[nd * m_stride] > P.eps;
-    if (!__syncthreads_or(occ)) return;
-    const int n = P.n_grid;
+    if (__ballot(occ) == 0ull) continue;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     int lo = 0, hi = nt;
     if (nd >= 0) {
-      int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), k = (int)(nd % n);
       if (tri_start) {
         const int e = i / n;
         lo = tri_start[e];
@@ -82,42 +207,108 @@ __global__ __launch_bounds__(kBlock) void k_closest(MpmP P, long long nn,
       qy = node_centre(j, P.dx);
       qz = node_centre(k, P.dx);
     }
-    // the block walks the union of its nodes' triangle ranges
+    // the chunk searches the union of its occupied nodes' triangle ranges; a
+    // node outside the common range checks each triangle against its own
     int t_lo = 0, t_hi = nt;
+    bool mixed = false;
     if (tri_start) {
-      if (threadIdx.x == 0) {
-        range[0] = 0x7fffffff;
-        range[1] = 0;
-      }
-      __syncthreads();
-      if (occ) {
-        atomicMin(&range[0], lo);
-        atomicMax(&range[1], hi);
-      }
-      __syncthreads();
-      t_lo = range[0];
-      t_hi = range[1];
+      t_lo = __builtin_amdgcn_readfirstlane(wave_min_i(occ ? lo : 0x7fffffff));
+      t_hi = __builtin_amdgcn_readfirstlane(wave_max_i(occ ? hi : 0));
+      mixed = __ballot(occ && (lo != t_lo || hi != t_hi)) != 0ull;
     }
-    float best = 100.0f;
-    int best_k = -1;
-    for (int t0 = t_lo; t0 < t_hi; t0 += kTriTile) {
-      int cnt = min(kTriTile, t_hi - t0);
-      __syncthreads();
-      if ((int)threadIdx.x < cnt) tile[threadIdx.x] = cen[t0 + threadIdx.x];
-      __syncthreads();
-      if (occ) {
-        const int q0 = max(lo - t0, 0), q1 = min(hi - t0, cnt);
-        for (int q = q0; q < q1; ++q) {
-          float d = centroid_dist(tile[q], qx, qy, qz);
-          if (d < best) {
-            best = d;
-            best_k = t0 + q;
-          }
+    if (!occ) lo = hi = 0;   // idle lanes take no triangle
+    const float qb = kBoundShrink * ((qx * qx + qy * qy) + qz * qz);
+    Closest r{100.0f, 1e4f, 1e4f * (1.0f + 1e-6f), -1};
+    // this wave's part of the batches overlapping [t_lo, t_hi)
+    const int b_lo = t_lo / kTriBatch;
+    const int b_hi = t_hi > t_lo ? (t_hi + kTriBatch - 1) / kTriBatch : b_lo;
+    const int per = (b_hi - b_lo + kClosestWaves - 1) / kClosestWaves;
+    const int wb0 = __builtin_amdgcn_readfirstlane(min(b_lo + w * per, b_hi));
+    const int wb1 = __builtin_amdgcn_readfirstlane(min(wb0 + per, b_hi));
+    // seed from the hint (hint and idx may be the same array: every read of
+    // a node's entry precedes the barrier before its write)
+    if (hint && occ) {
+      const int h = hint[nd * stride + col];
+      if (h >= lo && h < hi) closest_step<false>(cen[h], h, lo, hi, qx, qy, qz, r);
+    }
+    for (int b = wb0; b < wb1; ++b) {
+      const int t0 = max(b * kTriBatch, t_lo), t1 = min(b * kTriBatch + kTriBatch, t_hi);
+      if (kBounds) {
+        // skip when, for every node, the batch sphere (radius .w, inflated by
+        // 2e-5 relative) lies beyond T = best (1 + 1e-5): then every centroid
+        // c of it has |q - c| >= |q - centre| - r > T, and the float32 errors
+        // of these tests (a few 2^-24 relative) stay below the margins
+        const float4 s4 = bounds[b];
+        const float d0 = s4.x - qx, d1 = s4.y - qy, d2 = s4.z - qz;
+        const float dd = d0 * d0 + d1 * d1 + d2 * d2;
+        const float reach = s4.w + r.best * (1.0f + 1e-5f);
+        if (__ballot(t1 > t0 && lo < t1 && hi > t0 && !(dd > reach * reach)) == 0ull) continue;
+      }
+      if (t1 - t0 == kTriBatch) {
+        if (mixed)
+          closest_batch<true>(cen, t0, lo, hi, qx, qy, qz, qb, r);
+        else
+          closest_batch<false>(cen, t0, lo, hi, qx, qy, qz, qb, r);
+      } else {
+        for (int t = t0; t < t1; ++t) closest_step<true>(cen[t], t, lo, hi, qx, qy, qz, r);
+      }
+    }
+    s_best[w][lane] = r.best;
+    s_k[w][lane] = r.best_k;
+    __syncthreads();
+    if (w == 0 && occ) {
+      float bd = 100.0f;
+      int bk = -1;
+      for (int q = 0; q < kClosestWaves; ++q)
+        if (closest_before(s_best[q][lane], s_k[q][lane], bd, bk)) {
+          bd = s_best[q][lane];
+          bk = s_k[q][lane];
         }
-      }
+      idx[nd * stride + col] = bk;
     }
-    if (occ) idx[nd * stride + col] = best_k;
-  });
+    __syncthreads();
+  }
+}
+
+// Sphere around each run of kTriBatch centroids: the centre of their bounding
+// box and the largest centroid distance from it, inflated by 2e-5 relative
+// (float32 differences of floats are exact to 2^-24 relative, so the real
+// distances stay inside).
+__global__ __launch_bounds__(kBlock) void k_tri_batches(int nt, const float4* __restrict__ cen,
+                                                        float4* __restrict__ bounds) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t0 = b * kTriBatch;
+  if (t0 >= nt) return;
+  const int t1 = min(t0 + kTriBatch, nt);
+  float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  for (int t = t0; t < t1; ++t) {
+    const float4 c = cen[t];
+    lo[0] = fminf(lo[0], c.x), hi[0] = fmaxf(hi[0], c.x);
+    lo[1] = fminf(lo[1], c.y), hi[1] = fmaxf(hi[1], c.y);
+    lo[2] = fminf(lo[2], c.z), hi[2] = fmaxf(hi[2], c.z);
+  }
+  const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
+  float r2 = 0.f;
+  for (int t = t0; t < t1; ++t) {
+    const float4 c = cen[t];
+    const float d0 = c.x - cx, d1 = c.y - cy, d2 = c.z - cz;
+    r2 = fmaxf(r2, d0 * d0 + d1 * d1 + d2 * d2);
+  }
+  bounds[b] = make_float4(cx, cy, cz, sqrtf(r2) * (1.0f + 2e-5f));
+}
+
+// persistent grid of the closest-triangle search: one block per chunk up to a cap
+static unsigned closest_blocks(const dt_mpm_params& p, long long nn) {
+  long long c;
+  if (p.active_bricks && p.n_active_bricks) {
+    const long long nb = (p.n_grid + 3) >> 2;
+    const long long nbx = ((long long)(p.n_env > 1 ? p.n_env : 1) * p.n_grid + 3) >> 2;
+    c = nbx * nb * nb;
+  } else {
+    c = (nn + 63) / 64;
+  }
+  if (c < 1) c = 1;
+  return (unsigned)(c < 8192 ? c : 8192);
 }
 
 // ---------------------------------------------------------------------------
@@ -358,21 +549,37 @@ int dt_contact_centroids(int32_t n_tris, const int32_t* tris, const float* pos, 
   return check_launch("dt_contact_centroids");
 }
 
+static int closest_launch(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
+                          const int32_t* tri_start, int32_t n_tris, const float* centroids4,
+                          const float* bounds4, const int32_t* hint, int32_t* contact_idx,
+                          int32_t idx_stride, int32_t idx_col, void* stream, const char* name) {
+  DT_REQUIRE(n_tris >= 0 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
+                 idx_col < idx_stride,
+             "bad size/stride");
+  DT_REQUIRE(grid_m && contact_idx && (n_tris == 0 || centroids4), "NULL pointer");
+  long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
+                 prm->n_grid;
+  if (bounds4 && n_tris > 0)
+    hipLaunchKernelGGL(k_closest<true>, dim3(closest_blocks(*prm, nn)), dim3(kClosestThreads), 0,
+                       (hipStream_t)stream, to_dev_nodes(*prm), nn, grid_m, m_stride, n_tris,
+                       tri_start, (const float4*)centroids4, (const float4*)bounds4, hint,
+                       contact_idx, idx_stride, idx_col);
+  else
+    hipLaunchKernelGGL(k_closest<false>, dim3(closest_blocks(*prm, nn)), dim3(kClosestThreads), 0,
+                       (hipStream_t)stream, to_dev_nodes(*prm), nn, grid_m, m_stride, n_tris,
+                       tri_start, (const float4*)centroids4, (const float4*)nullptr, hint,
+                       contact_idx, idx_stride, idx_col);
+  return check_launch(name);
+}
+
 int dt_contact_closest(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
                        int32_t n_tris, const float* centroids4, int32_t* contact_idx,
                        int32_t idx_stride, int32_t idx_col, void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
-  DT_REQUIRE(n_tris >= 0 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
-                 idx_col < idx_stride,
-             "bad size/stride");
-  DT_REQUIRE(grid_m && contact_idx && (n_tris == 0 || centroids4), "NULL pointer");
   DT_REQUIRE(prm->n_env <= 1, "stacked environments: use dt_contact_closest_env");
-  long long nn = (long long)prm->n_grid * prm->n_grid * prm->n_grid;
-  hipLaunchKernelGGL(k_closest, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
-                     (hipStream_t)stream, to_dev_nodes(*prm), nn, grid_m, m_stride, n_tris, (const int32_t*)nullptr,
-                     (const float4*)centroids4, contact_idx, idx_stride, idx_col);
-  return check_launch("dt_contact_closest");
+  return closest_launch(prm, grid_m, m_stride, nullptr, n_tris, centroids4, nullptr, nullptr,
+                        contact_idx, idx_stride, idx_col, stream, "dt_contact_closest");
 }
 
 int dt_contact_closest_env(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
@@ -381,16 +588,33 @@ int dt_contact_closest_env(const dt_mpm_params* prm, const float* grid_m, int32_
                            void* stream) {
   int rc = validate_params(prm);
   if (rc) return rc;
-  DT_REQUIRE(n_tris >= 0 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
-                 idx_col < idx_stride,
-             "bad size/stride");
-  DT_REQUIRE(grid_m && contact_idx && tri_start && (n_tris == 0 || centroids4), "NULL pointer");
-  long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
-                 prm->n_grid;
-  hipLaunchKernelGGL(k_closest, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
-                     (hipStream_t)stream, to_dev_nodes(*prm), nn, grid_m, m_stride, n_tris, tri_start,
-                     (const float4*)centroids4, contact_idx, idx_stride, idx_col);
-  return check_launch("dt_contact_closest_env");
+  DT_REQUIRE(tri_start, "NULL tri_start");
+  return closest_launch(prm, grid_m, m_stride, tri_start, n_tris, centroids4, nullptr, nullptr,
+                        contact_idx, idx_stride, idx_col, stream, "dt_contact_closest_env");
+}
+
+int dt_contact_tri_batches(int32_t n_tris, const float* centroids4, float* batch_bounds4,
+                           void* stream) {
+  DT_REQUIRE(n_tris >= 0, "n_tris must be >= 0");
+  if (n_tris == 0) return DT_OK;
+  DT_REQUIRE(centroids4 && batch_bounds4, "NULL pointer");
+  const int nb = (n_tris + kTriBatch - 1) / kTriBatch;
+  hipLaunchKernelGGL(k_tri_batches, dim3(nblocks(nb)), dim3(kBlock), 0, (hipStream_t)stream,
+                     n_tris, (const float4*)centroids4, (float4*)batch_bounds4);
+  return check_launch("dt_contact_tri_batches");
+}
+
+int dt_contact_closest_batched(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
+                               const int32_t* tri_start, int32_t n_tris, const float* centroids4,
+                               const float* batch_bounds4, const int32_t* hint,
+                               int32_t* contact_idx, int32_t idx_stride, int32_t idx_col,
+                               void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(prm->n_env <= 1 || tri_start, "stacked environments need tri_start");
+  DT_REQUIRE(n_tris == 0 || batch_bounds4, "NULL batch bounds");
+  return closest_launch(prm, grid_m, m_stride, tri_start, n_tris, centroids4, batch_bounds4, hint,
+                        contact_idx, idx_stride, idx_col, stream, "dt_contact_closest_batched");
 }
 
 int dt_contact_collision(const dt_mpm_params* prm, const dt_contact_params* cp,

@@ -134,19 +134,25 @@ def identity_F(n):
 def config2_pad_press(n_particles=200_000, n_grid=128, dt=1e-5, n_substeps=500, seed=0):
     """Slab pad pressing an elastoplastic block.
 
-    Domain 0.1 m per side (dx = 0.1/128 m).  Block: n_particles on a lattice at
-    dx/2 spacing (+-0.25 dx seeded jitter) resting on the floor band, von Mises
-    yield 50 (E = 5e4, nu = 0.3, rho = 1000).  Pad: 40 x 40 x 6 vertex slab
-    (45,630 tets, 3,042 contact triangles), spacing dx, Neo-Hookean E = 1e5,
-    nu = 0.45, rho = 1000; contact face 0.25 dx above the block top; the fixed
-    top layer is driven down 2 mm at constant velocity over n_substeps.
-    Penalty contact kn = 1e4 (kd, kt, friction from the grasp task).
+    The object constants of the grasp task (``tasks/grasp_plastic.py:35-46``:
+    space_scale 10, obj_scale 4, rho 4), whose units its contact constants are
+    given in; a full node then holds a mass of ~4, far above the 1e-5
+    occupancy threshold.  Domain 10 per side (dx = 10/128).  Block: n_particles
+    on a lattice at dx/2 spacing (+-0.25 dx seeded jitter) resting on the floor
+    band, von Mises yield 50 (E = 5e4, nu = 0.3).  Pad: 40 x 40 x 6
+    vertex slab (45,630 tets, 3,042 contact triangles), spacing dx,
+    Neo-Hookean E = 1e5, nu = 0.45, density 2000 as the reference sensor's
+    (``sensor_model/fem_sensor.py:37``; with it the grasp task's shear
+    damping kt is stable on the pad's vertices); contact face 0.25 dx above the
+    block top; the fixed top layer is driven down 2 mm at constant velocity
+    over n_substeps.  Penalty contact kn = 1e4 (kd, kt, friction from the
+    grasp task).
     Returns a dict with everything the coupled runner needs.
     """
     from .params import VON_MISES
     from .sensor_model.meshes import slab_mesh
-    space = 0.1
-    prm = MPMParams(n_grid=n_grid, space_scale=space, rho=1000.0, dt=dt,
+    space = 10.0
+    prm = MPMParams(n_grid=n_grid, space_scale=space, obj_scale=4.0, rho=4.0, dt=dt,
                     gravity=(0.0, -9.8, 0.0))
     dx = prm.dx
     spacing = dx / 2
@@ -168,15 +174,15 @@ def config2_pad_press(n_particles=200_000, n_grid=128, dt=1e-5, n_substeps=500, 
     h = dx
     ox = 0.5 * space - 19.5 * h
     nodes, f2v, tris, layer, out_dir = slab_mesh(40, 40, 6, h=h, origin=(ox, top + 0.25 * dx, ox))
-    E, nu, rho_pad = 1e5, 0.45, 1000.0
+    E, nu, rho_pad = 1e5, 0.45, 2000.0
     mu, lam = E / 2 / (1 + nu), E * nu / (1 + nu) / (1 - 2 * nu)
     vol = (39 * h) * (39 * h) * (5 * h)
     mass = rho_pad * vol / len(nodes)
-    press = 2e-3 / (n_substeps * dt)
+    press = 0.2 / (n_substeps * dt)
     cv = np.zeros((n_substeps + 1, len(nodes), 3), np.float32)
     cv[:, :, 1] = -press
     # camera: 1 cm behind the contact face, looking at it (pad-local +y toward the face)
-    cam_origin = np.array([0.5 * space, top + 0.25 * dx + 0.01, 0.5 * space])
+    cam_origin = np.array([0.5 * space, top + 0.25 * dx + 1.0, 0.5 * space])
     Rwl = np.diag([1.0, -1.0, -1.0])     # world -> local rotation (180 deg about x)
     inv_T = np.eye(4)
     inv_T[:3, :3] = Rwl

@@ -434,7 +434,9 @@ int dt_clamp(int64_t n, float* a, float limit, void* stream);
    idx_stride entries per node (one column per pad), -1 = none.
    out_dir is a HOST pointer to 3 floats (the pad's outward direction). */
 
-/* triangle centroids (1/3)((p1 + p2) + p3), float4 per triangle */
+/* triangle centroids (1/3)((p1 + p2) + p3), float4 per triangle: xyz the
+   centroid, w = (1 - 4e-6)|c|^2, a bound the closest-triangle search uses to
+   skip triangles (any w between 0 and that value gives the same indices) */
 int dt_contact_centroids(int32_t n_tris, const int32_t* tris, const float* pos, float* centroids4,
                          void* stream);
 /* check_collision + find_closest (fem_sensor.py:343-359) for one pad: for each
@@ -454,6 +456,25 @@ int dt_contact_closest_env(const dt_mpm_params* prm, const float* grid_m, int32_
                            const int32_t* tri_start, int32_t n_tris, const float* centroids4,
                            int32_t* contact_idx, int32_t idx_stride, int32_t idx_col,
                            void* stream);
+/* Accelerated form of the same search (same indices, bit for bit): spheres
+   around each run of DT_CONTACT_TRI_RUN centroids let the search skip runs
+   beyond a node's best distance.  dt_contact_tri_batches writes the spheres,
+   float4 (centre, radius) per run, ceil(n_tris / DT_CONTACT_TRI_RUN) of them,
+   from the centroids of
+   dt_contact_centroids; tri_start as in dt_contact_closest_env, or NULL for a
+   single environment.  hint (optional, same layout as contact_idx, may be
+   contact_idx itself): a triangle per node to start from, such as the
+   previous substep's result; entries outside the node's range are ignored,
+   and any start gives the same result.  Replaces fem_sensor.py:343-359 like
+   dt_contact_closest. */
+#define DT_CONTACT_TRI_RUN 8
+int dt_contact_tri_batches(int32_t n_tris, const float* centroids4, float* batch_bounds4,
+                           void* stream);
+int dt_contact_closest_batched(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
+                               const int32_t* tri_start, int32_t n_tris, const float* centroids4,
+                               const float* batch_bounds4, const int32_t* hint,
+                               int32_t* contact_idx, int32_t idx_stride, int32_t idx_col,
+                               void* stream);
 /* collision (one pad): find_sdf + calculate_contact_force; grid_f += f*dt,
    ext_force[tri verts] += -f/3 */
 int dt_contact_collision(const dt_mpm_params* prm, const dt_contact_params* cp,

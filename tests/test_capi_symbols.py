@@ -55,3 +55,9 @@ def test_call_rejects_host_tensors_before_any_launch():
     with pytest.raises(ValueError):
         _lib._arg(torch.zeros(3))
     assert _lib._arg(None) is None and _lib._arg(7) == 7
+
+
+def test_binding_constants_match_header():
+    src = open(os.path.join(ROOT, "include", "difftactile.h")).read()
+    m = re.search(r"#define\s+DT_CONTACT_TRI_RUN\s+(\d+)", src)
+    assert m and int(m.group(1)) == _lib.TRI_RUN

@@ -162,16 +162,114 @@ def test_closest_triangle_bit_exact(gpu):
     cen = torch.empty(len(surf), 4, device="cuda")
     _lib.call("dt_contact_centroids", len(surf), dev(surf, torch.int32),
               dev(pos), P(cen), _lib.stream())
-    idx = torch.full((nn, 2), -7, dtype=torch.int32, device="cuda")
-    _lib.call("dt_contact_closest", prm.c_struct(), dev(m32), 1, len(surf), P(cen),
-              P(idx, torch.int32), 2, 1, _lib.stream())
     nodes_o, ref = oc.check_collision(op, m32, [torch.tensor(pos)], [surf])
-    got = idx.cpu().numpy()
     assert len(nodes_o) > 1000
-    np.testing.assert_array_equal(got[nodes_o, 1], ref[:, 0])
+    # a previous result to start from: the answer itself on occupied nodes
+    # (rolled by one so most starts are not the answer), garbage elsewhere
+    ref_full = np.random.default_rng(4).integers(-3, len(surf) + 3, nn).astype(np.int32)
+    ref_full[nodes_o] = np.roll(ref[:, 0], 1)
+    occ_mask = np.zeros(nn, bool)
+    occ_mask[nodes_o] = True
+    bnd = torch.empty(-(-len(surf) // _lib.TRI_RUN), 4, device="cuda")
+    _lib.call("dt_contact_tri_batches", len(surf), P(cen), P(bnd), _lib.stream())
+    for batched in (False, True):
+        idx = torch.full((nn, 2), -7, dtype=torch.int32, device="cuda")
+        if batched:
+            # the output as its own hint, holding the previous result
+            idx[:, 1] = torch.as_tensor(ref_full, device="cuda")
+            _lib.call("dt_contact_closest_batched", prm.c_struct(), dev(m32), 1, None, len(surf),
+                      P(cen), P(bnd), P(idx, torch.int32), P(idx, torch.int32), 2, 1,
+                      _lib.stream())
+        else:
+            _lib.call("dt_contact_closest", prm.c_struct(), dev(m32), 1, len(surf), P(cen),
+                      P(idx, torch.int32), 2, 1, _lib.stream())
+        got = idx.cpu().numpy()
+        np.testing.assert_array_equal(got[nodes_o, 1], ref[:, 0])
+        untouched = ~occ_mask
+        before = ref_full[untouched] if batched else -7
+        assert (got[untouched, 1] == before).all() and (got[:, 0] == -7).all()
+
+
+@pytest.mark.parametrize("node_set,batched", [(False, False), (True, False), (False, True),
+                                              (True, True)])
+def test_closest_triangle_ties_envs_and_node_set(gpu, node_set, batched):
+    """Closest triangle for two stacked environments, each searching only its
+    own pad, on lattice-aligned slabs (the two triangles of a cell are
+    equidistant from many node centres: the first minimum must win), over
+    every node or over the active bricks of a binning, by the plain search or
+    the one that skips runs of triangles by their bounding spheres; bit-exact
+    against the oracle's float32 brute force per environment."""
+    import ctypes
+    from difftactile_amd.batch import concat_pad_meshes
+    from tests.test_gpu_mpm import _workspace
+    n_env, n_grid = 2, 32
+    prm = MPMParams(n_grid=n_grid, rho=1000.0, dt=1e-4)
+    prm.n_env = n_env
+    dx = np.float32(prm.dx)
+    meshes = []
+    for e in range(n_env):
+        nodes, f2v, tris, layer, _ = slab_mesh(12 + 4 * e, 10, 3, h=float(dx),
+                                               origin=(float(dx) * (9 + e), 0.5, float(dx) * 8))
+        meshes.append((nodes.astype(np.float32), f2v, tris, layer))
+    nodes, _, tris, _, _, tri_start = concat_pad_meshes(meshes)
+    rng = np.random.default_rng(5)
+    npart = 6000
+    x = rng.uniform(0.2, 0.62, size=(npart, 3)).astype(np.float32)
+    env = rng.integers(0, n_env, npart).astype(np.int32)
+    # occupancy: the nodes of the particles' 3x3x3 stencils
+    base = np.clip(om.cell_base_np(x, prm.inv_dx).astype(np.int64), 0, n_grid - 3)
+    base[:, 0] += env.astype(np.int64) * n_grid
+    off = np.stack(np.meshgrid(*[np.arange(3)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    st = (base[:, None, :] + off[None]).reshape(-1, 3)
+    occ = np.unique((st[:, 0] * n_grid + st[:, 1]) * n_grid + st[:, 2])
+    nn = n_env * n_grid ** 3
+    m = np.zeros(nn, np.float32)
+    m[occ] = 1.0
+    c_prm = prm.c_struct()
+    env_t = dev(env, torch.int32)
+    c_prm.env_id = env_t.data_ptr()
+    if node_set:
+        ws, nbytes = _workspace(c_prm, npart)
+        _lib.call("dt_mpm_bin", c_prm, npart, dev(x), ws, nbytes, _lib.stream())
+        a, c = ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(_lib.lib().dt_mpm_active_bricks(c_prm, npart, ws.data_ptr(), nbytes,
+                                                   ctypes.byref(a), ctypes.byref(c)))
+        c_prm.active_bricks, c_prm.n_active_bricks = a.value, c.value
+    cen = torch.empty(len(tris), 4, device="cuda")
+    _lib.call("dt_contact_centroids", len(tris), dev(tris, torch.int32), dev(nodes), P(cen),
+              _lib.stream())
+    idx = torch.full((nn,), -7, dtype=torch.int32, device="cuda")
+    ts = dev(tri_start, torch.int32)
+    if batched:
+        bnd = torch.empty(-(-len(tris) // _lib.TRI_RUN), 4, device="cuda")
+        _lib.call("dt_contact_tri_batches", len(tris), P(cen), P(bnd), _lib.stream())
+        # a hint array of arbitrary entries (in range or not) changes nothing
+        hint = dev(np.random.default_rng(9).integers(-5, len(tris) + 5, nn), torch.int32)
+        _lib.call("dt_contact_closest_batched", c_prm, dev(m), 1, ts, len(tris), P(cen), P(bnd),
+                  P(hint, torch.int32), P(idx, torch.int32), 1, 0, _lib.stream())
+    else:
+        _lib.call("dt_contact_closest_env", c_prm, dev(m), 1, ts, len(tris), P(cen),
+                  P(idx, torch.int32), 1, 0, _lib.stream())
+    got = idx.cpu().numpy()
+    i, j, k = occ // (n_grid * n_grid), (occ // n_grid) % n_grid, occ % n_grid
+    q = (np.stack([i % n_grid, j, k], 1).astype(np.float32) + np.float32(0.5)) * dx
+    ties = 0
+    for e in range(n_env):
+        sel = (i // n_grid) == e
+        t0, t1 = tri_start[e], tri_start[e + 1]
+        ref = oc.find_closest_np(q[sel], nodes, tris[t0:t1])
+        np.testing.assert_array_equal(got[occ[sel]], ref + t0)
+        # count nodes whose best distance is attained more than once
+        cc = (nodes[tris[t0:t1, 0]] + nodes[tris[t0:t1, 1]]) + nodes[tris[t0:t1, 2]]
+        cc = np.float32(1.0 / 3.0) * cc
+        dd = cc[None] - q[sel][:, None]
+        dist = np.sqrt(((dd[..., 0] * dd[..., 0] + dd[..., 1] * dd[..., 1]) +
+                        dd[..., 2] * dd[..., 2]) + np.float32(1e-10))
+        ties += int(((dist == dist.min(1, keepdims=True)).sum(1) > 1).sum())
+    assert ties > 100
     untouched = np.ones(nn, bool)
-    untouched[nodes_o] = False
-    assert (got[untouched, 1] == -7).all() and (got[:, 0] == -7).all()
+    untouched[occ] = False
+    assert (got[untouched] == -7).all()
 
 
 def _build_facade(prm, mesh, pos, x, v, S=3):
