@@ -116,25 +116,38 @@ __global__ __launch_bounds__(kBlock) void k_fem_update(FemP P, int nt,
   if (kOut == 1) tbuf[4ll * t + 3] = make_float4(f3x, f3y, f3z, 0.f);
 }
 
-// out[v] += sum of tbuf[e] over the vertex's incident (tet, corner) entries e,
-// in the order of vert_tets (CSR by vertex)
+// out[v] += sum of tbuf[e] over the vertex's incident (tet, corner) entries e
+// (CSR by vertex).  kVertexLanes lanes share a vertex: lane r sums entries
+// r, r + kVertexLanes, ... in order, then a fixed butterfly adds the lanes'
+// sums, so the summation order is fixed and the short per-lane chains of
+// dependent loads overlap.
+constexpr int kVertexLanes = 8;
 __global__ __launch_bounds__(kBlock) void k_fem_vertex_sum(int nv,
                                                            const int32_t* __restrict__ vstart,
                                                            const int32_t* __restrict__ vtets,
                                                            const float4* __restrict__ tbuf,
                                                            float* __restrict__ out) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = (int)(g / kVertexLanes), r = (int)(g % kVertexLanes);
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  for (int e = vstart[v]; e < vstart[v + 1]; ++e) {
-    const float4 c = tbuf[vtets[e]];
-    sx += c.x;
-    sy += c.y;
-    sz += c.z;
+  if (v < nv) {
+    for (int e = vstart[v] + r; e < vstart[v + 1]; e += kVertexLanes) {
+      const float4 c = tbuf[vtets[e]];
+      sx += c.x;
+      sy += c.y;
+      sz += c.z;
+    }
   }
-  out[3ll * v + 0] += sx;
-  out[3ll * v + 1] += sy;
-  out[3ll * v + 2] += sz;
+  for (int o = kVertexLanes / 2; o > 0; o >>= 1) {
+    sx += __shfl_xor(sx, o);
+    sy += __shfl_xor(sy, o);
+    sz += __shfl_xor(sz, o);
+  }
+  if (v < nv && r == 0) {
+    out[3ll * v + 0] += sx;
+    out[3ll * v + 1] += sy;
+    out[3ll * v + 2] += sz;
+  }
 }
 
 // Adjoint of update: given g_vel (adjoint of vel[f] after the kernel), add
@@ -343,8 +356,9 @@ int dt_fem_update_csr(const dt_fem_params* prm, int32_t n_tets, int32_t n_verts,
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_fem_update<1>, dim3(nblocks(n_tets)), dim3(kBlock), 0, st, fem_dev(prm),
                      n_tets, f2v, B, pos, (float*)nullptr, (float4*)tet_buf);
-  hipLaunchKernelGGL(k_fem_vertex_sum, dim3(nblocks(n_verts)), dim3(kBlock), 0, st, n_verts,
-                     vert_start, vert_tets, (const float4*)tet_buf, vel);
+  hipLaunchKernelGGL(k_fem_vertex_sum, dim3(nblocks((long long)n_verts * kVertexLanes)),
+                     dim3(kBlock), 0, st, n_verts, vert_start, vert_tets, (const float4*)tet_buf,
+                     vel);
   return check_launch("dt_fem_update_csr");
 }
 
@@ -376,8 +390,9 @@ int dt_fem_update_adj_csr(const dt_fem_params* prm, int32_t n_tets, int32_t n_ve
   hipLaunchKernelGGL(k_fem_update_adj<1>, dim3(nblocks(n_tets)), dim3(kBlock), 0, st,
                      fem_dev(prm), n_tets, f2v, B, pos, g_vel, (float*)nullptr,
                      (float4*)tet_buf);
-  hipLaunchKernelGGL(k_fem_vertex_sum, dim3(nblocks(n_verts)), dim3(kBlock), 0, st, n_verts,
-                     vert_start, vert_tets, (const float4*)tet_buf, g_pos);
+  hipLaunchKernelGGL(k_fem_vertex_sum, dim3(nblocks((long long)n_verts * kVertexLanes)),
+                     dim3(kBlock), 0, st, n_verts, vert_start, vert_tets, (const float4*)tet_buf,
+                     g_pos);
   return check_launch("dt_fem_update_adj_csr");
 }
 
