@@ -34,6 +34,15 @@ class DtMpmParams(ctypes.Structure):
     ]
 
 
+class DtJawParams(ctypes.Structure):
+    _fields_ = [("ee_world", ctypes.c_float * 16), ("finger_t", (ctypes.c_float * 16) * 2),
+                ("dt", ctypes.c_float)]
+
+
+# floats per frame of the jaw chain (DT_JAW_FRAME_FLOATS)
+JAW_FRAME_FLOATS = 96
+
+
 class DtMaterial(ctypes.Structure):
     _fields_ = [
         ("mu", ctypes.c_float), ("lam", ctypes.c_float), ("yield_stress", ctypes.c_float),
@@ -123,6 +132,10 @@ _SIGS = {
     "dt_contact_centroids": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
     "dt_contact_closest": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp, _vp, _c_int,
                                     _c_int, _vp]),
+    "dt_jaw_chain": (_c_int, [_P(DtJawParams), _c_int, _vp, _vp, _vp]),
+    "dt_jaw_chain_adj": (_c_int, [_P(DtJawParams), _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_jaw_control_vel": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _c_float, _vp, _vp]),
+    "dt_jaw_control_vel_adj": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_float, _vp, _vp, _vp]),
     "dt_contact_tri_batches": (_c_int, [_c_int, _vp, _vp, _vp]),
     "dt_contact_closest_batched": (_c_int, [_P(DtMpmParams), _vp, _c_int, _vp, _c_int, _vp, _vp,
                                             _vp, _vp, _c_int, _c_int, _vp]),

@@ -520,6 +520,38 @@ int dt_markers_adj(int32_t n_surface, const int32_t* surface_id, const float* po
                    int32_t k_closest, const int32_t* interp_idx, const float* interp_w,
                    const float* g_pred, float* g_surf_loc, float* g_pos, void* stream);
 
+/* ---- parallel-jaw kinematics and pad control velocities ----
+   (sensor_model/gripper_kinematics.py).  A trajectory's frames are six
+   row-major 4x4 matrices each (DT_JAW_FRAME_FLOATS floats): ee_local, ee,
+   finger1_local, finger2_local, finger1_ee, finger2_ee (the fields of
+   Gripper, gripper_kinematics.py:82-93).  Controls are seven floats per step:
+   d_pos xyz, d_ori xyz (degrees per unit time), d_gripper (the d_pos /
+   d_ori / d_gripper fields, 104-107). */
+#define DT_JAW_FRAME_FLOATS 96
+typedef struct dt_jaw_params {
+  float ee_world[16];    /* end effector -> world (init_params, 125-127)      */
+  float finger_t[2][16]; /* end effector -> finger 1 / 2 (139-158)           */
+  float dt;              /* time step of the chain (Gripper.dt)               */
+} dt_jaw_params;
+/* Frames 1..n_steps from frame 0 (filled by the caller, as init_params does)
+   and controls[n_steps][7]: pos_transformation (186-192) then grip_kinematic
+   (169-183) per step. */
+int dt_jaw_chain(const dt_jaw_params* prm, int32_t n_steps, const float* controls, float* frames,
+                 void* stream);
+/* Adjoint of dt_jaw_chain (kinematic_grad, 219-229, for the chain part):
+   consumes g_frames[0..n_steps] (accumulated into, frame 0 last) and
+   accumulates g_controls[n_steps][7]. */
+int dt_jaw_chain_adj(const dt_jaw_params* prm, int32_t n_steps, const float* controls,
+                     const float* frames, float* g_frames, float* g_controls, void* stream);
+/* fem_sensor{1,2}_set_control_vel (194-216) for steps 0..n_steps-1:
+   control_vel[f][i] = ((E[f+1] x_i) - (E[f] x_i))_{0:3} / dt with E the pose of
+   finger 1 or 2 in frames[0..n_steps] and x_i = (init_x[i], 1). */
+int dt_jaw_control_vel(int32_t n_steps, int32_t n_verts, int32_t finger, const float* frames,
+                       const float* init_x, float dt, float* control_vel, void* stream);
+/* Adjoint: g_frames' finger-pose entries of frames 0..n_steps accumulate. */
+int dt_jaw_control_vel_adj(int32_t n_steps, int32_t n_verts, int32_t finger, const float* init_x,
+                           float dt, const float* g_control_vel, float* g_frames, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

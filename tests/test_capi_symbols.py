@@ -61,3 +61,10 @@ def test_binding_constants_match_header():
     src = open(os.path.join(ROOT, "include", "difftactile.h")).read()
     m = re.search(r"#define\s+DT_CONTACT_TRI_RUN\s+(\d+)", src)
     assert m and int(m.group(1)) == _lib.TRI_RUN
+
+
+def test_jaw_frame_size_matches_header():
+    src = open(os.path.join(ROOT, "include", "difftactile.h")).read()
+    m = re.search(r"#define\s+DT_JAW_FRAME_FLOATS\s+(\d+)", src)
+    assert m and int(m.group(1)) == _lib.JAW_FRAME_FLOATS == 6 * 16
+    assert ctypes.sizeof(_lib.DtJawParams) == 4 * (16 + 32 + 1)

@@ -1,0 +1,135 @@
+"""GPU parity of the jaw kinematics (dt_jaw_*) against the oracle, and its
+composition with a coupled rollout: a loss on the pad reaches the jaw
+controls.
+
+Tolerances: frames within 1e-5 of the FP64 oracle (relative to the largest
+entry); control velocities difference two nearly equal float32 poses and
+divide by dt, so they are held to 64 ulp of the transformed positions over dt;
+control adjoints within 1e-4 relative L2 of the oracle's FP64 reverse mode.
+"""
+import numpy as np
+import pytest
+import torch
+
+from difftactile_amd.sensor_model.gripper_kinematics import Gripper
+from oracle import kinematics as ok
+from tests.mpm_cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+D64 = torch.float64
+POSE = ([10.0, -20.0, 30.0], [1.0, 2.0, 3.0], 2.5)
+
+
+def _case(T=24, seed=0, dt=1e-2):
+    rng = np.random.default_rng(seed)
+    g = Gripper(dt=dt)
+    g.init_params(*POSE)
+    c = (rng.normal(size=(T, 7)) * [0.5, 0.5, 0.5, 20, 20, 20, 0.3]).astype(np.float32)
+    x1 = rng.normal(size=(300, 3)).astype(np.float32)
+    x2 = rng.normal(size=(200, 3)).astype(np.float32)
+    return g, c, x1, x2
+
+
+def _oracle(c, dt):
+    W, f1t, f2t, frame0 = ok.init_frames(*POSE)
+    return ok.chain(W, f1t, f2t, frame0, c, dt)
+
+
+def test_chain_and_control_velocities_match_the_oracle(gpu):
+    g, c, x1, x2 = _case()
+    fr = g.frames(torch.tensor(c)).cpu().double()
+    fo = _oracle(torch.tensor(c, dtype=D64), g.dt)
+    for f, mats in enumerate(fo):
+        for m, want in enumerate(mats):
+            err = (fr[f, m] - want).abs().max().item()
+            assert err <= 1e-5 * max(1.0, want.abs().max().item()), (f, m, err)
+    cv = g.control_velocities(torch.tensor(c, device="cuda"), torch.tensor(x1, device="cuda"),
+                              torch.tensor(x2, device="cuda"))
+    for k, x, got in ((1, x1, cv[0]), (2, x2, cv[1])):
+        xo = torch.tensor(x, dtype=D64)
+        want = ok.control_vel(fo, k, xo, g.dt)
+        xh = torch.cat([xo, torch.ones(len(xo), 1, dtype=D64)], 1)
+        scale = max((xh @ fr_m[3 + k].T).abs().max().item() for fr_m in fo)
+        atol = 64 * 2.0 ** -24 * scale / g.dt
+        assert got.shape == want.shape
+        assert (got.cpu().double() - want).abs().max().item() <= atol, k
+
+
+def test_control_velocity_adjoint_matches_the_oracle(gpu):
+    g, c, x1, x2 = _case(seed=1)
+    rng = np.random.default_rng(2)
+    w1 = rng.normal(size=(len(c), len(x1), 3)).astype(np.float32)
+    w2 = rng.normal(size=(len(c), len(x2), 3)).astype(np.float32)
+    cg = torch.tensor(c, device="cuda", requires_grad=True)
+    cv1, cv2 = g.control_velocities(cg, torch.tensor(x1, device="cuda"),
+                                    torch.tensor(x2, device="cuda"))
+    loss = (cv1 * torch.tensor(w1, device="cuda")).sum() + \
+        (cv2 * torch.tensor(w2, device="cuda")).sum()
+    got = torch.autograd.grad(loss, cg)[0].cpu().double()
+    co = torch.tensor(c, dtype=D64, requires_grad=True)
+    fo = _oracle(co, g.dt)
+    lo = (ok.control_vel(fo, 1, torch.tensor(x1, dtype=D64), g.dt) * torch.tensor(w1, dtype=D64)).sum() + \
+        (ok.control_vel(fo, 2, torch.tensor(x2, dtype=D64), g.dt) * torch.tensor(w2, dtype=D64)).sum()
+    want = torch.autograd.grad(lo, co)[0]
+    assert rel_l2(got, want) < 1e-4
+
+
+def test_jaw_controls_drive_a_coupled_rollout_differentiably(gpu):
+    """Finger 1's control velocities feed a pad pressing an MPM block; the
+    gradient of a loss on the rollout w.r.t. the jaw controls equals the
+    oracle's kinematic adjoint applied to the rollout's gradient w.r.t. the
+    control velocities."""
+    from difftactile_amd.coupled import CoupledSim, Pad
+    from difftactile_amd.params import Material, MPMParams
+    from difftactile_amd.sensor_model.meshes import slab_mesh
+    from difftactile_amd.sim import MPMSim
+    prm = MPMParams(n_grid=64, rho=1000.0, dt=2e-5)
+    steps = 5
+    h = 1.0 / 128
+    nodes, f2v, surf, layer, od = slab_mesh(16, 16, 4, h=h, origin=(0.42, 0.52, 0.42))
+    rng = np.random.default_rng(7)
+    x = rng.uniform([0.42, 0.44, 0.42], [0.54, 0.53, 0.54], size=(3000, 3)).astype(np.float32)
+    v = rng.normal(0, 0.2, size=(3000, 3)).astype(np.float32)
+    # a gripper whose finger 1 carries the pad at its world position
+    grip = Gripper(dt=prm.dt)
+    grip.init_params([0.0, 0.0, 0.0], [0.0, 0.0, 0.0], 3.0)
+    P1 = grip.finger_pose(1).astype(np.float64)
+    nh = np.concatenate([nodes, np.ones((len(nodes), 1))], 1)
+    init_x1 = (nh @ np.linalg.inv(P1).T)[:, :3].astype(np.float32)
+    init_x2 = init_x1[:8].copy()
+    c = np.zeros((steps + 1, 7), np.float32)
+    c[:, 1] = -0.5 + 0.05 * rng.normal(size=steps + 1)   # press down along world y
+    c[:, 5] = 2.0 * rng.normal(size=steps + 1)            # a little yaw
+    mu, lam, mass = 1e5 / 2 / 1.45, 1e5 * 0.45 / 1.45 / 0.1, 2e-4
+    contact = dict(kn=1e3, kd=10.0, kt=50.0, mu_f=0.5)
+
+    def rollout(cv1):
+        sim = MPMSim(prm, [Material(5e4, 0.3)], len(x), checkpoint_every=3, sort=True)
+        pad = Pad(nodes.astype(np.float32), f2v, surf, layer, od, mu, lam, mass, prm.dt)
+        cs = CoupledSim(sim, [pad], **contact)
+        st = [torch.tensor(x).cuda(), torch.tensor(v).cuda(), torch.zeros(len(x), 3, 3).cuda(),
+              torch.eye(3).repeat(len(x), 1, 1).cuda()]
+        pos = torch.tensor(nodes.astype(np.float32)).cuda()
+        pst = (pos.clone(), cv1[0].detach().clone(), pos.clone())
+        out = cs.rollout(*st, [pst], [cv1], steps)
+        wx = torch.tensor(rng_w.normal(size=(len(x), 3)).astype(np.float32)).cuda()
+        wp = torch.tensor(rng_w.normal(size=(len(nodes), 3)).astype(np.float32)).cuda()
+        return (out[0] * wx).sum() + (out[4][0][0] * wp).sum()
+
+    cg = torch.tensor(c, device="cuda", requires_grad=True)
+    cv1, _ = grip.control_velocities(cg, torch.tensor(init_x1, device="cuda"),
+                                     torch.tensor(init_x2, device="cuda"))
+    assert (cv1[:, :, 1].mean() + 0.5).abs() < 0.1
+    rng_w = np.random.default_rng(11)
+    got = torch.autograd.grad(rollout(cv1), cg)[0].cpu().double()
+    # the rollout's gradient w.r.t. the velocities, then the oracle's chain
+    cv_leaf = cv1.detach().clone().requires_grad_(True)
+    rng_w = np.random.default_rng(11)
+    g_cv = torch.autograd.grad(rollout(cv_leaf), cv_leaf)[0].cpu().double()
+    assert g_cv.abs().max() > 0
+    co = torch.tensor(c, dtype=D64, requires_grad=True)
+    W, f1t, f2t, frame0 = ok.init_frames([0.0, 0.0, 0.0], [0.0, 0.0, 0.0], 3.0)
+    fo = ok.chain(W, f1t, f2t, frame0, co, prm.dt)
+    cvo = ok.control_vel(fo, 1, torch.tensor(init_x1, dtype=D64), prm.dt)
+    want = torch.autograd.grad(cvo, co, grad_outputs=g_cv)[0]
+    assert rel_l2(got, want) < 1e-3
