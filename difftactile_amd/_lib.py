@@ -41,6 +41,8 @@ class DtJawParams(ctypes.Structure):
 
 # floats per frame of the jaw chain (DT_JAW_FRAME_FLOATS)
 JAW_FRAME_FLOATS = 96
+# blocks of the fixed-order reductions (DT_REDUCE_BLOCKS)
+REDUCE_BLOCKS = 256
 
 
 class DtMaterial(ctypes.Structure):
@@ -136,6 +138,13 @@ _SIGS = {
     "dt_jaw_chain_adj": (_c_int, [_P(DtJawParams), _c_int, _vp, _vp, _vp, _vp, _vp]),
     "dt_jaw_control_vel": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _c_float, _vp, _vp]),
     "dt_jaw_control_vel_adj": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_float, _vp, _vp, _vp]),
+    "dt_mpm_grid_mass": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp]),
+    "dt_mpm_grid_mass_adj": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp, _vp]),
+    "dt_deform_target_sdf": (_c_int, [_c_int, _c_float, _vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_deform_loss": (_c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dt_deform_loss_adj": (_c_int, [ctypes.c_int64, _vp, _vp, _vp, _c_float, _c_float, _vp, _vp,
+                                    _vp]),
+    "dt_grid_iou_terms": (_c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),
     "dt_contact_tri_batches": (_c_int, [_c_int, _vp, _vp, _vp]),
     "dt_contact_closest_batched": (_c_int, [_P(DtMpmParams), _vp, _c_int, _vp, _c_int, _vp, _vp,
                                             _vp, _vp, _c_int, _c_int, _vp]),

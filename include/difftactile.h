@@ -552,6 +552,40 @@ int dt_jaw_control_vel(int32_t n_steps, int32_t n_verts, int32_t finger, const f
 int dt_jaw_control_vel_adj(int32_t n_steps, int32_t n_verts, int32_t finger, const float* init_x,
                            float dt, const float* g_control_vel, float* g_frames, void* stream);
 
+/* ---- grid-mass deformation losses (tasks/deform_loss.py) ---- */
+/* Mass-only p2g (compute_grid_m_kernel, mpm_plastic.py:168-178): grid_m
+   [n^3] += sum_p w(node) p_mass over x [n][3]; one environment. */
+int dt_mpm_grid_mass(const dt_mpm_params* prm, int32_t n, const float* x, float* grid_m,
+                     void* stream);
+/* Its adjoint w.r.t. x (base held fixed): gx [n][3] accumulates. */
+int dt_mpm_grid_mass_adj(const dt_mpm_params* prm, int32_t n, const float* x,
+                         const float* g_grid_m, float* gx, void* stream);
+/* update_target (deform_loss.py:61-86): n_sweeps sweeps (the reference runs
+   2 n_grid) of the nearest-point distance propagation from the nodes whose
+   target density exceeds 1e-4; sdf [n^3] and nearest [n^3][3] receive the
+   result, sdf_tmp / nearest_tmp are scratch of the same sizes.  Nodes at
+   i dx; distance sqrt(|d|^2 + 1e-8); 1000 where no point has arrived. */
+int dt_deform_target_sdf(int32_t n_grid, float dx, const float* density, int32_t n_sweeps,
+                         float* sdf, float* nearest, float* sdf_tmp, float* nearest_tmp,
+                         void* stream);
+/* Sum of block partials: reductions use a caller scratch of
+   DT_REDUCE_BLOCKS x (number of terms) floats and a fixed order. */
+#define DT_REDUCE_BLOCKS 256
+/* out2 = (sum |m - t|, sum sdf m) (compute_density_loss_kernel,
+   compute_sdf_loss_kernel, deform_loss.py:99-107); scratch 2 DT_REDUCE_BLOCKS. */
+int dt_deform_loss(int64_t n_nodes, const float* grid_m, const float* target_density,
+                   const float* target_sdf, float* scratch, float* out2, void* stream);
+/* g_grid_m += g_loss (w_density sign(m - t) + w_sdf sdf), sign(0) = 0;
+   g_loss is a device scalar (sum_up_loss_kernel.grad, deform_loss.py:108-111). */
+int dt_deform_loss_adj(int64_t n_nodes, const float* grid_m, const float* target_density,
+                       const float* target_sdf, float w_density, float w_sdf, const float* g_loss,
+                       float* g_grid_m, void* stream);
+/* out5 = (max m, max t, sum m t, sum m, sum t), the terms of iou
+   (deform_loss.py:139-153: I = sum m t / max m / max t, U = sum m / max m +
+   sum t / max t, iou = I / (U - I)); scratch 5 DT_REDUCE_BLOCKS. */
+int dt_grid_iou_terms(int64_t n_nodes, const float* grid_m, const float* target_density,
+                      float* scratch, float* out5, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
