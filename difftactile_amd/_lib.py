@@ -43,6 +43,8 @@ class DtJawParams(ctypes.Structure):
 JAW_FRAME_FLOATS = 96
 # blocks of the fixed-order reductions (DT_REDUCE_BLOCKS)
 REDUCE_BLOCKS = 256
+# floats of the Kabsch-angle state (DT_KABSCH_STATE); the angle is entry 47
+KABSCH_STATE = 64
 
 
 class DtMaterial(ctypes.Structure):
@@ -145,6 +147,8 @@ _SIGS = {
     "dt_deform_loss_adj": (_c_int, [ctypes.c_int64, _vp, _vp, _vp, _c_float, _c_float, _vp, _vp,
                                     _vp]),
     "dt_grid_iou_terms": (_c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),
+    "dt_kabsch_angle": (_c_int, [_c_int, _vp, _vp, _c_float, _vp, _vp, _vp, _vp]),
+    "dt_kabsch_angle_adj": (_c_int, [_c_int, _vp, _vp, _c_float, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dt_contact_tri_batches": (_c_int, [_c_int, _vp, _vp, _vp]),
     "dt_contact_closest_batched": (_c_int, [_P(DtMpmParams), _vp, _c_int, _vp, _c_int, _vp, _vp,
                                             _vp, _vp, _c_int, _c_int, _vp]),

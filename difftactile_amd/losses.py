@@ -5,6 +5,9 @@
   (``extract_markers``, ``sensor_model/fem_sensor.py:161-187`` with the fisheye
   projection of ``fisheye_model.py:13-30``) and the summed squared marker
   displacement between the deformed and the rigidly moved ("virtual") pad.
+* ``kabsch_angle``: rotation angle of the masked particles over one substep
+  (``compute_COM`` / ``compute_H`` / ``compute_R``, ``object_model/multi_obj.py:309-340``,
+  one term of ``compute_angle``, ``tasks/box_open.py:400-406``).
 """
 import ctypes
 
@@ -103,3 +106,33 @@ class _MarkerDisp(torch.autograd.Function):
 def marker_displacement(model, pos, vpos):
     """Sum over markers of |pred - virtual|^2 (differentiable in pos and vpos)."""
     return _MarkerDisp.apply(model, pos, vpos)
+
+
+class _Kabsch(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, v_next, mask, dt):
+        x, v = x.detach().float().contiguous(), v_next.detach().float().contiguous()
+        m = mask.to(device=x.device, dtype=torch.int32).contiguous()
+        state = torch.zeros(_lib.KABSCH_STATE, device=x.device)
+        scratch = torch.empty(9 * _lib.REDUCE_BLOCKS, device=x.device)
+        _lib.call("dt_kabsch_angle", x.shape[0], x, v, float(dt), m, scratch, state, _lib.stream())
+        ctx.dt = float(dt)
+        ctx.save_for_backward(x, v, m, state)
+        return state[47].clone()
+
+    @staticmethod
+    def backward(ctx, g):
+        x, v, m, state = ctx.saved_tensors
+        st = state.clone()
+        gx, gv = torch.zeros_like(x), torch.zeros_like(v)
+        gg = g.detach().float().reshape(1).contiguous()
+        _lib.call("dt_kabsch_angle_adj", x.shape[0], x, v, ctx.dt, m, st, gg, gx, gv, _lib.stream())
+        return gx, gv, None, None
+
+
+def kabsch_angle(x, v_next, mask, dt):
+    """Rotation angle (radians, about z) of the masked particles between x and
+    x + dt v_next; x, v_next [n][3], mask [n] (nonzero = counted).  The
+    centres divide by all n particles, as the reference does.  Differentiable
+    w.r.t. x and v_next."""
+    return _Kabsch.apply(x, v_next, mask, dt)

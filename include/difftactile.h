@@ -586,6 +586,23 @@ int dt_deform_loss_adj(int64_t n_nodes, const float* grid_m, const float* target
 int dt_grid_iou_terms(int64_t n_nodes, const float* grid_m, const float* target_density,
                       float* scratch, float* out5, void* stream);
 
+/* ---- Kabsch rotation angle of the masked particles ----
+   (compute_COM / compute_H / compute_H_svd / compute_R, multi_obj.py:309-340,
+   and one term of compute_angle, box_open.py:400-406).  y = x + dt v_next;
+   c0 = sum_masked x / n, c1 = sum_masked y / n (all n particles, as the
+   reference divides); H = sum_masked (x - c0)(y - c1)^T = U diag(s) V^T;
+   R = V U^T; angle = atan2(R_10, R_11) in state[47].  scratch: 9
+   DT_REDUCE_BLOCKS floats; state: DT_KABSCH_STATE floats, also read by the
+   adjoint.  The adjoint differentiates R exactly through the polar rotation
+   of H (the reference's H_svd_grad, multi_obj.py:209-224, clamps 1/(s_j^2 -
+   s_i^2) instead); gx and gv_next accumulate, g_angle is a device scalar. */
+#define DT_KABSCH_STATE 64
+int dt_kabsch_angle(int32_t n, const float* x, const float* v_next, float dt, const int32_t* mask,
+                    float* scratch, float* state, void* stream);
+int dt_kabsch_angle_adj(int32_t n, const float* x, const float* v_next, float dt,
+                        const int32_t* mask, float* state, const float* g_angle, float* gx,
+                        float* gv_next, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,8 @@ Pinning (tests/test_oracle_deform.py):
                  the target; FP64 central differences through grid_mass.
   iou         -- 1 for identical grids, I / (U - I) of two disjoint-free
                  boxes by counting.
+  kabsch_angle (multi_obj.py:309-340, box_open.py:400-406) -- see its
+                 docstring.
 """
 import numpy as np
 import torch
@@ -122,3 +124,27 @@ def iou(m, target):
     inter = (m * t).sum() / m.max() / t.max()
     union = m.sum() / m.max() + t.sum() / t.max()
     return inter / (union - inter)
+
+
+def kabsch_angle(x, v_next, mask, dt):
+    """Rotation angle of the masked particles over one substep
+    (``multi_obj.py:309-340``, ``box_open.py:400-406``): c0, c1 are the masked
+    sums of x and y = x + dt v_next divided by all n particles (the
+    reference's quirk), H = sum_masked (x - c0)(y - c1)^T, R = V U^T of its
+    SVD (the transposed polar rotation, exact adjoint), angle = atan2(R_10, R_11).
+
+    Pinned (tests/test_oracle_deform.py): a rigid turn by theta about z gives
+    theta, for every particle masked with a translation, and for a partial
+    mask without one (then H = A R^T with A symmetric positive definite);
+    FP64 central differences of the angle w.r.t. x and v_next."""
+    from .mpm import _Polar, svd3_np
+    n = x.shape[0]
+    msk = torch.as_tensor(mask, dtype=torch.bool)
+    y = x + dt * v_next
+    c0 = x[msk].sum(0) / n
+    c1 = y[msk].sum(0) / n
+    H = (x[msk] - c0).T @ (y[msk] - c1)
+    U, s, V = (torch.as_tensor(a, dtype=x.dtype) for a in svd3_np(H.detach().numpy()))
+    Q = _Polar.apply(H, U, s, V)
+    R = Q.T
+    return torch.atan2(R[1, 0], R[1, 1])

@@ -92,3 +92,48 @@ def test_losses_and_iou_closed_forms():
     b[2:6] = 1.0
     inter, union = 2 * 64, 4 * 64 + 4 * 64 - 2 * 64
     assert abs(od.iou(a.reshape(-1), b.reshape(-1)) - inter / union) < 1e-12
+
+
+def _turn(theta_deg, n=300, seed=4):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.3, 0.7, size=(n, 3))
+    th = np.radians(theta_deg)
+    Rz = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    return x, Rz
+
+
+def test_kabsch_angle_of_a_rigid_turn():
+    dt = 1e-2
+    x, Rz = _turn(3.0)
+    # every particle masked, turned about the origin and translated
+    y = x @ Rz.T + [0.01, -0.02, 0.005]
+    v = (y - x) / dt
+    a = od.kabsch_angle(torch.tensor(x), torch.tensor(v), np.ones(len(x), bool), dt)
+    assert abs(a.item() - np.radians(3.0)) < 1e-10
+    # a partial mask and no translation: H = A R^T with A SPD
+    mask = np.arange(len(x)) % 3 == 0
+    y = x @ Rz.T
+    v = (y - x) / dt
+    a = od.kabsch_angle(torch.tensor(x), torch.tensor(v), mask, dt)
+    assert abs(a.item() - np.radians(3.0)) < 1e-10
+
+
+def test_kabsch_angle_gradient_matches_central_differences():
+    dt = 1e-2
+    x, Rz = _turn(2.0, n=60, seed=5)
+    rng = np.random.default_rng(6)
+    y = x @ Rz.T + 0.003 * rng.normal(size=x.shape)
+    v = (y - x) / dt
+    mask = rng.random(len(x)) < 0.7
+    xt = torch.tensor(x, requires_grad=True)
+    vt = torch.tensor(v, requires_grad=True)
+    gx, gv = torch.autograd.grad(od.kabsch_angle(xt, vt, mask, dt), [xt, vt])
+    h = 1e-6
+    for (p, a) in [(0, 0), (5, 1), (17, 2), (40, 0)]:
+        for arr, g, is_x in ((x, gx, True), (v, gv, False)):
+            e = np.zeros_like(arr)
+            e[p, a] = h
+            args_p = (torch.tensor(x + e), torch.tensor(v)) if is_x else (torch.tensor(x), torch.tensor(v + e))
+            args_m = (torch.tensor(x - e), torch.tensor(v)) if is_x else (torch.tensor(x), torch.tensor(v - e))
+            fd = (od.kabsch_angle(*args_p, mask, dt) - od.kabsch_angle(*args_m, mask, dt)) / (2 * h)
+            assert abs(fd.item() - g[p, a].item()) < 1e-6 * max(1.0, abs(fd.item())), (p, a, is_x)
