@@ -47,6 +47,15 @@ REDUCE_BLOCKS = 256
 KABSCH_STATE = 64
 
 
+class DtRopeParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_float), ("p_mass", ctypes.c_float),
+                ("table_height", ctypes.c_float), ("floor_friction", ctypes.c_float),
+                ("gravity", ctypes.c_float * 3), ("rest_length", ctypes.c_float),
+                ("bending_length", ctypes.c_float), ("stretch_compliance", ctypes.c_float),
+                ("bending_compliance", ctypes.c_float), ("stretch_relaxation", ctypes.c_float),
+                ("bending_relaxation", ctypes.c_float)]
+
+
 class DtMaterial(ctypes.Structure):
     _fields_ = [
         ("mu", ctypes.c_float), ("lam", ctypes.c_float), ("yield_stress", ctypes.c_float),
@@ -149,6 +158,14 @@ _SIGS = {
     "dt_grid_iou_terms": (_c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),
     "dt_kabsch_angle": (_c_int, [_c_int, _vp, _vp, _c_float, _vp, _vp, _vp, _vp]),
     "dt_kabsch_angle_adj": (_c_int, [_c_int, _vp, _vp, _c_float, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dt_rope_step": (_c_int, [_P(DtRopeParams), _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dt_rope_step_adj": (_c_int, [_P(DtRopeParams), _c_int] + [_vp] * 10 + [_vp]),
+    "dt_rope_closest": (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp]),
+    "dt_rope_collision": (_c_int, [_P(DtContactParams), _c_float, _c_int, _vp, _vp, _vp, _c_int,
+                                   _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dt_rope_collision_adj": (_c_int, [_P(DtContactParams), _c_float, _c_int, _vp, _vp, _vp,
+                                       _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                       _vp, _vp, _vp, _vp, _vp]),
     "dt_contact_tri_batches": (_c_int, [_c_int, _vp, _vp, _vp]),
     "dt_contact_closest_batched": (_c_int, [_P(DtMpmParams), _vp, _c_int, _vp, _c_int, _vp, _vp,
                                             _vp, _vp, _c_int, _c_int, _vp]),

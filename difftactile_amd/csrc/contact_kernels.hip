@@ -416,6 +416,85 @@ __global__ __launch_bounds__(kBlock) void k_collision(MpmP P, ContactP C, long l
   });
 }
 
+// Adjoint of one contact force given gf = dL/df: accumulates the pad's
+// vertex position and velocity adjoints (atomics) and returns the adjoints of
+// the query point q and its velocity qv.
+__device__ __forceinline__ void contact_adj_core(const ContactP& C, const float (&q)[3],
+                                                 const ContactFwd& w, const float (&gf)[3],
+                                                 const int (&tv)[3], float* __restrict__ g_pos,
+                                                 float* __restrict__ g_vel, float (&gq)[3],
+                                                 float (&gqv)[3]) {
+  float gfn[3] = {gf[0], gf[1], gf[2]};
+  float gvt[3] = {0.f, 0.f, 0.f};
+  if (w.shear) {
+    // ft = vt * (mag / vtn)
+    float s = gf[0] * w.vt[0] + gf[1] * w.vt[1] + gf[2] * w.vt[2];
+    float ratio = w.mag / w.vtn;
+    for (int a = 0; a < 3; ++a) gvt[a] += ratio * gf[a];
+    float gmag = s / w.vtn;
+    float gvtn = -s * w.mag / (w.vtn * w.vtn);
+    if (w.kt_branch) {
+      gvtn += C.kt * gmag;
+    } else if (w.fnn > 0.f) {
+      float gfnn = C.mu_f * gmag;
+      for (int a = 0; a < 3; ++a) gfn[a] += gfnn * w.fn[a] / w.fnn;
+    }
+    for (int a = 0; a < 3; ++a) gvt[a] += gvtn * w.vt[a] / w.vtn;
+  }
+  // vt = r - vnr n
+  float gr[3], gn[3], gvnr = 0.f;
+  for (int a = 0; a < 3; ++a) {
+    gr[a] = gvt[a];
+    gvnr -= gvt[a] * w.n[a];
+    gn[a] = -w.vnr * gvt[a];
+  }
+  // fn = -(kn + kd vn) sdf n
+  float kk2 = -(C.kn + C.kd * w.vn) * w.sdf;
+  float gfn_n = gfn[0] * w.n[0] + gfn[1] * w.n[1] + gfn[2] * w.n[2];
+  for (int a = 0; a < 3; ++a) gn[a] += kk2 * gfn[a];
+  float gsdf = -(C.kn + C.kd * w.vn) * gfn_n;
+  float gvn = -C.kd * w.sdf * gfn_n;
+  if (w.vnr > 0.f) gvnr += gvn;
+  // vnr = n . r
+  for (int a = 0; a < 3; ++a) {
+    gn[a] += gvnr * w.r[a];
+    gr[a] += gvnr * w.n[a];
+  }
+  // r = -qv + (1/3) sum vel
+  for (int a = 0; a < 3; ++a) {
+    gqv[a] = -gr[a];
+    float gvv = gr[a] / 3.f;
+    atomic_add_f(g_vel + 3ll * tv[0] + a, gvv);
+    atomic_add_f(g_vel + 3ll * tv[1] + a, gvv);
+    atomic_add_f(g_vel + 3ll * tv[2] + a, gvv);
+  }
+  // sdf = (q - p1) . n - offset
+  float gp1[3], ge1[3], ge2[3];
+  for (int a = 0; a < 3; ++a) {
+    gn[a] += gsdf * (q[a] - w.p1[a]);
+    gp1[a] = -gsdf * w.n[a];
+    gq[a] = gsdf * w.n[a];
+  }
+  // n = sgn * cr / s
+  float gnr[3] = {w.sgn * gn[0], w.sgn * gn[1], w.sgn * gn[2]};
+  float dotc = gnr[0] * w.cr[0] + gnr[1] * w.cr[1] + gnr[2] * w.cr[2];
+  float s3 = w.s * w.s * w.s;
+  float gcr[3];
+  for (int a = 0; a < 3; ++a) gcr[a] = gnr[a] / w.s - w.cr[a] * dotc / s3;
+  // cr = e1 x e2: g_e1 = e2 x g_cr, g_e2 = g_cr x e1
+  ge1[0] = w.e2[1] * gcr[2] - w.e2[2] * gcr[1];
+  ge1[1] = w.e2[2] * gcr[0] - w.e2[0] * gcr[2];
+  ge1[2] = w.e2[0] * gcr[1] - w.e2[1] * gcr[0];
+  ge2[0] = gcr[1] * w.e1[2] - gcr[2] * w.e1[1];
+  ge2[1] = gcr[2] * w.e1[0] - gcr[0] * w.e1[2];
+  ge2[2] = gcr[0] * w.e1[1] - gcr[1] * w.e1[0];
+  for (int a = 0; a < 3; ++a) {
+    atomic_add_f(g_pos + 3ll * tv[0] + a, gp1[a] - ge1[a] - ge2[a]);
+    atomic_add_f(g_pos + 3ll * tv[1] + a, ge1[a]);
+    atomic_add_f(g_pos + 3ll * tv[2] + a, ge2[a]);
+  }
+}
+
 // Adjoint of k_collision.  Inputs: g_gridf (adjoint of grid_f), g_ext (adjoint
 // of the pad's external force).  Accumulates into g_mv, g_m (grid, strided),
 // g_pos, g_vel (pad, atomics).
@@ -447,79 +526,93 @@ __global__ __launch_bounds__(kBlock) void k_collision_adj(
     for (int a = 0; a < 3; ++a)
       gf[a] = g_gridf[3 * nd + a] * P.dt -
               (g_ext[3ll * tv[0] + a] + g_ext[3ll * tv[1] + a] + g_ext[3ll * tv[2] + a]) / 3.f;
-    float gfn[3] = {gf[0], gf[1], gf[2]};
-    float gvt[3] = {0.f, 0.f, 0.f};
-    if (w.shear) {
-      // ft = vt * (mag / vtn)
-      float s = gf[0] * w.vt[0] + gf[1] * w.vt[1] + gf[2] * w.vt[2];
-      float ratio = w.mag / w.vtn;
-      for (int a = 0; a < 3; ++a) gvt[a] += ratio * gf[a];
-      float gmag = s / w.vtn;
-      float gvtn = -s * w.mag / (w.vtn * w.vtn);
-      if (w.kt_branch) {
-        gvtn += C.kt * gmag;
-      } else if (w.fnn > 0.f) {
-        float gfnn = C.mu_f * gmag;
-        for (int a = 0; a < 3; ++a) gfn[a] += gfnn * w.fn[a] / w.fnn;
-      }
-      for (int a = 0; a < 3; ++a) gvt[a] += gvtn * w.vt[a] / w.vtn;
-    }
-    // vt = r - vnr n
-    float gr[3], gn[3], gvnr = 0.f;
-    for (int a = 0; a < 3; ++a) {
-      gr[a] = gvt[a];
-      gvnr -= gvt[a] * w.n[a];
-      gn[a] = -w.vnr * gvt[a];
-    }
-    // fn = -(kn + kd vn) sdf n
-    float kk2 = -(C.kn + C.kd * w.vn) * w.sdf;
-    float gfn_n = gfn[0] * w.n[0] + gfn[1] * w.n[1] + gfn[2] * w.n[2];
-    for (int a = 0; a < 3; ++a) gn[a] += kk2 * gfn[a];
-    float gsdf = -(C.kn + C.kd * w.vn) * gfn_n;
-    float gvn = -C.kd * w.sdf * gfn_n;
-    if (w.vnr > 0.f) gvnr += gvn;
-    // vnr = n . r
-    for (int a = 0; a < 3; ++a) {
-      gn[a] += gvnr * w.r[a];
-      gr[a] += gvnr * w.n[a];
-    }
-    // r = -qv + (1/3) sum vel;  qv = mv / (m + eps)
-    float gqv[3] = {-gr[0], -gr[1], -gr[2]};
+    float gq[3], gqv[3];
+    contact_adj_core(C, q, w, gf, tv, g_pos, g_vel, gq, gqv);
+    // qv = mv / (m + eps); q is the fixed cell centre
     float gmm = 0.f;
     for (int a = 0; a < 3; ++a) {
       atomic_add_f(g_mv + nd * sgmv + a, gqv[a] * inv);
       gmm -= gqv[a] * mvv[a] * inv * inv;
-      float gvv = gr[a] / 3.f;
-      atomic_add_f(g_vel + 3ll * tv[0] + a, gvv);
-      atomic_add_f(g_vel + 3ll * tv[1] + a, gvv);
-      atomic_add_f(g_vel + 3ll * tv[2] + a, gvv);
     }
     atomic_add_f(g_m + nd * sgm, gmm);
-    // sdf = (q - p1) . n - offset
-    float gp1[3], ge1[3], ge2[3];
-    for (int a = 0; a < 3; ++a) {
-      gn[a] += gsdf * (q[a] - w.p1[a]);
-      gp1[a] = -gsdf * w.n[a];
-    }
-    // n = sgn * cr / s
-    float gnr[3] = {w.sgn * gn[0], w.sgn * gn[1], w.sgn * gn[2]};
-    float dotc = gnr[0] * w.cr[0] + gnr[1] * w.cr[1] + gnr[2] * w.cr[2];
-    float s3 = w.s * w.s * w.s;
-    float gcr[3];
-    for (int a = 0; a < 3; ++a) gcr[a] = gnr[a] / w.s - w.cr[a] * dotc / s3;
-    // cr = e1 x e2: g_e1 = e2 x g_cr, g_e2 = g_cr x e1
-    ge1[0] = w.e2[1] * gcr[2] - w.e2[2] * gcr[1];
-    ge1[1] = w.e2[2] * gcr[0] - w.e2[0] * gcr[2];
-    ge1[2] = w.e2[0] * gcr[1] - w.e2[1] * gcr[0];
-    ge2[0] = gcr[1] * w.e1[2] - gcr[2] * w.e1[1];
-    ge2[1] = gcr[2] * w.e1[0] - gcr[0] * w.e1[2];
-    ge2[2] = gcr[0] * w.e1[1] - gcr[1] * w.e1[0];
-    for (int a = 0; a < 3; ++a) {
-      atomic_add_f(g_pos + 3ll * tv[0] + a, gp1[a] - ge1[a] - ge2[a]);
-      atomic_add_f(g_pos + 3ll * tv[1] + a, ge1[a]);
-      atomic_add_f(g_pos + 3ll * tv[2] + a, ge2[a]);
-    }
   });
+}
+
+// ---------------------------------------------------------------------------
+// particle-wise contact of a rope with one pad (cable_straightening.py:287-319)
+// ---------------------------------------------------------------------------
+// find_closest with the particle position as the query (strict < from
+// (100, -1) in index order)
+__global__ __launch_bounds__(kBlock) void k_rope_closest(int n, const float* __restrict__ x,
+                                                         int nt, const float4* __restrict__ cen,
+                                                         int32_t* __restrict__ idx, int stride,
+                                                         int col) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p < n;
+  const float qx = live ? x[3 * p] : 0.f, qy = live ? x[3 * p + 1] : 0.f,
+              qz = live ? x[3 * p + 2] : 0.f;
+  Closest r{100.0f, 1e4f, 1e4f * (1.0f + 1e-6f), -1};
+  for (int t = 0; t < nt; ++t) closest_step<false>(cen[t], t, 0, nt, qx, qy, qz, r);
+  if (live) idx[(long long)p * stride + col] = r.best_k;
+}
+
+// rope impulse += f dt, pad external force += -f/3 per triangle vertex,
+// flag[p] = 1 where the particle touches this pad (C.sdf_offset: the radius)
+__global__ __launch_bounds__(kBlock) void k_rope_collision(
+    ContactP C, float dt, int n, const float* __restrict__ x, const float* __restrict__ v,
+    const int32_t* __restrict__ idx, int stride, int col, const int32_t* __restrict__ tris,
+    const float* __restrict__ pos, const float* __restrict__ vel, float o0, float o1, float o2,
+    float* __restrict__ impulse, float* __restrict__ ext, int32_t* __restrict__ flag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int k = idx[(long long)p * stride + col];
+  if (k < 0) return;
+  const float q[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+  const float qv[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
+  const int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
+  const float od[3] = {o0, o1, o2};
+  ContactFwd w;
+  contact_fwd(C, q, qv, pos, vel, tv, od, w);
+  if (!w.flag) return;
+  for (int a = 0; a < 3; ++a) {
+    impulse[3 * p + a] += w.f[a] * dt;
+    const float e = -w.f[a] / 3.f;
+    atomic_add_f(ext + 3ll * tv[0] + a, e);
+    atomic_add_f(ext + 3ll * tv[1] + a, e);
+    atomic_add_f(ext + 3ll * tv[2] + a, e);
+  }
+  if (flag) flag[p] = 1;
+}
+
+// adjoint: g_x, g_v (rope) and g_pos, g_vel (pad) accumulate
+__global__ __launch_bounds__(kBlock) void k_rope_collision_adj(
+    ContactP C, float dt, int n, const float* __restrict__ x, const float* __restrict__ v,
+    const int32_t* __restrict__ idx, int stride, int col, const int32_t* __restrict__ tris,
+    const float* __restrict__ pos, const float* __restrict__ vel, float o0, float o1, float o2,
+    const float* __restrict__ g_impulse, const float* __restrict__ g_ext,
+    float* __restrict__ g_x, float* __restrict__ g_v, float* __restrict__ g_pos,
+    float* __restrict__ g_vel) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int k = idx[(long long)p * stride + col];
+  if (k < 0) return;
+  const float q[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+  const float qv[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
+  const int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
+  const float od[3] = {o0, o1, o2};
+  ContactFwd w;
+  contact_fwd(C, q, qv, pos, vel, tv, od, w);
+  if (!w.flag) return;
+  float gf[3];
+  for (int a = 0; a < 3; ++a)
+    gf[a] = g_impulse[3 * p + a] * dt -
+            (g_ext[3ll * tv[0] + a] + g_ext[3ll * tv[1] + a] + g_ext[3ll * tv[2] + a]) / 3.f;
+  float gq[3], gqv[3];
+  contact_adj_core(C, q, w, gf, tv, g_pos, g_vel, gq, gqv);
+  for (int a = 0; a < 3; ++a) {
+    g_x[3 * p + a] += gq[a];
+    g_v[3 * p + a] += gqv[a];
+  }
 }
 
 static ContactP contact_dev(const dt_contact_params* c) {
@@ -665,6 +758,55 @@ int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* 
                      out_dir[2], g_grid_f, g_ext_force, g_grid_mv, gmv_stride, g_grid_m,
                      gm_stride, g_pos, g_vel);
   return check_launch("dt_contact_collision_adj");
+}
+
+int dt_rope_closest(int32_t n, const float* x, int32_t n_tris, const float* centroids4,
+                    int32_t* contact_idx, int32_t idx_stride, int32_t idx_col, void* stream) {
+  DT_REQUIRE(n >= 0 && n_tris >= 0 && idx_stride > 0 && idx_col >= 0 && idx_col < idx_stride,
+             "bad size/stride");
+  if (n == 0) return DT_OK;
+  DT_REQUIRE(x && contact_idx && (n_tris == 0 || centroids4), "NULL pointer");
+  hipLaunchKernelGGL(k_rope_closest, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                     (int)n, x, (int)n_tris, (const float4*)centroids4, contact_idx, idx_stride,
+                     idx_col);
+  return check_launch("dt_rope_closest");
+}
+
+int dt_rope_collision(const dt_contact_params* cp, float dt, int32_t n, const float* x,
+                      const float* v, const int32_t* contact_idx, int32_t idx_stride,
+                      int32_t idx_col, int32_t n_tris, const int32_t* tris, const float* pos,
+                      const float* vel, const float* out_dir, float* impulse, float* ext_force,
+                      int32_t* contact_flag, void* stream) {
+  DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
+  DT_REQUIRE(n >= 0 && n_tris >= 0 && idx_stride > 0 && idx_col >= 0 && idx_col < idx_stride,
+             "bad size/stride");
+  if (n == 0) return DT_OK;
+  DT_REQUIRE(x && v && contact_idx && tris && pos && vel && impulse && ext_force,
+             "NULL pointer");
+  hipLaunchKernelGGL(k_rope_collision, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                     contact_dev(cp), dt, (int)n, x, v, contact_idx, idx_stride, idx_col, tris, pos,
+                     vel, out_dir[0], out_dir[1], out_dir[2], impulse, ext_force, contact_flag);
+  return check_launch("dt_rope_collision");
+}
+
+int dt_rope_collision_adj(const dt_contact_params* cp, float dt, int32_t n, const float* x,
+                          const float* v, const int32_t* contact_idx, int32_t idx_stride,
+                          int32_t idx_col, int32_t n_tris, const int32_t* tris, const float* pos,
+                          const float* vel, const float* out_dir, const float* g_impulse,
+                          const float* g_ext_force, float* g_x, float* g_v, float* g_pos,
+                          float* g_vel, void* stream) {
+  DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
+  DT_REQUIRE(n >= 0 && n_tris >= 0 && idx_stride > 0 && idx_col >= 0 && idx_col < idx_stride,
+             "bad size/stride");
+  if (n == 0) return DT_OK;
+  DT_REQUIRE(x && v && contact_idx && tris && pos && vel && g_impulse && g_ext_force && g_x &&
+                 g_v && g_pos && g_vel,
+             "NULL pointer");
+  hipLaunchKernelGGL(k_rope_collision_adj, dim3(nblocks(n)), dim3(kBlock), 0,
+                     (hipStream_t)stream, contact_dev(cp), dt, (int)n, x, v, contact_idx,
+                     idx_stride, idx_col, tris, pos, vel, out_dir[0], out_dir[1], out_dir[2],
+                     g_impulse, g_ext_force, g_x, g_v, g_pos, g_vel);
+  return check_launch("dt_rope_collision_adj");
 }
 
 }  // extern "C"

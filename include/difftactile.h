@@ -603,6 +603,47 @@ int dt_kabsch_angle_adj(int32_t n, const float* x, const float* v_next, float dt
                         const int32_t* mask, float* state, const float* g_angle, float* gx,
                         float* gv_next, void* stream);
 
+/* ---- position-based rope and its particle-wise pad contact ----
+   (object_model/PBD_rope.py, tasks/cable_straightening.py) */
+typedef struct dt_rope_params {
+  float dt, p_mass, table_height, floor_friction; /* PBD_rope.py:17-28,71      */
+  float gravity[3];
+  float rest_length, bending_length;              /* 34-42                      */
+  float stretch_compliance, bending_compliance;   /* 49-52                      */
+  float stretch_relaxation, bending_relaxation;
+} dt_rope_params;
+/* update (PBD_rope.py:111-180) for n particles [n][3]: ext_impulse is the
+   contact impulse ext_f; particle 0 goes to fixed_pos (init_pos[0]); work:
+   12 n floats of scratch. */
+int dt_rope_step(const dt_rope_params* prm, int32_t n, const float* fixed_pos, const float* pos,
+                 const float* vel, const float* ext_impulse, float* work, float* pos_next,
+                 float* vel_next, void* stream);
+/* Adjoint (update.grad): g_pos, g_vel, g_ext_impulse accumulate. */
+int dt_rope_step_adj(const dt_rope_params* prm, int32_t n, const float* fixed_pos,
+                     const float* pos, const float* vel, const float* ext_impulse,
+                     const float* g_pos_next, const float* g_vel_next, float* work, float* g_pos,
+                     float* g_vel, float* g_ext_impulse, void* stream);
+/* check_collision for rope particles (cable_straightening.py:287-292): the
+   closest triangle to each particle position, as dt_contact_closest. */
+int dt_rope_closest(int32_t n, const float* x, int32_t n_tris, const float* centroids4,
+                    int32_t* contact_idx, int32_t idx_stride, int32_t idx_col, void* stream);
+/* collision for rope particles (294-319) with one pad: cp->sdf_offset is the
+   particle radius; impulse [n][3] += f dt, ext_force (pad) += -f/3 per
+   triangle vertex, contact_flag[p] = 1 where the particle touches (NULL:
+   not written). */
+int dt_rope_collision(const dt_contact_params* cp, float dt, int32_t n, const float* x,
+                      const float* v, const int32_t* contact_idx, int32_t idx_stride,
+                      int32_t idx_col, int32_t n_tris, const int32_t* tris, const float* pos,
+                      const float* vel, const float* out_dir, float* impulse, float* ext_force,
+                      int32_t* contact_flag, void* stream);
+/* Adjoint: g_x, g_v (rope) and g_pos, g_vel (pad) accumulate. */
+int dt_rope_collision_adj(const dt_contact_params* cp, float dt, int32_t n, const float* x,
+                          const float* v, const int32_t* contact_idx, int32_t idx_stride,
+                          int32_t idx_col, int32_t n_tris, const int32_t* tris, const float* pos,
+                          const float* vel, const float* out_dir, const float* g_impulse,
+                          const float* g_ext_force, float* g_x, float* g_v, float* g_pos,
+                          float* g_vel, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,135 @@
+"""GPU parity of the position-based rope and its particle-wise pad contact
+against the oracle.
+
+Tolerances: one substep within 1e-6 relative L2 of FP64 on the same float32
+inputs; adjoints within 1e-4; contact impulses and pad forces within 1e-5,
+closest triangles bit-exact; a 4-substep rope-pad rollout's gradient within
+1e-3 of the oracle's FP64 reverse mode.
+"""
+import numpy as np
+import pytest
+import torch
+
+from difftactile_amd.object_model.pbd_rope import PBDRope, rope_contact
+from difftactile_amd.sensor_model.meshes import slab_mesh
+from oracle import contact as oc
+from oracle import rope as orp
+from tests.mpm_cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+D64 = torch.float64
+
+
+def _rope(n=300, dt=1e-3, seed=0):
+    rope = PBDRope(dt=dt, p_rad=0.01, n_particles=n, table_height=0.0)
+    p0, v0 = rope.init([0.0, 0.05, 0.0], [0.0, 0.0, 1.0])
+    rng = np.random.default_rng(seed)
+    p = p0.cpu().numpy() + 0.003 * rng.normal(size=(n, 3))
+    p[::37, 1] = 0.0005                        # some particles at the floor
+    v = 0.3 * rng.normal(size=(n, 3))
+    v[::37, 1] = -2.0
+    e = 1e-7 * rng.normal(size=(n, 3))
+    rp = orp.RopeParams(n, dt=dt, p_rad=0.01, table_height=0.0)
+    return rope, rp, p.astype(np.float32), v.astype(np.float32), e.astype(np.float32)
+
+
+def test_rope_step_and_adjoint_match_the_oracle(gpu):
+    rope, rp, p, v, e = _rope()
+    args = [torch.tensor(a).cuda().requires_grad_(True) for a in (p, v, e)]
+    p1, v1 = rope.step(*args)
+    ref = [torch.tensor(a, dtype=D64, requires_grad=True) for a in (p, v, e)]
+    fixed = rope.init_pos[0].cpu().double()
+    q1, w1 = orp.step(rp, *ref, fixed)
+    assert rel_l2(p1.detach().cpu().double(), q1.detach()) < 1e-6
+    # velocities difference float32 positions (up to 6 long) a substep apart
+    # and divide by dt, ~4e-4 of rounding against changes of ~0.5: compare
+    # the change to 5e-3
+    assert rel_l2(v1.detach().cpu().double() - ref[1].detach(), (w1 - ref[1]).detach()) < 5e-3
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=p.shape).astype(np.float32)
+    b = (1e-3 * rng.normal(size=p.shape)).astype(np.float32)
+    g = torch.autograd.grad((p1 * torch.tensor(a).cuda()).sum() + (v1 * torch.tensor(b).cuda()).sum(),
+                            args)
+    go = torch.autograd.grad((q1 * torch.tensor(a, dtype=D64)).sum() +
+                             (w1 * torch.tensor(b, dtype=D64)).sum(), ref)
+    for k in range(3):
+        assert rel_l2(g[k].cpu().double(), go[k]) < 1e-4, k
+
+
+def _pad_case(seed=1):
+    nodes, f2v, tris, layer, od = slab_mesh(12, 12, 3, h=0.02, origin=(-0.1, 0.06, -0.02))
+    rng = np.random.default_rng(seed)
+    pos = (nodes + 1e-3 * rng.normal(size=nodes.shape)).astype(np.float32)
+    vel = (0.1 * rng.normal(size=nodes.shape)).astype(np.float32)
+    return pos, vel, tris.astype(np.int32), od
+
+
+def test_rope_contact_and_adjoint_match_the_oracle(gpu):
+    rope, rp, p, v, _ = _rope(n=200, seed=2)
+    r3 = np.random.default_rng(3)
+    # particles spread under the pad's contact face (y = 0.06), some within
+    # their radius of it
+    p[:, 0] = r3.uniform(-0.08, 0.1, len(p))
+    p[:, 2] = r3.uniform(0.0, 0.18, len(p))
+    p[:, 1] = 0.045 + 0.02 * r3.random(len(p))
+    pos, vel, tris, od = _pad_case()
+    kc = (1e3, 10.0, 50.0, 0.5)
+    args = [torch.tensor(a).cuda().requires_grad_(True) for a in (p, v, pos, vel)]
+    imp, ext, flag = rope_contact(rope, *args, torch.tensor(tris).cuda(), od, *kc)
+    idx = oc.find_closest_np(p, pos, tris)
+    ref = [torch.tensor(a, dtype=D64, requires_grad=True) for a in (p, v, pos, vel)]
+    imp_o, ext_o, flag_o = orp.contact(rp, ref[0], ref[1], ref[2], ref[3], tris,
+                                       torch.tensor(od, dtype=D64), idx, *kc)
+    assert flag_o.sum() > 10 and (~flag_o).sum() > 10
+    assert np.array_equal(flag.cpu().numpy().astype(bool), flag_o.numpy())
+    assert rel_l2(imp.detach().cpu().double(), imp_o.detach()) < 1e-5
+    assert rel_l2(ext.detach().cpu().double(), ext_o.detach()) < 1e-5
+    rng = np.random.default_rng(4)
+    wa = rng.normal(size=p.shape).astype(np.float32)
+    wb = rng.normal(size=pos.shape).astype(np.float32)
+    g = torch.autograd.grad((imp * torch.tensor(wa).cuda()).sum() +
+                            (ext * torch.tensor(wb).cuda()).sum(), args)
+    go = torch.autograd.grad((imp_o * torch.tensor(wa, dtype=D64)).sum() +
+                             (ext_o * torch.tensor(wb, dtype=D64)).sum(), ref)
+    for k in range(4):
+        assert rel_l2(g[k].cpu().double(), go[k]) < 1e-4, k
+
+
+def test_rope_pad_rollout_gradient_matches_the_oracle(gpu):
+    """Four substeps: pad contact impulse drives the rope step; the loss on
+    the final rope state reaches the initial rope velocity and the pad."""
+    rope, rp, p, v, _ = _rope(n=10, seed=6)
+    # a short rope along z just under the pad's contact face (y = 0.06)
+    p = rope.init_pos.cpu().numpy().copy()
+    p[:, 1] = 0.053
+    p = (p + 5e-4 * np.random.default_rng(7).normal(size=p.shape)).astype(np.float32)
+    v = (0.05 * np.random.default_rng(8).normal(size=p.shape)).astype(np.float32)
+    pos, vel, tris, od = _pad_case(seed=9)
+    kc = (1e3, 10.0, 50.0, 0.5)
+    tris_g = torch.tensor(tris).cuda()
+    wv = np.random.default_rng(10).normal(size=p.shape)
+
+    xg = torch.tensor(p).cuda()
+    vg = torch.tensor(v).cuda().requires_grad_(True)
+    pg = torch.tensor(pos).cuda().requires_grad_(True)
+    x, vv = xg, vg
+    for _ in range(4):
+        imp, _, _ = rope_contact(rope, x, vv, pg, torch.tensor(vel).cuda(), tris_g, od, *kc)
+        x, vv = rope.step(x, vv, imp)
+    loss = (x * torch.tensor(wv, dtype=torch.float32).cuda()).sum()
+    g_v, g_p = torch.autograd.grad(loss, [vg, pg])
+
+    xo = torch.tensor(p, dtype=D64)
+    vo0 = torch.tensor(v, dtype=D64, requires_grad=True)
+    po = torch.tensor(pos, dtype=D64, requires_grad=True)
+    fixed = rope.init_pos[0].cpu().double()
+    x, vv = xo, vo0
+    for _ in range(4):
+        idx = oc.find_closest_np(x.detach().numpy().astype(np.float32), pos, tris)
+        imp, _, _ = orp.contact(rp, x, vv, po, torch.tensor(vel, dtype=D64), tris,
+                                torch.tensor(od, dtype=D64), idx, *kc)
+        x, vv = orp.step(rp, x, vv, imp, fixed)
+    lo = (x * torch.tensor(wv)).sum()
+    gvo, gpo = torch.autograd.grad(lo, [vo0, po])
+    assert rel_l2(g_v.cpu().double(), gvo) < 1e-3
+    assert rel_l2(g_p.cpu().double(), gpo) < 1e-3
