@@ -49,6 +49,9 @@ def parse():
                          "population: 64 environments, batched vs a per-environment loop")
     ap.add_argument("--pop-substeps", type=int, default=300,
                     help="population workload: substeps per fwd+bwd step")
+    ap.add_argument("--pad-recompute", action="store_true",
+                    help="pad_press: keep checkpoints only (the backward pass recomputes "
+                         "every segment) instead of the resident trajectory")
     ap.add_argument("--pop-pads", action="store_true",
                     help="population workload: one FEM pad (10,800 tets) per environment "
                          "pressing its object")
@@ -280,8 +283,11 @@ def grad_check(torch):
 
 def bench_pad_press(args, torch):
     """Pad pressing a block: coupled fwd+bwd over the full press (500 substeps,
-    checkpoint every 50); loss = summed squared marker displacement of the pad at
-    the end; gradient w.r.t. the pad velocity sequence."""
+    segments of 50); loss = summed squared marker displacement of the pad at
+    the end; gradient w.r.t. the pad velocity sequence.  The segments the
+    forward pass keeps within the object's resident budget (a quarter of HBM)
+    are not recomputed by the backward pass; --pad-recompute keeps checkpoints
+    only."""
     import numpy as np
     from difftactile_amd import scenes
     from difftactile_amd.coupled import CoupledSim, Pad
@@ -291,10 +297,13 @@ def bench_pad_press(args, torch):
     prm, n = c["prm"], len(c["x"])
     T = c["n_substeps"]
     nodes, f2v, tris, layer, od = c["mesh"]
-    sim = MPMSim(prm, c["mats"], n, checkpoint_every=50, sort=True)
+    sim = MPMSim(prm, c["mats"], n, checkpoint_every=50, sort=True,
+                 resident_bytes=0 if args.pad_recompute else None)
     pad = Pad(nodes.astype(np.float32), f2v, tris, layer, od, c["pad_mu"], c["pad_lam"],
               c["pad_mass"], prm.dt)
     cs = CoupledSim(sim, [pad], **c["contact"])
+    seg_bytes = 51 * cs.substep_bytes()
+    kept = min(T // 50, sim.resident_bytes // seg_bytes)
     surf_ids = np.where(layer == layer.max())[0]
     mm = MarkerModel(nodes, surf_ids, c["inv_T"])
     dev = torch.device("cuda")
@@ -336,7 +345,8 @@ def bench_pad_press(args, torch):
                                   "loss = summed marker displacement"},
            "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(g).all()),
            "grad_dL_dvy_first_last": [float(gv[0, 1]), float(gv[-1, 1])],
-           "block_min_dy": dy}
+           "block_min_dy": dy, "resident_segments": f"{kept} of {T // 50}",
+           "resident_gb": round(kept * seg_bytes / 1e9, 1)}
     print(json.dumps(res), flush=True)
 
 

@@ -17,7 +17,10 @@ velocity per frame, ``gripper_fem.py:403-418``), so the gradient with respect
 to the pad velocity sequence is available.  Checkpointing as in ``sim.py``:
 one object + pad state every ``checkpoint_every`` substeps in HBM; the backward
 pass recomputes each segment keeping its frames, grids, contact impulses and
-closest-triangle indices.
+closest-triangle indices.  Segments whose frames and per-substep stores fit
+the object's ``resident_bytes`` (HBM set aside for the trajectory; a quarter
+of the device by default) are kept from the forward pass instead, so the
+backward pass does not recompute them.
 """
 import numpy as np
 import torch
@@ -106,6 +109,44 @@ class CoupledSim:
                   _lib.ctypes.byref(c))
         p.active_bricks, p.n_active_bricks = a.value, c.value
         return p
+
+    def substep_bytes(self):
+        """HBM bytes one kept substep holds for the backward pass: object and
+        pad frames, the packed grids, contact forces, closest triangles and the
+        pads' updated velocities."""
+        nn = self.sim.prm.n_nodes
+        b = 4 * self.sim.n * 24 + nn * (16 + 16 + 12 + 4 * self.npad)
+        for pd in self.pads:
+            b += 4 * 3 * pd.n_verts * 4
+        return b
+
+    def _new_store(self, hint):
+        sim, dev = self.sim, self.sim.device
+        nn = sim.prm.n_nodes
+        return {"g4": torch.empty(nn, 4, device=dev), "gv4": torch.empty(nn, 4, device=dev),
+                "gf": torch.empty(nn, 3, device=dev),
+                "idx": torch.empty(nn, self.npad, dtype=I32, device=dev), "hint": hint,
+                "vel_u": [torch.empty(pd.n_verts, 3, device=dev) for pd in self.pads]}
+
+    def _run_keep(self, st0, pst0, cvs, s0, k, mat_id):
+        """Advance substeps s0 .. s0+k-1 from (st0, pst0) keeping every frame
+        and each substep's store.  Returns (frames, pad frames, stores); st0
+        and pst0 are copied, not kept."""
+        frames = [[t.clone() for t in st0]]
+        pframes = [[[t.clone() for t in p] for p in pst0]]
+        stores = []
+        for j in range(k):
+            # closest-triangle starts: the previous substep's result
+            store = self._new_store(stores[-1]["idx"] if stores else self.idx)
+            nxt = [torch.empty_like(t) for t in frames[-1]]
+            pn = [[torch.empty_like(t) for t in p] for p in pframes[-1]]
+            f = s0 + j
+            self._fwd(frames[-1], nxt, pframes[-1], pn, [c[f] for c in cvs],
+                      [c[f + 1] for c in cvs], mat_id, store)
+            frames.append(nxt)
+            pframes.append(pn)
+            stores.append(store)
+        return frames, pframes, stores
 
     # -- one substep --------------------------------------------------------
     def _fwd(self, st, nxt, pads_in, pads_out, cv, cv1, mat_id, store=None):
@@ -267,20 +308,32 @@ class _CoupledRollout(torch.autograd.Function):
         nxt = [torch.empty_like(t) for t in st]
         pst = pads0
         pnx = [[torch.empty_like(t) for t in p] for p in pst]
-        ckpts = []
-        step = 0
+        ckpts, kept = [], []
+        keep_ok = any(ctx.needs_input_grad)
+        used = 0
         for (s0, k) in _segments(n_substeps, sim.checkpoint_every):
+            nbytes = (k + 1) * cs.substep_bytes()
+            if keep_ok and used + nbytes <= sim.resident_bytes:
+                # resident segment: frames and stores kept for the backward pass
+                used += nbytes
+                seg = cs._run_keep(st, pst, cvs, s0, k, mat_id)
+                ckpts.append(None)
+                kept.append(seg)
+                st = [t.clone() for t in seg[0][-1]]
+                pst = [[t.clone() for t in p] for p in seg[1][-1]]
+                cs.idx.copy_(seg[2][-1]["idx"])
+                continue
             ckpts.append(([t.clone() for t in st], [[t.clone() for t in p] for p in pst]))
-            for _ in range(k):
-                cs._fwd(st, nxt, pst, pnx, [c[step] for c in cvs], [c[step + 1] for c in cvs],
-                        mat_id)
+            kept.append(None)
+            for f in range(s0, s0 + k):
+                cs._fwd(st, nxt, pst, pnx, [c[f] for c in cvs], [c[f + 1] for c in cvs], mat_id)
                 st, nxt = nxt, st
                 pst, pnx = pnx, pst
-                step += 1
         out = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(st, W, shapes)]
         for p in pst:
             out += [t.clone() for t in p]
         ctx.cs, ctx.n_substeps, ctx.ckpts, ctx.shapes = cs, n_substeps, ckpts, shapes
+        ctx.kept = kept
         ctx.perm, ctx.mat_id, ctx.env = perm, mat_id, env
         ctx.cv_shapes = [c.shape for c in cvs]
         ctx.save_for_backward(*cvs)
@@ -304,28 +357,17 @@ class _CoupledRollout(torch.autograd.Function):
         g_cv = [torch.zeros(sh, device=dev) for sh in ctx.cv_shapes]
         gbuf = [torch.empty_like(t) for t in g]
         gpbuf = [[torch.empty_like(t) for t in p] for p in gp]
-        nn = sim.prm.n_nodes
         segs = _segments(ctx.n_substeps, sim.checkpoint_every)
         sim._set_env(ctx.env)
-        for (s0, k), (ck, ckp) in zip(reversed(segs), reversed(ctx.ckpts)):
-            # recompute the segment, keeping what the adjoint needs
-            frames = [[t.clone() for t in ck]]
-            pframes = [[[t.clone() for t in p] for p in ckp]]
-            stores = []
-            for j in range(k):
-                store = {"g4": torch.empty(nn, 4, device=dev), "gv4": torch.empty(nn, 4, device=dev),
-                         "gf": torch.empty(nn, 3, device=dev),
-                         "idx": torch.empty(nn, cs.npad, dtype=I32, device=dev),
-                         "hint": stores[-1]["idx"] if stores else cs.idx,
-                         "vel_u": [torch.empty(pd.n_verts, 3, device=dev) for pd in cs.pads]}
-                nxt = [torch.empty_like(t) for t in frames[-1]]
-                pn = [[torch.empty_like(t) for t in p] for p in pframes[-1]]
-                f = s0 + j
-                cs._fwd(frames[-1], nxt, pframes[-1], pn, [c[f] for c in cvs],
-                        [c[f + 1] for c in cvs], ctx.mat_id, store)
-                frames.append(nxt)
-                pframes.append(pn)
-                stores.append(store)
+        for i in reversed(range(len(segs))):
+            s0, k = segs[i]
+            if ctx.kept[i] is not None:
+                frames, pframes, stores = ctx.kept[i]
+                ctx.kept[i] = None
+            else:
+                # recompute the segment, keeping what the adjoint needs
+                ck, ckp = ctx.ckpts[i]
+                frames, pframes, stores = cs._run_keep(ck, ckp, cvs, s0, k, ctx.mat_id)
             for j in reversed(range(k)):
                 f = s0 + j
                 cs._adj(frames[j], pframes[j], stores[j], g, gp, gbuf, gpbuf,

@@ -352,12 +352,16 @@ def test_coupled_substep_forward_and_adjoint_match_oracle(gpu):
         assert err < 1e-4, (k, err)
 
 
-@pytest.mark.parametrize("sort,sparse", [(False, False), (False, True), (True, True)])
-def test_coupled_rollout_gradient_matches_oracle(gpu, sort, sparse):
-    """Fused coupled rollout (object + pad + contact, checkpointed) vs the oracle's
-    reverse mode over the same composition, including d/d(pad velocity sequence);
+@pytest.mark.parametrize("sort,sparse,resident", [(False, False, "none"), (False, True, "none"),
+                                                  (True, True, "none"), (True, True, "all"),
+                                                  (True, True, "first")])
+def test_coupled_rollout_gradient_matches_oracle(gpu, sort, sparse, resident):
+    """Fused coupled rollout (object + pad + contact) vs the oracle's reverse
+    mode over the same composition, including d/d(pad velocity sequence);
     dense object stages, binned brick passes on the caller's order, and binned
-    passes on cell-sorted particles."""
+    passes on cell-sorted particles.  resident: which of the two segments the
+    forward pass keeps for the backward pass (none: both recomputed from
+    checkpoints; first: the first kept, the second recomputed)."""
     from difftactile_amd.coupled import CoupledSim, Pad
     from difftactile_amd.params import Material
     from difftactile_amd.sim import MPMSim
@@ -370,6 +374,9 @@ def test_coupled_rollout_gradient_matches_oracle(gpu, sort, sparse):
     sim = MPMSim(prm, [Material(5e4, 0.3)], n, checkpoint_every=4, sort=sort, sparse_grid=sparse)
     pad = Pad(nodes.astype(np.float32), f2v, surf, layer, od, mu, lam, mass, prm.dt)
     cs = CoupledSim(sim, [pad], kn=1e3, kd=10.0, kt=50.0, mu_f=0.5, clamp_grad=False)
+    # segments of 4 and 2 substeps: 5 and 3 kept substeps' worth of bytes
+    sim.resident_bytes = {"none": 0, "all": 8 * cs.substep_bytes(),
+                          "first": 5 * cs.substep_bytes()}[resident]
     rng = np.random.default_rng(11)
     cv = np.zeros((steps + 1, len(nodes), 3), np.float32)
     cv[:, :, 1] = -0.5 + 0.05 * rng.normal(size=(steps + 1, 1))
