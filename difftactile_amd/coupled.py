@@ -115,10 +115,18 @@ class CoupledSim:
         pad frames, the packed grids, contact forces, closest triangles and the
         pads' updated velocities."""
         nn = self.sim.prm.n_nodes
-        b = 4 * self.sim.n * 24 + nn * (16 + 16 + 12 + 4 * self.npad)
+        b = 4 * self.sim.n * 24 + nn * (16 + 16 + 12 + 4 * self.npad) + self._bin_slot_bytes()
         for pd in self.pads:
             b += 4 * 3 * pd.n_verts * 4
         return b
+
+    def _bin_slot_bytes(self):
+        """Bytes of one substep's saved binning (0 on the dense path)."""
+        if not self.sim.sparse_grid:
+            return 0
+        nb = _lib.ctypes.c_size_t()
+        _lib.call("dt_mpm_bin_slot_bytes", self.sim.c_prm, self.sim.n, _lib.ctypes.byref(nb))
+        return nb.value
 
     def _new_store(self, hint):
         sim, dev = self.sim, self.sim.device
@@ -126,7 +134,9 @@ class CoupledSim:
         return {"g4": torch.empty(nn, 4, device=dev), "gv4": torch.empty(nn, 4, device=dev),
                 "gf": torch.empty(nn, 3, device=dev),
                 "idx": torch.empty(nn, self.npad, dtype=I32, device=dev), "hint": hint,
-                "vel_u": [torch.empty(pd.n_verts, 3, device=dev) for pd in self.pads]}
+                "vel_u": [torch.empty(pd.n_verts, 3, device=dev) for pd in self.pads],
+                "bin": (torch.empty(self._bin_slot_bytes(), dtype=torch.uint8, device=dev)
+                        if sim.sparse_grid else None)}
 
     def _run_keep(self, st0, pst0, cvs, s0, k, mat_id):
         """Advance substeps s0 .. s0+k-1 from (st0, pst0) keeping every frame
@@ -169,6 +179,9 @@ class CoupledSim:
             # written nor read
             ws, nb = sim._workspace()
             _lib.call("dt_mpm_bin", prm, sim.n, P(st[0]), ws, nb, s)
+            if store and store.get("bin") is not None:
+                # kept for the adjoint, which then need not bin again
+                _lib.call("dt_mpm_bin_copy", prm, sim.n, ws, nb, P(store["bin"], torch.uint8), 1, s)
             prm = self._node_set(prm, ws, nb)
             _lib.call("dt_mpm_p2g_binned", prm, mats, nm, sim.n, mid, *[P(t) for t in st],
                       P(nxt[3]), ws, nb, s)
@@ -239,10 +252,13 @@ class CoupledSim:
             g_ext.append(ge)
         prm_clamp = self.c_prm_clamp
         if sim.sparse_grid:
-            # the binning of the forward substep's positions again: the same
-            # active bricks as the stored grids
+            # the forward substep's binning (saved, or made again from its
+            # positions): the same active bricks as the stored grids
             ws, nb = sim._workspace()
-            _lib.call("dt_mpm_bin", prm, sim.n, P(st[0]), ws, nb, s)
+            if store.get("bin") is not None:
+                _lib.call("dt_mpm_bin_copy", prm, sim.n, ws, nb, P(store["bin"], torch.uint8), 0, s)
+            else:
+                _lib.call("dt_mpm_bin", prm, sim.n, P(st[0]), ws, nb, s)
             prm = self._node_set(prm, ws, nb)
             prm_clamp = self._node_set(prm_clamp, ws, nb)
             _lib.call("dt_mpm_g2p4_adj_binned", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]),

@@ -82,8 +82,11 @@ size_t bin_history_slot_bytes(const dt_mpm_params* prm, int n) {
 }
 
 BinWs bin_slot(const BinWs& ws, int s, int n) {
+  return bin_view(ws, ws.hist + (size_t)s * ws.hist_slot_bytes, n);
+}
+
+BinWs bin_view(const BinWs& ws, char* b, int n) {
   BinWs v = ws;
-  char* b = ws.hist + (size_t)s * ws.hist_slot_bytes;
   const long long nbk = ws.n_bricks;
   v.perm = reinterpret_cast<int32_t*>(b);
   b += align256(sizeof(int32_t) * (size_t)n);
@@ -96,6 +99,36 @@ BinWs bin_slot(const BinWs& ws, int s, int n) {
   v.n_list = reinterpret_cast<int32_t*>(b);
   v.n_alist = reinterpret_cast<int32_t*>(b + 256);
   return v;
+}
+
+// Copies one binning into another view: perm, brick starts, and the
+// occupied- and active-brick lists up to their device-side counts.
+__global__ __launch_bounds__(kBlock) void k_bin_copy(
+    int n, long long nbk, const int32_t* __restrict__ perm, const int32_t* __restrict__ start,
+    const int32_t* __restrict__ list, const int32_t* __restrict__ n_list,
+    const int32_t* __restrict__ alist, const int32_t* __restrict__ n_alist,
+    int32_t* __restrict__ perm2, int32_t* __restrict__ start2, int32_t* __restrict__ list2,
+    int32_t* __restrict__ n_list2, int32_t* __restrict__ alist2, int32_t* __restrict__ n_alist2) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int no = *n_list, na = *n_alist;
+  if (i < n) perm2[i] = perm[i];
+  if (i <= nbk) start2[i] = start[i];
+  if (i < no) list2[i] = list[i];
+  if (i < na) alist2[i] = alist[i];
+  if (i == 0) {
+    *n_list2 = no;
+    *n_alist2 = na;
+  }
+}
+
+void launch_bin_copy(const BinWs& from, const BinWs& to, int n, hipStream_t st) {
+  const long long nbk = from.n_bricks;
+  const long long m = (long long)n > nbk + 1 ? (long long)n : nbk + 1;
+  hipLaunchKernelGGL(k_bin_copy, dim3(nblocks(m)), dim3(kBlock), 0, st, n, nbk,
+                     (const int32_t*)from.perm, (const int32_t*)from.start,
+                     (const int32_t*)from.list, (const int32_t*)from.n_list,
+                     (const int32_t*)from.alist, (const int32_t*)from.n_alist, to.perm, to.start,
+                     to.list, to.n_list, to.alist, to.n_alist);
 }
 
 // Active lanes form a prefix of every wavefront (p = block * 256 + thread).

@@ -42,6 +42,11 @@ struct BinWs {
 size_t bin_history_slot_bytes(const dt_mpm_params* prm, int n);
 // View of ws whose per-substep arrays live in history slot s.
 BinWs bin_slot(const BinWs& ws, int s, int n);
+// View of ws whose per-substep arrays (perm, start, list, alist and the two
+// counts) live at base, laid out as one history slot.
+BinWs bin_view(const BinWs& ws, char* base, int n);
+// Copies the binning of view from into view to (lists up to their counts).
+void launch_bin_copy(const BinWs& from, const BinWs& to, int n, hipStream_t st);
 
 long long brick_total(const dt_mpm_params* prm);
 // Total bytes of the workspace for n particles; fills *ws when base != NULL.

@@ -2519,6 +2519,28 @@ int dt_mpm_bin_perm(const dt_mpm_params* prm, int32_t n, const void* workspace,
   return DT_OK;
 }
 
+int dt_mpm_bin_slot_bytes(const dt_mpm_params* prm, int32_t n, size_t* bytes) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n >= 0, "n must be >= 0");
+  DT_REQUIRE(bytes, "NULL bytes");
+  *bytes = bin_history_slot_bytes(prm, n);
+  return DT_OK;
+}
+
+int dt_mpm_bin_copy(const dt_mpm_params* prm, int32_t n, void* workspace, size_t workspace_bytes,
+                    void* slot, int32_t save, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(n >= 0, "n must be >= 0");
+  DT_REQUIRE(slot && ((uintptr_t)slot & 255u) == 0, "slot must be a 256-byte aligned buffer");
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
+  const BinWs sv = bin_view(ws, static_cast<char*>(slot), n);
+  launch_bin_copy(save ? ws : sv, save ? sv : ws, n, DT_STREAM(stream));
+  return check_launch("binning copy");
+}
+
 int dt_mpm_p2g_binned(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                       const int32_t* mat_id, const float* x, const float* v, const float* C,
                       const float* F, float* F_next, void* workspace, size_t workspace_bytes,

@@ -317,6 +317,16 @@ int dt_mpm_zero_nodes(const dt_mpm_params* prm, float* buf, int32_t width, void*
    perm[brick_start[b] .. brick_start[b+1]) in no particular order. */
 int dt_mpm_bin_perm(const dt_mpm_params* prm, int32_t n, const void* workspace,
                     size_t workspace_bytes, const int32_t** perm, const int32_t** brick_start);
+/* Bytes of one saved binning for n particles (perm, brick starts, occupied-
+   and active-brick lists and their counts); slots are 256-byte aligned. */
+int dt_mpm_bin_slot_bytes(const dt_mpm_params* prm, int32_t n, size_t* bytes);
+/* save = 1: copies the last dt_mpm_bin of the workspace into slot;
+   save = 0: copies a saved binning from slot back into the workspace, after
+   which the binned passes and dt_mpm_active_bricks see it as if dt_mpm_bin
+   had run again on the same x (the adjoint of a substep reuses its forward
+   binning this way). */
+int dt_mpm_bin_copy(const dt_mpm_params* prm, int32_t n, void* workspace, size_t workspace_bytes,
+                    void* slot, int32_t save, void* stream);
 /* Brick-owned fused p2g over the binned particles: writes F_next and the
    per-brick tiles in the workspace (after dt_mpm_bin on the same x). */
 int dt_mpm_p2g_binned(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,

@@ -475,6 +475,47 @@ def test_binning_is_a_counting_sort_by_stencil_brick(gpu, n_grid, n, sort):
     assert np.array_equal(start, want)
 
 
+def _active_list(c_prm, n, ws, nbytes):
+    import ctypes
+    a, c = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(_lib.lib().dt_mpm_active_bricks(c_prm, n, ws.data_ptr(), nbytes, ctypes.byref(a),
+                                               ctypes.byref(c)))
+    o_a, o_c = a.value - ws.data_ptr(), c.value - ws.data_ptr()
+    n_act = int(ws[o_c:o_c + 4].view(torch.int32).item())
+    return ws[o_a:o_a + 4 * n_act].view(torch.int32).cpu().numpy()
+
+
+@pytest.mark.parametrize("n_grid", [32, 132])
+def test_saved_binning_restores_after_another(gpu, n_grid):
+    """dt_mpm_bin_copy: a binning saved to a slot and copied back after the
+    workspace binned other positions gives the first binning's perm, brick
+    starts and active-brick list again (one-workgroup and device-wide scans)."""
+    import ctypes
+    prm = default_prm(n_grid)
+    c_prm = prm.c_struct()
+    n = 9000
+    rng = np.random.default_rng(n_grid)
+    xa = rng.uniform(0.1, 0.5, size=(n, 3)).astype(np.float32)
+    xb = rng.uniform(0.4, 0.9, size=(n, 3)).astype(np.float32)
+    ws, nbytes = _workspace(c_prm, n)
+    n_bricks = ((n_grid + 3) // 4) ** 3
+    sb = ctypes.c_size_t()
+    call("dt_mpm_bin_slot_bytes", c_prm, n, ctypes.byref(sb))
+    slot = torch.full((sb.value,), 0xAB, dtype=torch.uint8, device="cuda")
+    call("dt_mpm_bin", c_prm, n, dev(xa), ws, nbytes, _lib.stream())
+    want = _bin_arrays(c_prm, n, ws, nbytes, n_bricks) + (_active_list(c_prm, n, ws, nbytes),)
+    call("dt_mpm_bin_copy", c_prm, n, ws, nbytes, slot.data_ptr(), 1, _lib.stream())
+    call("dt_mpm_bin", c_prm, n, dev(xb), ws, nbytes, _lib.stream())
+    other = _active_list(c_prm, n, ws, nbytes)
+    assert len(other) != len(want[2]) or not np.array_equal(other, want[2])
+    call("dt_mpm_bin_copy", c_prm, n, ws, nbytes, slot.data_ptr(), 0, _lib.stream())
+    got = _bin_arrays(c_prm, n, ws, nbytes, n_bricks) + (_active_list(c_prm, n, ws, nbytes),)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    with pytest.raises(RuntimeError):
+        call("dt_mpm_bin_copy", c_prm, n, ws, nbytes, slot.data_ptr() + 4, 0, _lib.stream())
+
+
 def test_binning_of_no_particles(gpu):
     prm = default_prm(32)
     c_prm = prm.c_struct()
