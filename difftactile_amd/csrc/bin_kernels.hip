@@ -29,7 +29,7 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   size_t scan_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
                                    (int32_t*)nullptr, (int)(nbk + 1));
-  const size_t sz[10] = {sizeof(int32_t) * (size_t)(nbk + 2),
+  const size_t sz[11] = {sizeof(int32_t) * (size_t)(nbk + 2),
                         sizeof(int32_t) * (size_t)(nbk + 1),
                         sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t) * (size_t)n,
@@ -38,9 +38,10 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
                         sizeof(float4) * (size_t)nbk * kTileNodes,
                         scan_bytes,
                         sizeof(int32_t) * (size_t)nbk,
-                        sizeof(int32_t)};
-  size_t off[10], total = 0;
-  for (int i = 0; i < 10; ++i) {
+                        sizeof(int32_t),
+                        sizeof(int32_t) * (size_t)(kSlabMax + kSlabMax * kSlabMax)};
+  size_t off[11], total = 0;
+  for (int i = 0; i < 11; ++i) {
     off[i] = total;
     total += align256(sz[i] ? sz[i] : 1);
   }
@@ -57,6 +58,8 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
     ws->scan_tmp = b + off[7];
     ws->alist = reinterpret_cast<int32_t*>(b + off[8]);
     ws->n_alist = reinterpret_cast<int32_t*>(b + off[9]);
+    ws->slab_sum = reinterpret_cast<int32_t*>(b + off[10]);
+    ws->row_occ = reinterpret_cast<uint32_t*>(ws->slab_sum + kSlabMax);
     ws->scan_bytes = scan_bytes;
     ws->n_bricks = nbk;
     ws->nb = (prm->n_grid + 3) >> 2;
@@ -198,104 +201,85 @@ __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int 
   wave_append(act, (int32_t)b, alist, n_alist);
 }
 
-// One workgroup for grids of at most 32 bricks per axis: the brick counts
-// (plus the zero sentinel) go to LDS, each thread scans a run of 33 of them
-// after a block scan of the run totals, and the occupied / active lists come
-// from one 32-bit occupancy word per brick row (i, j) (bit k), so a brick's
-// active test (it or a -1 neighbour occupied) is a shift and an OR of four
-// words.  The counts are zeroed behind the read, so the next binning on this
-// workspace needs no separate clear.
-constexpr int kScanThreads = 1024;
-constexpr int kScanRun = 33;
-constexpr int kScanEdge = 32;   // bricks per axis
+// Grids of at most 32 bricks per axis (one environment) scan in two
+// launches of one workgroup per x-slab of nb x nb bricks (brick (i, j, k) =
+// (i nb + j) nb + k, slab i, row (i, j)):
+//   k_bin_slab_scan   slab-local exclusive scan of the counts into start, the
+//                     slab's total, and one 32-bit occupancy word per row
+//                     (bit k: brick (i, j, k) holds particles); the counts are
+//                     zeroed behind the read, so the next binning on this
+//                     workspace needs no separate clear;
+//   k_bin_slab_lists  adds the earlier slabs' totals to start; every
+//                     workgroup scans the per-row counts of occupied and
+//                     active bricks (active: it or a -1 neighbour occupied, a
+//                     shift and an OR of four words) over all rows, and writes
+//                     its own rows' entries of both lists, so both come out
+//                     ascending.
+constexpr int kSlabThreads = kSlabMax * kSlabMax;
 
-__global__ __launch_bounds__(kScanThreads) void k_bin_scan_lists(
-    int nb, int32_t* __restrict__ count, int32_t* __restrict__ start, int32_t* __restrict__ list,
-    int32_t* __restrict__ n_list, int32_t* __restrict__ alist, int32_t* __restrict__ n_alist) {
-  // element e at e + e / 33: the 33 elements of a run are consecutive and
-  // consecutive runs start two banks apart
-  __shared__ int32_t c[kScanThreads * (kScanRun + 1)];
-  __shared__ uint32_t word[kScanThreads];
-  __shared__ int32_t wsum[kScanThreads / 64], wocc[kScanThreads / 64], wact[kScanThreads / 64];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int nbk = nb * nb * nb, ne = nbk + 1;
-  {
-    int v[kScanRun];
-#pragma unroll
-    for (int r = 0; r < kScanRun; ++r) {
-      const int e = t + r * kScanThreads;
-      v[r] = e < ne ? count[e] : 0;
-    }
-#pragma unroll
-    for (int r = 0; r < kScanRun; ++r) {
-      const int e = t + r * kScanThreads;
-      if (e < ne) count[e] = 0;
-      c[e + e / kScanRun] = v[r];
-    }
-  }
-  __syncthreads();
-  // run totals, block exclusive scan over the 1024 runs
-  int* run = c + t * (kScanRun + 1);
-  int tot = 0;
-#pragma unroll
-  for (int r = 0; r < kScanRun; ++r) tot += run[r];
-  int incl = tot;
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
   for (int d = 1; d < 64; d <<= 1) {
-    const int u = __shfl_up(incl, d);
-    if (lane >= d) incl += u;
+    const int u = __shfl_up(v, d);
+    if (lane >= d) v += u;
   }
+  return v;
+}
+
+__global__ __launch_bounds__(kSlabThreads) void k_bin_slab_scan(int nb, int32_t* __restrict__ count,
+                                                                int32_t* __restrict__ start,
+                                                                int32_t* __restrict__ slab_sum,
+                                                                uint32_t* __restrict__ row_occ) {
+  __shared__ int32_t wsum[kSlabThreads / 64];
+  __shared__ uint32_t word[kSlabMax];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int per = nb * nb;
+  const long long e = (long long)blockIdx.x * per + t;
+  if (t < kSlabMax) word[t] = 0u;
+  int v = 0;
+  if (t < per) {
+    v = count[e];
+    count[e] = 0;
+  }
+  const int incl = wave_incl_scan(v);
   if (lane == 63) wsum[w] = incl;
   __syncthreads();
-  int base = incl - tot;
+  if (v > 0) atomicOr(&word[t / nb], 1u << (t % nb));
+  int base = 0;
   for (int q = 0; q < w; ++q) base += wsum[q];
-#pragma unroll
-  for (int r = 0; r < kScanRun; ++r) {
-    const int v = run[r];
-    run[r] = base;
-    base += v;
-  }
+  if (t < per) start[e] = base + incl - v;
+  if (t == kSlabThreads - 1) slab_sum[blockIdx.x] = base + incl;
   __syncthreads();
-#pragma unroll
-  for (int r = 0; r < kScanRun; ++r) {
-    const int e = t + r * kScanThreads;
-    if (e < ne) start[e] = c[e + e / kScanRun];
+  if (t < nb) row_occ[blockIdx.x * nb + t] = word[t];
+}
+
+__global__ __launch_bounds__(kSlabThreads) void k_bin_slab_lists(
+    int nb, const int32_t* __restrict__ slab_sum, const uint32_t* __restrict__ row_occ,
+    int32_t* __restrict__ start, int32_t* __restrict__ list, int32_t* __restrict__ n_list,
+    int32_t* __restrict__ alist, int32_t* __restrict__ n_alist) {
+  __shared__ int32_t wocc[kSlabThreads / 64], wact[kSlabThreads / 64];
+  __shared__ int32_t slab_off;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int per = nb * nb, slab = blockIdx.x;
+  if (w == 0) {
+    int v = lane < slab ? slab_sum[lane] : 0;
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    if (lane == 0) slab_off = v;
   }
-  // occupancy word of brick row t = (i, j): bit k set when brick (i, j, k)
-  // holds particles
-  const int nrow = nb * nb;
-  uint32_t occ = 0u;
-  if (t < nrow) {
-    const int b0 = t * nb;
-    int prev = c[b0 + b0 / kScanRun];
-    for (int k = 0; k < nb; ++k) {
-      const int e = b0 + k + 1;
-      const int nxt = c[e + e / kScanRun];
-      if (nxt > prev) occ |= 1u << k;
-      prev = nxt;
-    }
-    word[t] = occ;
-  }
-  __syncthreads();
-  uint32_t act = 0u;
-  if (t < nrow) {
+  // row t = (i, j) over the whole grid
+  uint32_t occ = 0u, act = 0u;
+  if (t < per) {
     const int i = t / nb, j = t - i * nb;
+    occ = row_occ[t];
     uint32_t x = occ;
-    if (j > 0) x |= word[t - 1];
-    if (i > 0) x |= word[t - nb];
-    if (i > 0 && j > 0) x |= word[t - nb - 1];
+    if (j > 0) x |= row_occ[t - 1];
+    if (i > 0) x |= row_occ[t - nb];
+    if (i > 0 && j > 0) x |= row_occ[t - nb - 1];
     act = x | (x << 1);
     if (nb < 32) act &= (1u << nb) - 1u;
   }
-  // both lists ascending: per-row counts, a block scan, then each row writes
   const int no = __popc(occ), na = __popc(act);
-  int io = no, ia = na;
-  for (int d = 1; d < 64; d <<= 1) {
-    const int uo = __shfl_up(io, d), ua = __shfl_up(ia, d);
-    if (lane >= d) {
-      io += uo;
-      ia += ua;
-    }
-  }
+  const int io = wave_incl_scan(no), ia = wave_incl_scan(na);
   if (lane == 63) {
     wocc[w] = io;
     wact[w] = ia;
@@ -306,43 +290,50 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan_lists(
     bo += wocc[q];
     ba += wact[q];
   }
-  const int rb = t * nb;
-  for (uint32_t m = occ; m; m &= m - 1u) list[bo++] = rb + __ffs((int)m) - 1;
-  for (uint32_t m = act; m; m &= m - 1u) alist[ba++] = rb + __ffs((int)m) - 1;
-  if (t == kScanThreads - 1) {
-    *n_list = bo;
-    *n_alist = ba;
+  if (slab == 0 && t == kSlabThreads - 1) {
+    *n_list = bo + no;
+    *n_alist = ba + na;
+  }
+  if (t < per) start[(long long)slab * per + t] += slab_off;
+  if (slab == nb - 1 && t == 0) start[(long long)nb * per] = slab_off + slab_sum[slab];
+  if (t < per && t / nb == slab) {
+    const int rb = t * nb;
+    for (uint32_t m = occ; m; m &= m - 1u) list[bo++] = rb + __ffs((int)m) - 1;
+    for (uint32_t m = act; m; m &= m - 1u) alist[ba++] = rb + __ffs((int)m) - 1;
   }
 }
 
-// one workgroup when the grid is a single environment of at most 32 bricks
+// the slab scan when the grid is a single environment of at most 32 bricks
 // per axis; that scan leaves the counts zero
-static bool one_block_scan(const BinWs& ws, int nb) {
-  return nb <= kScanEdge && ws.n_bricks == (long long)nb * nb * nb;
+static bool slab_scan_fits(const BinWs& ws, int nb) {
+  return nb <= kSlabMax && ws.n_bricks == (long long)nb * nb * nb;
 }
 
-bool bin_counts_fusable(const BinWs& ws) { return one_block_scan(ws, ws.nb); }
+bool bin_counts_fusable(const BinWs& ws) { return slab_scan_fits(ws, ws.nb); }
 
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st,
                bool count_clear, bool counted) {
   const long long nbk = ws.n_bricks;
   const int nb = (P.n_grid + 3) >> 2;
-  const bool one_block = one_block_scan(ws, nb);
-  if (counted && !one_block) {
-    set_error("binning: fused counts need the one-workgroup scan");
+  const bool slab = slab_scan_fits(ws, nb);
+  if (counted && !slab) {
+    set_error("binning: fused counts need the slab scan");
     return DT_EINVAL;
   }
   if (!counted) {
-    if (!one_block || !count_clear)
+    if (!slab || !count_clear)
       hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nbk + 2)), dim3(kBlock), 0, st, nbk + 2,
                          ws.count, ws.n_list, ws.n_alist);
     if (n > 0)
       hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
                          ws.rank, ws.count);
   }
-  if (one_block) {
-    hipLaunchKernelGGL(k_bin_scan_lists, dim3(1), dim3(kScanThreads), 0, st, nb, ws.count,
-                       ws.start, ws.list, ws.n_list, ws.alist, ws.n_alist);
+  if (slab) {
+    hipLaunchKernelGGL(k_bin_slab_scan, dim3(nb), dim3(kSlabThreads), 0, st, nb, ws.count,
+                       ws.start, ws.slab_sum, ws.row_occ);
+    hipLaunchKernelGGL(k_bin_slab_lists, dim3(nb), dim3(kSlabThreads), 0, st, nb,
+                       (const int32_t*)ws.slab_sum, (const uint32_t*)ws.row_occ, ws.start,
+                       ws.list, ws.n_list, ws.alist, ws.n_alist);
   } else {
     size_t bytes = ws.scan_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(ws.scan_tmp, bytes, ws.count, ws.start, (int)(nbk + 1),

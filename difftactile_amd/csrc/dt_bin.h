@@ -11,6 +11,8 @@ namespace dt {
 
 constexpr int kTileEdge = 6;   // a brick's 4 cells plus the stencil's 2-node reach
 constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
+// bricks per axis up to which a single-environment grid uses the slab scan
+constexpr int kSlabMax = 32;
 
 // Views into one caller-provided workspace buffer.
 struct BinWs {
@@ -28,6 +30,8 @@ struct BinWs {
   float4* tiles;    // [n_bricks * kTileNodes] per-brick partial node sums
   void* scan_tmp;
   size_t scan_bytes;
+  int32_t* slab_sum;  // [kSlabMax]            slab scan: particles per x-slab
+  uint32_t* row_occ;  // [kSlabMax * kSlabMax] slab scan: occupancy word per brick row
   long long n_bricks;
   int nb;            // bricks along y and z
   // Optional binning history (one slot per substep of a trajectory): the
@@ -77,7 +81,7 @@ __device__ __forceinline__ int wave_run_rank(bool active, int key, int* counts) 
 constexpr int kBrickBlocks = 2048;
 // count -> brick scan -> occupied / active brick lists -> fill, on st.
 // count_clear: the workspace's brick counts are known to be zero (a previous
-// binning on it used the one-workgroup scan, which clears them behind it).
+// binning on it used the slab scan, which clears them behind it).
 // counted: key / rank / count already hold x's counts (made by the previous
 // substep's G2P); needs bin_counts_fusable.
 int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_t st,

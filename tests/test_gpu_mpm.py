@@ -447,7 +447,7 @@ def _bin_arrays(c_prm, n, ws, nbytes, n_bricks):
     return perm, start
 
 
-# 132^3 has 35,937 bricks: past the one-workgroup scan, so the device-wide
+# 132^3 has 35,937 bricks: past the slab scan, so the device-wide
 # scan and the flags kernel run
 @pytest.mark.parametrize("n_grid,n,sort", [(32, 20000, False), (64, 50000, True), (30, 7000, False),
                                           (132, 30000, True)])
@@ -489,7 +489,7 @@ def _active_list(c_prm, n, ws, nbytes):
 def test_saved_binning_restores_after_another(gpu, n_grid):
     """dt_mpm_bin_copy: a binning saved to a slot and copied back after the
     workspace binned other positions gives the first binning's perm, brick
-    starts and active-brick list again (one-workgroup and device-wide scans)."""
+    starts and active-brick list again (slab and device-wide scans)."""
     import ctypes
     prm = default_prm(n_grid)
     c_prm = prm.c_struct()
