@@ -1415,12 +1415,13 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
 // ----------------------------------------------------------------------------
 constexpr int kSlots = 64 * 27;
 struct BrickStage {
-  // particle q: pd[6q+0] = (wx0, wx1, wx2, b0), pd[6q+1] = (wy0.., b1),
-  // pd[6q+2] = (wz0.., b2), pd[6q+3..5] = M row-major (3 floats of padding).
+  // particle q at row r = stage_row(q): pd[r+0] = (wx0, wx1, wx2, b0),
+  // pd[r+1] = (wy0.., b1), pd[r+2] = (wz0.., b2), pd[r+3..5] = M row-major
+  // (3 floats of padding).
   // Before the staging of a chunk the same LDS holds the brick's 6x6x6-node
   // grid-velocity tile read by the g2p adjoint's gather (gtile).
   union {
-    float4 pd[6 * kBlock];
+    float4 pd[6 * kBlock + kBlock / 8];
     float4 gtile[kTileNodes];
   };
   float4 slot[kSlots + 1];   // (cell, j, k, i): (sum w u0, w u1, w u2, w); last stays 0
@@ -1429,6 +1430,13 @@ struct BrickStage {
   int cells[64];             // occupied cells of the chunk
   int n_cells;
 };
+
+// First float4 of staged particle q.  Six float4 per particle put particles
+// q and q + 8 on the same LDS banks; the lanes of one wavefront in
+// brick_items read the i-th particle of several cells at once, and cells
+// hold ~8 particles, so one float4 of skew per 8 particles spreads those
+// reads over the banks.
+__device__ __forceinline__ int stage_row(int q) { return 6 * q + (q >> 3); }
 
 __device__ __forceinline__ int local_cell(const Stencil& s) {
   return ((s.base[0] & 3) * 4 + (s.base[1] & 3)) * 4 + (s.base[2] & 3);
@@ -1452,7 +1460,7 @@ __device__ __forceinline__ void chunk_scan(BrickStage& sh) {
 
 __device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Stencil& s,
                                                const float (&b)[3], const M3& M) {
-  float4* q = sh.pd + 6 * t;
+  float4* q = sh.pd + stage_row(t);
   q[0] = make_float4(s.w[0][0], s.w[1][0], s.w[2][0], b[0]);
   q[1] = make_float4(s.w[0][1], s.w[1][1], s.w[2][1], b[1]);
   q[2] = make_float4(s.w[0][2], s.w[1][2], s.w[2][2], b[2]);
@@ -1472,8 +1480,8 @@ __device__ __forceinline__ void brick_items(BrickStage& sh) {
     const int q0 = sh.cs[cell], q1 = sh.cs[cell + 1];
     float4 acc0 = make_float4(0.f, 0.f, 0.f, 0.f), acc1 = acc0, acc2 = acc0;
     for (int q = q0; q < q1; ++q) {
-      const float4* pq = sh.pd + 6 * q;
-      const float* fq = pf + 24 * q;
+      const float4* pq = sh.pd + stage_row(q);
+      const float* fq = pf + 4 * stage_row(q);
       const float4 X = pq[0], M0 = pq[3], M1 = pq[4];
       const float wy = fq[4 + j], b1 = fq[7], wz = fq[8 + k], b2 = fq[11], m22 = fq[20];
       const float wyz = wy * wz;
