@@ -122,6 +122,7 @@ _SIGS = {
     "dt_mpm_g2p4_adj_binned": (_c_int, [_P(DtMpmParams), _c_int] + [_vp] * 6 + [_vp, _sz, _vp]),
     "dt_mpm_grid_op4_adj_tiles": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _vp, _vp,
                                            _vp]),
+    "dt_mpm_grid_op4_adj_tiles_f": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz] + [_vp] * 5 + [_vp]),
     "dt_mpm_brick_count": (ctypes.c_int64, [_P(DtMpmParams)]),
     "dt_mpm_mark_bricks": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp]),
     "dt_mpm_zero4_bricks": (_c_int, [_P(DtMpmParams), _vp, _vp, _vp]),

@@ -263,12 +263,14 @@ class CoupledSim:
             prm_clamp = self._node_set(prm_clamp, ws, nb)
             _lib.call("dt_mpm_g2p4_adj_binned", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]),
                       P(g_next[1]), P(g_next[2]), P(g[0]), ws, nb, s)
-            _lib.call("dt_mpm_tiles_sum", prm, sim.n, ws, nb, P(ggv4), s)
+            # tile sums fused into the grid_op adjoint
+            _lib.call("dt_mpm_grid_op4_adj_tiles_f", prm_clamp, sim.n, ws, nb, P(g4), P(gf), None,
+                      P(gg4), P(ggf), s)
         else:
             _lib.call("dt_mpm_zero4", nn, P(ggv4), s)
             _lib.call("dt_mpm_g2p4_adj", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]), P(g_next[1]),
                       P(g_next[2]), P(g[0]), P(ggv4), s)
-        _lib.call("dt_mpm_grid_op4_adj", prm_clamp, P(g4), P(gf), P(ggv4), P(gg4), P(ggf), s)
+            _lib.call("dt_mpm_grid_op4_adj", prm_clamp, P(g4), P(gf), P(ggv4), P(gg4), P(ggf), s)
         if self.clamp_grad:
             for k in range(len(self.pads)):
                 for t in (g_pads[k][0], g_pads[k][1]):

@@ -1225,6 +1225,41 @@ __global__ __launch_bounds__(kBlock) void k_grid_op4_adj_tiles(MpmP P,
   }
 }
 
+// The same with the contact force grid gf (as dt_mpm_grid_op4_adj): writes
+// gg4 and ggf (d/d gf), and the summed gv4 when it is non-NULL.
+__global__ __launch_bounds__(kBlock) void k_grid_op4_adj_tiles_f(
+    MpmP P, const int32_t* __restrict__ alist, const int32_t* __restrict__ n_alist,
+    const float4* __restrict__ g4, const float* __restrict__ gf, const float4* __restrict__ tiles,
+    const int32_t* __restrict__ start, float4* __restrict__ gv4, float4* __restrict__ gg4,
+    float* __restrict__ ggf) {
+  DT_ACTIVE_NODES(nd) {
+#pragma clang fp contract(off)
+    const float4 gi = tile_sum(P, nd, tiles, start);
+    if (gv4) gv4[nd] = gi;
+    const float4 q = g4[nd];
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    float of0 = 0.f, of1 = 0.f, of2 = 0.f;
+    if (q.w > P.eps) {
+      const float f0 = gf[3 * nd], f1 = gf[3 * nd + 1], f2 = gf[3 * nd + 2];
+      float vo[3];
+      bool mask[3];
+      grid_node_fwd(P, nd, q.x, q.y, q.z, q.w, f0, f1, f2, vo, mask);
+      float g0 = mask[0] ? 0.f : gi.x, g1 = mask[1] ? 0.f : gi.y, g2 = mask[2] ? 0.f : gi.z;
+      float inv_m = 1.f / (q.w + P.eps);
+      float gmm = -inv_m * inv_m * (g0 * (q.x + f0) + g1 * (q.y + f1) + g2 * (q.z + f2));
+      if (P.clamp_grad) gmm = fminf(fmaxf(gmm, -P.grad_clamp), P.grad_clamp);
+      out = make_float4(inv_m * g0, inv_m * g1, inv_m * g2, gmm);
+      of0 = inv_m * g0;
+      of1 = inv_m * g1;
+      of2 = inv_m * g2;
+    }
+    gg4[nd] = out;
+    ggf[3 * nd] = of0;
+    ggf[3 * nd + 1] = of1;
+    ggf[3 * nd + 2] = of2;
+  }
+}
+
 #undef DT_ACTIVE_NODES
 
 __global__ __launch_bounds__(kBlock) void k_zero_i32(long long n, int32_t* __restrict__ a) {
@@ -2625,6 +2660,22 @@ int dt_mpm_grid_op4_adj_tiles(const dt_mpm_params* prm, int32_t n, const void* w
                      (const int32_t*)ws.n_alist, (const float4*)grid4, (const float4*)ws.tiles,
                      (const int32_t*)ws.start, (float4*)g_grid_v4, (float4*)g_grid4);
   return check_launch("dt_mpm_grid_op4_adj_tiles");
+}
+
+int dt_mpm_grid_op4_adj_tiles_f(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                                size_t workspace_bytes, const float* grid4, const float* grid_f,
+                                float* g_grid_v4, float* g_grid4, float* g_grid_f, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(grid4 && grid_f && g_grid4 && g_grid_f, "NULL grid pointer");
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
+  hipLaunchKernelGGL(k_grid_op4_adj_tiles_f, dim3(active_blocks(ws)), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.alist,
+                     (const int32_t*)ws.n_alist, (const float4*)grid4, grid_f,
+                     (const float4*)ws.tiles, (const int32_t*)ws.start, (float4*)g_grid_v4,
+                     (float4*)g_grid4, g_grid_f);
+  return check_launch("dt_mpm_grid_op4_adj_tiles_f");
 }
 
 int dt_mpm_mark_bricks(const dt_mpm_params* prm, int32_t n, const float* x, int32_t* flags,

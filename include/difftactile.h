@@ -352,6 +352,14 @@ int dt_mpm_g2p4_adj_binned(const dt_mpm_params* prm, int32_t n, const float* x,
 int dt_mpm_grid_op4_adj_tiles(const dt_mpm_params* prm, int32_t n, const void* workspace,
                               size_t workspace_bytes, const float* grid4, float* g_grid_v4,
                               float* g_grid4, void* stream);
+/* The same with the contact force grid of dt_mpm_grid_op4_adj (grid_f, 3
+   floats per node): writes g_grid4 and g_grid_f on the active bricks, and the
+   summed d/d grid_v_out into g_grid_v4 unless it is NULL.  The coupled
+   substep's adjoint (grasp_plastic.py:167-178) uses it after
+   dt_mpm_g2p4_adj_binned. */
+int dt_mpm_grid_op4_adj_tiles_f(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                                size_t workspace_bytes, const float* grid4, const float* grid_f,
+                                float* g_grid_v4, float* g_grid4, float* g_grid_f, void* stream);
 
 /* Active-brick passes on explicit flags. */
 int64_t dt_mpm_brick_count(const dt_mpm_params* prm);

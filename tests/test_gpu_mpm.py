@@ -532,7 +532,9 @@ def test_node_set_passes_cover_the_active_bricks_only(gpu, n_env):
     particle's 3x3x3 stencil touches (and only those or their stencil-side
     neighbours, each once).  dt_mpm_zero_nodes and dt_mpm_grid_op4 restricted
     to it write exactly the listed bricks' nodes, equal bit for bit to the
-    dense pass there, and leave every other node as it was."""
+    dense pass there, and leave every other node as it was; so does the
+    tile-summing grid_op adjoint with contact forces against tiles_sum +
+    grid_op4_adj."""
     import ctypes
     n_grid, n = 32, 4000
     prm = default_prm(n_grid)
@@ -601,6 +603,23 @@ def test_node_set_passes_cover_the_active_bricks_only(gpu, n_env):
         hs, hd = a_s.cpu(), a_d.cpu()
         assert torch.equal(hs[sel], hd[sel]), (hs[sel] - hd[sel]).abs().max()
         assert (hs[~sel] == 9.0).all()
+    # grid_op adjoint with the g2p-adjoint tile sums fused in (contact force
+    # form, gradient clamp on): equal to tiles_sum + grid_op4_adj
+    g2 = lambda *sh: torch.randn(*sh, device="cuda")
+    call("dt_mpm_g2p4_adj_binned", c_prm, n, dev(x), g2(nn, 4), g2(n, 3), g2(n, 3), g2(n, 3, 3),
+         torch.empty(n, 3, device="cuda"), ws, nbytes, _lib.stream())
+    c_clamp = type(c_set).from_buffer_copy(c_set)
+    c_clamp.clamp_grad, c_clamp.grad_clamp = 1, 0.5
+    ggv = torch.full((nn, 4), 9.0, device="cuda")
+    call("dt_mpm_tiles_sum", c_clamp, n, ws, nbytes, ggv, _lib.stream())
+    ref = [torch.full((nn, 4), 9.0, device="cuda"), torch.full((nn, 3), 9.0, device="cuda")]
+    call("dt_mpm_grid_op4_adj", c_clamp, g4, gf, ggv, *ref, _lib.stream())
+    got = [torch.full((nn, 4), 9.0, device="cuda"), torch.full((nn, 3), 9.0, device="cuda")]
+    gvs = torch.full((nn, 4), 9.0, device="cuda")
+    call("dt_mpm_grid_op4_adj_tiles_f", c_clamp, n, ws, nbytes, g4, gf, gvs, *got, _lib.stream())
+    assert (ref[0][:, 3].abs() == 0.5).any()
+    for a_g, a_r in zip(got + [gvs], ref + [ggv]):
+        assert torch.equal(a_g.cpu(), a_r.cpu())
 
 
 def test_workspace_too_small_is_rejected(gpu):
