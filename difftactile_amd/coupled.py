@@ -243,9 +243,9 @@ class CoupledSim:
         g_ext = []
         for k, pad in enumerate(self.pads):
             gp, gvu, gvp = g_pads[k]
-            for t in (gp, gvu, gvp):
-                t.zero_()
-            ge = torch.zeros(pad.n_verts, 3, device=sim.device)
+            ge = torch.empty(pad.n_verts, 3, device=sim.device)
+            # the adjoint kernels below accumulate: one launch clears all four
+            torch._foreach_zero_([gp, gvu, gvp, ge])
             gpn, gvn, gvpn = g_pads_next[k]
             _lib.call("dt_fem_update2_adj", pad.fem, pad.n_verts, P(pad.layer, I32), P(gpn),
                       P(gvn), P(gvpn), P(gp), P(gvu), P(gvp), P(ge), P(g_cv[k]), P(g_cv1[k]), s)
