@@ -210,7 +210,7 @@ def algorithmic_bytes(n, active, has_mat):
     }
 
 
-def cpu_baseline(sample, segment_steps=4):
+def cpu_baseline(sample, segment_steps=16):
     """Oracle (torch, float32, host cores) fwd+bwd substeps on a sample of the workload."""
     import torch
     from difftactile_amd import scenes
