@@ -119,6 +119,7 @@ _SIGS = {
                           + [_vp, _sz, _vp]),
     "dt_mpm_grid_op4_tiles": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _vp, _vp]),
     "dt_mpm_tiles_sum": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _vp]),
+    "dt_mpm_tiles_sum_zero": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _vp, _c_int, _vp]),
     "dt_mpm_g2p4_adj_binned": (_c_int, [_P(DtMpmParams), _c_int] + [_vp] * 6 + [_vp, _sz, _vp]),
     "dt_mpm_grid_op4_adj_tiles": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _vp, _vp,
                                            _vp]),

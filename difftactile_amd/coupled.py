@@ -185,7 +185,8 @@ class CoupledSim:
             prm = self._node_set(prm, ws, nb)
             _lib.call("dt_mpm_p2g_binned", prm, mats, nm, sim.n, mid, *[P(t) for t in st],
                       P(nxt[3]), ws, nb, s)
-            _lib.call("dt_mpm_tiles_sum", prm, sim.n, ws, nb, P(g4), s)
+            # the tile sums, with the contact force grid cleared on the same nodes
+            _lib.call("dt_mpm_tiles_sum_zero", prm, sim.n, ws, nb, P(g4), P(gf), 3, s)
         else:
             _lib.call("dt_mpm_zero4", nn, P(g4), s)
             _lib.call("dt_mpm_p2g_fused", prm, mats, nm, sim.n, mid, *[P(t) for t in st], P(g4),
@@ -208,9 +209,7 @@ class CoupledSim:
             _lib.call("dt_contact_closest_batched", prm, g4.data_ptr() + 12, 4,
                       None if pad.tri_start is None else P(pad.tri_start, I32), pad.n_tris,
                       P(pad.cen), P(pad.tri_bounds), P(hint, I32), P(idx, I32), self.npad, k, s)
-        if sim.sparse_grid:
-            _lib.call("dt_mpm_zero_nodes", prm, P(gf), 3, s)
-        else:
+        if not sim.sparse_grid:
             gf.zero_()
         exts = []
         for k, pad in enumerate(self.pads):

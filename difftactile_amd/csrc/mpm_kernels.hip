@@ -1173,6 +1173,19 @@ __global__ __launch_bounds__(kBlock) void k_tiles_sum(MpmP P, const int32_t* __r
   DT_ACTIVE_NODES(nd) { out4[nd] = tile_sum(P, nd, tiles, start); }
 }
 
+// the same, also clearing width floats per node of zbuf on the same nodes
+__global__ __launch_bounds__(kBlock) void k_tiles_sum_zero(MpmP P, const int32_t* __restrict__ alist,
+                                                           const int32_t* __restrict__ n_alist,
+                                                           const float4* __restrict__ tiles,
+                                                           const int32_t* __restrict__ start,
+                                                           float4* __restrict__ out4,
+                                                           float* __restrict__ zbuf, int width) {
+  DT_ACTIVE_NODES(nd) {
+    out4[nd] = tile_sum(P, nd, tiles, start);
+    for (int c = 0; c < width; ++c) zbuf[nd * width + c] = 0.f;
+  }
+}
+
 // grid_op over the active bricks with (mv, m) summed from the p2g tiles;
 // writes the summed grid4 and grid_v4
 __global__ __launch_bounds__(kBlock) void k_grid_op4_tiles(MpmP P, const int32_t* __restrict__ alist,
@@ -2627,6 +2640,22 @@ int dt_mpm_tiles_sum(const dt_mpm_params* prm, int32_t n, const void* workspace,
                      to_dev(*prm), (const int32_t*)ws.alist, (const int32_t*)ws.n_alist,
                      (const float4*)ws.tiles, (const int32_t*)ws.start, (float4*)out4);
   return check_launch("dt_mpm_tiles_sum");
+}
+
+int dt_mpm_tiles_sum_zero(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                          size_t workspace_bytes, float* out4, float* zero_buf, int32_t width,
+                          void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(out4 && zero_buf, "NULL grid pointer");
+  DT_REQUIRE(width >= 1 && width <= 4, "width must be 1..4");
+  BinWs ws{};
+  if ((rc = workspace_views(prm, n, const_cast<void*>(workspace), workspace_bytes, ws))) return rc;
+  hipLaunchKernelGGL(k_tiles_sum_zero, dim3(active_blocks(ws)), dim3(kBlock), 0,
+                     DT_STREAM(stream), to_dev(*prm), (const int32_t*)ws.alist,
+                     (const int32_t*)ws.n_alist, (const float4*)ws.tiles, (const int32_t*)ws.start,
+                     (float4*)out4, zero_buf, width);
+  return check_launch("dt_mpm_tiles_sum_zero");
 }
 
 int dt_mpm_g2p4_adj_binned(const dt_mpm_params* prm, int32_t n, const float* x,

@@ -341,6 +341,12 @@ int dt_mpm_grid_op4_tiles(const dt_mpm_params* prm, int32_t n, const void* works
    (d/d v, 0)) into out4 on the active bricks; other nodes are untouched. */
 int dt_mpm_tiles_sum(const dt_mpm_params* prm, int32_t n, const void* workspace,
                      size_t workspace_bytes, float* out4, void* stream);
+/* dt_mpm_tiles_sum that also zeroes width (1..4) floats per node of zero_buf
+   on the same nodes (the coupled substep clears its contact force grid
+   this way). */
+int dt_mpm_tiles_sum_zero(const dt_mpm_params* prm, int32_t n, const void* workspace,
+                          size_t workspace_bytes, float* out4, float* zero_buf, int32_t width,
+                          void* stream);
 /* Brick-owned g2p adjoint over the binned particles: OVERWRITES gx with its
    partial and writes the grid-velocity adjoint tiles. */
 int dt_mpm_g2p4_adj_binned(const dt_mpm_params* prm, int32_t n, const float* x,

@@ -612,6 +612,12 @@ def test_node_set_passes_cover_the_active_bricks_only(gpu, n_env):
     c_clamp.clamp_grad, c_clamp.grad_clamp = 1, 0.5
     ggv = torch.full((nn, 4), 9.0, device="cuda")
     call("dt_mpm_tiles_sum", c_clamp, n, ws, nbytes, ggv, _lib.stream())
+    # tile sums with a buffer cleared on the same nodes
+    ggv_z, zb = torch.full((nn, 4), 9.0, device="cuda"), torch.full((nn, 3), 7.0, device="cuda")
+    call("dt_mpm_tiles_sum_zero", c_clamp, n, ws, nbytes, ggv_z, zb, 3, _lib.stream())
+    assert torch.equal(ggv_z.cpu(), ggv.cpu())
+    hz = zb.cpu()
+    assert (hz[sel] == 0).all() and (hz[~sel] == 7.0).all()
     ref = [torch.full((nn, 4), 9.0, device="cuda"), torch.full((nn, 3), 9.0, device="cuda")]
     call("dt_mpm_grid_op4_adj", c_clamp, g4, gf, ggv, *ref, _lib.stream())
     got = [torch.full((nn, 4), 9.0, device="cuda"), torch.full((nn, 3), 9.0, device="cuda")]
