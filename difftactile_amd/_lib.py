@@ -74,6 +74,7 @@ class DtContactParams(ctypes.Structure):
     _fields_ = [
         ("kn", ctypes.c_float), ("kd", ctypes.c_float), ("kt", ctypes.c_float),
         ("mu_f", ctypes.c_float), ("norm_eps", ctypes.c_float), ("sdf_offset", ctypes.c_float),
+        ("unclamped_vn", ctypes.c_int32),
     ]
 
 

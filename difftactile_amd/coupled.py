@@ -218,6 +218,8 @@ class CoupledSim:
                       P(idx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
                       P(vel_u[k]), pad.od(), P(gf), P(ext), s)
             exts.append(ext)
+        if store is not None:
+            store["ext"] = exts
         _lib.call("dt_mpm_grid_op4", prm, P(g4), P(gf), P(gv4), s)
         _lib.call("dt_mpm_g2p4", prm, sim.n, P(st[0]), P(gv4), P(nxt[0]), P(nxt[1]), P(nxt[2]), s)
         for k, pad in enumerate(self.pads):

@@ -4,7 +4,9 @@
 //   find_closest            sensor_model/fem_sensor.py:343-359
 //   find_sdf                sensor_model/fem_sensor.py:361-394
 //   update_contact_force    sensor_model/fem_sensor.py:401-406, object_model/mpm_elastic.py:130-132
-//   calculate_contact_force tasks/grasp_plastic.py:204-216
+//   calculate_contact_force tasks/grasp_plastic.py:204-216 (grasp form) and
+//                           tasks/cable_straightening.py:211-223 (cable form:
+//                           no max(., 0) on n.v_rel, shear norm without eps)
 //   check_collision         tasks/grasp_plastic.py:269-276
 //   collision               tasks/grasp_plastic.py:279-299
 //
@@ -19,6 +21,7 @@ namespace dt {
 
 struct ContactP {
   float kn, kd, kt, mu_f, norm_eps, sdf_offset;
+  int unclamped_vn;   // 1: the cable form (no max(., 0) on the normal velocity)
 };
 
 // Lower bound on the squared centroid distance from the expanded form,
@@ -364,9 +367,10 @@ __device__ __forceinline__ void contact_fwd(const ContactP& C, const float (&q)[
     w.r[a] = -rel;
   }
   w.flag = (w.sdf < 0.f) && (u >= 0.f) && (v >= 0.f) && (u + v <= 1.f);
-  // calculate_contact_force(sdf, n, r)
+  // calculate_contact_force(sdf, n, r): grasp form, or the cable form when
+  // C.unclamped_vn (with C.norm_eps = 0)
   w.vnr = w.n[0] * w.r[0] + w.n[1] * w.r[1] + w.n[2] * w.r[2];
-  w.vn = fmaxf(w.vnr, 0.f);
+  w.vn = C.unclamped_vn ? w.vnr : fmaxf(w.vnr, 0.f);
   float kk = -(C.kn + C.kd * w.vn) * w.sdf;
   for (int a = 0; a < 3; ++a) w.fn[a] = kk * w.n[a];
   for (int a = 0; a < 3; ++a) w.vt[a] = w.r[a] - w.vnr * w.n[a];
@@ -454,7 +458,7 @@ __device__ __forceinline__ void contact_adj_core(const ContactP& C, const float 
   for (int a = 0; a < 3; ++a) gn[a] += kk2 * gfn[a];
   float gsdf = -(C.kn + C.kd * w.vn) * gfn_n;
   float gvn = -C.kd * w.sdf * gfn_n;
-  if (w.vnr > 0.f) gvnr += gvn;
+  if (C.unclamped_vn || w.vnr > 0.f) gvnr += gvn;
   // vnr = n . r
   for (int a = 0; a < 3; ++a) {
     gn[a] += gvnr * w.r[a];
@@ -623,6 +627,7 @@ static ContactP contact_dev(const dt_contact_params* c) {
   d.mu_f = c->mu_f;
   d.norm_eps = c->norm_eps;
   d.sdf_offset = c->sdf_offset;
+  d.unclamped_vn = c->unclamped_vn ? 1 : 0;
   return d;
 }
 

@@ -127,9 +127,12 @@ class _RopeStep(torch.autograd.Function):
 
 
 def _contact_params(kn, kd, kt, mu_f, offset):
+    """The cable task's contact force (cable_straightening.py:211-223): the
+    normal velocity is not clipped at zero and the shear norm has no eps."""
     cp = _lib.DtContactParams()
     cp.kn, cp.kd, cp.kt, cp.mu_f = kn, kd, kt, mu_f
-    cp.norm_eps, cp.sdf_offset = 1e-11, float(offset)
+    cp.norm_eps, cp.sdf_offset = 0.0, float(offset)
+    cp.unclamped_vn = 1
     return cp
 
 

@@ -90,7 +90,8 @@ def config4_population(n_env=64, seed=0, dt=1e-5, pads=False):
 
     pads: each environment also gets a slab pad of 16 x 16 x 9 vertices
     (10,800 tets, spacing 1/128, stable Neo-Hookean E = 1e5, nu = 0.45) resting
-    a quarter cell above its ball and driven down at 0.4 m/s (the C2 press
+    a quarter cell above its ball, off the grid's cell centres by (0.21, 0.43)
+    of its spacing, and driven down at 0.4 m/s (the C2 press
     rate); the dict then carries pad_mesh (nodes, f2v, tris, layer), pad_out,
     pad (mu, lam, mass) and pad_cv (the per-vertex control velocity)."""
     rng = np.random.default_rng(seed)
@@ -111,9 +112,10 @@ def config4_population(n_env=64, seed=0, dt=1e-5, pads=False):
             from .sensor_model.meshes import slab_mesh
             h = 1.0 / 128
             top = float(np.max(x[:, 1]))
+            # moved off the 128^3 cell centres by (0.21, 0.43) h (see config2_pad_press)
             nodes, f2v, tris, layer, od = slab_mesh(16, 16, 9, h=h,
-                                                    origin=(0.5 - 7.5 * h, top + 0.25 * dx,
-                                                            0.5 - 7.5 * h))
+                                                    origin=(0.5 - 7.29 * h, top + 0.25 * dx,
+                                                            0.5 - 7.07 * h))
             E_p, nu_p = 1e5, 0.45
             vol = (15 * h) * (15 * h) * (8 * h)
             cv = np.zeros((len(nodes), 3), np.float32)
@@ -131,7 +133,8 @@ def identity_F(n):
     return np.tile(np.eye(3, dtype=np.float32), (n, 1, 1))
 
 
-def config2_pad_press(n_particles=200_000, n_grid=128, dt=1e-5, n_substeps=500, seed=0):
+def config2_pad_press(n_particles=200_000, n_grid=128, dt=1e-5, n_substeps=500, seed=0,
+                      pad_verts=(40, 40, 6), press_substeps=None):
     """Slab pad pressing an elastoplastic block.
 
     The object constants of the grasp task (``tasks/grasp_plastic.py:35-46``:
@@ -147,6 +150,9 @@ def config2_pad_press(n_particles=200_000, n_grid=128, dt=1e-5, n_substeps=500, 
     block top; the fixed top layer is driven down 2 mm at constant velocity
     over n_substeps.  Penalty contact kn = 1e4 (kd, kt, friction from the
     grasp task).
+    pad_verts = (nx, nz, ny) vertices of the slab; press_substeps: the press
+    rate is 2 mm over that many substeps (default n_substeps), so a shorter
+    run of the same scene keeps the rate.
     Returns a dict with everything the coupled runner needs.
     """
     from .params import VON_MISES
@@ -172,13 +178,23 @@ def config2_pad_press(n_particles=200_000, n_grid=128, dt=1e-5, n_substeps=500, 
     mats = [Material(5e4, 0.3, VON_MISES, yield_stress=50.0)]
     # pad
     h = dx
-    ox = 0.5 * space - 19.5 * h
-    nodes, f2v, tris, layer, out_dir = slab_mesh(40, 40, 6, h=h, origin=(ox, top + 0.25 * dx, ox))
+    pnx, pnz, pny = pad_verts
+    # centred, then moved by (0.21, 0.43) h.  Centred, a pad of an odd vertex
+    # count has its vertices exactly at the cell centres where the contact
+    # queries sit, and the closest triangle and the inside test are decided by
+    # rounding alone; other shifts leave queries equidistant from two triangle
+    # centroids (a shift of (0.3, 0.1) h does).  With this one the nearest
+    # centroid of a flat pad is 0.27 h away from a query and the next 0.47 h,
+    # and every query is 0.15 h or more from a triangle edge.
+    ox = 0.5 * space - 0.5 * (pnx - 1) * h + 0.21 * h
+    oz = 0.5 * space - 0.5 * (pnz - 1) * h + 0.43 * h
+    nodes, f2v, tris, layer, out_dir = slab_mesh(pnx, pnz, pny, h=h,
+                                                 origin=(ox, top + 0.25 * dx, oz))
     E, nu, rho_pad = 1e5, 0.45, 2000.0
     mu, lam = E / 2 / (1 + nu), E * nu / (1 + nu) / (1 - 2 * nu)
-    vol = (39 * h) * (39 * h) * (5 * h)
+    vol = ((pnx - 1) * h) * ((pnz - 1) * h) * ((pny - 1) * h)
     mass = rho_pad * vol / len(nodes)
-    press = 0.2 / (n_substeps * dt)
+    press = 0.2 / ((press_substeps or n_substeps) * dt)
     cv = np.zeros((n_substeps + 1, len(nodes), 3), np.float32)
     cv[:, :, 1] = -press
     # camera: 1 cm behind the contact face, looking at it (pad-local +y toward the face)

@@ -106,8 +106,12 @@ typedef struct dt_fem_params {
 /* Penalty contact constants (grasp_plastic.py:57-64, 96). */
 typedef struct dt_contact_params {
   float kn, kd, kt, mu_f; /* normal stiffness, normal damping, shear stiffness, friction */
-  float norm_eps;         /* eps of the shear-velocity norm (1e-11)                   */
+  float norm_eps;         /* eps of the shear-velocity norm: 1e-11 in the grasp form
+                             (grasp_plastic.py:212), 0 in the cable form
+                             (cable_straightening.py:219, .norm() without eps)       */
   float sdf_offset;       /* find_sdf offset (0 for the grid contact)                 */
+  int32_t unclamped_vn;   /* 0: v_n = max(n.v_rel, 0) (grasp form, grasp_plastic.py:209);
+                             1: v_n = n.v_rel (cable form, cable_straightening.py:216)  */
 } dt_contact_params;
 
 const char* dt_last_error(void);

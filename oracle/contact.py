@@ -12,8 +12,11 @@ Pinning (tests/test_oracle_contact.py):
   find_sdf        -- closed forms: signed plane distance, barycentric
                      coordinates reproduce the projected point.
   contact_force   -- closed forms of the normal / tangential split; Coulomb
-                     cap; FP64 central differences away from the kinks.
-  collision       -- Newton's third law: sum(grid_f)/dt + sum(ext_force) = 0.
+                     cap; the grasp and cable forms told apart by a separating
+                     normal velocity and by a shear speed just below 1e-4.
+  collision       -- Newton's third law: sum(grid_f)/dt + sum(ext_force) = 0;
+                     the query velocity mv / (m + 1e-5) through closed-form
+                     forces under a flat pad (rest, pure approach, pure shear).
 """
 import numpy as np
 import torch
@@ -102,22 +105,31 @@ def find_sdf(q, qv, idx, pos, vel, tris, out_dir, eps=1e-10, offset=0.0):
     return sdf, -n, rel_v, flag
 
 
-def contact_force(sdf, norm_v, rel_v, kn, kd, kt, mu_f, norm_eps=1e-11):
-    """Penalty contact force, grasp variant (``grasp_plastic.py:204-216``).
+def contact_force(sdf, norm_v, rel_v, kn, kd, kt, mu_f, norm_eps=1e-11, clamp_vn=True):
+    """Penalty contact force.
 
+    Grasp form (``grasp_plastic.py:204-216``, the defaults):
     v_n = max(n . v_rel, 0); f_n = -(kn + kd v_n) sdf n;
     v_t = v_rel - (n . v_rel) n; |v_t| = sqrt(v_t.v_t + 1e-11);
     if |v_t| > 1e-4: f_t = v_t / |v_t| * min(kt |v_t|, mu_f |f_n|) (|f_n| without eps);
     f = f_n + f_t.
+
+    Cable form (``cable_straightening.py:211-223``: ``clamp_vn=False,
+    norm_eps=0``): v_n = n . v_rel (negative when separating) and
+    |v_t| = sqrt(v_t.v_t) (Taichi's ``.norm()`` without eps).
     """
     vn_raw = (norm_v * rel_v).sum(-1)
-    vn = torch.clamp(vn_raw, min=0)
+    vn = torch.clamp(vn_raw, min=0) if clamp_vn else vn_raw
     fn = -(kn + kd * vn)[:, None] * sdf[:, None] * norm_v
     vt = rel_v - vn_raw[:, None] * norm_v
-    vtn = torch.sqrt((vt * vt).sum(-1) + norm_eps)
+    s2 = (vt * vt).sum(-1) + norm_eps
+    shear = torch.sqrt(s2.detach()) > 1e-4
+    # the unselected branch is evaluated on 1 so that sqrt and the division
+    # stay finite in reverse mode (no eps: |v_t| may be 0 there)
+    vtn = torch.sqrt(torch.where(shear, s2, torch.ones_like(s2)))
     fnn = torch.sqrt((fn * fn).sum(-1))
     mag = torch.minimum(kt * vtn, mu_f * fnn)
-    ft = torch.where((vtn > 1e-4)[:, None], vt / vtn[:, None] * mag[:, None], torch.zeros_like(vt))
+    ft = torch.where(shear[:, None], vt / vtn[:, None] * mag[:, None], torch.zeros_like(vt))
     return fn + ft
 
 

@@ -27,10 +27,12 @@ class _ClampGrad(torch.autograd.Function):
         return torch.clamp(g, -1000.0, 1000.0)
 
 
-def coupled_substep(prm, state, mats, pads, cp, clamp=False):
+def coupled_substep(prm, state, mats, pads, cp, clamp=False, info=None):
     """state = (x, v, C, F); pads = list of dicts (pos, vel, vpos, cv, cv_next, f2v, B,
     layer, tris, out_dir, mu, lam, mass, dt); cp = (kn, kd, kt, mu_f).
-    Returns (next state, list of (pos', vel', vpos'), contact_idx, nodes)."""
+    Returns (next state, list of (pos', vel', vpos'), contact_idx, nodes).
+    info: a dict that receives the contact impulse grid ("grid_f") and the pads'
+    external forces ("ext")."""
     x, v, C, F = state
     Fn = om.compute_new_F(C, F, prm.dt)
     U, s, V = om.svd3(Fn, clamp=1e-30 if getattr(prm, "svd_exact", False) else 1e-8)
@@ -41,6 +43,8 @@ def coupled_substep(prm, state, mats, pads, cp, clamp=False):
     sensors = [(p["pos"], vu, p["tris"], p["out_dir"]) for p, vu in zip(pads, vel_u)]
     kn, kd, kt, mu_f = cp
     grid_f, exts = oc.collision(prm, mv, m, nodes, idx, sensors, kn, kd, kt, mu_f)
+    if info is not None:
+        info["grid_f"], info["ext"] = grid_f, exts
     m_g = _ClampGrad.apply(m) if clamp else m
     gv = om.grid_op(prm, mv, m_g, grid_f)
     nx, nv, nC = om.g2p(prm, x, gv)

@@ -25,7 +25,9 @@ Pinning (tests/test_oracle_rope.py):
   * FP64 central differences of a random linear functional of one substep
     w.r.t. positions, velocities and external forces;
   * contact: zero force outside the pad (sdf >= 0), Newton's third law between
-    the rope's impulse and the pad's external force.
+    the rope's impulse and the pad's external force; a particle leaving the pad
+    (n . v_rel < 0) feels the damped spring -(kn + kd n.v_rel) sdf (the cable
+    form keeps the negative normal velocity).
 """
 import math
 
@@ -95,8 +97,10 @@ def step(rp, pos, vel, ext_f, fixed):
 
 def contact(rp, pos, vel, pad_pos, pad_vel, tris, out_dir, idx, kn, kd, kt, mu_f):
     """Particle-wise penalty contact with one pad (cable_straightening.py:
-    301-319): the sdf is offset by the particle radius.  Returns (rope impulse
-    f dt per particle, pad external force with -f/3 per triangle vertex, flag)."""
+    301-319): the sdf is offset by the particle radius and the force is the
+    cable task's own form (cable_straightening.py:211-223: no max(., 0) on the
+    normal velocity, shear norm without eps).  Returns (rope impulse f dt per
+    particle, pad external force with -f/3 per triangle vertex, flag)."""
     tris_t = torch.as_tensor(np.asarray(tris), dtype=torch.int64)
     idx = torch.as_tensor(idx, dtype=torch.int64)
     ok = idx >= 0
@@ -104,7 +108,7 @@ def contact(rp, pos, vel, pad_pos, pad_vel, tris, out_dir, idx, kn, kd, kt, mu_f
     sdf, nv, rv, flag = oc.find_sdf(pos, vel, idx_c, pad_pos, pad_vel, tris_t, out_dir,
                                     offset=rp.p_rad)
     flag = flag & ok
-    f = oc.contact_force(sdf, -nv, -rv, kn, kd, kt, mu_f)
+    f = oc.contact_force(sdf, -nv, -rv, kn, kd, kt, mu_f, norm_eps=0.0, clamp_vn=False)
     f = torch.where(flag[:, None], f, torch.zeros_like(f))
     e = torch.zeros_like(pad_pos)
     t = tris_t[idx_c]

@@ -138,6 +138,81 @@ def test_contact_force_closed_forms():
     assert torch.allclose(f[2], torch.tensor([mu * fn, fn, 0.0], dtype=F64), atol=1e-12)
 
 
+def test_contact_force_cable_form_closed_forms():
+    """The cable task's force (cable_straightening.py:211-223) keeps a negative
+    normal velocity and has no eps in the shear norm; the grasp form
+    (grasp_plastic.py:204-216) clips the first and adds 1e-11 to the second."""
+    kn, kd, kt, mu = 10.0, 2.0, 3.0, 0.5
+    n = torch.tensor([[0.0, 1.0, 0.0]] * 2, dtype=F64)
+    sdf = torch.tensor([-0.1] * 2, dtype=F64)
+    # separating: n . r = -0.3
+    r = torch.tensor([[0.0, -0.3, 0.0]] * 2, dtype=F64)
+    cable = oc.contact_force(sdf, n, r, kn, kd, kt, mu, norm_eps=0.0, clamp_vn=False)
+    grasp = oc.contact_force(sdf, n, r, kn, kd, kt, mu)
+    assert torch.allclose(cable[0], torch.tensor([0.0, (kn - kd * 0.3) * 0.1, 0.0], dtype=F64))
+    assert torch.allclose(grasp[0], torch.tensor([0.0, kn * 0.1, 0.0], dtype=F64))
+    # shear speed 0.99975e-4: below 1e-4 exactly, above it with the 1e-11 eps
+    vt = 0.99975e-4
+    r = torch.tensor([[vt, 0.0, 0.0]] * 2, dtype=F64)
+    cable = oc.contact_force(sdf, n, r, kn, kd, kt, mu, norm_eps=0.0, clamp_vn=False)
+    grasp = oc.contact_force(sdf, n, r, kn, kd, kt, mu)
+    assert torch.allclose(cable[0], torch.tensor([0.0, kn * 0.1, 0.0], dtype=F64))
+    t = kt * (vt * vt + 1e-11) ** 0.5           # kt |v_t| below mu |f_n| = 0.5
+    assert torch.allclose(grasp[0], torch.tensor([vt / (vt * vt + 1e-11) ** 0.5 * t, kn * 0.1,
+                                                  0.0], dtype=F64), rtol=1e-12)
+    # reverse mode stays finite with no eps at zero shear
+    r0 = torch.zeros(2, 3, dtype=F64, requires_grad=True)
+    g = torch.autograd.grad(oc.contact_force(sdf, n, r0, kn, kd, kt, mu, norm_eps=0.0,
+                                             clamp_vn=False).sum(), r0)[0]
+    assert torch.isfinite(g).all() and torch.allclose(g[:, 1], torch.full((2,), -kd * -0.1 * 1,
+                                                                           dtype=F64))
+
+
+def test_collision_query_velocity_is_momentum_over_mass_plus_eps():
+    """Under a flat pad the force at a node is a closed form of its query
+    velocity qv = mv / (m + 1e-5) (grasp_plastic.py:283-285): a pad moving at
+    qv gives the bare spring kn |sdf|; a node approaching along -n at speed c
+    adds kd c; a large tangential qv gives the Coulomb cap mu |f_n| along it."""
+    prm = om.MPMParams(n_grid=16, rho=1.0, dt=1e-4)
+    n = prm.n_grid
+    y0 = 0.53
+    # one big flat triangle pair at y = y0, outward +y, covering every node centre
+    pos = torch.tensor([[-1.0, y0, -1.0], [3.0, y0, -1.0], [-1.0, y0, 3.0], [3.0, y0, 3.0]],
+                       dtype=F64)
+    tris = np.array([[0, 2, 1], [1, 2, 3]])
+    od = torch.tensor([0.0, 1.0, 0.0], dtype=F64)
+    nodes = np.array([(i * n + 7) * n + k for i in (5, 6) for k in (4, 9)])   # j = 7: below y0
+    m0 = 3e-5                                       # comparable to the 1e-5 eps
+    grid_m = torch.zeros(n ** 3, dtype=F64)
+    grid_m[nodes] = m0
+    kn, kd, kt, mu = 1e3, 20.0, 50.0, 0.5
+    q = oc.node_centres(prm, nodes)
+    sdf = q[:, 1].astype(np.float64) - y0
+    assert (sdf < 0).all()
+    idx = np.zeros((len(nodes), 1), np.int32)
+    scale = m0 / (m0 + prm.eps)
+    for case in ("rest", "approach", "shear"):
+        # the pad body lies below y0 (outward +y): "approach" moves deeper in
+        u = {"rest": [0.2, -0.1, 0.3], "approach": [0.0, -0.5, 0.0], "shear": [40.0, 0.0, 0.0]}[case]
+        u = np.array(u)
+        grid_mv = torch.zeros(n ** 3, 3, dtype=F64)
+        grid_mv[nodes] = torch.tensor(m0 * u, dtype=F64)
+        pad_v = torch.tensor(scale * u if case == "rest" else np.zeros(3), dtype=F64).expand(4, 3)
+        gf, ext = oc.collision(prm, grid_mv, grid_m, nodes, idx, [(pos, pad_v, tris, od)],
+                               kn, kd, kt, mu)
+        f = gf[nodes].numpy() / prm.dt
+        fn = kn * -sdf
+        if case == "rest":
+            want = np.stack([0 * sdf, fn, 0 * sdf], 1)
+        elif case == "approach":
+            # r = -(qv - v_pad) = -scale u; n . r = 0.5 scale > 0 (approaching)
+            want = np.stack([0 * sdf, (kn + kd * 0.5 * scale) * -sdf, 0 * sdf], 1)
+        else:
+            # r = -scale u (tangential); kt |v_t| far above mu |f_n|
+            want = np.stack([-mu * fn, fn, 0 * sdf], 1)
+        np.testing.assert_allclose(f, want, rtol=1e-10)
+
+
 def test_collision_newtons_third_law_and_fd():
     prm = om.MPMParams(n_grid=32, rho=1000.0, dt=1e-4)
     nodes, f2v, surf, layer, od = slab_mesh(16, 16, 3, h=1 / 64, origin=(0.3, 0.5, 0.3))

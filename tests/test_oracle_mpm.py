@@ -234,6 +234,42 @@ def test_p2g_conserves_mass_and_momentum():
     assert torch.allclose(mv.sum(0), prm.p_mass * v.sum(0), rtol=1e-10, atol=1e-14)
 
 
+def test_p2g_second_moment_fixes_the_stress_and_affine_terms():
+    """For quadratic B-splines sum_i w_i d_i d_i^T = dx^2/4 I and sum_i w_i d_i = 0
+    (d_i = x_i - x_p), so one particle's scatter has the second moment
+    sum_i mv_i (x_i - x_p)^T = -dt p_vol tau + p_mass C dx^2/4 and
+    sum_i m_i d_i d_i^T = p_mass dx^2/4 I (mpm_elastic.py:183-200).  tau comes
+    from scipy's polar decomposition, not from the oracle."""
+    import scipy.linalg
+    prm = om.MPMParams(n_grid=16, rho=1000.0, dt=1e-4)
+    n = prm.n_grid
+    rng = np.random.default_rng(7)
+    table = [om.Material(5e4, 0.3), om.Material(2e5, 0.25), om.Material(1e4, 0.45)]
+    idx = torch.arange(n ** 3)
+    node_x = torch.stack([idx // (n * n), (idx // n) % n, idx % n], 1).to(F64) * prm.dx
+    for trial in range(24):
+        mid = trial % 3
+        x = torch.tensor(rng.uniform(0.3, 0.7, size=(1, 3)), dtype=F64)
+        v = torch.tensor(rng.normal(0, 0.5, size=(1, 3)), dtype=F64)
+        C = torch.tensor(rng.normal(0, 3.0, size=(1, 3, 3)), dtype=F64)
+        F = np.eye(3) + 0.15 * rng.normal(size=(3, 3))
+        assert np.linalg.det(F) > 0
+        Ft = torch.tensor(F[None], dtype=F64)
+        U, s, V = om.svd3(Ft)
+        mats = om.material_arrays(table, np.array([mid]), F64)
+        mv, m, _ = om.p2g(prm, x, v, C, Ft, U, s, V, mats)
+        R, _ = scipy.linalg.polar(F)
+        J = np.linalg.det(F)
+        mu, lam = table[mid].mu, table[mid].lam
+        tau = 2 * mu * (F - R) @ F.T + lam * J * (J - 1) * np.eye(3)
+        d = node_x - x                                   # (n^3, 3)
+        mom2 = (mv[:, :, None] * d[:, None, :]).sum(0).numpy()
+        want = -prm.dt * prm.p_vol * tau + prm.p_mass * C[0].numpy() * prm.dx ** 2 / 4
+        np.testing.assert_allclose(mom2, want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+        m2 = (m[:, None, None] * d[:, :, None] * d[:, None, :]).sum(0).numpy()
+        np.testing.assert_allclose(m2, prm.p_mass * prm.dx ** 2 / 4 * np.eye(3), atol=1e-18)
+
+
 def test_g2p_reproduces_affine_fields_exactly():
     prm = om.MPMParams(n_grid=32, dt=1e-4)
     n = prm.n_grid

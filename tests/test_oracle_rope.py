@@ -93,3 +93,30 @@ def test_contact_is_zero_outside_and_obeys_action_reaction():
     imp_f, ext_f, flag_f = orp.contact(rp, far, v, pos, vel, tris, _t(od), idx_f,
                                        1e3, 10.0, 50.0, 0.5)
     assert not flag_f.any() and (imp_f == 0).all() and (ext_f == 0).all()
+
+
+def test_contact_keeps_the_cable_forms_negative_normal_velocity():
+    """A particle under a flat pad moving away from it (n . v_rel < 0 in the
+    force's frame) feels -(kn + kd n.v_rel) sdf along n: the cable task's
+    force (cable_straightening.py:211-223) does not clip the normal velocity
+    at zero; a shear speed just under 1e-4 adds no tangential force (no eps)."""
+    from oracle import contact as oc
+    rp = orp.RopeParams(2, dt=1e-3, p_rad=0.01)
+    pos = _t([[-1.0, 0.2, -1.0], [3.0, 0.2, -1.0], [-1.0, 0.2, 3.0]])
+    tris = np.array([[0, 2, 1]])
+    od = _t([0.0, -1.0, 0.0])                         # contact face looks down
+    x = _t([[0.3, 0.195, 0.3], [0.4, 0.195, 0.4]])   # 5 mm below: sdf = -0.005 - r
+    kn, kd, kt, mu = 1e3, 30.0, 50.0, 0.5
+    pad_v = torch.zeros(3, 3, dtype=D64)
+    # particle 0 falls (moves away from the face), particle 1 slides slowly
+    v = _t([[0.0, -2.0, 0.0], [0.99975e-4, 0.0, 0.0]])
+    idx = np.zeros(2, np.int64)
+    imp, ext, flag = orp.contact(rp, x, v, pos, pad_v, tris, od, idx, kn, kd, kt, mu)
+    assert flag.all()
+    sdf = 0.2 - 0.195 - rp.p_rad                      # n = -y: sdf = (x - p1) . n - r
+    # the force f = -(kn + kd v_n) sdf n acts along the face normal n = -y with
+    # r = -(v - v_pad) = -v, so v_n = n . r = -2 for the falling particle
+    vn0 = -2.0
+    np.testing.assert_allclose(imp[0].numpy() / rp.dt, [0.0, (kn + kd * vn0) * sdf, 0.0],
+                               rtol=1e-12)
+    np.testing.assert_allclose(imp[1].numpy() / rp.dt, [0.0, kn * sdf, 0.0], rtol=1e-12)
