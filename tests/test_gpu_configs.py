@@ -119,8 +119,9 @@ def test_c2_scaled_pad_press_matches_the_oracle(gpu):
     assert rel_l2(g_cv.cpu(), go_cv) < 1e-3
     assert rel_l2(g_x.cpu(), go_x) < 1e-3
     lm32 = oracle(torch.float32)[4]
-    tol = 3 * abs(float(lm32) - float(lm64))
-    assert abs(float(loss_m.detach()) - float(lm64)) <= max(tol, 1e-4 * float(lm64))
+    tol = 3 * abs(float(lm32.detach()) - float(lm64.detach()))
+    lm = float(lm64.detach())
+    assert abs(float(loss_m.detach()) - lm) <= max(tol, 1e-4 * lm)
 
 
 def _sorted_state(sim, x, v):
