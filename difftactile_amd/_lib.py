@@ -108,6 +108,8 @@ _SIGS = {
                                 _c_int] + [_vp] * 12 + [_vp, _sz, _vp, _vp]),
     "dt_mpm_advance": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 8
                        + [_c_int, _vp, _vp, _vp, _sz, _vp]),
+    "dt_mpm_advance_static": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 4
+                              + [_c_int, _vp]),
     "dt_mpm_workspace_bytes": (_c_int, [_P(DtMpmParams), _c_int, _P(_sz)]),
     "dt_mpm_workspace_history_bytes": (_c_int, [_P(DtMpmParams), _c_int, _c_int, _P(_sz)]),
     "dt_mpm_bin": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _sz, _vp]),

@@ -2040,6 +2040,64 @@ void hist_clear(const void* ws) {
 }  // namespace dt
 
 // ============================================================================
+// Static scratch of dt_mpm_advance_static (callers that cannot allocate)
+// ============================================================================
+// The ping-pong state, the two single-substep grids and the binning workspace
+// of dt_mpm_advance_static live in this array.  Every GPU the library runs on
+// holds its own copy (kStaticScratchBytes, zero-filled at load) until the
+// process ends.  Host code never passes the array's name to a kernel: its
+// device address is looked up once per device (hipGetSymbolAddress, with that
+// device current) and that address is used instead.
+namespace dt_static {
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+constexpr int kMaxDevices = 64;
+void* g_scratch_addr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of the current device's copy of scratch.
+int scratch_address(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    dt::set_error("dt_mpm_advance_static: no current HIP device (or device id >= 64)");
+    return DT_ELAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      dt::set_error(std::string("dt_mpm_advance_static: hipGetSymbolAddress failed: ") +
+                    hipGetErrorString(e));
+      return DT_ELAUNCH;
+    }
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return DT_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Byte offsets of (x2, v2, C2, F2, grid4, grid_v4, workspace) in the scratch
+// and the total for n particles on prm's grid.
+size_t scratch_layout(const dt_mpm_params* prm, int n, size_t (&off)[7]) {
+  const long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid *
+                       prm->n_grid * prm->n_grid;
+  const size_t sz[7] = {sizeof(float) * 3 * (size_t)n, sizeof(float) * 3 * (size_t)n,
+                        sizeof(float) * 9 * (size_t)n, sizeof(float) * 9 * (size_t)n,
+                        sizeof(float) * 4 * (size_t)nn, sizeof(float) * 4 * (size_t)nn,
+                        dt::bin_ws_layout(prm, n, nullptr, nullptr)};
+  size_t total = 0;
+  for (int i = 0; i < 7; ++i) {
+    off[i] = total;
+    total += align256(sz[i] ? sz[i] : 1);
+  }
+  return total;
+}
+}  // namespace dt_static
+
+// ============================================================================
 // C-ABI: MPM entry points
 // ============================================================================
 using namespace dt;
@@ -2378,6 +2436,54 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     hipMemcpyAsync(F, a[3], sizeof(float) * 9 * (size_t)n, hipMemcpyDeviceToDevice, st);
   }
   return check_launch("dt_mpm_advance");
+}
+
+int dt_mpm_advance_static(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats,
+                          int32_t n, const int32_t* mat_id, float* x, float* v, float* C,
+                          float* F, int32_t n_substeps, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  if ((rc = check_mats(mats, n_mats))) return rc;
+  DT_REQUIRE(n >= 0 && n_substeps >= 0, "n and n_substeps must be >= 0");
+  DT_REQUIRE(n == 0 || (x && v && C && F), "NULL pointer");
+  size_t off[7];
+  const size_t need = dt_static::scratch_layout(prm, n, off);
+  if (need > dt_static::kStaticScratchBytes) {
+    set_error("dt_mpm_advance_static: " + std::to_string(n) + " particles on this grid need " +
+              std::to_string(need) + " bytes of scratch; the static scratch holds " +
+              std::to_string(dt_static::kStaticScratchBytes) + " (use dt_mpm_advance)");
+    return DT_EINVAL;
+  }
+  void* base = nullptr;
+  if ((rc = dt_static::scratch_address(&base))) return rc;
+  char* b = static_cast<char*>(base);
+  float* x2 = reinterpret_cast<float*>(b + off[0]);
+  float* v2 = reinterpret_cast<float*>(b + off[1]);
+  float* C2 = reinterpret_cast<float*>(b + off[2]);
+  float* F2 = reinterpret_cast<float*>(b + off[3]);
+  float* g4 = reinterpret_cast<float*>(b + off[4]);
+  float* gv4 = reinterpret_cast<float*>(b + off[5]);
+  BinWs ws{};
+  bin_ws_layout(prm, n, b + off[6], &ws);
+  // the first binning clears the brick counts it accumulates into (k_bin_zero);
+  // nothing else in the scratch is read before this call writes it
+  const bool fuse = bin_counts_fusable(ws);
+  hipStream_t st = DT_STREAM(stream);
+  float *p[4] = {x, v, C, F}, *q[4] = {x2, v2, C2, F2};
+  for (int s = 0; s < n_substeps; ++s) {
+    int r = launch_substep_fwd(prm, mats, n, mat_id, p[0], p[1], p[2], p[3], g4, gv4, q[0],
+                               q[1], q[2], q[3], st, &ws, s > 0, fuse && s > 0,
+                               fuse && s + 1 < n_substeps);
+    if (r) return r;
+    for (int k = 0; k < 4; ++k) { float* t = p[k]; p[k] = q[k]; q[k] = t; }
+  }
+  if (p[0] != x && n > 0) {
+    hipMemcpyAsync(x, p[0], sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(v, p[1], sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(C, p[2], sizeof(float) * 9 * (size_t)n, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(F, p[3], sizeof(float) * 9 * (size_t)n, hipMemcpyDeviceToDevice, st);
+  }
+  return check_launch("dt_mpm_advance_static");
 }
 
 

@@ -286,6 +286,22 @@ int dt_mpm_advance(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                    float* v2, float* C2, float* F2, int32_t n_substeps, float* scratch4,
                    float* scratch_v4, void* workspace, size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference: dt_mpm_advance for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  Same result as dt_mpm_advance on the
+   sparse path, with the ping-pong state, the two grids and the binning
+   workspace in a static scratch array the library holds on every device it
+   runs on (64 MiB each, until the process ends).  The caller passes only the
+   state, advanced in place.  Rejects (DT_EINVAL) a particle count and grid
+   whose scratch exceeds 64 MiB: 96 B per particle, 32 B per grid node, plus
+   the workspace (dt_mpm_workspace_bytes); a 64^3 grid holds about 400k
+   particles.  Calls no hipMalloc / hipFree and captures no graph.  Every call
+   on one device shares that device's scratch: calls that overlap on one GPU
+   (different streams, threads) race. */
+int dt_mpm_advance_static(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats,
+                          int32_t n, const int32_t* mat_id, float* x, float* v, float* C,
+                          float* F, int32_t n_substeps, void* stream);
+
 /* ---- sparse path: workspace, binning and brick-owned scatter passes ----
    (no reference counterpart: the reference scatters with per-particle
    atomics into a dense grid, mpm_elastic.py:175-208; these stages give the
