@@ -2,15 +2,18 @@
 
 Workload (one GPU): the multi-material sphere of 1M particles on a 128^3 grid
 (rigid elastic core E=1e7 inside a Drucker-Prager shell), float32.  One step
-is a 50-substep checkpointed rollout forward plus its backward (recompute of
-the segment + adjoint sweep), i.e. 50 fwd+bwd substeps; inputs are resident in
-HBM before the timed region.
+is a 1000-substep trajectory forward plus its backward, checkpointed every
+50 substeps: segments whose frames and grids fit the resident budget (a
+quarter of HBM) are kept from the forward pass, the others are recomputed
+from their checkpoints in the backward pass.  Every step starts from the
+same deformed state (the undeformed sphere advanced 200 substeps), and the
+inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line (rank 0).  Also reports, for the dominant kernel, the
 achieved algorithmic bandwidth against MI355X's 8 TB/s (HIP events on the
 stream the kernel runs on), a CPU baseline (the oracle on a bounded sample,
-host cores) and the relative error of the GPU gradient against FP64 central
-differences of the oracle on a small scene.
+host cores) and the relative errors of the GPU gradient against FP64 central
+differences of the oracle for 16 seeded parameters of a scaled sphere.
 """
 import argparse
 import json
@@ -34,6 +37,12 @@ def parse():
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--n-grid", type=int, default=128)
     ap.add_argument("--segment", type=int, default=50)
+    ap.add_argument("--substeps", type=int, default=1000,
+                    help="sphere workload: substeps of the trajectory one step runs fwd+bwd")
+    ap.add_argument("--predeform", type=int, default=200,
+                    help="sphere workload: substeps advanced before timing (the start state)")
+    ap.add_argument("--resident-frac", type=float, default=0.25,
+                    help="sphere workload: fraction of HBM for resident trajectory segments")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--breakdown", action="store_true",
                     help="also time each stage in isolation on the initial state")
@@ -237,48 +246,144 @@ def cpu_baseline(sample, segment_steps=16):
                       f"{dt_s:.1f} s"}
 
 
-def grad_check(torch):
-    """Directional derivative of a trajectory loss w.r.t. v0: GPU adjoint (float32)
-    vs FP64 central differences of the oracle (eps 1e-4)."""
+GC_SCENE = dict(n_particles=20000, n_grid=64)
+GC_STEPS = 100
+GC_EPS = 1e-4
+GC_NV = 12          # velocity directions; then mu, lam of the core and of the shell
+
+
+def _gc_problem():
+    """The scaled sphere of the gradient check and its 16 seeded parameters:
+    theta_k (k < 12) moves v0 along a seeded direction d_k; theta_12..15 scale
+    (mu_core, lam_core, mu_shell, lam_shell) by (1 + theta).  Loss: a seeded
+    linear functional of the final positions."""
     from difftactile_amd import scenes
-    from difftactile_amd.params import DRUCKER_PRAGER, ELASTIC, Material, MPMParams
-    from difftactile_amd.sim import MPMSim
+    prm, mats, x, v, mid = scenes.config3_multi_material(**GC_SCENE)
+    rng = np.random.default_rng(11)
+    dirs = rng.normal(size=(GC_NV,) + x.shape)
+    w = rng.normal(size=x.shape)
+    return prm, mats, x, v, mid, dirs, w
+
+
+def _gc_init():
+    import torch
+    torch.set_num_threads(1)
+
+
+def _gc_fd_loss(job):
+    """Oracle (FP64) loss at theta_k = sign * eps (one pool job)."""
+    import torch
     from oracle import mpm as om
-    prm = MPMParams(n_grid=32, rho=1000.0, dt=1e-5)
-    x, _ = scenes.sphere_lattice(2000, prm.dx / 2, (0.5, 0.3, 0.5), jitter=0.25 * prm.dx, seed=3)
-    n = len(x)
-    d = np.linalg.norm(x - np.array([0.5, 0.3, 0.5], np.float32), axis=1)
-    mid = (d > 0.5 * d.max()).astype(np.int32)
-    mats = [Material(1e6, 0.3, ELASTIC), Material(1e5, 0.3, DRUCKER_PRAGER)]
-    rng = np.random.default_rng(5)
-    v0 = rng.normal(0, 0.3, size=(n, 3)).astype(np.float32)
-    steps = 20
-    w = rng.normal(size=(n, 3))
-    dirv = rng.normal(size=(n, 3))
-    sim = MPMSim(prm, mats, n, mat_id=mid, checkpoint_every=7)
-    st = [torch.tensor(x).cuda(), torch.tensor(v0).cuda().requires_grad_(True),
-          torch.zeros(n, 3, 3).cuda(), torch.eye(3).repeat(n, 1, 1).cuda()]
-    out = sim.rollout(*st, steps)
-    loss = (out[0] * torch.tensor(w, dtype=torch.float32).cuda()).sum()
-    g = torch.autograd.grad(loss, st[1])[0].double().cpu().numpy()
-    gpu_dd = float((g * dirv).sum())
-    op = om.MPMParams(n_grid=32, rho=1000.0, dt=1e-5)
+    k, sign = job
+    prm, mats, x, v, mid, dirs, w = _gc_problem()
+    op = om.MPMParams(n_grid=prm.n_grid, rho=prm.rho, dt=prm.dt, gravity=prm.gravity,
+                      svd_exact=prm.svd_exact)
     oms = [om.Material(m.E, m.nu, m.kind, m.yield_stress) for m in mats]
-    omats = om.material_arrays(oms, mid, torch.float64)
-    eps = 1e-4
+    for o, m in zip(oms, mats):
+        o.dp_alpha = m.dp_alpha
+    mu, lam, kind, yld, alpha = om.material_arrays(oms, mid, torch.float64)
+    vv = v.astype(np.float64)
+    if k < GC_NV:
+        vv = vv + sign * GC_EPS * dirs[k]
+    else:
+        sel = torch.as_tensor(mid == (k - GC_NV) // 2)
+        scale = 1.0 + sign * GC_EPS
+        if (k - GC_NV) % 2 == 0:
+            mu = torch.where(sel, mu * scale, mu)
+        else:
+            lam = torch.where(sel, lam * scale, lam)
+    n = len(x)
+    st = (torch.tensor(x, dtype=torch.float64), torch.tensor(vv),
+          torch.zeros(n, 3, 3, dtype=torch.float64),
+          torch.eye(3, dtype=torch.float64).repeat(n, 1, 1))
+    with torch.no_grad():
+        out = om.rollout(op, st, (mu, lam, kind, yld, alpha), GC_STEPS)
+    return float((out[0] * torch.tensor(w)).sum())
 
-    def L(sign):
-        vv = torch.tensor(v0, dtype=torch.float64) + sign * eps * torch.tensor(dirv)
-        s = (torch.tensor(x, dtype=torch.float64), vv, torch.zeros(n, 3, 3, dtype=torch.float64),
-             torch.eye(3, dtype=torch.float64).repeat(n, 1, 1))
-        with torch.no_grad():
-            o = om.rollout(op, s, omats, steps)
-        return float((o[0] * torch.tensor(w)).sum())
 
-    fd = (L(1) - L(-1)) / (2 * eps)
-    return {"rel_err": abs(gpu_dd - fd) / max(abs(fd), 1e-30), "gpu": gpu_dd, "fd_fp64": fd,
-            "case": f"{n} particles, 32^3, {steps} substeps, core+DP shell, d/dv0 along a "
-                    "random direction, eps 1e-4"}
+def _gc_oracle_ad(torch, dtype):
+    """The oracle's reverse mode (the exact derivative of its arithmetic) of
+    the check's 16 parameters, in dtype, on the host's cores."""
+    from torch.utils.checkpoint import checkpoint
+    from oracle import mpm as om
+    prm, mats, x, v, mid, dirs, w = _gc_problem()
+    op = om.MPMParams(n_grid=prm.n_grid, rho=prm.rho, dt=prm.dt, gravity=prm.gravity,
+                      svd_exact=prm.svd_exact)
+    oms = [om.Material(m.E, m.nu, m.kind, m.yield_stress) for m in mats]
+    for o, m in zip(oms, mats):
+        o.dp_alpha = m.dp_alpha
+    n = len(x)
+    mu_t = torch.tensor([m.mu for m in oms], dtype=dtype, requires_grad=True)
+    lam_t = torch.tensor([m.lam for m in oms], dtype=dtype, requires_grad=True)
+    _, _, kind, yld, alpha = om.material_arrays(oms, mid, dtype)
+    idx = torch.as_tensor(mid).long()
+    v0 = torch.tensor(v).to(dtype).requires_grad_(True)
+
+    def seg(x_, v_, C_, F_, mu_, lam_):
+        s = (x_, v_, C_, F_)
+        for _ in range(10):
+            s = om.substep(op, s, (mu_[idx], lam_[idx], kind, yld, alpha))
+        return s
+
+    s = (torch.tensor(x).to(dtype), v0, torch.zeros(n, 3, 3, dtype=dtype),
+         torch.eye(3, dtype=dtype).repeat(n, 1, 1))
+    for _ in range(GC_STEPS // 10):
+        s = checkpoint(seg, *s, mu_t, lam_t, use_reentrant=False)
+    loss = (s[0] * torch.tensor(w).to(dtype)).sum()
+    gv, gmu, glam = torch.autograd.grad(loss, [v0, mu_t, lam_t])
+    ad = [float((gv.double().numpy() * dirs[k]).sum()) for k in range(GC_NV)]
+    for c in range(2):
+        ad += [oms[c].mu * float(gmu[c]), oms[c].lam * float(glam[c])]
+    return ad
+
+
+def grad_check(torch, pool):
+    """16 directional derivatives of a trajectory loss: the GPU adjoint
+    (float32, both 50-substep segments recomputed from checkpoints) against
+    FP64 central differences (eps 1e-4) of the oracle, computed on the host's
+    cores by the worker pool."""
+    from difftactile_amd.sim import MPMSim
+    prm, mats, x, v, mid, dirs, w = _gc_problem()
+    n = len(x)
+    jobs = [(k, s) for k in range(GC_NV + 4) for s in (1, -1)]
+    pending = pool.map_async(_gc_fd_loss, jobs)
+    sim = MPMSim(prm, mats, n, mat_id=mid, checkpoint_every=50, resident_bytes=0,
+                 material_grad=True)
+    st = [torch.tensor(x).cuda(), torch.tensor(v).cuda().requires_grad_(True),
+          torch.zeros(n, 3, 3).cuda(), torch.eye(3).repeat(n, 1, 1).cuda()]
+    out = sim.rollout(*st, GC_STEPS)
+    loss = (out[0] * torch.tensor(w, dtype=torch.float32).cuda()).sum()
+    gv = torch.autograd.grad(loss, st[1])[0].double().cpu().numpy()
+    gm = sim.material_grad.double().cpu().numpy()
+    ad = [float((gv * dirs[k]).sum()) for k in range(GC_NV)]
+    for c in range(2):
+        ad += [mats[c].mu * gm[c, 0], mats[c].lam * gm[c, 1]]
+    fl = pending.get()
+    fd = [(fl[2 * k] - fl[2 * k + 1]) / (2 * GC_EPS) for k in range(GC_NV + 4)]
+    threads = torch.get_num_threads()
+    torch.set_num_threads(min(16, os.cpu_count() or 1))
+    rm64 = _gc_oracle_ad(torch, torch.float64)
+    rm32 = _gc_oracle_ad(torch, torch.float32)
+    torch.set_num_threads(threads)
+    rel = lambda a, b: [abs(p - q) / max(abs(q), 1e-30) for p, q in zip(a, b)]
+    r_fd, r_rm, fd_rm, f32 = rel(ad, fd), rel(ad, rm64), rel(fd, rm64), rel(rm32, rm64)
+    return {"rel_err_max": max(r_fd), "rel_err_median": float(np.median(r_fd)),
+            "rel_err": r_fd,
+            "rel_err_vs_fp64_reverse_mode": r_rm,
+            "rel_err_vs_fp64_reverse_mode_median": float(np.median(r_rm)),
+            "fd_vs_fp64_reverse_mode": fd_rm,
+            "fp32_oracle_vs_fp64_reverse_mode": f32,
+            "gpu": ad, "fd_fp64": fd, "reverse_mode_fp64": rm64,
+            "note": "rel_err: GPU vs FP64 central differences; fd_vs_fp64_reverse_mode: the "
+                    "oracle's own FD vs its exact FP64 derivative (non-zero where Drucker-"
+                    "Prager return-map kinks lie within eps of the trajectory); "
+                    "fp32_oracle_vs_fp64_reverse_mode: the oracle's reverse mode in float32 "
+                    "(the float32 floor of each parameter)",
+            "case": f"{n} particles, {prm.n_grid}^3, {GC_STEPS} substeps (two 50-substep "
+                    "segments, both recomputed), core + Drucker-Prager shell; 16 seeded "
+                    "parameters: 12 directions of v0, mu and lam of each material; loss a "
+                    "seeded linear functional of the final positions; FP64 central "
+                    "differences of the oracle, eps 1e-4"}
 
 
 def bench_pad_press(args, torch):
@@ -495,6 +600,12 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not "
               "part of this build", file=sys.stderr)
         sys.exit(2)
+    pool = None
+    if args.workload == "sphere" and not args.no_grad_check:
+        # host workers for the FP64 finite differences, forked before anything
+        # touches the GPU (they run the CPU oracle only)
+        import multiprocessing as mp
+        pool = mp.get_context("fork").Pool(min(16, os.cpu_count() or 1), initializer=_gc_init)
     import torch
     if args.workload == "pad_press":
         bench_pad_press(args, torch)
@@ -503,18 +614,29 @@ def main():
         bench_population(args, torch)
         return
     from difftactile_amd import scenes
-    from difftactile_amd.sim import MPMSim
+    from difftactile_amd.sim import MPMSim, _segments
 
     dev = torch.device("cuda:0")
     prm, mats, x, v, mid = scenes.config3_multi_material(n_particles=args.particles,
                                                          n_grid=args.n_grid)
     n = len(x)
-    sim = MPMSim(prm, mats, n, mat_id=mid, device=dev, checkpoint_every=args.segment, sort=True)
+    total_mem = torch.cuda.get_device_properties(dev).total_memory
+    sim = MPMSim(prm, mats, n, mat_id=mid, device=dev, checkpoint_every=args.segment, sort=True,
+                 resident_bytes=int(args.resident_frac * total_mem))
     st = [torch.tensor(x, device=dev), torch.tensor(v, device=dev),
           torch.zeros(n, 3, 3, device=dev), torch.tensor(scenes.identity_F(n), device=dev)]
+    log(f"[bench] {n} particles, {prm.n_grid}^3: advancing {args.predeform} substeps to the "
+        "start state")
+    st = list(sim.step_no_grad(*st, args.predeform))
     w = torch.randn(n, 3, device=dev, generator=torch.Generator(device=dev).manual_seed(1))
-    T = args.segment
-    resident = sim.segment_bytes(T) <= sim.resident_bytes
+    T = args.substeps
+    # which segments the forward pass keeps (the rule of sim._Rollout.forward)
+    used, kept, segs = 0, 0, _segments(T, args.segment)
+    for _, k in segs:
+        if used + sim.segment_bytes(k) <= sim.resident_bytes:
+            used += sim.segment_bytes(k)
+            kept += 1
+    recomputed_substeps = sum(k for _, k in segs[kept:])
 
     def step():
         leaves = [t.detach().requires_grad_(True) for t in st]
@@ -522,7 +644,9 @@ def main():
         loss = (out[0] * w).sum() + out[1].sum()
         return torch.autograd.grad(loss, leaves[1])[0]
 
-    log(f"[bench] {n} particles, {prm.n_grid}^3, segment {T}; warmup {args.warmup}")
+    log(f"[bench] trajectory {T} substeps, segments of {args.segment}: {kept} of {len(segs)} "
+        f"resident ({used / 1e9:.1f} GB), {recomputed_substeps} substeps recomputed; "
+        f"warmup {args.warmup}")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -564,7 +688,7 @@ def main():
         mat_sorted = sim._gather_i(perm, sim.mat_id)
         iso, _ = kernel_breakdown(torch, sim, sorted_state, mat_sorted, args.kernel_reps)
         isolated = {k: round(v, 5) for k, v in iso.items()}
-        log(f"[bench] isolated stages on the initial state (ms): {json.dumps(isolated)}")
+        log(f"[bench] isolated stages on the start state (ms): {json.dumps(isolated)}")
     if args.calib_copy:
         src = torch.ones(64 * 2 ** 20, device=dev)
         dst = torch.empty_like(src)
@@ -575,7 +699,8 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get(dom)
-    # whole-substep algorithmic bytes: 480 B/particle + 128 B/active node (+8 B/particle ids)
+    # whole-substep algorithmic bytes: 480 B/particle + 128 B/active node (+8 B/particle ids);
+    # recompute traffic is not counted
     b_sub = 480 * n + 128 * active + 8 * n
     result = {
         "metric": "fwd+bwd particle-substeps/s at 1M particles",
@@ -591,14 +716,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"multi-material sphere (elastic core E=1e7 + Drucker-Prager "
-                               f"shell), {n} particles, {prm.n_grid}^3 grid, "
-                               f"{T}-substep rollout fwd+bwd per step"
-                               + (" (segment frames and grids kept in HBM for the adjoint "
-                                  "sweep)" if resident else " (segment recomputed from its "
-                                  "checkpoint in the backward pass)"),
+                               f"shell), {n} particles, {prm.n_grid}^3 grid; one step = a "
+                               f"{T}-substep trajectory fwd+bwd from a state deformed by "
+                               f"{args.predeform} substeps, checkpointed every {args.segment} "
+                               f"substeps; {kept} of {len(segs)} segments kept in HBM "
+                               f"({used / 1e9:.1f} GB, {args.resident_frac:g} of HBM), the "
+                               f"others recomputed from their checkpoints in the backward pass",
                    "particles": n, "n_grid": prm.n_grid, "substeps_per_step": T,
-                   "segment_resident": resident,
-                   "active_nodes": active},
+                   "checkpoint_every": args.segment, "segments": len(segs),
+                   "segments_resident": kept, "substeps_recomputed": recomputed_substeps,
+                   "recompute_share": recomputed_substeps / T,
+                   "predeform_substeps": args.predeform, "active_nodes": active},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "substep_roofline_frac": b_sub * (value / n) / (HBM_PEAK_GBS * 1e9),
@@ -608,9 +736,11 @@ def main():
     }
     if isolated is not None:
         result["kernels_isolated_ms"] = isolated
-    if not args.no_grad_check:
-        log("[bench] gradient check vs FP64 FD")
-        result["grad_check"] = grad_check(torch)
+    if pool is not None:
+        log("[bench] gradient check: 16 parameters vs FP64 central differences")
+        result["grad_check"] = grad_check(torch, pool)
+        pool.close()
+        pool.join()
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         result["cpu_baseline"] = cpu_baseline(args.cpu_sample)

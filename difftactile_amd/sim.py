@@ -92,7 +92,9 @@ class MPMSim:
 
     def _set_env(self, env):
         """Points the parameter structs at the environment ids of the particle
-        order the next launches see."""
+        order the next launches see (and keeps that tensor alive as long as the
+        structs name it)."""
+        self._env_ref = env
         p = None if env is None else _lib.ptr(env, torch.int32)
         self.c_prm.env_id = p
         self.c_prm_aos.env_id = p
