@@ -43,6 +43,9 @@ def parse():
                     help="sphere workload: substeps advanced before timing (the start state)")
     ap.add_argument("--resident-frac", type=float, default=0.25,
                     help="sphere workload: fraction of HBM for resident trajectory segments")
+    ap.add_argument("--no-resort", action="store_true",
+                    help="sphere workload: sort the particles once per trajectory, not at "
+                         "every checkpoint")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--breakdown", action="store_true",
                     help="also time each stage in isolation on the initial state")
@@ -622,7 +625,7 @@ def main():
     n = len(x)
     total_mem = torch.cuda.get_device_properties(dev).total_memory
     sim = MPMSim(prm, mats, n, mat_id=mid, device=dev, checkpoint_every=args.segment, sort=True,
-                 resident_bytes=int(args.resident_frac * total_mem))
+                 resident_bytes=int(args.resident_frac * total_mem), resort=not args.no_resort)
     st = [torch.tensor(x, device=dev), torch.tensor(v, device=dev),
           torch.zeros(n, 3, 3, device=dev), torch.tensor(scenes.identity_F(n), device=dev)]
     log(f"[bench] {n} particles, {prm.n_grid}^3: advancing {args.predeform} substeps to the "

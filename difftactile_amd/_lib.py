@@ -197,6 +197,7 @@ _SIGS = {
     "dt_scatter_rows": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     "dt_gather_rows_soa": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     "dt_scatter_rows_soa": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "dt_permute_soa": (_c_int, [_c_int, _c_int, _vp, _c_int, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -1,13 +1,13 @@
 // Per-substep binning of particles by the brick of their stencil base
 // (counting sort over bricks).
 //
-// k_bin_count gives every particle its rank inside its brick with one global
-// atomic per run of equal keys in a wavefront (cell-sorted particles make long
-// runs), an exclusive scan over the brick counts gives the brick starts, and
-// k_bin_fill writes the permutation.  The active-brick list (occupied bricks
-// and their +1 neighbours, which a stencil can reach) and the occupied-brick
-// list come from the counts.  The order inside a brick
-// follows the atomics; the brick passes sort each chunk by cell themselves.
+// k_bin_count gives every particle a slot inside its brick with one global
+// atomic per distinct brick of a wavefront, an exclusive scan over the brick
+// counts gives the brick starts, and k_bin_fill writes the permutation.  The
+// active-brick list (occupied bricks and their +1 neighbours, which a stencil
+// can reach) and the occupied-brick list come from the counts.  The slots
+// follow the order the atomics ran in; k_bin_sort then sorts each brick's
+// particles by index, so the permutation is the same on every run.
 #include <hipcub/hipcub.hpp>
 
 #include "dt_bin.h"
@@ -29,7 +29,7 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   size_t scan_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
                                    (int32_t*)nullptr, (int)(nbk + 1));
-  const size_t sz[11] = {sizeof(int32_t) * (size_t)(nbk + 2),
+  const size_t sz[12] = {sizeof(int32_t) * (size_t)(nbk + 2),
                         sizeof(int32_t) * (size_t)(nbk + 1),
                         sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t) * (size_t)n,
@@ -39,9 +39,10 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
                         scan_bytes,
                         sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t),
-                        sizeof(int32_t) * (size_t)(kSlabMax + kSlabMax * kSlabMax)};
-  size_t off[11], total = 0;
-  for (int i = 0; i < 11; ++i) {
+                        sizeof(int32_t) * (size_t)(kSlabMax + kSlabMax * kSlabMax),
+                        sizeof(float) * 32 * (size_t)nblocks(n)};
+  size_t off[12], total = 0;
+  for (int i = 0; i < 12; ++i) {
     off[i] = total;
     total += align256(sz[i] ? sz[i] : 1);
   }
@@ -60,6 +61,7 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
     ws->n_alist = reinterpret_cast<int32_t*>(b + off[9]);
     ws->slab_sum = reinterpret_cast<int32_t*>(b + off[10]);
     ws->row_occ = reinterpret_cast<uint32_t*>(ws->slab_sum + kSlabMax);
+    ws->mat_part = reinterpret_cast<float*>(b + off[11]);
     ws->scan_bytes = scan_bytes;
     ws->n_bricks = nbk;
     ws->nb = (prm->n_grid + 3) >> 2;
@@ -134,7 +136,6 @@ void launch_bin_copy(const BinWs& from, const BinWs& to, int n, hipStream_t st) 
                      to.list, to.n_list, to.alist, to.n_alist);
 }
 
-// Active lanes form a prefix of every wavefront (p = block * 256 + thread).
 __global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float* __restrict__ x,
                                                       int32_t* __restrict__ key,
                                                       int32_t* __restrict__ rank,
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float
     ld_v3(x, p, pl_vec(P.soa, n), xp);
     k = bin_key(xp, P.inv_dx, P.n_grid, env_x0(P, p)) >> 6;
   }
-  const int r = wave_run_rank(active, k, count);
+  const int r = wave_key_rank(active, k, count);
   if (active) {
     key[p] = k;
     rank[p] = r;
@@ -160,6 +161,169 @@ __global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __res
                                                      int32_t* __restrict__ perm) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) perm[start[key[p]] + rank[p]] = p;
+}
+
+// Sorts each occupied brick's particles perm[start[b] .. start[b+1]) by index
+// (ascending).  Blocks stride over the occupied-brick list.  A brick of at most
+// kSortCap particles: every run of 64 is sorted in a wavefront's registers (a
+// bitonic network over the lanes, no barriers), then each particle's place is
+// its rank in its run plus, for every other run, the number of its entries
+// below it (a binary search; indices are distinct).  A larger brick is sorted
+// so in chunks of kSortCap, and merged by the same rule through tmp (n
+// entries) and copied back.
+constexpr int kSortCap = 2048;
+
+// ascending sort of one value per lane over the 64 lanes
+__device__ __forceinline__ int wave_sort64(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int o = __shfl_xor(v, j);
+      const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+      v = (lower == up) ? min(v, o) : max(v, o);
+    }
+  }
+  return v;
+}
+
+// entries of the ascending a[0, len) below v
+__device__ __forceinline__ int count_below(const int32_t* a, int len, int v) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// sorts src[0, m) (m <= kSortCap) into dst[0, m); a, b: LDS of kSortCap
+__device__ __forceinline__ void block_sort(const int32_t* src, int32_t* dst, int m, int32_t* a,
+                                           int32_t* b) {
+  // runs of 64 sorted in registers (padding INT_MAX sorts to the end)
+  for (int i0 = 0; i0 < m; i0 += blockDim.x) {
+    const int i = i0 + threadIdx.x;
+    const int v = wave_sort64(i < m ? src[i] : 0x7fffffff);
+    if (i < ((m + 63) & ~63)) a[i] = v;
+  }
+  __syncthreads();
+  const int runs = (m + 63) >> 6;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const int v = a[i], r = i >> 6;
+    int place = i & 63;
+    for (int q = 0; q < runs; ++q)
+      if (q != r) place += count_below(a + 64 * q, min(64, m - 64 * q), v);
+    b[place] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < m; i += blockDim.x) dst[i] = b[i];
+  __syncthreads();
+}
+
+// The fill gives each (wavefront, brick) pair one run of consecutive slots
+// holding that wavefront's particles of the brick in lane order (ascending
+// indices; wavefront w covers indices [64 w, 64 w + 64)), with the runs in the
+// order their atomics ran.  Ordering the runs by wavefront therefore sorts the
+// brick: the runs (at most 64 here) are found by a block scan of the run
+// heads, sorted by wavefront in one wavefront's registers, and moved to their
+// places.  Returns false, doing nothing, when the brick has more than 64 runs.
+__device__ __forceinline__ bool block_order_runs(int32_t* seg, int m, int32_t* a, int32_t* b,
+                                                 int32_t* run_at, int32_t* run_to,
+                                                 int32_t* wsum) {
+  constexpr int kPer = kSortCap / kBlock;   // elements per thread
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (int i = t; i < m; i += blockDim.x) a[i] = seg[i];
+  __syncthreads();
+  // heads of the thread's kPer consecutive elements
+  const int i0 = t * kPer;
+  int heads = 0;
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int i = i0 + u;
+    if (i < m && (i == 0 || (a[i] >> 6) != (a[i - 1] >> 6))) ++heads;
+  }
+  int incl = heads;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d);
+    if (lane >= d) incl += v;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int base = incl - heads;
+  for (int q = 0; q < w; ++q) base += wsum[q];
+  const int runs = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  if (runs > 64) {
+    __syncthreads();
+    return false;
+  }
+  // run r starts at run_at[r]
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int i = i0 + u;
+    if (i < m && (i == 0 || (a[i] >> 6) != (a[i - 1] >> 6))) run_at[base++] = i;
+  }
+  if (t == 0) run_at[runs] = m;
+  __syncthreads();
+  if (w == 0) {
+    // sort (wavefront, run) pairs, then each run's new start = the lengths of
+    // the runs of earlier wavefronts
+    const int key = lane < runs ? ((a[run_at[lane]] >> 6) << 6) | lane : 0x7fffffff;
+    const int sk = wave_sort64(key);
+    const int r = sk & 63;
+    const int len = lane < runs ? run_at[r + 1] - run_at[r] : 0;
+    int in2 = len;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(in2, d);
+      if (lane >= d) in2 += v;
+    }
+    if (lane < runs) run_to[r] = in2 - len;
+  }
+  __syncthreads();
+  // element i of run r moves to run_to[r] + (i - run_at[r])
+  for (int r = w; r < runs; r += kBlock / 64) {
+    const int s = run_at[r], e = run_at[r + 1], d = run_to[r] - s;
+    for (int i = s + lane; i < e; i += 64) b[i + d] = a[i];
+  }
+  __syncthreads();
+  for (int i = t; i < m; i += blockDim.x) seg[i] = b[i];
+  __syncthreads();
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_sort(const int32_t* __restrict__ start,
+                                                     const int32_t* __restrict__ list,
+                                                     const int32_t* __restrict__ n_list,
+                                                     int32_t* __restrict__ perm,
+                                                     int32_t* __restrict__ tmp) {
+  __shared__ int32_t a[kSortCap], b[kSortCap];
+  __shared__ int32_t run_at[65], run_to[64], wsum[kBlock / 64];
+  const int n_occ = *n_list;
+  for (int li = blockIdx.x; li < n_occ; li += gridDim.x) {
+    const int bk = list[li];
+    const int s0 = start[bk], m = start[bk + 1] - s0;
+    if (m < 2) continue;
+    if (m <= kSortCap) {
+      if (!block_order_runs(perm + s0, m, a, b, run_at, run_to, wsum))
+        block_sort(perm + s0, perm + s0, m, a, b);
+      continue;
+    }
+    // chunks sorted in place, then merged by rank through tmp
+    for (int c0 = 0; c0 < m; c0 += kSortCap)
+      block_sort(perm + s0 + c0, perm + s0 + c0, min(kSortCap, m - c0), a, b);
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int v = perm[s0 + i];
+      const int own = i / kSortCap;
+      int place = i - own * kSortCap;
+      for (int c0 = 0; c0 < m; c0 += kSortCap)
+        if (c0 / kSortCap != own) place += count_below(perm + s0 + c0, min(kSortCap, m - c0), v);
+      tmp[s0 + place] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += blockDim.x) perm[s0 + i] = tmp[s0 + i];
+    __syncthreads();
+  }
 }
 
 // Appends the lanes with pred set to list (one atomic per wavefront).
@@ -345,9 +509,15 @@ int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_
                        (const int32_t*)ws.count, ws.list, ws.n_list, ws.alist,
                        ws.n_alist);
   }
-  if (n > 0)
+  if (n > 0) {
     hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
                        ws.start, ws.perm);
+    // rank is dead after the fill: the large-brick path of the sort uses it
+    const long long nb_sort = ws.n_bricks < kBrickBlocks ? ws.n_bricks : kBrickBlocks;
+    hipLaunchKernelGGL(k_bin_sort, dim3((unsigned)(nb_sort > 0 ? nb_sort : 1)), dim3(kBlock), 0,
+                       st, (const int32_t*)ws.start, (const int32_t*)ws.list,
+                       (const int32_t*)ws.n_list, ws.perm, ws.rank);
+  }
   return check_launch("particle binning");
 }
 

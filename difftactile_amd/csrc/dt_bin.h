@@ -1,9 +1,9 @@
 // Per-substep particle binning for the brick-owned scatter passes.
 //
 // Particles are binned by the 4x4x4-cell brick of their stencil base.  After
-// launch_bin, perm[start[b] .. start[b+1]) lists the particles of brick b (in
-// no particular order inside the brick); the brick passes order each chunk of
-// a brick by cell in LDS.
+// launch_bin, perm[start[b] .. start[b+1]) lists the particles of brick b in
+// ascending index order; the brick passes order each chunk of a brick by cell
+// in LDS (stably, so the order is the same on every run).
 #pragma once
 #include "dt_common.h"
 
@@ -28,6 +28,7 @@ struct BinWs {
   int32_t* rank;    // [n]            position of the particle inside its brick
   int32_t* perm;    // [n]            particles in brick order
   float4* tiles;    // [n_bricks * kTileNodes] per-brick partial node sums
+  float* mat_part;  // [blocks of n particles][32] per-block material-adjoint partials
   void* scan_tmp;
   size_t scan_bytes;
   int32_t* slab_sum;  // [kSlabMax]            slab scan: particles per x-slab
@@ -55,31 +56,52 @@ void launch_bin_copy(const BinWs& from, const BinWs& to, int n, hipStream_t st);
 long long brick_total(const dt_mpm_params* prm);
 // Total bytes of the workspace for n particles; fills *ws when base != NULL.
 size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws);
-// Rank of this lane among the lanes of its key: lanes with equal keys in
-// consecutive lane order form a run, and the first lane of each run adds the
-// run's length to counts[key] with one atomic (LDS or global).  Active lanes
-// must form a prefix of the wavefront.  Returns the lane's slot among its key:
-// the count before the run's atomic plus its offset in the run.
-__device__ __forceinline__ int wave_run_rank(bool active, int key, int* counts) {
+// Rank of this lane among the active lanes of the wavefront that share its
+// key (any int; the lanes of a key need not be adjacent), and one atomic per
+// distinct key of the wavefront adding that key's lane count to counts[key]
+// (LDS or global).  Returns the count before the wavefront's atomic plus the
+// lane's rank among its key's lanes in lane order.  The loop runs once per
+// distinct key, with a wave-uniform trip count.
+__device__ __forceinline__ int wave_key_rank(bool active, int key, int* counts) {
   const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(key, 1);
-  const bool head = active && (lane == 0 || prev != key);
-  const unsigned long long hm = __ballot(head);
-  const unsigned long long act = __ballot(active);
-  const unsigned long long le = (2ull << lane) - 1ull;   // lanes <= this lane
-  const int my_head = active ? 63 - __clzll(hm & le) : 0;
-  int old = 0;
-  if (head) {
-    const unsigned long long later = hm & ~le;
-    const int end = later ? __ffsll((long long)later) - 1 : __popcll(act);
-    old = atomicAdd(&counts[key], end - lane);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long todo = __ballot(active);
+  int result = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int cur = __shfl(key, leader);
+    const bool mine = active && key == cur;
+    const unsigned long long peers = __ballot(mine);
+    int old = 0;
+    if (lane == leader) old = atomicAdd(&counts[cur], __popcll(peers));
+    old = __shfl(old, leader);
+    if (mine) result = old + __popcll(peers & below);
+    todo &= ~peers;
   }
-  return __shfl(old, my_head) + (lane - my_head);
+  return result;
+}
+
+// Mask of the active lanes whose 6-bit key equals this lane's (six ballots);
+// meaningful for active lanes only.
+__device__ __forceinline__ unsigned long long wave_match6(bool active, int key) {
+  unsigned long long m = __ballot(active);
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    const bool bit = (key >> b) & 1;
+    const unsigned long long bal = __ballot(active && bit);
+    m &= bit ? bal : ~bal;
+  }
+  return m;
 }
 
 // Persistent grid of the brick-owned passes (blocks stride over the list).
 constexpr int kBrickBlocks = 2048;
-// count -> brick scan -> occupied / active brick lists -> fill, on st.
+// count -> brick scan -> occupied / active brick lists -> fill -> sort of
+// each brick's particles by index, on st.  The counts and the brick starts
+// are integers and exact; the sort makes perm[start[b] .. start[b+1]) the
+// brick's particles in ascending index order whatever order the counting
+// atomics ran in, so the brick passes see the same particles in the same
+// order on every run.
 // count_clear: the workspace's brick counts are known to be zero (a previous
 // binning on it used the slab scan, which clears them behind it).
 // counted: key / rank / count already hold x's counts (made by the previous

@@ -17,7 +17,8 @@ struct Entry {
   std::vector<uint64_t> key;
   hipGraphExec_t exec;
 };
-constexpr size_t kMaxGraphs = 16;
+// a 1000-substep checkpointed trajectory replays ~50 distinct segment graphs
+constexpr size_t kMaxGraphs = 256;
 std::mutex g_mu;
 std::list<Entry> g_cache;   // front = most recently used
 bool g_enabled = true;

@@ -588,18 +588,23 @@ __device__ __forceinline__ void affine_adj(const MpmP& P, const M3& gA, M3& gtau
   }
 }
 
-// Adds the block's per-particle material adjoints (gmu, glam) into
-// g_mats[2 id], g_mats[2 id + 1]: per wavefront one shuffle sum per material
-// present, one LDS atomic per wavefront and material, then one global atomic
-// per block and material (ids beyond the LDS table add per wavefront).  Every
-// thread of the block calls it.
+// Adds the block's per-particle material adjoints (gmu, glam) of material
+// ids below kMatSlots: per wavefront one shuffle sum per material present,
+// stored in the wavefront's own LDS slots; then the sums over the block's
+// wavefronts, in wavefront order, are added to part[blockIdx.x][...] (the
+// block's own slots, accumulated over the launches of a trajectory: no
+// atomics, the same sums on every run) or, with part NULL, to g_mats by one
+// global atomic per block and material.  Ids at or beyond kMatSlots add per
+// wavefront to g_mats (atomics, order not fixed).  Every thread of the block
+// calls it.
 constexpr int kMatSlots = 16;
 __device__ __forceinline__ void block_mat_grad(bool active, int id, float gmu, float glam,
-                                               int n_mats, float* __restrict__ g_mats) {
-  __shared__ float red[2 * kMatSlots];
-  const int t = threadIdx.x, lane = t & 63;
-  if (t < 2 * kMatSlots) red[t] = 0.f;
-  __syncthreads();
+                                               int n_mats, float* __restrict__ g_mats,
+                                               float* __restrict__ part) {
+  __shared__ float red[kBlock / 64][2 * kMatSlots];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  // each wavefront clears its own row (its later writes follow in program order)
+  if (lane < 2 * kMatSlots) red[w][lane] = 0.f;
   unsigned long long todo = __ballot(active);
   while (todo) {
     const int leader = __ffsll((long long)todo) - 1;
@@ -613,8 +618,8 @@ __device__ __forceinline__ void block_mat_grad(bool active, int id, float gmu, f
     }
     if (lane == leader) {
       if (cur < kMatSlots) {
-        atomicAdd(&red[2 * cur], a);
-        atomicAdd(&red[2 * cur + 1], b);
+        red[w][2 * cur] = a;
+        red[w][2 * cur + 1] = b;
       } else {
         atomicAdd(&g_mats[2 * cur], a);
         atomicAdd(&g_mats[2 * cur + 1], b);
@@ -623,7 +628,53 @@ __device__ __forceinline__ void block_mat_grad(bool active, int id, float gmu, f
     todo &= ~__ballot(mine);
   }
   __syncthreads();
-  if (t < 2 * min(n_mats, kMatSlots) && red[t] != 0.f) atomicAdd(&g_mats[t], red[t]);
+  if (t < 2 * min(n_mats, kMatSlots)) {
+    float sum = 0.f;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) sum += red[q][t];
+    if (part)
+      part[(long long)blockIdx.x * (2 * kMatSlots) + t] += sum;
+    else if (sum != 0.f)
+      atomicAdd(&g_mats[t], sum);
+  }
+}
+
+// g_mats[v] += the sum over blocks b of part[b][v] (v < n_vals), in a fixed
+// order: thread t sums blocks t, t + 256, ..., then a pairwise tree over the
+// threads.  One block.
+__global__ __launch_bounds__(kBlock) void k_mat_part_reduce(int n_blocks, int n_vals,
+                                                            const float* __restrict__ part,
+                                                            float* __restrict__ g_mats) {
+  __shared__ float red[kBlock][2 * kMatSlots + 1];
+  const int t = threadIdx.x;
+  float acc[2 * kMatSlots];
+#pragma unroll
+  for (int v = 0; v < 2 * kMatSlots; ++v) acc[v] = 0.f;
+  for (int b = t; b < n_blocks; b += kBlock) {
+    const float4* q = reinterpret_cast<const float4*>(part + (long long)b * (2 * kMatSlots));
+#pragma unroll
+    for (int v = 0; v < 2 * kMatSlots / 4; ++v) {
+      const float4 f = q[v];
+      acc[4 * v] += f.x;
+      acc[4 * v + 1] += f.y;
+      acc[4 * v + 2] += f.z;
+      acc[4 * v + 3] += f.w;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 2 * kMatSlots; ++v) red[t][v] = acc[v];
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int v = 0; v < 2 * kMatSlots; ++v) red[t][v] += red[t + h][v];
+    __syncthreads();
+  }
+  if (t < n_vals) g_mats[t] += red[0][t];
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_f(long long n, float* __restrict__ a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.f;
 }
 
 // p2g.grad (reference kernel form): emits gU, gS, gV for svd_grad, the direct
@@ -760,7 +811,9 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
   add_m3(gV + 9ll * p, gvv);
   add_m3(gS + 9ll * p, gs);
   }
-  if (g_mats) block_mat_grad(active, active ? (mat_id ? mat_id[p] : 0) : 0, gmu, glam, n_mats, g_mats);
+  if (g_mats)
+    block_mat_grad(active, active ? (mat_id ? mat_id[p] : 0) : 0, gmu, glam, n_mats, g_mats,
+                   nullptr);
 }
 
 // fused adjoint of (compute_new_F + svd + return map + p2g): writes the
@@ -860,7 +913,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
     const float* __restrict__ F, const float* __restrict__ ggrid4,
     const float* __restrict__ gFnext, float* __restrict__ gx_io, float* __restrict__ gv_out,
     float* __restrict__ gC_out, float* __restrict__ gF_out, int n_mats,
-    float* __restrict__ g_mats) {
+    float* __restrict__ g_mats, float* __restrict__ mat_part) {
   __shared__ float park[21][kBlock];   // U (9), sigma (3), V (9) of each thread's particle
   const int p = xcd_block() * blockDim.x + threadIdx.x;
   float gmu = 0.f, glam = 0.f;
@@ -874,7 +927,8 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_fused_adj(
   if (active)
     p2g_fused_adj_particle<kExact>(P, mats, n, mat_id, x, v, C, F, ggrid4, gFnext, gx_io, gv_out,
                                    gC_out, gF_out, p, park, gmu, glam);
-  block_mat_grad(active, active ? (mat_id ? mat_id[p] : 0) : 0, gmu, glam, n_mats, g_mats);
+  block_mat_grad(active, active ? (mat_id ? mat_id[p] : 0) : 0, gmu, glam, n_mats, g_mats,
+                 mat_part);
 }
 
 // ----------------------------------------------------------------------------
@@ -1298,8 +1352,7 @@ __device__ __forceinline__ void g2p_core(const MpmP& P, const Stencil& s, const 
 }
 
 // kCount: also makes the brick counts of the next binning from x1 (the key,
-// rank and count arrays k_bin_count would write; active lanes form a prefix
-// of every wavefront, as wave_run_rank needs)
+// rank and count arrays k_bin_count would write)
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __restrict__ x,
                                                 const float* __restrict__ gv, int sgv,
@@ -1331,7 +1384,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
     if (kCount) k = bin_key(nx, P.inv_dx, P.n_grid, env_x0(P, p)) >> 6;
   }
   if (kCount) {
-    const int r = wave_run_rank(active, k, bcount);
+    const int r = wave_key_rank(active, k, bcount);
     if (active) {
       bkey[p] = k;
       brank[p] = r;
@@ -1451,8 +1504,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
 // brick-owned scatter over binned particles: block b processes the particles
 // of brick b in chunks of 256 and owns the 6x6x6-node tile of partial sums its
 // stencils reach.
-//   per chunk  1. thread t computes particle t; an LDS counting sort by cell
-//                 (64 bins) gives its staging slot, where it stages (w, b, M);
+//   per chunk  1. thread t computes particle t; a stable LDS counting sort by
+//                 cell (64 bins, per-wavefront counts) gives its staging slot,
+//                 where it stages (w, b, M);
 //              2. work item (cell, j, k) sums, over the cell's particles, the
 //                 contributions w_o (b + M o) to the three nodes o = (i, j, k),
 //                 i = 0..2, into its own LDS slots (lanes of one cell read the
@@ -1473,9 +1527,10 @@ struct BrickStage {
     float4 gtile[kTileNodes];
   };
   float4 slot[kSlots + 1];   // (cell, j, k, i): (sum w u0, w u1, w u2, w); last stays 0
-  int cs[65];                // cell starts of the chunk (staging slots)
-  int hist[64];              // particles per cell of the chunk
-  int cells[64];             // occupied cells of the chunk
+  uint16_t cs[66];           // cell starts of the chunk (staging slots)
+  uint16_t hist[kBlock / 64][64]; // per wavefront: particles per cell of the chunk,
+                                  //   then its first staging slot in each cell
+  uint8_t cells[64];         // occupied cells of the chunk
   int n_cells;
 };
 
@@ -1490,19 +1545,34 @@ __device__ __forceinline__ int local_cell(const Stencil& s) {
   return ((s.base[0] & 3) * 4 + (s.base[1] & 3)) * 4 + (s.base[2] & 3);
 }
 
-// wavefront 0: cell starts (exclusive scan of hist) and occupied-cell list
+// wavefront 0: cell starts (exclusive scan of the per-cell totals), each
+// wavefront's first slot in each cell (its particles follow those of the
+// wavefronts before it: the staging order is the chunk order, cell by cell,
+// so it is the same on every run) and the occupied-cell list
 __device__ __forceinline__ void chunk_scan(BrickStage& sh) {
+  constexpr int kW = kBlock / 64;
   const int l = threadIdx.x;   // 0..63, one full wavefront
-  const int h = sh.hist[l];
+  int hw[kW], h = 0;
+#pragma unroll
+  for (int w = 0; w < kW; ++w) {
+    hw[w] = sh.hist[w][l];
+    h += hw[w];
+  }
   int incl = h;
   for (int d = 1; d < 64; d <<= 1) {
     const int u = __shfl_up(incl, d);
     if (l >= d) incl += u;
   }
-  sh.cs[l] = incl - h;
-  if (l == 63) sh.cs[64] = incl;
+  int at = incl - h;
+  sh.cs[l] = (uint16_t)at;
+  if (l == 63) sh.cs[64] = (uint16_t)incl;
+#pragma unroll
+  for (int w = 0; w < kW; ++w) {
+    sh.hist[w][l] = (uint16_t)at;
+    at += hw[w];
+  }
   const unsigned long long m = __ballot(h > 0);
-  if (h > 0) sh.cells[__popcll(m & ((1ull << l) - 1ull))] = l;
+  if (h > 0) sh.cells[__popcll(m & ((1ull << l) - 1ull))] = (uint8_t)l;
   if (l == 0) sh.n_cells = __popcll(m);
 }
 
@@ -1643,7 +1713,7 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
         if (t < kTileNodes) sh.gtile[t] = tq;
         if (last && b2 >= 0) tq2 = fetch(b2);
       }
-      if (t < 64) sh.hist[t] = 0;
+      sh.hist[t >> 6][t & 63] = 0;
       __syncthreads();
       Stencil s;
       float bb[3];
@@ -1653,9 +1723,15 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
         compute(in, p, b, s, bb, M);
         lc = local_cell(s);
       }
-      // the chunk's particles arrive mostly in cell order: one LDS atomic per
-      // run of equal cells
-      const int r = wave_run_rank(i < c1, lc, sh.hist);
+      // rank among this wavefront's particles of the same cell; the first of
+      // them writes the wavefront's count of that cell (no atomics)
+      const unsigned long long peers = wave_match6(i < c1, lc);
+      const int wv = t >> 6;
+      int r = 0;
+      if (i < c1) {
+        r = __popcll(peers & ((1ull << (t & 63)) - 1ull));
+        if (r == 0) sh.hist[wv][lc] = (uint16_t)__popcll(peers);
+      }
       if (has_next) {
         p = pn;
         load(p, in);
@@ -1663,7 +1739,7 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
       __syncthreads();
       if (t < 64) chunk_scan(sh);
       __syncthreads();
-      if (i < c1) stage_particle(sh, sh.cs[lc] + r, s, bb, M);
+      if (i < c1) stage_particle(sh, sh.hist[wv][lc] + r, s, bb, M);
       __syncthreads();
       brick_items(sh);
       __syncthreads();
@@ -1800,6 +1876,23 @@ __global__ __launch_bounds__(kBlock) void k_scatter_soa(long long n, int width,
   dst[r * width + c] = src[e];
 }
 
+// [w][n] -> [w][n] reordered: gather dst[c][i] = src[c][perm[i]], or scatter
+// dst[c][perm[i]] = src[c][i]
+template <bool kScatter>
+__global__ __launch_bounds__(kBlock) void k_permute_soa(long long n, int width,
+                                                        const int32_t* __restrict__ perm,
+                                                        const float* __restrict__ src,
+                                                        float* __restrict__ dst) {
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * width) return;
+  const long long c = e / n, i = e % n;
+  const long long r = c * n + perm[i];
+  if (kScatter)
+    dst[r] = src[e];
+  else
+    dst[e] = src[r];
+}
+
 __global__ __launch_bounds__(kBlock) void k_scatter_rows(long long total, int width,
                                                          const int32_t* __restrict__ perm,
                                                          const float* __restrict__ src,
@@ -1852,10 +1945,11 @@ static void launch_p2g_fused_adj(const MpmP& P, hipStream_t st, const dt_materia
                                  int n_mats, int n, const int32_t* mat_id, const float* x,
                                  const float* v, const float* C, const float* F,
                                  const float* ggrid4, const float* gF1, float* gx, float* gv,
-                                 float* gC, float* gF, float* g_mats) {
+                                 float* gC, float* gF, float* g_mats,
+                                 float* mat_part = nullptr) {
 #define DT_P2G_ADJ(E, G)                                                                       \
   hipLaunchKernelGGL((k_p2g_fused_adj<E, G>), dim3(nblocks(n)), dim3(kBlock), 0, st, P, mats, n, \
-                     mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF, n_mats, g_mats)
+                     mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF, n_mats, g_mats, mat_part)
   if (P.svd_exact) {
     if (g_mats) DT_P2G_ADJ(true, true);
     else DT_P2G_ADJ(true, false);
@@ -1941,7 +2035,8 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n_
                        const float* gx1, const float* gv1, const float* gC1, const float* gF1,
                        float* ggrid4, float* ggv4, float* gx, float* gv, float* gC, float* gF,
                        float* g_mats, hipStream_t st, const BinWs* ws = nullptr,
-                       bool binned = false, bool count_clear = false) {
+                       bool binned = false, bool count_clear = false,
+                       float* mat_part = nullptr) {
   MpmP P = to_dev(*prm);
   long long nn = grid_nodes(prm);
   if (ws) {
@@ -1974,7 +2069,7 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n_
   if (n > 0) {
     StageTimer tm(STAGE_P2G_ADJ, st);
     launch_p2g_fused_adj(P, st, mats, n_mats, n, mat_id, x, v, C, F, ggrid4, gF1, gx, gv, gC, gF,
-                         g_mats);
+                         g_mats, mat_part);
   }
   return check_launch("dt_mpm_substep_adj");
 }
@@ -2347,8 +2442,15 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   long long nn = grid_nodes(prm);
   hipStream_t st = DT_STREAM(stream);
   float *a[4] = {gx, gv, gC, gF}, *b[4] = {gx2, gv2, gC2, gF2};
+  // material adjoints with a workspace: per-block partials accumulated over
+  // the sweep, reduced in a fixed order at its end (the same sums every run)
+  const bool mat_det = workspace && g_mats && n > 0 && n_substeps > 0;
+  const long long part_n = (long long)nblocks(n) * 2 * kMatSlots;
   auto body = [&](hipStream_t cs) -> int {
     float *p[4] = {gx, gv, gC, gF}, *q[4] = {gx2, gv2, gC2, gF2};
+    if (mat_det)
+      hipLaunchKernelGGL(k_zero_f, dim3(nblocks(part_n)), dim3(kBlock), 0, cs, part_n,
+                         ws.mat_part);
     for (int s = n_substeps - 1; s >= 0; --s) {
       long long o = (long long)s * n;
       BinWs slot = hist ? bin_slot(ws, s, n) : ws;
@@ -2357,10 +2459,14 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                                  F + 9 * o, grids4 + (long long)s * nn * 4,
                                  grids_v4 + (long long)s * nn * 4, p[0], p[1], p[2], p[3], g_grid4,
                                  g_grid_v4, q[0], q[1], q[2], q[3], g_mats, cs, wp, hist,
-                                 s != n_substeps - 1);
+                                 s != n_substeps - 1, mat_det ? ws.mat_part : nullptr);
       if (r) return r;
       for (int k = 0; k < 4; ++k) { float* t = p[k]; p[k] = q[k]; q[k] = t; }
     }
+    if (mat_det)
+      hipLaunchKernelGGL(k_mat_part_reduce, dim3(1), dim3(kBlock), 0, cs, nblocks(n),
+                         2 * (n_mats < kMatSlots ? n_mats : kMatSlots), (const float*)ws.mat_part,
+                         g_mats);
     return DT_OK;
   };
   std::vector<uint64_t> key{hist ? 4u : 2u};
@@ -2866,6 +2972,21 @@ int dt_gather_rows_soa(int32_t n, int32_t width, const int32_t* perm, const floa
   hipLaunchKernelGGL(k_gather_soa, dim3(nblocks(total)), dim3(kBlock), 0, DT_STREAM(stream),
                      (long long)n, width, perm, src_rows, dst_soa);
   return check_launch("dt_gather_rows_soa");
+}
+
+int dt_permute_soa(int32_t n, int32_t width, const int32_t* perm, int32_t scatter,
+                   const float* src_soa, float* dst_soa, void* stream) {
+  DT_REQUIRE(n >= 0 && width > 0, "bad size");
+  if (n == 0) return DT_OK;
+  DT_REQUIRE(perm && src_soa && dst_soa && src_soa != dst_soa, "NULL or aliased pointer");
+  long long total = (long long)n * width;
+  if (scatter)
+    hipLaunchKernelGGL(k_permute_soa<true>, dim3(nblocks(total)), dim3(kBlock), 0,
+                       DT_STREAM(stream), (long long)n, width, perm, src_soa, dst_soa);
+  else
+    hipLaunchKernelGGL(k_permute_soa<false>, dim3(nblocks(total)), dim3(kBlock), 0,
+                       DT_STREAM(stream), (long long)n, width, perm, src_soa, dst_soa);
+  return check_launch("dt_permute_soa");
 }
 
 int dt_scatter_rows_soa(int32_t n, int32_t width, const int32_t* perm, const float* src_soa,

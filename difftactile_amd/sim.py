@@ -13,7 +13,9 @@ Here the same structure lives entirely in HBM:
   does not recompute them.  Both give the same gradients.
 
 Particles are ordered by cell key (a stable integer radix sort) at the start
-of each call; the order is undone on output so callers see their own order.
+of each call and again at every checkpoint (``resort``); the order is undone
+on output, and the backward pass undoes each re-sort, so callers see their own
+order.
 """
 import ctypes
 
@@ -41,7 +43,7 @@ class MPMSim:
 
     def __init__(self, prm, materials, n_particles, mat_id=None, device="cuda",
                  checkpoint_every=50, sort=True, sparse_grid=True, resident_bytes=None,
-                 material_grad=False, env_id=None):
+                 material_grad=False, env_id=None, resort=True):
         self.prm = prm
         if resident_bytes is None:
             # a quarter of the device's memory for resident segments
@@ -83,7 +85,12 @@ class MPMSim:
             self.env_id = None
         self.checkpoint_every = int(checkpoint_every)
         self.sort = bool(sort)
+        # resort: with sort, the particles are sorted by cell again at every
+        # checkpoint of a rollout (the order decays as they move: equal brick
+        # keys stop being neighbours and grid reads stop being coalesced)
+        self.resort = bool(resort)
         self._ws = {}
+        self._free = {}
         # material_grad: backward passes accumulate d loss / d (mu, lam) of each
         # material row here (cleared by zero_material_grad), as the reference's
         # mu_0.grad / lambda_0.grad accumulate
@@ -148,9 +155,75 @@ class MPMSim:
                   tmp.data_ptr(), tb, _lib.stream())
         return keys_out, perm
 
-    def _gather(self, perm, t, width):
+    # -- buffer pool ------------------------------------------------------
+    # A rollout takes its state, id, frame and checkpoint buffers from here
+    # and returns them after its backward pass, so the next rollout runs on
+    # the same device addresses and the library's graph cache replays its
+    # captured segments instead of capturing them again.
+    def _take(self, taken, key, make):
+        free = self._free.setdefault(key, [])
+        bufs = free.pop() if free else make()
+        taken.append((key, bufs))
+        return bufs
+
+    def _give_back(self, taken):
+        for key, bufs in taken:
+            self._free.setdefault(key, []).append(bufs)
+        taken.clear()
+
+    def release_buffers(self):
+        """Frees the pooled rollout buffers (resident frames included)."""
+        self._free.clear()
+
+    def _state_bufs(self, widths):
+        return [torch.empty((w, self.n), device=self.device) for w in widths]
+
+    def _id_buf(self):
+        return torch.empty(self.n, dtype=torch.int32, device=self.device)
+
+    def _permute(self, q, t, width, scatter=False, out=None):
+        """[width][n] reordered by q: gather out[:, i] = t[:, q[i]], or the
+        inverse, scatter out[:, q[i]] = t[:, i] (int32 arrays as bit patterns)."""
+        out = torch.empty_like(t) if out is None else out
+        src, dst = (t, out) if t.dtype == torch.float32 else (t.view(torch.float32),
+                                                              out.view(torch.float32))
+        _lib.call("dt_permute_soa", self.n, width, _lib.ptr(q, torch.int32), int(scatter),
+                  _lib.ptr(src), _lib.ptr(dst), _lib.stream())
+        return out
+
+    def resort_state(self, state, mat_id, env, order, out=None):
+        """Sorts the SoA state (x first) by cell key again.  Returns the
+        permuted (state, mat_id, env, order) and q, the permutation applied
+        (new position i holds the particle at old position q[i]).  out: the
+        buffers to permute into (state list, mat_id, env, order, q)."""
+        n = self.n
+        keys = self._buf("keys", (n,), torch.int32)
+        keys_out = self._buf("keys_out", (n,), torch.int32)
+        if out is None:
+            out = ([torch.empty_like(t) for t in state],
+                   None if mat_id is None else torch.empty_like(mat_id),
+                   None if env is None else torch.empty_like(env),
+                   None if order is None else torch.empty_like(order), self._id_buf())
+        st_o, mid_o, env_o, ord_o, q = out
+        self._set_env(env)
+        _lib.call("dt_mpm_cell_keys", self.c_prm, n, _lib.ptr(state[0]),
+                  _lib.ptr(keys, torch.int32), _lib.stream())
+        tb = int(_lib.lib().dt_sort_tmp_bytes(n))
+        tmp = self._buf("sort_tmp", (max(tb, 1),), torch.uint8)
+        _lib.call("dt_sort_by_key", n, self.prm.key_bits(), _lib.ptr(keys, torch.int32),
+                  _lib.ptr(keys_out, torch.int32), _lib.ptr(q, torch.int32), tmp.data_ptr(), tb,
+                  _lib.stream())
+        state = [self._permute(q, t, t.shape[0], out=o) for t, o in zip(state, st_o)]
+        mat_id = None if mat_id is None else self._permute(q, mat_id, 1, out=mid_o)
+        env = None if env is None else self._permute(q, env, 1, out=env_o)
+        order = None if order is None else self._permute(q, order, 1, out=ord_o)
+        self._set_env(env)
+        return state, mat_id, env, order, q
+
+    def _gather(self, perm, t, width, out=None):
         """[n][width] in caller order -> [width][n] in sorted order."""
-        out = torch.empty((width, self.n), dtype=torch.float32, device=self.device)
+        if out is None:
+            out = torch.empty((width, self.n), dtype=torch.float32, device=self.device)
         src = t.contiguous()
         _lib.call("dt_gather_rows_soa", self.n, width, perm, src, out, _lib.stream())
         return out
@@ -163,9 +236,9 @@ class MPMSim:
                   _lib.ptr(out), _lib.stream())
         return out
 
-    def _gather_i(self, perm, t):
+    def _gather_i(self, perm, t, out=None):
         # int32 ids gathered through the float path (bit pattern preserved)
-        out = torch.empty_like(t)
+        out = torch.empty_like(t) if out is None else out
         _lib.call("dt_gather_rows", self.n, 1, _lib.ptr(perm, torch.int32),
                   _lib.ptr(t.view(torch.float32)), _lib.ptr(out.view(torch.float32)),
                   _lib.stream())
@@ -255,65 +328,100 @@ class _Rollout(torch.autograd.Function):
                 raise ValueError("state tensors must be float32 on the GPU")
         x, v, C, F = (t.contiguous() for t in (x, v, C, F))
         shapes = [t.shape for t in (x, v, C, F)]
+        W = _Rollout.WIDTHS
+        taken = []
         perm = sim.sort_order(x)[1] if (sim.sort and sim.n > 0) else None
-        state = [sim._gather(perm, t, w) for t, w in zip((x, v, C, F), _Rollout.WIDTHS)]
+        sbuf = [sim._take(taken, ("state", 0), lambda: sim._state_bufs(W)),
+                sim._take(taken, ("state", 1), lambda: sim._state_bufs(W))]
+        state = [sim._gather(perm, t, w, out=o) for t, w, o in zip((x, v, C, F), W, sbuf[0])]
+        ids0 = sim._take(taken, ("ids", 0), lambda: [sim._id_buf() for _ in range(4)])
         if sim.mat_id is not None and perm is not None:
-            mat_id = sim._gather_i(perm, sim.mat_id)
+            mat_id = sim._gather_i(perm, sim.mat_id, out=ids0[0])
         else:
             mat_id = sim.mat_id
         env = sim.env_id
         if env is not None and perm is not None:
-            env = sim._gather_i(perm, env)
+            env = sim._gather_i(perm, env, out=ids0[1])
         sim._set_env(env)
-        ckpts = []
+        # order[i]: the caller's index of the particle at position i
+        order = perm
+        ckpts, meta = [], []
         used = 0
-        for (s, k) in _segments(n_substeps, sim.checkpoint_every):
+        n, nn = sim.n, sim.prm.n_nodes
+        for i, (s, k) in enumerate(_segments(n_substeps, sim.checkpoint_every)):
+            q = None
+            if s > 0 and order is not None and sim.resort:
+                ids = sim._take(taken, ("ids", i), lambda: [sim._id_buf() for _ in range(4)])
+                out = (sbuf[i % 2], None if mat_id is None else ids[0],
+                       None if env is None else ids[1], ids[2], ids[3])
+                state, mat_id, env, order, q = sim.resort_state(state, mat_id, env, order,
+                                                                out=out)
+            meta.append((mat_id, env, q))
             nbytes = sim.segment_bytes(k)
             if ctx is not None and used + nbytes <= sim.resident_bytes:
                 # resident segment: keep frames and grids for the backward pass
                 used += nbytes
-                n, nn = sim.n, sim.prm.n_nodes
-                fr = [torch.empty((k + 1, w, n), device=sim.device) for w in _Rollout.WIDTHS]
-                grids = (torch.empty((k, nn, 4), device=sim.device),
-                         torch.empty((k, nn, 4), device=sim.device))
-                for f, t in zip(fr, state):
+                fr = sim._take(taken, ("frames", i, k), lambda: (
+                    [torch.empty((k + 1, w, n), device=sim.device) for w in W],
+                    (torch.empty((k, nn, 4), device=sim.device),
+                     torch.empty((k, nn, 4), device=sim.device))))
+                frames, grids = fr
+                for f, t in zip(frames, state):
                     f[0].copy_(t)
-                sim.run_frames(*fr, k, mat_id, keep_grids=True, grids=grids)
-                for f, t in zip(fr, state):
+                sim.run_frames(*frames, k, mat_id, keep_grids=True, grids=grids)
+                for f, t in zip(frames, state):
                     t.copy_(f[k])
-                ckpts.append((fr, grids))
+                ckpts.append((frames, grids))
             else:
                 if ctx is not None:
-                    ckpts.append([t.clone() for t in state])
+                    ck = sim._take(taken, ("ckpt", i), lambda: sim._state_bufs(W))
+                    for c, t in zip(ck, state):
+                        c.copy_(t)
+                    ckpts.append(ck)
                 sim.advance(state, k, mat_id)
-        out = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(state, _Rollout.WIDTHS, shapes)]
+        out = [sim._scatter(order, t, w, sh) for t, w, sh in zip(state, W, shapes)]
         if ctx is not None:
             ctx.sim = sim
             ctx.n_substeps = n_substeps
             ctx.perm = perm
-            ctx.mat_id = mat_id
-            ctx.env = env
+            ctx.order = order
+            ctx.meta = meta
             ctx.ckpts = ckpts
             ctx.shapes = shapes
+            ctx.taken = taken
+        else:
+            sim._give_back(taken)
         return tuple(out)
 
     @staticmethod
     def backward(ctx, gx, gv, gC, gF):
         sim = ctx.sim
-        perm = ctx.perm
+        W = _Rollout.WIDTHS
+        taken = ctx.taken
+        gbuf = [sim._take(taken, ("grad", 0), lambda: sim._state_bufs(W)),
+                sim._take(taken, ("grad", 1), lambda: sim._state_bufs(W))]
         g = [t if t is not None else torch.zeros(sh, device=sim.device)
              for t, sh in zip((gx, gv, gC, gF), ctx.shapes)]
-        g = [sim._gather(perm, t.float(), w) for t, w in zip(g, _Rollout.WIDTHS)]
+        g = [sim._gather(ctx.order, t.float(), w, out=o) for t, w, o in zip(g, W, gbuf[0])]
+        cur = 0
         segs = _segments(ctx.n_substeps, sim.checkpoint_every)
-        sim._set_env(ctx.env)
-        for (s, k), ck in zip(reversed(segs), reversed(ctx.ckpts)):
+        for (s, k), ck, (mat_id, env, q) in zip(reversed(segs), reversed(ctx.ckpts),
+                                                reversed(ctx.meta)):
+            sim._set_env(env)
             if isinstance(ck, tuple):           # resident segment
                 (X, V, Cb, Fb), (G4, GV4) = ck
             else:                               # recompute from the checkpoint
                 X, V, Cb, Fb = sim._frames(k, "R%d" % k)
                 X[0].copy_(ck[0]); V[0].copy_(ck[1]); Cb[0].copy_(ck[2]); Fb[0].copy_(ck[3])
-                G4, GV4 = sim.run_frames(X, V, Cb, Fb, k, ctx.mat_id, keep_grids=True)
-            sim.adj_frames(X, V, Cb, Fb, k, ctx.mat_id, G4, GV4, g)
+                G4, GV4 = sim.run_frames(X, V, Cb, Fb, k, mat_id, keep_grids=True)
+            sim.adj_frames(X, V, Cb, Fb, k, mat_id, G4, GV4, g)
+            if q is not None:
+                # back to the order before this segment's re-sort
+                cur = 1 - cur
+                g = [sim._permute(q, t, w, scatter=True, out=o)
+                     for t, w, o in zip(g, W, gbuf[cur])]
         ctx.ckpts = None
-        g = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(g, _Rollout.WIDTHS, ctx.shapes)]
+        ctx.meta = None
+        g = [sim._scatter(ctx.perm, t, w, sh) for t, w, sh in zip(g, W, ctx.shapes)]
+        sim._give_back(taken)
         return (None, None, *g)

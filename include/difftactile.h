@@ -94,8 +94,11 @@ typedef struct dt_material {
    p2g.grad accumulates mu_0.grad / lambda_0.grad (mpm_elastic.py:68-69,
    multi_obj.py:67-68): the stress terms plus, inside an active return map,
    the von Mises threshold tau_y / (2 mu) (mpm_plastic.py:157-161) and the
-   Drucker-Prager cone slope (3 lam + 2 mu) / (2 mu) alpha.  The sums are
-   float atomics (order not fixed). */
+   Drucker-Prager cone slope (3 lam + 2 mu) / (2 mu) alpha.  dt_mpm_run_adj
+   with a workspace sums them in a fixed order (per-block partials in the
+   workspace, reduced at the end of the sweep), so they repeat bit for bit;
+   the single-substep entry points add per-block sums with float atomics
+   (order not fixed). */
 
 /* FEM pad material and integration constants (fem_sensor.py:35-49). */
 typedef struct dt_fem_params {
@@ -135,7 +138,7 @@ int dt_timing_read(int32_t stage, double* total_ms, int64_t* launches);
 /* ---- graph replay (no reference counterpart: launch-overhead removal) ----
    dt_mpm_run_fwd / dt_mpm_run_adj / dt_mpm_advance capture their launch
    sequence into a HIP graph the first time they see a set of arguments and
-   replay it with one hipGraphLaunch afterwards (16 most recent argument sets
+   replay it with one hipGraphLaunch afterwards (256 most recent argument sets
    kept per process).  On by default; off while stage timing is on. */
 int dt_graphs_enable(int32_t on);
 int dt_graph_stats(int64_t* hits, int64_t* misses);
@@ -423,6 +426,12 @@ int dt_scatter_rows(int32_t n, int32_t width, const int32_t* perm, const float* 
 /* [n][width] -> [width][n]: dst[c*n + i] = src[perm[i]*width + c] (perm NULL = identity). */
 int dt_gather_rows_soa(int32_t n, int32_t width, const int32_t* perm, const float* src_rows,
                        float* dst_soa, void* stream);
+/* [width][n] -> [width][n] reordered (no reference counterpart: the re-sort of
+   a trajectory's particles at its checkpoints): scatter 0: dst[c*n + i] =
+   src[c*n + perm[i]]; scatter 1: dst[c*n + perm[i]] = src[c*n + i] (the
+   inverse).  Any 4-byte element (int32 ids as their bit patterns). */
+int dt_permute_soa(int32_t n, int32_t width, const int32_t* perm, int32_t scatter,
+                   const float* src_soa, float* dst_soa, void* stream);
 /* [width][n] -> [n][width]: dst[perm[i]*width + c] = src[c*n + i] (perm NULL = identity). */
 int dt_scatter_rows_soa(int32_t n, int32_t width, const int32_t* perm, const float* src_soa,
                         float* dst_rows, void* stream);

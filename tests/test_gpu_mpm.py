@@ -449,13 +449,22 @@ def _bin_arrays(c_prm, n, ws, nbytes, n_bricks):
 
 # 132^3 has 35,937 bricks: past the slab scan, so the device-wide
 # scan and the flags kernel run
-@pytest.mark.parametrize("n_grid,n,sort", [(32, 20000, False), (64, 50000, True), (30, 7000, False),
-                                          (132, 30000, True)])
-def test_binning_is_a_counting_sort_by_stencil_brick(gpu, n_grid, n, sort):
+@pytest.mark.parametrize("n_grid,n,sort,cluster", [(32, 20000, False, False),
+                                                  (64, 50000, True, False),
+                                                  (30, 7000, False, False),
+                                                  (132, 30000, True, False),
+                                                  (32, 30000, False, True)])
+def test_binning_is_a_counting_sort_by_stencil_brick(gpu, n_grid, n, sort, cluster):
+    """The binning permutation is the stable counting sort by brick, bit for
+    bit: every brick's particles in ascending index order whatever order the
+    counting atomics ran in (cluster: ~30k particles in a few bricks, past the
+    2048-particle LDS sort of one brick)."""
     prm = default_prm(n_grid)
     rng = np.random.default_rng(n_grid + n)
     # the whole box and a margin outside it (clamped bases)
     x = rng.uniform(-0.05, 1.05, size=(n, 3)).astype(np.float32)
+    if cluster:
+        x = rng.uniform(0.4, 0.4 + 6.0 / n_grid, size=(n, 3)).astype(np.float32)
     keys = om.bin_keys_np(x, prm.inv_dx, n_grid)
     if sort:
         x = x[np.argsort(keys, kind="stable")]
@@ -469,10 +478,11 @@ def test_binning_is_a_counting_sort_by_stencil_brick(gpu, n_grid, n, sort):
     call("dt_mpm_bin", c_prm, n, xg, ws, nbytes, _lib.stream())
     n_bricks = ((n_grid + 3) // 4) ** 3
     perm, start = _bin_arrays(c_prm, n, ws, nbytes, n_bricks)
-    assert np.array_equal(np.sort(perm), np.arange(n))
-    assert np.all(np.diff(bricks[perm]) >= 0)
+    assert np.array_equal(perm, np.argsort(bricks, kind="stable"))
     want = np.searchsorted(np.sort(bricks), np.arange(n_bricks + 1), side="left")
     assert np.array_equal(start, want)
+    if cluster:
+        assert np.diff(start).max() > 2048
 
 
 def _active_list(c_prm, n, ws, nbytes):
