@@ -3,9 +3,9 @@
 Workload (one GPU): the multi-material sphere of 1M particles on a 128^3 grid
 (rigid elastic core E=1e7 inside a Drucker-Prager shell), float32.  One step
 is a 1000-substep trajectory forward plus its backward, checkpointed every
-50 substeps: segments whose frames and grids fit the resident budget (a
-quarter of HBM) are kept from the forward pass, the others are recomputed
-from their checkpoints in the backward pass.  Every step starts from the
+50 substeps: segments whose frames and grids fit the resident budget (half
+of the 288 GB of HBM) are kept from the forward pass, the others are
+recomputed from their checkpoints in the backward pass.  Every step starts from the
 same deformed state (the undeformed sphere advanced 200 substeps), and the
 inputs are resident in HBM before the timed region.
 
@@ -41,7 +41,7 @@ def parse():
                     help="sphere workload: substeps of the trajectory one step runs fwd+bwd")
     ap.add_argument("--predeform", type=int, default=200,
                     help="sphere workload: substeps advanced before timing (the start state)")
-    ap.add_argument("--resident-frac", type=float, default=0.25,
+    ap.add_argument("--resident-frac", type=float, default=0.5,
                     help="sphere workload: fraction of HBM for resident trajectory segments")
     ap.add_argument("--no-resort", action="store_true",
                     help="sphere workload: sort the particles once per trajectory, not at "
