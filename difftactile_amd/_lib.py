@@ -185,6 +185,13 @@ _SIGS = {
                                           _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp,
                                           _P(ctypes.c_float), _vp, _vp, _vp, _c_int, _vp, _c_int,
                                           _vp, _vp, _vp]),
+    "dt_contact_collision_fx": (_c_int, [_P(DtMpmParams), _P(DtContactParams), _vp, _c_int, _vp,
+                                         _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                         _P(ctypes.c_float), _vp, _c_int, _vp, _vp, _vp, _vp]),
+    "dt_contact_collision_adj_fx": (_c_int, [_P(DtMpmParams), _P(DtContactParams), _vp, _c_int,
+                                             _vp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp,
+                                             _vp, _P(ctypes.c_float), _vp, _vp, _vp, _c_int, _vp,
+                                             _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "dt_loss_sum_positions": (_c_int, [_c_int, _c_int, _vp, _vp, _vp]),
     "dt_loss_com_adj": (_c_int, [_c_int, _c_int, _P(ctypes.c_float), _vp, _vp]),
     "dt_markers_fwd": (_c_int, [_c_int, _vp, _vp, _vp, _P(ctypes.c_float), _P(ctypes.c_float),

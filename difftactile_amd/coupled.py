@@ -78,6 +78,10 @@ class Pad:
         self.vert_start = torch.as_tensor(vs, device=self.device)
         self.vert_tets = torch.as_tensor(vt, device=self.device)
         self.tet_buf = torch.empty(self.n_tets, 4, 4, device=self.device)
+        # exact (fixed-point) vertex sums of the contact terms; left zero by
+        # every call
+        self.fx_acc = torch.zeros(6 * self.n_verts, dtype=torch.int64, device=self.device)
+        self.fx_scale = torch.zeros(4, dtype=torch.int32, device=self.device)
 
     def od(self):
         return self.out_dir.ctypes.data_as(_lib.ctypes.POINTER(_lib.ctypes.c_float))
@@ -214,9 +218,10 @@ class CoupledSim:
         exts = []
         for k, pad in enumerate(self.pads):
             ext = torch.zeros(pad.n_verts, 3, device=sim.device)
-            _lib.call("dt_contact_collision", prm, self.cp, P(g4), 4, g4.data_ptr() + 12, 4,
+            _lib.call("dt_contact_collision_fx", prm, self.cp, P(g4), 4, g4.data_ptr() + 12, 4,
                       P(idx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
-                      P(vel_u[k]), pad.od(), P(gf), P(ext), s)
+                      P(vel_u[k]), pad.od(), P(gf), pad.n_verts, P(ext),
+                      pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
             exts.append(ext)
         if store is not None:
             store["ext"] = exts
@@ -277,10 +282,11 @@ class CoupledSim:
                 for t in (g_pads[k][0], g_pads[k][1]):
                     _lib.call("dt_clamp", t.numel(), P(t), 1000.0, s)
         for k, pad in enumerate(self.pads):
-            _lib.call("dt_contact_collision_adj", prm, self.cp, P(g4), 4, g4.data_ptr() + 12, 4,
-                      P(idx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
-                      P(store["vel_u"][k]), pad.od(), P(ggf), P(g_ext[k]), P(gg4), 4,
-                      gg4.data_ptr() + 12, 4, P(g_pads[k][0]), P(g_pads[k][1]), s)
+            _lib.call("dt_contact_collision_adj_fx", prm, self.cp, P(g4), 4, g4.data_ptr() + 12,
+                      4, P(idx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32),
+                      P(pads_in[k][0]), P(store["vel_u"][k]), pad.od(), P(ggf), P(g_ext[k]),
+                      P(gg4), 4, gg4.data_ptr() + 12, 4, pad.n_verts, P(g_pads[k][0]),
+                      P(g_pads[k][1]), pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
         for k, pad in enumerate(self.pads):
             _lib.call("dt_fem_update_adj_csr", pad.fem, pad.n_tets, pad.n_verts, pad.f2v, pad.B,
                       pads_in[k][0], g_pads[k][1], g_pads[k][0], pad.vert_start, pad.vert_tets,

@@ -420,14 +420,13 @@ __global__ __launch_bounds__(kBlock) void k_collision(MpmP P, ContactP C, long l
   });
 }
 
-// Adjoint of one contact force given gf = dL/df: accumulates the pad's
-// vertex position and velocity adjoints (atomics) and returns the adjoints of
-// the query point q and its velocity qv.
-__device__ __forceinline__ void contact_adj_core(const ContactP& C, const float (&q)[3],
-                                                 const ContactFwd& w, const float (&gf)[3],
-                                                 const int (&tv)[3], float* __restrict__ g_pos,
-                                                 float* __restrict__ g_vel, float (&gq)[3],
-                                                 float (&gqv)[3]) {
+// Adjoint of one contact force given gf = dL/df: the adjoints of the pad's
+// three vertex velocities (gvv, the same for the three) and positions (gp[c]
+// for corner c), and of the query point q and its velocity qv.
+__device__ __forceinline__ void contact_adj_terms(const ContactP& C, const float (&q)[3],
+                                                  const ContactFwd& w, const float (&gf)[3],
+                                                  float (&gvv)[3], float (&gp)[3][3],
+                                                  float (&gq)[3], float (&gqv)[3]) {
   float gfn[3] = {gf[0], gf[1], gf[2]};
   float gvt[3] = {0.f, 0.f, 0.f};
   if (w.shear) {
@@ -467,10 +466,7 @@ __device__ __forceinline__ void contact_adj_core(const ContactP& C, const float 
   // r = -qv + (1/3) sum vel
   for (int a = 0; a < 3; ++a) {
     gqv[a] = -gr[a];
-    float gvv = gr[a] / 3.f;
-    atomic_add_f(g_vel + 3ll * tv[0] + a, gvv);
-    atomic_add_f(g_vel + 3ll * tv[1] + a, gvv);
-    atomic_add_f(g_vel + 3ll * tv[2] + a, gvv);
+    gvv[a] = gr[a] / 3.f;
   }
   // sdf = (q - p1) . n - offset
   float gp1[3], ge1[3], ge2[3];
@@ -493,10 +489,26 @@ __device__ __forceinline__ void contact_adj_core(const ContactP& C, const float 
   ge2[1] = gcr[2] * w.e1[0] - gcr[0] * w.e1[2];
   ge2[2] = gcr[0] * w.e1[1] - gcr[1] * w.e1[0];
   for (int a = 0; a < 3; ++a) {
-    atomic_add_f(g_pos + 3ll * tv[0] + a, gp1[a] - ge1[a] - ge2[a]);
-    atomic_add_f(g_pos + 3ll * tv[1] + a, ge1[a]);
-    atomic_add_f(g_pos + 3ll * tv[2] + a, ge2[a]);
+    gp[0][a] = gp1[a] - ge1[a] - ge2[a];
+    gp[1][a] = ge1[a];
+    gp[2][a] = ge2[a];
   }
+}
+
+// contact_adj_terms with the vertex adjoints added to g_pos / g_vel by float
+// atomics (order not fixed)
+__device__ __forceinline__ void contact_adj_core(const ContactP& C, const float (&q)[3],
+                                                 const ContactFwd& w, const float (&gf)[3],
+                                                 const int (&tv)[3], float* __restrict__ g_pos,
+                                                 float* __restrict__ g_vel, float (&gq)[3],
+                                                 float (&gqv)[3]) {
+  float gvv[3], gp[3][3];
+  contact_adj_terms(C, q, w, gf, gvv, gp, gq, gqv);
+  for (int c = 0; c < 3; ++c)
+    for (int a = 0; a < 3; ++a) {
+      atomic_add_f(g_vel + 3ll * tv[c] + a, gvv[a]);
+      atomic_add_f(g_pos + 3ll * tv[c] + a, gp[c][a]);
+    }
 }
 
 // Adjoint of k_collision.  Inputs: g_gridf (adjoint of grid_f), g_ext (adjoint
@@ -539,6 +551,203 @@ __global__ __launch_bounds__(kBlock) void k_collision_adj(
       gmm -= gqv[a] * mvv[a] * inv * inv;
     }
     atomic_add_f(g_m + nd * sgm, gmm);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// exact (order-free) sums of the pad vertex terms
+// ---------------------------------------------------------------------------
+// The pads' vertex terms (forward -f/3, adjoint d/d vel and d/d pos) are added
+// as int64 fixed-point numbers, whose sums do not depend on the order the
+// atomics run in: pass 1 takes the largest term magnitude (atomicMax on its
+// float bits, itself order-free), pass 2 adds every term scaled by 2^(40 - e)
+// (e: that magnitude's binary exponent, so a term stays below 2^41 and 2^21
+// terms on one vertex below 2^62), pass 3 converts each vertex's sums back to
+// float (one rounding) and clears them.  The quantum is 2^-40 of the largest
+// term.  scale_state: per scale, the pass-1 maximum (cleared by pass 3) and
+// the copy pass 2 leaves for pass 3.
+__device__ __forceinline__ int fx_exp(uint32_t bits) {
+  int e = (int)((bits >> 23) & 255u) - 127;
+  return e < -80 ? -80 : (e > 80 ? 80 : e);
+}
+__device__ __forceinline__ long long fx_q(float v, int e) {
+  return __float2ll_rn(v * ldexpf(1.f, 40 - e));
+}
+__device__ __forceinline__ void fx_add(long long* acc, float v, int e) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(acc), (unsigned long long)fx_q(v, e));
+}
+__device__ __forceinline__ float fx_float(long long a, int e) {
+  return (float)a * ldexpf(1.f, e - 40);
+}
+__device__ __forceinline__ void fx_max(uint32_t* m, float v) {
+  atomicMax(m, __float_as_uint(fabsf(v)));
+}
+
+// the grid contact of one node (collision's body): flag, force and triangle
+__device__ __forceinline__ bool node_contact(const MpmP& P, const ContactP& C, long long nd,
+                                             const float* __restrict__ mv, int smv,
+                                             const float* __restrict__ m, int sm,
+                                             const int32_t* __restrict__ idx, int stride,
+                                             int col, const int32_t* __restrict__ tris,
+                                             const float* __restrict__ pos,
+                                             const float* __restrict__ vel, const float (&od)[3],
+                                             ContactFwd& w, int (&tv)[3], float (&q)[3],
+                                             float& inv, float (&mvv)[3]) {
+  const float mm = m[nd * sm];
+  if (!(mm > P.eps)) return false;
+  const int k = idx[nd * stride + col];
+  if (k < 0) return false;
+  const int n = P.n_grid;
+  const int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
+  q[0] = node_centre(i % n, P.dx);
+  q[1] = node_centre(j, P.dx);
+  q[2] = node_centre(kk, P.dx);
+  inv = 1.f / (mm + P.eps);
+  for (int a = 0; a < 3; ++a) mvv[a] = mv[nd * smv + a];
+  const float qv[3] = {mvv[0] * inv, mvv[1] * inv, mvv[2] * inv};
+  tv[0] = tris[3 * k];
+  tv[1] = tris[3 * k + 1];
+  tv[2] = tris[3 * k + 2];
+  contact_fwd(C, q, qv, pos, vel, tv, od, w);
+  return w.flag;
+}
+
+// forward pass 1: grid_f += f dt; the largest |f_a| / 3
+__global__ __launch_bounds__(kBlock) void k_collision_fx1(
+    MpmP P, ContactP C, long long nn, const float* __restrict__ mv, int smv,
+    const float* __restrict__ m, int sm, const int32_t* __restrict__ idx, int stride, int col,
+    const int32_t* __restrict__ tris, const float* __restrict__ pos, const float* __restrict__ vel,
+    float o0, float o1, float o2, float* __restrict__ gridf, uint32_t* __restrict__ scale) {
+  const float od[3] = {o0, o1, o2};
+  node_loop(P, nn, [&](long long nd) {
+    ContactFwd w;
+    int tv[3];
+    float q[3], inv, mvv[3];
+    if (!node_contact(P, C, nd, mv, smv, m, sm, idx, stride, col, tris, pos, vel, od, w, tv, q,
+                      inv, mvv))
+      return;
+    float mx = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      gridf[3 * nd + a] += w.f[a] * P.dt;
+      mx = fmaxf(mx, fabsf(w.f[a]));
+    }
+    fx_max(scale, mx / 3.f);
+  });
+}
+
+// forward pass 2: the -f/3 terms into the fixed-point vertex sums
+__global__ __launch_bounds__(kBlock) void k_collision_fx2(
+    MpmP P, ContactP C, long long nn, const float* __restrict__ mv, int smv,
+    const float* __restrict__ m, int sm, const int32_t* __restrict__ idx, int stride, int col,
+    const int32_t* __restrict__ tris, const float* __restrict__ pos, const float* __restrict__ vel,
+    float o0, float o1, float o2, long long* __restrict__ acc, uint32_t* __restrict__ scale) {
+  const float od[3] = {o0, o1, o2};
+  const uint32_t bits = scale[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) scale[1] = bits;
+  const int e = fx_exp(bits);
+  node_loop(P, nn, [&](long long nd) {
+    ContactFwd w;
+    int tv[3];
+    float q[3], inv, mvv[3];
+    if (!node_contact(P, C, nd, mv, smv, m, sm, idx, stride, col, tris, pos, vel, od, w, tv, q,
+                      inv, mvv))
+      return;
+    for (int c = 0; c < 3; ++c)
+      for (int a = 0; a < 3; ++a) fx_add(acc + 3ll * tv[c] + a, -w.f[a] / 3.f, e);
+  });
+}
+
+// pass 3: out[v] += sums, sums cleared, the pass-1 maxima cleared
+__global__ __launch_bounds__(kBlock) void k_fx_out(int nv, int n_arrays,
+                                                   long long* __restrict__ acc,
+                                                   uint32_t* __restrict__ scale,
+                                                   float* __restrict__ out0,
+                                                   float* __restrict__ out1) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per = 3ll * nv;
+  if (i < per * n_arrays) {
+    const int s = (int)(i / per);
+    const long long r = i - s * per;
+    const int e = fx_exp(scale[n_arrays + s]);
+    float* out = s == 0 ? out0 : out1;
+    out[r] += fx_float(acc[i], e);
+    acc[i] = 0;
+  }
+  if (i < n_arrays) scale[i] = 0u;
+}
+
+// adjoint pass 1: the largest vertex terms (velocity, position)
+__global__ __launch_bounds__(kBlock) void k_collision_adj_fx1(
+    MpmP P, ContactP C, long long nn, const float* __restrict__ mv, int smv,
+    const float* __restrict__ m, int sm, const int32_t* __restrict__ idx, int stride, int col,
+    const int32_t* __restrict__ tris, const float* __restrict__ pos, const float* __restrict__ vel,
+    float o0, float o1, float o2, const float* __restrict__ g_gridf,
+    const float* __restrict__ g_ext, uint32_t* __restrict__ scale) {
+  const float od[3] = {o0, o1, o2};
+  node_loop(P, nn, [&](long long nd) {
+    ContactFwd w;
+    int tv[3];
+    float q[3], inv, mvv[3];
+    if (!node_contact(P, C, nd, mv, smv, m, sm, idx, stride, col, tris, pos, vel, od, w, tv, q,
+                      inv, mvv))
+      return;
+    float gf[3];
+    for (int a = 0; a < 3; ++a)
+      gf[a] = g_gridf[3 * nd + a] * P.dt -
+              (g_ext[3ll * tv[0] + a] + g_ext[3ll * tv[1] + a] + g_ext[3ll * tv[2] + a]) / 3.f;
+    float gvv[3], gp[3][3], gq[3], gqv[3];
+    contact_adj_terms(C, q, w, gf, gvv, gp, gq, gqv);
+    float mv_ = 0.f, mp = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      mv_ = fmaxf(mv_, fabsf(gvv[a]));
+      mp = fmaxf(mp, fmaxf(fabsf(gp[0][a]), fmaxf(fabsf(gp[1][a]), fabsf(gp[2][a]))));
+    }
+    fx_max(scale, mv_);
+    fx_max(scale + 1, mp);
+  });
+}
+
+// adjoint pass 2: the grid adjoints (node-owned) and the vertex terms into the
+// fixed-point sums acc[0 .. 3 nv) (velocity) and acc[3 nv .. 6 nv) (position)
+__global__ __launch_bounds__(kBlock) void k_collision_adj_fx2(
+    MpmP P, ContactP C, long long nn, const float* __restrict__ mv, int smv,
+    const float* __restrict__ m, int sm, const int32_t* __restrict__ idx, int stride, int col,
+    const int32_t* __restrict__ tris, const float* __restrict__ pos, const float* __restrict__ vel,
+    float o0, float o1, float o2, const float* __restrict__ g_gridf,
+    const float* __restrict__ g_ext, float* __restrict__ g_mv, int sgmv, float* __restrict__ g_m,
+    int sgm, int nv, long long* __restrict__ acc, uint32_t* __restrict__ scale) {
+  const float od[3] = {o0, o1, o2};
+  const uint32_t bv = scale[0], bp = scale[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scale[2] = bv;
+    scale[3] = bp;
+  }
+  const int ev = fx_exp(bv), ep = fx_exp(bp);
+  node_loop(P, nn, [&](long long nd) {
+    ContactFwd w;
+    int tv[3];
+    float q[3], inv, mvv[3];
+    if (!node_contact(P, C, nd, mv, smv, m, sm, idx, stride, col, tris, pos, vel, od, w, tv, q,
+                      inv, mvv))
+      return;
+    float gf[3];
+    for (int a = 0; a < 3; ++a)
+      gf[a] = g_gridf[3 * nd + a] * P.dt -
+              (g_ext[3ll * tv[0] + a] + g_ext[3ll * tv[1] + a] + g_ext[3ll * tv[2] + a]) / 3.f;
+    float gvv[3], gp[3][3], gq[3], gqv[3];
+    contact_adj_terms(C, q, w, gf, gvv, gp, gq, gqv);
+    // qv = mv / (m + eps): one node per thread, so these adds are ordered
+    float gmm = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      g_mv[nd * sgmv + a] += gqv[a] * inv;
+      gmm -= gqv[a] * mvv[a] * inv * inv;
+    }
+    g_m[nd * sgm] += gmm;
+    for (int c = 0; c < 3; ++c)
+      for (int a = 0; a < 3; ++a) {
+        fx_add(acc + 3ll * tv[c] + a, gvv[a], ev);
+        fx_add(acc + 3ll * nv + 3ll * tv[c] + a, gp[c][a], ep);
+      }
   });
 }
 
@@ -763,6 +972,78 @@ int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* 
                      out_dir[2], g_grid_f, g_ext_force, g_grid_mv, gmv_stride, g_grid_m,
                      gm_stride, g_pos, g_vel);
   return check_launch("dt_contact_collision_adj");
+}
+
+int dt_contact_collision_fx(const dt_mpm_params* prm, const dt_contact_params* cp,
+                            const float* grid_mv, int32_t mv_stride, const float* grid_m,
+                            int32_t m_stride, const int32_t* contact_idx, int32_t idx_stride,
+                            int32_t idx_col, int32_t n_tris, const int32_t* tris,
+                            const float* pos, const float* vel, const float* out_dir,
+                            float* grid_f, int32_t n_verts, float* ext_force, int64_t* acc,
+                            uint32_t* scale_state, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
+  DT_REQUIRE(mv_stride >= 3 && m_stride > 0 && idx_stride > 0 && idx_col >= 0 &&
+                 idx_col < idx_stride && n_tris >= 0 && n_verts >= 0,
+             "bad size/stride");
+  DT_REQUIRE(grid_mv && grid_m && contact_idx && tris && pos && vel && grid_f && ext_force &&
+                 acc && scale_state,
+             "NULL pointer");
+  long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
+                 prm->n_grid;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = (unsigned)node_blocks(*prm, nn);
+  const MpmP P = to_dev_nodes(*prm);
+  const ContactP C = contact_dev(cp);
+  hipLaunchKernelGGL(k_collision_fx1, dim3(nb), dim3(kBlock), 0, st, P, C, nn, grid_mv, mv_stride,
+                     grid_m, m_stride, contact_idx, idx_stride, idx_col, tris, pos, vel,
+                     out_dir[0], out_dir[1], out_dir[2], grid_f, scale_state);
+  hipLaunchKernelGGL(k_collision_fx2, dim3(nb), dim3(kBlock), 0, st, P, C, nn, grid_mv, mv_stride,
+                     grid_m, m_stride, contact_idx, idx_stride, idx_col, tris, pos, vel,
+                     out_dir[0], out_dir[1], out_dir[2], (long long*)acc, scale_state);
+  hipLaunchKernelGGL(k_fx_out, dim3(nblocks(3ll * (n_verts > 0 ? n_verts : 1))), dim3(kBlock), 0,
+                     st, (int)n_verts, 1, (long long*)acc, scale_state, ext_force,
+                     (float*)nullptr);
+  return check_launch("dt_contact_collision_fx");
+}
+
+int dt_contact_collision_adj_fx(const dt_mpm_params* prm, const dt_contact_params* cp,
+                                const float* grid_mv, int32_t mv_stride, const float* grid_m,
+                                int32_t m_stride, const int32_t* contact_idx,
+                                int32_t idx_stride, int32_t idx_col, int32_t n_tris,
+                                const int32_t* tris, const float* pos, const float* vel,
+                                const float* out_dir, const float* g_grid_f,
+                                const float* g_ext_force, float* g_grid_mv, int32_t gmv_stride,
+                                float* g_grid_m, int32_t gm_stride, int32_t n_verts,
+                                float* g_pos, float* g_vel, int64_t* acc,
+                                uint32_t* scale_state, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
+  DT_REQUIRE(mv_stride >= 3 && m_stride > 0 && gmv_stride >= 3 && gm_stride > 0 &&
+                 idx_stride > 0 && idx_col >= 0 && idx_col < idx_stride && n_tris >= 0 &&
+                 n_verts >= 0,
+             "bad size/stride");
+  DT_REQUIRE(grid_mv && grid_m && contact_idx && tris && pos && vel && g_grid_f && g_ext_force &&
+                 g_grid_mv && g_grid_m && g_pos && g_vel && acc && scale_state,
+             "NULL pointer");
+  long long nn = (long long)(prm->n_env > 1 ? prm->n_env : 1) * prm->n_grid * prm->n_grid *
+                 prm->n_grid;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = (unsigned)node_blocks(*prm, nn);
+  const MpmP P = to_dev_nodes(*prm);
+  const ContactP C = contact_dev(cp);
+  hipLaunchKernelGGL(k_collision_adj_fx1, dim3(nb), dim3(kBlock), 0, st, P, C, nn, grid_mv,
+                     mv_stride, grid_m, m_stride, contact_idx, idx_stride, idx_col, tris, pos,
+                     vel, out_dir[0], out_dir[1], out_dir[2], g_grid_f, g_ext_force, scale_state);
+  hipLaunchKernelGGL(k_collision_adj_fx2, dim3(nb), dim3(kBlock), 0, st, P, C, nn, grid_mv,
+                     mv_stride, grid_m, m_stride, contact_idx, idx_stride, idx_col, tris, pos,
+                     vel, out_dir[0], out_dir[1], out_dir[2], g_grid_f, g_ext_force, g_grid_mv,
+                     gmv_stride, g_grid_m, gm_stride, (int)n_verts, (long long*)acc, scale_state);
+  hipLaunchKernelGGL(k_fx_out, dim3(nblocks(6ll * (n_verts > 0 ? n_verts : 1))), dim3(kBlock), 0,
+                     st, (int)n_verts, 2, (long long*)acc, scale_state, g_vel, g_pos);
+  return check_launch("dt_contact_collision_adj_fx");
 }
 
 int dt_rope_closest(int32_t n, const float* x, int32_t n_tris, const float* centroids4,

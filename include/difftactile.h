@@ -546,6 +546,31 @@ int dt_contact_collision_adj(const dt_mpm_params* prm, const dt_contact_params* 
                              const float* g_grid_f, const float* g_ext_force, float* g_grid_mv,
                              int32_t gmv_stride, float* g_grid_m, int32_t gm_stride, float* g_pos,
                              float* g_vel, void* stream);
+/* The same two kernels with the pad-vertex sums exact and order-free, so they
+   repeat bit for bit (no reference counterpart; collision and collision.grad,
+   grasp_plastic.py:279-299): the vertex terms are added as int64 fixed-point
+   numbers at 2^-40 of the largest term (three launches: largest term, sums,
+   conversion).  acc: n_verts*3 (forward) or n_verts*6 (adjoint) int64,
+   scale_state: 2 (forward) or 4 (adjoint) uint32; both zero before the first
+   call and left zero by every call (one pair per pad, not shared by calls
+   that may overlap).  ext_force, g_pos, g_vel accumulate (+=). */
+int dt_contact_collision_fx(const dt_mpm_params* prm, const dt_contact_params* cp,
+                            const float* grid_mv, int32_t mv_stride, const float* grid_m,
+                            int32_t m_stride, const int32_t* contact_idx, int32_t idx_stride,
+                            int32_t idx_col, int32_t n_tris, const int32_t* tris,
+                            const float* pos, const float* vel, const float* out_dir,
+                            float* grid_f, int32_t n_verts, float* ext_force, int64_t* acc,
+                            uint32_t* scale_state, void* stream);
+int dt_contact_collision_adj_fx(const dt_mpm_params* prm, const dt_contact_params* cp,
+                                const float* grid_mv, int32_t mv_stride, const float* grid_m,
+                                int32_t m_stride, const int32_t* contact_idx,
+                                int32_t idx_stride, int32_t idx_col, int32_t n_tris,
+                                const int32_t* tris, const float* pos, const float* vel,
+                                const float* out_dir, const float* g_grid_f,
+                                const float* g_ext_force, float* g_grid_mv, int32_t gmv_stride,
+                                float* g_grid_m, int32_t gm_stride, int32_t n_verts,
+                                float* g_pos, float* g_vel, int64_t* acc,
+                                uint32_t* scale_state, void* stream);
 
 
 /* ---- losses on the measured path ---- */

@@ -71,3 +71,37 @@ def test_resorting_at_checkpoints_changes_only_rounding(gpu):
         assert rel_l2(a.cpu(), b.cpu()) < t
     for a, b in zip(res[0][1], res[1][1]):
         assert rel_l2(a.cpu(), b.cpu()) < 1e-3
+
+
+def test_coupled_rollout_repeats_bit_for_bit(gpu):
+    """The pad press (object, FEM pad, contact, clamp) twice: the pad-vertex
+    contact sums are exact fixed-point sums and the FEM vertex forces are
+    gathered per vertex, so states and the gradients w.r.t. the pad velocity
+    sequence and the initial positions repeat bit for bit."""
+    from difftactile_amd.coupled import CoupledSim, Pad
+    c = scenes.config2_pad_press(n_particles=6000, n_grid=32, n_substeps=20,
+                                 pad_verts=(12, 12, 3), press_substeps=100)
+    prm, n = c["prm"], len(c["x"])
+    nodes, f2v, tris, layer, od = c["mesh"]
+    runs = []
+    for _ in range(2):
+        sim = MPMSim(prm, c["mats"], n, checkpoint_every=8)
+        pad = Pad(nodes.astype(np.float32), f2v, tris, layer, od, c["pad_mu"], c["pad_lam"],
+                  c["pad_mass"], prm.dt)
+        cs = CoupledSim(sim, [pad], **c["contact"])
+        x0 = torch.tensor(c["x"]).cuda().requires_grad_(True)
+        cv = torch.tensor(c["control_vel"]).cuda().requires_grad_(True)
+        pos0 = torch.tensor(nodes, dtype=torch.float32).cuda()
+        out = cs.rollout(x0, torch.tensor(c["v"]).cuda(), torch.zeros(n, 3, 3).cuda(),
+                         torch.eye(3).repeat(n, 1, 1).cuda(),
+                         [(pos0.clone(), cv[0].detach().clone(), pos0.clone())], [cv], 20)
+        w = torch.randn(n, 3, generator=torch.Generator().manual_seed(8)).cuda()
+        wp = torch.randn(len(nodes), 3, generator=torch.Generator().manual_seed(9)).cuda()
+        loss = (out[0] * w).sum() + (out[4][0][0] * wp).sum() + (out[4][0][1] * wp).sum()
+        g = torch.autograd.grad(loss, [x0, cv])
+        torch.cuda.synchronize()
+        runs.append([t.detach().clone() for t in out[:4] + tuple(out[4][0])] +
+                    [t.clone() for t in g])
+    assert float(runs[0][-1].abs().max()) > 0
+    for a, b in zip(runs[0], runs[1]):
+        assert torch.equal(a, b)
