@@ -192,6 +192,13 @@ _SIGS = {
                                              _vp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp,
                                              _vp, _P(ctypes.c_float), _vp, _vp, _vp, _c_int, _vp,
                                              _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_rope_collision_fx": (_c_int, [_P(DtContactParams), _c_float, _c_int, _vp, _vp, _vp,
+                                      _c_int, _c_int, _c_int, _vp, _vp, _vp, _P(ctypes.c_float),
+                                      _vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_rope_collision_adj_fx": (_c_int, [_P(DtContactParams), _c_float, _c_int, _vp, _vp, _vp,
+                                          _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                          _P(ctypes.c_float), _vp, _vp, _vp, _vp, _c_int, _vp,
+                                          _vp, _vp, _vp, _vp]),
     "dt_loss_sum_positions": (_c_int, [_c_int, _c_int, _vp, _vp, _vp]),
     "dt_loss_com_adj": (_c_int, [_c_int, _c_int, _P(ctypes.c_float), _vp, _vp]),
     "dt_markers_fwd": (_c_int, [_c_int, _vp, _vp, _vp, _P(ctypes.c_float), _P(ctypes.c_float),

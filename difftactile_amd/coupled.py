@@ -88,9 +88,34 @@ class Pad:
 
 
 class CoupledSim:
-    def __init__(self, sim: MPMSim, pads, kn, kd, kt, mu_f, norm_eps=1e-11, clamp_grad=True):
+    """An MPM object, FEM pads and optionally a position-based cable (``rope``,
+    a ``PBDRope`` whose particle 0 stays at ``rope.init_pos[0]``), coupled by
+    grid penalty contact (object - pads) and particle-wise contact (cable -
+    pads, the cable task's force with ``rope_contact`` = dict(kn, kd, kt,
+    mu_f), default the grid constants).  With a cable the substep composes the
+    grasp and cable tasks' updates (grasp_plastic.py:151-164,
+    cable_straightening.py:166-173): p2g -> pad update -> grid contact ->
+    cable contact -> grid_op -> g2p -> cable step -> pad update2 with the sum
+    of both contacts' pad forces; the adjoint reverses it, clipping the
+    cable's d/d pos, d/d vel as the cable task's clamp_grid does."""
+
+    def __init__(self, sim: MPMSim, pads, kn, kd, kt, mu_f, norm_eps=1e-11, clamp_grad=True,
+                 rope=None, rope_contact=None):
         self.sim = sim
         self.pads = list(pads)
+        self.rope = rope
+        if rope is not None:
+            from .object_model.pbd_rope import _contact_params
+            rc = rope_contact or dict(kn=kn, kd=kd, kt=kt, mu_f=mu_f)
+            self.rope_cp = _contact_params(rc["kn"], rc["kd"], rc["kt"], rc["mu_f"], rope.p_rad)
+            self.rope_prm = rope.params()
+            if rope.init_pos is None:
+                raise ValueError("the cable needs init() (its fixed end) before coupling")
+            self.rope_fixed = np.ascontiguousarray(rope.init_pos[0].detach().cpu().numpy(),
+                                                   np.float32)
+            nr = rope.n_particles
+            self.rope_work = torch.empty(12 * nr, device=sim.device)
+            self.rope_gimp = torch.empty(nr, 3, device=sim.device)
         self.cp = _lib.DtContactParams()
         self.cp.kn, self.cp.kd, self.cp.kt, self.cp.mu_f = kn, kd, kt, mu_f
         self.cp.norm_eps, self.cp.sdf_offset = norm_eps, 0.0
@@ -103,6 +128,9 @@ class CoupledSim:
         self.npad = max(1, len(self.pads))
         self.grid_f = torch.zeros(nn, 3, device=d)
         self.idx = torch.full((nn, self.npad), -1, dtype=I32, device=d)
+        if rope is not None:
+            self.ridx = torch.full((rope.n_particles, self.npad), -1, dtype=I32, device=d)
+            self.rimp = torch.zeros(rope.n_particles, 3, device=d)
 
     def _node_set(self, base, ws, nb):
         """Copy of the parameter struct base whose grid-node passes run over
@@ -122,7 +150,12 @@ class CoupledSim:
         b = 4 * self.sim.n * 24 + nn * (16 + 16 + 12 + 4 * self.npad) + self._bin_slot_bytes()
         for pd in self.pads:
             b += 4 * 3 * pd.n_verts * 4
+        if self.rope is not None:
+            b += self.rope.n_particles * (4 * 9 + 4 * self.npad)
         return b
+
+    def _fixed_ptr(self):
+        return self.rope_fixed.ctypes.data_as(_lib.ctypes.POINTER(_lib.ctypes.c_float))
 
     def _bin_slot_bytes(self):
         """Bytes of one substep's saved binning (0 on the dense path)."""
@@ -139,6 +172,10 @@ class CoupledSim:
                 "gf": torch.empty(nn, 3, device=dev),
                 "idx": torch.empty(nn, self.npad, dtype=I32, device=dev), "hint": hint,
                 "vel_u": [torch.empty(pd.n_verts, 3, device=dev) for pd in self.pads],
+                "ridx": (None if self.rope is None else
+                         torch.empty(self.rope.n_particles, self.npad, dtype=I32, device=dev)),
+                "rimp": (None if self.rope is None else
+                         torch.empty(self.rope.n_particles, 3, device=dev)),
                 "bin": (torch.empty(self._bin_slot_bytes(), dtype=torch.uint8, device=dev)
                         if sim.sparse_grid else None)}
 
@@ -223,10 +260,29 @@ class CoupledSim:
                       P(vel_u[k]), pad.od(), P(gf), pad.n_verts, P(ext),
                       pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
             exts.append(ext)
+        npd = len(self.pads)
+        if self.rope is not None:
+            # cable - pad contact (the cable's state follows the pads' in pads_in)
+            rpos, rvel = pads_in[npd]
+            nr = self.rope.n_particles
+            ridx = store["ridx"] if store else self.ridx
+            rimp = store["rimp"] if store else self.rimp
+            rimp.zero_()
+            for k, pad in enumerate(self.pads):
+                _lib.call("dt_rope_closest", nr, P(rpos), pad.n_tris, P(pad.cen), P(ridx, I32),
+                          self.npad, k, s)
+                _lib.call("dt_rope_collision_fx", self.rope_cp, sim.prm.dt, nr, P(rpos), P(rvel),
+                          P(ridx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32),
+                          P(pads_in[k][0]), P(vel_u[k]), pad.od(), P(rimp), pad.n_verts,
+                          P(exts[k]), None, pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
         if store is not None:
             store["ext"] = exts
         _lib.call("dt_mpm_grid_op4", prm, P(g4), P(gf), P(gv4), s)
         _lib.call("dt_mpm_g2p4", prm, sim.n, P(st[0]), P(gv4), P(nxt[0]), P(nxt[1]), P(nxt[2]), s)
+        if self.rope is not None:
+            ro = pads_out[npd]
+            _lib.call("dt_rope_step", self.rope_prm, nr, self._fixed_ptr(), P(rpos), P(rvel),
+                      P(rimp), P(self.rope_work), P(ro[0]), P(ro[1]), s)
         for k, pad in enumerate(self.pads):
             pos, _, vpos = pads_in[k]
             o = pads_out[k]
@@ -256,6 +312,20 @@ class CoupledSim:
             _lib.call("dt_fem_update2_adj", pad.fem, pad.n_verts, P(pad.layer, I32), P(gpn),
                       P(gvn), P(gvpn), P(gp), P(gvu), P(gvp), P(ge), P(g_cv[k]), P(g_cv1[k]), s)
             g_ext.append(ge)
+        npd = len(self.pads)
+        if self.rope is not None:
+            rpos, rvel = pads_in[npd]
+            nr = self.rope.n_particles
+            grp, grv = g_pads[npd]
+            gimp = self.rope_gimp
+            torch._foreach_zero_([grp, grv, gimp])
+            gpn, gvn = g_pads_next[npd]
+            _lib.call("dt_rope_step_adj", self.rope_prm, nr, self._fixed_ptr(), P(rpos), P(rvel),
+                      P(store["rimp"]), P(gpn), P(gvn), P(self.rope_work), P(grp), P(grv),
+                      P(gimp), s)
+            if self.clamp_grad:
+                for t in (grp, grv):
+                    _lib.call("dt_clamp", t.numel(), P(t), 1000.0, s)
         prm_clamp = self.c_prm_clamp
         if sim.sparse_grid:
             # the forward substep's binning (saved, or made again from its
@@ -287,6 +357,13 @@ class CoupledSim:
                       P(pads_in[k][0]), P(store["vel_u"][k]), pad.od(), P(ggf), P(g_ext[k]),
                       P(gg4), 4, gg4.data_ptr() + 12, 4, pad.n_verts, P(g_pads[k][0]),
                       P(g_pads[k][1]), pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
+        if self.rope is not None:
+            for k, pad in enumerate(self.pads):
+                _lib.call("dt_rope_collision_adj_fx", self.rope_cp, sim.prm.dt, nr, P(rpos),
+                          P(rvel), P(store["ridx"], I32), self.npad, k, pad.n_tris,
+                          P(pad.tris, I32), P(pads_in[k][0]), P(store["vel_u"][k]), pad.od(),
+                          P(gimp), P(g_ext[k]), P(grp), P(grv), pad.n_verts, P(g_pads[k][0]),
+                          P(g_pads[k][1]), pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
         for k, pad in enumerate(self.pads):
             _lib.call("dt_fem_update_adj_csr", pad.fem, pad.n_tets, pad.n_verts, pad.f2v, pad.B,
                       pads_in[k][0], g_pads[k][1], g_pads[k][0], pad.vert_start, pad.vert_tets,
@@ -295,18 +372,26 @@ class CoupledSim:
                   P(g_next[3]), P(g[0]), P(g[1]), P(g[2]), P(g[3]), sim._mat_grad_ptr(), s)
 
     # -- public --------------------------------------------------------------
-    def rollout(self, x, v, C, F, pad_states, control_vel, n_substeps):
+    def rollout(self, x, v, C, F, pad_states, control_vel, n_substeps, rope_state=None):
         """x, v, C, F: object state ([n][k] layout); pad_states: list of (pos, vel, vpos)
         per pad; control_vel: list of per-pad (n_substeps+1, n_verts, 3) velocity
-        sequences.  Returns (x, v, C, F, [(pos, vel, vpos) per pad]) after n_substeps,
-        differentiable with respect to every tensor input."""
+        sequences; rope_state: (pos, vel) of the cable (with a cable).  Returns
+        (x, v, C, F, [(pos, vel, vpos) per pad]) after n_substeps, and the cable's
+        (pos, vel) last with a cable, differentiable with respect to every tensor
+        input."""
         flat = [x, v, C, F]
         for ps in pad_states:
             flat += list(ps)
         flat += list(control_vel)
+        if self.rope is not None:
+            if rope_state is None:
+                raise ValueError("a scene with a cable needs rope_state")
+            flat += list(rope_state)
         out = _CoupledRollout.apply(self, int(n_substeps), *flat)
         obj = out[:4]
         pads = [tuple(out[4 + 3 * k: 7 + 3 * k]) for k in range(len(self.pads))]
+        if self.rope is not None:
+            return (*obj, pads, tuple(out[4 + 3 * len(self.pads):]))
         return (*obj, pads)
 
 
@@ -317,7 +402,10 @@ class _CoupledRollout(torch.autograd.Function):
         npd = len(cs.pads)
         x, v, C, F = (t.contiguous() for t in flat[:4])
         pads0 = [[t.contiguous().clone() for t in flat[4 + 3 * k: 7 + 3 * k]] for k in range(npd)]
-        cvs = [t.contiguous() for t in flat[4 + 3 * npd:]]
+        cvs = [t.contiguous() for t in flat[4 + 3 * npd: 4 + 4 * npd]]
+        if cs.rope is not None:
+            # the cable's state rides along as one more body after the pads
+            pads0.append([t.contiguous().float().clone() for t in flat[4 + 4 * npd:]])
         shapes = [t.shape for t in (x, v, C, F)]
         W = (3, 3, 9, 9)
         perm = sim.sort_order(x)[1] if (sim.sort and sim.n > 0) else None
@@ -379,6 +467,10 @@ class _CoupledRollout(torch.autograd.Function):
             trio = gout[4 + 3 * k: 7 + 3 * k]
             gp.append([(t if t is not None else torch.zeros(cs.pads[k].n_verts, 3, device=dev)
                         ).contiguous().clone() for t in trio])
+        if cs.rope is not None:
+            nr = cs.rope.n_particles
+            gp.append([(t if t is not None else torch.zeros(nr, 3, device=dev)
+                        ).contiguous().float().clone() for t in gout[4 + 3 * npd:]])
         g_cv = [torch.zeros(sh, device=dev) for sh in ctx.cv_shapes]
         gbuf = [torch.empty_like(t) for t in g]
         gpbuf = [[torch.empty_like(t) for t in p] for p in gp]
@@ -400,7 +492,9 @@ class _CoupledRollout(torch.autograd.Function):
                 g, gbuf = gbuf, g
                 gp, gpbuf = gpbuf, gp
         res = [sim._scatter(ctx.perm, t, w, sh) for t, w, sh in zip(g, W, ctx.shapes)]
-        for p in gp:
+        for p in gp[:npd]:
             res += list(p)
         res += g_cv
+        for p in gp[npd:]:
+            res += list(p)
         return (None, None, *res)

@@ -828,6 +828,108 @@ __global__ __launch_bounds__(kBlock) void k_rope_collision_adj(
   }
 }
 
+// the rope contact with the pad-vertex sums exact (the fixed-point passes
+// above): pass 1 the largest term, pass 2 the impulse on the rope (one
+// particle per thread) and the fixed-point pad terms
+__device__ __forceinline__ bool rope_contact_at(const ContactP& C, int p,
+                                                const float* __restrict__ x,
+                                                const float* __restrict__ v,
+                                                const int32_t* __restrict__ idx, int stride,
+                                                int col, const int32_t* __restrict__ tris,
+                                                const float* __restrict__ pos,
+                                                const float* __restrict__ vel,
+                                                const float (&od)[3], float (&q)[3], int (&tv)[3],
+                                                ContactFwd& w) {
+  const int k = idx[(long long)p * stride + col];
+  if (k < 0) return false;
+  for (int a = 0; a < 3; ++a) q[a] = x[3 * p + a];
+  const float qv[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
+  tv[0] = tris[3 * k];
+  tv[1] = tris[3 * k + 1];
+  tv[2] = tris[3 * k + 2];
+  contact_fwd(C, q, qv, pos, vel, tv, od, w);
+  return w.flag;
+}
+
+template <int kPass>
+__global__ __launch_bounds__(kBlock) void k_rope_collision_fx(
+    ContactP C, float dt, int n, const float* __restrict__ x, const float* __restrict__ v,
+    const int32_t* __restrict__ idx, int stride, int col, const int32_t* __restrict__ tris,
+    const float* __restrict__ pos, const float* __restrict__ vel, float o0, float o1, float o2,
+    float* __restrict__ impulse, int32_t* __restrict__ flag, long long* __restrict__ acc,
+    uint32_t* __restrict__ scale) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const float od[3] = {o0, o1, o2};
+  const uint32_t bits = kPass == 2 ? scale[0] : 0u;
+  if (kPass == 2 && p == 0) scale[1] = bits;
+  if (p >= n) return;
+  float q[3];
+  int tv[3];
+  ContactFwd w;
+  if (!rope_contact_at(C, p, x, v, idx, stride, col, tris, pos, vel, od, q, tv, w)) return;
+  if (kPass == 1) {
+    fx_max(scale, fmaxf(fabsf(w.f[0]), fmaxf(fabsf(w.f[1]), fabsf(w.f[2]))) / 3.f);
+    return;
+  }
+  const int e = fx_exp(bits);
+  for (int a = 0; a < 3; ++a) {
+    impulse[3 * p + a] += w.f[a] * dt;
+    for (int c = 0; c < 3; ++c) fx_add(acc + 3ll * tv[c] + a, -w.f[a] / 3.f, e);
+  }
+  if (flag) flag[p] = 1;
+}
+
+template <int kPass>
+__global__ __launch_bounds__(kBlock) void k_rope_collision_adj_fx(
+    ContactP C, float dt, int n, const float* __restrict__ x, const float* __restrict__ v,
+    const int32_t* __restrict__ idx, int stride, int col, const int32_t* __restrict__ tris,
+    const float* __restrict__ pos, const float* __restrict__ vel, float o0, float o1, float o2,
+    const float* __restrict__ g_impulse, const float* __restrict__ g_ext, float* __restrict__ g_x,
+    float* __restrict__ g_v, int nv, long long* __restrict__ acc, uint32_t* __restrict__ scale) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const float od[3] = {o0, o1, o2};
+  uint32_t bv = 0u, bp = 0u;
+  if (kPass == 2) {
+    bv = scale[0];
+    bp = scale[1];
+    if (p == 0) {
+      scale[2] = bv;
+      scale[3] = bp;
+    }
+  }
+  if (p >= n) return;
+  float q[3];
+  int tv[3];
+  ContactFwd w;
+  if (!rope_contact_at(C, p, x, v, idx, stride, col, tris, pos, vel, od, q, tv, w)) return;
+  float gf[3];
+  for (int a = 0; a < 3; ++a)
+    gf[a] = g_impulse[3 * p + a] * dt -
+            (g_ext[3ll * tv[0] + a] + g_ext[3ll * tv[1] + a] + g_ext[3ll * tv[2] + a]) / 3.f;
+  float gvv[3], gp[3][3], gq[3], gqv[3];
+  contact_adj_terms(C, q, w, gf, gvv, gp, gq, gqv);
+  if (kPass == 1) {
+    float mv_ = 0.f, mp = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      mv_ = fmaxf(mv_, fabsf(gvv[a]));
+      mp = fmaxf(mp, fmaxf(fabsf(gp[0][a]), fmaxf(fabsf(gp[1][a]), fabsf(gp[2][a]))));
+    }
+    fx_max(scale, mv_);
+    fx_max(scale + 1, mp);
+    return;
+  }
+  const int ev = fx_exp(bv), ep = fx_exp(bp);
+  for (int a = 0; a < 3; ++a) {
+    g_x[3 * p + a] += gq[a];
+    g_v[3 * p + a] += gqv[a];
+  }
+  for (int c = 0; c < 3; ++c)
+    for (int a = 0; a < 3; ++a) {
+      fx_add(acc + 3ll * tv[c] + a, gvv[a], ev);
+      fx_add(acc + 3ll * nv + 3ll * tv[c] + a, gp[c][a], ep);
+    }
+}
+
 static ContactP contact_dev(const dt_contact_params* c) {
   ContactP d;
   d.kn = c->kn;
@@ -1044,6 +1146,64 @@ int dt_contact_collision_adj_fx(const dt_mpm_params* prm, const dt_contact_param
   hipLaunchKernelGGL(k_fx_out, dim3(nblocks(6ll * (n_verts > 0 ? n_verts : 1))), dim3(kBlock), 0,
                      st, (int)n_verts, 2, (long long*)acc, scale_state, g_vel, g_pos);
   return check_launch("dt_contact_collision_adj_fx");
+}
+
+int dt_rope_collision_fx(const dt_contact_params* cp, float dt, int32_t n, const float* x,
+                         const float* v, const int32_t* contact_idx, int32_t idx_stride,
+                         int32_t idx_col, int32_t n_tris, const int32_t* tris, const float* pos,
+                         const float* vel, const float* out_dir, float* impulse, int32_t n_verts,
+                         float* ext_force, int32_t* contact_flag, int64_t* acc,
+                         uint32_t* scale_state, void* stream) {
+  DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
+  DT_REQUIRE(n >= 0 && n_tris >= 0 && n_verts >= 0 && idx_stride > 0 && idx_col >= 0 &&
+                 idx_col < idx_stride,
+             "bad size/stride");
+  if (n == 0) return DT_OK;
+  DT_REQUIRE(x && v && contact_idx && tris && pos && vel && impulse && ext_force && acc &&
+                 scale_state,
+             "NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const ContactP C = contact_dev(cp);
+  hipLaunchKernelGGL(k_rope_collision_fx<1>, dim3(nblocks(n)), dim3(kBlock), 0, st, C, dt, (int)n,
+                     x, v, contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0],
+                     out_dir[1], out_dir[2], impulse, contact_flag, (long long*)acc, scale_state);
+  hipLaunchKernelGGL(k_rope_collision_fx<2>, dim3(nblocks(n)), dim3(kBlock), 0, st, C, dt, (int)n,
+                     x, v, contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0],
+                     out_dir[1], out_dir[2], impulse, contact_flag, (long long*)acc, scale_state);
+  hipLaunchKernelGGL(k_fx_out, dim3(nblocks(3ll * (n_verts > 0 ? n_verts : 1))), dim3(kBlock), 0,
+                     st, (int)n_verts, 1, (long long*)acc, scale_state, ext_force,
+                     (float*)nullptr);
+  return check_launch("dt_rope_collision_fx");
+}
+
+int dt_rope_collision_adj_fx(const dt_contact_params* cp, float dt, int32_t n, const float* x,
+                             const float* v, const int32_t* contact_idx, int32_t idx_stride,
+                             int32_t idx_col, int32_t n_tris, const int32_t* tris,
+                             const float* pos, const float* vel, const float* out_dir,
+                             const float* g_impulse, const float* g_ext_force, float* g_x,
+                             float* g_v, int32_t n_verts, float* g_pos, float* g_vel,
+                             int64_t* acc, uint32_t* scale_state, void* stream) {
+  DT_REQUIRE(cp && out_dir, "NULL contact params / out_dir");
+  DT_REQUIRE(n >= 0 && n_tris >= 0 && n_verts >= 0 && idx_stride > 0 && idx_col >= 0 &&
+                 idx_col < idx_stride,
+             "bad size/stride");
+  if (n == 0) return DT_OK;
+  DT_REQUIRE(x && v && contact_idx && tris && pos && vel && g_impulse && g_ext_force && g_x &&
+                 g_v && g_pos && g_vel && acc && scale_state,
+             "NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const ContactP C = contact_dev(cp);
+  hipLaunchKernelGGL(k_rope_collision_adj_fx<1>, dim3(nblocks(n)), dim3(kBlock), 0, st, C, dt,
+                     (int)n, x, v, contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0],
+                     out_dir[1], out_dir[2], g_impulse, g_ext_force, g_x, g_v, (int)n_verts,
+                     (long long*)acc, scale_state);
+  hipLaunchKernelGGL(k_rope_collision_adj_fx<2>, dim3(nblocks(n)), dim3(kBlock), 0, st, C, dt,
+                     (int)n, x, v, contact_idx, idx_stride, idx_col, tris, pos, vel, out_dir[0],
+                     out_dir[1], out_dir[2], g_impulse, g_ext_force, g_x, g_v, (int)n_verts,
+                     (long long*)acc, scale_state);
+  hipLaunchKernelGGL(k_fx_out, dim3(nblocks(6ll * (n_verts > 0 ? n_verts : 1))), dim3(kBlock), 0,
+                     st, (int)n_verts, 2, (long long*)acc, scale_state, g_vel, g_pos);
+  return check_launch("dt_rope_collision_adj_fx");
 }
 
 int dt_rope_closest(int32_t n, const float* x, int32_t n_tris, const float* centroids4,

@@ -207,3 +207,137 @@ def config2_pad_press(n_particles=200_000, n_grid=128, dt=1e-5, n_substeps=500, 
                 v=np.zeros((len(x), 3), np.float32), mesh=(nodes, f2v, tris, layer, out_dir),
                 pad_mu=mu, pad_lam=lam, pad_mass=mass, control_vel=cv, inv_T=inv_T,
                 contact=dict(kn=1e4, kd=269.44, kt=154.78, mu_f=53.85), n_substeps=n_substeps)
+
+
+def _rot_z(deg):
+    c, s = np.cos(np.radians(deg)), np.sin(np.radians(deg))
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def config5_gripper(n_particles=2_000_000, n_grid=256, dt=1e-5, n_substeps=1000,
+                    pad_verts=(40, 40, 6), n_rope=2001, rope_length=30.0, close_speed=10.0,
+                    seed=0):
+    """Full gripper scene: two FEM pads on a parallel jaw, a position-based
+    cable and an elastoplastic MPM object, coupled by penalty contact.
+
+    Units of the grasp task (cm: space_scale 10, obj_scale 4, rho 4,
+    ``tasks/grasp_plastic.py:35-46``), domain 10 per side, dx = 10 / n_grid.
+    Object: n_particles at dx/2 spacing (+-0.25 dx seeded jitter) in a tall
+    block (x : y : z = 2 : 5 : 1.5), an elastic core (E 5e4) inside a von Mises
+    shell (E 5e4, yield 50), its bottom at mid-height of the pads.  Pads: two
+    slab meshes of pad_verts vertices (spacing dx, 40 x 40 x 6 = 45,630 tets),
+    stable Neo-Hookean E 1e5, nu 0.45, density 2000, their contact faces a
+    quarter cell from the block's sides and facing each other along x.  The
+    two pads are the same mesh turned 180 degrees apart, so their face
+    diagonals run opposite ways; the contact query points (cell centres) sit
+    at (0.45, 0.3) of a face cell, where the nearest triangle centroid is the
+    triangle under the point for either diagonal, with no distance ties.
+    They sit on the fingers of
+    the reference's parallel-jaw gripper (``sensor_model/gripper_kinematics.py``:
+    base 14 x 2, fingers 7.5 x 1.4; the grip width puts each finger frame on
+    its pad's face), whose controls close the jaw at close_speed per finger.
+    Cable: n_rope particles (n_rope - 1 segments) of rope_length along z under
+    the block, through the jaw, touching the left pad; stretch and bending
+    compliance 1e-6, the cable task's radius 0.25 and density 10
+    (``tasks/cable_straightening.py:34-44``), particle 0 held.  Contact: the
+    grasp task's kd / kt / friction with kn = 1e4 (grid), and the cable task's
+    force and constants (kn 55.33, kd 239.97, kt 94.35, friction 4.9,
+    cable_straightening.py:65-68) for the cable.  The reference has the object-pad and cable-pad halves in separate
+    tasks (grasp_plastic.py:151-164, cable_straightening.py:166-173); the
+    coupled substep composes them.
+    Returns a dict with everything the coupled runner needs."""
+    from .sensor_model.meshes import slab_mesh
+    space = 10.0
+    prm = MPMParams(n_grid=n_grid, space_scale=space, obj_scale=4.0, rho=4.0, dt=dt,
+                    gravity=(0.0, -9.8, 0.0))
+    dx = prm.dx
+    sp = dx / 2
+    # block lattice x : y : z = 2 : 5 : 1.5
+    unit = (n_particles / (2 * 5 * 1.5)) ** (1.0 / 3.0)
+    nx, nz = max(2, int(round(2 * unit))), max(2, int(round(1.5 * unit)))
+    ny = -(-n_particles // (nx * nz))
+    rng = np.random.default_rng(seed)
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    p = np.stack([ii, jj, kk], -1).reshape(-1, 3)[:n_particles].astype(np.float64)
+    pnx, pnz, pny = pad_verts
+    h = dx
+    pad_w = (pnx - 1) * h                     # pad extent along the block's y
+    pad_d = (pnz - 1) * h                     # and z
+    y_pad0 = (prm.bound + 2 + 0.05) * dx     # pads' lower edge: cell centres 0.45 h above
+    y_block0 = y_pad0 + 0.5 * pad_w
+    x = np.empty_like(p)
+    x[:, 0] = 0.5 * space + (p[:, 0] - (nx - 1) / 2) * sp
+    x[:, 1] = y_block0 + p[:, 1] * sp
+    x[:, 2] = 0.5 * space + (p[:, 2] - (nz - 1) / 2) * sp
+    x += rng.uniform(-0.25 * dx, 0.25 * dx, size=x.shape)
+    # the block's left side on a grid node: the node below it, whose cell
+    # centre (the contact query point) lies half a cell outside the block,
+    # then sits a quarter cell behind the left pad's face, so both pads touch
+    # the block from the first substep
+    x[:, 0] += (np.ceil(x[:, 0].min() / dx) - x[:, 0].min() / dx) * dx
+    half = 0.5 * (x[:, 0].max() - x[:, 0].min())
+    centre = x.mean(0)
+    r = np.linalg.norm((x - centre) / np.array([half, 0.5 * (x[:, 1].max() - x[:, 1].min()),
+                                               0.5 * (x[:, 2].max() - x[:, 2].min())]), axis=1)
+    mat_id = np.where(r < 0.5, 0, 1).astype(np.int32)
+    from .params import VON_MISES
+    mats = [Material(5e4, 0.3, ELASTIC), Material(5e4, 0.3, VON_MISES, yield_stress=50.0)]
+    # pads: the slab's contact face (local y = 0, normal -y) turned to face +x
+    # (left pad, rotation +90 about z) or -x (right pad, -90)
+    gap = 0.25 * dx
+    face_l = x[:, 0].min() - gap
+    face_r = x[:, 0].max() + gap
+    nodes0, f2v, tris, layer, _ = slab_mesh(pnx, pnz, pny, h=h, origin=(0.0, 0.0, 0.0))
+    z0 = (np.floor((0.5 * space - 0.5 * pad_d) / h) + 0.2) * h   # cell centres 0.3 h in
+    pads = []
+    for side, face, deg, od in ((1, face_l, 90.0, (1.0, 0.0, 0.0)),
+                                (2, face_r, -90.0, (-1.0, 0.0, 0.0))):
+        R = _rot_z(deg)
+        loc = nodes0.copy()
+        loc[:, 0] -= 0.5 * pad_w              # centre the face on the block's x-y centre line
+        w = loc @ R.T
+        w[:, 0] += face
+        w[:, 1] += y_pad0 + 0.5 * pad_w
+        w[:, 2] += z0
+        pads.append(dict(nodes=w, R=R, out_dir=np.asarray(od), face=face))
+    E, nu, rho_pad = 1e5, 0.45, 2000.0
+    mu, lam = E / 2 / (1 + nu), E * nu / (1 + nu) / (1 - 2 * nu)
+    vol = pad_w * pad_d * ((pny - 1) * h)
+    mass = rho_pad * vol / len(nodes0)
+    # the jaw: ee on the pads' centre line, grip width placing finger 1 on the
+    # left pad's face (finger 2 then sits on the right one by symmetry)
+    from .sensor_model.gripper_kinematics import BASE_HEIGHT, BASE_WIDTH, FINGER_HEIGHT
+    ee_t = (0.5 * (face_l + face_r), y_pad0 + 0.5 * pad_w + BASE_HEIGHT + FINGER_HEIGHT, z0 +
+            0.5 * pad_d)
+    grip_width = face_l - (ee_t[0] - BASE_WIDTH / 2)
+    controls = np.zeros((n_substeps + 1, 7), np.float32)
+    controls[:, 6] = close_speed
+    # cable along z under the block's lower end, between the jaws, touching the
+    # left pad (its surface a tenth of a cell into the pad at the start)
+    r_rad, r_rho = 0.25, 10.0
+    rest = rope_length / (n_rope - 1)
+    rope_start = (face_l + r_rad - 0.1 * dx, y_pad0 + 0.25 * pad_w, 0.5 * space - 0.5 * rope_length)
+    # markers: camera 1 unit behind each pad's face, looking at it (pad-local
+    # +y toward the face, as config2_pad_press)
+    inv_T = []
+    for pd in pads:
+        # slab-local (nodes0) -> world
+        T = np.eye(4)
+        T[:3, :3] = pd["R"]
+        T[:3, 3] = pd["nodes"][0] - pd["R"] @ nodes0[0]
+        Rwl = np.diag([1.0, -1.0, -1.0])
+        cam = np.eye(4)
+        cam[:3, :3] = Rwl
+        cam[:3, 3] = -Rwl @ np.array([0.5 * pad_w, 1.0, 0.5 * pad_d])
+        inv_T.append(cam @ np.linalg.inv(T))
+    return dict(prm=prm, mats=mats, x=x.astype(np.float32), v=np.zeros((len(x), 3), np.float32),
+                mat_id=mat_id, pads=[(pd["nodes"].astype(np.float32), f2v, tris, layer,
+                                      pd["out_dir"]) for pd in pads],
+                pad_mu=mu, pad_lam=lam, pad_mass=mass, inv_T=inv_T,
+                gripper=dict(rot=(0.0, 0.0, 0.0), trans=ee_t, grip_width=grip_width, dt=dt),
+                controls=controls,
+                rope=dict(n=n_rope, p_rad=r_rad, p_rho=r_rho, rest_length=rest, start=rope_start,
+                          ori=(0.0, 0.0, 1.0), compliance=1e-6),
+                contact=dict(kn=1e4, kd=269.44, kt=154.78, mu_f=53.85),
+                rope_contact=dict(kn=55.33, kd=239.97, kt=94.35, mu_f=4.9),
+                n_substeps=n_substeps, target_com=(centre + np.array([0.0, 0.05, 0.0])))

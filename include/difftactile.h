@@ -721,6 +721,24 @@ int dt_rope_collision_adj(const dt_contact_params* cp, float dt, int32_t n, cons
                           const float* vel, const float* out_dir, const float* g_impulse,
                           const float* g_ext_force, float* g_x, float* g_v, float* g_pos,
                           float* g_vel, void* stream);
+/* The same two kernels with the pad-vertex sums exact and order-free (the
+   int64 fixed-point passes of dt_contact_collision_fx; no reference
+   counterpart), so they repeat bit for bit; same acc / scale_state rules
+   (n_verts*3 or *6 int64, 2 or 4 uint32, zero before and after).  The
+   impulse, g_x, g_v (per particle) and ext_force, g_pos, g_vel accumulate. */
+int dt_rope_collision_fx(const dt_contact_params* cp, float dt, int32_t n, const float* x,
+                         const float* v, const int32_t* contact_idx, int32_t idx_stride,
+                         int32_t idx_col, int32_t n_tris, const int32_t* tris, const float* pos,
+                         const float* vel, const float* out_dir, float* impulse, int32_t n_verts,
+                         float* ext_force, int32_t* contact_flag, int64_t* acc,
+                         uint32_t* scale_state, void* stream);
+int dt_rope_collision_adj_fx(const dt_contact_params* cp, float dt, int32_t n, const float* x,
+                             const float* v, const int32_t* contact_idx, int32_t idx_stride,
+                             int32_t idx_col, int32_t n_tris, const int32_t* tris,
+                             const float* pos, const float* vel, const float* out_dir,
+                             const float* g_impulse, const float* g_ext_force, float* g_x,
+                             float* g_v, int32_t n_verts, float* g_pos, float* g_vel,
+                             int64_t* acc, uint32_t* scale_state, void* stream);
 
 #ifdef __cplusplus
 }
