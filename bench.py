@@ -55,10 +55,12 @@ def parse():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device-to-device copy (the reference "
                          "kernel of known traffic for rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "pad_press", "population"],
+    ap.add_argument("--workload", default="sphere",
+                    choices=["sphere", "pad_press", "population", "gripper"],
                     help="sphere: 1M-particle multi-material sphere (default, the metric's "
                          "workload); pad_press: slab pad pressing a von Mises block; "
-                         "population: 64 environments, batched vs a per-environment loop")
+                         "population: 64 environments, batched vs a per-environment loop; "
+                         "gripper: jaw, two pads, a cable and a 2M-particle block")
     ap.add_argument("--pop-substeps", type=int, default=300,
                     help="population workload: substeps per fwd+bwd step")
     ap.add_argument("--pad-recompute", action="store_true",
@@ -597,6 +599,100 @@ def bench_population(args, torch):
     print(json.dumps(res), flush=True)
 
 
+def bench_gripper(args, torch):
+    """Full gripper scene: the jaw's controls drive two 45,630-tet pads that
+    grip a 2M-particle elastoplastic block on a 256^3 grid, with a
+    2,001-particle cable touching the left pad; fwd+bwd over a 1000-substep
+    trajectory checkpointed every 50 substeps (segments beyond the resident
+    budget recomputed); loss = squared distance of the block's centre of mass
+    from a target plus both pads' summed marker displacement; gradient w.r.t.
+    the jaw controls (through the kinematic chain)."""
+    import numpy as np
+    from difftactile_amd import scenes
+    from difftactile_amd.coupled import CoupledSim, Pad
+    from difftactile_amd.losses import MarkerModel, com_loss, marker_displacement
+    from difftactile_amd.object_model.pbd_rope import PBDRope
+    from difftactile_amd.sensor_model.gripper_kinematics import Gripper
+    from difftactile_amd.sim import MPMSim
+    c = scenes.config5_gripper()
+    prm, n = c["prm"], len(c["x"])
+    T = c["n_substeps"]
+    dev = torch.device("cuda")
+    total_mem = torch.cuda.get_device_properties(dev).total_memory
+    sim = MPMSim(prm, c["mats"], n, mat_id=c["mat_id"], checkpoint_every=50, sort=True,
+                 resident_bytes=int(args.resident_frac * total_mem))
+    pads = [Pad(nodes, f2v, tris, layer, od, c["pad_mu"], c["pad_lam"], c["pad_mass"], prm.dt)
+            for nodes, f2v, tris, layer, od in c["pads"]]
+    r = c["rope"]
+    rope = PBDRope(dt=prm.dt, p_rad=r["p_rad"], p_rho=r["p_rho"], n_particles=r["n"],
+                   rest_length=r["rest_length"])
+    rope.stretch_compliance = rope.bending_compliance = r["compliance"]
+    rp0, rv0 = rope.init(r["start"], r["ori"])
+    cs = CoupledSim(sim, pads, **c["contact"], rope=rope, rope_contact=c["rope_contact"])
+    g = c["gripper"]
+    grip = Gripper(dt=g["dt"])
+    grip.init_params(g["rot"], g["trans"], g["grip_width"])
+    init_x, mms = [], []
+    for k, (nodes, f2v, tris, layer, od) in enumerate(c["pads"]):
+        Pk = grip.finger_pose(k + 1).astype(np.float64)
+        nh = np.concatenate([nodes.astype(np.float64), np.ones((len(nodes), 1))], 1)
+        init_x.append(torch.tensor((nh @ np.linalg.inv(Pk).T)[:, :3].astype(np.float32),
+                                   device=dev))
+        mms.append(MarkerModel(nodes, np.where(layer == layer.max())[0], c["inv_T"][k]))
+    seg_bytes = 51 * cs.substep_bytes()
+    kept = min(T // 50, sim.resident_bytes // seg_bytes)
+    x0 = torch.tensor(c["x"], device=dev)
+    v0 = torch.tensor(c["v"], device=dev)
+    C0 = torch.zeros(n, 3, 3, device=dev)
+    F0 = torch.eye(3, device=dev).repeat(n, 1, 1)
+    pos0 = [torch.tensor(p[0], device=dev) for p in c["pads"]]
+    ctrl0 = torch.tensor(c["controls"], device=dev)
+    tgt = torch.tensor(c["target_com"], dtype=torch.float32, device=dev)
+
+    def step():
+        ctrl = ctrl0.clone().requires_grad_(True)
+        cv1, cv2 = grip.control_velocities(ctrl, *init_x)
+        pst = [(p.clone(), cvk[0].detach().clone(), p.clone()) for p, cvk in zip(pos0, (cv1, cv2))]
+        out = cs.rollout(x0, v0, C0, F0, pst, [cv1, cv2], T, rope_state=(rp0.clone(), rv0.clone()))
+        loss = ((com_loss(out[0]) - tgt) ** 2).sum()
+        for mm, (pos, _, vpos) in zip(mms, out[4]):
+            loss = loss + marker_displacement(mm, pos, vpos)
+        gc = torch.autograd.grad(loss, ctrl)[0]
+        return loss, gc, out
+
+    log(f"[bench] gripper: {n} particles, {prm.n_grid}^3, 2 x {pads[0].n_tets} tets, "
+        f"{rope.n_particles} cable particles, {T} substeps")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, gc, out = step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gap = [float(out[4][0][0][:, 0].detach().max() - float(c["pads"][0][0][:, 0].max())),
+           float(out[4][1][0][:, 0].detach().min() - float(c["pads"][1][0][:, 0].min()))]
+    # the cable's velocity is (p' - p) / dt of float32 positions: a substep
+    # moves a particle only when it displaces it by half a position ulp
+    cable_move = float((out[5][0].detach() - rp0).norm(dim=1).max())
+    res = {"metric": "fwd+bwd particle-substeps/s (gripper scene)",
+           "value": n * T * args.steps / wall, "unit": "particle-substeps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+           "substeps_per_s": T * args.steps / wall, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"parallel jaw with two pads ({pads[0].n_tets} tets each), "
+                                  f"a {rope.n_particles}-particle cable and an elastoplastic "
+                                  f"block ({n} particles, {prm.n_grid}^3), {T} substeps fwd+bwd "
+                                  "checkpointed every 50, loss = centre-of-mass error + marker "
+                                  "displacement, gradient w.r.t. the jaw controls",
+                      "resident_segments": f"{kept} of {T // 50}",
+                      "resident_gb": round(kept * seg_bytes / 1e9, 1)},
+           "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(gc).all()),
+           "grad_abs_max": float(gc.abs().max()), "pad_face_travel": gap,
+           "cable_max_displacement": cable_move}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -615,6 +711,9 @@ def main():
         return
     if args.workload == "population":
         bench_population(args, torch)
+        return
+    if args.workload == "gripper":
+        bench_gripper(args, torch)
         return
     from difftactile_amd import scenes
     from difftactile_amd.sim import MPMSim, _segments
