@@ -605,12 +605,13 @@ def bench_gripper(args, torch):
     2,001-particle cable touching the left pad; fwd+bwd over a 1000-substep
     trajectory checkpointed every 50 substeps (segments beyond the resident
     budget recomputed); loss = squared distance of the block's centre of mass
-    from a target plus both pads' summed marker displacement; gradient w.r.t.
-    the jaw controls (through the kinematic chain)."""
+    from a target plus the reference's marker loss (squared mean marker offset
+    of each pad); gradient w.r.t. the jaw controls (through the kinematic
+    chain)."""
     import numpy as np
     from difftactile_amd import scenes
     from difftactile_amd.coupled import CoupledSim, Pad
-    from difftactile_amd.losses import MarkerModel, com_loss, marker_displacement
+    from difftactile_amd.losses import MarkerModel, com_loss, marker_motion_loss
     from difftactile_amd.object_model.pbd_rope import PBDRope
     from difftactile_amd.sensor_model.gripper_kinematics import Gripper
     from difftactile_amd.sim import MPMSim
@@ -654,9 +655,8 @@ def bench_gripper(args, torch):
         cv1, cv2 = grip.control_velocities(ctrl, *init_x)
         pst = [(p.clone(), cvk[0].detach().clone(), p.clone()) for p, cvk in zip(pos0, (cv1, cv2))]
         out = cs.rollout(x0, v0, C0, F0, pst, [cv1, cv2], T, rope_state=(rp0.clone(), rv0.clone()))
-        loss = ((com_loss(out[0]) - tgt) ** 2).sum()
-        for mm, (pos, _, vpos) in zip(mms, out[4]):
-            loss = loss + marker_displacement(mm, pos, vpos)
+        loss = ((com_loss(out[0]) - tgt) ** 2).sum() + \
+            marker_motion_loss(mms, [(pos, vpos) for pos, _, vpos in out[4]])
         gc = torch.autograd.grad(loss, ctrl)[0]
         return loss, gc, out
 
@@ -683,8 +683,8 @@ def bench_gripper(args, torch):
            "config": {"workload": f"parallel jaw with two pads ({pads[0].n_tets} tets each), "
                                   f"a {rope.n_particles}-particle cable and an elastoplastic "
                                   f"block ({n} particles, {prm.n_grid}^3), {T} substeps fwd+bwd "
-                                  "checkpointed every 50, loss = centre-of-mass error + marker "
-                                  "displacement, gradient w.r.t. the jaw controls",
+                                  "checkpointed every 50, loss = centre-of-mass error + mean "
+                                  "marker offset, gradient w.r.t. the jaw controls",
                       "resident_segments": f"{kept} of {T // 50}",
                       "resident_gb": round(kept * seg_bytes / 1e9, 1)},
            "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(gc).all()),

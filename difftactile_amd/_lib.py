@@ -199,12 +199,17 @@ _SIGS = {
                                           _c_int, _c_int, _c_int, _vp, _vp, _vp,
                                           _P(ctypes.c_float), _vp, _vp, _vp, _vp, _c_int, _vp,
                                           _vp, _vp, _vp, _vp]),
-    "dt_loss_sum_positions": (_c_int, [_c_int, _c_int, _vp, _vp, _vp]),
-    "dt_loss_com_adj": (_c_int, [_c_int, _c_int, _P(ctypes.c_float), _vp, _vp]),
+    "dt_loss_com": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "dt_loss_com_adj": (_c_int, [_c_int, _c_int, _vp, _vp, _vp]),
+    "dt_contact_force_sum": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_contact_force_sum_adj": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
+    "dt_force_loss": (_c_int, [_c_int, _vp, _P(ctypes.c_float), _c_float, _c_float, _vp, _vp]),
+    "dt_marker_offset": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    "dt_marker_offset_adj": (_c_int, [_c_int, _vp, _vp, _vp, _vp]),
     "dt_markers_fwd": (_c_int, [_c_int, _vp, _vp, _vp, _P(ctypes.c_float), _P(ctypes.c_float),
                                 _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dt_markers_adj": (_c_int, [_c_int, _vp, _vp, _P(ctypes.c_float), _P(ctypes.c_float), _c_int,
-                                _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+                                _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dt_sort_tmp_bytes": (ctypes.c_int64, [_c_int]),
     "dt_sort_by_key": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),
     "dt_gather_rows": (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),

@@ -277,6 +277,8 @@ class CoupledSim:
                           P(exts[k]), None, pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
         if store is not None:
             store["ext"] = exts
+        # the pads' external forces of the latest substep (rollout(return_ext=True))
+        self._last_ext = exts
         _lib.call("dt_mpm_grid_op4", prm, P(g4), P(gf), P(gv4), s)
         _lib.call("dt_mpm_g2p4", prm, sim.n, P(st[0]), P(gv4), P(nxt[0]), P(nxt[1]), P(nxt[2]), s)
         if self.rope is not None:
@@ -290,9 +292,12 @@ class CoupledSim:
                       P(vel_u[k]), P(vpos), P(exts[k]), P(cv[k]), P(cv1[k]), P(o[0]), P(o[1]),
                       P(o[2]), s)
 
-    def _adj(self, st, pads_in, store, g_next, g_pads_next, g, g_pads, g_cv, g_cv1, mat_id):
+    def _adj(self, st, pads_in, store, g_next, g_pads_next, g, g_pads, g_cv, g_cv1, mat_id,
+             g_ext_out=None):
         """Adjoint of one substep.  g_next/g_pads_next: adjoints of the outputs;
-        writes g (object, overwritten), g_pads (overwritten), accumulates g_cv, g_cv1."""
+        writes g (object, overwritten), g_pads (overwritten), accumulates g_cv, g_cv1;
+        g_ext_out: adjoints of this substep's pad external forces as outputs
+        (or None)."""
         sim, s = self.sim, _lib.stream()
         nn = sim.prm.n_nodes
         prm = sim.c_prm
@@ -311,6 +316,8 @@ class CoupledSim:
             gpn, gvn, gvpn = g_pads_next[k]
             _lib.call("dt_fem_update2_adj", pad.fem, pad.n_verts, P(pad.layer, I32), P(gpn),
                       P(gvn), P(gvpn), P(gp), P(gvu), P(gvp), P(ge), P(g_cv[k]), P(g_cv1[k]), s)
+            if g_ext_out is not None and g_ext_out[k] is not None:
+                ge.add_(g_ext_out[k])
             g_ext.append(ge)
         npd = len(self.pads)
         if self.rope is not None:
@@ -372,13 +379,18 @@ class CoupledSim:
                   P(g_next[3]), P(g[0]), P(g[1]), P(g[2]), P(g[3]), sim._mat_grad_ptr(), s)
 
     # -- public --------------------------------------------------------------
-    def rollout(self, x, v, C, F, pad_states, control_vel, n_substeps, rope_state=None):
+    def rollout(self, x, v, C, F, pad_states, control_vel, n_substeps, rope_state=None,
+                return_ext=False):
         """x, v, C, F: object state ([n][k] layout); pad_states: list of (pos, vel, vpos)
         per pad; control_vel: list of per-pad (n_substeps+1, n_verts, 3) velocity
         sequences; rope_state: (pos, vel) of the cable (with a cable).  Returns
-        (x, v, C, F, [(pos, vel, vpos) per pad]) after n_substeps, and the cable's
-        (pos, vel) last with a cable, differentiable with respect to every tensor
-        input."""
+        (x, v, C, F, [(pos, vel, vpos) per pad]) after n_substeps, then the cable's
+        (pos, vel) with a cable, then with return_ext the pads' external forces of
+        the last substep (the reference's ``external_force_field[sub_steps - 2]``,
+        read by ``compute_contact_force``), all differentiable with respect to
+        every tensor input."""
+        if return_ext and n_substeps < 1:
+            raise ValueError("return_ext needs at least one substep")
         flat = [x, v, C, F]
         for ps in pad_states:
             flat += list(ps)
@@ -387,17 +399,23 @@ class CoupledSim:
             if rope_state is None:
                 raise ValueError("a scene with a cable needs rope_state")
             flat += list(rope_state)
-        out = _CoupledRollout.apply(self, int(n_substeps), *flat)
+        out = _CoupledRollout.apply(self, int(n_substeps), bool(return_ext), *flat)
+        npd = len(self.pads)
         obj = out[:4]
-        pads = [tuple(out[4 + 3 * k: 7 + 3 * k]) for k in range(len(self.pads))]
+        pads = [tuple(out[4 + 3 * k: 7 + 3 * k]) for k in range(npd)]
+        res = [*obj, pads]
+        o = 4 + 3 * npd
         if self.rope is not None:
-            return (*obj, pads, tuple(out[4 + 3 * len(self.pads):]))
-        return (*obj, pads)
+            res.append(tuple(out[o:o + 2]))
+            o += 2
+        if return_ext:
+            res.append(list(out[o:o + npd]))
+        return tuple(res)
 
 
 class _CoupledRollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, cs, n_substeps, *flat):
+    def forward(ctx, cs, n_substeps, return_ext, *flat):
         sim = cs.sim
         npd = len(cs.pads)
         x, v, C, F = (t.contiguous() for t in flat[:4])
@@ -445,6 +463,9 @@ class _CoupledRollout(torch.autograd.Function):
         out = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(st, W, shapes)]
         for p in pst:
             out += [t.clone() for t in p]
+        if return_ext:
+            out += [e.clone() for e in cs._last_ext]
+        ctx.return_ext = return_ext
         ctx.cs, ctx.n_substeps, ctx.ckpts, ctx.shapes = cs, n_substeps, ckpts, shapes
         ctx.kept = kept
         ctx.perm, ctx.mat_id, ctx.env = perm, mat_id, env
@@ -467,10 +488,15 @@ class _CoupledRollout(torch.autograd.Function):
             trio = gout[4 + 3 * k: 7 + 3 * k]
             gp.append([(t if t is not None else torch.zeros(cs.pads[k].n_verts, 3, device=dev)
                         ).contiguous().clone() for t in trio])
+        o = 4 + 3 * npd
         if cs.rope is not None:
             nr = cs.rope.n_particles
             gp.append([(t if t is not None else torch.zeros(nr, 3, device=dev)
-                        ).contiguous().float().clone() for t in gout[4 + 3 * npd:]])
+                        ).contiguous().float().clone() for t in gout[o:o + 2]])
+            o += 2
+        g_ext_out = None
+        if ctx.return_ext:
+            g_ext_out = [None if t is None else t.float().contiguous() for t in gout[o:o + npd]]
         g_cv = [torch.zeros(sh, device=dev) for sh in ctx.cv_shapes]
         gbuf = [torch.empty_like(t) for t in g]
         gpbuf = [[torch.empty_like(t) for t in p] for p in gp]
@@ -482,13 +508,18 @@ class _CoupledRollout(torch.autograd.Function):
                 frames, pframes, stores = ctx.kept[i]
                 ctx.kept[i] = None
             else:
+                if ctx.ckpts[i] is None:
+                    raise RuntimeError("a coupled rollout's backward runs once (it releases the "
+                                       "segments it kept); run the rollout again for another "
+                                       "gradient")
                 # recompute the segment, keeping what the adjoint needs
                 ck, ckp = ctx.ckpts[i]
                 frames, pframes, stores = cs._run_keep(ck, ckp, cvs, s0, k, ctx.mat_id)
             for j in reversed(range(k)):
                 f = s0 + j
                 cs._adj(frames[j], pframes[j], stores[j], g, gp, gbuf, gpbuf,
-                        [gc[f] for gc in g_cv], [gc[f + 1] for gc in g_cv], ctx.mat_id)
+                        [gc[f] for gc in g_cv], [gc[f + 1] for gc in g_cv], ctx.mat_id,
+                        g_ext_out if f == ctx.n_substeps - 1 else None)
                 g, gbuf = gbuf, g
                 gp, gpbuf = gpbuf, gp
         res = [sim._scatter(ctx.perm, t, w, sh) for t, w, sh in zip(g, W, ctx.shapes)]
@@ -497,4 +528,4 @@ class _CoupledRollout(torch.autograd.Function):
         res += g_cv
         for p in gp[npd:]:
             res += list(p)
-        return (None, None, *res)
+        return (None, None, None, *res)

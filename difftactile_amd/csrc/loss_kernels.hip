@@ -1,47 +1,156 @@
-// Losses on the measured path (gfx950): centre of mass and tactile markers.
+// Losses on the measured path (gfx950): centre of mass, tactile markers and
+// the pads' contact force.
 //
 // Reference (difftactile/):
 //   compute_obj_pos        tasks/grasp_elastic.py:240-243 (mean particle position)
 //   extract_markers        sensor_model/fem_sensor.py:161-187
 //   project_3d_2d          sensor_model/fisheye_model.py:13-30 (equidistant fisheye)
 //   compute_marker_motion  tasks/grasp_plastic.py:240-246 (mean marker offset)
+//   compute_marker_loss    tasks/grasp_plastic.py:254-256
+//   compute_contact_force  tasks/grasp_plastic.py:218-230 (1/6 per triangle vertex)
+//   compute_force_loss     tasks/grasp_plastic.py:232-238
+//
+// Every reduction is float32 in a fixed order (per-block partials, then one
+// block over the partials), so a loss and its gradient repeat bit for bit;
+// the adjoints gather (the marker interpolation through a surface-point CSR)
+// or add equal values, never order-dependent float atomics.
 #include "dt_common.h"
 #include "dt_device.h"
 
 namespace dt {
 
 // ---------------------------------------------------------------------------
-// centre of mass: out[c] += sum_p x[p][c] (float64 accumulation)
+// fixed-order sums
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_sum_positions(int n, int soa,
-                                                          const float* __restrict__ x,
-                                                          double* __restrict__ out) {
-  __shared__ double red[3][kBlock];
-  PL L = pl_vec(soa, n);
-  double s[3] = {0.0, 0.0, 0.0};
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (long long)gridDim.x * blockDim.x)
-    for (int c = 0; c < 3; ++c) s[c] += (double)x[p * L.sp + c * L.sc];
-  for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = s[c];
+// Sum of v over the block (a wave butterfly, then the block's waves in
+// order); thread q < kTerms writes term q, times scale, to out[q].
+template <int kTerms>
+__device__ __forceinline__ void loss_block_sum(float (&v)[kTerms], float* out, float scale) {
+  __shared__ float part[kBlock / 64][kTerms];
+  for (int q = 0; q < kTerms; ++q)
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < kTerms; ++q) part[threadIdx.x >> 6][q] = v[q];
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
-    __syncthreads();
+  if ((int)threadIdx.x < kTerms) {
+    float t = part[0][threadIdx.x];
+    for (int w = 1; w < kBlock / 64; ++w) t += part[w][threadIdx.x];
+    out[threadIdx.x] = t * scale;
   }
-  if (threadIdx.x == 0)
-    for (int c = 0; c < 3; ++c) atomicAdd(&out[c], red[c][0]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_com_adj(int n, int soa, float g0, float g1, float g2,
+// blocks of a reduction over n items (a function of n only)
+static inline unsigned reduce_grid(long long n) {
+  const long long b = (n + kBlock - 1) / kBlock;
+  return (unsigned)(b < DT_REDUCE_BLOCKS ? (b < 1 ? 1 : b) : DT_REDUCE_BLOCKS);
+}
+
+// out[q] = scale * sum over nb blocks of partials[kTerms b + q] (one block)
+template <int kTerms>
+__global__ __launch_bounds__(kBlock) void k_sum_block_partials(int nb,
+                                                               const float* __restrict__ partials,
+                                                               float scale, float* __restrict__ out) {
+  float v[kTerms];
+  for (int q = 0; q < kTerms; ++q)
+    v[q] = (int)threadIdx.x < nb ? partials[(long long)kTerms * threadIdx.x + q] : 0.f;
+  loss_block_sum<kTerms>(v, out, scale);
+}
+
+// ---------------------------------------------------------------------------
+// centre of mass
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_com_partials(int n, int soa, const float* __restrict__ x,
+                                                         float* __restrict__ partials) {
+  PL L = pl_vec(soa, n);
+  float s[3] = {0.f, 0.f, 0.f};
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long long)gridDim.x * blockDim.x)
+    for (int c = 0; c < 3; ++c) s[c] += x[p * L.sp + c * L.sc];
+  loss_block_sum<3>(s, partials + 3ll * blockIdx.x, 1.f);
+}
+
+// gx[p] += g / n, g: device float[3]
+__global__ __launch_bounds__(kBlock) void k_com_adj(int n, int soa, const float* __restrict__ g,
                                                     float* __restrict__ gx) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   PL L = pl_vec(soa, n);
   float inv = 1.f / (float)n;
-  gx[p * L.sp] += g0 * inv;
-  gx[p * L.sp + L.sc] += g1 * inv;
-  gx[p * L.sp + 2 * L.sc] += g2 * inv;
+  gx[p * L.sp] += g[0] * inv;
+  gx[p * L.sp + L.sc] += g[1] * inv;
+  gx[p * L.sp + 2 * L.sc] += g[2] * inv;
+}
+
+// ---------------------------------------------------------------------------
+// contact force of a pad and the force loss
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_contact_force_partials(int nt,
+                                                                   const int32_t* __restrict__ tris,
+                                                                   const float* __restrict__ ext,
+                                                                   float* __restrict__ partials) {
+  float s[3] = {0.f, 0.f, 0.f};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
+       t += (long long)gridDim.x * blockDim.x)
+    for (int k = 0; k < 3; ++k) {
+      const long long v = tris[3 * t + k];
+      for (int c = 0; c < 3; ++c) s[c] += ext[3 * v + c];
+    }
+  loss_block_sum<3>(s, partials + 3ll * blockIdx.x, 1.f);
+}
+
+// g_ext[v] += g / 6 for each of the pad's triangles holding v: all of a
+// vertex's addends are equal, so the sum does not depend on their order
+__global__ __launch_bounds__(kBlock) void k_contact_force_adj(int nt,
+                                                              const int32_t* __restrict__ tris,
+                                                              const float* __restrict__ g,
+                                                              float* __restrict__ g_ext) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const float e[3] = {g[0] * (1.f / 6.f), g[1] * (1.f / 6.f), g[2] * (1.f / 6.f)};
+  for (int k = 0; k < 3; ++k) {
+    const long long v = tris[3 * t + k];
+    for (int c = 0; c < 3; ++c) atomic_add_f(g_ext + 3 * v + c, e[c]);
+  }
+}
+
+constexpr int kMaxForcePads = 8;
+struct ForceLossArgs {
+  float inv_rot[kMaxForcePads][9];
+  float mu, beta;
+  int n;
+};
+
+// p_k = inv_rot_k cf_k; loss = beta sum_k |p_k.x| / |mu p_k.y| when every
+// |p_k.x| > 1e-3, else 0.  out[0] = loss, out[1 + 3k + c] = d loss / d cf_k[c].
+__global__ __launch_bounds__(64) void k_force_loss(ForceLossArgs A, const float* __restrict__ cf,
+                                                   float* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  float p[kMaxForcePads][3];
+  bool on = A.n > 0;
+  for (int k = 0; k < A.n; ++k) {
+    for (int r = 0; r < 3; ++r)
+      p[k][r] = A.inv_rot[k][3 * r] * cf[3 * k] + A.inv_rot[k][3 * r + 1] * cf[3 * k + 1] +
+                A.inv_rot[k][3 * r + 2] * cf[3 * k + 2];
+    on = on && fabsf(p[k][0]) > 1e-3f;
+  }
+  float loss = 0.f;
+  for (int k = 0; k < A.n; ++k) {
+    float gp[3] = {0.f, 0.f, 0.f};
+    if (on) {
+      const float den = fabsf(A.mu * p[k][1]);
+      loss += A.beta * (fabsf(p[k][0]) / den);
+      // d/dx |x| / |mu y| = sign(x) / |mu y|; d/dy = -|x| sign(y) / (|mu| y^2)
+      const float sx = p[k][0] > 0.f ? 1.f : (p[k][0] < 0.f ? -1.f : 0.f);
+      const float sy = p[k][1] > 0.f ? 1.f : (p[k][1] < 0.f ? -1.f : 0.f);
+      gp[0] = A.beta * sx / den;
+      gp[1] = -A.beta * fabsf(p[k][0]) * sy / (fabsf(A.mu) * p[k][1] * p[k][1]);
+    }
+    // d loss / d cf = inv_rot^T d loss / d p
+    for (int c = 0; c < 3; ++c)
+      out[1 + 3 * k + c] = A.inv_rot[k][c] * gp[0] + A.inv_rot[k][3 + c] * gp[1] +
+                           A.inv_rot[k][6 + c] * gp[2];
+  }
+  out[0] = loss;
 }
 
 // ---------------------------------------------------------------------------
@@ -163,19 +272,56 @@ __global__ __launch_bounds__(kBlock) void k_marker_interp(int nm, int kc,
   virt[2 * m + 1] = b1;
 }
 
-// adjoint of predict_markers w.r.t. the surface projections (scatter)
-__global__ __launch_bounds__(kBlock) void k_marker_interp_adj(int nm, int kc,
-                                                              const int32_t* __restrict__ idx,
+// adjoint of predict_markers w.r.t. the surface projections, gathered per
+// surface point over its (marker, neighbour) entries in entry order:
+// entries s_start[s] .. s_start[s+1] of s_entry hold m k_closest + j for every
+// interp_idx[m][j] == s
+__global__ __launch_bounds__(kBlock) void k_marker_interp_adj(int ns, int kc,
+                                                              const int32_t* __restrict__ s_start,
+                                                              const int32_t* __restrict__ s_entry,
                                                               const float* __restrict__ w,
                                                               const float* __restrict__ gpred,
                                                               float* __restrict__ gloc) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  float g0 = 0.f, g1 = 0.f;
+  for (int e = s_start[s]; e < s_start[s + 1]; ++e) {
+    const int mj = s_entry[e];
+    const int m = mj / kc;
+    g0 += w[mj] * gpred[2 * m];
+    g1 += w[mj] * gpred[2 * m + 1];
+  }
+  gloc[2 * s] = g0;
+  gloc[2 * s + 1] = g1;
+}
+
+// mean marker offset: out2 = scale * sum_m (pred[m] - virt[m]) (one block partial per block)
+__global__ __launch_bounds__(kBlock) void k_marker_offset_partials(int nm,
+                                                                   const float* __restrict__ pred,
+                                                                   const float* __restrict__ virt,
+                                                                   float* __restrict__ partials) {
+  float s[2] = {0.f, 0.f};
+  for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm;
+       m += (long long)gridDim.x * blockDim.x) {
+    s[0] += pred[2 * m] - virt[2 * m];
+    s[1] += pred[2 * m + 1] - virt[2 * m + 1];
+  }
+  loss_block_sum<2>(s, partials + 2ll * blockIdx.x, 1.f);
+}
+
+// g_pred[m] += g2 / nm, g_virt[m] -= g2 / nm (g2: device float[2])
+__global__ __launch_bounds__(kBlock) void k_marker_offset_adj(int nm, const float* __restrict__ g2,
+                                                              float* __restrict__ gpred,
+                                                              float* __restrict__ gvirt) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nm) return;
-  for (int j = 0; j < kc; ++j) {
-    int s = idx[m * kc + j];
-    float ww = w[m * kc + j];
-    atomic_add_f(gloc + 2 * s, ww * gpred[2 * m]);
-    atomic_add_f(gloc + 2 * s + 1, ww * gpred[2 * m + 1]);
+  const float inv = 1.f / (float)nm;
+  const float a = g2[0] * inv, b = g2[1] * inv;
+  gpred[2 * m] += a;
+  gpred[2 * m + 1] += b;
+  if (gvirt) {
+    gvirt[2 * m] -= a;
+    gvirt[2 * m + 1] -= b;
   }
 }
 
@@ -197,9 +343,10 @@ __global__ __launch_bounds__(kBlock) void k_surface_project_adj(MarkerArgs A, in
   project_adj(A.K, q, gp, gcam);
   // cam = (q0, q2, q1), q = T[:3,:3] p + t
   float gq[3] = {gcam[0], gcam[2], gcam[1]};
+  // surface ids are distinct: one thread per vertex
   for (int c = 0; c < 3; ++c) {
     float g = A.T[0 * 4 + c] * gq[0] + A.T[1 * 4 + c] * gq[1] + A.T[2 * 4 + c] * gq[2];
-    atomic_add_f(gpos + 3ll * v + c, g);
+    gpos[3ll * v + c] += g;
   }
 }
 
@@ -209,16 +356,16 @@ using namespace dt;
 
 extern "C" {
 
-int dt_loss_sum_positions(int32_t n, int32_t soa, const float* x, double* out3, void* stream) {
-  DT_REQUIRE(n >= 0, "n must be >= 0");
-  DT_REQUIRE(out3, "NULL pointer");
-  if (n == 0) return DT_OK;
-  DT_REQUIRE(x, "NULL pointer");
-  int blocks = nblocks(n);
-  if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(k_sum_positions, dim3(blocks), dim3(kBlock), 0, (hipStream_t)stream, n, soa,
-                     x, out3);
-  return check_launch("dt_loss_sum_positions");
+int dt_loss_com(int32_t n, int32_t soa, const float* x, float* scratch, float* out3,
+                void* stream) {
+  DT_REQUIRE(n >= 1, "n must be >= 1");
+  DT_REQUIRE(x && scratch && out3, "NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_com_partials, dim3(nb), dim3(kBlock), 0, st, n, soa, x, scratch);
+  hipLaunchKernelGGL(k_sum_block_partials<3>, dim3(1), dim3(kBlock), 0, st, (int)nb, scratch,
+                     1.f / (float)n, out3);
+  return check_launch("dt_loss_com");
 }
 
 int dt_loss_com_adj(int32_t n, int32_t soa, const float* g_com3, float* gx, void* stream) {
@@ -226,8 +373,68 @@ int dt_loss_com_adj(int32_t n, int32_t soa, const float* g_com3, float* gx, void
   if (n == 0) return DT_OK;
   DT_REQUIRE(g_com3 && gx, "NULL pointer");
   hipLaunchKernelGGL(k_com_adj, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream, n, soa,
-                     g_com3[0], g_com3[1], g_com3[2], gx);
+                     g_com3, gx);
   return check_launch("dt_loss_com_adj");
+}
+
+int dt_contact_force_sum(int32_t n_tris, const int32_t* tris, const float* ext, float* scratch,
+                         float* out3, void* stream) {
+  DT_REQUIRE(n_tris >= 0, "n_tris must be >= 0");
+  DT_REQUIRE(scratch && out3 && (n_tris == 0 || (tris && ext)), "NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = reduce_grid(n_tris);
+  hipLaunchKernelGGL(k_contact_force_partials, dim3(nb), dim3(kBlock), 0, st, n_tris, tris, ext,
+                     scratch);
+  hipLaunchKernelGGL(k_sum_block_partials<3>, dim3(1), dim3(kBlock), 0, st, (int)nb, scratch,
+                     1.f / 6.f, out3);
+  return check_launch("dt_contact_force_sum");
+}
+
+int dt_contact_force_sum_adj(int32_t n_tris, const int32_t* tris, const float* g3, float* g_ext,
+                             void* stream) {
+  DT_REQUIRE(n_tris >= 0, "n_tris must be >= 0");
+  if (n_tris == 0) return DT_OK;
+  DT_REQUIRE(tris && g3 && g_ext, "NULL pointer");
+  hipLaunchKernelGGL(k_contact_force_adj, dim3(nblocks(n_tris)), dim3(kBlock), 0,
+                     (hipStream_t)stream, n_tris, tris, g3, g_ext);
+  return check_launch("dt_contact_force_sum_adj");
+}
+
+int dt_force_loss(int32_t n_pads, const float* cf, const float* inv_rot9, float mu, float beta,
+                  float* out, void* stream) {
+  DT_REQUIRE(n_pads >= 1 && n_pads <= kMaxForcePads, "n_pads must be in [1, 8]");
+  DT_REQUIRE(cf && inv_rot9 && out, "NULL pointer");
+  ForceLossArgs A;
+  for (int k = 0; k < n_pads; ++k)
+    for (int i = 0; i < 9; ++i) A.inv_rot[k][i] = inv_rot9[9 * k + i];
+  A.mu = mu;
+  A.beta = beta;
+  A.n = n_pads;
+  hipLaunchKernelGGL(k_force_loss, dim3(1), dim3(64), 0, (hipStream_t)stream, A, cf, out);
+  return check_launch("dt_force_loss");
+}
+
+int dt_marker_offset(int32_t n_markers, const float* pred, const float* virt, float* scratch,
+                     float* out2, void* stream) {
+  DT_REQUIRE(n_markers >= 1, "n_markers must be >= 1");
+  DT_REQUIRE(pred && virt && scratch && out2, "NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = reduce_grid(n_markers);
+  hipLaunchKernelGGL(k_marker_offset_partials, dim3(nb), dim3(kBlock), 0, st, n_markers, pred,
+                     virt, scratch);
+  hipLaunchKernelGGL(k_sum_block_partials<2>, dim3(1), dim3(kBlock), 0, st, (int)nb, scratch,
+                     1.f / (float)n_markers, out2);
+  return check_launch("dt_marker_offset");
+}
+
+int dt_marker_offset_adj(int32_t n_markers, const float* g2, float* g_pred, float* g_virt,
+                         void* stream) {
+  DT_REQUIRE(n_markers >= 0, "n_markers must be >= 0");
+  if (n_markers == 0) return DT_OK;
+  DT_REQUIRE(g2 && g_pred, "NULL pointer");
+  hipLaunchKernelGGL(k_marker_offset_adj, dim3(nblocks(n_markers)), dim3(kBlock), 0,
+                     (hipStream_t)stream, n_markers, g2, g_pred, g_virt);
+  return check_launch("dt_marker_offset_adj");
 }
 
 static MarkerArgs marker_args(const float* inv_trans16, const float* fisheye5) {
@@ -264,17 +471,20 @@ int dt_markers_fwd(int32_t n_surface, const int32_t* surface_id, const float* po
 
 int dt_markers_adj(int32_t n_surface, const int32_t* surface_id, const float* pos,
                    const float* inv_trans16, const float* fisheye5, int32_t n_markers,
-                   int32_t k_closest, const int32_t* interp_idx, const float* interp_w,
-                   const float* g_pred, float* g_surf_loc, float* g_pos, void* stream) {
+                   int32_t k_closest, const int32_t* surf_start, const int32_t* surf_entry,
+                   const float* interp_w, const float* g_pred, float* g_surf_loc, float* g_pos,
+                   void* stream) {
   DT_REQUIRE(n_surface >= 0 && n_markers >= 0 && k_closest > 0, "bad size");
   DT_REQUIRE(inv_trans16 && fisheye5, "NULL host parameter");
-  DT_REQUIRE(surface_id && pos && interp_idx && interp_w && g_pred && g_surf_loc && g_pos,
+  DT_REQUIRE(surface_id && pos && surf_start && surf_entry && interp_w && g_pred && g_surf_loc &&
+                 g_pos,
              "NULL pointer");
   MarkerArgs A = marker_args(inv_trans16, fisheye5);
   hipStream_t st = (hipStream_t)stream;
-  if (n_markers > 0)
-    hipLaunchKernelGGL(k_marker_interp_adj, dim3(nblocks(n_markers)), dim3(kBlock), 0, st,
-                       n_markers, k_closest, interp_idx, interp_w, g_pred, g_surf_loc);
+  if (n_surface > 0)
+    hipLaunchKernelGGL(k_marker_interp_adj, dim3(nblocks(n_surface)), dim3(kBlock), 0, st,
+                       n_surface, k_closest, surf_start, surf_entry, interp_w, g_pred,
+                       g_surf_loc);
   if (n_surface > 0)
     hipLaunchKernelGGL(k_surface_project_adj, dim3(nblocks(n_surface)), dim3(kBlock), 0, st, A,
                        n_surface, surface_id, pos, g_surf_loc, g_pos);

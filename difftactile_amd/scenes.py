@@ -334,6 +334,7 @@ def config5_gripper(n_particles=2_000_000, n_grid=256, dt=1e-5, n_substeps=1000,
                 mat_id=mat_id, pads=[(pd["nodes"].astype(np.float32), f2v, tris, layer,
                                       pd["out_dir"]) for pd in pads],
                 pad_mu=mu, pad_lam=lam, pad_mass=mass, inv_T=inv_T,
+                pad_inv_rot=[pd["R"].T.copy() for pd in pads],
                 gripper=dict(rot=(0.0, 0.0, 0.0), trans=ee_t, grip_width=grip_width, dt=dt),
                 controls=controls,
                 rope=dict(n=n_rope, p_rad=r_rad, p_rho=r_rho, rest_length=rest, start=rope_start,

@@ -575,11 +575,39 @@ int dt_contact_collision_adj_fx(const dt_mpm_params* prm, const dt_contact_param
 
 /* ---- losses on the measured path ---- */
 
-/* compute_obj_pos, tasks/grasp_elastic.py:240-243: out3[c] += sum_p x[p][c]
-   (float64, device; caller zeroes it and divides by n).  soa: x layout. */
-int dt_loss_sum_positions(int32_t n, int32_t soa, const float* x, double* out3, void* stream);
-/* adjoint of the mean position: gx[p] += g_com3 / n  (g_com3: HOST float[3]) */
+/* Reductions here are float32 in a fixed order (per-block partials in a
+   caller scratch of DT_REDUCE_BLOCKS x terms floats, then one block over
+   them), so every result repeats bit for bit. */
+/* compute_obj_pos, tasks/grasp_elastic.py:240-243: out3 = mean_p x[p] (device
+   float[3]); scratch 3 DT_REDUCE_BLOCKS floats; soa: x layout. */
+int dt_loss_com(int32_t n, int32_t soa, const float* x, float* scratch, float* out3, void* stream);
+/* adjoint of the mean position: gx[p] += g_com3 / n (g_com3: device float[3]) */
 int dt_loss_com_adj(int32_t n, int32_t soa, const float* g_com3, float* gx, void* stream);
+/* compute_contact_force, tasks/grasp_plastic.py:218-230: out3 = 1/6 sum over
+   the pad's contact triangles of the external force at their three vertices
+   (device float[3]); scratch 3 DT_REDUCE_BLOCKS floats. */
+int dt_contact_force_sum(int32_t n_tris, const int32_t* tris, const float* ext, float* scratch,
+                         float* out3, void* stream);
+/* adjoint: g_ext[v] += (triangles holding v) / 6 * g3 (g3: device float[3]) */
+int dt_contact_force_sum_adj(int32_t n_tris, const int32_t* tris, const float* g3, float* g_ext,
+                             void* stream);
+/* compute_force_loss, tasks/grasp_plastic.py:232-238, over n_pads (1..8) pads:
+   p_k = inv_rot_k cf_k (inv_rot9: HOST row-major 3x3 per pad, the sensors'
+   inv_rot; cf: device n_pads x 3); loss = beta sum_k |p_k.x| / |mu p_k.y| when
+   every |p_k.x| > 1e-3, else 0.  out (device, 1 + 3 n_pads floats): out[0] =
+   loss, out[1 + 3k + c] = d loss / d cf_k[c]. */
+int dt_force_loss(int32_t n_pads, const float* cf, const float* inv_rot9, float mu, float beta,
+                  float* out, void* stream);
+/* compute_marker_motion, tasks/grasp_plastic.py:240-246: out2 = mean over the
+   markers of pred - virt (device float[2]); scratch 2 DT_REDUCE_BLOCKS floats.
+   The reference's loop divides both sensors' sums by sensor 1's marker count;
+   here each pad's by its own (the same for pads of one mesh). */
+int dt_marker_offset(int32_t n_markers, const float* pred, const float* virt, float* scratch,
+                     float* out2, void* stream);
+/* adjoint: g_pred[m] += g2 / n_markers; g_virt[m] -= g2 / n_markers (g_virt may
+   be NULL; g2: device float[2]) */
+int dt_marker_offset_adj(int32_t n_markers, const float* g2, float* g_pred, float* g_virt,
+                         void* stream);
 /* extract_markers, sensor_model/fem_sensor.py:161-187 with project_3d_2d,
    fisheye_model.py:13-30: surface vertices (surface_id) of pos / vpos mapped
    by inv_trans16 (HOST row-major 4x4), reordered (x, z, y), projected with
@@ -591,12 +619,15 @@ int dt_markers_fwd(int32_t n_surface, const int32_t* surface_id, const float* po
                    int32_t n_markers, int32_t k_closest, const int32_t* interp_idx,
                    const float* interp_w, float* surf_loc, float* surf_vloc, float* pred,
                    float* virt, void* stream);
-/* adjoint of pred w.r.t. pos: g_surf_loc (scratch [n_surface][2], zeroed by the
-   caller) and g_pos accumulate. */
+/* adjoint of pred w.r.t. pos: g_surf_loc (scratch [n_surface][2]) is written,
+   g_pos accumulates.  The interpolation's adjoint gathers per surface point:
+   surf_entry[surf_start[s] .. surf_start[s+1]) lists m k_closest + j for every
+   interp_idx[m][j] == s (surface ids distinct). */
 int dt_markers_adj(int32_t n_surface, const int32_t* surface_id, const float* pos,
                    const float* inv_trans16, const float* fisheye5, int32_t n_markers,
-                   int32_t k_closest, const int32_t* interp_idx, const float* interp_w,
-                   const float* g_pred, float* g_surf_loc, float* g_pos, void* stream);
+                   int32_t k_closest, const int32_t* surf_start, const int32_t* surf_entry,
+                   const float* interp_w, const float* g_pred, float* g_surf_loc, float* g_pos,
+                   void* stream);
 
 /* ---- parallel-jaw kinematics and pad control velocities ----
    (sensor_model/gripper_kinematics.py).  A trajectory's frames are six

@@ -3,12 +3,14 @@ position-based cable and an elastoplastic object in one differentiable
 rollout (coupled.CoupledSim with a cable).
 
 * Scaled (6,000 particles, 32^3, 12 x 12 x 3 pads, a 101-particle cable, 6
-  substeps): object, pad and cable states against the FP64 oracle
-  composition (oracle.gripper_scene), and the gradient of a random linear
-  functional of the final object, pad and cable positions with respect to the
-  jaw controls (through Gripper.control_velocities), the initial object
-  positions and the initial cable velocities against the oracle's reverse
-  mode, with the adjoint clamp on.
+  substeps): object, pad and cable states and the last substep's pad
+  external forces against the FP64 oracle composition
+  (oracle.gripper_scene); the gradients of a random linear functional of the
+  final object, pad and cable positions and of the grasp task's pose + force
+  loss (centre-of-mass error, sliding ratio of the pads' contact forces) with
+  respect to the jaw controls (through Gripper.control_velocities), the
+  initial object positions and the initial cable velocities against the
+  oracle's reverse mode, with the adjoint clamp on.
 * Full size (2M particles, 256^3, 40 x 40 x 6 pads, a 2,001-particle cable):
   over a few substeps, the contact impulses on the object and the cable
   balance the pads' contact forces, the scatter conserves mass, and the
@@ -20,11 +22,13 @@ import torch
 
 from difftactile_amd import _lib, scenes
 from difftactile_amd.coupled import CoupledSim, Pad
+from difftactile_amd.losses import com_loss, contact_force, force_loss
 from difftactile_amd.object_model.pbd_rope import PBDRope
 from difftactile_amd.sensor_model.gripper_kinematics import Gripper
 from difftactile_amd.sim import MPMSim, workspace_bytes
 from oracle import gripper_scene as ogs
 from oracle import kinematics as ok
+from oracle import markers as ok_m
 from tests.gripper_cases import oracle_B, oracle_pads, oracle_rope_params, small_scene
 from tests.mpm_cases import oracle_mats, oracle_params, rel_l2
 
@@ -89,6 +93,19 @@ def test_c5_scaled_gripper_matches_the_oracle(gpu):
     loss = (objo[0] * f32(wx)).sum() + sum((p[0] * f32(w)).sum() for p, w in zip(pado, wp)) + \
         (ropeo[0] * f32(wr)).sum()
     g_ctrl, g_x, g_rv, g_cv1, g_cv2 = torch.autograd.grad(loss, [ctrl, x0, rvl, cv1, cv2])
+    # the grasp task's pose and force terms: centre-of-mass error and the
+    # sliding ratio of the last substep's pad contact forces (a second rollout:
+    # a rollout's backward runs once)
+    cv1, cv2 = grip.control_velocities(ctrl, *[torch.tensor(x, device=dev) for x in init_x])
+    outf = cs.rollout(x0, v0, C0, F0, pst, [cv1, cv2], steps, rope_state=(rp0.clone(), rv0),
+                      return_ext=True)
+    exto = outf[6]
+    tgt = torch.tensor(c["target_com"], dtype=torch.float32, device=dev)
+    mu_f = c["contact"]["mu_f"]
+    tris_d = [torch.tensor(np.asarray(p[2]), dtype=torch.int32, device=dev) for p in c["pads"]]
+    loss_f = ((com_loss(outf[0]) - tgt) ** 2).sum() + \
+        force_loss([contact_force(t, e) for t, e in zip(tris_d, exto)], c["pad_inv_rot"], mu_f)
+    gf_ctrl, gf_x = torch.autograd.grad(loss_f, [ctrl, x0])
 
     def oracle(dtype):
         """The oracle's rollout from the same float32 inputs (control
@@ -108,6 +125,7 @@ def test_c5_scaled_gripper_matches_the_oracle(gpu):
         ps = [(pos[k], cvo[k][0].detach().clone(), pos[k].clone()) for k in range(2)]
         rs = (rpos, rvo)
         g_contacts = r_contacts = 0
+        info = {}
         for f in range(steps):
             pd = oracle_pads(c, [p[0] for p in ps], [p[1] for p in ps], [p[2] for p in ps], cvo, f,
                              B, dtype)
@@ -119,9 +137,9 @@ def test_c5_scaled_gripper_matches_the_oracle(gpu):
             g_contacts = max(g_contacts, int((info["grid_f"].abs().sum(1) > 0).sum()))
             r_contacts = max(r_contacts, int((info["impulse"].abs().sum(1) > 0).sum()))
             ps = [tuple(p) for p in po]
-        return state, ps, rs, (so, cvo, rvo), (g_contacts, r_contacts)
+        return state, ps, rs, (so, cvo, rvo), (g_contacts, r_contacts), info["ext"]
 
-    state, ps, rs, (so, cvo, rvo), (g_contacts, r_contacts) = oracle(F64)
+    state, ps, rs, (so, cvo, rvo), (g_contacts, r_contacts), ext64 = oracle(F64)
     assert g_contacts > 20 and r_contacts > 5, (g_contacts, r_contacts)
     for got, want in zip(objo, state):
         assert rel_l2(got.detach().cpu(), want.detach()) < 1e-4
@@ -133,13 +151,15 @@ def test_c5_scaled_gripper_matches_the_oracle(gpu):
     # and |p| ~ 5 one position ulp is 0.05 of velocity, so any float32
     # evaluation of it is rounding-bound; it is held to three times the
     # float32 oracle's distance from float64
-    rs32 = oracle(torch.float32)[2]
+    o32 = oracle(torch.float32)
+    rs32, ext32 = o32[2], o32[5]
     tol = 3 * rel_l2(rs32[1].detach().double(), rs[1].detach())
     assert rel_l2(ropeo[1].detach().cpu(), rs[1].detach()) <= max(tol, 1e-4), tol
     lo = (state[0] * torch.tensor(wx)).sum() + sum((p[0] * torch.tensor(w)).sum()
                                                    for p, w in zip(ps, wp)) + \
         (rs[0] * torch.tensor(wr)).sum()
-    go_cv1, go_cv2, go_x, go_rv = torch.autograd.grad(lo, [cvo[0], cvo[1], so[0], rvo])
+    go_cv1, go_cv2, go_x, go_rv = torch.autograd.grad(lo, [cvo[0], cvo[1], so[0], rvo],
+                                                      retain_graph=True)
     for got, want in ((g_cv1, go_cv1), (g_cv2, go_cv2), (g_x, go_x), (g_rv, go_rv)):
         assert float(want.abs().max()) > 0
         assert rel_l2(got.cpu().double(), want) < 1e-3
@@ -149,9 +169,31 @@ def test_c5_scaled_gripper_matches_the_oracle(gpu):
     W, f1t, f2t, frame0 = ok.init_frames(g["rot"], g["trans"], g["grip_width"])
     fo = ok.chain(W, f1t, f2t, frame0, co, prm.dt)
     cvs = [ok.control_vel(fo, k + 1, torch.tensor(init_x[k], dtype=F64), prm.dt) for k in range(2)]
-    want = torch.autograd.grad(cvs, co, grad_outputs=[go_cv1, go_cv2])[0]
+    want = torch.autograd.grad(cvs, co, grad_outputs=[go_cv1, go_cv2], retain_graph=True)[0]
     assert float(want.abs().max()) > 0
     assert rel_l2(g_ctrl.cpu().double(), want) < 1e-3
+    # pose + force loss.  The cable's pad force carries its rounding-bound
+    # velocity (kd n.v), and the force loss is the ratio of the pads' small
+    # tangential force sum to the normal one: both are held to three times
+    # the float32 oracle's distance from float64
+    for got, want_e, e32 in zip(exto, ext64, ext32):
+        tol = 3 * rel_l2(e32.detach().double(), want_e.detach())
+        assert rel_l2(got.detach().cpu(), want_e.detach()) <= max(tol, 1e-4), tol
+
+    def pose_force(st0, ext):
+        cf = [ok_m.contact_force(e, p[2]) for e, p in zip(ext, c["pads"])]
+        return ((st0.mean(0) - torch.tensor(c["target_com"]).to(st0.dtype)) ** 2).sum() + \
+            ok_m.force_loss(cf, c["pad_inv_rot"], mu_f)
+
+    lof = pose_force(state[0], ext64)
+    lof32 = pose_force(o32[0][0], ext32)
+    tol = 3 * abs(float(lof32.detach()) - float(lof.detach()))
+    assert abs(float(loss_f.detach()) - float(lof.detach())) <= max(tol, 1e-4 * abs(float(lof)))
+    gof_cv1, gof_cv2, gof_x = torch.autograd.grad(lof, [cvo[0], cvo[1], so[0]])
+    wantf = torch.autograd.grad(cvs, co, grad_outputs=[gof_cv1, gof_cv2])[0]
+    assert float(wantf.abs().max()) > 0
+    assert rel_l2(gf_ctrl.cpu().double(), wantf) < 1e-3
+    assert rel_l2(gf_x.cpu().double(), gof_x) < 1e-3
 
 
 def test_c5_full_size_contact_balance_mass_and_gradient(gpu):

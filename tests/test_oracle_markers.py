@@ -60,3 +60,38 @@ def test_marker_loss_reverse_mode_matches_fd():
     with torch.no_grad():
         fd = (L(pos + h * d) - L(pos - h * d)) / (2 * h)
     assert abs((g * d).sum().item() - fd.item()) <= 1e-6 * max(1.0, abs(fd.item()))
+
+
+def test_contact_force_closed_forms():
+    rng = np.random.default_rng(4)
+    tris = np.array([[0, 1, 2], [1, 2, 3], [2, 3, 4], [0, 2, 4]])
+    e = torch.tensor([1.5, -2.0, 0.25], dtype=torch.float64)
+    ext = e.repeat(5, 1)
+    assert torch.allclose(ok.contact_force(ext, tris), len(tris) / 2 * e, rtol=0, atol=1e-15)
+    # one loaded vertex counts once per triangle holding it, a sixth each
+    ext = torch.zeros(5, 3, dtype=torch.float64)
+    ext[2] = torch.tensor(rng.normal(size=3))
+    assert torch.allclose(ok.contact_force(ext, tris), 4 / 6 * ext[2], rtol=0, atol=1e-15)
+
+
+def test_force_loss_closed_forms():
+    I3 = np.eye(3)
+    q = np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])  # quarter turn about z
+    mu, beta = 0.5, 2.0
+    c1 = torch.tensor([3.0, 4.0, 0.0], dtype=torch.float64, requires_grad=True)
+    c2 = torch.tensor([-1.0, 2.0, 0.0], dtype=torch.float64, requires_grad=True)
+    loss = ok.force_loss([c1, c2], [I3, I3], mu, beta)
+    assert abs(float(loss) - beta * (3 / (mu * 4) + 1 / (mu * 2))) < 1e-15
+    g1, g2 = torch.autograd.grad(loss, [c1, c2])
+    assert torch.allclose(g1, torch.tensor([beta / (mu * 4), -beta * 3 / (mu * 16), 0.0],
+                                           dtype=torch.float64))
+    assert torch.allclose(g2, torch.tensor([-beta / (mu * 2), -beta * 1 / (mu * 4), 0.0],
+                                           dtype=torch.float64))
+    # the sensor frame: cf = q^T p gives the same loss with inv_rot = q
+    r1 = torch.tensor(q.T @ [3.0, 4.0, 0.0])
+    r2 = torch.tensor(q.T @ [-1.0, 2.0, 0.0])
+    assert abs(float(ok.force_loss([r1, r2], [q, q], mu, beta)) - float(loss)) < 1e-14
+    # either pad below the 1e-3 shear gate: no loss
+    small = torch.tensor([5e-4, 4.0, 0.0], dtype=torch.float64)
+    assert float(ok.force_loss([small, c2], [I3, I3], mu, beta)) == 0.0
+    assert float(ok.force_loss([c1, small], [I3, I3], mu, beta)) == 0.0
