@@ -41,6 +41,11 @@ class DtJawParams(ctypes.Structure):
 
 # floats per frame of the jaw chain (DT_JAW_FRAME_FLOATS)
 JAW_FRAME_FLOATS = 96
+# dome pad pose state (DT_DOME_*): field offsets and size
+DOME_POSE = dict(trans_world=(0, (4, 4)), trans_local=(16, (4, 4)), trans_h=(32, (4, 4)),
+                 inv_trans_h=(48, (4, 4)), dtrans_h=(64, (4, 4)), rot_world=(80, (3, 3)),
+                 rot_local=(89, (3, 3)), rot_h=(98, (3, 3)), inv_rot=(107, (3, 3)))
+DOME_POSE_FLOATS = 116
 # blocks of the fixed-order reductions (DT_REDUCE_BLOCKS)
 REDUCE_BLOCKS = 256
 # floats of the Kabsch-angle state (DT_KABSCH_STATE); the angle is entry 47
@@ -156,6 +161,10 @@ _SIGS = {
     "dt_jaw_chain_adj": (_c_int, [_P(DtJawParams), _c_int, _vp, _vp, _vp, _vp, _vp]),
     "dt_jaw_control_vel": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _c_float, _vp, _vp]),
     "dt_jaw_control_vel_adj": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_float, _vp, _vp, _vp]),
+    "dt_dome_pose_control": (_c_int, [_c_int, _vp, _c_float, _vp, _vp, _vp]),
+    "dt_dome_pose_control_adj": (_c_int, [_c_int, _vp, _c_float, _vp, _vp, _vp, _vp, _vp]),
+    "dt_dome_control_vel": (_c_int, [_c_int, _vp, _vp, _c_float, _vp, _vp]),
+    "dt_dome_control_vel_adj": (_c_int, [_c_int, _vp, _vp, _vp, _c_float, _vp, _vp, _vp, _vp]),
     "dt_mpm_grid_mass": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp]),
     "dt_mpm_grid_mass_adj": (_c_int, [_P(DtMpmParams), _c_int, _vp, _vp, _vp, _vp]),
     "dt_deform_target_sdf": (_c_int, [_c_int, _c_float, _vp, _c_int, _vp, _vp, _vp, _vp, _vp]),

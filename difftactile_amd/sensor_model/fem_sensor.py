@@ -2,9 +2,12 @@
 
 Same field layout (``pos``, ``vel``, ``virtual_pos``, ``external_force_field``
 as ``[sub_steps, n_verts, 3]`` with ``.grad`` twins; ``B``, ``f2v``,
-``contact_seg``, ``layer_id``, ``control_vel``, ``out_direction``) and kernel
-methods (``init_pos``, ``update``/``update.grad``, ``update2``/``update2.grad``,
-``set_vel``, ``reset_contact``, window and cache helpers).  Kernels run in
+``contact_seg``, ``layer_id``, ``control_vel``, ``out_direction``, the pose
+fields ``d_pos``, ``d_ori``, ``trans_*``, ``rot_*``, ``inv_*``, ``dtrans_h``) and
+kernel methods (``init_pos``, ``update``/``update.grad``,
+``update2``/``update2.grad``, ``set_pose_control``, ``set_pose_control_bp``,
+``set_trans_h``, ``set_control_vel`` with their ``.grad``, ``set_vel``,
+``reset_contact``, window and cache helpers).  Kernels run in
 libdifftactile_hip.so.  The camera/marker model (``init_cam_model``, which
 needs OpenCV blob detection on ``init.png``) is not part of this package.
 """
@@ -74,16 +77,31 @@ class FEMDomeSensor:
         self.external_force_field = Field((S, V), (3,), device=d, needs_grad=True)
         self.surf_f = Field(S, (3,), device=d, needs_grad=True)
         self.out_direction = Field((), (3,), device=d)
-        self.rot_h = Field((), (3, 3), device=d)
-        self.trans_h = Field((), (4, 4), device=d)
-        self.inv_rot = Field((), (3, 3), device=d)
-        self.inv_trans_h = Field((), (4, 4), device=d)
+        # pose control (fem_sensor.py:93-107): d_pos / d_ori and the pose fields
+        self.d_pos = Field((), (3,), device=d, needs_grad=True)
+        self.d_ori = Field((), (3,), device=d, needs_grad=True)
+        self.trans_world = Field((), (4, 4), device=d, needs_grad=True)
+        self.trans_local = Field((), (4, 4), device=d, needs_grad=True)
+        self.trans_h = Field((), (4, 4), device=d, needs_grad=True)
+        self.inv_trans_h = Field((), (4, 4), device=d, needs_grad=True)
+        self.dtrans_h = Field((), (4, 4), device=d, needs_grad=True)
+        self.rot_world = Field((), (3, 3), device=d, needs_grad=True)
+        self.rot_local = Field((), (3, 3), device=d, needs_grad=True)
+        self.rot_h = Field((), (3, 3), device=d, needs_grad=True)
+        self.inv_rot = Field((), (3, 3), device=d, needs_grad=True)
+        self._pose_before = {}
         cv_shape = (S, V) if self.GRIPPER else V
         self.control_vel = Field(cv_shape, (3,), device=d, needs_grad=True)
         self.cache = dict()
         self._centroids = torch.zeros(self.num_triangles, 4, device=d)
         self.update = Kernel(self._update, self._update_grad)
         self.update2 = Kernel(self._update2, self._update2_grad)
+        self.set_pose_control = Kernel(lambda: self._pose_step(0),
+                                       lambda: self._pose_step_grad(0))
+        self.set_pose_control_bp = Kernel(lambda: self._pose_step(2),
+                                          lambda: self._pose_step_grad(2))
+        self.set_trans_h = Kernel(self._set_trans_h, self._set_trans_h_grad)
+        self.set_control_vel = Kernel(self._set_control_vel, self._set_control_vel_grad)
 
     # -- parameters --------------------------------------------------------
     def set_material_params(self, E, nu):
@@ -110,13 +128,96 @@ class FEMDomeSensor:
         self.init_matrix(T)
 
     def init_matrix(self, T):
-        """Pose from a 4x4 matrix (``gripper_fem.py`` ``init(matrix)``)."""
+        """Pose from a 4x4 matrix (``gripper_fem.py`` ``init(matrix)``; the
+        world pose of ``init``, fem_sensor.py:112-128, with identity local
+        parts)."""
         T = np.asarray(T, dtype=np.float64)
+        self.trans_world[None] = T
+        self.trans_local[None] = np.eye(4)
         self.trans_h[None] = T
         self.inv_trans_h[None] = np.linalg.inv(T)
+        self.dtrans_h[None] = np.eye(4)
+        self.rot_world[None] = T[:3, :3]
+        self.rot_local[None] = np.eye(3)
         self.rot_h[None] = T[:3, :3]
         self.inv_rot[None] = np.linalg.inv(T[:3, :3])
         self.init_pos()
+
+    # -- pose control (fem_sensor.py:196-266) ------------------------------
+    def _pose_pack(self, grad=False):
+        parts = []
+        for name in _lib.DOME_POSE:
+            f = getattr(self, name)
+            parts.append((f.grad if grad else f).data.reshape(-1))
+        return torch.cat(parts).contiguous()
+
+    def _pose_unpack(self, buf, grad=False):
+        for name, (off, shape) in _lib.DOME_POSE.items():
+            f = getattr(self, name)
+            (f.grad if grad else f).data.copy_(buf[off:off + shape[0] * shape[1]].view(shape))
+
+    def _step_time(self):
+        return self.dt * (self.sub_steps - 1)
+
+    def _d6(self, grad=False):
+        if grad:
+            return torch.cat([self.d_pos.grad.data, self.d_ori.grad.data]).contiguous()
+        return torch.cat([self.d_pos.data, self.d_ori.data]).contiguous()
+
+    def _pose_call(self, mode, d6):
+        pin = self._pose_pack()
+        self._pose_before[mode] = (pin, d6)
+        pout = torch.empty_like(pin)
+        _lib.call("dt_dome_pose_control", mode, d6, self._step_time(), pin, pout, _lib.stream())
+        self._pose_unpack(pout)
+
+    def _pose_call_grad(self, mode):
+        pin, d6 = self._pose_before[mode]
+        g = self._pose_pack(grad=True)
+        g6 = torch.zeros(6, device=self.device)
+        _lib.call("dt_dome_pose_control_adj", mode, d6, self._step_time(), pin, g, g, g6,
+                  _lib.stream())
+        self._pose_unpack(g, grad=True)
+        return g6
+
+    def _pose_step(self, mode):
+        """``set_pose_control`` (mode 0, local frame, 229-245) /
+        ``set_pose_control_bp`` (mode 2, 248-256) with the current d_pos, d_ori."""
+        self._pose_call(mode, self._d6())
+
+    def _pose_step_grad(self, mode):
+        g6 = self._pose_call_grad(mode)
+        self.d_pos.grad.data += g6[:3]
+        self.d_ori.grad.data += g6[3:]
+
+    def _set_trans_h(self, d_pos, d_ori):
+        """``set_trans_h`` (217-227): the world-frame pose step."""
+        d6 = torch.tensor(np.concatenate([np.asarray(d_pos, np.float32),
+                                          np.asarray(d_ori, np.float32)]), device=self.device)
+        self._pose_call(1, d6)
+
+    def _set_trans_h_grad(self, d_pos=None, d_ori=None):
+        """Adjoint of set_trans_h: the pose fields' gradients; returns
+        d/d(d_pos, d_ori) as a device (6,) (they are kernel arguments)."""
+        return self._pose_call_grad(1)
+
+    def _set_control_vel(self, f):
+        """``set_control_vel`` (258-266): control_vel from dtrans_h and
+        virtual_pos[f]."""
+        pose = self._pose_pack()
+        cv = self.control_vel.data[f] if self.GRIPPER else self.control_vel.data
+        _lib.call("dt_dome_control_vel", self.n_verts, self.virtual_pos.data[f], pose,
+                  self._step_time(), cv, _lib.stream())
+
+    def _set_control_vel_grad(self, f):
+        pose = self._pose_pack()
+        gpose = self._pose_pack(grad=True)
+        gcv = self.control_vel.grad.data[f] if self.GRIPPER else self.control_vel.grad.data
+        scratch = torch.empty(12 * _lib.REDUCE_BLOCKS, device=self.device)
+        _lib.call("dt_dome_control_vel_adj", self.n_verts, self.virtual_pos.data[f], pose,
+                  gcv.contiguous(), self._step_time(), scratch, gpose,
+                  self.virtual_pos.grad.data[f], _lib.stream())
+        self._pose_unpack(gpose, grad=True)
 
     def init_pos(self):
         """``init_pos`` (``fem_sensor.py:324-340``): world rest positions, B, out_direction."""
@@ -224,7 +325,10 @@ class FEMDomeSensor:
             getattr(self, k).data[0].copy_(c[k])
 
     def clear_loss_grad(self):
-        for fld in (self.E_init, self.nu_init, self.mu, self.lam, self.control_vel):
+        """``clear_loss_grad`` (fem_sensor.py:530-546)."""
+        for fld in (self.E_init, self.nu_init, self.mu, self.lam, self.control_vel, self.d_pos,
+                    self.d_ori, self.rot_h, self.rot_local, self.inv_rot, self.trans_h,
+                    self.trans_local, self.inv_trans_h, self.dtrans_h):
             fld.grad.fill(0.0)
 
     def clear_step_grad(self, f):

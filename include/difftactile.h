@@ -661,6 +661,48 @@ int dt_jaw_control_vel(int32_t n_steps, int32_t n_verts, int32_t finger, const f
 int dt_jaw_control_vel_adj(int32_t n_steps, int32_t n_verts, int32_t finger, const float* init_x,
                            float dt, const float* g_control_vel, float* g_frames, void* stream);
 
+/* ---- dome pad pose control (sensor_model/fem_sensor.py:196-266) ----
+   A pose is DT_DOME_POSE_FLOATS floats of device memory holding the
+   FEMDomeSensor fields below (row-major).  d6 (device) = (d_pos xyz, d_ori xyz
+   in degrees per unit time); step_time = dt (sub_steps - 1), the span one
+   call covers.  The step transform is eul2mat (196-210): R = Rz Ry Rx of
+   d_ori step_time, t = d_pos step_time. */
+#define DT_DOME_TRANS_WORLD 0    /* [16] pad -> world at init (fixed) */
+#define DT_DOME_TRANS_LOCAL 16   /* [16] accumulated local steps      */
+#define DT_DOME_TRANS_H 32       /* [16] trans_world trans_local      */
+#define DT_DOME_INV_TRANS_H 48   /* [16]                              */
+#define DT_DOME_DTRANS_H 64      /* [16] W T W^-1 of the last step    */
+#define DT_DOME_ROT_WORLD 80     /* [9]                               */
+#define DT_DOME_ROT_LOCAL 89     /* [9]                               */
+#define DT_DOME_ROT_H 98         /* [9]                               */
+#define DT_DOME_INV_ROT 107      /* [9]                               */
+#define DT_DOME_POSE_FLOATS 116
+/* mode 0: set_pose_control (229-245): dtrans_h = W T W^-1, trans_local =
+   T trans_local, trans_h = W trans_local, rot_local = R rot_local, rot_h =
+   rot_world rot_local; mode 1: set_trans_h (217-227): trans_h = T trans_h,
+   rot_h = R rot_h; mode 2: set_pose_control_bp (248-256): dtrans_h only.
+   Every mode then sets inv_trans_h and inv_rot (general inverses).
+   pose_out may be pose_in. */
+int dt_dome_pose_control(int32_t mode, const float* d6, float step_time, const float* pose_in,
+                         float* pose_out, void* stream);
+/* adjoint (set_pose_control.grad): g_pose_out holds d/d(every field of the
+   output pose); g_pose_in receives the gradient of the input pose (the
+   fields the mode overwrites get 0, the ones it reads their gradient; it may
+   be g_pose_out), g_d6 accumulates d/d(d_pos, d_ori). */
+int dt_dome_pose_control_adj(int32_t mode, const float* d6, float step_time, const float* pose_in,
+                             const float* g_pose_out, float* g_pose_in, float* g_d6,
+                             void* stream);
+/* set_control_vel (258-266): control_vel[i] = ((dtrans_h [vpos_i; 1]) -
+   vpos_i)_{0:3} / step_time. */
+int dt_dome_control_vel(int32_t n_verts, const float* vpos, const float* pose, float step_time,
+                        float* control_vel, void* stream);
+/* adjoint: g_pose's dtrans_h accumulates sum_i g_cv_i [vpos_i; 1]^T / step_time
+   (fixed order; scratch 12 DT_REDUCE_BLOCKS floats); g_vpos (may be NULL)
+   accumulates (dtrans_h^T g_cv_i - g_cv_i) / step_time. */
+int dt_dome_control_vel_adj(int32_t n_verts, const float* vpos, const float* pose,
+                            const float* g_control_vel, float step_time, float* scratch,
+                            float* g_pose, float* g_vpos, void* stream);
+
 /* ---- grid-mass deformation losses (tasks/deform_loss.py) ---- */
 /* Mass-only p2g (compute_grid_m_kernel, mpm_plastic.py:168-178): grid_m
    [n^3] += sum_p w(node) p_mass over x [n][3]; one environment. */

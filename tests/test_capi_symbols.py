@@ -68,3 +68,17 @@ def test_jaw_frame_size_matches_header():
     m = re.search(r"#define\s+DT_JAW_FRAME_FLOATS\s+(\d+)", src)
     assert m and int(m.group(1)) == _lib.JAW_FRAME_FLOATS == 6 * 16
     assert ctypes.sizeof(_lib.DtJawParams) == 4 * (16 + 32 + 1)
+
+
+def test_dome_pose_layout_matches_header():
+    src = open(os.path.join(ROOT, "include", "difftactile.h")).read()
+    names = dict(trans_world="TRANS_WORLD", trans_local="TRANS_LOCAL", trans_h="TRANS_H",
+                 inv_trans_h="INV_TRANS_H", dtrans_h="DTRANS_H", rot_world="ROT_WORLD",
+                 rot_local="ROT_LOCAL", rot_h="ROT_H", inv_rot="INV_ROT")
+    end = 0
+    for field, (off, shape) in _lib.DOME_POSE.items():
+        m = re.search(r"#define\s+DT_DOME_%s\s+(\d+)" % names[field], src)
+        assert m and int(m.group(1)) == off == end, field
+        end = off + shape[0] * shape[1]
+    m = re.search(r"#define\s+DT_DOME_POSE_FLOATS\s+(\d+)", src)
+    assert m and int(m.group(1)) == _lib.DOME_POSE_FLOATS == end

@@ -133,3 +133,70 @@ def test_jaw_controls_drive_a_coupled_rollout_differentiably(gpu):
     cvo = ok.control_vel(fo, 1, torch.tensor(init_x1, dtype=D64), prm.dt)
     want = torch.autograd.grad(cvo, co, grad_outputs=g_cv)[0]
     assert rel_l2(got, want) < 1e-3
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_dome_pose_control_and_adjoint_match_the_oracle(gpu, mode):
+    """The dome pad's pose step (mode 0 set_pose_control, 1 set_trans_h, 2
+    set_pose_control_bp) after a first local step, then set_control_vel: every
+    pose field and the control velocities against the FP64 oracle; the
+    adjoint of a random linear functional of the control velocities and of
+    the pose outputs w.r.t. d_pos, d_ori, the input pose and the virtual
+    positions against the oracle's reverse mode."""
+    from difftactile_amd.sensor_model.fem_sensor import FEMDomeSensor
+    from oracle import dome_pose as odp
+    sen = FEMDomeSensor(dt=5e-5, sub_steps=50)
+    sen.init(20.0, -10.0, 35.0, 0.1, 0.2, -0.3)
+    st = sen.dt * (sen.sub_steps - 1)
+    rng = np.random.default_rng(31 + mode)
+    a1, b1 = rng.normal(0, 0.5, 3), rng.normal(0, 30.0, 3)
+    a2, b2 = rng.normal(0, 0.5, 3), rng.normal(0, 30.0, 3)
+    # a first local step, then the step under test
+    sen.d_pos[None], sen.d_ori[None] = a1, b1
+    sen.set_pose_control()
+    if mode == 1:
+        sen.set_trans_h(a2, b2)
+    else:
+        sen.d_pos[None], sen.d_ori[None] = a2, b2
+        sen.set_pose_control() if mode == 0 else sen.set_pose_control_bp()
+    sen.set_control_vel(0)
+    torch.cuda.synchronize()
+    W = torch.tensor(sen.trans_world.to_numpy(), dtype=D64)
+    t = lambda v: torch.tensor(np.asarray(v, np.float32), dtype=D64)
+    p1 = odp.pose_step(odp.init_pose(W), t(a1), t(b1), st, 0)
+    pin = {k: v.clone().requires_grad_(True) for k, v in p1.items()}
+    a2t, b2t = t(a2).requires_grad_(True), t(b2).requires_grad_(True)
+    p2 = odp.pose_step(pin, a2t, b2t, st, mode)
+    vp = torch.tensor(sen.virtual_pos.to_numpy()[0], dtype=D64, requires_grad=True)
+    cv = odp.control_vel(vp, p2["dtrans_h"], st)
+    for k in odp.POSE_FIELDS:
+        got = torch.tensor(getattr(sen, k).to_numpy(), dtype=D64)
+        assert rel_l2(got, p2[k].detach()) < 1e-5, k
+    assert rel_l2(torch.tensor(sen.control_vel.to_numpy(), dtype=D64), cv.detach()) < 1e-4
+    # adjoint
+    gcv = rng.normal(size=cv.shape).astype(np.float32)
+    g_out = {k: rng.normal(size=tuple(p2[k].shape)).astype(np.float32)
+             for k in ("inv_trans_h", "inv_rot", "trans_h", "rot_h")}
+    sen.clear_loss_grad()
+    sen.virtual_pos.grad.fill(0.0)
+    sen.control_vel.grad.from_numpy(gcv)
+    for k, g in g_out.items():
+        getattr(sen, k).grad.from_numpy(g)
+    sen.set_control_vel.grad(0)
+    if mode == 1:
+        g6 = sen.set_trans_h.grad(a2, b2)
+        g_dpos, g_dori = g6[:3].cpu().double(), g6[3:].cpu().double()
+    else:
+        (sen.set_pose_control if mode == 0 else sen.set_pose_control_bp).grad()
+        g_dpos = torch.tensor(sen.d_pos.grad.to_numpy(), dtype=D64)
+        g_dori = torch.tensor(sen.d_ori.grad.to_numpy(), dtype=D64)
+    L = (cv * torch.tensor(gcv, dtype=D64)).sum() + sum(
+        (p2[k] * torch.tensor(g, dtype=D64)).sum() for k, g in g_out.items())
+    read = {0: ("trans_local", "rot_local"), 1: ("trans_h", "rot_h"), 2: ("trans_h", "rot_h")}[mode]
+    wrt = [a2t, b2t, vp] + [pin[k] for k in read]
+    want = torch.autograd.grad(L, wrt, allow_unused=True)
+    got = [g_dpos, g_dori, torch.tensor(sen.virtual_pos.grad.to_numpy()[0], dtype=D64)] + \
+        [torch.tensor(getattr(sen, k).grad.to_numpy(), dtype=D64) for k in read]
+    for name, g, w in zip(["d_pos", "d_ori", "virtual_pos"] + list(read), got, want):
+        assert float(w.abs().max()) > 0, name
+        assert rel_l2(g, w) < 1e-4, name

@@ -115,3 +115,79 @@ def test_adjoint_matches_central_differences():
         e[f, j] = h
         fd = (loss(c0 + e) - loss(c0 - e)) / (2 * h)
         assert abs(fd.item() - g[f, j].item()) <= 1e-5 * max(1.0, abs(fd.item())), (f, j)
+
+
+# ---- dome pad pose control -------------------------------------------------
+from oracle import dome_pose as odp  # noqa: E402
+
+
+def _rot(axis, deg):
+    t = torch.zeros(3, dtype=torch.float64)
+    t[axis] = deg
+    return odp.eul2mat(t, torch.zeros(3, dtype=torch.float64))[1]
+
+
+def test_dome_eul2mat_is_rz_ry_rx():
+    e = torch.eye(3, dtype=torch.float64)
+    # right-hand quarter turns: x -> y about z, z -> x about y, y -> z about x
+    assert torch.allclose(_rot(2, 90.0) @ e[0], e[1], atol=1e-15)
+    assert torch.allclose(_rot(1, 90.0) @ e[2], e[0], atol=1e-15)
+    assert torch.allclose(_rot(0, 90.0) @ e[1], e[2], atol=1e-15)
+    ang = torch.tensor([30.0, -50.0, 70.0], dtype=torch.float64)
+    T, R = odp.eul2mat(ang, torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64))
+    want = _rot(2, 70.0) @ _rot(1, -50.0) @ _rot(0, 30.0)
+    assert torch.allclose(R, want, atol=1e-15)
+    assert not torch.allclose(R, _rot(0, 30.0) @ _rot(1, -50.0) @ _rot(2, 70.0), atol=1e-3)
+    assert torch.allclose(R @ R.T, e, atol=1e-14) and abs(float(torch.det(R)) - 1) < 1e-14
+    assert torch.allclose(T[:3, 3], torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64))
+
+
+def _pad_pose(seed=0):
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(seed)
+    W = np.eye(4)
+    W[:3, :3] = Rotation.from_rotvec(np.deg2rad(rng.uniform(-40, 40, 3))).as_matrix()
+    W[:3, 3] = rng.uniform(-1, 1, 3)
+    return odp.init_pose(W), rng
+
+
+def test_dome_pose_control_closed_forms():
+    pose, rng = _pad_pose()
+    st = 5e-5 * 49
+    v = torch.tensor(rng.normal(size=(50, 3)))
+    z3 = torch.zeros(3, dtype=torch.float64)
+    # zero controls: the pose stays, the velocity is zero
+    p0 = odp.pose_step(pose, z3, z3, st)
+    for k in odp.POSE_FIELDS:
+        assert torch.allclose(p0[k], pose[k], atol=1e-14), k
+    assert torch.allclose(odp.control_vel(v, p0["dtrans_h"], st), torch.zeros(50, 3, dtype=D64),
+                          atol=1e-12)
+    # a pure translation (local frame): every vertex moves at R_world d_pos
+    d = torch.tensor([0.3, -0.2, 0.5], dtype=torch.float64)
+    p1 = odp.pose_step(pose, d, z3, st)
+    cv = odp.control_vel(v, p1["dtrans_h"], st)
+    assert torch.allclose(cv, (pose["rot_world"] @ d).expand(50, 3), atol=1e-10)
+    # a pure turn about the pad's local z by theta: rotation about the pad
+    # frame's z axis through its origin
+    w = 40.0
+    p2 = odp.pose_step(pose, z3, torch.tensor([0.0, 0.0, w], dtype=torch.float64), st)
+    th = math.radians(w * st)
+    W = pose["trans_world"]
+    Rz = torch.tensor([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0],
+                       [0, 0, 1]], dtype=torch.float64)
+    local = (v - W[:3, 3]) @ W[:3, :3]          # W^-1 v for a rigid W
+    moved = local @ Rz.T @ W[:3, :3].T + W[:3, 3]
+    assert torch.allclose(odp.control_vel(v, p2["dtrans_h"], st), (moved - v) / st, atol=1e-9)
+    # two local steps compose as T2 T1; the world step left-multiplies trans_h
+    a = torch.tensor([0.1, 0.2, -0.3], dtype=torch.float64)
+    b = torch.tensor([10.0, -20.0, 5.0], dtype=torch.float64)
+    q = odp.pose_step(odp.pose_step(pose, a, b, st), b * 0.01, a * 50, st)
+    T1 = odp.eul2mat(b * st, a * st)[0]
+    T2 = odp.eul2mat(a * 50 * st, b * 0.01 * st)[0]
+    assert torch.allclose(q["trans_local"], T2 @ T1, atol=1e-13)
+    assert torch.allclose(q["trans_h"], W @ T2 @ T1, atol=1e-13)
+    assert torch.allclose(q["inv_trans_h"] @ q["trans_h"], torch.eye(4, dtype=torch.float64),
+                          atol=1e-13)
+    qw = odp.pose_step(pose, a, b, st, mode=1)
+    assert torch.allclose(qw["trans_h"], T1 @ W, atol=1e-13)
+    assert torch.allclose(qw["rot_h"], T1[:3, :3] @ pose["rot_h"], atol=1e-13)
