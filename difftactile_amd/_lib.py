@@ -92,6 +92,8 @@ _SIGS = {
     "dt_timing_read": (_c_int, [_c_int, _P(ctypes.c_double), _P(ctypes.c_int64)]),
     "dt_graphs_enable": (_c_int, [_c_int]),
     "dt_graph_stats": (_c_int, [_P(ctypes.c_int64), _P(ctypes.c_int64)]),
+    "dt_mpm_init_object": (_c_int, [_c_int, _vp, _P(ctypes.c_float), _P(ctypes.c_float), _vp,
+                                    _vp, _vp, _vp]),
     "dt_mpm_compute_new_F": (_c_int, [_c_int, _vp, _vp, _vp, _c_float, _vp]),
     "dt_mpm_compute_new_F_adj": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_float, _vp]),
     "dt_mpm_svd": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),

@@ -345,6 +345,25 @@ __device__ __forceinline__ void p2g_block(const MpmP& P, bool active, int n, con
 // ----------------------------------------------------------------------------
 // kernels: compute_new_F, svd, svd_grad
 // ----------------------------------------------------------------------------
+// init_object: x = R p + t, v = ball_vel, F = I (one thread per particle)
+struct InitObj {
+  float T[12], v[3];
+};
+__global__ __launch_bounds__(kBlock) void k_init_object(int n, InitObj A,
+                                                        const float* __restrict__ pts,
+                                                        float* __restrict__ x,
+                                                        float* __restrict__ v,
+                                                        float* __restrict__ F) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+  for (int r = 0; r < 3; ++r) {
+    x[3 * i + r] = A.T[4 * r] * p[0] + A.T[4 * r + 1] * p[1] + A.T[4 * r + 2] * p[2] + A.T[4 * r + 3];
+    v[3 * i + r] = A.v[r];
+  }
+  for (int k = 0; k < 9; ++k) F[9 * i + k] = (k % 4 == 0) ? 1.f : 0.f;
+}
+
 __global__ __launch_bounds__(kBlock) void k_compute_new_F(int n, const float* __restrict__ C,
                                                           const float* __restrict__ F,
                                                           float* __restrict__ Fn, float dt) {
@@ -2198,6 +2217,20 @@ size_t scratch_layout(const dt_mpm_params* prm, int n, size_t (&off)[7]) {
 using namespace dt;
 
 extern "C" {
+
+int dt_mpm_init_object(int32_t n, const float* particles, const float* trans16,
+                       const float* ball_vel3, float* x, float* v, float* F, void* stream) {
+  DT_REQUIRE(n >= 0, "n must be >= 0");
+  DT_REQUIRE(trans16 && ball_vel3, "NULL host parameter");
+  if (n == 0) return DT_OK;
+  DT_REQUIRE(particles && x && v && F, "NULL pointer");
+  InitObj A;
+  for (int i = 0; i < 12; ++i) A.T[i] = trans16[i];
+  for (int i = 0; i < 3; ++i) A.v[i] = ball_vel3[i];
+  hipLaunchKernelGGL(k_init_object, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream), n, A,
+                     particles, x, v, F);
+  return check_launch("dt_mpm_init_object");
+}
 
 int dt_mpm_compute_new_F(int32_t n, const float* C, const float* F, float* F_new, float dt,
                          void* stream) {

@@ -130,11 +130,11 @@ class MPMBase:
 
     def init_object(self):
         """``init_object`` (``mpm_elastic.py:107-114``): x0 = T p, v0 = ball_vel, F0 = I."""
-        T = self.trans_h.data
-        p = self.particles.data
-        self.x_0.data[0] = p @ T[:3, :3].T + T[:3, 3]
-        self.v_0.data[0] = self.ball_vel.data
-        self.F_0.data[0] = torch.eye(3, device=self.device)
+        T = np.ascontiguousarray(self.trans_h.to_numpy(), np.float32).reshape(-1)
+        bv = np.ascontiguousarray(self.ball_vel.to_numpy(), np.float32)
+        fp = lambda a: a.ctypes.data_as(_lib.ctypes.POINTER(_lib.ctypes.c_float))
+        _lib.call("dt_mpm_init_object", self.n_particles, self.particles.data, fp(T), fp(bv),
+                  self.x_0.data[0], self.v_0.data[0], self.F_0.data[0], _lib.stream())
 
     def set_state(self, x, v, C=None, F=None, f=0):
         """Write frame f from arrays (particles-from-array entry the configs need)."""

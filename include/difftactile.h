@@ -145,6 +145,10 @@ int dt_graph_stats(int64_t* hits, int64_t* misses);
 
 /* ---- MPM object, one entry point per reference kernel (frame f views) ---- */
 
+/* init_object, mpm_elastic.py:107-114: x = trans_h [p; 1] (trans16: HOST
+   row-major 4x4), v = ball_vel3 (HOST), F = I, for the n rest particles p. */
+int dt_mpm_init_object(int32_t n, const float* particles, const float* trans16,
+                       const float* ball_vel3, float* x, float* v, float* F, void* stream);
 /* compute_new_F, mpm_elastic.py:134-137: F_new = (I + dt C) F */
 int dt_mpm_compute_new_F(int32_t n, const float* C, const float* F, float* F_new, float dt,
                          void* stream);

@@ -13,6 +13,9 @@ Layout follows the reference's ``to_numpy()`` shapes: particles ``(N, 3)`` and
 ``(i, j, k)`` at ``(i * n + j) * n + k``.
 
 Pinning (tests/test_oracle_mpm.py):
+  init_object    -- identity pose keeps the rest particles; a pure translation
+                    shifts them; a quarter turn about z maps (x, y, z) to
+                    (-y, x, z); v is ball_vel and F the identity everywhere.
   cell_base      -- closed form trunc(f32(x) * f32(inv_dx) - 0.5f) checked by a
                     brute-force float64 re-derivation on lattices; sort order
                     against np.argsort(kind="stable").
@@ -179,6 +182,16 @@ def stencil_nodes(base, w, n_grid):
 # ---------------------------------------------------------------------------
 # F update and SVD
 # ---------------------------------------------------------------------------
+
+def init_object(particles, trans_h, ball_vel):
+    """``init_object`` (``mpm_elastic.py:107-114``): x = (trans_h [p; 1])_{0:3},
+    v = ball_vel, F = I."""
+    ph = torch.cat([particles, torch.ones(len(particles), 1, dtype=particles.dtype)], 1)
+    x = (ph @ trans_h.T)[:, :3]
+    v = ball_vel.reshape(1, 3).expand(len(particles), 3).clone()
+    F = torch.eye(3, dtype=particles.dtype).repeat(len(particles), 1, 1)
+    return x, v, F
+
 
 def compute_new_F(C, F, dt):
     """``F_new = (I + dt C) F`` (``mpm_elastic.py:134-137``)."""

@@ -845,3 +845,72 @@ def test_c5_size_sparse_substeps_conserve_mass_and_momentum(gpu):
         assert (g[:3] - mom).abs().max().item() / mom.abs().max().item() < 1e-4
     for f in frames:
         assert torch.isfinite(f[k]).all()
+
+
+def test_facade_init_object_and_two_window_bptt_carry_match_the_oracle(gpu):
+    """The object facade as the reference's tasks drive it: init_object from
+    a pose, two windows of S-1 substeps joined by memory_to_cache; backward
+    through window 2, memory_from_cache, the recomputed window 1 and its
+    substep_grad (mpm_elastic.py:107-114, 262-344).  Initial state against the
+    oracle's init_object; d loss / d (x0, v0) and d/d (mu, lam) of a linear
+    functional of the final positions against the oracle's reverse mode over
+    the 2 (S-1) substeps (the per-kernel chain uses the reference's
+    svd_grad)."""
+    from difftactile_amd.object_model.mpm_elastic import MPMObj
+    from difftactile_amd.params import Material
+    S = 4
+    rng = np.random.default_rng(12)
+    rest = rng.uniform(-0.06, 0.06, size=(3000, 3)).astype(np.float32)
+    obj = MPMObj(dt=2e-5, sub_steps=S, particles=rest, n_grid=32, rho=1000.0)
+    obj.set_material(5e4, 0.3)
+    obj.init([0.5, 0.45, 0.52], [10.0, 20.0, 30.0], [0.1, -0.3, 0.2])
+    torch.cuda.synchronize()
+    T = torch.tensor(obj.trans_h.to_numpy(), dtype=F64)
+    x0, v0, F0 = om.init_object(torch.tensor(rest, dtype=F64), T,
+                                torch.tensor(obj.ball_vel.to_numpy(), dtype=F64))
+    assert rel_l2(host64(obj.x_0.data[0]), x0) < 1e-6
+    assert rel_l2(host64(obj.v_0.data[0]), v0) < 1e-6
+    assert torch.equal(host64(obj.F_0.data[0]), F0)
+    # a shearing, spinning start velocity, so the material gradient is well
+    # conditioned (a rigidly moving block leaves F - R at rounding level)
+    A = rng.normal(0, 3.0, size=(3, 3))
+    v0 = (v0 + (x0 - x0.mean(0)) @ torch.tensor(A).T).float().double()
+    obj.v_0.data[0] = v0.float().cuda()
+    # forward: window 1, cache, window 2
+    for t in range(2):
+        obj.reset()
+        for f in range(S - 1):
+            obj.substep(f)
+        if t == 0:
+            obj.memory_to_cache(0)
+    x_end = host64(obj.x_0.data[S - 1])
+    w = rng.normal(size=(len(rest), 3)).astype(np.float32)
+    # backward: window 2, then window 1 recomputed from the cache
+    obj.clear_loss_grad()
+    obj.x_0.grad.data[S - 1] = torch.tensor(w, device="cuda")
+    for f in reversed(range(S - 1)):
+        obj.substep_grad(f)
+    obj.memory_from_cache(0)
+    obj.reset()
+    for f in range(S - 1):
+        obj.substep(f)
+    for f in reversed(range(S - 1)):
+        obj.substep_grad(f)
+    torch.cuda.synchronize()
+    # oracle over the 2 (S - 1) substeps
+    op = oracle_params(obj.prm)
+    op.svd_exact = False
+    n = len(rest)
+    mats, mu_t, lam_t = oracle_mats_leaf([Material(5e4, 0.3)], np.zeros(n, int), F64)
+    st = [x0.clone().requires_grad_(True), v0.clone().requires_grad_(True),
+          torch.zeros(n, 3, 3, dtype=F64), F0]
+    out = om.rollout(op, tuple(st), mats, 2 * (S - 1))
+    assert rel_l2(x_end, out[0].detach()) < 1e-5
+    # the recomputed window 1 ends where the first forward pass did
+    assert rel_l2(host64(obj.x_0.data[S - 1]), om.rollout(op, tuple(st), mats, S - 1)[0].detach()) < 1e-5
+    loss = (out[0] * torch.tensor(w, dtype=F64)).sum()
+    gx, gv, gmu, glam = torch.autograd.grad(loss, [st[0], st[1], mu_t, lam_t])
+    assert rel_l2(host64(obj.x_0.grad.data[0]), gx) < 1e-4
+    assert rel_l2(host64(obj.v_0.grad.data[0]), gv) < 1e-4
+    assert rel_l2(host64(obj.mu_0.grad.data).reshape(-1), gmu) < 1e-3
+    assert rel_l2(host64(obj.lambda_0.grad.data).reshape(-1), glam) < 1e-3

@@ -459,3 +459,19 @@ def test_bin_keys_name_the_cell_of_the_stencil_base():
     dec = np.stack([(b2 // (nb * nb)) * 4 + (l2 >> 4), ((b2 // nb) % nb) * 4 + ((l2 >> 2) & 3),
                     (b2 % nb) * 4 + (l2 & 3)], 1)
     assert dec.min() >= 0 and dec.max() <= n - 3
+
+
+def test_init_object_closed_forms():
+    rng = np.random.default_rng(3)
+    p = torch.tensor(rng.normal(size=(40, 3)))
+    bv = torch.tensor([0.5, -1.0, 2.0], dtype=torch.float64)
+    x, v, F = om.init_object(p, torch.eye(4, dtype=torch.float64), bv)
+    assert torch.equal(x, p) and torch.equal(v, bv.expand(40, 3))
+    assert torch.equal(F, torch.eye(3, dtype=torch.float64).expand(40, 3, 3))
+    T = torch.eye(4, dtype=torch.float64)
+    T[:3, 3] = torch.tensor([1.0, 2.0, -3.0])
+    assert torch.allclose(om.init_object(p, T, bv)[0], p + T[:3, 3], atol=1e-15)
+    T = torch.eye(4, dtype=torch.float64)
+    T[:2, :2] = torch.tensor([[0.0, -1.0], [1.0, 0.0]])
+    want = torch.stack([-p[:, 1], p[:, 0], p[:, 2]], 1)
+    assert torch.allclose(om.init_object(p, T, bv)[0], want, atol=1e-15)
