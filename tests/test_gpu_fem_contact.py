@@ -48,6 +48,43 @@ def _dome(seed=0, pert=0.01):
     return nodes.astype(np.float32), pos, f2v, surf, layer
 
 
+def test_fem_update_on_the_slab_is_within_1e5_of_fp64(gpu):
+    """The C2 slab (40 x 40 x 6 vertices, Kuhn tetrahedra, cond(Dm) < 10) with
+    C2's material: B, the updated velocity and its increment dt f / m are
+    within 1e-5 relative L2 of the FP64 oracle, the adjoint within 1e-5."""
+    from difftactile_amd.scenes import config2_pad_press
+    c = config2_pad_press(n_particles=1000, n_grid=32)
+    nodes, f2v, tris, layer, od = c["mesh"]
+    nodes = nodes.astype(np.float32)
+    nt, nv = len(f2v), len(nodes)
+    rng = np.random.default_rng(4)
+    h = float(np.abs(nodes[1] - nodes[0]).max())
+    pos = (nodes + 0.05 * h * rng.normal(size=nodes.shape)).astype(np.float32)
+    B = torch.empty(nt, 3, 3, device="cuda")
+    _lib.call("dt_fem_init_B", nt, dev(f2v, torch.int32), dev(nodes), P(B), _lib.stream())
+    B_ref = of.init_B(torch.tensor(nodes, dtype=F64), torch.tensor(f2v).long())
+    assert float(torch.linalg.cond(torch.linalg.inv(B_ref)).max()) < 10
+    assert rel_l2(h64(B), B_ref) < 1e-6
+    mu, lam, mass, dt = c["pad_mu"], c["pad_lam"], c["pad_mass"], c["prm"].dt
+    prm = fem_params(mu, lam, dt, mass)
+    vel0 = (1e-4 * rng.normal(size=(nv, 3))).astype(np.float32)
+    vel = dev(vel0)
+    _lib.call("dt_fem_update", prm, nt, dev(f2v, torch.int32), B, dev(pos), P(vel), _lib.stream())
+    posd = torch.tensor(pos, dtype=F64, requires_grad=True)
+    vref = of.update(posd, torch.tensor(vel0, dtype=F64), torch.tensor(f2v).long(), h64(B), mu,
+                     lam, dt, mass)
+    assert rel_l2(h64(vel), vref.detach()) < 1e-5
+    dv = vref.detach() - torch.tensor(vel0, dtype=F64)
+    assert float(dv.norm()) > float(torch.tensor(vel0).norm())   # the forces dominate
+    assert rel_l2(h64(vel) - torch.tensor(vel0, dtype=F64), dv) < 1e-5
+    gvel = rng.normal(size=(nv, 3)).astype(np.float32)
+    gpos = torch.zeros(nv, 3, device="cuda")
+    _lib.call("dt_fem_update_adj", prm, nt, dev(f2v, torch.int32), B, dev(pos), dev(gvel),
+              P(gpos), _lib.stream())
+    g_ref = torch.autograd.grad((vref * torch.tensor(gvel, dtype=F64)).sum(), posd)[0]
+    assert rel_l2(h64(gpos), g_ref) < 1e-5
+
+
 def test_fem_init_B_update_and_adjoint(gpu):
     nodes, pos, f2v, surf, layer = _dome()
     nt, nv = len(f2v), len(nodes)
