@@ -16,7 +16,17 @@ The oracle restates, on the CPU, the arithmetic of the reference kernels:
                     and ``gripper_fem.py``;
 * ``contact.py`` -- grid penalty contact (find_closest, find_sdf,
                     calculate_contact_force, check_collision, collision,
-                    clamp_grid) following ``difftactile/tasks/grasp_plastic.py``.
+                    clamp_grid) following ``difftactile/tasks/grasp_plastic.py``;
+* ``rope.py``, ``coupled.py``, ``gripper_scene.py`` -- the PBD cable and its
+                    pad contact, the object-pad substep and the gripper
+                    scene's composition;
+* ``kinematics.py``, ``dome_pose.py`` -- the jaw chain and the dome pad's pose
+                    control; ``markers.py`` -- markers and the task losses;
+                    ``deform.py`` -- the grid-mass deformation losses;
+* ``cpu_mpm.cpp`` (``cpu_mpm.py``, ``build_cpu.py``) -- a C++17/OpenMP
+                    float32 restatement of one MPM substep and its adjoint in
+                    the reference's loop structure, checked against
+                    ``mpm.py`` and timed as the CPU baseline.
 
 Forward code is written in torch on the CPU so that reverse-mode autograd over
 it reproduces what Taichi's reverse-mode AD (``kernel.grad``) does for the
