@@ -48,6 +48,8 @@ def parse():
                     help="sphere workload: sort the particles once per trajectory, not at "
                          "every checkpoint")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="sphere workload: use the cached (else default) launch shapes")
     ap.add_argument("--breakdown", action="store_true",
                     help="also time each stage in isolation on the initial state")
     ap.add_argument("--cpu-substeps", type=int, default=96,
@@ -744,6 +746,13 @@ def main():
     log(f"[bench] {n} particles, {prm.n_grid}^3: advancing {args.predeform} substeps to the "
         "start state")
     st = list(sim.step_no_grad(*st, args.predeform))
+    # launch shapes: tuned on this workload (4-substep fwd+bwd rollouts from the start state)
+    from difftactile_amd import autotune
+    if args.no_tune:
+        launch, tune_ms = autotune.apply_cached(prm, n) or autotune.get_config(), None
+    else:
+        log("[bench] tuning launch shapes")
+        launch, tune_ms = autotune.tune(sim, st, n_substeps=4, reps=3)
     w = torch.randn(n, 3, device=dev, generator=torch.Generator(device=dev).manual_seed(1))
     T = args.substeps
     # which segments the forward pass keeps (the rule of sim._Rollout.forward)
@@ -842,7 +851,8 @@ def main():
                    "checkpoint_every": args.segment, "segments": len(segs),
                    "segments_resident": kept, "substeps_recomputed": recomputed_substeps,
                    "recompute_share": recomputed_substeps / T,
-                   "predeform_substeps": args.predeform, "active_nodes": active},
+                   "predeform_substeps": args.predeform, "active_nodes": active,
+                   "launch_shapes": launch, "tuned_4substep_ms": tune_ms},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "substep_roofline_frac": b_sub * (value / n) / (HBM_PEAK_GBS * 1e9),

@@ -39,6 +39,11 @@ class DtJawParams(ctypes.Structure):
                 ("dt", ctypes.c_float)]
 
 
+class DtLaunchConfig(ctypes.Structure):
+    _fields_ = [("brick_blocks", ctypes.c_int32), ("node_blocks", ctypes.c_int32),
+                ("sort_blocks", ctypes.c_int32)]
+
+
 # floats per frame of the jaw chain (DT_JAW_FRAME_FLOATS)
 JAW_FRAME_FLOATS = 96
 # dome pad pose state (DT_DOME_*): field offsets and size
@@ -163,6 +168,8 @@ _SIGS = {
     "dt_jaw_chain_adj": (_c_int, [_P(DtJawParams), _c_int, _vp, _vp, _vp, _vp, _vp]),
     "dt_jaw_control_vel": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _c_float, _vp, _vp]),
     "dt_jaw_control_vel_adj": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_float, _vp, _vp, _vp]),
+    "dt_set_launch_config": (_c_int, [_P(DtLaunchConfig)]),
+    "dt_get_launch_config": (_c_int, [_P(DtLaunchConfig)]),
     "dt_dome_pose_control": (_c_int, [_c_int, _vp, _c_float, _vp, _vp, _vp]),
     "dt_dome_pose_control_adj": (_c_int, [_c_int, _vp, _c_float, _vp, _vp, _vp, _vp, _vp]),
     "dt_dome_control_vel": (_c_int, [_c_int, _vp, _vp, _c_float, _vp, _vp]),

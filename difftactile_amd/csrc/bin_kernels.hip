@@ -513,7 +513,8 @@ int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_
     hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
                        ws.start, ws.perm);
     // rank is dead after the fill: the large-brick path of the sort uses it
-    const long long nb_sort = ws.n_bricks < kBrickBlocks ? ws.n_bricks : kBrickBlocks;
+    const long long sort_cap = launch_config().sort_blocks;
+    const long long nb_sort = ws.n_bricks < sort_cap ? ws.n_bricks : sort_cap;
     hipLaunchKernelGGL(k_bin_sort, dim3((unsigned)(nb_sort > 0 ? nb_sort : 1)), dim3(kBlock), 0,
                        st, (const int32_t*)ws.start, (const int32_t*)ws.list,
                        (const int32_t*)ws.n_list, ws.perm, ws.rank);

@@ -11,6 +11,8 @@ namespace dt {
 
 void set_error(const std::string& msg);
 int check_launch(const char* what);
+// the process-wide launch shapes (dt_set_launch_config; zeros replaced by the defaults)
+dt_launch_config launch_config();
 
 #define DT_REQUIRE(cond, msg)                 \
   do {                                        \

@@ -4,6 +4,7 @@
 // a replay is exactly the launch sequence that was captured; the cache keeps
 // the most recently used entries.  The captured sequences contain kernels
 // only (no memset or memcpy nodes).
+#include <atomic>
 #include <list>
 #include <mutex>
 #include <utility>
@@ -23,6 +24,8 @@ std::mutex g_mu;
 std::list<Entry> g_cache;   // front = most recently used
 bool g_enabled = true;
 long long g_hits = 0, g_misses = 0;
+// launch shapes: atomics, read by launches that run inside run_graphed's lock
+std::atomic<int32_t> g_brick_blocks{0}, g_node_blocks{0}, g_sort_blocks{0};
 std::vector<std::pair<int, hipStream_t>> g_capture_streams;   // per device
 
 hipStream_t capture_stream(int dev) {
@@ -42,6 +45,10 @@ int run_graphed(std::vector<uint64_t> key, hipStream_t st,
   if (hipGetDevice(&dev) != hipSuccess) return body(st);
   key.push_back((uint64_t)dev);
   std::lock_guard<std::mutex> lock(g_mu);
+  // the launch shapes the body reads
+  const dt_launch_config lc = launch_config();
+  key.push_back(((uint64_t)(uint32_t)lc.brick_blocks << 32) | (uint32_t)lc.node_blocks);
+  key.push_back((uint64_t)(uint32_t)lc.sort_blocks);
   for (auto it = g_cache.begin(); it != g_cache.end(); ++it) {
     if (it->key == key) {
       g_cache.splice(g_cache.begin(), g_cache, it);
@@ -89,6 +96,17 @@ int run_graphed(std::vector<uint64_t> key, hipStream_t st,
   return DT_OK;
 }
 
+dt_launch_config launch_config() {
+  dt_launch_config c;
+  c.brick_blocks = g_brick_blocks.load();
+  c.node_blocks = g_node_blocks.load();
+  c.sort_blocks = g_sort_blocks.load();
+  if (c.brick_blocks <= 0) c.brick_blocks = 2048;
+  if (c.node_blocks <= 0) c.node_blocks = 1024;
+  if (c.sort_blocks <= 0) c.sort_blocks = 2048;
+  return c;
+}
+
 }  // namespace dt
 
 using namespace dt;
@@ -98,6 +116,23 @@ extern "C" {
 int dt_graphs_enable(int32_t on) {
   std::lock_guard<std::mutex> lock(g_mu);
   g_enabled = on != 0;
+  return DT_OK;
+}
+
+int dt_set_launch_config(const dt_launch_config* cfg) {
+  DT_REQUIRE(cfg, "NULL config");
+  DT_REQUIRE(cfg->brick_blocks >= 0 && cfg->brick_blocks <= 65536 && cfg->node_blocks >= 0 &&
+                 cfg->node_blocks <= 65536 && cfg->sort_blocks >= 0 && cfg->sort_blocks <= 65536,
+             "launch shapes must be in [0, 65536]");
+  g_brick_blocks.store(cfg->brick_blocks);
+  g_node_blocks.store(cfg->node_blocks);
+  g_sort_blocks.store(cfg->sort_blocks);
+  return DT_OK;
+}
+
+int dt_get_launch_config(dt_launch_config* cfg) {
+  DT_REQUIRE(cfg, "NULL config");
+  *cfg = launch_config();
   return DT_OK;
 }
 

@@ -1981,11 +1981,13 @@ static void launch_p2g_fused_adj(const MpmP& P, hipStream_t st, const dt_materia
 
 static unsigned active_blocks(const BinWs& ws) {
   const long long b = (ws.n_bricks + 3) / 4;
-  return (unsigned)(b < 1024 ? b : 1024);
+  const long long cap = launch_config().node_blocks;
+  return (unsigned)(b < cap ? b : cap);
 }
 
 static unsigned brick_blocks(const BinWs& ws) {
-  return (unsigned)(ws.n_bricks < kBrickBlocks ? ws.n_bricks : kBrickBlocks);
+  const long long cap = launch_config().brick_blocks;
+  return (unsigned)(ws.n_bricks < cap ? ws.n_bricks : cap);
 }
 
 // bin the particles of x (also sets the active-brick flags)

@@ -43,8 +43,16 @@ class MPMSim:
 
     def __init__(self, prm, materials, n_particles, mat_id=None, device="cuda",
                  checkpoint_every=50, sort=True, sparse_grid=True, resident_bytes=None,
-                 material_grad=False, env_id=None, resort=True):
+                 material_grad=False, env_id=None, resort=True, launch=None):
         self.prm = prm
+        # launch shapes of the persistent passes (process-wide): "cache" sets
+        # the autotuner's cached shapes of this workload, a dict sets those
+        if launch == "cache":
+            from . import autotune
+            autotune.apply_cached(prm, int(n_particles))
+        elif isinstance(launch, dict):
+            from . import autotune
+            autotune.set_config(launch)
         if resident_bytes is None:
             # a quarter of the device's memory for resident segments
             dev = torch.device(device)

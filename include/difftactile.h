@@ -143,6 +143,23 @@ int dt_timing_read(int32_t stage, double* total_ms, int64_t* launches);
 int dt_graphs_enable(int32_t on);
 int dt_graph_stats(int64_t* hits, int64_t* misses);
 
+/* ---- launch shapes (no reference counterpart: the autotuner's knobs) ----
+   Process-wide grid sizes of the persistent launches; 0 keeps the default.
+   brick_blocks: blocks of the brick-owned scatter / gather passes (default
+   2048; each block strides over the occupied bricks); node_blocks: blocks of
+   the active-brick node passes (default 1024, 4 bricks per block and
+   iteration); sort_blocks: blocks of the per-brick segment sort (default
+   2048).  Results do not depend on them, except the order of the
+   material-adjoint partial sums (per block of particles, not of bricks: they
+   do not change either).  Graph keys include the setting. */
+typedef struct dt_launch_config {
+  int32_t brick_blocks;
+  int32_t node_blocks;
+  int32_t sort_blocks;
+} dt_launch_config;
+int dt_set_launch_config(const dt_launch_config* cfg);
+int dt_get_launch_config(dt_launch_config* cfg);
+
 /* ---- MPM object, one entry point per reference kernel (frame f views) ---- */
 
 /* init_object, mpm_elastic.py:107-114: x = trans_h [p; 1] (trans16: HOST
