@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--pad-recompute", action="store_true",
                     help="pad_press: keep checkpoints only (the backward pass recomputes "
                          "every segment) instead of the resident trajectory")
+    ap.add_argument("--pop-grouped", action="store_true",
+                    help="population workload: one stacked batch per grid size instead of one "
+                         "ragged batch")
     ap.add_argument("--pop-pads", action="store_true",
                     help="population workload: one FEM pad (10,800 tets) per environment "
                          "pressing its object")
@@ -478,9 +481,9 @@ def bench_pad_press(args, torch):
 
 def bench_population(args, torch):
     """Batched population: 64 environments (5k-200k particles, 64^3 or 128^3
-    grids) advanced fwd+bwd for --pop-substeps substeps, once as one stacked
-    batch per grid size (EnvBatch) and once as a loop over per-environment
-    simulations.  Both recompute 10-substep segments from checkpoints (no
+    grids) advanced fwd+bwd for --pop-substeps substeps, once batched (one
+    ragged EnvBatch over both grid sizes, or with --pop-grouped one per grid
+    size) and once as a loop over per-environment simulations.  Both recompute 10-substep segments from checkpoints (no
     resident segments: the stacked 128^3 grids of 300 substeps exceed HBM).
     Loss per environment: its final centre of mass along a seeded direction;
     the gradient is taken with respect to its initial velocities."""
@@ -513,8 +516,13 @@ def bench_population(args, torch):
     for e, d in enumerate(envs):
         groups.setdefault(d["prm"].n_grid, []).append(e)
     batches, sims = [], []
-    for ids in groups.values():
-        spec = [(envs[e]["mats"], len(envs[e]["x"]), envs[e]["mat_id"]) for e in ids]
+    # ragged (default): all environments in one launch sequence, each keeping
+    # its own grid inside a slot of the largest; grouped: one per grid size
+    layout = "grouped" if args.pop_grouped else "ragged"
+    batch_groups = list(groups.values()) if args.pop_grouped else [list(range(len(envs)))]
+    for ids in batch_groups:
+        spec = [(envs[e]["mats"], len(envs[e]["x"]), envs[e]["mat_id"], envs[e]["prm"])
+                for e in ids]
         kw = dict(checkpoint_every=seg, sort=True, resident_bytes=0)
         if pads:
             d0 = envs[ids[0]]
@@ -607,11 +615,15 @@ def bench_population(args, torch):
            "config": {"workload": f"{len(envs)} environments, {n_tot} particles "
                                   "(log-uniform 5e3-2e5 per environment), 64^3 below 3e4 "
                                   f"particles else 128^3, {T} substeps fwd+bwd, 10-substep "
-                                  "segments recomputed; batched = one stacked launch sequence "
-                                  "per grid size, loop = one simulation per environment"
+                                  "segments recomputed; batched = "
+                                  + ("one launch sequence over all environments (ragged grids)"
+                                     if layout == "ragged" else
+                                     "one stacked launch sequence per grid size")
+                                  + ", loop = one simulation per environment"
                                   + ("; each environment with a 10,800-tet FEM pad pressing "
                                      "its object (penalty contact)" if pads else ""),
-                      "groups": {f"{n}^3": len(ids) for n, ids in groups.items()}}}
+                      "groups": {f"{n}^3": len(ids) for n, ids in groups.items()},
+                      "layout": layout}}
     print(json.dumps(res), flush=True)
 
 

@@ -30,7 +30,8 @@ class DtMpmParams(ctypes.Structure):
         ("gravity", ctypes.c_float * 3), ("grad_clamp", ctypes.c_float),
         ("soa", ctypes.c_int32), ("svd_exact", ctypes.c_int32), ("n_env", ctypes.c_int32),
         ("env_id", ctypes.c_void_p), ("active_bricks", ctypes.c_void_p),
-        ("n_active_bricks", ctypes.c_void_p),
+        ("n_active_bricks", ctypes.c_void_p), ("env_ngrid", ctypes.c_void_p),
+        ("env_scale", ctypes.c_void_p),
     ]
 
 

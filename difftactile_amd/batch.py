@@ -22,19 +22,35 @@ from .sim import MPMSim
 
 
 class EnvBatch:
-    """E independent MPM environments of one grid size in one ``MPMSim``.
+    """E independent MPM environments in one ``MPMSim``.
 
     prm:  ``MPMParams`` of one environment (grid, time step, gravity, walls).
-    envs: one ``(materials, n_particles, mat_id or None)`` per environment.
+    envs: one ``(materials, n_particles, mat_id or None)`` per environment,
+          optionally with a fourth entry, the environment's own ``MPMParams``:
+          environments of different grids then share one launch sequence
+          ("ragged": each keeps its grid size, dx and particle mass inside a
+          slot of the largest grid; they share dt, gravity, walls and eps).
     Further keyword arguments go to ``MPMSim`` (``checkpoint_every``,
-    ``sort``, ``sparse_grid``, ``resident_bytes``, ``material_grad``).
+    ``sort``, ``resident_bytes``, ``material_grad``); ragged batches run the
+    sparse path.
     """
 
     def __init__(self, prm, envs, device="cuda", **sim_kwargs):
         if len(envs) < 1:
             raise ValueError("a batch needs at least one environment")
+        env_prms = [e[3] if len(e) > 3 and e[3] is not None else prm for e in envs]
+        ragged = any((q.n_grid, q.dx, q.p_mass) != (prm.n_grid, prm.dx, prm.p_mass)
+                     for q in env_prms)
         self.prm = copy.copy(prm)
         self.prm.n_env = len(envs)
+        if ragged:
+            self.prm.n_grid = max(q.n_grid for q in env_prms)
+            if sim_kwargs.get("sparse_grid") is False:
+                raise ValueError("ragged batches run the sparse path")
+            self.prm.set_ragged(env_prms, device=device)
+        self.env_prms = env_prms
+        self.ragged = ragged
+        envs = [e[:3] for e in envs]
         mats, mids, ids, counts, n_mats = [], [], [], [], []
         for e, (materials, n, mat_id) in enumerate(envs):
             n = int(n)

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float
   if (active) {
     float xp[3];
     ld_v3(x, p, pl_vec(P.soa, n), xp);
-    k = bin_key(xp, P.inv_dx, P.n_grid, env_x0(P, p)) >> 6;
+    k = bin_key(xp, at_particle(P, p).inv_dx, P.n_grid, env_x0(P, p)) >> 6;
   }
   const int r = wave_key_rank(active, k, count);
   if (active) {

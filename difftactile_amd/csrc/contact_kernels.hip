@@ -206,9 +206,10 @@ __global__ __launch_bounds__(kClosestThreads) void k_closest(MpmP P, long long n
         lo = tri_start[e];
         hi = tri_start[e + 1];
       }
-      qx = node_centre(i % n, P.dx);
-      qy = node_centre(j, P.dx);
-      qz = node_centre(k, P.dx);
+      const float dxe = at_xnode(P, i).dx;   // the node's environment
+      qx = node_centre(i % n, dxe);
+      qy = node_centre(j, dxe);
+      qz = node_centre(k, dxe);
     }
     // the chunk searches the union of its occupied nodes' triangle ranges; a
     // node outside the common range checks each triangle against its own
@@ -402,7 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_collision(MpmP P, ContactP C, long l
     if (k < 0) return;
     const int n = P.n_grid;
     int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
-    float q[3] = {node_centre(i % n, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
+    const float dxe = at_xnode(P, i).dx;
+    float q[3] = {node_centre(i % n, dxe), node_centre(j, dxe), node_centre(kk, dxe)};
     float inv = 1.f / (mm + P.eps);
     float qv[3] = {mv[nd * smv] * inv, mv[nd * smv + 1] * inv, mv[nd * smv + 2] * inv};
     int tv[3] = {tris[3 * k], tris[3 * k + 1], tris[3 * k + 2]};
@@ -528,7 +530,8 @@ __global__ __launch_bounds__(kBlock) void k_collision_adj(
     if (k < 0) return;
     const int n = P.n_grid;
     int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
-    float q[3] = {node_centre(i % n, P.dx), node_centre(j, P.dx), node_centre(kk, P.dx)};
+    const float dxe = at_xnode(P, i).dx;
+    float q[3] = {node_centre(i % n, dxe), node_centre(j, dxe), node_centre(kk, dxe)};
     float inv = 1.f / (mm + P.eps);
     float mvv[3] = {mv[nd * smv], mv[nd * smv + 1], mv[nd * smv + 2]};
     float qv[3] = {mvv[0] * inv, mvv[1] * inv, mvv[2] * inv};
@@ -599,9 +602,10 @@ __device__ __forceinline__ bool node_contact(const MpmP& P, const ContactP& C, l
   if (k < 0) return false;
   const int n = P.n_grid;
   const int i = (int)(nd / ((long long)n * n)), j = (int)((nd / n) % n), kk = (int)(nd % n);
-  q[0] = node_centre(i % n, P.dx);
-  q[1] = node_centre(j, P.dx);
-  q[2] = node_centre(kk, P.dx);
+  const float dxe = at_xnode(P, i).dx;
+  q[0] = node_centre(i % n, dxe);
+  q[1] = node_centre(j, dxe);
+  q[2] = node_centre(kk, dxe);
   inv = 1.f / (mm + P.eps);
   for (int a = 0; a < 3; ++a) mvv[a] = mv[nd * smv + a];
   const float qv[3] = {mvv[0] * inv, mvv[1] * inv, mvv[2] * inv};

@@ -44,6 +44,11 @@ struct MpmP {
   float dx, inv_dx, dt, p_vol, p_mass, eps;
   float g[3];
   float grad_clamp;
+  // ragged environments: per-environment grid size and (dx, inv_dx, p_mass,
+  // p_vol); n_wall is the grid size the walls of this view act at
+  const int32_t* env_ngrid;
+  const float4* env_scale;
+  int n_wall;
 };
 
 inline MpmP to_dev(const dt_mpm_params& p) {
@@ -70,12 +75,39 @@ inline MpmP to_dev(const dt_mpm_params& p) {
   d.g[1] = p.gravity[1];
   d.g[2] = p.gravity[2];
   d.grad_clamp = p.grad_clamp;
+  d.env_ngrid = (d.n_env > 1 && p.env_scale) ? p.env_ngrid : nullptr;
+  d.env_scale = (d.n_env > 1 && p.env_ngrid) ? reinterpret_cast<const float4*>(p.env_scale)
+                                             : nullptr;
+  if (!d.env_scale) d.env_ngrid = nullptr;
+  d.n_wall = p.n_grid;
   return d;
 }
 
 // First x-node of particle p's environment (environments stacked along x).
 __device__ __forceinline__ int env_x0(const MpmP& P, long long p) {
   return P.env_id ? P.env_id[p] * P.n_grid : 0;
+}
+
+// The view of P for environment e: its dx, inv_dx, p_mass, p_vol and wall
+// grid size when the environments are ragged, P itself otherwise.
+__device__ __forceinline__ MpmP at_env(const MpmP& P, int e) {
+  MpmP Q = P;
+  if (P.env_scale) {
+    const float4 s = P.env_scale[e];
+    Q.dx = s.x;
+    Q.inv_dx = s.y;
+    Q.p_mass = s.z;
+    Q.p_vol = s.w;
+    Q.n_wall = P.env_ngrid[e];
+  }
+  return Q;
+}
+__device__ __forceinline__ MpmP at_particle(const MpmP& P, long long p) {
+  return P.env_scale ? at_env(P, P.env_id[p]) : P;
+}
+// environment of x-node i (environments stacked along x)
+__device__ __forceinline__ MpmP at_xnode(const MpmP& P, int i) {
+  return P.env_scale ? at_env(P, i / P.n_grid) : P;
 }
 
 // Node of lane threadIdx&63 in 4x4x4-node brick b (-1 outside the grid).

@@ -510,9 +510,10 @@ __device__ __forceinline__ void p2g_load(const MpmP& P, int n, const int32_t* ma
 // compute_new_F + svd + return map of particle p; writes F_next and returns
 // the stencil and the scatter coefficients (b, M) of the p2g contribution
 // w_o (p_mass v + A (o - fx) dx) = w_o (b + M o)
-__device__ __forceinline__ void p2g_fused_compute(const MpmP& P, const dt_material* mats, int n,
+__device__ __forceinline__ void p2g_fused_compute(const MpmP& P0, const dt_material* mats, int n,
                                                   const P2GIn& in, float* F_next, int p,
                                                   Stencil& s, float (&b)[3], M3& M) {
+  const MpmP P = at_particle(P0, p);
   PL Lm = pl_mat(P.soa, n);
   M3 a = m3_eye();
   for (int i = 0; i < 9; ++i) a.a[i] += P.dt * in.c.a[i];
@@ -851,12 +852,13 @@ __device__ __forceinline__ void opaque(M3& m) {
 // and recomputes only the return map for the stress adjoint.
 template <bool kExact>
 __device__ __forceinline__ void p2g_fused_adj_particle(
-    const MpmP& P, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
+    const MpmP& P0, const dt_material* __restrict__ mats, int n, const int32_t* __restrict__ mat_id,
     const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ C,
     const float* __restrict__ F, const float* __restrict__ ggrid4,
     const float* __restrict__ gFnext, float* __restrict__ gx_io, float* __restrict__ gv_out,
     float* __restrict__ gC_out, float* __restrict__ gF_out, int p, float (*park)[kBlock],
     float& gmu, float& glam) {
+  const MpmP P = at_particle(P0, p);
   const int t = threadIdx.x;
   PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
   const Mat m = load_mat(mats, mat_id, p);
@@ -976,8 +978,10 @@ __device__ __forceinline__ void grid_node_fwd(const MpmP& P, long long nd, float
   vo[1] = inv_m * mv1 + inv_m * f1 + P.dt * P.g[1];
   vo[2] = inv_m * mv2 + inv_m * f2 + P.dt * P.g[2];
   int idx[3] = {i % n, j, k};   // x index inside the node's environment
+  // walls of the node's environment (its own grid size when ragged)
+  const int nw = P.env_scale ? P.env_ngrid[i / n] : n;
   for (int a = 0; a < 3; ++a) {
-    mask[a] = (idx[a] < P.bound && vo[a] < 0.f) || (idx[a] > n - P.bound && vo[a] > 0.f);
+    mask[a] = (idx[a] < P.bound && vo[a] < 0.f) || (idx[a] > nw - P.bound && vo[a] > 0.f);
     if (P.sticky_floor && j < P.bound) mask[a] = true;
     if (mask[a]) vo[a] = 0.f;
   }
@@ -1384,23 +1388,24 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
   if (!kCount && !active) return;
   int k = -1;
   if (active) {
+    const MpmP Q = at_particle(P, p);
     PL Lv = pl_vec(P.soa, n), Lm = pl_mat(P.soa, n);
     Stencil s;
     float xp[3];
     ld_v3(x, p, Lv, xp);
-    make_stencil(xp, P.inv_dx, P.n_grid, s, env_x0(P, p));
+    make_stencil(xp, Q.inv_dx, P.n_grid, s, env_x0(P, p));
     float nv[3];
     M3 nC;
     if (sgv == 4)
-      g2p_core(P, s, Nodes4{(const float4*)gv, P.n_grid}.at(s.base), nv, nC);
+      g2p_core(Q, s, Nodes4{(const float4*)gv, P.n_grid}.at(s.base), nv, nC);
     else
-      g2p_core(P, s, NodesStrided<false>{gv, nullptr, sgv, 0, P.n_grid}.at(s.base), nv, nC);
+      g2p_core(Q, s, NodesStrided<false>{gv, nullptr, sgv, 0, P.n_grid}.at(s.base), nv, nC);
     float nx[3];
     for (int a = 0; a < 3; ++a) nx[a] = xp[a] + P.dt * nv[a];
     st_v3(v1, p, Lv, nv);
     st_v3(x1, p, Lv, nx);
     st_m3L(C1, p, Lm, nC);
-    if (kCount) k = bin_key(nx, P.inv_dx, P.n_grid, env_x0(P, p)) >> 6;
+    if (kCount) k = bin_key(nx, Q.inv_dx, P.n_grid, env_x0(P, p)) >> 6;
   }
   if (kCount) {
     const int r = wave_key_rank(active, k, bcount);
@@ -1448,9 +1453,10 @@ __device__ __forceinline__ void g2p_adj_load(const MpmP& P, int n, const float* 
 }
 
 template <int kGxAssign, class Gv>
-__device__ __forceinline__ void g2p_adj_compute(const MpmP& P, int n, const G2PAdjIn& in,
+__device__ __forceinline__ void g2p_adj_compute(const MpmP& P0, int n, const G2PAdjIn& in,
                                                 const Gv& gvsrc, float* gx, int p, Stencil& s,
                                                 float (&b)[3], M3& M) {
+  const MpmP P = at_particle(P0, p);
   PL Lv = pl_vec(P.soa, n);
   float gvt[3], lgx[3];
   make_stencil(in.x, P.inv_dx, P.n_grid, s, env_x0(P, p));
@@ -1685,12 +1691,12 @@ __device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
 //   fetch(b)                    (kTile) node t of brick b's 6x6x6 grid tile,
 //                               stored to sh.gtile at every chunk start of b
 //                               (the staging of a chunk overwrites it)
-template <bool kTile, class In, class Load, class Compute, class Fetch>
+template <bool kTile, class In, class MassOf, class Load, class Compute, class Fetch>
 __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __restrict__ perm,
                                            const int32_t* __restrict__ start,
                                            const int32_t* __restrict__ list,
                                            const int32_t* __restrict__ n_list,
-                                           float4* __restrict__ tiles, float mass_scale,
+                                           float4* __restrict__ tiles, const MassOf& mass_of,
                                            const Load& load, const Compute& compute,
                                            const Fetch& fetch) {
   const int t = threadIdx.x;
@@ -1765,7 +1771,7 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
     }
     float4 acc = brick_gather(sh);
     if (t < kTileNodes) {
-      acc.w *= mass_scale;
+      acc.w *= mass_of(b);
       tiles[b * kTileNodes + t] = acc;
     }
     __syncthreads();
@@ -1791,8 +1797,11 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_binned(MpmP P, const dt_mater
                                                        const int32_t* __restrict__ n_list,
                                                        float4* __restrict__ tiles) {
   __shared__ BrickStage sh;
+  const int nbk = (P.n_grid + 3) >> 2;
   brick_loop<false, P2GIn>(
-      sh, perm, start, list, n_list, tiles, P.p_mass,
+      sh, perm, start, list, n_list, tiles,
+      // the brick's particles all live in its environment
+      [&](long long bb) { return at_xnode(P, (int)(bb / ((long long)nbk * nbk)) * 4).p_mass; },
       [&](int p, P2GIn& in) { p2g_load(P, n, mat_id, x, v, C, F, p, in); },
       [&](const P2GIn& in, int p, long long, Stencil& s, float(&bb)[3], M3& M) {
         p2g_fused_compute(P, mats, n, in, F_next, p, s, bb, M);
@@ -1836,7 +1845,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_g2p_adj_binned(MpmP P, int n,
   __shared__ BrickStage sh;
   const int nb = (P.n_grid + 3) >> 2;
   brick_loop<true, G2PAdjIn>(
-      sh, perm, start, list, n_list, tiles, 0.f,
+      sh, perm, start, list, n_list, tiles, [](long long) { return 0.f; },
       [&](int p, G2PAdjIn& in) { g2p_adj_load(P, n, x, gx1, gv1, gC1, p, in); },
       [&](const G2PAdjIn& in, int p, long long b, Stencil& s, float(&bb)[3], M3& M) {
         const GvTile src{sh.gtile, (int)(b / ((long long)nb * nb)) * 4, (int)((b / nb) % nb) * 4,
@@ -1855,7 +1864,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(MpmP P, int n, const float
   if (p >= n) return;
   float xp[3];
   ld_v3(x, p, pl_vec(P.soa, n), xp);
-  keys[p] = cell_key(xp, P.inv_dx, P.n_grid, env_x0(P, p));
+  keys[p] = cell_key(xp, at_particle(P, p).inv_dx, P.n_grid, env_x0(P, p));
 }
 
 __global__ __launch_bounds__(kBlock) void k_gather_rows(long long total, int width,
@@ -2029,6 +2038,7 @@ int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
                        (const float4*)ws->tiles, (const int32_t*)ws->start, (float4*)grid4,
                        (float4*)grid_v4);
   } else {
+    DT_REQUIRE(!P.env_scale, "ragged environments need the binned path (a workspace)");
     hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)grid4);
     Grid g{grid4, grid4 + 3, 4, 4};
     if (n > 0)
@@ -2079,6 +2089,7 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n_
                        (const float4*)grid4, (const float4*)ws->tiles,
                        (const int32_t*)ws->start, (float4*)ggv4, (float4*)ggrid4);
   } else {
+    DT_REQUIRE(!P.env_scale, "ragged environments need the binned path (a workspace)");
     hipLaunchKernelGGL(k_zero4, dim3(nblocks(nn)), dim3(kBlock), 0, st, nn, (float4*)ggv4);
     if (n > 0)
       hipLaunchKernelGGL(k_g2p_adj<1>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, grid_v4, 4,
@@ -2277,6 +2288,7 @@ int dt_mpm_p2g(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats
                const int32_t* mat_id, const float* x, const float* v, const float* C,
                const float* F_new, const float* U, const float* S, const float* V, float* grid_mv,
                float* grid_m, float* F_next, void* stream) {
+  DT_REQUIRE(!prm || !prm->env_scale, "ragged environments need the binned path");
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
@@ -2296,6 +2308,7 @@ int dt_mpm_p2g_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
                    const float* g_grid_mv, const float* g_grid_m, const float* gF_next, float* gx,
                    float* gv, float* gC, float* gF_new, float* gU, float* gS, float* gV,
                    float* g_mats, void* stream) {
+  DT_REQUIRE(!prm || !prm->env_scale, "ragged environments need the binned path");
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
@@ -2362,6 +2375,7 @@ int dt_mpm_g2p(const dt_mpm_params* prm, int32_t n, const float* x, const float*
 int dt_mpm_g2p_adj(const dt_mpm_params* prm, int32_t n, const float* x, const float* grid_v_out,
                    const float* gx_next, const float* gv_next, const float* gC_next, float* gx,
                    float* g_grid_v_out, void* stream) {
+  DT_REQUIRE(!prm || !prm->env_scale, "ragged environments need the binned path");
   int rc = validate_params(prm);
   if (rc) return rc;
   DT_REQUIRE(n >= 0, "n must be >= 0");
@@ -2641,6 +2655,7 @@ int dt_mpm_zero4(int64_t n_nodes, float* buf4, void* stream) {
 int dt_mpm_p2g_fused(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                      const int32_t* mat_id, const float* x, const float* v, const float* C,
                      const float* F, float* grid4, float* F_next, void* stream) {
+  DT_REQUIRE(!prm || !prm->env_scale, "ragged environments need the binned path");
   int rc = validate_params(prm);
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
@@ -2681,6 +2696,7 @@ int dt_mpm_g2p4(const dt_mpm_params* prm, int32_t n, const float* x, const float
 int dt_mpm_g2p4_adj(const dt_mpm_params* prm, int32_t n, const float* x, const float* grid_v4,
                     const float* gx_next, const float* gv_next, const float* gC_next, float* gx,
                     float* g_grid_v4, void* stream) {
+  DT_REQUIRE(!prm || !prm->env_scale, "ragged environments need the binned path");
   int rc = validate_params(prm);
   if (rc) return rc;
   DT_REQUIRE(n >= 0, "n must be >= 0");
@@ -2956,6 +2972,7 @@ int dt_mpm_grid_op4_adj_tiles_f(const dt_mpm_params* prm, int32_t n, const void*
 
 int dt_mpm_mark_bricks(const dt_mpm_params* prm, int32_t n, const float* x, int32_t* flags,
                        void* stream) {
+  DT_REQUIRE(!prm || !prm->env_scale, "ragged environments need the binned path");
   int rc = validate_params(prm);
   if (rc) return rc;
   DT_REQUIRE(n >= 0, "n must be >= 0");

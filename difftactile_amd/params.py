@@ -47,6 +47,29 @@ class MPMParams:
         # environments stacked along x (see dt_mpm_params.n_env): environment e
         # occupies x in [e, e + 1) * space_scale
         self.n_env = max(1, int(n_env))
+        # ragged environments (set by set_ragged): device tables of each
+        # environment's grid size and (dx, inv_dx, p_mass, p_vol)
+        self.env_ngrid = None
+        self.env_scale = None
+
+    def set_ragged(self, env_prms, device="cuda"):
+        """Stack environments of different grids: environment e keeps the
+        grid size, dx, p_mass and p_vol of env_prms[e] inside an n_grid^3
+        slot (n_grid must be at least every environment's); dt, gravity,
+        walls and eps are this object's."""
+        import torch
+        if len(env_prms) != self.n_env:
+            raise ValueError("one MPMParams per environment")
+        for q in env_prms:
+            if q.n_grid > self.n_grid or q.n_grid % 4:
+                raise ValueError("environment grids must be multiples of 4 within the slot")
+            if (q.dt, q.gravity, q.bound, q.eps, q.sticky_floor) != (
+                    self.dt, self.gravity, self.bound, self.eps, self.sticky_floor):
+                raise ValueError("ragged environments share dt, gravity, walls and eps")
+        self.env_ngrid = torch.tensor([q.n_grid for q in env_prms], dtype=torch.int32,
+                                      device=device)
+        self.env_scale = torch.tensor([[q.dx, q.inv_dx, q.p_mass, q.p_vol] for q in env_prms],
+                                      dtype=torch.float32, device=device).contiguous()
 
     @property
     def n_nodes(self):
@@ -70,6 +93,9 @@ class MPMParams:
         p.soa = int(soa)
         p.svd_exact = int(self.svd_exact)
         p.n_env = self.n_env
+        if self.env_ngrid is not None:
+            p.env_ngrid = self.env_ngrid.data_ptr()
+            p.env_scale = self.env_scale.data_ptr()
         return p
 
     def key_bits(self):

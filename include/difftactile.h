@@ -77,6 +77,17 @@ typedef struct dt_mpm_params {
                            (dt_mpm_active_bricks) covers every node a stencil of
                            its particles touches, so passes restricted to it leave
                            the other nodes unwritten and unread.                 */
+  const int32_t* env_ngrid; /* ragged environments (NULL: every environment uses
+                           n_grid, dx, p_mass, p_vol above): device [n_env] grid
+                           size n_e <= n_grid of each environment; it occupies
+                           nodes [0, n_e)^3 of its n_grid^3 slot, its walls act at
+                           n_e - bound.  Honoured by the binned (sparse) path:
+                           dt_mpm_bin / dt_mpm_cell_keys, dt_mpm_p2g_binned,
+                           dt_mpm_tiles_sum*, dt_mpm_grid_op4*, dt_mpm_g2p4*,
+                           dt_mpm_p2g_fused_adj, dt_mpm_run_fwd / _adj and the
+                           contact passes; the dense entry points reject it.   */
+  const float* env_scale; /* device [n_env][4]: dx, inv_dx, p_mass, p_vol of each
+                           environment (with env_ngrid).                          */
 } dt_mpm_params;
 
 /* One row of the material table (multi_obj.py:65-74 generalised). */

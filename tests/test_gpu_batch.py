@@ -145,3 +145,47 @@ def test_batched_pads_match_separate_coupled_simulations(gpu, sparse):
             err = rel_l2(a, b)
             assert err < 1e-3, (e, k, err)
     assert touched == len(envs), "every pad must move"
+
+
+def test_ragged_environments_match_separate_simulations(gpu):
+    """Environments of 32^3, 48^3 and 64^3 grids in one ragged batch (one
+    launch sequence, each environment keeping its grid, dx, particle mass and
+    walls inside a 64^3 slot) against each one simulated alone on its own
+    grid: states and d loss / d (x0, v0, C0, F0) bit for bit (the brick-owned
+    sums see the same particles in the same order)."""
+    from difftactile_amd.params import MPMParams
+    grids = (32, 48, 64)
+    prms = [MPMParams(n_grid=g, rho=1000.0, dt=2e-5) for g in grids]
+    envs = _envs()
+    steps, every = 8, 3
+    batch = EnvBatch(prms[0], [(materials(k), len(x), mid, q) for (k, x, _, _, _, mid), q in
+                               zip(envs, prms)], checkpoint_every=every, sort=True)
+    assert batch.ragged and batch.prm.n_grid == 64
+    states = [[torch.tensor(a).cuda().requires_grad_(True) for a in (x, v, C, F)]
+              for _, x, v, C, F, _ in envs]
+    out = batch.rollout(states, steps)
+    seeds = _seeds(out, 11)
+    loss = sum((o * s).sum() for oe, se in zip(out, seeds) for o, s in zip(oe, se))
+    grads = torch.autograd.grad(loss, [t for st in states for t in st])
+    for e, (kind, x, v, C, F, mid) in enumerate(envs):
+        sim = MPMSim(prms[e], materials(kind), len(x), mat_id=mid, checkpoint_every=every,
+                     sort=True)
+        st = [torch.tensor(a).cuda().requires_grad_(True) for a in (x, v, C, F)]
+        ref = sim.rollout(*st, steps)
+        assert float((ref[0] - st[0]).abs().max()) > 0
+        for got, want in zip(out[e], ref):
+            assert torch.equal(got.detach(), want.detach()), (e, rel_l2(got.detach(), want.detach()))
+        g_ref = torch.autograd.grad(sum((o * s).sum() for o, s in zip(ref, seeds[e])), st)
+        for got, want in zip(grads[4 * e: 4 * e + 4], g_ref):
+            assert torch.equal(got, want), (e, rel_l2(got, want))
+
+
+def test_ragged_batches_reject_the_dense_path(gpu):
+    from difftactile_amd.params import MPMParams
+    prms = [MPMParams(n_grid=g, rho=1000.0, dt=2e-5) for g in (32, 64)]
+    with pytest.raises(ValueError):
+        EnvBatch(prms[0], [(materials("elastic"), 10, None, q) for q in prms], sparse_grid=False)
+    bad = MPMParams(n_grid=64, rho=1000.0, dt=1e-5)
+    with pytest.raises(ValueError):
+        EnvBatch(prms[0], [(materials("elastic"), 10, None, prms[0]),
+                           (materials("elastic"), 10, None, bad)])

@@ -192,9 +192,10 @@ def test_c2_full_size_contact_balance_mass_and_gradient(gpu):
 
 def test_c4_population_batched_matches_the_loop_and_the_oracle(gpu):
     """C4 at full size (64 environments, 5k-200k particles, 64^3 / 128^3
-    grids, a 10,800-tet pad each): one stacked batch per grid size against
-    one simulation per environment, state and d loss / d v0 per environment,
-    over 3 substeps; the smallest environment also against the FP64 oracle."""
+    grids, a 10,800-tet pad each): one stacked batch per grid size, and all
+    64 in one ragged batch, against one simulation per environment, state and
+    d loss / d v0 per environment over 3 substeps (the ragged batch to 1e-6 /
+    1e-5); the smallest environment also against the FP64 oracle."""
     from difftactile_amd.batch import CoupledEnvBatch
     envs = scenes.config4_population(pads=True)
     assert len(envs) == 64
@@ -232,6 +233,19 @@ def test_c4_population_batched_matches_the_loop_and_the_oracle(gpu):
         loss = sum((r[0][0].mean(0) * w[e]).sum() for r, e in zip(res, ids))
         for e, r, ge in zip(ids, res, torch.autograd.grad(loss, [s[1] for s in sts])):
             got_x[e], got_g[e] = r[0][0].detach(), ge
+    # all 64 in one ragged batch: one launch sequence over both grid sizes
+    d0 = envs[0]
+    rb = CoupledEnvBatch(d0["prm"], [(d["mats"], len(d["x"]), d["mat_id"], d["prm"]) for d in envs],
+                         [d["pad_mesh"] for d in envs], dict(out_dir=d0["pad_out"], **d0["pad"]),
+                         contact, checkpoint_every=2, sort=True)
+    assert rb.batch.ragged
+    sts = [state(e) for e in range(64)]
+    ps = [pad_state(e) for e in range(64)]
+    res = rb.rollout(sts, [p[0] for p in ps], [p[1] for p in ps], T)
+    loss = sum((r[0][0].mean(0) * w[e]).sum() for e, r in enumerate(res))
+    rag_g = torch.autograd.grad(loss, [s[1] for s in sts])
+    rag_x = [r[0][0].detach() for r in res]
+    del rb, res
     small = int(np.argmin([len(d["x"]) for d in envs]))
     for e, d in enumerate(envs):
         nodes, f2v, tris, layer = d["pad_mesh"]
@@ -246,6 +260,11 @@ def test_c4_population_batched_matches_the_loop_and_the_oracle(gpu):
         g = torch.autograd.grad((out[0].mean(0) * w[e]).sum(), st[1])[0]
         assert rel_l2(got_x[e], out[0].detach()) < 1e-5, e
         assert rel_l2(got_g[e], g) < 1e-4, e
+        # the pads' vertex sums are exact fixed point at the scale of the
+        # largest term of the whole call, so a batch rounds them at its own
+        # scale: the object sums match bit for bit, the contact sums to rounding
+        assert rel_l2(rag_x[e], out[0].detach()) < 1e-6, e
+        assert rel_l2(rag_g[e], g) < 1e-5, e
         if e != small:
             continue
         # the smallest environment against the oracle
