@@ -56,6 +56,8 @@ def parse():
                     help="CPU baseline: substeps of the trajectory timed fwd+bwd on the host")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad-check", action="store_true")
+    ap.add_argument("--no-population", action="store_true",
+                    help="sphere workload: skip the batched-population line (C4 with pads)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 256 MiB device-to-device copy (the reference "
                          "kernel of known traffic for rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
@@ -264,10 +266,10 @@ def cpu_baseline(prm, mats, mid, state, n_substeps):
     return {"value": n * n_substeps / dt_s, "unit": "particle-substeps/s",
             "cores": cpu.threads(), "kind": "port",
             "sample": f"CPU restatement of the reference's kernels (C++17/OpenMP float32, not "
-                      f"Taichi): fwd then bwd of {n_substeps} substeps of the same {n}-particle "
-                      f"{prm.n_grid}^3 trajectory from the same start state, {cpu.threads()} "
-                      f"threads on {model}; {dt_s:.1f} s",
-            "grad_finite": bool(all(np.isfinite(a).all() for a in g))}
+                      f"Taichi; the reference's svd_grad with its 1e-8 clamp): fwd then bwd of "
+                      f"{n_substeps} substeps of the same {n}-particle {prm.n_grid}^3 trajectory "
+                      f"from the same start state, {cpu.threads()} threads on {model}; "
+                      f"{dt_s:.1f} s"}
 
 
 GC_SCENE = dict(n_particles=20000, n_grid=64)
@@ -480,6 +482,10 @@ def bench_pad_press(args, torch):
 
 
 def bench_population(args, torch):
+    print(json.dumps(population(args, torch)), flush=True)
+
+
+def population(args, torch):
     """Batched population: 64 environments (5k-200k particles, 64^3 or 128^3
     grids) advanced fwd+bwd for --pop-substeps substeps, once batched (one
     ragged EnvBatch over both grid sizes, or with --pop-grouped one per grid
@@ -624,7 +630,7 @@ def bench_population(args, torch):
                                      "its object (penalty contact)" if pads else ""),
                       "groups": {f"{n}^3": len(ids) for n, ids in groups.items()},
                       "layout": layout}}
-    print(json.dumps(res), flush=True)
+    return res
 
 
 def bench_gripper(args, torch):
@@ -879,6 +885,21 @@ def main():
         result["grad_check"] = grad_check(torch, pool)
         pool.close()
         pool.join()
+    if not args.no_population:
+        # the batched population (C4 with pads) beside the headline: the
+        # sphere's buffers go first
+        log("[bench] population (64 environments with pads, batched vs loop)")
+        del step
+        sim.release_buffers()
+        del sim
+        torch.cuda.empty_cache()
+        pargs = argparse.Namespace(**vars(args))
+        pargs.pop_pads, pargs.steps = True, 1
+        pop = population(pargs, torch)
+        result["population"] = {k: pop[k] for k in (
+            "metric", "value", "unit", "ms_per_step", "loop", "speedup_vs_loop",
+            "state_rel_diff_batched_vs_loop_max", "grad_rel_diff_per_env_median")}
+        result["population"]["workload"] = pop["config"]["workload"]
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         host = [t.detach().cpu().numpy() for t in st]
