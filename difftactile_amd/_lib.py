@@ -169,6 +169,10 @@ _SIGS = {
     "dt_jaw_chain_adj": (_c_int, [_P(DtJawParams), _c_int, _vp, _vp, _vp, _vp, _vp]),
     "dt_jaw_control_vel": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _c_float, _vp, _vp]),
     "dt_jaw_control_vel_adj": (_c_int, [_c_int, _c_int, _c_int, _vp, _c_float, _vp, _vp, _vp]),
+    "dt_graph_replay": (_c_int, [_vp, _vp, _c_int, _P(ctypes.c_int32)]),
+    "dt_graph_begin": (_c_int, [_vp, _P(ctypes.c_int32)]),
+    "dt_graph_end": (_c_int, [_vp, _vp, _vp, _c_int]),
+    "dt_graph_abort": (_c_int, [_vp]),
     "dt_set_launch_config": (_c_int, [_P(DtLaunchConfig)]),
     "dt_get_launch_config": (_c_int, [_P(DtLaunchConfig)]),
     "dt_dome_pose_control": (_c_int, [_c_int, _vp, _c_float, _vp, _vp, _vp]),

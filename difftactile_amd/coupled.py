@@ -100,8 +100,14 @@ class CoupledSim:
     cable's d/d pos, d/d vel as the cable task's clamp_grid does."""
 
     def __init__(self, sim: MPMSim, pads, kn, kd, kt, mu_f, norm_eps=1e-11, clamp_grad=True,
-                 rope=None, rope_contact=None):
+                 rope=None, rope_contact=None, graphs=False):
         self.sim = sim
+        # graphs: each substep's forward and adjoint launch sequence is
+        # captured once into a HIP graph and replayed while the pointers it
+        # touches repeat (they do across rollouts of a steady caller); a
+        # caller whose buffers keep moving turns it off after 64 substeps
+        self.graphs = bool(graphs)
+        self.graph_hits = self.graph_misses = 0
         self.pads = list(pads)
         self.rope = rope
         if rope is not None:
@@ -131,6 +137,12 @@ class CoupledSim:
         if rope is not None:
             self.ridx = torch.full((rope.n_particles, self.npad), -1, dtype=I32, device=d)
             self.rimp = torch.zeros(rope.n_particles, 3, device=d)
+        # per-pad buffers of the substeps that keep no store (no allocation
+        # inside a substep: it may be captured)
+        self._vu = [torch.empty(pd.n_verts, 3, device=d) for pd in self.pads]
+        self._ext = [torch.empty(pd.n_verts, 3, device=d) for pd in self.pads]
+        self._ge = [torch.empty(pd.n_verts, 3, device=d) for pd in self.pads]
+        self._last_ext = self._ext
 
     def _node_set(self, base, ws, nb):
         """Copy of the parameter struct base whose grid-node passes run over
@@ -165,32 +177,48 @@ class CoupledSim:
         _lib.call("dt_mpm_bin_slot_bytes", self.sim.c_prm, self.sim.n, _lib.ctypes.byref(nb))
         return nb.value
 
-    def _new_store(self, hint):
-        sim, dev = self.sim, self.sim.device
+    def _pool(self, taken, key, shape, dtype=torch.float32):
+        """A rollout buffer from the simulation's pool (taken records it for
+        the return); None taken: a fresh tensor."""
+        make = lambda: torch.empty(tuple(shape), dtype=dtype, device=self.sim.device)
+        if taken is None:
+            return make()
+        return self.sim._take(taken, ("coupled", id(self)) + tuple(key) + (tuple(shape), dtype),
+                              make)
+
+    def _copy_in(self, taken, key, t):
+        return self._pool(taken, key, t.shape, t.dtype).copy_(t)
+
+    def _new_store(self, hint, taken=None, skey=()):
+        sim = self.sim
         nn = sim.prm.n_nodes
-        return {"g4": torch.empty(nn, 4, device=dev), "gv4": torch.empty(nn, 4, device=dev),
-                "gf": torch.empty(nn, 3, device=dev),
-                "idx": torch.empty(nn, self.npad, dtype=I32, device=dev), "hint": hint,
-                "vel_u": [torch.empty(pd.n_verts, 3, device=dev) for pd in self.pads],
+        b = lambda name, shape, dtype=torch.float32: self._pool(taken, ("store",) + skey + (name,),
+                                                                shape, dtype)
+        return {"g4": b("g4", (nn, 4)), "gv4": b("gv4", (nn, 4)), "gf": b("gf", (nn, 3)),
+                "idx": b("idx", (nn, self.npad), I32), "hint": hint,
+                "vel_u": [b(("vel_u", k), (pd.n_verts, 3)) for k, pd in enumerate(self.pads)],
+                "ext": [b(("ext", k), (pd.n_verts, 3)) for k, pd in enumerate(self.pads)],
                 "ridx": (None if self.rope is None else
-                         torch.empty(self.rope.n_particles, self.npad, dtype=I32, device=dev)),
-                "rimp": (None if self.rope is None else
-                         torch.empty(self.rope.n_particles, 3, device=dev)),
-                "bin": (torch.empty(self._bin_slot_bytes(), dtype=torch.uint8, device=dev)
+                         b("ridx", (self.rope.n_particles, self.npad), I32)),
+                "rimp": (None if self.rope is None else b("rimp", (self.rope.n_particles, 3))),
+                "bin": (b("bin", (self._bin_slot_bytes(),), torch.uint8)
                         if sim.sparse_grid else None)}
 
-    def _run_keep(self, st0, pst0, cvs, s0, k, mat_id):
+    def _run_keep(self, st0, pst0, cvs, s0, k, mat_id, taken=None, seg=()):
         """Advance substeps s0 .. s0+k-1 from (st0, pst0) keeping every frame
         and each substep's store.  Returns (frames, pad frames, stores); st0
-        and pst0 are copied, not kept."""
-        frames = [[t.clone() for t in st0]]
-        pframes = [[[t.clone() for t in p] for p in pst0]]
+        and pst0 are copied, not kept.  taken / seg: the frames and stores
+        come from the buffer pool under keys of segment seg."""
+        fb = lambda j, a, t: self._pool(taken, ("frame",) + seg + (j, a), t.shape, t.dtype)
+        pb = lambda j, b_, c, t: self._pool(taken, ("pframe",) + seg + (j, b_, c), t.shape, t.dtype)
+        frames = [[fb(0, a, t).copy_(t) for a, t in enumerate(st0)]]
+        pframes = [[[pb(0, b_, c, t).copy_(t) for c, t in enumerate(p)] for b_, p in enumerate(pst0)]]
         stores = []
         for j in range(k):
             # closest-triangle starts: the previous substep's result
-            store = self._new_store(stores[-1]["idx"] if stores else self.idx)
-            nxt = [torch.empty_like(t) for t in frames[-1]]
-            pn = [[torch.empty_like(t) for t in p] for p in pframes[-1]]
+            store = self._new_store(stores[-1]["idx"] if stores else self.idx, taken, seg + (j,))
+            nxt = [fb(j + 1, a, t) for a, t in enumerate(frames[-1])]
+            pn = [[pb(j + 1, b_, c, t) for c, t in enumerate(p)] for b_, p in enumerate(pframes[-1])]
             f = s0 + j
             self._fwd(frames[-1], nxt, pframes[-1], pn, [c[f] for c in cvs],
                       [c[f + 1] for c in cvs], mat_id, store)
@@ -199,10 +227,88 @@ class CoupledSim:
             stores.append(store)
         return frames, pframes, stores
 
+    # -- graph capture of a substep -------------------------------------------
+    def _graph_key(self, tag, *objs):
+        words = [tag, id(self)]
+
+        def add(o):
+            if o is None:
+                words.append(0)
+            elif isinstance(o, torch.Tensor):
+                words.extend((o.data_ptr(), o.numel()))
+            elif isinstance(o, dict):
+                for k in sorted(o):
+                    add(o[k])
+            elif isinstance(o, (list, tuple)):
+                words.append(len(o))
+                for t in o:
+                    add(t)
+            else:
+                words.append(int(o) & 0xFFFFFFFFFFFFFFFF)
+
+        for o in objs:
+            add(o)
+        for prm in (self.sim.c_prm, self.c_prm_clamp):
+            b = bytes(prm)
+            b += bytes(-len(b) % 8)
+            words.extend(np.frombuffer(b, np.uint64).tolist())
+        return np.asarray(words, dtype=np.uint64)
+
+    def _warm(self):
+        """Allocate the lazily made buffers a substep uses (before a capture)."""
+        sim = self.sim
+        nn = sim.prm.n_nodes
+        if sim.sparse_grid:
+            sim._workspace()
+        sim._grid_scratch()
+        sim._buf("cgg4", (nn, 4))
+        sim._buf("cggv4", (nn, 4))
+        sim._buf("cggf", (nn, 3))
+
+    def _graphed(self, key, body):
+        main = _lib.stream()
+        hit = _lib.ctypes.c_int32(0)
+        _lib.call("dt_graph_replay", main, key.ctypes.data, len(key), _lib.ctypes.byref(hit))
+        if hit.value:
+            self.graph_hits += 1
+            return
+        self._warm()
+        # captured on a stream of its own (the default stream cannot be);
+        # nothing runs during the capture, the graph then runs on main
+        if getattr(self, "_cap_stream", None) is None:
+            self._cap_stream = torch.cuda.Stream(device=self.sim.device)
+        side = self._cap_stream
+        cap = _lib.ctypes.c_int32(0)
+        _lib.call("dt_graph_begin", side.cuda_stream, _lib.ctypes.byref(cap))
+        if not cap.value:
+            body()
+            return
+        try:
+            with torch.cuda.stream(side):
+                body()
+        except BaseException:
+            _lib.call("dt_graph_abort", side.cuda_stream)
+            raise
+        _lib.call("dt_graph_end", side.cuda_stream, main, key.ctypes.data, len(key))
+        self.graph_misses += 1
+        calls = self.graph_hits + self.graph_misses
+        if calls >= 64 and self.graph_hits < self.graph_misses:
+            self.graphs = False
+
     # -- one substep --------------------------------------------------------
     def _fwd(self, st, nxt, pads_in, pads_out, cv, cv1, mat_id, store=None):
         """st/nxt: [x, v, C, F] SoA; pads_in/out: per pad [pos, vel, vpos];
         store: dict receiving what the adjoint needs (or None)."""
+        exts = store["ext"] if store else self._ext
+        self._last_ext = exts
+        if not self.graphs:
+            return self._fwd_body(st, nxt, pads_in, pads_out, cv, cv1, mat_id, store)
+        key = self._graph_key(1, st, nxt, pads_in, pads_out, cv, cv1, mat_id, store,
+                              self.ridx if self.rope is not None else None)
+        self._graphed(key, lambda: self._fwd_body(st, nxt, pads_in, pads_out, cv, cv1, mat_id,
+                                                  store))
+
+    def _fwd_body(self, st, nxt, pads_in, pads_out, cv, cv1, mat_id, store=None):
         sim, s = self.sim, _lib.stream()
         nn = sim.prm.n_nodes
         prm = sim.c_prm
@@ -235,7 +341,7 @@ class CoupledSim:
         vel_u = []
         for k, pad in enumerate(self.pads):
             pos, vel, _ = pads_in[k]
-            vu = store["vel_u"][k] if store else torch.empty_like(vel)
+            vu = store["vel_u"][k] if store else self._vu[k]
             vu.copy_(vel)
             _lib.call("dt_fem_update_csr", pad.fem, pad.n_tets, pad.n_verts, pad.f2v, pad.B,
                       pos, vu, pad.vert_start, pad.vert_tets, pad.tet_buf, s)
@@ -252,14 +358,14 @@ class CoupledSim:
                       P(pad.cen), P(pad.tri_bounds), P(hint, I32), P(idx, I32), self.npad, k, s)
         if not sim.sparse_grid:
             gf.zero_()
-        exts = []
+        exts = store["ext"] if store else self._ext
         for k, pad in enumerate(self.pads):
-            ext = torch.zeros(pad.n_verts, 3, device=sim.device)
+            ext = exts[k]
+            ext.zero_()
             _lib.call("dt_contact_collision_fx", prm, self.cp, P(g4), 4, g4.data_ptr() + 12, 4,
                       P(idx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32), P(pads_in[k][0]),
                       P(vel_u[k]), pad.od(), P(gf), pad.n_verts, P(ext),
                       pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
-            exts.append(ext)
         npd = len(self.pads)
         if self.rope is not None:
             # cable - pad contact (the cable's state follows the pads' in pads_in)
@@ -275,10 +381,6 @@ class CoupledSim:
                           P(ridx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32),
                           P(pads_in[k][0]), P(vel_u[k]), pad.od(), P(rimp), pad.n_verts,
                           P(exts[k]), None, pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
-        if store is not None:
-            store["ext"] = exts
-        # the pads' external forces of the latest substep (rollout(return_ext=True))
-        self._last_ext = exts
         _lib.call("dt_mpm_grid_op4", prm, P(g4), P(gf), P(gv4), s)
         _lib.call("dt_mpm_g2p4", prm, sim.n, P(st[0]), P(gv4), P(nxt[0]), P(nxt[1]), P(nxt[2]), s)
         if self.rope is not None:
@@ -298,6 +400,15 @@ class CoupledSim:
         writes g (object, overwritten), g_pads (overwritten), accumulates g_cv, g_cv1;
         g_ext_out: adjoints of this substep's pad external forces as outputs
         (or None)."""
+        args = (st, pads_in, store, g_next, g_pads_next, g, g_pads, g_cv, g_cv1, mat_id,
+                g_ext_out)
+        if not self.graphs:
+            return self._adj_body(*args)
+        key = self._graph_key(2, *args)
+        self._graphed(key, lambda: self._adj_body(*args))
+
+    def _adj_body(self, st, pads_in, store, g_next, g_pads_next, g, g_pads, g_cv, g_cv1, mat_id,
+                  g_ext_out=None):
         sim, s = self.sim, _lib.stream()
         nn = sim.prm.n_nodes
         prm = sim.c_prm
@@ -310,7 +421,7 @@ class CoupledSim:
         g_ext = []
         for k, pad in enumerate(self.pads):
             gp, gvu, gvp = g_pads[k]
-            ge = torch.empty(pad.n_verts, 3, device=sim.device)
+            ge = self._ge[k]
             # the adjoint kernels below accumulate: one launch clears all four
             torch._foreach_zero_([gp, gvu, gvp, ge])
             gpn, gvn, gvpn = g_pads_next[k]
@@ -414,47 +525,62 @@ class CoupledSim:
 
 
 class _CoupledRollout(torch.autograd.Function):
+    # Buffers come from the simulation's pool and return to it after the
+    # backward pass (or at the end of a forward pass without one), so the
+    # next rollout runs on the same addresses: graphed substeps replay.
     @staticmethod
     def forward(ctx, cs, n_substeps, return_ext, *flat):
         sim = cs.sim
         npd = len(cs.pads)
+        taken = []
+        P = lambda key, t: cs._copy_in(taken, key, t)
         x, v, C, F = (t.contiguous() for t in flat[:4])
-        pads0 = [[t.contiguous().clone() for t in flat[4 + 3 * k: 7 + 3 * k]] for k in range(npd)]
-        cvs = [t.contiguous() for t in flat[4 + 3 * npd: 4 + 4 * npd]]
+        pads0 = [[P(("p0", k, c), flat[4 + 3 * k + c].contiguous()) for c in range(3)]
+                 for k in range(npd)]
+        cvs = [P(("cv", k), t.contiguous()) for k, t in enumerate(flat[4 + 3 * npd: 4 + 4 * npd])]
         if cs.rope is not None:
             # the cable's state rides along as one more body after the pads
-            pads0.append([t.contiguous().float().clone() for t in flat[4 + 4 * npd:]])
+            pads0.append([P(("r0", c), t.contiguous().float())
+                          for c, t in enumerate(flat[4 + 4 * npd:])])
         shapes = [t.shape for t in (x, v, C, F)]
         W = (3, 3, 9, 9)
         perm = sim.sort_order(x)[1] if (sim.sort and sim.n > 0) else None
-        st = [sim._gather(perm, t, w) for t, w in zip((x, v, C, F), W)]
+        st = [sim._gather(perm, t, w, out=cs._pool(taken, ("st", a), (w, sim.n)))
+              for a, (t, w) in enumerate(zip((x, v, C, F), W))]
         if sim.mat_id is not None and perm is not None:
-            mat_id = sim._gather_i(perm, sim.mat_id)
+            mat_id = sim._gather_i(perm, sim.mat_id, out=cs._pool(taken, ("mid",), (sim.n,), I32))
         else:
             mat_id = sim.mat_id
         env = sim.env_id
         if env is not None and perm is not None:
-            env = sim._gather_i(perm, env)
+            env = sim._gather_i(perm, env, out=cs._pool(taken, ("env",), (sim.n,), I32))
         sim._set_env(env)
-        nxt = [torch.empty_like(t) for t in st]
+        nxt = [cs._pool(taken, ("nx", a), t.shape) for a, t in enumerate(st)]
         pst = pads0
-        pnx = [[torch.empty_like(t) for t in p] for p in pst]
+        pnx = [[cs._pool(taken, ("pn", k, c), t.shape, t.dtype) for c, t in enumerate(p)]
+               for k, p in enumerate(pst)]
         ckpts, kept = [], []
         keep_ok = any(ctx.needs_input_grad)
         used = 0
-        for (s0, k) in _segments(n_substeps, sim.checkpoint_every):
+        for i, (s0, k) in enumerate(_segments(n_substeps, sim.checkpoint_every)):
             nbytes = (k + 1) * cs.substep_bytes()
             if keep_ok and used + nbytes <= sim.resident_bytes:
                 # resident segment: frames and stores kept for the backward pass
                 used += nbytes
-                seg = cs._run_keep(st, pst, cvs, s0, k, mat_id)
+                seg = cs._run_keep(st, pst, cvs, s0, k, mat_id, taken, ("seg", i))
                 ckpts.append(None)
                 kept.append(seg)
-                st = [t.clone() for t in seg[0][-1]]
-                pst = [[t.clone() for t in p] for p in seg[1][-1]]
+                st = [P(("sc", a), t) for a, t in enumerate(seg[0][-1])]
+                pst = [[P(("psc", b_, c), t) for c, t in enumerate(p)]
+                       for b_, p in enumerate(seg[1][-1])]
                 cs.idx.copy_(seg[2][-1]["idx"])
                 continue
-            ckpts.append(([t.clone() for t in st], [[t.clone() for t in p] for p in pst]))
+            if keep_ok:
+                ckpts.append(([P(("ck", i, a), t) for a, t in enumerate(st)],
+                              [[P(("ckp", i, b_, c), t) for c, t in enumerate(p)]
+                               for b_, p in enumerate(pst)]))
+            else:
+                ckpts.append(None)
             kept.append(None)
             for f in range(s0, s0 + k):
                 cs._fwd(st, nxt, pst, pnx, [c[f] for c in cvs], [c[f + 1] for c in cvs], mat_id)
@@ -470,7 +596,10 @@ class _CoupledRollout(torch.autograd.Function):
         ctx.kept = kept
         ctx.perm, ctx.mat_id, ctx.env = perm, mat_id, env
         ctx.cv_shapes = [c.shape for c in cvs]
-        ctx.save_for_backward(*cvs)
+        ctx.cvs = cvs
+        ctx.taken = taken
+        if not keep_ok:
+            sim._give_back(taken)
         return tuple(out)
 
     @staticmethod
@@ -478,28 +607,38 @@ class _CoupledRollout(torch.autograd.Function):
         cs = ctx.cs
         sim = cs.sim
         npd = len(cs.pads)
-        cvs = ctx.saved_tensors
+        cvs = ctx.cvs
+        taken = ctx.taken
         W = (3, 3, 9, 9)
         dev = sim.device
-        g = [sim._gather(ctx.perm, (t if t is not None else torch.zeros(sh, device=dev)).float(), w)
-             for t, w, sh in zip(gout[:4], W, ctx.shapes)]
+        n = sim.n
+
+        def pooled(key, t, shape, dtype=torch.float32):
+            b = cs._pool(taken, key, shape, dtype)
+            return b.zero_() if t is None else b.copy_(t)
+
+        g = [sim._gather(ctx.perm, (t if t is not None else torch.zeros(sh, device=dev)).float(), w,
+                         out=cs._pool(taken, ("g", a), (w, n)))
+             for a, (t, w, sh) in enumerate(zip(gout[:4], W, ctx.shapes))]
         gp = []
         for k in range(npd):
             trio = gout[4 + 3 * k: 7 + 3 * k]
-            gp.append([(t if t is not None else torch.zeros(cs.pads[k].n_verts, 3, device=dev)
-                        ).contiguous().clone() for t in trio])
+            gp.append([pooled(("gp", k, c), t, (cs.pads[k].n_verts, 3))
+                       for c, t in enumerate(trio)])
         o = 4 + 3 * npd
         if cs.rope is not None:
             nr = cs.rope.n_particles
-            gp.append([(t if t is not None else torch.zeros(nr, 3, device=dev)
-                        ).contiguous().float().clone() for t in gout[o:o + 2]])
+            gp.append([pooled(("gr", c), None if t is None else t.float(), (nr, 3))
+                       for c, t in enumerate(gout[o:o + 2])])
             o += 2
         g_ext_out = None
         if ctx.return_ext:
-            g_ext_out = [None if t is None else t.float().contiguous() for t in gout[o:o + npd]]
-        g_cv = [torch.zeros(sh, device=dev) for sh in ctx.cv_shapes]
-        gbuf = [torch.empty_like(t) for t in g]
-        gpbuf = [[torch.empty_like(t) for t in p] for p in gp]
+            g_ext_out = [None if t is None else pooled(("gext", k), t.float(), t.shape)
+                         for k, t in enumerate(gout[o:o + npd])]
+        g_cv = [pooled(("gcv", k), None, sh) for k, sh in enumerate(ctx.cv_shapes)]
+        gbuf = [cs._pool(taken, ("gb", a), t.shape) for a, t in enumerate(g)]
+        gpbuf = [[cs._pool(taken, ("gpb", k, c), t.shape) for c, t in enumerate(p)]
+                 for k, p in enumerate(gp)]
         segs = _segments(ctx.n_substeps, sim.checkpoint_every)
         sim._set_env(ctx.env)
         for i in reversed(range(len(segs))):
@@ -514,7 +653,8 @@ class _CoupledRollout(torch.autograd.Function):
                                        "gradient")
                 # recompute the segment, keeping what the adjoint needs
                 ck, ckp = ctx.ckpts[i]
-                frames, pframes, stores = cs._run_keep(ck, ckp, cvs, s0, k, ctx.mat_id)
+                frames, pframes, stores = cs._run_keep(ck, ckp, cvs, s0, k, ctx.mat_id, taken,
+                                                       ("seg", i))
             for j in reversed(range(k)):
                 f = s0 + j
                 cs._adj(frames[j], pframes[j], stores[j], g, gp, gbuf, gpbuf,
@@ -524,8 +664,10 @@ class _CoupledRollout(torch.autograd.Function):
                 gp, gpbuf = gpbuf, gp
         res = [sim._scatter(ctx.perm, t, w, sh) for t, w, sh in zip(g, W, ctx.shapes)]
         for p in gp[:npd]:
-            res += list(p)
-        res += g_cv
+            res += [t.clone() for t in p]
+        res += [t.clone() for t in g_cv]
         for p in gp[npd:]:
-            res += list(p)
+            res += [t.clone() for t in p]
+        sim._give_back(taken)
+        ctx.taken = []
         return (None, None, None, *res)

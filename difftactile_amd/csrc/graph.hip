@@ -96,6 +96,23 @@ int run_graphed(std::vector<uint64_t> key, hipStream_t st,
   return DT_OK;
 }
 
+namespace {
+// graphs captured around a caller's own launch sequence (dt_graph_begin /
+// dt_graph_end): keys start with this tag, apart from run_graphed's
+constexpr uint64_t kCallerTag = 0x43414c4c45524b59ull;
+std::vector<uint64_t> caller_key(const uint64_t* key, int n, int dev) {
+  std::vector<uint64_t> k;
+  k.reserve(n + 4);
+  k.push_back(kCallerTag);
+  for (int i = 0; i < n; ++i) k.push_back(key[i]);
+  k.push_back((uint64_t)dev);
+  const dt_launch_config lc = launch_config();
+  k.push_back(((uint64_t)(uint32_t)lc.brick_blocks << 32) | (uint32_t)lc.node_blocks);
+  k.push_back((uint64_t)(uint32_t)lc.sort_blocks);
+  return k;
+}
+}  // namespace
+
 dt_launch_config launch_config() {
   dt_launch_config c;
   c.brick_blocks = g_brick_blocks.load();
@@ -116,6 +133,83 @@ extern "C" {
 int dt_graphs_enable(int32_t on) {
   std::lock_guard<std::mutex> lock(g_mu);
   g_enabled = on != 0;
+  return DT_OK;
+}
+
+int dt_graph_replay(void* stream, const uint64_t* key, int32_t n_key, int32_t* hit) {
+  DT_REQUIRE(hit && (key || n_key == 0) && n_key >= 0, "bad key");
+  *hit = 0;
+  if (!g_enabled || timing_enabled()) return DT_OK;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return DT_OK;
+  const std::vector<uint64_t> k = caller_key(key, n_key, dev);
+  std::lock_guard<std::mutex> lock(g_mu);
+  for (auto it = g_cache.begin(); it != g_cache.end(); ++it) {
+    if (it->key == k) {
+      g_cache.splice(g_cache.begin(), g_cache, it);
+      ++g_hits;
+      if (hipGraphLaunch(g_cache.front().exec, (hipStream_t)stream) != hipSuccess) {
+        set_error("hipGraphLaunch failed");
+        return DT_ELAUNCH;
+      }
+      *hit = 1;
+      return DT_OK;
+    }
+  }
+  return DT_OK;
+}
+
+int dt_graph_begin(void* stream, int32_t* capturing) {
+  DT_REQUIRE(capturing, "NULL output");
+  *capturing = 0;
+  if (!g_enabled || timing_enabled()) return DT_OK;
+  if (hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return DT_OK;
+  }
+  *capturing = 1;
+  return DT_OK;
+}
+
+int dt_graph_end(void* stream, void* launch_stream, const uint64_t* key, int32_t n_key) {
+  DT_REQUIRE((key || n_key == 0) && n_key >= 0, "bad key");
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g);
+  if (e != hipSuccess || g == nullptr) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    set_error("graph capture failed");
+    return DT_ELAUNCH;
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    set_error("hipGraphInstantiate failed");
+    return DT_ELAUNCH;
+  }
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(g_mu);
+  ++g_misses;
+  g_cache.push_front({caller_key(key, n_key, dev), exec});
+  while (g_cache.size() > kMaxGraphs) {
+    (void)hipGraphExecDestroy(g_cache.back().exec);
+    g_cache.pop_back();
+  }
+  // the captured work has not run yet
+  if (hipGraphLaunch(exec, (hipStream_t)launch_stream) != hipSuccess) {
+    set_error("hipGraphLaunch failed");
+    return DT_ELAUNCH;
+  }
+  return DT_OK;
+}
+
+int dt_graph_abort(void* stream) {
+  hipGraph_t g = nullptr;
+  (void)hipStreamEndCapture((hipStream_t)stream, &g);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
   return DT_OK;
 }
 

@@ -154,6 +154,23 @@ int dt_timing_read(int32_t stage, double* total_ms, int64_t* launches);
 int dt_graphs_enable(int32_t on);
 int dt_graph_stats(int64_t* hits, int64_t* misses);
 
+/* Graphs of a caller's own launch sequence (no reference counterpart):
+   dt_graph_replay launches the graph cached under key (n_key words; the
+   caller puts every pointer and value its sequence depends on in it) and
+   sets *hit, or sets *hit = 0.  On a miss the caller brackets its launches,
+   issued on a capture stream of its own (not the legacy default stream),
+   with dt_graph_begin (captures that stream; *capturing = 0 when graphs are
+   off: then run the launches eagerly and skip dt_graph_end) and
+   dt_graph_end (instantiates, caches under key, launches the graph once on
+   launch_stream).  The sequence may contain kernels only: no allocation,
+   memset, copy or host synchronisation between the two calls. */
+int dt_graph_replay(void* stream, const uint64_t* key, int32_t n_key, int32_t* hit);
+int dt_graph_begin(void* stream, int32_t* capturing);
+int dt_graph_end(void* stream, void* launch_stream, const uint64_t* key, int32_t n_key);
+/* ends a capture begun by dt_graph_begin and discards it (the caller's
+   launches failed) */
+int dt_graph_abort(void* stream);
+
 /* ---- launch shapes (no reference counterpart: the autotuner's knobs) ----
    Process-wide grid sizes of the persistent launches; 0 keeps the default.
    brick_blocks: blocks of the brick-owned scatter / gather passes (default
