@@ -643,6 +643,7 @@ class _CoupledRollout(torch.autograd.Function):
         sim._set_env(ctx.env)
         for i in reversed(range(len(segs))):
             s0, k = segs[i]
+            seg_taken = None
             if ctx.kept[i] is not None:
                 frames, pframes, stores = ctx.kept[i]
                 ctx.kept[i] = None
@@ -651,10 +652,12 @@ class _CoupledRollout(torch.autograd.Function):
                     raise RuntimeError("a coupled rollout's backward runs once (it releases the "
                                        "segments it kept); run the rollout again for another "
                                        "gradient")
-                # recompute the segment, keeping what the adjoint needs
+                # recompute the segment, keeping what the adjoint needs (one
+                # set of recompute buffers, back to the pool after each segment)
                 ck, ckp = ctx.ckpts[i]
-                frames, pframes, stores = cs._run_keep(ck, ckp, cvs, s0, k, ctx.mat_id, taken,
-                                                       ("seg", i))
+                seg_taken = []
+                frames, pframes, stores = cs._run_keep(ck, ckp, cvs, s0, k, ctx.mat_id,
+                                                       seg_taken, ("recompute",))
             for j in reversed(range(k)):
                 f = s0 + j
                 cs._adj(frames[j], pframes[j], stores[j], g, gp, gbuf, gpbuf,
@@ -662,6 +665,9 @@ class _CoupledRollout(torch.autograd.Function):
                         g_ext_out if f == ctx.n_substeps - 1 else None)
                 g, gbuf = gbuf, g
                 gp, gpbuf = gpbuf, gp
+            if seg_taken is not None:
+                del frames, pframes, stores
+                sim._give_back(seg_taken)
         res = [sim._scatter(ctx.perm, t, w, sh) for t, w, sh in zip(g, W, ctx.shapes)]
         for p in gp[:npd]:
             res += [t.clone() for t in p]
