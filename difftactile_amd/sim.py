@@ -257,12 +257,31 @@ class MPMSim:
         return None if mat_id is None else _lib.ptr(mat_id, torch.int32)
 
     def segment_bytes(self, k):
-        """HBM bytes of one k-substep segment's frames and grids."""
-        return 4 * (self.n * 24 * (k + 1) + 2 * 4 * k * self.prm.n_nodes)
+        """HBM bytes of one resident k-substep segment: frames, grids and (on
+        the sparse path) its own workspace with the binning history."""
+        b = 4 * (self.n * 24 * (k + 1) + 2 * 4 * k * self.prm.n_nodes)
+        if self.sparse_grid:
+            b += workspace_bytes(self.c_prm, self.n, k)
+        return b
 
-    def run_frames(self, X, V, C, F, k, mat_id, keep_grids, grids=None):
+    def segment_workspace(self, k):
+        """A workspace with room for the binning history of a k-substep
+        segment (a resident segment keeps its own, so its adjoint sweep reuses
+        the forward binning after later segments ran)."""
+        if not self.sparse_grid:
+            return None
+        return torch.empty(workspace_bytes(self.c_prm, self.n, k), dtype=torch.uint8,
+                           device=self.device)
+
+    def _ws_args(self, ws, history_substeps):
+        if ws is None:
+            return self._workspace(history_substeps)
+        return _lib.ptr(ws, torch.uint8), ws.numel()
+
+    def run_frames(self, X, V, C, F, k, mat_id, keep_grids, grids=None, ws=None):
         """Advance frames 0..k of frame-major buffers X[k+1,n,3] ... in place.
-        keep_grids: keep every substep's grids (in `grids` when given)."""
+        keep_grids: keep every substep's grids (in `grids` when given); ws: the
+        workspace to bin in (default the simulation's)."""
         nn = self.prm.n_nodes
         if keep_grids:
             if grids is not None:
@@ -279,7 +298,7 @@ class MPMSim:
         _lib.call("dt_mpm_run_fwd", self.c_prm, _lib.ptr(self.mats), len(self.materials), self.n,
                   self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
                   g4p, gv4p, _lib.ptr(s4), _lib.ptr(sv4),
-                  *self._workspace(k if keep_grids else 0), _lib.stream())
+                  *self._ws_args(ws, k if keep_grids else 0), _lib.stream())
         return G4, GV4
 
     def advance(self, state, k, mat_id):
@@ -290,8 +309,9 @@ class MPMSim:
                   self._mid(mat_id), *[_lib.ptr(t) for t in state], *[_lib.ptr(t) for t in tmp],
                   int(k), _lib.ptr(s4), _lib.ptr(sv4), *self._workspace(), _lib.stream())
 
-    def adj_frames(self, X, V, C, F, k, mat_id, G4, GV4, g):
-        """Reverse sweep over k substeps; g = [gx, gv, gC, gF] seeded at frame k, in place."""
+    def adj_frames(self, X, V, C, F, k, mat_id, G4, GV4, g, ws=None):
+        """Reverse sweep over k substeps; g = [gx, gv, gC, gF] seeded at frame k, in place;
+        ws: the workspace the forward binned in (its history is reused)."""
         nn = self.prm.n_nodes
         gg4 = self._buf("gg4", (nn, 4))
         ggv4 = self._buf("ggv4", (nn, 4))
@@ -300,7 +320,7 @@ class MPMSim:
                   self._mid(mat_id), _lib.ptr(X), _lib.ptr(V), _lib.ptr(C), _lib.ptr(F), k,
                   _lib.ptr(G4), _lib.ptr(GV4), _lib.ptr(gg4), _lib.ptr(ggv4),
                   *[_lib.ptr(t) for t in g], *[_lib.ptr(t) for t in scratch],
-                  *self._workspace(k), self._mat_grad_ptr(), _lib.stream())
+                  *self._ws_args(ws, k), self._mat_grad_ptr(), _lib.stream())
 
     def _frames(self, k, prefix="F"):
         n = self.n
@@ -372,14 +392,15 @@ class _Rollout(torch.autograd.Function):
                 fr = sim._take(taken, ("frames", i, k), lambda: (
                     [torch.empty((k + 1, w, n), device=sim.device) for w in W],
                     (torch.empty((k, nn, 4), device=sim.device),
-                     torch.empty((k, nn, 4), device=sim.device))))
-                frames, grids = fr
+                     torch.empty((k, nn, 4), device=sim.device)),
+                    sim.segment_workspace(k)))
+                frames, grids, seg_ws = fr
                 for f, t in zip(frames, state):
                     f[0].copy_(t)
-                sim.run_frames(*frames, k, mat_id, keep_grids=True, grids=grids)
+                sim.run_frames(*frames, k, mat_id, keep_grids=True, grids=grids, ws=seg_ws)
                 for f, t in zip(frames, state):
                     t.copy_(f[k])
-                ckpts.append((frames, grids))
+                ckpts.append((frames, grids, seg_ws))
             else:
                 if ctx is not None:
                     ck = sim._take(taken, ("ckpt", i), lambda: sim._state_bufs(W))
@@ -416,13 +437,14 @@ class _Rollout(torch.autograd.Function):
         for (s, k), ck, (mat_id, env, q) in zip(reversed(segs), reversed(ctx.ckpts),
                                                 reversed(ctx.meta)):
             sim._set_env(env)
+            seg_ws = None
             if isinstance(ck, tuple):           # resident segment
-                (X, V, Cb, Fb), (G4, GV4) = ck
+                (X, V, Cb, Fb), (G4, GV4), seg_ws = ck
             else:                               # recompute from the checkpoint
                 X, V, Cb, Fb = sim._frames(k, "R%d" % k)
                 X[0].copy_(ck[0]); V[0].copy_(ck[1]); Cb[0].copy_(ck[2]); Fb[0].copy_(ck[3])
                 G4, GV4 = sim.run_frames(X, V, Cb, Fb, k, mat_id, keep_grids=True)
-            sim.adj_frames(X, V, Cb, Fb, k, mat_id, G4, GV4, g)
+            sim.adj_frames(X, V, Cb, Fb, k, mat_id, G4, GV4, g, ws=seg_ws)
             if q is not None:
                 # back to the order before this segment's re-sort
                 cur = 1 - cur
