@@ -191,6 +191,8 @@ _SIGS = {
     "dt_rope_step": (_c_int, [_P(DtRopeParams), _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dt_rope_step_adj": (_c_int, [_P(DtRopeParams), _c_int] + [_vp] * 10 + [_vp]),
     "dt_rope_closest": (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _c_int, _c_int, _vp]),
+    "dt_rope_closest_batched": (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _c_int, _c_int,
+                                         _vp]),
     "dt_rope_collision": (_c_int, [_P(DtContactParams), _c_float, _c_int, _vp, _vp, _vp, _c_int,
                                    _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dt_rope_collision_adj": (_c_int, [_P(DtContactParams), _c_float, _c_int, _vp, _vp, _vp,

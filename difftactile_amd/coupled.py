@@ -375,8 +375,10 @@ class CoupledSim:
             rimp = store["rimp"] if store else self.rimp
             rimp.zero_()
             for k, pad in enumerate(self.pads):
-                _lib.call("dt_rope_closest", nr, P(rpos), pad.n_tris, P(pad.cen), P(ridx, I32),
-                          self.npad, k, s)
+                # the entries' previous contents seed the search (any start
+                # gives the same result)
+                _lib.call("dt_rope_closest_batched", nr, P(rpos), pad.n_tris, P(pad.cen),
+                          P(pad.tri_bounds), P(ridx, I32), P(ridx, I32), self.npad, k, s)
                 _lib.call("dt_rope_collision_fx", self.rope_cp, sim.prm.dt, nr, P(rpos), P(rvel),
                           P(ridx, I32), self.npad, k, pad.n_tris, P(pad.tris, I32),
                           P(pads_in[k][0]), P(vel_u[k]), pad.od(), P(rimp), pad.n_verts,

@@ -759,18 +759,66 @@ __global__ __launch_bounds__(kBlock) void k_collision_adj_fx2(
 // particle-wise contact of a rope with one pad (cable_straightening.py:287-319)
 // ---------------------------------------------------------------------------
 // find_closest with the particle position as the query (strict < from
-// (100, -1) in index order)
-__global__ __launch_bounds__(kBlock) void k_rope_closest(int n, const float* __restrict__ x,
-                                                         int nt, const float4* __restrict__ cen,
-                                                         int32_t* __restrict__ idx, int stride,
-                                                         int col) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = p < n;
-  const float qx = live ? x[3 * p] : 0.f, qy = live ? x[3 * p + 1] : 0.f,
-              qz = live ? x[3 * p + 2] : 0.f;
-  Closest r{100.0f, 1e4f, 1e4f * (1.0f + 1e-6f), -1};
-  for (int t = 0; t < nt; ++t) closest_step<false>(cen[t], t, 0, nt, qx, qy, qz, r);
-  if (live) idx[(long long)p * stride + col] = r.best_k;
+// (100, -1) in index order: the lexicographic minimum of (distance, index)),
+// searched as k_closest searches nodes: a block takes 64 particles at a time,
+// its eight wavefronts each scan an eighth of the triangle runs at
+// wave-uniform indices, runs beyond every lane's best (kBounds) and groups of
+// four beyond a lane's best are skipped, and the parts' minima are combined
+// in LDS.  hint as for k_closest (it may be idx itself: every read of a
+// particle's entry precedes the barrier before its write).
+template <bool kBounds>
+__global__ __launch_bounds__(kClosestThreads) void k_point_closest(
+    int n, const float* __restrict__ x, int nt, const float4* __restrict__ cen,
+    const float4* __restrict__ bounds, const int32_t* hint, int32_t* idx, int stride, int col) {
+  __shared__ float s_best[kClosestWaves][64];
+  __shared__ int s_k[kClosestWaves][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n_chunks = (n + 63) / 64;
+  const int n_runs = (nt + kTriBatch - 1) / kTriBatch;
+  const int per = (n_runs + kClosestWaves - 1) / kClosestWaves;
+  const int wb0 = min(w * per, n_runs), wb1 = min(wb0 + per, n_runs);
+  for (int ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
+    const int p = ch * 64 + lane;
+    const bool live = p < n;
+    const float qx = live ? x[3ll * p] : 0.f, qy = live ? x[3ll * p + 1] : 0.f,
+                qz = live ? x[3ll * p + 2] : 0.f;
+    const float qb = kBoundShrink * ((qx * qx + qy * qy) + qz * qz);
+    Closest r{100.0f, 1e4f, 1e4f * (1.0f + 1e-6f), -1};
+    if (hint && live) {
+      const int h = hint[(long long)p * stride + col];
+      if (h >= 0 && h < nt) closest_step<false>(cen[h], h, 0, nt, qx, qy, qz, r);
+    }
+    for (int b = wb0; b < wb1; ++b) {
+      const int t0 = b * kTriBatch, t1 = min(t0 + kTriBatch, nt);
+      if (kBounds) {
+        // the skip test of k_closest, over the live lanes
+        const float4 s4 = bounds[b];
+        const float d0 = s4.x - qx, d1 = s4.y - qy, d2 = s4.z - qz;
+        const float dd = d0 * d0 + d1 * d1 + d2 * d2;
+        const float reach = s4.w + r.best * (1.0f + 1e-5f);
+        if (__ballot(live && !(dd > reach * reach)) == 0ull) continue;
+      }
+      if (t1 - t0 == kTriBatch)
+        closest_batch<false>(cen, t0, 0, nt, qx, qy, qz, qb, r);
+      else
+        for (int t = t0; t < t1; ++t) closest_step<false>(cen[t], t, 0, nt, qx, qy, qz, r);
+    }
+    s_best[w][lane] = r.best;
+    s_k[w][lane] = r.best_k;
+    __syncthreads();
+    if (w == 0 && live) {
+      float bd = 100.0f;
+      int bk = -1;
+      for (int q = 0; q < kClosestWaves; ++q)
+        if (closest_before(s_best[q][lane], s_k[q][lane], bd, bk)) {
+          bd = s_best[q][lane];
+          bk = s_k[q][lane];
+        }
+      idx[(long long)p * stride + col] = bk;
+    }
+    __syncthreads();
+  }
 }
 
 // rope impulse += f dt, pad external force += -f/3 per triangle vertex,
@@ -1210,16 +1258,39 @@ int dt_rope_collision_adj_fx(const dt_contact_params* cp, float dt, int32_t n, c
   return check_launch("dt_rope_collision_adj_fx");
 }
 
-int dt_rope_closest(int32_t n, const float* x, int32_t n_tris, const float* centroids4,
-                    int32_t* contact_idx, int32_t idx_stride, int32_t idx_col, void* stream) {
+static int point_closest_launch(int32_t n, const float* x, int32_t n_tris,
+                                const float* centroids4, const float* bounds4,
+                                const int32_t* hint, int32_t* contact_idx, int32_t idx_stride,
+                                int32_t idx_col, void* stream, const char* name) {
   DT_REQUIRE(n >= 0 && n_tris >= 0 && idx_stride > 0 && idx_col >= 0 && idx_col < idx_stride,
              "bad size/stride");
   if (n == 0) return DT_OK;
   DT_REQUIRE(x && contact_idx && (n_tris == 0 || centroids4), "NULL pointer");
-  hipLaunchKernelGGL(k_rope_closest, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream,
-                     (int)n, x, (int)n_tris, (const float4*)centroids4, contact_idx, idx_stride,
-                     idx_col);
-  return check_launch("dt_rope_closest");
+  const int blocks = (int)std::min<long long>((n + 63ll) / 64, 4096);
+  if (bounds4 && n_tris > 0)
+    hipLaunchKernelGGL(k_point_closest<true>, dim3(blocks), dim3(kClosestThreads), 0,
+                       (hipStream_t)stream, (int)n, x, (int)n_tris, (const float4*)centroids4,
+                       (const float4*)bounds4, hint, contact_idx, idx_stride, idx_col);
+  else
+    hipLaunchKernelGGL(k_point_closest<false>, dim3(blocks), dim3(kClosestThreads), 0,
+                       (hipStream_t)stream, (int)n, x, (int)n_tris, (const float4*)centroids4,
+                       (const float4*)nullptr, hint, contact_idx, idx_stride, idx_col);
+  return check_launch(name);
+}
+
+int dt_rope_closest(int32_t n, const float* x, int32_t n_tris, const float* centroids4,
+                    int32_t* contact_idx, int32_t idx_stride, int32_t idx_col, void* stream) {
+  return point_closest_launch(n, x, n_tris, centroids4, nullptr, nullptr, contact_idx, idx_stride,
+                              idx_col, stream, "dt_rope_closest");
+}
+
+int dt_rope_closest_batched(int32_t n, const float* x, int32_t n_tris, const float* centroids4,
+                            const float* batch_bounds4, const int32_t* hint,
+                            int32_t* contact_idx, int32_t idx_stride, int32_t idx_col,
+                            void* stream) {
+  DT_REQUIRE(n_tris == 0 || batch_bounds4, "NULL batch bounds");
+  return point_closest_launch(n, x, n_tris, centroids4, batch_bounds4, hint, contact_idx,
+                              idx_stride, idx_col, stream, "dt_rope_closest_batched");
 }
 
 int dt_rope_collision(const dt_contact_params* cp, float dt, int32_t n, const float* x,

@@ -827,6 +827,14 @@ int dt_rope_step_adj(const dt_rope_params* prm, int32_t n, const float* fixed_po
    closest triangle to each particle position, as dt_contact_closest. */
 int dt_rope_closest(int32_t n, const float* x, int32_t n_tris, const float* centroids4,
                     int32_t* contact_idx, int32_t idx_stride, int32_t idx_col, void* stream);
+/* dt_rope_closest with the run spheres of dt_contact_tri_batches (written for
+   these centroids) and an optional hint per particle, as
+   dt_contact_closest_batched: same result, most runs skipped.  hint may be
+   contact_idx itself (its previous contents) or NULL. */
+int dt_rope_closest_batched(int32_t n, const float* x, int32_t n_tris, const float* centroids4,
+                            const float* batch_bounds4, const int32_t* hint,
+                            int32_t* contact_idx, int32_t idx_stride, int32_t idx_col,
+                            void* stream);
 /* collision for rope particles (294-319) with one pad: cp->sdf_offset is the
    particle radius; impulse [n][3] += f dt, ext_force (pad) += -f/3 per
    triangle vertex, contact_flag[p] = 1 where the particle touches (NULL:

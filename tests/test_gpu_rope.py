@@ -133,3 +133,63 @@ def test_rope_pad_rollout_gradient_matches_the_oracle(gpu):
     gvo, gpo = torch.autograd.grad(lo, [vo0, po])
     assert rel_l2(g_v.cpu().double(), gvo) < 1e-3
     assert rel_l2(g_p.cpu().double(), gpo) < 1e-3
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_rope_closest_ties_far_particles_and_hints(gpu, batched):
+    """Closest triangle per particle, bit-exact against the oracle's float32
+    brute force: particles on the pad's lattice (the two triangles of a cell
+    are equidistant from many of them: the first minimum must win), particles
+    far from every triangle (-1 beyond 100), a count that is no multiple of
+    64; the search that skips runs by their bounding spheres gives the same
+    answer from arbitrary hints and from its own previous result."""
+    from difftactile_amd import _lib
+    h = 0.02
+    nodes, _, tris, _, _ = slab_mesh(20, 14, 3, h=h, origin=(-0.2, 0.06, -0.1))
+    nodes = nodes.astype(np.float32)
+    tris = tris.astype(np.int32)
+    rng = np.random.default_rng(11)
+    g = np.stack(np.meshgrid(np.arange(-12, 13), np.arange(-2, 6), np.arange(-8, 9),
+                             indexing="ij"), -1).reshape(-1, 3)
+    lat = (g.astype(np.float32) * np.float32(h / 2) +
+           np.array([0.0, 0.06, 0.03], np.float32)).astype(np.float32)
+    rnd = rng.uniform(-0.3, 0.3, size=(1000, 3)).astype(np.float32)
+    far = np.array([[150.0, 0.0, 0.0], [0.0, -120.0, 5.0], [60.0, 60.0, 60.0]], np.float32)
+    q = np.concatenate([lat, rnd, far]).astype(np.float32)
+    n = len(q)
+    assert n % 64 != 0
+    ref = oc.find_closest_np(q, nodes, tris)
+    assert (ref == -1).sum() == 3          # the three beyond 100
+    c = np.float32(1 / 3) * ((nodes[tris[:, 0]] + nodes[tris[:, 1]]) + nodes[tris[:, 2]])
+    d = c[None] - q[:, None]
+    dist = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2] +
+                   np.float32(1e-10))
+    assert ((dist == dist.min(1, keepdims=True)).sum(1) > 1).sum() > 1000   # ties
+    s = _lib.stream()
+    tg = torch.tensor(tris).cuda()
+    cen = torch.empty(len(tris), 4, device="cuda")
+    pg = torch.tensor(nodes).cuda()
+    _lib.call("dt_contact_centroids", len(tris), tg, pg, cen, s)
+    xq = torch.tensor(q).cuda()
+    idx = torch.full((n, 3), -7, dtype=torch.int32, device="cuda")
+    if not batched:
+        _lib.call("dt_rope_closest", n, xq, len(tris), cen, idx, 3, 1, s)
+        got = idx.cpu().numpy()
+        np.testing.assert_array_equal(got[:, 1], ref)
+        assert (got[:, [0, 2]] == -7).all()
+        return
+    bnd = torch.empty(-(-len(tris) // _lib.TRI_RUN), 4, device="cuda")
+    _lib.call("dt_contact_tri_batches", len(tris), cen, bnd, s)
+    hint = torch.tensor(rng.integers(-5, len(tris) + 5, (n, 3)), dtype=torch.int32).cuda()
+    _lib.call("dt_rope_closest_batched", n, xq, len(tris), cen, bnd, hint, idx, 3, 1, s)
+    got = idx.cpu().numpy()
+    np.testing.assert_array_equal(got[:, 1], ref)
+    assert (got[:, [0, 2]] == -7).all()
+    # the output as its own hint, holding a previous result (rolled by one),
+    # then the result itself; no hint at all
+    for start in (np.roll(ref, 1), ref, None):
+        if start is not None:
+            idx[:, 1] = torch.tensor(start).cuda()
+        _lib.call("dt_rope_closest_batched", n, xq, len(tris), cen, bnd,
+                  None if start is None else idx, idx, 3, 1, s)
+        np.testing.assert_array_equal(idx[:, 1].cpu().numpy(), ref)
