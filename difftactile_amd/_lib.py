@@ -16,9 +16,16 @@ _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
 
 
-# centroids per bounding sphere of the accelerated closest-triangle search
-# (DT_CONTACT_TRI_RUN)
+# centroids per run and runs per group of the accelerated closest-triangle
+# search (DT_CONTACT_TRI_RUN, DT_CONTACT_TRI_GROUP)
 TRI_RUN = 8
+TRI_GROUP = 8
+
+
+def tri_bounds_count(n_tris):
+    """float4 dt_contact_tri_batches writes for n_tris triangles: two per
+    run box and per group box (DT_CONTACT_TRI_BOUNDS)."""
+    return 2 * (-(-n_tris // TRI_RUN) + -(-n_tris // (TRI_RUN * TRI_GROUP)))
 
 
 class DtMpmParams(ctypes.Structure):

@@ -70,9 +70,9 @@ class Pad:
         self.fem = _lib.DtFemParams()
         self.fem.mu, self.fem.lam, self.fem.dt, self.fem.mass = mu, lam, dt, mass
         self.cen = torch.empty(self.n_tris, 4, device=self.device)
-        # bounding spheres of runs of centroids (the accelerated search)
-        runs = (self.n_tris + _lib.TRI_RUN - 1) // _lib.TRI_RUN
-        self.tri_bounds = torch.empty(max(1, runs), 4, device=self.device)
+        # bounding boxes of runs and groups of centroids (the accelerated search)
+        self.tri_bounds = torch.empty(max(1, _lib.tri_bounds_count(self.n_tris)), 4,
+                                      device=self.device)
         # vertex -> (tet, corner) CSR for the atomic-free tet-force passes
         vs, vt = vertex_tet_csr(np.asarray(f2v), self.n_verts)
         self.vert_start = torch.as_tensor(vs, device=self.device)

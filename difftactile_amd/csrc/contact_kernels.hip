@@ -77,17 +77,22 @@ __device__ __forceinline__ float centroid_dist2(const float4 c, float qx, float 
 // of them:
 //
 // * a block takes one chunk of 64 nodes at a time (an active brick, or 64
-//   consecutive nodes); its eight wavefronts each take an eighth of the
-//   chunk's triangle range, reading centroids at wave-uniform indices
-//   (scalar loads), and the parts' minima are combined at the end;
+//   consecutive nodes); the chunk's triangle range is cut into runs of
+//   kTriBatch triangles and groups of kTriGroup runs, its eight wavefronts
+//   take the groups in turn (wave w: groups w, w + 8, ...), reading centroids
+//   at wave-uniform indices (scalar loads), and the waves' minima are
+//   combined at the end;
 // * a node may start from a hinted triangle (its result of the previous
 //   substep, say: any index in its range is a valid start), which makes its
-//   best distance tight from the first batch on;
-// * with batch bounds (a sphere around each run of kTriBatch centroids), a
-//   batch whose sphere lies beyond the best distance of all 64 nodes is
-//   skipped without loading its centroids;
-// * inside a visited batch, groups of four centroids whose distance lower
-//   bounds (centroid .w) all exceed the best are skipped.
+//   best distance tight from the first run on;
+// * with bounds (the bounding box of each group's and each run's
+//   centroids), a group, and then a run, whose box lies beyond the best
+//   distance of all 64 nodes is skipped without loading what it holds (boxes
+//   rather than spheres: a pad's surface is flat on the scale of a run, and a
+//   flat box stays tight across it, so a node deep under a face visits only
+//   the runs around its foot point);
+// * inside a visited run, sets of four centroids whose distance lower bounds
+//   (centroid .w) all exceed the best are skipped.
 //
 // Skips use bounds with relative margins far above float32 rounding (stated
 // at each test), so a skipped triangle's float32 distance is strictly larger
@@ -95,6 +100,7 @@ __device__ __forceinline__ float centroid_dist2(const float4 c, float qx, float 
 constexpr int kClosestThreads = 512;
 constexpr int kClosestWaves = kClosestThreads / 64;
 constexpr int kTriBatch = DT_CONTACT_TRI_RUN;
+constexpr int kTriGroup = DT_CONTACT_TRI_GROUP;
 
 // best = lexicographic minimum of (distance, index) so far, best_s its squared
 // distance, thr_s = best_s (1 + 1e-6): every s whose square root (correctly
@@ -165,6 +171,63 @@ __device__ __forceinline__ int wave_max_i(int v) {
   return v;
 }
 
+// Whether some lane that takes triangles of [t0, t1) (its own range [lo, hi))
+// may find one at or below its best inside the box b[0] (lower corner, .w
+// unused) .. b[1] (upper corner).  The box is out of a lane's reach when the
+// float32 squared distance dd from q to the box exceeds T^2, T = best
+// (1 + 1e-5): each of dd's per-axis gaps is one correctly rounded difference
+// and dd adds three squares (a few 2^-24 relative in all), so every centroid
+// c inside is then more than best (1 + 0.99e-5) from q, and its float32
+// distance (six roundings more) is still above best: it could neither win
+// nor tie.
+__device__ __forceinline__ bool box_reachable(const float4* __restrict__ b, int t0, int t1, int lo,
+                                              int hi, float qx, float qy, float qz, float best) {
+  const float4 l = b[0], u = b[1];
+  const float g0 = fmaxf(fmaxf(l.x - qx, qx - u.x), 0.f);
+  const float g1 = fmaxf(fmaxf(l.y - qy, qy - u.y), 0.f);
+  const float g2 = fmaxf(fmaxf(l.z - qz, qz - u.z), 0.f);
+  const float dd = g0 * g0 + g1 * g1 + g2 * g2;
+  const float reach = best * (1.0f + 1e-5f);
+  return __ballot(t1 > t0 && lo < t1 && hi > t0 && !(dd > reach * reach)) != 0ull;
+}
+
+// Wave w's part of one chunk's search over the triangles [t_lo, t_hi): the
+// groups of runs overlapping it, w, w + kClosestWaves, ... of them.  bounds
+// (kBounds): the boxes of dt_contact_tri_batches, two float4 each, n_runs run
+// boxes then the group boxes.  mixed: some lane's range [lo, hi) is not [t_lo, t_hi)
+// (each triangle is then checked against the lane's own range).
+template <bool kBounds>
+__device__ __forceinline__ void closest_search(const float4* __restrict__ cen,
+                                               const float4* __restrict__ bounds, int n_runs,
+                                               int w, int t_lo, int t_hi, int lo, int hi,
+                                               bool mixed, float qx, float qy, float qz,
+                                               Closest& r) {
+  const float qb = kBoundShrink * ((qx * qx + qy * qy) + qz * qz);
+  const int b_lo = t_lo / kTriBatch;
+  const int b_hi = t_hi > t_lo ? (t_hi + kTriBatch - 1) / kTriBatch : b_lo;
+  const int g_lo = b_lo / kTriGroup;
+  const int g_hi = b_hi > b_lo ? (b_hi + kTriGroup - 1) / kTriGroup : g_lo;
+  for (int g = g_lo + w; g < g_hi; g += kClosestWaves) {
+    const int r0 = max(g * kTriGroup, b_lo), r1 = min(g * kTriGroup + kTriGroup, b_hi);
+    if (kBounds &&
+        !box_reachable(bounds + 2 * (n_runs + g), max(r0 * kTriBatch, t_lo),
+                       min(r1 * kTriBatch, t_hi), lo, hi, qx, qy, qz, r.best))
+      continue;
+    for (int b = r0; b < r1; ++b) {
+      const int t0 = max(b * kTriBatch, t_lo), t1 = min(b * kTriBatch + kTriBatch, t_hi);
+      if (kBounds && !box_reachable(bounds + 2 * b, t0, t1, lo, hi, qx, qy, qz, r.best)) continue;
+      if (t1 - t0 == kTriBatch) {
+        if (mixed)
+          closest_batch<true>(cen, t0, lo, hi, qx, qy, qz, qb, r);
+        else
+          closest_batch<false>(cen, t0, lo, hi, qx, qy, qz, qb, r);
+      } else {
+        for (int t = t0; t < t1; ++t) closest_step<true>(cen[t], t, lo, hi, qx, qy, qz, r);
+      }
+    }
+  }
+}
+
 template <bool kBounds>
 __global__ __launch_bounds__(kClosestThreads) void k_closest(MpmP P, long long nn,
                                                     const float* __restrict__ gm, int m_stride,
@@ -221,42 +284,15 @@ __global__ __launch_bounds__(kClosestThreads) void k_closest(MpmP P, long long n
       mixed = __ballot(occ && (lo != t_lo || hi != t_hi)) != 0ull;
     }
     if (!occ) lo = hi = 0;   // idle lanes take no triangle
-    const float qb = kBoundShrink * ((qx * qx + qy * qy) + qz * qz);
     Closest r{100.0f, 1e4f, 1e4f * (1.0f + 1e-6f), -1};
-    // this wave's part of the batches overlapping [t_lo, t_hi)
-    const int b_lo = t_lo / kTriBatch;
-    const int b_hi = t_hi > t_lo ? (t_hi + kTriBatch - 1) / kTriBatch : b_lo;
-    const int per = (b_hi - b_lo + kClosestWaves - 1) / kClosestWaves;
-    const int wb0 = __builtin_amdgcn_readfirstlane(min(b_lo + w * per, b_hi));
-    const int wb1 = __builtin_amdgcn_readfirstlane(min(wb0 + per, b_hi));
     // seed from the hint (hint and idx may be the same array: every read of
     // a node's entry precedes the barrier before its write)
     if (hint && occ) {
       const int h = hint[nd * stride + col];
       if (h >= lo && h < hi) closest_step<false>(cen[h], h, lo, hi, qx, qy, qz, r);
     }
-    for (int b = wb0; b < wb1; ++b) {
-      const int t0 = max(b * kTriBatch, t_lo), t1 = min(b * kTriBatch + kTriBatch, t_hi);
-      if (kBounds) {
-        // skip when, for every node, the batch sphere (radius .w, inflated by
-        // 2e-5 relative) lies beyond T = best (1 + 1e-5): then every centroid
-        // c of it has |q - c| >= |q - centre| - r > T, and the float32 errors
-        // of these tests (a few 2^-24 relative) stay below the margins
-        const float4 s4 = bounds[b];
-        const float d0 = s4.x - qx, d1 = s4.y - qy, d2 = s4.z - qz;
-        const float dd = d0 * d0 + d1 * d1 + d2 * d2;
-        const float reach = s4.w + r.best * (1.0f + 1e-5f);
-        if (__ballot(t1 > t0 && lo < t1 && hi > t0 && !(dd > reach * reach)) == 0ull) continue;
-      }
-      if (t1 - t0 == kTriBatch) {
-        if (mixed)
-          closest_batch<true>(cen, t0, lo, hi, qx, qy, qz, qb, r);
-        else
-          closest_batch<false>(cen, t0, lo, hi, qx, qy, qz, qb, r);
-      } else {
-        for (int t = t0; t < t1; ++t) closest_step<true>(cen[t], t, lo, hi, qx, qy, qz, r);
-      }
-    }
+    closest_search<kBounds>(cen, bounds, (nt + kTriBatch - 1) / kTriBatch, w, t_lo, t_hi, lo, hi,
+                            mixed, qx, qy, qz, r);
     s_best[w][lane] = r.best;
     s_k[w][lane] = r.best_k;
     __syncthreads();
@@ -274,16 +310,17 @@ __global__ __launch_bounds__(kClosestThreads) void k_closest(MpmP P, long long n
   }
 }
 
-// Sphere around each run of kTriBatch centroids: the centre of their bounding
-// box and the largest centroid distance from it, inflated by 2e-5 relative
-// (float32 differences of floats are exact to 2^-24 relative, so the real
-// distances stay inside).
+// Bounding box of each run of kTriBatch centroids and of each group of
+// kTriGroup runs (exact minima and maxima), two float4 each: the lower corner
+// (.w = 0) then the upper.  One thread per run; the threads of a group are
+// kTriGroup consecutive lanes of a wavefront, and threads past the last run
+// take an empty run.
 __global__ __launch_bounds__(kBlock) void k_tri_batches(int nt, const float4* __restrict__ cen,
                                                         float4* __restrict__ bounds) {
+  static_assert(64 % kTriGroup == 0 && (kTriGroup & (kTriGroup - 1)) == 0, "group of lanes");
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  const int t0 = b * kTriBatch;
-  if (t0 >= nt) return;
-  const int t1 = min(t0 + kTriBatch, nt);
+  const int n_runs = (nt + kTriBatch - 1) / kTriBatch;
+  const int t0 = min(b * kTriBatch, nt), t1 = min(t0 + kTriBatch, nt);
   float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
   for (int t = t0; t < t1; ++t) {
     const float4 c = cen[t];
@@ -291,14 +328,20 @@ __global__ __launch_bounds__(kBlock) void k_tri_batches(int nt, const float4* __
     lo[1] = fminf(lo[1], c.y), hi[1] = fmaxf(hi[1], c.y);
     lo[2] = fminf(lo[2], c.z), hi[2] = fmaxf(hi[2], c.z);
   }
-  const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
-  float r2 = 0.f;
-  for (int t = t0; t < t1; ++t) {
-    const float4 c = cen[t];
-    const float d0 = c.x - cx, d1 = c.y - cy, d2 = c.z - cz;
-    r2 = fmaxf(r2, d0 * d0 + d1 * d1 + d2 * d2);
+  float glo[3] = {lo[0], lo[1], lo[2]}, ghi[3] = {hi[0], hi[1], hi[2]};
+  for (int o = 1; o < kTriGroup; o <<= 1)
+    for (int a = 0; a < 3; ++a) {
+      glo[a] = fminf(glo[a], __shfl_xor(glo[a], o));
+      ghi[a] = fmaxf(ghi[a], __shfl_xor(ghi[a], o));
+    }
+  if (b >= n_runs) return;
+  bounds[2 * b] = make_float4(lo[0], lo[1], lo[2], 0.f);
+  bounds[2 * b + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+  if (b % kTriGroup == 0) {
+    const int g = n_runs + b / kTriGroup;
+    bounds[2 * g] = make_float4(glo[0], glo[1], glo[2], 0.f);
+    bounds[2 * g + 1] = make_float4(ghi[0], ghi[1], ghi[2], 0.f);
   }
-  bounds[b] = make_float4(cx, cy, cz, sqrtf(r2) * (1.0f + 2e-5f));
 }
 
 // persistent grid of the closest-triangle search: one block per chunk up to a cap
@@ -761,11 +804,11 @@ __global__ __launch_bounds__(kBlock) void k_collision_adj_fx2(
 // find_closest with the particle position as the query (strict < from
 // (100, -1) in index order: the lexicographic minimum of (distance, index)),
 // searched as k_closest searches nodes: a block takes 64 particles at a time,
-// its eight wavefronts each scan an eighth of the triangle runs at
-// wave-uniform indices, runs beyond every lane's best (kBounds) and groups of
-// four beyond a lane's best are skipped, and the parts' minima are combined
-// in LDS.  hint as for k_closest (it may be idx itself: every read of a
-// particle's entry precedes the barrier before its write).
+// its eight wavefronts take the groups of triangle runs in turn (with
+// kBounds, groups and runs whose boxes lie beyond every lane's best are
+// skipped) and their
+// minima are combined in LDS.  hint as for k_closest (it may be idx itself:
+// every read of a particle's entry precedes the barrier before its write).
 template <bool kBounds>
 __global__ __launch_bounds__(kClosestThreads) void k_point_closest(
     int n, const float* __restrict__ x, int nt, const float4* __restrict__ cen,
@@ -776,34 +819,19 @@ __global__ __launch_bounds__(kClosestThreads) void k_point_closest(
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int n_chunks = (n + 63) / 64;
   const int n_runs = (nt + kTriBatch - 1) / kTriBatch;
-  const int per = (n_runs + kClosestWaves - 1) / kClosestWaves;
-  const int wb0 = min(w * per, n_runs), wb1 = min(wb0 + per, n_runs);
   for (int ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
     const int p = ch * 64 + lane;
     const bool live = p < n;
     const float qx = live ? x[3ll * p] : 0.f, qy = live ? x[3ll * p + 1] : 0.f,
                 qz = live ? x[3ll * p + 2] : 0.f;
-    const float qb = kBoundShrink * ((qx * qx + qy * qy) + qz * qz);
     Closest r{100.0f, 1e4f, 1e4f * (1.0f + 1e-6f), -1};
     if (hint && live) {
       const int h = hint[(long long)p * stride + col];
       if (h >= 0 && h < nt) closest_step<false>(cen[h], h, 0, nt, qx, qy, qz, r);
     }
-    for (int b = wb0; b < wb1; ++b) {
-      const int t0 = b * kTriBatch, t1 = min(t0 + kTriBatch, nt);
-      if (kBounds) {
-        // the skip test of k_closest, over the live lanes
-        const float4 s4 = bounds[b];
-        const float d0 = s4.x - qx, d1 = s4.y - qy, d2 = s4.z - qz;
-        const float dd = d0 * d0 + d1 * d1 + d2 * d2;
-        const float reach = s4.w + r.best * (1.0f + 1e-5f);
-        if (__ballot(live && !(dd > reach * reach)) == 0ull) continue;
-      }
-      if (t1 - t0 == kTriBatch)
-        closest_batch<false>(cen, t0, 0, nt, qx, qy, qz, qb, r);
-      else
-        for (int t = t0; t < t1; ++t) closest_step<false>(cen[t], t, 0, nt, qx, qy, qz, r);
-    }
+    // lanes past the end take no triangle
+    closest_search<kBounds>(cen, bounds, n_runs, w, 0, nt, 0, live ? nt : 0, false, qx, qy, qz,
+                            r);
     s_best[w][lane] = r.best;
     s_k[w][lane] = r.best_k;
     __syncthreads();

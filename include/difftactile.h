@@ -558,18 +558,24 @@ int dt_contact_closest_env(const dt_mpm_params* prm, const float* grid_m, int32_
                            const int32_t* tri_start, int32_t n_tris, const float* centroids4,
                            int32_t* contact_idx, int32_t idx_stride, int32_t idx_col,
                            void* stream);
-/* Accelerated form of the same search (same indices, bit for bit): spheres
-   around each run of DT_CONTACT_TRI_RUN centroids let the search skip runs
-   beyond a node's best distance.  dt_contact_tri_batches writes the spheres,
-   float4 (centre, radius) per run, ceil(n_tris / DT_CONTACT_TRI_RUN) of them,
-   from the centroids of
-   dt_contact_centroids; tri_start as in dt_contact_closest_env, or NULL for a
-   single environment.  hint (optional, same layout as contact_idx, may be
+/* Accelerated form of the same search (same indices, bit for bit): the
+   bounding boxes of each run of DT_CONTACT_TRI_RUN centroids and of each
+   group of DT_CONTACT_TRI_GROUP runs let the search skip groups and runs
+   beyond a node's best distance.  dt_contact_tri_batches writes the boxes
+   from the centroids of dt_contact_centroids, two float4 each (lower corner,
+   upper corner; .w = 0), DT_CONTACT_TRI_BOUNDS(n_tris) float4 in all: the
+   ceil(n_tris / DT_CONTACT_TRI_RUN) runs' boxes first, then the groups'.  tri_start as in dt_contact_closest_env, or
+   NULL for a single environment.  hint (optional, same layout as contact_idx, may be
    contact_idx itself): a triangle per node to start from, such as the
    previous substep's result; entries outside the node's range are ignored,
    and any start gives the same result.  Replaces fem_sensor.py:343-359 like
    dt_contact_closest. */
 #define DT_CONTACT_TRI_RUN 8
+#define DT_CONTACT_TRI_GROUP 8
+#define DT_CONTACT_TRI_BOUNDS(n_tris)                                      \
+  (2 * (((n_tris) + DT_CONTACT_TRI_RUN - 1) / DT_CONTACT_TRI_RUN +         \
+        ((n_tris) + DT_CONTACT_TRI_RUN * DT_CONTACT_TRI_GROUP - 1) /       \
+            (DT_CONTACT_TRI_RUN * DT_CONTACT_TRI_GROUP)))
 int dt_contact_tri_batches(int32_t n_tris, const float* centroids4, float* batch_bounds4,
                            void* stream);
 int dt_contact_closest_batched(const dt_mpm_params* prm, const float* grid_m, int32_t m_stride,
@@ -827,7 +833,7 @@ int dt_rope_step_adj(const dt_rope_params* prm, int32_t n, const float* fixed_po
    closest triangle to each particle position, as dt_contact_closest. */
 int dt_rope_closest(int32_t n, const float* x, int32_t n_tris, const float* centroids4,
                     int32_t* contact_idx, int32_t idx_stride, int32_t idx_col, void* stream);
-/* dt_rope_closest with the run spheres of dt_contact_tri_batches (written for
+/* dt_rope_closest with the boxes of dt_contact_tri_batches (written for
    these centroids) and an optional hint per particle, as
    dt_contact_closest_batched: same result, most runs skipped.  hint may be
    contact_idx itself (its previous contents) or NULL. */

@@ -61,6 +61,11 @@ def test_binding_constants_match_header():
     src = open(os.path.join(ROOT, "include", "difftactile.h")).read()
     m = re.search(r"#define\s+DT_CONTACT_TRI_RUN\s+(\d+)", src)
     assert m and int(m.group(1)) == _lib.TRI_RUN
+    m = re.search(r"#define\s+DT_CONTACT_TRI_GROUP\s+(\d+)", src)
+    assert m and int(m.group(1)) == _lib.TRI_GROUP
+    # run boxes then group boxes, two float4 each
+    assert [_lib.tri_bounds_count(n) for n in (0, 1, 8, 9, 64, 65, 3042)] == \
+        [0, 4, 4, 6, 18, 22, 2 * (381 + 48)]
 
 
 def test_jaw_frame_size_matches_header():

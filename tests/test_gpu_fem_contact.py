@@ -207,7 +207,7 @@ def test_closest_triangle_bit_exact(gpu):
     ref_full[nodes_o] = np.roll(ref[:, 0], 1)
     occ_mask = np.zeros(nn, bool)
     occ_mask[nodes_o] = True
-    bnd = torch.empty(-(-len(surf) // _lib.TRI_RUN), 4, device="cuda")
+    bnd = torch.empty(_lib.tri_bounds_count(len(surf)), 4, device="cuda")
     _lib.call("dt_contact_tri_batches", len(surf), P(cen), P(bnd), _lib.stream())
     for batched in (False, True):
         idx = torch.full((nn, 2), -7, dtype=torch.int32, device="cuda")
@@ -234,7 +234,7 @@ def test_closest_triangle_ties_envs_and_node_set(gpu, node_set, batched):
     own pad, on lattice-aligned slabs (the two triangles of a cell are
     equidistant from many node centres: the first minimum must win), over
     every node or over the active bricks of a binning, by the plain search or
-    the one that skips runs of triangles by their bounding spheres; bit-exact
+    the one that skips runs of triangles by their bounding boxes; bit-exact
     against the oracle's float32 brute force per environment."""
     import ctypes
     from difftactile_amd.batch import concat_pad_meshes
@@ -278,7 +278,7 @@ def test_closest_triangle_ties_envs_and_node_set(gpu, node_set, batched):
     idx = torch.full((nn,), -7, dtype=torch.int32, device="cuda")
     ts = dev(tri_start, torch.int32)
     if batched:
-        bnd = torch.empty(-(-len(tris) // _lib.TRI_RUN), 4, device="cuda")
+        bnd = torch.empty(_lib.tri_bounds_count(len(tris)), 4, device="cuda")
         _lib.call("dt_contact_tri_batches", len(tris), P(cen), P(bnd), _lib.stream())
         # a hint array of arbitrary entries (in range or not) changes nothing
         hint = dev(np.random.default_rng(9).integers(-5, len(tris) + 5, nn), torch.int32)

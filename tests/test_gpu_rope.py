@@ -141,7 +141,7 @@ def test_rope_closest_ties_far_particles_and_hints(gpu, batched):
     brute force: particles on the pad's lattice (the two triangles of a cell
     are equidistant from many of them: the first minimum must win), particles
     far from every triangle (-1 beyond 100), a count that is no multiple of
-    64; the search that skips runs by their bounding spheres gives the same
+    64; the search that skips runs by their bounding boxes gives the same
     answer from arbitrary hints and from its own previous result."""
     from difftactile_amd import _lib
     h = 0.02
@@ -178,7 +178,7 @@ def test_rope_closest_ties_far_particles_and_hints(gpu, batched):
         np.testing.assert_array_equal(got[:, 1], ref)
         assert (got[:, [0, 2]] == -7).all()
         return
-    bnd = torch.empty(-(-len(tris) // _lib.TRI_RUN), 4, device="cuda")
+    bnd = torch.empty(_lib.tri_bounds_count(len(tris)), 4, device="cuda")
     _lib.call("dt_contact_tri_batches", len(tris), cen, bnd, s)
     hint = torch.tensor(rng.integers(-5, len(tris) + 5, (n, 3)), dtype=torch.int32).cuda()
     _lib.call("dt_rope_closest_batched", n, xq, len(tris), cen, bnd, hint, idx, 3, 1, s)
