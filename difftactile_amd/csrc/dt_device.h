@@ -428,15 +428,24 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
       }
     }
     st.pG = st.mode == 3 ? 1.f : __expf((1.f - st.pa) * tr * (1.f / 3.f));
-    if (st.mode) {
-      float d[3] = {__expf(st.epsn[0]), __expf(st.epsn[1]), __expf(st.epsn[2])};
-      udv(U, d, V, st.Fp);
+    st.R = m3_mul_bt(U, V);
+    if (st.mode == 3) {
+      // cone tip: F_p = U V^T = R and J = 1 exactly, so tau = 0 exactly (as
+      // U diag(1) V^T and det of it in float32 it would be rounding noise of
+      // size mu 1e-7, whose adjoint is noise too)
+      st.Fp = st.R;
+      st.J = 1.f;
+    } else {
+      if (st.mode) {
+        float d[3] = {__expf(st.epsn[0]), __expf(st.epsn[1]), __expf(st.epsn[2])};
+        udv(U, d, V, st.Fp);
+      }
+      st.J = m3_det(st.Fp);
     }
-    st.J = m3_det(st.Fp);
   } else {
     st.J = sig[0] * sig[1] * sig[2];
+    st.R = m3_mul_bt(U, V);
   }
-  st.R = m3_mul_bt(U, V);
   // tau = 2 mu (Fp - R) Fp^T + lam J (J - 1) I
   M3 D;
   for (int i = 0; i < 9; ++i) D.a[i] = st.Fp.a[i] - st.R.a[i];
@@ -480,6 +489,15 @@ DT_HD void stress_adj(const M3& Fn, const M3& U, const float (&sig)[3], const M3
   float gJ = m.lam * (2.f * st.J - 1.f) * trg;
   gmu = 2.f * gm;
   glam = trg * st.J * (st.J - 1.f);
+  if (m.kind != KIND_ELASTIC && st.mode == 3) {
+    // cone tip: tau = 0 whatever F, mu and lam (F_p = R, J = 1), so it has no
+    // adjoint, and F_next = R takes its adjoint to F through the rotation
+    // alone (the two paths' tau terms would cancel only to rounding)
+    gR = gFp_in;
+    gFp = m3_zero();
+    gJ = 0.f;
+    gmu = glam = 0.f;
+  }
 
   R gsig[3] = {R(0), R(0), R(0)};
   // U/V adjoints of the return-map path (F_p = U diag(d) V^T), reference form

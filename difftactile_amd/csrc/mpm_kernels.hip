@@ -755,6 +755,13 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(
     for (int i = 0; i < 9; ++i) gmu += 2.f * gtF.a[i] * D.a[i];
     glam = trg * st.J * (st.J - 1.f);
   }
+  if (m.kind != KIND_ELASTIC && st.mode == 3) {
+    // cone tip: tau = 0 whatever F (F_p = R, J = 1); F_next = R's adjoint
+    // reaches U and V through the rotation alone
+    gR = gFnext_p;
+    gFp = m3_zero();
+    gJ = 0.f;
+  }
   M3d gUd = m3d_mul(m3d_from(gR), m3d_from(vv));
   M3d gVd = m3d_mul_at(m3d_from(gR), m3d_from(u));
   double gsig[3] = {0.0, 0.0, 0.0};
