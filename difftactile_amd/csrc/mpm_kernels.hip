@@ -1835,6 +1835,35 @@ __device__ __forceinline__ float4 fetch_gv_tile(const MpmP& P, long long b, cons
   return q;
 }
 
+// node t of brick b's 6x6x6-node tile of grid velocities made from the kept
+// (mv, m) grid g4 by the node update of the forward pass (no contact force):
+// the same function of the same inputs as k_grid_op4_tiles, so the same bits
+__device__ __forceinline__ float4 fetch_gv_tile_op(const MpmP& P, long long b,
+                                                   const float4* __restrict__ g4) {
+  const int n = P.n_grid, nb = (n + 3) >> 2;
+  const int bz = (int)(b % nb), by = (int)((b / nb) % nb), bx = (int)(b / ((long long)nb * nb));
+  const int t = threadIdx.x;
+  float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (t < kTileNodes) {
+    const int i = bx * 4 + t / (kTileEdge * kTileEdge), j = by * 4 + (t / kTileEdge) % kTileEdge,
+              k = bz * 4 + t % kTileEdge;
+    if (i < P.nx && j < n && k < n) {
+      const long long nd = node_index(i, j, k, n);
+      const float4 g = g4[nd];
+      if (g.w > P.eps) {
+        float vo[3];
+        bool mask[3];
+        grid_node_fwd(P, nd, g.x, g.y, g.z, g.w, 0.f, 0.f, 0.f, vo, mask);
+        q = make_float4(vo[0], vo[1], vo[2], 0.f);
+      }
+    }
+  }
+  return q;
+}
+
+// kFromMass: gv is the kept (mv, m) grid (float4 per node) and each brick's
+// velocity tile is made from it; else gv is the velocity grid (stride sgv)
+template <bool kFromMass>
 __global__ __launch_bounds__(kBlock, 3) void k_g2p_adj_binned(MpmP P, int n,
                                                            const float* __restrict__ x,
                                                            const float* __restrict__ gv, int sgv,
@@ -1859,7 +1888,10 @@ __global__ __launch_bounds__(kBlock, 3) void k_g2p_adj_binned(MpmP P, int n,
                          (int)(b % nb) * 4};
         g2p_adj_compute<1>(P, n, in, src, gx, p, s, bb, M);
       },
-      [&](long long b) { return fetch_gv_tile(P, b, gv, sgv); });
+      [&](long long b) {
+        return kFromMass ? fetch_gv_tile_op(P, b, reinterpret_cast<const float4*>(gv))
+                         : fetch_gv_tile(P, b, gv, sgv);
+      });
 }
 
 // ----------------------------------------------------------------------------
@@ -2085,10 +2117,17 @@ int launch_substep_adj(const dt_mpm_params* prm, const dt_material* mats, int n_
     }
     {
       StageTimer tm(STAGE_G2P_ADJ, st);
-      hipLaunchKernelGGL(k_g2p_adj_binned, dim3(brick_blocks(*ws)), dim3(kBlock), 0, st, P, n, x,
-                       grid_v4, 4, gx1, gv1, gC1, gx, (const int32_t*)ws->perm,
-                       (const int32_t*)ws->start, (const int32_t*)ws->list,
-                       (const int32_t*)ws->n_list, ws->tiles);
+      // grid_v4 NULL: the velocities come from grid4 (no contact force)
+      if (grid_v4)
+        hipLaunchKernelGGL(k_g2p_adj_binned<false>, dim3(brick_blocks(*ws)), dim3(kBlock), 0, st,
+                           P, n, x, grid_v4, 4, gx1, gv1, gC1, gx, (const int32_t*)ws->perm,
+                           (const int32_t*)ws->start, (const int32_t*)ws->list,
+                           (const int32_t*)ws->n_list, ws->tiles);
+      else
+        hipLaunchKernelGGL(k_g2p_adj_binned<true>, dim3(brick_blocks(*ws)), dim3(kBlock), 0, st,
+                           P, n, x, grid4, 4, gx1, gv1, gC1, gx, (const int32_t*)ws->perm,
+                           (const int32_t*)ws->start, (const int32_t*)ws->list,
+                           (const int32_t*)ws->n_list, ws->tiles);
     }
     StageTimer tm(STAGE_GRID_OP_ADJ, st);
     hipLaunchKernelGGL(k_grid_op4_adj_tiles, dim3(active_blocks(*ws)), dim3(kBlock), 0, st, P,
@@ -2436,8 +2475,14 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   if (rc) return rc;
   if ((rc = check_mats(mats, n_mats))) return rc;
   DT_REQUIRE(n >= 0 && n_substeps >= 0, "n and n_substeps must be >= 0");
-  bool keep = grids4 != nullptr && grids_v4 != nullptr;
+  // kept grids: grids4 and grids_v4, or on the sparse path grids4 alone (the
+  // adjoint remakes the velocities from it) with scratch_v4 as the
+  // substep's velocity grid
+  const bool keep = grids4 != nullptr && (grids_v4 != nullptr || workspace != nullptr);
+  const bool keep_v = keep && grids_v4 != nullptr;
   DT_REQUIRE(keep || (scratch4 && scratch_v4), "need kept grids or scratch grids");
+  DT_REQUIRE(keep_v || scratch_v4, "need kept velocity grids or a scratch velocity grid");
+  DT_REQUIRE(grids4 || !grids_v4, "grids_v4 without grids4");
   BinWs ws{};
   if (workspace &&
       (rc = workspace_views(prm, n, workspace, workspace_bytes, ws, keep ? n_substeps : 0)))
@@ -2450,7 +2495,7 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
     for (int s = 0; s < n_substeps; ++s) {
       long long o = (long long)s * n, o1 = (long long)(s + 1) * n;
       float* g4 = keep ? grids4 + (long long)s * nn * 4 : scratch4;
-      float* gv4 = keep ? grids_v4 + (long long)s * nn * 4 : scratch_v4;
+      float* gv4 = keep_v ? grids_v4 + (long long)s * nn * 4 : scratch_v4;
       BinWs slot = hist ? bin_slot(ws, s, n) : ws;
       const BinWs* wp = workspace ? &slot : nullptr;
       int r = launch_substep_fwd(prm, mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o, F + 9 * o,
@@ -2494,7 +2539,8 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
   // reuse the forward pass's binning when this workspace holds it for these frames
   const bool hist = workspace && ws.hist_slots >= n_substeps && n_substeps > 0 &&
                     hist_matches(workspace, x, n, n_substeps);
-  DT_REQUIRE(grids4 && grids_v4 && g_grid4 && g_grid_v4, "NULL grid pointer");
+  DT_REQUIRE(grids4 && g_grid4 && g_grid_v4, "NULL grid pointer");
+  DT_REQUIRE(grids_v4 || workspace, "grids_v4 NULL needs the sparse path (a workspace)");
   long long nn = grid_nodes(prm);
   hipStream_t st = DT_STREAM(stream);
   float *a[4] = {gx, gv, gC, gF}, *b[4] = {gx2, gv2, gC2, gF2};
@@ -2513,7 +2559,8 @@ int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
       const BinWs* wp = workspace ? &slot : nullptr;
       int r = launch_substep_adj(prm, mats, n_mats, n, mat_id, x + 3 * o, v + 3 * o, C + 9 * o,
                                  F + 9 * o, grids4 + (long long)s * nn * 4,
-                                 grids_v4 + (long long)s * nn * 4, p[0], p[1], p[2], p[3], g_grid4,
+                                 grids_v4 ? grids_v4 + (long long)s * nn * 4 : nullptr, p[0], p[1],
+                                 p[2], p[3], g_grid4,
                                  g_grid_v4, q[0], q[1], q[2], q[3], g_mats, cs, wp, hist,
                                  s != n_substeps - 1, mat_det ? ws.mat_part : nullptr);
       if (r) return r;
@@ -2939,7 +2986,7 @@ int dt_mpm_g2p4_adj_binned(const dt_mpm_params* prm, int32_t n, const float* x,
   DT_REQUIRE(n == 0 || (x && gx_next && gv_next && gC_next && gx), "NULL pointer");
   BinWs ws{};
   if ((rc = workspace_views(prm, n, workspace, workspace_bytes, ws))) return rc;
-  hipLaunchKernelGGL(k_g2p_adj_binned, dim3(brick_blocks(ws)), dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_g2p_adj_binned<false>, dim3(brick_blocks(ws)), dim3(kBlock), 0,
                      DT_STREAM(stream), to_dev(*prm), n, x, grid_v4, 4, gx_next, gv_next,
                      gC_next, gx, (const int32_t*)ws.perm, (const int32_t*)ws.start,
                      (const int32_t*)ws.list, (const int32_t*)ws.n_list, ws.tiles);

@@ -259,10 +259,20 @@ class MPMSim:
     def segment_bytes(self, k):
         """HBM bytes of one resident k-substep segment: frames, grids and (on
         the sparse path) its own workspace with the binning history."""
-        b = 4 * (self.n * 24 * (k + 1) + 2 * 4 * k * self.prm.n_nodes)
+        # the sparse path keeps the (mv, m) grid of each substep only: its
+        # adjoint remakes the velocities from it
+        grids = 1 if self.sparse_grid else 2
+        b = 4 * (self.n * 24 * (k + 1) + grids * 4 * k * self.prm.n_nodes)
         if self.sparse_grid:
             b += workspace_bytes(self.c_prm, self.n, k)
         return b
+
+    def _kept_grids(self, k):
+        """Fresh kept grids of a k-substep segment: (G4, GV4), GV4 None on the
+        sparse path."""
+        nn = self.prm.n_nodes
+        return (torch.empty((k, nn, 4), device=self.device),
+                None if self.sparse_grid else torch.empty((k, nn, 4), device=self.device))
 
     def segment_workspace(self, k):
         """A workspace with room for the binning history of a k-substep
@@ -288,8 +298,9 @@ class MPMSim:
                 G4, GV4 = grids
             else:
                 G4 = self._buf("G4", (k, nn, 4))
-                GV4 = self._buf("GV4", (k, nn, 4))
-            s4, sv4 = None, None
+                GV4 = None if self.sparse_grid else self._buf("GV4", (k, nn, 4))
+            # sparse path: the velocity grid of each substep is scratch
+            s4, sv4 = None, (self._grid_scratch()[1] if GV4 is None else None)
             g4p, gv4p = _lib.ptr(G4), _lib.ptr(GV4)
         else:
             G4 = GV4 = None
@@ -391,9 +402,7 @@ class _Rollout(torch.autograd.Function):
                 used += nbytes
                 fr = sim._take(taken, ("frames", i, k), lambda: (
                     [torch.empty((k + 1, w, n), device=sim.device) for w in W],
-                    (torch.empty((k, nn, 4), device=sim.device),
-                     torch.empty((k, nn, 4), device=sim.device)),
-                    sim.segment_workspace(k)))
+                    sim._kept_grids(k), sim.segment_workspace(k)))
                 frames, grids, seg_ws = fr
                 for f, t in zip(frames, state):
                     f[0].copy_(t)

@@ -313,7 +313,11 @@ int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int3
    by cell, visits only the 4x4x4-node bricks some stencil touches, and
    scatters with the brick-owned tile passes; grid nodes outside those bricks
    are neither written nor read.  NULL keeps the dense passes over all n^3
-   nodes. */
+   nodes.  On the sparse path grids_v4 may be NULL with grids4 kept: the
+   substep's velocity grid then lives in scratch_v4 and dt_mpm_run_adj (given
+   grids_v4 NULL too) remakes each brick's velocities from grids4 by the same
+   node update, bit for bit, so a kept trajectory holds 16 B per node and
+   substep instead of 32. */
 int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, float* x, float* v, float* C, float* F,
                    int32_t n_substeps, float* grids4, float* grids_v4, float* scratch4,
@@ -321,7 +325,9 @@ int dt_mpm_run_fwd(const dt_mpm_params* prm, const dt_material* mats, int32_t n_
 /* Reverse sweep over frames [0, n_substeps] stored by dt_mpm_run_fwd with
    grids kept.  Seeds: gx..gF hold the adjoint of frame n_substeps on entry and
    the adjoint of frame 0 on return; gx2..gF2 are same-size scratch.  g_mats
-   accumulates the material adjoints over the sweep (may be NULL). */
+   accumulates the material adjoints over the sweep (may be NULL).  grids_v4
+   NULL (sparse path only): the velocities come from grids4, as kept by a
+   dt_mpm_run_fwd with grids_v4 NULL. */
 int dt_mpm_run_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats, int32_t n,
                    const int32_t* mat_id, const float* x, const float* v, const float* C,
                    const float* F, int32_t n_substeps, const float* grids4, const float* grids_v4,

@@ -815,6 +815,46 @@ def test_adjoint_reuses_forward_binning_history(gpu):
         assert rel_l2(host64(a), host64(b)) < 1e-5
 
 
+def test_trajectory_without_kept_velocity_grids_is_bit_identical(gpu):
+    """Sparse trajectory loops keeping the (mv, m) grids only (grids_v4 NULL,
+    the substep's velocity grid in scratch_v4): the forward frames and grids
+    and the adjoint, whose G2P pass remakes each brick's velocities from the
+    kept grid, equal those of the loops that keep both grids, bit for bit."""
+    prm, mats, mat_id, x, v, C, F = _facade_inputs("multi", n=3000, seed=43)
+    C = C * 0.1
+    n, nn, k = len(x), prm.n_nodes, 3
+    c_prm = prm.c_struct()
+    mt = material_table(mats, "cuda")
+    mid = dev(mat_id, torch.int32)
+    seeds = [_seed((n,) + a.shape[1:], 700 + i).cuda() for i, a in enumerate((x, v, C, F))]
+    results = []
+    for keep_v in (True, False):
+        ws, nbytes = _workspace(c_prm, n)
+        fr = [torch.zeros((k + 1,) + a.shape, device="cuda") for a in (x, v, C, F)]
+        for f, a in zip(fr, (x, v, C, F)):
+            f[0] = dev(a)
+        G4 = torch.zeros(k, nn, 4, device="cuda")
+        GV4 = torch.zeros(k, nn, 4, device="cuda") if keep_v else None
+        sv4 = None if keep_v else torch.full((nn, 4), float("nan"), device="cuda")
+        call("dt_mpm_run_fwd", c_prm, mt, len(mats), n, mid, *fr, k, G4, GV4, None, sv4, ws,
+             nbytes, _lib.stream())
+        g = [s_.clone() for s_ in seeds]
+        g2 = [torch.empty_like(s_) for s_ in seeds]
+        gg4 = torch.zeros(nn, 4, device="cuda")
+        ggv4 = torch.zeros(nn, 4, device="cuda")
+        gmat = torch.zeros(len(mats), 2, device="cuda")
+        call("dt_mpm_run_adj", c_prm, mt, len(mats), n, mid, *fr, k, G4, GV4, gg4, ggv4, *g, *g2,
+             ws, nbytes, gmat, _lib.stream())
+        torch.cuda.synchronize()
+        results.append([t.cpu() for t in fr] + [G4.cpu()] + [t.cpu() for t in g] + [gmat.cpu()])
+    for a, b in zip(*results):
+        assert torch.equal(a, b)
+    # the loops without workspace (dense) still need both grids
+    with pytest.raises(RuntimeError):
+        call("dt_mpm_run_adj", c_prm, mt, len(mats), n, mid, *fr, k, G4, None, gg4, ggv4, *g,
+             *g2, None, 0, None, _lib.stream())
+
+
 def test_c5_size_sparse_substeps_conserve_mass_and_momentum(gpu):
     """2M particles on 256^3 (the C5 object size) through the sparse path:
     every substep's binned P2G grid holds the total mass and the particle
