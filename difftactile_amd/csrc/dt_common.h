@@ -49,7 +49,11 @@ struct MpmP {
   const int32_t* env_ngrid;
   const float4* env_scale;
   int n_wall;
+  // always 0: the struct has no padding bytes, so its bytes can key the graph
+  // cache (uninitialised padding would make equal parameters look different)
+  int key_pad;
 };
+static_assert(sizeof(MpmP) == 120, "MpmP must have no padding bytes (graph keys hash them)");
 
 inline MpmP to_dev(const dt_mpm_params& p) {
   MpmP d;
@@ -80,6 +84,7 @@ inline MpmP to_dev(const dt_mpm_params& p) {
                                              : nullptr;
   if (!d.env_scale) d.env_ngrid = nullptr;
   d.n_wall = p.n_grid;
+  d.key_pad = 0;
   return d;
 }
 

@@ -392,6 +392,7 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
   st.Fp = Fn;
   st.pa = 0.f;
   st.pG = 1.f;
+  float J;
   if (m.kind != KIND_ELASTIC) {
     float eps[3];
     for (int i = 0; i < 3; ++i) {
@@ -434,23 +435,26 @@ DT_HD void stress_fwd(const M3& Fn, const M3& U, const float (&sig)[3], const M3
       // U diag(1) V^T and det of it in float32 it would be rounding noise of
       // size mu 1e-7, whose adjoint is noise too)
       st.Fp = st.R;
-      st.J = 1.f;
+      J = 1.f;
     } else {
       if (st.mode) {
         float d[3] = {__expf(st.epsn[0]), __expf(st.epsn[1]), __expf(st.epsn[2])};
         udv(U, d, V, st.Fp);
       }
-      st.J = m3_det(st.Fp);
+      J = m3_det(st.Fp);
     }
   } else {
-    st.J = sig[0] * sig[1] * sig[2];
+    J = sig[0] * sig[1] * sig[2];
     st.R = m3_mul_bt(U, V);
   }
+  // J is kept in a scalar until here: assigning st.J in the branches made the
+  // compiler keep it in private memory
+  st.J = J;
   // tau = 2 mu (Fp - R) Fp^T + lam J (J - 1) I
   M3 D;
   for (int i = 0; i < 9; ++i) D.a[i] = st.Fp.a[i] - st.R.a[i];
   tau = m3_mul_bt(D, st.Fp);
-  float l = m.lam * st.J * (st.J - 1.f);
+  float l = m.lam * J * (J - 1.f);
   for (int i = 0; i < 9; ++i) tau.a[i] *= 2.f * m.mu;
   tau.a[0] += l; tau.a[4] += l; tau.a[8] += l;
 }

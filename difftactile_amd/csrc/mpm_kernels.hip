@@ -494,7 +494,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmP P, const dt_material* __res
 struct P2GIn {
   float x[3], v[3];
   M3 c, f;
-  int mid;
+  int mid, env;   // material row; environment (0 unless stacked)
 };
 __device__ __forceinline__ void p2g_load(const MpmP& P, int n, const int32_t* mat_id,
                                          const float* x, const float* v, const float* C,
@@ -505,15 +505,17 @@ __device__ __forceinline__ void p2g_load(const MpmP& P, int n, const int32_t* ma
   ld_v3(x, p, Lv, in.x);
   ld_v3(v, p, Lv, in.v);
   in.mid = mat_id ? mat_id[p] : 0;
+  in.env = P.env_id ? P.env_id[p] : 0;
 }
 
 // compute_new_F + svd + return map of particle p; writes F_next and returns
 // the stencil and the scatter coefficients (b, M) of the p2g contribution
 // w_o (p_mass v + A (o - fx) dx) = w_o (b + M o)
-__device__ __forceinline__ void p2g_fused_compute(const MpmP& P0, const dt_material* mats, int n,
-                                                  const P2GIn& in, float* F_next, int p,
-                                                  Stencil& s, float (&b)[3], M3& M) {
-  const MpmP P = at_particle(P0, p);
+// P: the view of the parameters for the particle's environment, x0 the
+// environment's first x-node
+__device__ __forceinline__ void p2g_compute_at(const MpmP& P, int x0, const dt_material* mats,
+                                               int n, const P2GIn& in, float* F_next, int p,
+                                               Stencil& s, float (&b)[3], M3& M) {
   PL Lm = pl_mat(P.soa, n);
   M3 a = m3_eye();
   for (int i = 0; i < 9; ++i) a.a[i] += P.dt * in.c.a[i];
@@ -527,12 +529,18 @@ __device__ __forceinline__ void p2g_fused_compute(const MpmP& P0, const dt_mater
   stress_fwd(fn, u, sig, vv, m, st, tau);
   const M3 A = affine_of(P, tau, in.c);
   st_m3L(F_next, p, Lm, st.Fp);
-  make_stencil(in.x, P.inv_dx, P.n_grid, s, env_x0(P, p));
+  make_stencil(in.x, P.inv_dx, P.n_grid, s, x0);
   for (int q = 0; q < 3; ++q) {
     b[q] = P.p_mass * in.v[q] -
            (A.a[3 * q] * s.fx[0] + A.a[3 * q + 1] * s.fx[1] + A.a[3 * q + 2] * s.fx[2]) * P.dx;
     for (int r = 0; r < 3; ++r) M.a[3 * q + r] = A.a[3 * q + r] * P.dx;
   }
+}
+
+__device__ __forceinline__ void p2g_fused_compute(const MpmP& P, const dt_material* mats, int n,
+                                                  const P2GIn& in, float* F_next, int p,
+                                                  Stencil& s, float (&b)[3], M3& M) {
+  p2g_compute_at(at_particle(P, p), in.env * P.n_grid, mats, n, in, F_next, p, s, b, M);
 }
 
 __device__ __forceinline__ void p2g_fused_particle(const MpmP& P, const dt_material* mats, int n,
@@ -1448,6 +1456,7 @@ struct GvTile {
 struct G2PAdjIn {
   float x[3], g1[3], v1[3];
   M3 gc;
+  int env;   // environment (0 unless stacked)
 };
 __device__ __forceinline__ void g2p_adj_load(const MpmP& P, int n, const float* x, const float* gx1,
                                              const float* gv1, const float* gC1, int p,
@@ -1457,16 +1466,18 @@ __device__ __forceinline__ void g2p_adj_load(const MpmP& P, int n, const float* 
   ld_v3(gx1, p, Lv, in.g1);
   ld_v3(gv1, p, Lv, in.v1);
   in.gc = ld_m3L(gC1, p, Lm);
+  in.env = P.env_id ? P.env_id[p] : 0;
 }
 
+// P: the view of the parameters for the particle's environment, x0 the
+// environment's first x-node
 template <int kGxAssign, class Gv>
-__device__ __forceinline__ void g2p_adj_compute(const MpmP& P0, int n, const G2PAdjIn& in,
-                                                const Gv& gvsrc, float* gx, int p, Stencil& s,
-                                                float (&b)[3], M3& M) {
-  const MpmP P = at_particle(P0, p);
+__device__ __forceinline__ void g2p_adj_compute_at(const MpmP& P, int x0, int n,
+                                                   const G2PAdjIn& in, const Gv& gvsrc, float* gx,
+                                                   int p, Stencil& s, float (&b)[3], M3& M) {
   PL Lv = pl_vec(P.soa, n);
   float gvt[3], lgx[3];
-  make_stencil(in.x, P.inv_dx, P.n_grid, s, env_x0(P, p));
+  make_stencil(in.x, P.inv_dx, P.n_grid, s, x0);
   for (int a = 0; a < 3; ++a) {
     gvt[a] = in.v1[a] + P.dt * in.g1[a];
     lgx[a] = in.g1[a];
@@ -1491,6 +1502,14 @@ __device__ __forceinline__ void g2p_adj_compute(const MpmP& P0, int n, const G2P
     if (kGxAssign) gx[o] = lgx[a];
     else gx[o] += lgx[a];
   }
+}
+
+template <int kGxAssign, class Gv>
+__device__ __forceinline__ void g2p_adj_compute(const MpmP& P, int n, const G2PAdjIn& in,
+                                                const Gv& gvsrc, float* gx, int p, Stencil& s,
+                                                float (&b)[3], M3& M) {
+  g2p_adj_compute_at<kGxAssign>(at_particle(P, p), in.env * P.n_grid, n, in, gvsrc, gx, p, s, b,
+                                M);
 }
 
 template <int kGxAssign, class Gv>
@@ -1539,15 +1558,20 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(MpmP P, int n, const float* 
 //   per chunk  1. thread t computes particle t; a stable LDS counting sort by
 //                 cell (64 bins, per-wavefront counts) gives its staging slot,
 //                 where it stages (w, b, M);
-//              2. work item (cell, j, k) sums, over the cell's particles, the
-//                 contributions w_o (b + M o) to the three nodes o = (i, j, k),
-//                 i = 0..2, into its own LDS slots (lanes of one cell read the
-//                 same staged particle: LDS broadcast);
-//   per brick  3. tile node thread t < 216 adds the 27 slots that can name it
-//                 (slots of absent cells stay zero).
+//              2. work item (cy, cz, j, k) owns the x-row of six tile nodes
+//                 (x, cy + j, cz + k): it sums, over the particles of the four
+//                 cells (cx, cy, cz), cx = 0..3, the contributions
+//                 w_o (b + M o) to the nodes x = cx + i, i = 0..2, and adds
+//                 the row to its own LDS slots (the nine lanes of one (cy, cz)
+//                 read the same staged particle: LDS broadcast);
+//   per brick  3. tile node thread t < 216 adds the at most nine row slots
+//                 that can name it.
+// The row slots (864 float4) keep the block within 40 KB of LDS, so four
+// blocks share a CU.
 // No atomics anywhere; the tiles are summed per node by the grid passes below.
 // ----------------------------------------------------------------------------
-constexpr int kSlots = 64 * 27;
+constexpr int kRowItems = 16 * 9;                  // (cy, cz, j, k)
+constexpr int kSlots = kRowItems * kTileEdge;      // (cy, cz, j, k, x)
 struct BrickStage {
   // particle q at row r = stage_row(q): pd[r+0] = (wx0, wx1, wx2, b0),
   // pd[r+1] = (wy0.., b1), pd[r+2] = (wz0.., b2), pd[r+3..5] = M row-major
@@ -1558,12 +1582,10 @@ struct BrickStage {
     float4 pd[6 * kBlock + kBlock / 8];
     float4 gtile[kTileNodes];
   };
-  float4 slot[kSlots + 1];   // (cell, j, k, i): (sum w u0, w u1, w u2, w); last stays 0
+  float4 slot[kSlots + 1];   // (cy, cz, j, k, x): (sum w u0, w u1, w u2, w); last stays 0
   uint16_t cs[66];           // cell starts of the chunk (staging slots)
   uint16_t hist[kBlock / 64][64]; // per wavefront: particles per cell of the chunk,
                                   //   then its first staging slot in each cell
-  uint8_t cells[64];         // occupied cells of the chunk
-  int n_cells;
 };
 
 // First float4 of staged particle q.  Six float4 per particle put particles
@@ -1580,7 +1602,7 @@ __device__ __forceinline__ int local_cell(const Stencil& s) {
 // wavefront 0: cell starts (exclusive scan of the per-cell totals), each
 // wavefront's first slot in each cell (its particles follow those of the
 // wavefronts before it: the staging order is the chunk order, cell by cell,
-// so it is the same on every run) and the occupied-cell list
+// so it is the same on every run)
 __device__ __forceinline__ void chunk_scan(BrickStage& sh) {
   constexpr int kW = kBlock / 64;
   const int l = threadIdx.x;   // 0..63, one full wavefront
@@ -1603,9 +1625,6 @@ __device__ __forceinline__ void chunk_scan(BrickStage& sh) {
     sh.hist[w][l] = (uint16_t)at;
     at += hw[w];
   }
-  const unsigned long long m = __ballot(h > 0);
-  if (h > 0) sh.cells[__popcll(m & ((1ull << l) - 1ull))] = (uint8_t)l;
-  if (l == 0) sh.n_cells = __popcll(m);
 }
 
 __device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Stencil& s,
@@ -1619,50 +1638,61 @@ __device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Sten
   q[5] = make_float4(M.a[8], 0.f, 0.f, 0.f);
 }
 
-// step 2: items over the occupied cells of the chunk
+// step 2: x-row items (cy, cz, j, k) over the chunk's particles; thread t < 144
+// takes item t
 __device__ __forceinline__ void brick_items(BrickStage& sh) {
-  const int n_items = sh.n_cells * 9;
+  const int it = threadIdx.x;
+  if (it >= kRowItems) return;
+  const int yz = it / 9, jk = it - 9 * yz;
+  const int cy = yz >> 2, cz = yz & 3;
+  const int j = jk / 3, k = jk - 3 * j;
+  const float fj = (float)j, fk = (float)k;
   const float* pf = reinterpret_cast<const float*>(sh.pd);
-  for (int it = threadIdx.x; it < n_items; it += blockDim.x) {
-    const int cell = sh.cells[it / 9], jk = it % 9;
-    const int j = jk / 3, k = jk - 3 * j;
-    const float fj = (float)j, fk = (float)k;
+  float4 acc[kTileEdge];
+#pragma unroll
+  for (int x = 0; x < kTileEdge; ++x) acc[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int cx = 0; cx < 4; ++cx) {
+    const int cell = (cx * 4 + cy) * 4 + cz;
     const int q0 = sh.cs[cell], q1 = sh.cs[cell + 1];
-    float4 acc0 = make_float4(0.f, 0.f, 0.f, 0.f), acc1 = acc0, acc2 = acc0;
     for (int q = q0; q < q1; ++q) {
       const float4* pq = sh.pd + stage_row(q);
       const float* fq = pf + 4 * stage_row(q);
       const float4 X = pq[0], M0 = pq[3], M1 = pq[4];
       const float wy = fq[4 + j], b1 = fq[7], wz = fq[8 + k], b2 = fq[11], m22 = fq[20];
       const float wyz = wy * wz;
-      // u = b + M (0, j, k); node i adds w_i (u + M (i, 0, 0))
+      // u = b + M (0, j, k); node (cx + i, j, k) adds w_i (u + M (i, 0, 0))
       const float u0 = X.w + M0.y * fj + M0.z * fk;
       const float u1 = b1 + M1.x * fj + M1.y * fk;
       const float u2 = b2 + M1.w * fj + m22 * fk;
       const float m00 = M0.x, m10 = M0.w, m20 = M1.z;
       const float w0 = X.x * wyz, w1 = X.y * wyz, w2 = X.z * wyz;
-      acc0.x += w0 * u0;
-      acc0.y += w0 * u1;
-      acc0.z += w0 * u2;
-      acc0.w += w0;
-      acc1.x += w1 * (u0 + m00);
-      acc1.y += w1 * (u1 + m10);
-      acc1.z += w1 * (u2 + m20);
-      acc1.w += w1;
-      acc2.x += w2 * (u0 + 2.f * m00);
-      acc2.y += w2 * (u1 + 2.f * m10);
-      acc2.z += w2 * (u2 + 2.f * m20);
-      acc2.w += w2;
+      float4& a0 = acc[cx];
+      float4& a1 = acc[cx + 1];
+      float4& a2 = acc[cx + 2];
+      a0.x += w0 * u0;
+      a0.y += w0 * u1;
+      a0.z += w0 * u2;
+      a0.w += w0;
+      a1.x += w1 * (u0 + m00);
+      a1.y += w1 * (u1 + m10);
+      a1.z += w1 * (u2 + m20);
+      a1.w += w1;
+      a2.x += w2 * (u0 + 2.f * m00);
+      a2.y += w2 * (u1 + 2.f * m10);
+      a2.z += w2 * (u2 + 2.f * m20);
+      a2.w += w2;
     }
-    float4* sl = sh.slot + (cell * 9 + jk) * 3;
-    const float4 o0 = sl[0], o1 = sl[1], o2 = sl[2];
-    sl[0] = make_float4(o0.x + acc0.x, o0.y + acc0.y, o0.z + acc0.z, o0.w + acc0.w);
-    sl[1] = make_float4(o1.x + acc1.x, o1.y + acc1.y, o1.z + acc1.z, o1.w + acc1.w);
-    sl[2] = make_float4(o2.x + acc2.x, o2.y + acc2.y, o2.z + acc2.z, o2.w + acc2.w);
+  }
+  float4* sl = sh.slot + it * kTileEdge;
+#pragma unroll
+  for (int x = 0; x < kTileEdge; ++x) {
+    const float4 o = sl[x];
+    sl[x] = make_float4(o.x + acc[x].x, o.y + acc[x].y, o.z + acc[x].z, o.w + acc[x].w);
   }
 }
 
-// step 3: tile node thread t adds the slots of the cells that reach it
+// step 3: tile node thread t adds the row slots of the items that reach it
 __device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const int t = threadIdx.x;
@@ -1670,22 +1700,40 @@ __device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
   const int a0 = t / (kTileEdge * kTileEdge), a1 = (t / kTileEdge) % kTileEdge,
             a2 = t % kTileEdge;
 #pragma unroll
-  for (int o0 = 0; o0 < 3; ++o0)
+  for (int j = 0; j < 3; ++j)
 #pragma unroll
-    for (int o1 = 0; o1 < 3; ++o1)
-#pragma unroll
-      for (int o2 = 0; o2 < 3; ++o2) {
-        const int ca = a0 - o0, cb = a1 - o1, cc = a2 - o2;
-        const bool in = (unsigned)ca < 4u && (unsigned)cb < 4u && (unsigned)cc < 4u;
-        const int cell = (ca * 4 + cb) * 4 + cc;
-        const int idx = in ? (cell * 9 + o1 * 3 + o2) * 3 + o0 : kSlots;
-        const float4 v = sh.slot[idx];
-        acc.x += v.x;
-        acc.y += v.y;
-        acc.z += v.z;
-        acc.w += v.w;
-      }
+    for (int k = 0; k < 3; ++k) {
+      const int cy = a1 - j, cz = a2 - k;
+      const bool in = (unsigned)cy < 4u && (unsigned)cz < 4u;
+      const int idx = in ? ((cy * 4 + cz) * 9 + j * 3 + k) * kTileEdge + a0 : kSlots;
+      const float4 v = sh.slot[idx];
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
   return acc;
+}
+
+__device__ __forceinline__ float uniform_f(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+// the view of P for brick b's environment: every particle of a brick lives in
+// one environment (the slots of a ragged batch are multiples of 4 nodes).
+// The environment is uniform over the block, so its fields are kept in scalar
+// registers.
+__device__ __forceinline__ MpmP at_brick(const MpmP& P, long long b) {
+  if (!P.env_scale) return P;
+  const int nb = (P.n_grid + 3) >> 2;
+  const int e = (int)(b / ((long long)nb * nb)) * 4 / P.n_grid;
+  MpmP Q = at_env(P, __builtin_amdgcn_readfirstlane(e));
+  Q.dx = uniform_f(Q.dx);
+  Q.inv_dx = uniform_f(Q.inv_dx);
+  Q.p_mass = uniform_f(Q.p_mass);
+  Q.p_vol = uniform_f(Q.p_vol);
+  Q.n_wall = __builtin_amdgcn_readfirstlane(Q.n_wall);
+  return Q;
 }
 
 // Blocks stride over the occupied-brick list (a persistent grid: unoccupied
@@ -1694,12 +1742,16 @@ __device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
 // the first of the block's next brick) are in flight, and perm of chunk k + 1
 // is read at the start of chunk k so those loads need not wait for it.
 //   load(p, in)                 issues the loads of particle p's inputs
-//   compute(in, p, b, s, bb, M) particle p of brick b: stencil and (b, M)
+//   compute(in, p, Q, b, s, bb, M)  particle p of brick b (Q: the parameters
+//                               of its environment, read once per brick):
+//                               stencil and (b, M)
+//   mass_of(Q)                  the particle mass of a brick's tile
 //   fetch(b)                    (kTile) node t of brick b's 6x6x6 grid tile,
 //                               stored to sh.gtile at every chunk start of b
 //                               (the staging of a chunk overwrites it)
 template <bool kTile, class In, class MassOf, class Load, class Compute, class Fetch>
-__device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __restrict__ perm,
+__device__ __forceinline__ void brick_loop(const MpmP& P, BrickStage& sh,
+                                           const int32_t* __restrict__ perm,
                                            const int32_t* __restrict__ start,
                                            const int32_t* __restrict__ list,
                                            const int32_t* __restrict__ n_list,
@@ -1729,7 +1781,8 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
       s02 = start[b2];
       s12 = start[b2 + 1];
     }
-    for (int l = t; l < kSlots + 1; l += blockDim.x) sh.slot[l] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const MpmP Q = at_brick(P, b);
+    for (int l = t; l < kSlots + 1; l += kBlock) sh.slot[l] = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 tq2 = tq;
     for (int c0 = s0; c0 < s1; c0 += kBlock) {
       const int c1 = min(c0 + (int)kBlock, s1);
@@ -1752,7 +1805,7 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
       M3 M;
       int lc = 0;
       if (i < c1) {
-        compute(in, p, b, s, bb, M);
+        compute(in, p, Q, b, s, bb, M);
         lc = local_cell(s);
       }
       // rank among this wavefront's particles of the same cell; the first of
@@ -1778,7 +1831,7 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
     }
     float4 acc = brick_gather(sh);
     if (t < kTileNodes) {
-      acc.w *= mass_of(b);
+      acc.w *= mass_of(Q);
       tiles[b * kTileNodes + t] = acc;
     }
     __syncthreads();
@@ -1791,7 +1844,7 @@ __device__ __forceinline__ void brick_loop(BrickStage& sh, const int32_t* __rest
   }
 }
 
-__global__ __launch_bounds__(kBlock, 3) void k_p2g_binned(MpmP P, const dt_material* __restrict__ mats,
+__global__ __launch_bounds__(kBlock, 4) void k_p2g_binned(MpmP P, const dt_material* __restrict__ mats,
                                                        int n, const int32_t* __restrict__ mat_id,
                                                        const float* __restrict__ x,
                                                        const float* __restrict__ v,
@@ -1804,14 +1857,11 @@ __global__ __launch_bounds__(kBlock, 3) void k_p2g_binned(MpmP P, const dt_mater
                                                        const int32_t* __restrict__ n_list,
                                                        float4* __restrict__ tiles) {
   __shared__ BrickStage sh;
-  const int nbk = (P.n_grid + 3) >> 2;
   brick_loop<false, P2GIn>(
-      sh, perm, start, list, n_list, tiles,
-      // the brick's particles all live in its environment
-      [&](long long bb) { return at_xnode(P, (int)(bb / ((long long)nbk * nbk)) * 4).p_mass; },
+      P, sh, perm, start, list, n_list, tiles, [](const MpmP& Q) { return Q.p_mass; },
       [&](int p, P2GIn& in) { p2g_load(P, n, mat_id, x, v, C, F, p, in); },
-      [&](const P2GIn& in, int p, long long, Stencil& s, float(&bb)[3], M3& M) {
-        p2g_fused_compute(P, mats, n, in, F_next, p, s, bb, M);
+      [&](const P2GIn& in, int p, const MpmP& Q, long long, Stencil& s, float(&bb)[3], M3& M) {
+        p2g_compute_at(Q, in.env * P.n_grid, mats, n, in, F_next, p, s, bb, M);
       },
       [](long long) { return make_float4(0.f, 0.f, 0.f, 0.f); });
 }
@@ -1864,7 +1914,7 @@ __device__ __forceinline__ float4 fetch_gv_tile_op(const MpmP& P, long long b,
 // kFromMass: gv is the kept (mv, m) grid (float4 per node) and each brick's
 // velocity tile is made from it; else gv is the velocity grid (stride sgv)
 template <bool kFromMass>
-__global__ __launch_bounds__(kBlock, 3) void k_g2p_adj_binned(MpmP P, int n,
+__global__ __launch_bounds__(kBlock, 4) void k_g2p_adj_binned(MpmP P, int n,
                                                            const float* __restrict__ x,
                                                            const float* __restrict__ gv, int sgv,
                                                            const float* __restrict__ gx1,
@@ -1881,12 +1931,13 @@ __global__ __launch_bounds__(kBlock, 3) void k_g2p_adj_binned(MpmP P, int n,
   __shared__ BrickStage sh;
   const int nb = (P.n_grid + 3) >> 2;
   brick_loop<true, G2PAdjIn>(
-      sh, perm, start, list, n_list, tiles, [](long long) { return 0.f; },
+      P, sh, perm, start, list, n_list, tiles, [](const MpmP&) { return 0.f; },
       [&](int p, G2PAdjIn& in) { g2p_adj_load(P, n, x, gx1, gv1, gC1, p, in); },
-      [&](const G2PAdjIn& in, int p, long long b, Stencil& s, float(&bb)[3], M3& M) {
+      [&](const G2PAdjIn& in, int p, const MpmP& Q, long long b, Stencil& s, float(&bb)[3],
+          M3& M) {
         const GvTile src{sh.gtile, (int)(b / ((long long)nb * nb)) * 4, (int)((b / nb) % nb) * 4,
                          (int)(b % nb) * 4};
-        g2p_adj_compute<1>(P, n, in, src, gx, p, s, bb, M);
+        g2p_adj_compute_at<1>(Q, in.env * P.n_grid, n, in, src, gx, p, s, bb, M);
       },
       [&](long long b) {
         return kFromMass ? fetch_gv_tile_op(P, b, reinterpret_cast<const float4*>(gv))
@@ -1987,6 +2038,8 @@ int validate_params(const dt_mpm_params* p) {
                  (1ll << 31),
              "n_env * n_grid^3 must stay below 2^31 nodes");
   DT_REQUIRE(p->dx > 0.f && p->inv_dx > 0.f, "dx and inv_dx must be positive");
+  DT_REQUIRE(p->n_env <= 1 || !p->env_scale || !p->env_ngrid || p->n_grid % 4 == 0,
+             "ragged environments need a slot grid size that is a multiple of 4");
   return DT_OK;
 }
 

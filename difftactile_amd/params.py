@@ -60,6 +60,8 @@ class MPMParams:
         import torch
         if len(env_prms) != self.n_env:
             raise ValueError("one MPMParams per environment")
+        if self.n_grid % 4:
+            raise ValueError("the slot grid size of a ragged batch must be a multiple of 4")
         for q in env_prms:
             if q.n_grid > self.n_grid or q.n_grid % 4:
                 raise ValueError("environment grids must be multiples of 4 within the slot")
