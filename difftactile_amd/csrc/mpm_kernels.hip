@@ -1639,7 +1639,9 @@ __device__ __forceinline__ void stage_particle(BrickStage& sh, int t, const Sten
 }
 
 // step 2: x-row items (cy, cz, j, k) over the chunk's particles; thread t < 144
-// takes item t
+// takes item t.  kMass: also sum the weights (the mass channel); the g2p
+// adjoint has none.
+template <bool kMass>
 __device__ __forceinline__ void brick_items(BrickStage& sh) {
   const int it = threadIdx.x;
   if (it >= kRowItems) return;
@@ -1673,15 +1675,17 @@ __device__ __forceinline__ void brick_items(BrickStage& sh) {
       a0.x += w0 * u0;
       a0.y += w0 * u1;
       a0.z += w0 * u2;
-      a0.w += w0;
       a1.x += w1 * (u0 + m00);
       a1.y += w1 * (u1 + m10);
       a1.z += w1 * (u2 + m20);
-      a1.w += w1;
       a2.x += w2 * (u0 + 2.f * m00);
       a2.y += w2 * (u1 + 2.f * m10);
       a2.z += w2 * (u2 + 2.f * m20);
-      a2.w += w2;
+      if (kMass) {
+        a0.w += w0;
+        a1.w += w1;
+        a2.w += w2;
+      }
     }
   }
   float4* sl = sh.slot + it * kTileEdge;
@@ -1826,7 +1830,7 @@ __device__ __forceinline__ void brick_loop(const MpmP& P, BrickStage& sh,
       __syncthreads();
       if (i < c1) stage_particle(sh, sh.hist[wv][lc] + r, s, bb, M);
       __syncthreads();
-      brick_items(sh);
+      brick_items<!kTile>(sh);
       __syncthreads();
     }
     float4 acc = brick_gather(sh);
