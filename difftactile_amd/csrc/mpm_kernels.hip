@@ -1696,10 +1696,14 @@ __device__ __forceinline__ void brick_items(BrickStage& sh) {
   }
 }
 
-// step 3: tile node thread t adds the row slots of the items that reach it
+// step 3: tile node thread t adds the row slots of the items that reach it.
+// The thread index goes through an empty asm so its nine slot indices are
+// computed here each time, not hoisted out of the brick loop and held (or
+// spilled) across the whole kernel.
 __device__ __forceinline__ float4 brick_gather(const BrickStage& sh) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int t = threadIdx.x;
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
   if (t >= kTileNodes) return acc;
   const int a0 = t / (kTileEdge * kTileEdge), a1 = (t / kTileEdge) % kTileEdge,
             a2 = t % kTileEdge;
