@@ -224,8 +224,9 @@ def algorithmic_bytes(n, active, has_mat):
         "grid_op4_tiles": active * 32,
         # read x, write x v C; gather v per active node
         "g2p4": n * (12 + 60) + active * 16,
-        # read x, gx' gv' gC' (60), write gx; produce g_v per active node
-        "g2p4_adj_binned": n * (12 + 60 + 12) + active * 16,
+        # read x, gx' gv' gC' (60), write gx; read the forward's grid velocity
+        # (remade from the kept (mv, m)) and produce g_v per active node
+        "g2p4_adj_binned": n * (12 + 60 + 12) + active * 32,
         # read (mv, m), g_v; write (g_mv, g_m) per active node
         "grid_op4_adj_tiles": active * 48,
         # read x v C F (+id), gx partial, gF'; write gx gv gC gF; gather (g_mv, g_m)
