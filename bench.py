@@ -434,8 +434,6 @@ def bench_pad_press(args, torch):
     pad = Pad(nodes.astype(np.float32), f2v, tris, layer, od, c["pad_mu"], c["pad_lam"],
               c["pad_mass"], prm.dt)
     cs = CoupledSim(sim, [pad], **c["contact"])
-    seg_bytes = 51 * cs.substep_bytes()
-    kept = min(T // 50, sim.resident_bytes // seg_bytes)
     surf_ids = np.where(layer == layer.max())[0]
     mm = MarkerModel(nodes, surf_ids, c["inv_T"])
     dev = torch.device("cuda")
@@ -477,8 +475,9 @@ def bench_pad_press(args, torch):
                                   "loss = summed marker displacement"},
            "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(g).all()),
            "grad_dL_dvy_first_last": [float(gv[0, 1]), float(gv[-1, 1])],
-           "block_min_dy": dy, "resident_segments": f"{kept} of {T // 50}",
-           "resident_gb": round(kept * seg_bytes / 1e9, 1)}
+           "block_min_dy": dy,
+           "resident_segments": f"{cs.last_resident[0]} of {cs.last_resident[1]}",
+           "resident_gb": round(cs.last_resident[2] / 1e9, 1)}
     print(json.dumps(res), flush=True)
 
 
@@ -675,8 +674,6 @@ def bench_gripper(args, torch):
         init_x.append(torch.tensor((nh @ np.linalg.inv(Pk).T)[:, :3].astype(np.float32),
                                    device=dev))
         mms.append(MarkerModel(nodes, np.where(layer == layer.max())[0], c["inv_T"][k]))
-    seg_bytes = 51 * cs.substep_bytes()
-    kept = min(T // 50, sim.resident_bytes // seg_bytes)
     x0 = torch.tensor(c["x"], device=dev)
     v0 = torch.tensor(c["v"], device=dev)
     C0 = torch.zeros(n, 3, 3, device=dev)
@@ -720,8 +717,8 @@ def bench_gripper(args, torch):
                                   f"block ({n} particles, {prm.n_grid}^3), {T} substeps fwd+bwd "
                                   "checkpointed every 50, loss = centre-of-mass error + mean "
                                   "marker offset, gradient w.r.t. the jaw controls",
-                      "resident_segments": f"{kept} of {T // 50}",
-                      "resident_gb": round(kept * seg_bytes / 1e9, 1)},
+                      "resident_segments": f"{cs.last_resident[0]} of {cs.last_resident[1]}",
+                      "resident_gb": round(cs.last_resident[2] / 1e9, 1)},
            "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(gc).all()),
            "grad_abs_max": float(gc.abs().max()), "pad_face_travel": gap,
            "cable_max_displacement": cable_move}

@@ -138,6 +138,8 @@ _SIGS = {
     "dt_mpm_bin_copy": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _c_int, _vp]),
     "dt_mpm_active_bricks": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _P(_vp), _P(_vp)]),
     "dt_mpm_zero_nodes": (_c_int, [_P(DtMpmParams), _vp, _c_int, _vp]),
+    "dt_mpm_brick_nodes_copy": (_c_int, [_P(DtMpmParams), _c_int, _P(_vp), _P(_vp), _P(_c_int),
+                                         _c_int, _c_int, _vp, _vp]),
     "dt_mpm_p2g_binned": (_c_int, [_P(DtMpmParams), _vp, _c_int, _c_int, _vp] + [_vp] * 5
                           + [_vp, _sz, _vp]),
     "dt_mpm_grid_op4_tiles": (_c_int, [_P(DtMpmParams), _c_int, _vp, _sz, _vp, _vp, _vp]),

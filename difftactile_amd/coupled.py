@@ -143,6 +143,14 @@ class CoupledSim:
         self._ext = [torch.empty(pd.n_verts, 3, device=d) for pd in self.pads]
         self._ge = [torch.empty(pd.n_verts, 3, device=d) for pd in self.pads]
         self._last_ext = self._ext
+        # compact stores (sparse grids): a kept substep keeps the grids of its
+        # active bricks only (the adjoint's node passes read no other node);
+        # the substep runs on the same working grids as one that keeps nothing.
+        # None: when whole-grid stores do not fit the resident budget; True:
+        # always; False: never
+        self.compact_stores = None
+        self.overflow = torch.zeros(1, dtype=I32, device=d)
+        self.last_resident = (0, 0, 0)
 
     def _node_set(self, base, ws, nb):
         """Copy of the parameter struct base whose grid-node passes run over
@@ -154,11 +162,12 @@ class CoupledSim:
         p.active_bricks, p.n_active_bricks = a.value, c.value
         return p
 
-    def substep_bytes(self):
+    def substep_bytes(self, cap=None):
         """HBM bytes one kept substep holds for the backward pass: object and
         pad frames, the packed grids, contact forces, closest triangles and the
-        pads' updated velocities."""
-        nn = self.sim.prm.n_nodes
+        pads' updated velocities.  cap: the store keeps the nodes of at most
+        cap active bricks (None: every node)."""
+        nn = self.sim.prm.n_nodes if cap is None else 64 * cap
         b = 4 * self.sim.n * 24 + nn * (16 + 16 + 12 + 4 * self.npad) + self._bin_slot_bytes()
         for pd in self.pads:
             b += 4 * 3 * pd.n_verts * 4
@@ -189,13 +198,52 @@ class CoupledSim:
     def _copy_in(self, taken, key, t):
         return self._pool(taken, key, t.shape, t.dtype).copy_(t)
 
-    def _new_store(self, hint, taken=None, skey=()):
+    def store_capacity(self, x):
+        """Active bricks a compact store of a segment starting at the sorted
+        positions x has room for: the bricks active there, a quarter more and
+        64, rounded up to 1/8 of their power of two (so a steady caller's
+        segments reuse the same pooled buffers); None without compact stores.
+        Reads one count from the device."""
         sim = self.sim
-        nn = sim.prm.n_nodes
+        if self.compact_stores is False or not sim.sparse_grid:
+            return None
+        ws, nb = sim._workspace()
+        s = _lib.stream()
+        _lib.call("dt_mpm_bin", sim.c_prm, sim.n, P(x), ws, nb, s)
+        a, c = _lib.ctypes.c_void_p(), _lib.ctypes.c_void_p()
+        _lib.call("dt_mpm_active_bricks", sim.c_prm, sim.n, ws, nb, _lib.ctypes.byref(a),
+                  _lib.ctypes.byref(c))
+        wst = sim._ws["bin_ws"]
+        off = c.value - wst.data_ptr()
+        n_act = int(wst[off:off + 4].view(I32).item())
+        want = int(n_act * 1.25) + 64
+        g = max(64, 1 << max(0, want.bit_length() - 4))
+        total = -(-sim.prm.n_nodes // 64)
+        return min(-(-want // g) * g, total + 64)
+
+    def _store_copy(self, prm, store, direction, grids):
+        """Pack (direction 0) the active bricks' nodes of the working grids
+        (g4, gv4, gf, idx) into a compact store, or unpack (1) them back."""
+        sv = _lib.ctypes.c_void_p * 4
+        names = ("g4", "gv4", "gf", "idx")
+        dense = sv(*[t.data_ptr() for t in grids])
+        comp = sv(*[store[c].data_ptr() for c in names])
+        widths = (_lib.ctypes.c_int32 * 4)(4, 4, 3, self.npad)
+        _lib.call("dt_mpm_brick_nodes_copy", prm, 4, dense, comp, widths, store["cap"], direction,
+                  P(self.overflow, I32) if direction == 0 else None, _lib.stream())
+
+    def _overflowed(self):
+        hit = bool(self.overflow.item())
+        self.overflow.zero_()
+        return hit
+
+    def _new_store(self, hint, taken=None, skey=(), cap=None):
+        sim = self.sim
+        nn = sim.prm.n_nodes if cap is None else 64 * cap
         b = lambda name, shape, dtype=torch.float32: self._pool(taken, ("store",) + skey + (name,),
                                                                 shape, dtype)
         return {"g4": b("g4", (nn, 4)), "gv4": b("gv4", (nn, 4)), "gf": b("gf", (nn, 3)),
-                "idx": b("idx", (nn, self.npad), I32), "hint": hint,
+                "idx": b("idx", (nn, self.npad), I32), "hint": hint, "cap": cap,
                 "vel_u": [b(("vel_u", k), (pd.n_verts, 3)) for k, pd in enumerate(self.pads)],
                 "ext": [b(("ext", k), (pd.n_verts, 3)) for k, pd in enumerate(self.pads)],
                 "ridx": (None if self.rope is None else
@@ -204,11 +252,12 @@ class CoupledSim:
                 "bin": (b("bin", (self._bin_slot_bytes(),), torch.uint8)
                         if sim.sparse_grid else None)}
 
-    def _run_keep(self, st0, pst0, cvs, s0, k, mat_id, taken=None, seg=()):
+    def _run_keep(self, st0, pst0, cvs, s0, k, mat_id, taken=None, seg=(), cap=None):
         """Advance substeps s0 .. s0+k-1 from (st0, pst0) keeping every frame
         and each substep's store.  Returns (frames, pad frames, stores); st0
         and pst0 are copied, not kept.  taken / seg: the frames and stores
-        come from the buffer pool under keys of segment seg."""
+        come from the buffer pool under keys of segment seg.  cap: compact
+        stores with room for cap active bricks (None: dense stores)."""
         fb = lambda j, a, t: self._pool(taken, ("frame",) + seg + (j, a), t.shape, t.dtype)
         pb = lambda j, b_, c, t: self._pool(taken, ("pframe",) + seg + (j, b_, c), t.shape, t.dtype)
         frames = [[fb(0, a, t).copy_(t) for a, t in enumerate(st0)]]
@@ -216,7 +265,8 @@ class CoupledSim:
         stores = []
         for j in range(k):
             # closest-triangle starts: the previous substep's result
-            store = self._new_store(stores[-1]["idx"] if stores else self.idx, taken, seg + (j,))
+            store = self._new_store(stores[-1]["idx"] if stores else self.idx, taken, seg + (j,),
+                                    cap)
             nxt = [fb(j + 1, a, t) for a, t in enumerate(frames[-1])]
             pn = [[pb(j + 1, b_, c, t) for c, t in enumerate(p)] for b_, p in enumerate(pframes[-1])]
             f = s0 + j
@@ -312,12 +362,14 @@ class CoupledSim:
         sim, s = self.sim, _lib.stream()
         nn = sim.prm.n_nodes
         prm = sim.c_prm
-        g4 = store["g4"] if store else sim._grid_scratch()[0]
-        gv4 = store["gv4"] if store else sim._grid_scratch()[1]
-        gf = store["gf"] if store else self.grid_f
-        idx = store["idx"] if store else self.idx
+        compact = bool(store) and store.get("cap") is not None
+        dense = bool(store) and not compact
+        g4 = store["g4"] if dense else sim._grid_scratch()[0]
+        gv4 = store["gv4"] if dense else sim._grid_scratch()[1]
+        gf = store["gf"] if dense else self.grid_f
+        idx = store["idx"] if dense else self.idx
         # closest-triangle starts: the previous substep's result
-        hint = store.get("hint", idx) if store else idx
+        hint = store.get("hint", idx) if dense else idx
         mats, nm = P(sim.mats), len(sim.materials)
         mid = None if mat_id is None else P(mat_id, I32)
         if sim.sparse_grid:
@@ -384,6 +436,9 @@ class CoupledSim:
                           P(pads_in[k][0]), P(vel_u[k]), pad.od(), P(rimp), pad.n_verts,
                           P(exts[k]), None, pad.fx_acc.data_ptr(), pad.fx_scale.data_ptr(), s)
         _lib.call("dt_mpm_grid_op4", prm, P(g4), P(gf), P(gv4), s)
+        if compact:
+            # keep the active bricks' nodes of the grids the adjoint reads
+            self._store_copy(prm, store, 0, (g4, gv4, gf, idx))
         _lib.call("dt_mpm_g2p4", prm, sim.n, P(st[0]), P(gv4), P(nxt[0]), P(nxt[1]), P(nxt[2]), s)
         if self.rope is not None:
             ro = pads_out[npd]
@@ -414,7 +469,12 @@ class CoupledSim:
         sim, s = self.sim, _lib.stream()
         nn = sim.prm.n_nodes
         prm = sim.c_prm
-        g4, gv4, gf, idx = store["g4"], store["gv4"], store["gf"], store["idx"]
+        compact = store.get("cap") is not None
+        if compact:
+            # the working grids, filled from the store below
+            (g4, gv4), gf, idx = sim._grid_scratch(), self.grid_f, self.idx
+        else:
+            g4, gv4, gf, idx = store["g4"], store["gv4"], store["gf"], store["idx"]
         gg4 = sim._buf("cgg4", (nn, 4))
         ggv4 = sim._buf("cggv4", (nn, 4))
         ggf = sim._buf("cggf", (nn, 3))
@@ -457,6 +517,9 @@ class CoupledSim:
                 _lib.call("dt_mpm_bin", prm, sim.n, P(st[0]), ws, nb, s)
             prm = self._node_set(prm, ws, nb)
             prm_clamp = self._node_set(prm_clamp, ws, nb)
+            if compact:
+                # the active bricks of this binning are the ones the forward kept
+                self._store_copy(prm, store, 1, (g4, gv4, gf, idx))
             _lib.call("dt_mpm_g2p4_adj_binned", prm, sim.n, P(st[0]), P(gv4), P(g_next[0]),
                       P(g_next[1]), P(g_next[2]), P(g[0]), ws, nb, s)
             # tile sums fused into the grid_op adjoint
@@ -564,18 +627,33 @@ class _CoupledRollout(torch.autograd.Function):
         ckpts, kept = [], []
         keep_ok = any(ctx.needs_input_grad)
         used = 0
-        for i, (s0, k) in enumerate(_segments(n_substeps, sim.checkpoint_every)):
-            nbytes = (k + 1) * cs.substep_bytes()
+        segs = _segments(n_substeps, sim.checkpoint_every)
+        # compact stores only when whole-grid stores of every segment do not
+        # fit the budget (packing them costs a launch per substep each way)
+        compact = keep_ok and sim.sparse_grid and (
+            cs.compact_stores is True or (cs.compact_stores is None and sum(
+                (k + 1) * cs.substep_bytes() for _, k in segs) > sim.resident_bytes))
+        ctx.compact = compact
+        for i, (s0, k) in enumerate(segs):
+            cap = cs.store_capacity(st[0]) if compact else None
+            nbytes = (k + 1) * cs.substep_bytes(cap)
             if keep_ok and used + nbytes <= sim.resident_bytes:
                 # resident segment: frames and stores kept for the backward pass
                 used += nbytes
-                seg = cs._run_keep(st, pst, cvs, s0, k, mat_id, taken, ("seg", i))
-                ckpts.append(None)
-                kept.append(seg)
+                seg = cs._run_keep(st, pst, cvs, s0, k, mat_id, taken, ("seg", i), cap)
+                if cap is not None and cs._overflowed():
+                    # more active bricks than the stores hold: the backward
+                    # pass recomputes the segment from its first frame
+                    ckpts.append((seg[0][0], seg[1][0]))
+                    kept.append(None)
+                else:
+                    ckpts.append(None)
+                    kept.append(seg)
                 st = [P(("sc", a), t) for a, t in enumerate(seg[0][-1])]
                 pst = [[P(("psc", b_, c), t) for c, t in enumerate(p)]
                        for b_, p in enumerate(seg[1][-1])]
-                cs.idx.copy_(seg[2][-1]["idx"])
+                if cap is None:
+                    cs.idx.copy_(seg[2][-1]["idx"])
                 continue
             if keep_ok:
                 ckpts.append(([P(("ck", i, a), t) for a, t in enumerate(st)],
@@ -588,6 +666,8 @@ class _CoupledRollout(torch.autograd.Function):
                 cs._fwd(st, nxt, pst, pnx, [c[f] for c in cvs], [c[f + 1] for c in cvs], mat_id)
                 st, nxt = nxt, st
                 pst, pnx = pnx, pst
+        # (segments kept for the backward pass, segments, bytes they hold)
+        cs.last_resident = (sum(kp is not None for kp in kept), len(kept), used)
         out = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(st, W, shapes)]
         for p in pst:
             out += [t.clone() for t in p]
@@ -658,8 +738,17 @@ class _CoupledRollout(torch.autograd.Function):
                 # set of recompute buffers, back to the pool after each segment)
                 ck, ckp = ctx.ckpts[i]
                 seg_taken = []
+                cap = cs.store_capacity(ck[0]) if ctx.compact else None
                 frames, pframes, stores = cs._run_keep(ck, ckp, cvs, s0, k, ctx.mat_id,
-                                                       seg_taken, ("recompute",))
+                                                       seg_taken, ("recompute",), cap)
+                if cap is not None and cs._overflowed():
+                    # the recomputed segment outgrew its compact stores: again
+                    # with dense ones
+                    del frames, pframes, stores
+                    sim._give_back(seg_taken)
+                    seg_taken = []
+                    frames, pframes, stores = cs._run_keep(ck, ckp, cvs, s0, k, ctx.mat_id,
+                                                           seg_taken, ("recompute",))
             for j in reversed(range(k)):
                 f = s0 + j
                 cs._adj(frames[j], pframes[j], stores[j], g, gp, gbuf, gpbuf,

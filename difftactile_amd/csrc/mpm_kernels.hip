@@ -1128,6 +1128,44 @@ __global__ __launch_bounds__(kBlock) void k_zero_nodes(MpmP P, long long nn, flo
   });
 }
 
+// Up to four node arrays moved together by k_brick_nodes_copy
+struct NodeArrays {
+  const uint32_t* src[4];
+  uint32_t* dst[4];
+  int width[4];
+  int n;
+};
+constexpr int kMaxNodeArrays = 4;
+
+// Nodes of the active bricks of P (64 per brick, in list order) between dense
+// node arrays and compact ones, width[a] 32-bit words per node of array a:
+// kPack copies dense (src) -> compact (dst), else compact (src) -> dense
+// (dst).  Only the first cap bricks are copied; a pack of more sets *overflow
+// (lane 0 of block 0, a vector store).  Lanes of nodes outside the grid copy
+// nothing.
+template <bool kPack>
+__global__ __launch_bounds__(kBlock) void k_brick_nodes_copy(MpmP P, NodeArrays A, int cap,
+                                                             int32_t* __restrict__ overflow) {
+  const int n_act = *P.n_alist;
+  if (kPack && overflow && blockIdx.x == 0 && threadIdx.x == 0 && n_act > cap) *overflow = 1;
+  const int n = n_act < cap ? n_act : cap;
+  const int l = threadIdx.x & 63;
+  for (int ai = blockIdx.x * 4 + (threadIdx.x >> 6); ai < n; ai += gridDim.x * 4) {
+    const long long nd = active_node(P, P.alist[ai]);
+    if (nd < 0) continue;
+    const long long cn = (long long)ai * 64 + l;
+    for (int a = 0; a < A.n; ++a) {
+      const int w = A.width[a];
+      const uint32_t* __restrict__ src = A.src[a];
+      uint32_t* __restrict__ dst = A.dst[a];
+      for (int c = 0; c < w; ++c) {
+        if (kPack) dst[cn * w + c] = src[nd * w + c];
+        else dst[nd * w + c] = src[cn * w + c];
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_zero4(long long nn, float4* __restrict__ a) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nn) a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -2842,6 +2880,37 @@ int dt_mpm_zero_nodes(const dt_mpm_params* prm, float* buf, int32_t width, void*
   hipLaunchKernelGGL(k_zero_nodes, dim3(node_blocks(*prm, nn)), dim3(kBlock), 0,
                      DT_STREAM(stream), to_dev_nodes(*prm), nn, buf, (int)width);
   return check_launch("dt_mpm_zero_nodes");
+}
+
+int dt_mpm_brick_nodes_copy(const dt_mpm_params* prm, int32_t n_arrays, void* const* dense,
+                            void* const* compact, const int32_t* widths, int32_t capacity,
+                            int32_t direction, int32_t* overflow, void* stream) {
+  int rc = validate_params(prm);
+  if (rc) return rc;
+  DT_REQUIRE(prm->active_bricks && prm->n_active_bricks,
+             "dt_mpm_brick_nodes_copy needs the node set of prm (active_bricks)");
+  DT_REQUIRE(n_arrays >= 1 && n_arrays <= kMaxNodeArrays && dense && compact && widths,
+             "n_arrays must be in [1, 4] with its pointer and width arrays");
+  DT_REQUIRE(capacity >= 0, "capacity must be >= 0");
+  DT_REQUIRE(direction == 0 || direction == 1, "direction must be 0 (pack) or 1 (unpack)");
+  NodeArrays A{};
+  A.n = n_arrays;
+  for (int a = 0; a < n_arrays; ++a) {
+    DT_REQUIRE(dense[a] && compact[a] && widths[a] > 0, "NULL array or width < 1");
+    const bool pack = direction == 0;
+    A.src[a] = static_cast<const uint32_t*>(pack ? dense[a] : compact[a]);
+    A.dst[a] = static_cast<uint32_t*>(pack ? compact[a] : dense[a]);
+    A.width[a] = widths[a];
+  }
+  const MpmP P = to_dev_nodes(*prm);
+  const dim3 grid(node_blocks(*prm, grid_nodes(prm)));
+  if (direction == 0)
+    hipLaunchKernelGGL(k_brick_nodes_copy<true>, grid, dim3(kBlock), 0, DT_STREAM(stream), P, A,
+                       (int)capacity, overflow);
+  else
+    hipLaunchKernelGGL(k_brick_nodes_copy<false>, grid, dim3(kBlock), 0, DT_STREAM(stream), P, A,
+                       (int)capacity, nullptr);
+  return check_launch("dt_mpm_brick_nodes_copy");
 }
 
 int dt_mpm_p2g_fused_adj(const dt_mpm_params* prm, const dt_material* mats, int32_t n_mats,

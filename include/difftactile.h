@@ -390,6 +390,21 @@ int dt_mpm_active_bricks(const dt_mpm_params* prm, int32_t n, const void* worksp
 /* Zeroes width floats per node over the node set of prm (all nodes, or the
    active bricks). */
 int dt_mpm_zero_nodes(const dt_mpm_params* prm, float* buf, int32_t width, void* stream);
+/* Copies, for n_arrays (1..4) node arrays at once, widths[a] 32-bit words per
+   node between the dense array dense[a] and the compact array compact[a]
+   holding the nodes of prm's active bricks (64 per brick, in the order of the
+   active-brick list; prm must carry the node set): direction 0 packs
+   dense -> compact, 1 unpacks compact -> dense (other dense nodes are left as
+   they are).  dense, compact and widths are host arrays of device pointers
+   and widths.  Only the first capacity bricks are copied; a pack of more
+   bricks sets *overflow (a device int32, NULL: not reported) to 1.  No
+   counterpart in the reference, which keeps every substep's dense grids of a
+   window in its [sub_steps, n_grid^3] fields (mpm_elastic.py:82-87): the
+   coupled rollout keeps only the active bricks' nodes of each kept substep
+   when whole grids do not fit its HBM budget. */
+int dt_mpm_brick_nodes_copy(const dt_mpm_params* prm, int32_t n_arrays, void* const* dense,
+                            void* const* compact, const int32_t* widths, int32_t capacity,
+                            int32_t direction, int32_t* overflow, void* stream);
 /* Device pointers into the workspace: perm (n) and brick_start
    (dt_mpm_brick_count + 1) of the last dt_mpm_bin; the particles of brick b are
    perm[brick_start[b] .. brick_start[b+1]) in no particular order. */

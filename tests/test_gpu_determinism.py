@@ -105,3 +105,57 @@ def test_coupled_rollout_repeats_bit_for_bit(gpu):
     assert float(runs[0][-1].abs().max()) > 0
     for a, b in zip(runs[0], runs[1]):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("mode", ["dense", "overflow", "recompute_overflow"])
+def test_compact_stores_match_dense_stores(gpu, mode):
+    """Kept coupled substeps store only their active bricks' nodes (compact
+    stores, used when whole-grid stores do not fit the budget): the pad press's states
+    and gradients equal those of whole-grid stores bit for bit.  overflow: the
+    compact stores get room for one brick, so every kept segment overflows and
+    the backward pass recomputes it (its recompute overflows too and runs on
+    whole-grid stores); recompute_overflow: nothing is kept, the recomputed
+    segments overflow and run again on whole-grid stores."""
+    from difftactile_amd.coupled import CoupledSim, Pad
+    c = scenes.config2_pad_press(n_particles=6000, n_grid=32, n_substeps=20,
+                                 pad_verts=(12, 12, 3), press_substeps=100)
+    prm, n = c["prm"], len(c["x"])
+    nodes, f2v, tris, layer, od = c["mesh"]
+
+    def run(compact, tiny_cap=False, resident=True):
+        sim = MPMSim(prm, c["mats"], n, checkpoint_every=8,
+                     resident_bytes=None if resident else 0)
+        pad = Pad(nodes.astype(np.float32), f2v, tris, layer, od, c["pad_mu"], c["pad_lam"],
+                  c["pad_mass"], prm.dt)
+        cs = CoupledSim(sim, [pad], **c["contact"])
+        cs.compact_stores = compact
+        if tiny_cap:
+            cs.store_capacity = lambda x: 1
+        x0 = torch.tensor(c["x"]).cuda().requires_grad_(True)
+        cv = torch.tensor(c["control_vel"]).cuda().requires_grad_(True)
+        pos0 = torch.tensor(nodes, dtype=torch.float32).cuda()
+        out = cs.rollout(x0, torch.tensor(c["v"]).cuda(), torch.zeros(n, 3, 3).cuda(),
+                         torch.eye(3).repeat(n, 1, 1).cuda(),
+                         [(pos0.clone(), cv[0].detach().clone(), pos0.clone())], [cv], 20)
+        kept = cs.last_resident[0]
+        w = torch.randn(n, 3, generator=torch.Generator().manual_seed(8)).cuda()
+        wp = torch.randn(len(nodes), 3, generator=torch.Generator().manual_seed(9)).cuda()
+        loss = (out[0] * w).sum() + (out[4][0][0] * wp).sum() + (out[4][0][1] * wp).sum()
+        g = torch.autograd.grad(loss, [x0, cv])
+        torch.cuda.synchronize()
+        return kept, [t.detach().clone() for t in out[:4] + tuple(out[4][0])] + \
+            [t.clone() for t in g]
+
+    kept_d, ref = run(False)
+    assert kept_d == 3 and float(ref[-1].abs().max()) > 0
+    if mode == "dense":
+        kept_c, got = run(True)
+        assert kept_c == 3
+    elif mode == "overflow":
+        kept_c, got = run(True, tiny_cap=True)
+        assert kept_c == 0
+    else:
+        kept_c, got = run(True, tiny_cap=True, resident=False)
+        assert kept_c == 0
+    for a, b in zip(ref, got):
+        assert torch.equal(a, b)

@@ -430,9 +430,11 @@ def test_coupled_rollout_gradient_matches_oracle(gpu, sort, sparse, resident, cl
     sim = MPMSim(prm, [Material(5e4, 0.3)], n, checkpoint_every=4, sort=sort, sparse_grid=sparse)
     pad = Pad(nodes.astype(np.float32), f2v, surf, layer, od, mu, lam, mass, prm.dt)
     cs = CoupledSim(sim, [pad], kn=1e3, kd=10.0, kt=50.0, mu_f=0.5, clamp_grad=clamp)
-    # segments of 4 and 2 substeps: 5 and 3 kept substeps' worth of bytes
+    # segments of 4 and 2 substeps: 5 and 3 kept substeps' worth of bytes (a
+    # compact store's size follows the active bricks of its segment's start)
+    cap = cs.store_capacity(dev(x).t().contiguous()) if sparse else None
     sim.resident_bytes = {"none": 0, "all": 8 * cs.substep_bytes(),
-                          "first": 5 * cs.substep_bytes()}[resident]
+                          "first": 5 * cs.substep_bytes(cap)}[resident]
     rng = np.random.default_rng(11)
     cv = np.zeros((steps + 1, len(nodes), 3), np.float32)
     cv[:, :, 1] = -0.5 + 0.05 * rng.normal(size=(steps + 1, 1))
@@ -449,6 +451,7 @@ def test_coupled_rollout_gradient_matches_oracle(gpu, sort, sparse, resident, cl
     loss = loss + sum((o * dev(wi)).sum() for o, wi in zip(out[4][0], wp))
     leaves = st + [pst[0], pst[1], cvt]
     grads = torch.autograd.grad(loss, leaves)
+    assert cs.last_resident[0] == {"none": 0, "all": 2, "first": 1}[resident]
     # oracle
     op = oracle_params(prm)
     mats = om.material_arrays([om.Material(5e4, 0.3)], np.zeros(n, int), F64)
