@@ -1256,43 +1256,55 @@ __global__ __launch_bounds__(kBlock) void k_grid_op4_adj_bricks(MpmP P,
   gg4[nd] = out;
 }
 
-// Node value from the brick tiles: node g (brick B, local l) receives the tile
-// of brick B - d for every d in {0,1}^3 with l + 4 d < 6 whose brick holds
-// particles.  Fixed summation order.
-__device__ __forceinline__ float4 tile_sum(const MpmP& P, long long nd,
+// Node value from the brick tiles: node (brick bk, local l) receives the tile
+// of brick bk - d for every d in {0,1}^3 with l + 4 d < 6 whose brick holds
+// particles, summed in the order of d.  bk is one brick per wavefront: the
+// occupancy of the up to eight bricks whose tiles reach into it is read once
+// per wavefront (scalar loads), and a lane's tile loads are issued together.
+__device__ __forceinline__ float4 tile_sum(const MpmP& P, int bk,
                                            const float4* __restrict__ tiles,
                                            const int32_t* __restrict__ start) {
-  const int n = P.n_grid, nb = (n + 3) >> 2;
-  const int g2 = (int)(nd % n), g1 = (int)((nd / n) % n), g0 = (int)(nd / ((long long)n * n));
-  float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int d0 = 0; d0 < 2; ++d0) {
-    const int B0 = (g0 >> 2) - d0, l0 = (g0 & 3) + 4 * d0;
-    if (B0 < 0 || l0 >= kTileEdge) continue;
-    for (int d1 = 0; d1 < 2; ++d1) {
-      const int B1 = (g1 >> 2) - d1, l1 = (g1 & 3) + 4 * d1;
-      if (B1 < 0 || l1 >= kTileEdge) continue;
-      for (int d2 = 0; d2 < 2; ++d2) {
-        const int B2 = (g2 >> 2) - d2, l2 = (g2 & 3) + 4 * d2;
-        if (B2 < 0 || l2 >= kTileEdge) continue;
-        const long long B = ((long long)B0 * nb + B1) * nb + B2;
-        if (start[B] == start[B + 1]) continue;
-        const float4 q = tiles[B * kTileNodes + (l0 * kTileEdge + l1) * kTileEdge + l2];
-        r.x += q.x;
-        r.y += q.y;
-        r.z += q.z;
-        r.w += q.w;
-      }
+  const int nb = (P.n_grid + 3) >> 2;
+  const int bz = bk % nb, by = (bk / nb) % nb, bx = bk / (nb * nb);
+  const int l = threadIdx.x & 63;
+  const int l0 = l >> 4, l1 = (l >> 2) & 3, l2 = l & 3;
+  float4 q[8];
+  bool use[8];
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    const int d0 = d >> 2, d1 = (d >> 1) & 1, d2 = d & 1;
+    const int B0 = bx - d0, B1 = by - d1, B2 = bz - d2;
+    long long B = 0;
+    bool occ = B0 >= 0 && B1 >= 0 && B2 >= 0;
+    if (occ) {
+      B = ((long long)B0 * nb + B1) * nb + B2;
+      occ = start[B] != start[B + 1];
     }
+    use[d] = occ && (d0 == 0 || l0 < 2) && (d1 == 0 || l1 < 2) && (d2 == 0 || l2 < 2);
+    q[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (use[d])
+      q[d] = tiles[B * kTileNodes + ((l0 + 4 * d0) * kTileEdge + (l1 + 4 * d1)) * kTileEdge +
+                   (l2 + 4 * d2)];
   }
+  float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int d = 0; d < 8; ++d)
+    if (use[d]) {
+      r.x += q[d].x;
+      r.y += q[d].y;
+      r.z += q[d].z;
+      r.w += q[d].w;
+    }
   return r;
 }
 
 // Grid passes over the active-brick list: a persistent grid, 4 bricks (one per
-// wavefront) per block and iteration.
+// wavefront) per block and iteration; bk is the wavefront's brick.
 #define DT_ACTIVE_NODES(nd)                                                          \
   const int n_act = *n_alist;                                                        \
   for (int ai = blockIdx.x * 4 + (threadIdx.x >> 6); ai < n_act; ai += gridDim.x * 4) \
-    for (long long nd = active_node(P, alist[ai]); nd >= 0; nd = -1)
+    for (int bk = __builtin_amdgcn_readfirstlane(alist[ai]); bk >= 0; bk = -1)       \
+      for (long long nd = active_node(P, bk); nd >= 0; nd = -1)
 
 // node values of the active bricks summed from the tiles (no grid_op)
 __global__ __launch_bounds__(kBlock) void k_tiles_sum(MpmP P, const int32_t* __restrict__ alist,
@@ -1300,7 +1312,7 @@ __global__ __launch_bounds__(kBlock) void k_tiles_sum(MpmP P, const int32_t* __r
                                                       const float4* __restrict__ tiles,
                                                       const int32_t* __restrict__ start,
                                                       float4* __restrict__ out4) {
-  DT_ACTIVE_NODES(nd) { out4[nd] = tile_sum(P, nd, tiles, start); }
+  DT_ACTIVE_NODES(nd) { out4[nd] = tile_sum(P, bk, tiles, start); }
 }
 
 // the same, also clearing width floats per node of zbuf on the same nodes
@@ -1311,7 +1323,7 @@ __global__ __launch_bounds__(kBlock) void k_tiles_sum_zero(MpmP P, const int32_t
                                                            float4* __restrict__ out4,
                                                            float* __restrict__ zbuf, int width) {
   DT_ACTIVE_NODES(nd) {
-    out4[nd] = tile_sum(P, nd, tiles, start);
+    out4[nd] = tile_sum(P, bk, tiles, start);
     for (int c = 0; c < width; ++c) zbuf[nd * width + c] = 0.f;
   }
 }
@@ -1325,7 +1337,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_op4_tiles(MpmP P, const int32_t
                                                            float4* __restrict__ g4,
                                                            float4* __restrict__ v4) {
   DT_ACTIVE_NODES(nd) {
-    const float4 q = tile_sum(P, nd, tiles, start);
+    const float4 q = tile_sum(P, bk, tiles, start);
     g4[nd] = q;
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
     if (q.w > P.eps) {
@@ -1350,7 +1362,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_op4_adj_tiles(MpmP P,
                                                                float4* __restrict__ gg4) {
   DT_ACTIVE_NODES(nd) {
 #pragma clang fp contract(off)
-    const float4 gi = tile_sum(P, nd, tiles, start);
+    const float4 gi = tile_sum(P, bk, tiles, start);
     gv4[nd] = gi;
     const float4 q = g4[nd];
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1377,7 +1389,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_op4_adj_tiles_f(
     float* __restrict__ ggf) {
   DT_ACTIVE_NODES(nd) {
 #pragma clang fp contract(off)
-    const float4 gi = tile_sum(P, nd, tiles, start);
+    const float4 gi = tile_sum(P, bk, tiles, start);
     if (gv4) gv4[nd] = gi;
     const float4 q = g4[nd];
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
