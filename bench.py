@@ -433,7 +433,10 @@ def bench_pad_press(args, torch):
                  resident_bytes=0 if args.pad_recompute else None)
     pad = Pad(nodes.astype(np.float32), f2v, tris, layer, od, c["pad_mu"], c["pad_lam"],
               c["pad_mass"], prm.dt)
-    cs = CoupledSim(sim, [pad], **c["contact"])
+    # segment graphs: this scene's substeps are short enough that launching
+    # them one by one shows (148 -> 115 ms per step); on the gripper and the
+    # population the kernels are long and eager launches are 3-5 % faster
+    cs = CoupledSim(sim, [pad], **c["contact"], graphs=True)
     surf_ids = np.where(layer == layer.max())[0]
     mm = MarkerModel(nodes, surf_ids, c["inv_T"])
     dev = torch.device("cuda")
@@ -477,7 +480,8 @@ def bench_pad_press(args, torch):
            "grad_dL_dvy_first_last": [float(gv[0, 1]), float(gv[-1, 1])],
            "block_min_dy": dy,
            "resident_segments": f"{cs.last_resident[0]} of {cs.last_resident[1]}",
-           "resident_gb": round(cs.last_resident[2] / 1e9, 1)}
+           "resident_gb": round(cs.last_resident[2] / 1e9, 1),
+           "segment_graphs": {"captures": cs.graph_misses, "replays": cs.graph_hits}}
     print(json.dumps(res), flush=True)
 
 
@@ -721,7 +725,8 @@ def bench_gripper(args, torch):
                       "resident_gb": round(cs.last_resident[2] / 1e9, 1)},
            "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(gc).all()),
            "grad_abs_max": float(gc.abs().max()), "pad_face_travel": gap,
-           "cable_max_displacement": cable_move}
+           "cable_max_displacement": cable_move,
+           "segment_graphs": {"captures": cs.graph_misses, "replays": cs.graph_hits}}
     print(json.dumps(res), flush=True)
 
 

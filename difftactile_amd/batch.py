@@ -135,9 +135,11 @@ class CoupledEnvBatch:
     pad_meshes: one ``(nodes_rest, f2v, tris, layer)`` per environment
     pad:        dict(out_dir, mu, lam, mass) of the pads
     contact:    dict(kn, kd, kt, mu_f) of the penalty contact
+    graphs:     capture each checkpoint segment into a HIP graph (CoupledSim)
     """
 
-    def __init__(self, prm, envs, pad_meshes, pad, contact, device="cuda", **sim_kwargs):
+    def __init__(self, prm, envs, pad_meshes, pad, contact, device="cuda", graphs=False,
+                 **sim_kwargs):
         from .coupled import CoupledSim, Pad
         if len(pad_meshes) != len(envs):
             raise ValueError("one pad mesh per environment")
@@ -146,7 +148,7 @@ class CoupledEnvBatch:
         self.vert_offsets = vs
         self.pad = Pad(nodes, f2v, tris, layer, pad["out_dir"], pad["mu"], pad["lam"],
                        pad["mass"], prm.dt, device=device, tri_start=ts)
-        self.cs = CoupledSim(self.batch.sim, [self.pad], **contact)
+        self.cs = CoupledSim(self.batch.sim, [self.pad], **contact, graphs=graphs)
 
     def rollout(self, states, pad_states, control_vels, n_substeps):
         """states: per environment (x, v, C, F); pad_states: per environment

@@ -102,12 +102,14 @@ class CoupledSim:
     def __init__(self, sim: MPMSim, pads, kn, kd, kt, mu_f, norm_eps=1e-11, clamp_grad=True,
                  rope=None, rope_contact=None, graphs=False):
         self.sim = sim
-        # graphs: each substep's forward and adjoint launch sequence is
-        # captured once into a HIP graph and replayed while the pointers it
-        # touches repeat (they do across rollouts of a steady caller); a
-        # caller whose buffers keep moving turns it off after 64 substeps
+        # graphs: each checkpoint segment's forward and adjoint launch
+        # sequences (a lone substep's, outside a rollout) are captured once
+        # into a HIP graph and replayed while the pointers they touch repeat
+        # (they do across rollouts of a steady caller); a caller whose
+        # buffers keep moving turns it off after 512 captures
         self.graphs = bool(graphs)
         self.graph_hits = self.graph_misses = 0
+        self._in_segment = False
         self.pads = list(pads)
         self.rope = rope
         if rope is not None:
@@ -263,22 +265,61 @@ class CoupledSim:
         frames = [[fb(0, a, t).copy_(t) for a, t in enumerate(st0)]]
         pframes = [[[pb(0, b_, c, t).copy_(t) for c, t in enumerate(p)] for b_, p in enumerate(pst0)]]
         stores = []
+        # every buffer first (nothing is allocated while a segment is captured)
         for j in range(k):
             # closest-triangle starts: the previous substep's result
-            store = self._new_store(stores[-1]["idx"] if stores else self.idx, taken, seg + (j,),
-                                    cap)
-            nxt = [fb(j + 1, a, t) for a, t in enumerate(frames[-1])]
-            pn = [[pb(j + 1, b_, c, t) for c, t in enumerate(p)] for b_, p in enumerate(pframes[-1])]
-            f = s0 + j
-            self._fwd(frames[-1], nxt, pframes[-1], pn, [c[f] for c in cvs],
-                      [c[f + 1] for c in cvs], mat_id, store)
-            frames.append(nxt)
-            pframes.append(pn)
-            stores.append(store)
+            stores.append(self._new_store(stores[-1]["idx"] if stores else self.idx, taken,
+                                          seg + (j,), cap))
+            frames.append([fb(j + 1, a, t) for a, t in enumerate(frames[-1])])
+            pframes.append([[pb(j + 1, b_, c, t) for c, t in enumerate(p)]
+                            for b_, p in enumerate(pframes[-1])])
+        calls = [(frames[j], frames[j + 1], pframes[j], pframes[j + 1],
+                  [c[s0 + j] for c in cvs], [c[s0 + j + 1] for c in cvs], mat_id, stores[j])
+                 for j in range(k)]
+        self._fwd_segment(calls)
         return frames, pframes, stores
+
+    def _fwd_segment(self, calls):
+        """The substeps _fwd(*c) for c in calls, in order, as one captured
+        graph when graphs are on."""
+        if not calls:
+            return
+        rid = self.ridx if self.rope is not None else None
+
+        def body():
+            for c in calls:
+                self._fwd(*c)
+
+        self._segment([1, rid] + calls, body)
+        last = calls[-1]
+        self._last_ext = last[7]["ext"] if len(last) > 7 and last[7] else self._ext
+
+    def _adj_segment(self, calls):
+        """The substep adjoints _adj(*c) for c in calls, in order, as one
+        captured graph when graphs are on."""
+        if not calls:
+            return
+
+        def body():
+            for c in calls:
+                self._adj(*c)
+
+        self._segment([2] + calls, body)
+
+    def _segment(self, keyobjs, body):
+        if not self.graphs or self._in_segment:
+            return body()
+        key = self._graph_key(3, *keyobjs)
+        self._in_segment = True
+        try:
+            self._graphed(key, body)
+        finally:
+            self._in_segment = False
 
     # -- graph capture of a substep -------------------------------------------
     def _graph_key(self, tag, *objs):
+        # lazily made buffers first, so the key names the ones a capture uses
+        self._warm()
         words = [tag, id(self)]
 
         def add(o):
@@ -298,6 +339,9 @@ class CoupledSim:
 
         for o in objs:
             add(o)
+        # the simulation's scratch buffers (a regrown one moves)
+        for name in sorted(self.sim._ws, key=str):
+            add(self.sim._ws[name])
         for prm in (self.sim.c_prm, self.c_prm_clamp):
             b = bytes(prm)
             b += bytes(-len(b) % 8)
@@ -341,8 +385,9 @@ class CoupledSim:
             raise
         _lib.call("dt_graph_end", side.cuda_stream, main, key.ctypes.data, len(key))
         self.graph_misses += 1
-        calls = self.graph_hits + self.graph_misses
-        if calls >= 64 and self.graph_hits < self.graph_misses:
+        # more distinct sequences than the library's graph cache holds
+        # (256) never replay: stop capturing
+        if self.graph_misses >= 512 and self.graph_hits < self.graph_misses:
             self.graphs = False
 
     # -- one substep --------------------------------------------------------
@@ -351,7 +396,7 @@ class CoupledSim:
         store: dict receiving what the adjoint needs (or None)."""
         exts = store["ext"] if store else self._ext
         self._last_ext = exts
-        if not self.graphs:
+        if not self.graphs or self._in_segment:
             return self._fwd_body(st, nxt, pads_in, pads_out, cv, cv1, mat_id, store)
         key = self._graph_key(1, st, nxt, pads_in, pads_out, cv, cv1, mat_id, store,
                               self.ridx if self.rope is not None else None)
@@ -459,7 +504,7 @@ class CoupledSim:
         (or None)."""
         args = (st, pads_in, store, g_next, g_pads_next, g, g_pads, g_cv, g_cv1, mat_id,
                 g_ext_out)
-        if not self.graphs:
+        if not self.graphs or self._in_segment:
             return self._adj_body(*args)
         key = self._graph_key(2, *args)
         self._graphed(key, lambda: self._adj_body(*args))
@@ -662,10 +707,13 @@ class _CoupledRollout(torch.autograd.Function):
             else:
                 ckpts.append(None)
             kept.append(None)
+            calls = []
             for f in range(s0, s0 + k):
-                cs._fwd(st, nxt, pst, pnx, [c[f] for c in cvs], [c[f + 1] for c in cvs], mat_id)
+                calls.append((st, nxt, pst, pnx, [c[f] for c in cvs], [c[f + 1] for c in cvs],
+                              mat_id))
                 st, nxt = nxt, st
                 pst, pnx = pnx, pst
+            cs._fwd_segment(calls)
         # (segments kept for the backward pass, segments, bytes they hold)
         cs.last_resident = (sum(kp is not None for kp in kept), len(kept), used)
         out = [sim._scatter(perm, t, w, sh) for t, w, sh in zip(st, W, shapes)]
@@ -749,13 +797,15 @@ class _CoupledRollout(torch.autograd.Function):
                     seg_taken = []
                     frames, pframes, stores = cs._run_keep(ck, ckp, cvs, s0, k, ctx.mat_id,
                                                            seg_taken, ("recompute",))
+            calls = []
             for j in reversed(range(k)):
                 f = s0 + j
-                cs._adj(frames[j], pframes[j], stores[j], g, gp, gbuf, gpbuf,
-                        [gc[f] for gc in g_cv], [gc[f + 1] for gc in g_cv], ctx.mat_id,
-                        g_ext_out if f == ctx.n_substeps - 1 else None)
+                calls.append((frames[j], pframes[j], stores[j], g, gp, gbuf, gpbuf,
+                              [gc[f] for gc in g_cv], [gc[f + 1] for gc in g_cv], ctx.mat_id,
+                              g_ext_out if f == ctx.n_substeps - 1 else None))
                 g, gbuf = gbuf, g
                 gp, gpbuf = gpbuf, gp
+            cs._adj_segment(calls)
             if seg_taken is not None:
                 del frames, pframes, stores
                 sim._give_back(seg_taken)

@@ -175,7 +175,10 @@ class MPMSim:
         return bufs
 
     def _give_back(self, taken):
-        for key, bufs in taken:
+        # last taken, first back: a key taken several times in one rollout
+        # hands out its buffers in the same order in the next one (graph
+        # keys name the addresses)
+        for key, bufs in reversed(taken):
             self._free.setdefault(key, []).append(bufs)
         taken.clear()
 
