@@ -78,6 +78,9 @@ def parse():
     ap.add_argument("--pop-pads", action="store_true",
                     help="population workload: one FEM pad (10,800 tets) per environment "
                          "pressing its object")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the trajectory loops eagerly instead of replaying their "
+                         "captured HIP graphs")
     return ap.parse_args()
 
 
@@ -743,6 +746,9 @@ def main():
         import multiprocessing as mp
         pool = mp.get_context("fork").Pool(min(16, os.cpu_count() or 1), initializer=_gc_init)
     import torch
+    if args.no_graphs:
+        from difftactile_amd import _lib
+        _lib.call("dt_graphs_enable", 0)
     if args.workload == "pad_press":
         bench_pad_press(args, torch)
         return

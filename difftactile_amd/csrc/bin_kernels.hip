@@ -1,13 +1,15 @@
 // Per-substep binning of particles by the brick of their stencil base
 // (counting sort over bricks).
 //
-// k_bin_count gives every particle a slot inside its brick with one global
-// atomic per distinct brick of a wavefront, an exclusive scan over the brick
-// counts gives the brick starts, and k_bin_fill writes the permutation.  The
-// active-brick list (occupied bricks and their +1 neighbours, which a stencil
-// can reach) and the occupied-brick list come from the counts.  The slots
-// follow the order the atomics ran in; k_bin_sort then sorts each brick's
-// particles by index, so the permutation is the same on every run.
+// k_bin_count (or a G2P's fused count) counts every brick's particles with
+// one global atomic per distinct brick of a wavefront and records that
+// (wavefront, brick) run in the brick's run table; an exclusive scan over the
+// brick counts gives the brick starts, and k_bin_fill writes the permutation.
+// The active-brick list (occupied bricks and their +1 neighbours, which a
+// stencil can reach) and the occupied-brick list come from the counts.  The
+// fill orders a brick's particles by index from its run table; a brick whose
+// runs overflow the table is filled in the atomics' order and k_bin_sort
+// sorts it by index, so the permutation is the same on every run.
 #include <hipcub/hipcub.hpp>
 
 #include "dt_bin.h"
@@ -29,7 +31,7 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   size_t scan_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
                                    (int32_t*)nullptr, (int)(nbk + 1));
-  const size_t sz[12] = {sizeof(int32_t) * (size_t)(nbk + 2),
+  const size_t sz[17] = {sizeof(int32_t) * (size_t)(nbk + 2),
                         sizeof(int32_t) * (size_t)(nbk + 1),
                         sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t) * (size_t)n,
@@ -40,9 +42,14 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
                         sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t),
                         sizeof(int32_t) * (size_t)(kSlabMax + kSlabMax * kSlabMax),
-                        sizeof(float) * 32 * (size_t)nblocks(n)};
-  size_t off[12], total = 0;
-  for (int i = 0; i < 12; ++i) {
+                        sizeof(float) * 32 * (size_t)nblocks(n),
+                        sizeof(int32_t) * (size_t)nbk,
+                        sizeof(int32_t) * (size_t)nbk,
+                        sizeof(int32_t) * (size_t)nbk * kRuns,
+                        sizeof(int32_t) * (size_t)nbk,
+                        sizeof(int32_t) * 2};
+  size_t off[17], total = 0;
+  for (int i = 0; i < 17; ++i) {
     off[i] = total;
     total += align256(sz[i] ? sz[i] : 1);
   }
@@ -62,6 +69,11 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
     ws->slab_sum = reinterpret_cast<int32_t*>(b + off[10]);
     ws->row_occ = reinterpret_cast<uint32_t*>(ws->slab_sum + kSlabMax);
     ws->mat_part = reinterpret_cast<float*>(b + off[11]);
+    ws->nruns = reinterpret_cast<int32_t*>(b + off[12]);
+    ws->nsnap = reinterpret_cast<int32_t*>(b + off[13]);
+    ws->runs = reinterpret_cast<int32_t*>(b + off[14]);
+    ws->ovf = reinterpret_cast<int32_t*>(b + off[15]);
+    ws->n_ovf = reinterpret_cast<int32_t*>(b + off[16]);
     ws->scan_bytes = scan_bytes;
     ws->n_bricks = nbk;
     ws->nb = (prm->n_grid + 3) >> 2;
@@ -69,14 +81,19 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   return total;
 }
 
+// clears a[0, n), b[0, nb) and three counters
 __global__ __launch_bounds__(kBlock) void k_bin_zero(long long n, int32_t* __restrict__ a,
+                                                     long long nb, int32_t* __restrict__ b,
                                                      int32_t* __restrict__ one,
-                                                     int32_t* __restrict__ two) {
+                                                     int32_t* __restrict__ two,
+                                                     int32_t* __restrict__ three) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
+  if (i < nb) b[i] = 0;
   if (i == 0) {
     *one = 0;
     *two = 0;
+    *three = 0;
   }
 }
 
@@ -139,7 +156,8 @@ void launch_bin_copy(const BinWs& from, const BinWs& to, int n, hipStream_t st) 
 __global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float* __restrict__ x,
                                                       int32_t* __restrict__ key,
                                                       int32_t* __restrict__ rank,
-                                                      int32_t* __restrict__ count) {
+                                                      int32_t* __restrict__ count,
+                                                      BinRuns R) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = p < n;
   int k = -1;
@@ -148,23 +166,48 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float
     ld_v3(x, p, pl_vec(P.soa, n), xp);
     k = bin_key(xp, at_particle(P, p).inv_dx, P.n_grid, env_x0(P, p)) >> 6;
   }
-  const int r = wave_key_rank(active, k, count);
+  const int r = wave_key_rank_runs(active, k, count, R, p >> 6);
   if (active) {
     key[p] = k;
     rank[p] = r;
   }
 }
 
+// perm[start[key] + place] = p.  A brick of at most kRuns runs: place = the
+// lanes of its runs from earlier wavefronts + the particle's rank among its
+// wavefront's lanes of the brick (ascending index order).  More runs: place =
+// the counting atomics' rank, and k_bin_sort orders the brick afterwards.
+// nsnap: runs per brick of this binning.  Wavefront w covers indices
+// [64 w, 64 w + 64), as in the counting pass.
 __global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __restrict__ key,
                                                      const int32_t* __restrict__ rank,
                                                      const int32_t* __restrict__ start,
+                                                     const int32_t* __restrict__ nsnap,
+                                                     const int32_t* __restrict__ runs,
                                                      int32_t* __restrict__ perm) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) perm[start[key[p]] + rank[p]] = p;
+  const bool active = p < n;
+  const int k = active ? key[p] : -1;
+  const int lr = wave_lane_rank(active, k);
+  if (!active) return;
+  const int nr = nsnap[k];
+  int place;
+  if (nr <= kRuns) {
+    const int w = p >> 6;
+    const int32_t* rt = runs + (long long)k * kRuns;
+    place = lr;
+    for (int q = 0; q < nr; ++q) {
+      const int e = rt[q];
+      if ((e >> 6) < w) place += (e & 63) + 1;
+    }
+  } else {
+    place = rank[p];
+  }
+  perm[start[k] + place] = p;
 }
 
-// Sorts each occupied brick's particles perm[start[b] .. start[b+1]) by index
-// (ascending).  Blocks stride over the occupied-brick list.  A brick of at most
+// Sorts the particles perm[start[b] .. start[b+1]) of each brick on the
+// overflow list by index (ascending).  Blocks stride over the list.  A brick of at most
 // kSortCap particles: every run of 64 is sorted in a wavefront's registers (a
 // bitonic network over the lanes, no barriers), then each particle's place is
 // its rank in its run plus, for every other run, the number of its entries
@@ -299,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_sort(const int32_t* __restrict__
                                                      int32_t* __restrict__ tmp) {
   __shared__ int32_t a[kSortCap], b[kSortCap];
   __shared__ int32_t run_at[65], run_to[64], wsum[kBlock / 64];
-  const int n_occ = *n_list;
+  const int n_occ = __builtin_amdgcn_readfirstlane(*n_list);
   for (int li = blockIdx.x; li < n_occ; li += gridDim.x) {
     const int bk = list[li];
     const int s0 = start[bk], m = start[bk + 1] - s0;
@@ -393,7 +436,10 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 __global__ __launch_bounds__(kSlabThreads) void k_bin_slab_scan(int nb, int32_t* __restrict__ count,
                                                                 int32_t* __restrict__ start,
                                                                 int32_t* __restrict__ slab_sum,
-                                                                uint32_t* __restrict__ row_occ) {
+                                                                uint32_t* __restrict__ row_occ,
+                                                                int32_t* __restrict__ nruns,
+                                                                int32_t* __restrict__ nsnap,
+                                                                int32_t* __restrict__ n_ovf) {
   __shared__ int32_t wsum[kSlabThreads / 64];
   __shared__ uint32_t word[kSlabMax];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -404,6 +450,12 @@ __global__ __launch_bounds__(kSlabThreads) void k_bin_slab_scan(int nb, int32_t*
   if (t < per) {
     v = count[e];
     count[e] = 0;
+    nsnap[e] = nruns[e];
+    nruns[e] = 0;
+  }
+  if (blockIdx.x == 0 && t == 0) {
+    n_ovf[1] = n_ovf[0];
+    n_ovf[0] = 0;
   }
   const int incl = wave_incl_scan(v);
   if (lane == 63) wsum[w] = incl;
@@ -487,14 +539,18 @@ int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_
   if (!counted) {
     if (!slab || !count_clear)
       hipLaunchKernelGGL(k_bin_zero, dim3(nblocks(nbk + 2)), dim3(kBlock), 0, st, nbk + 2,
-                         ws.count, ws.n_list, ws.n_alist);
+                         ws.count, nbk, ws.nruns, ws.n_list, ws.n_alist, ws.n_ovf);
     if (n > 0)
       hipLaunchKernelGGL(k_bin_count, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x, ws.key,
-                         ws.rank, ws.count);
+                         ws.rank, ws.count, bin_runs(ws));
   }
+  // the slab scan clears the run counts behind it (for a G2P's fused count)
+  // and leaves this binning's in nsnap / n_ovf[1]
+  const int32_t* nsnap = slab ? ws.nsnap : ws.nruns;
+  const int32_t* n_ovf = slab ? ws.n_ovf + 1 : ws.n_ovf;
   if (slab) {
     hipLaunchKernelGGL(k_bin_slab_scan, dim3(nb), dim3(kSlabThreads), 0, st, nb, ws.count,
-                       ws.start, ws.slab_sum, ws.row_occ);
+                       ws.start, ws.slab_sum, ws.row_occ, ws.nruns, ws.nsnap, ws.n_ovf);
     hipLaunchKernelGGL(k_bin_slab_lists, dim3(nb), dim3(kSlabThreads), 0, st, nb,
                        (const int32_t*)ws.slab_sum, (const uint32_t*)ws.row_occ, ws.start,
                        ws.list, ws.n_list, ws.alist, ws.n_alist);
@@ -511,13 +567,13 @@ int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_
   }
   if (n > 0) {
     hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
-                       ws.start, ws.perm);
+                       ws.start, nsnap, (const int32_t*)ws.runs, ws.perm);
     // rank is dead after the fill: the large-brick path of the sort uses it
     const long long sort_cap = launch_config().sort_blocks;
     const long long nb_sort = ws.n_bricks < sort_cap ? ws.n_bricks : sort_cap;
     hipLaunchKernelGGL(k_bin_sort, dim3((unsigned)(nb_sort > 0 ? nb_sort : 1)), dim3(kBlock), 0,
-                       st, (const int32_t*)ws.start, (const int32_t*)ws.list,
-                       (const int32_t*)ws.n_list, ws.perm, ws.rank);
+                       st, (const int32_t*)ws.start, (const int32_t*)ws.ovf, n_ovf, ws.perm,
+                       ws.rank);
   }
   return check_launch("particle binning");
 }

@@ -4,6 +4,13 @@
 // launch_bin, perm[start[b] .. start[b+1]) lists the particles of brick b in
 // ascending index order; the brick passes order each chunk of a brick by cell
 // in LDS (stably, so the order is the same on every run).
+//
+// The counting pass records every (wavefront, brick) run it counts in a run
+// table of kRuns entries per brick.  The fill then places a particle at the
+// lengths of its brick's runs of earlier wavefronts plus its rank among its
+// wavefront's lanes of the brick: ascending index order with no sort.  A
+// brick of more than kRuns runs (particles in no spatial order) is placed in
+// the counting atomics' order and sorted afterwards.
 #pragma once
 #include "dt_common.h"
 
@@ -13,6 +20,8 @@ constexpr int kTileEdge = 6;   // a brick's 4 cells plus the stencil's 2-node re
 constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
 // bricks per axis up to which a single-environment grid uses the slab scan
 constexpr int kSlabMax = 32;
+// run-table entries per brick
+constexpr int kRuns = 32;
 
 // Views into one caller-provided workspace buffer.
 struct BinWs {
@@ -33,6 +42,11 @@ struct BinWs {
   size_t scan_bytes;
   int32_t* slab_sum;  // [kSlabMax]            slab scan: particles per x-slab
   uint32_t* row_occ;  // [kSlabMax * kSlabMax] slab scan: occupancy word per brick row
+  int32_t* nruns;   // [n_bricks]          runs counted per brick (zero between binnings)
+  int32_t* nsnap;   // [n_bricks]          the slab scan's copy of nruns (it clears nruns)
+  int32_t* runs;    // [n_bricks * kRuns]  (wavefront << 6) | (lanes - 1) of each run
+  int32_t* ovf;     // [n_bricks]          bricks of more than kRuns runs
+  int32_t* n_ovf;   // [2]                 count of ovf; [1] the slab scan's copy
   long long n_bricks;
   int nb;            // bricks along y and z
   // Optional binning history (one slot per substep of a trajectory): the
@@ -94,12 +108,69 @@ __device__ __forceinline__ unsigned long long wave_match6(bool active, int key) 
   return m;
 }
 
+// The run-table side of a counting pass.
+struct BinRuns {
+  int32_t* nruns;
+  int32_t* runs;
+  int32_t* ovf;
+  int32_t* n_ovf;
+};
+__host__ __device__ inline BinRuns bin_runs(const BinWs& ws) {
+  return BinRuns{ws.nruns, ws.runs, ws.ovf, ws.n_ovf};
+}
+
+// wave_key_rank that also records each (wavefront, key) run in the run table
+// (w: the wavefront's index, particle index >> 6)
+__device__ __forceinline__ int wave_key_rank_runs(bool active, int key, int* counts,
+                                                  const BinRuns& R, int w) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long todo = __ballot(active);
+  int result = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int cur = __shfl(key, leader);
+    const bool mine = active && key == cur;
+    const unsigned long long peers = __ballot(mine);
+    int old = 0;
+    if (lane == leader) {
+      const int m = __popcll(peers);
+      old = atomicAdd(&counts[cur], m);
+      const int slot = atomicAdd(&R.nruns[cur], 1);
+      if (slot < kRuns) R.runs[(long long)cur * kRuns + slot] = (w << 6) | (m - 1);
+      else if (slot == kRuns) R.ovf[atomicAdd(R.n_ovf, 1)] = cur;
+    }
+    old = __shfl(old, leader);
+    if (mine) result = old + __popcll(peers & below);
+    todo &= ~peers;
+  }
+  return result;
+}
+
+// This lane's rank among the active lanes of its wavefront sharing its key
+// (the lane part of wave_key_rank, without atomics).
+__device__ __forceinline__ int wave_lane_rank(bool active, int key) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long todo = __ballot(active);
+  int result = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int cur = __shfl(key, leader);
+    const bool mine = active && key == cur;
+    const unsigned long long peers = __ballot(mine);
+    if (mine) result = __popcll(peers & below);
+    todo &= ~peers;
+  }
+  return result;
+}
+
 // Persistent grid of the brick-owned passes (blocks stride over the list).
 constexpr int kBrickBlocks = 2048;
 // count -> brick scan -> occupied / active brick lists -> fill -> sort of
-// each brick's particles by index, on st.  The counts and the brick starts
-// are integers and exact; the sort makes perm[start[b] .. start[b+1]) the
-// brick's particles in ascending index order whatever order the counting
+// the bricks with more runs than the run table holds, on st.  The counts and
+// the brick starts are integers and exact; perm[start[b] .. start[b+1]) is
+// the brick's particles in ascending index order whatever order the counting
 // atomics ran in, so the brick passes see the same particles in the same
 // order on every run.
 // count_clear: the workspace's brick counts are known to be zero (a previous

@@ -1447,7 +1447,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
                                                 float* __restrict__ x1, float* __restrict__ v1,
                                                 float* __restrict__ C1, int32_t* __restrict__ bkey,
                                                 int32_t* __restrict__ brank,
-                                                int32_t* __restrict__ bcount) {
+                                                int32_t* __restrict__ bcount, BinRuns R) {
   const int p = xcd_block() * blockDim.x + threadIdx.x;
   const bool active = p < n;
   if (!kCount && !active) return;
@@ -1473,7 +1473,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmP P, int n, const float* __re
     if (kCount) k = bin_key(nx, Q.inv_dx, P.n_grid, env_x0(P, p)) >> 6;
   }
   if (kCount) {
-    const int r = wave_key_rank(active, k, bcount);
+    const int r = wave_key_rank_runs(active, k, bcount, R, p >> 6);
     if (active) {
       bkey[p] = k;
       brank[p] = r;
@@ -2201,11 +2201,12 @@ int launch_substep_fwd(const dt_mpm_params* prm, const dt_material* mats, int n,
     StageTimer tm(STAGE_G2P, st);
     if (count_next)
       hipLaunchKernelGGL(k_g2p<true>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x,
-                         (const float*)grid_v4, 4, x1, v1, C1, ws->key, ws->rank, ws->count);
+                         (const float*)grid_v4, 4, x1, v1, C1, ws->key, ws->rank, ws->count,
+                         bin_runs(*ws));
     else
       hipLaunchKernelGGL(k_g2p<false>, dim3(nblocks(n)), dim3(kBlock), 0, st, P, n, x,
                          (const float*)grid_v4, 4, x1, v1, C1, (int32_t*)nullptr,
-                         (int32_t*)nullptr, (int32_t*)nullptr);
+                         (int32_t*)nullptr, (int32_t*)nullptr, BinRuns{});
   }
   return check_launch("dt_mpm_substep_fwd");
 }
@@ -2525,7 +2526,7 @@ int dt_mpm_g2p(const dt_mpm_params* prm, int32_t n, const float* x, const float*
   DT_REQUIRE(x && grid_v_out && x_next && v_next && C_next, "NULL pointer");
   hipLaunchKernelGGL(k_g2p<false>, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream),
                      to_dev(*prm), n, x, grid_v_out, 3, x_next, v_next, C_next, (int32_t*)nullptr,
-                     (int32_t*)nullptr, (int32_t*)nullptr);
+                     (int32_t*)nullptr, (int32_t*)nullptr, BinRuns{});
   return check_launch("dt_mpm_g2p");
 }
 
@@ -2854,7 +2855,7 @@ int dt_mpm_g2p4(const dt_mpm_params* prm, int32_t n, const float* x, const float
   DT_REQUIRE(x && grid_v4 && x_next && v_next && C_next, "NULL pointer");
   hipLaunchKernelGGL(k_g2p<false>, dim3(nblocks(n)), dim3(kBlock), 0, DT_STREAM(stream),
                      to_dev(*prm), n, x, grid_v4, 4, x_next, v_next, C_next, (int32_t*)nullptr,
-                     (int32_t*)nullptr, (int32_t*)nullptr);
+                     (int32_t*)nullptr, (int32_t*)nullptr, BinRuns{});
   return check_launch("dt_mpm_g2p4");
 }
 

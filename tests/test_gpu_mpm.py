@@ -453,12 +453,19 @@ def _bin_arrays(c_prm, n, ws, nbytes, n_bricks):
                                                   (64, 50000, True, False),
                                                   (30, 7000, False, False),
                                                   (132, 30000, True, False),
-                                                  (32, 30000, False, True)])
+                                                  (32, 30000, False, True),
+                                                  (32, 30000, "mixed", False),
+                                                  (132, 30000, "mixed", False),
+                                                  (132, 30000, False, True)])
 def test_binning_is_a_counting_sort_by_stencil_brick(gpu, n_grid, n, sort, cluster):
     """The binning permutation is the stable counting sort by brick, bit for
     bit: every brick's particles in ascending index order whatever order the
-    counting atomics ran in (cluster: ~30k particles in a few bricks, past the
-    2048-particle LDS sort of one brick)."""
+    counting atomics ran in.  Sorted particles give every brick a few
+    (wavefront, brick) runs, placed from the run table; random order gives
+    bricks more runs than the table holds, sorted after the fill; mixed: the
+    first fifth of sorted particles shuffled among themselves, so one binning
+    has bricks of both kinds (cluster: ~30k particles in a few bricks, past
+    the 2048-particle LDS sort of one brick)."""
     prm = default_prm(n_grid)
     rng = np.random.default_rng(n_grid + n)
     # the whole box and a margin outside it (clamped bases)
@@ -468,6 +475,9 @@ def test_binning_is_a_counting_sort_by_stencil_brick(gpu, n_grid, n, sort, clust
     keys = om.bin_keys_np(x, prm.inv_dx, n_grid)
     if sort:
         x = x[np.argsort(keys, kind="stable")]
+        if sort == "mixed":
+            head = n // 5
+            x[:head] = x[rng.permutation(head)]
         keys = om.bin_keys_np(x, prm.inv_dx, n_grid)
     bricks = keys >> 6
     c_prm = prm.c_struct()
