@@ -28,7 +28,7 @@ from . import _lib
 CANDIDATES = {
     "brick_blocks": [512, 768, 1024, 1536, 2048, 3072, 4096],
     "node_blocks": [256, 512, 1024, 2048, 4096],
-    "sort_blocks": [512, 1024, 2048, 4096],
+    "sort_blocks": [128, 512, 1024, 2048, 4096],
 }
 DEFAULT = {"brick_blocks": 2048, "node_blocks": 1024, "sort_blocks": 2048}
 PACKAGE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_launch.json")
