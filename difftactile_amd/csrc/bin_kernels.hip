@@ -190,20 +190,30 @@ __global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __res
   const int k = active ? key[p] : -1;
   const int lr = wave_lane_rank(active, k);
   if (!active) return;
+  // the row's first eight entries are loaded with the run count and the
+  // brick start (entries past the count are left over from earlier
+  // binnings and masked out)
+  const int32_t* rt = runs + (long long)k * kRuns;
   const int nr = nsnap[k];
+  const int s0 = start[k];
+  int e8[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) e8[q] = rt[q];
+  const int w = p >> 6;
   int place;
   if (nr <= kRuns) {
-    const int w = p >> 6;
-    const int32_t* rt = runs + (long long)k * kRuns;
     place = lr;
-    for (int q = 0; q < nr; ++q) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < nr && (e8[q] >> 6) < w) place += (e8[q] & 63) + 1;
+    for (int q = 8; q < nr; ++q) {
       const int e = rt[q];
       if ((e >> 6) < w) place += (e & 63) + 1;
     }
   } else {
     place = rank[p];
   }
-  perm[start[k] + place] = p;
+  perm[s0 + place] = p;
 }
 
 // Sorts the particles perm[start[b] .. start[b+1]) of each brick on the
