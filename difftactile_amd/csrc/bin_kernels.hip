@@ -31,7 +31,7 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
   size_t scan_bytes = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
                                    (int32_t*)nullptr, (int)(nbk + 1));
-  const size_t sz[17] = {sizeof(int32_t) * (size_t)(nbk + 2),
+  const size_t sz[18] = {sizeof(int32_t) * (size_t)(nbk + 2),
                         sizeof(int32_t) * (size_t)(nbk + 1),
                         sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t) * (size_t)n,
@@ -47,9 +47,10 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
                         sizeof(int32_t) * (size_t)nbk,
                         sizeof(int32_t) * (size_t)nbk * kRuns,
                         sizeof(int32_t) * (size_t)nbk,
-                        sizeof(int32_t) * 2};
-  size_t off[17], total = 0;
-  for (int i = 0; i < 17; ++i) {
+                        sizeof(int32_t) * 2,
+                        sizeof(int32_t) * (size_t)nbk};
+  size_t off[18], total = 0;
+  for (int i = 0; i < 18; ++i) {
     off[i] = total;
     total += align256(sz[i] ? sz[i] : 1);
   }
@@ -74,6 +75,7 @@ size_t bin_ws_layout(const dt_mpm_params* prm, int n, void* base, BinWs* ws) {
     ws->runs = reinterpret_cast<int32_t*>(b + off[14]);
     ws->ovf = reinterpret_cast<int32_t*>(b + off[15]);
     ws->n_ovf = reinterpret_cast<int32_t*>(b + off[16]);
+    ws->lstart = reinterpret_cast<int32_t*>(b + off[17]);
     ws->scan_bytes = scan_bytes;
     ws->n_bricks = nbk;
     ws->nb = (prm->n_grid + 3) >> 2;
@@ -178,24 +180,19 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(MpmP P, int n, const float
 // wavefront's lanes of the brick (ascending index order).  More runs: place =
 // the counting atomics' rank, and k_bin_sort orders the brick afterwards.
 // nsnap: runs per brick of this binning.  Wavefront w covers indices
-// [64 w, 64 w + 64), as in the counting pass.
-__global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __restrict__ key,
-                                                     const int32_t* __restrict__ rank,
-                                                     const int32_t* __restrict__ start,
-                                                     const int32_t* __restrict__ nsnap,
-                                                     const int32_t* __restrict__ runs,
-                                                     int32_t* __restrict__ perm) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = p < n;
-  const int k = active ? key[p] : -1;
-  const int lr = wave_lane_rank(active, k);
-  if (!active) return;
+// [64 w, 64 w + 64), as in the counting pass.  add: added to start[key].
+__device__ __forceinline__ void fill_particle(int p, int k, int lr, int add,
+                                              const int32_t* __restrict__ rank,
+                                              const int32_t* __restrict__ start,
+                                              const int32_t* __restrict__ nsnap,
+                                              const int32_t* __restrict__ runs,
+                                              int32_t* __restrict__ perm) {
   // the row's first eight entries are loaded with the run count and the
   // brick start (entries past the count are left over from earlier
   // binnings and masked out)
   const int32_t* rt = runs + (long long)k * kRuns;
   const int nr = nsnap[k];
-  const int s0 = start[k];
+  const int s0 = start[k] + add;
   int e8[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) e8[q] = rt[q];
@@ -214,6 +211,19 @@ __global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __res
     place = rank[p];
   }
   perm[s0 + place] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_fill(int n, const int32_t* __restrict__ key,
+                                                     const int32_t* __restrict__ rank,
+                                                     const int32_t* __restrict__ start,
+                                                     const int32_t* __restrict__ nsnap,
+                                                     const int32_t* __restrict__ runs,
+                                                     int32_t* __restrict__ perm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = p < n;
+  const int k = active ? key[p] : -1;
+  const int lr = wave_lane_rank(active, k);
+  if (active) fill_particle(p, k, lr, 0, rank, start, nsnap, runs, perm);
 }
 
 // Sorts the particles perm[start[b] .. start[b+1]) of each brick on the
@@ -421,17 +431,20 @@ __global__ __launch_bounds__(kBlock) void k_brick_flags(long long n_bricks, int 
 // Grids of at most 32 bricks per axis (one environment) scan in two
 // launches of one workgroup per x-slab of nb x nb bricks (brick (i, j, k) =
 // (i nb + j) nb + k, slab i, row (i, j)):
-//   k_bin_slab_scan   slab-local exclusive scan of the counts into start, the
-//                     slab's total, and one 32-bit occupancy word per row
-//                     (bit k: brick (i, j, k) holds particles); the counts are
-//                     zeroed behind the read, so the next binning on this
-//                     workspace needs no separate clear;
-//   k_bin_slab_lists  adds the earlier slabs' totals to start; every
-//                     workgroup scans the per-row counts of occupied and
-//                     active bricks (active: it or a -1 neighbour occupied, a
-//                     shift and an OR of four words) over all rows, and writes
-//                     its own rows' entries of both lists, so both come out
-//                     ascending.
+//   k_bin_slab_scan        slab-local exclusive scan of the counts into
+//                          lstart, the slab's total, and one 32-bit occupancy
+//                          word per row (bit k: brick (i, j, k) holds
+//                          particles); the counts and run counts are zeroed
+//                          behind the read, so the next binning on this
+//                          workspace needs no separate clear;
+//   k_bin_slab_lists_fill  its first nb workgroups add the earlier slabs'
+//                          totals to lstart into start; every one of them
+//                          scans the per-row counts of occupied and active
+//                          bricks (active: it or a -1 neighbour occupied, a
+//                          shift and an OR of four words) over all rows, and
+//                          writes its own rows' entries of both lists, so
+//                          both come out ascending.  Its other workgroups
+//                          fill the permutation meanwhile.
 constexpr int kSlabThreads = kSlabMax * kSlabMax;
 
 __device__ __forceinline__ int wave_incl_scan(int v) {
@@ -444,7 +457,7 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 }
 
 __global__ __launch_bounds__(kSlabThreads) void k_bin_slab_scan(int nb, int32_t* __restrict__ count,
-                                                                int32_t* __restrict__ start,
+                                                                int32_t* __restrict__ lstart,
                                                                 int32_t* __restrict__ slab_sum,
                                                                 uint32_t* __restrict__ row_occ,
                                                                 int32_t* __restrict__ nruns,
@@ -473,20 +486,39 @@ __global__ __launch_bounds__(kSlabThreads) void k_bin_slab_scan(int nb, int32_t*
   if (v > 0) atomicOr(&word[t / nb], 1u << (t % nb));
   int base = 0;
   for (int q = 0; q < w; ++q) base += wsum[q];
-  if (t < per) start[e] = base + incl - v;
+  if (t < per) lstart[e] = base + incl - v;
   if (t == kSlabThreads - 1) slab_sum[blockIdx.x] = base + incl;
   __syncthreads();
   if (t < nb) row_occ[blockIdx.x * nb + t] = word[t];
 }
 
-__global__ __launch_bounds__(kSlabThreads) void k_bin_slab_lists(
+// Workgroups [0, nb) make the lists and the final starts (slab lists);
+// the others fill the permutation from the slab-local starts plus their
+// slab's offset (the exclusive prefix of the slab totals), so the fill does
+// not wait for the lists.
+__global__ __launch_bounds__(kSlabThreads) void k_bin_slab_lists_fill(
     int nb, const int32_t* __restrict__ slab_sum, const uint32_t* __restrict__ row_occ,
-    int32_t* __restrict__ start, int32_t* __restrict__ list, int32_t* __restrict__ n_list,
-    int32_t* __restrict__ alist, int32_t* __restrict__ n_alist) {
+    const int32_t* __restrict__ lstart, int32_t* __restrict__ start,
+    int32_t* __restrict__ list, int32_t* __restrict__ n_list, int32_t* __restrict__ alist,
+    int32_t* __restrict__ n_alist, int n, const int32_t* __restrict__ key,
+    const int32_t* __restrict__ rank, const int32_t* __restrict__ nsnap,
+    const int32_t* __restrict__ runs, int32_t* __restrict__ perm) {
   __shared__ int32_t wocc[kSlabThreads / 64], wact[kSlabThreads / 64];
   __shared__ int32_t slab_off;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int per = nb * nb, slab = blockIdx.x;
+  const int per = nb * nb;
+  if ((int)blockIdx.x >= nb) {
+    const int p = ((int)blockIdx.x - nb) * kSlabThreads + t;
+    const bool active = p < n;
+    const int k = active ? key[p] : -1;
+    const int sv = lane < nb ? slab_sum[lane] : 0;
+    const int excl = wave_incl_scan(sv) - sv;
+    const int add = __shfl(excl, active ? k / per : 0);
+    const int lr = wave_lane_rank(active, k);
+    if (active) fill_particle(p, k, lr, add, rank, lstart, nsnap, runs, perm);
+    return;
+  }
+  const int slab = blockIdx.x;
   if (w == 0) {
     int v = lane < slab ? slab_sum[lane] : 0;
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
@@ -520,7 +552,7 @@ __global__ __launch_bounds__(kSlabThreads) void k_bin_slab_lists(
     *n_list = bo + no;
     *n_alist = ba + na;
   }
-  if (t < per) start[(long long)slab * per + t] += slab_off;
+  if (t < per) start[(long long)slab * per + t] = lstart[(long long)slab * per + t] + slab_off;
   if (slab == nb - 1 && t == 0) start[(long long)nb * per] = slab_off + slab_sum[slab];
   if (t < per && t / nb == slab) {
     const int rb = t * nb;
@@ -560,10 +592,13 @@ int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_
   const int32_t* n_ovf = slab ? ws.n_ovf + 1 : ws.n_ovf;
   if (slab) {
     hipLaunchKernelGGL(k_bin_slab_scan, dim3(nb), dim3(kSlabThreads), 0, st, nb, ws.count,
-                       ws.start, ws.slab_sum, ws.row_occ, ws.nruns, ws.nsnap, ws.n_ovf);
-    hipLaunchKernelGGL(k_bin_slab_lists, dim3(nb), dim3(kSlabThreads), 0, st, nb,
-                       (const int32_t*)ws.slab_sum, (const uint32_t*)ws.row_occ, ws.start,
-                       ws.list, ws.n_list, ws.alist, ws.n_alist);
+                       ws.lstart, ws.slab_sum, ws.row_occ, ws.nruns, ws.nsnap, ws.n_ovf);
+    hipLaunchKernelGGL(k_bin_slab_lists_fill, dim3(nb + nblocks(n, kSlabThreads)),
+                       dim3(kSlabThreads), 0, st, nb, (const int32_t*)ws.slab_sum,
+                       (const uint32_t*)ws.row_occ, (const int32_t*)ws.lstart, ws.start, ws.list,
+                       ws.n_list, ws.alist, ws.n_alist, n, (const int32_t*)ws.key,
+                       (const int32_t*)ws.rank, (const int32_t*)ws.nsnap,
+                       (const int32_t*)ws.runs, ws.perm);
   } else {
     size_t bytes = ws.scan_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(ws.scan_tmp, bytes, ws.count, ws.start, (int)(nbk + 1),
@@ -576,8 +611,9 @@ int launch_bin(const MpmP& P, int n, const float* x, const BinWs& ws, hipStream_
                        ws.n_alist);
   }
   if (n > 0) {
-    hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
-                       ws.start, nsnap, (const int32_t*)ws.runs, ws.perm);
+    if (!slab)
+      hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(n)), dim3(kBlock), 0, st, n, ws.key, ws.rank,
+                         ws.start, nsnap, (const int32_t*)ws.runs, ws.perm);
     // rank is dead after the fill: the large-brick path of the sort uses it
     const long long sort_cap = launch_config().sort_blocks;
     const long long nb_sort = ws.n_bricks < sort_cap ? ws.n_bricks : sort_cap;

@@ -47,6 +47,7 @@ struct BinWs {
   int32_t* runs;    // [n_bricks * kRuns]  (wavefront << 6) | (lanes - 1) of each run
   int32_t* ovf;     // [n_bricks]          bricks of more than kRuns runs
   int32_t* n_ovf;   // [2]                 count of ovf; [1] the slab scan's copy
+  int32_t* lstart;  // [n_bricks]          slab scan: slab-local brick starts
   long long n_bricks;
   int nb;            // bricks along y and z
   // Optional binning history (one slot per substep of a trajectory): the
